@@ -1,0 +1,928 @@
+// C-ABI of libsphgpu.so: handle lifecycle, scene upload, the per-step
+// drivers of the three solvers (the reference's sphStep overrides) and
+// readback.  All device work runs on the handle's single HIP stream.
+#include "sph_common.h"
+#include "sphgpu.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using namespace sphgpu;
+
+namespace {
+
+struct SceneDesc {
+    float mn[3], mx[3];
+    bool set = false;
+};
+
+struct Phase {
+    float rho0, visc;
+};
+
+struct ProfEvent {
+    hipEvent_t a, b;
+    double bytes;
+};
+
+}  // namespace
+
+struct sphgpu_handle {
+    sphgpu_params P{};
+    std::string err;
+    std::vector<SceneDesc> scenes;
+    std::vector<Phase> phases;
+    // host staging (add order)
+    std::vector<float> fx, fv;
+    std::vector<int> fphase, fscene;
+    std::vector<float> bx;
+    std::vector<int> bsolid, bscene;
+    std::vector<float> solidVisc;
+    long long nextId = 0;
+
+    bool initialized = false;
+    int frame = 0;
+    hipStream_t st = nullptr;
+    Dev D{};
+    std::vector<SceneGrid> hgrids;
+    std::vector<uint32_t> hKeyBase;
+    int keyBits = 1;
+    long long nLive = 0;
+    std::vector<void*> allocs;
+    // sort
+    SortScratch sort{};
+    uint32_t* sk = nullptr;
+    uint32_t* sv = nullptr;
+    // small device buffers
+    int* layoutOut = nullptr;          // [npad_new, live]
+    double* statsPart = nullptr;       // per tile 4
+    double* statsOut = nullptr;        // per scene 4
+    // pinned host mirrors
+    int* hActive = nullptr;
+    int* hIters = nullptr;
+    double* hErr = nullptr;
+    int* hCounters = nullptr;
+    double* hStats = nullptr;
+    int* hLayout = nullptr;
+    float* hDt = nullptr;
+    std::vector<sphgpu_stats> sceneStats;
+    // profiling
+    std::string profName;
+    std::vector<ProfEvent> prof;
+    size_t profUsed = 0;
+};
+
+namespace {
+
+int fail(sphgpu_handle* h, int code, const char* fmt, ...) {
+    if (h) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        h->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHECK(h, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((h), SPHGPU_E_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int dalloc(sphgpu_handle* h, T** p, size_t count) {
+    void* m = nullptr;
+    size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    hipError_t e = hipMalloc(&m, bytes);
+    if (e != hipSuccess) return fail(h, SPHGPU_E_DEVICE, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    h->allocs.push_back(m);
+    *p = static_cast<T*>(m);
+    return 0;
+}
+
+uint32_t spread2(uint32_t v) {
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// ---- profiling helpers
+struct ProfScope {
+    sphgpu_handle* h;
+    bool on;
+    ProfScope(sphgpu_handle* hh, const char* name, double bytes) : h(hh), on(false) {
+        if (h->profName.empty() || h->profName != name) return;
+        if (h->profUsed == h->prof.size()) {
+            ProfEvent e{};
+            if (hipEventCreate(&e.a) != hipSuccess || hipEventCreate(&e.b) != hipSuccess) return;
+            h->prof.push_back(e);
+        }
+        ProfEvent& e = h->prof[h->profUsed];
+        e.bytes = bytes;
+        hipEventRecord(e.a, h->st);
+        on = true;
+    }
+    ~ProfScope() {
+        if (!on) return;
+        hipEventRecord(h->prof[h->profUsed].b, h->st);
+        ++h->profUsed;
+    }
+};
+
+double nf_live(const sphgpu_handle* h) { return double(h->nLive); }
+
+// ---- neighbour search pipeline ------------------------------------------
+int run_ns(sphgpu_handle* h) {
+    Dev& D = h->D;
+    hipStream_t st = h->st;
+    int npad = D.npad;
+    {
+        ProfScope ps(h, "ns_keys", nf_live(h) * 20.0);
+        launch_fluid_keys_into(D, h->sk, h->sv, st);
+    }
+    {
+        ProfScope ps(h, "ns_sort", nf_live(h) * 16.0 * ((h->keyBits + 7) / 8));
+        radix_sort_pairs(h->sk, h->sv, npad, h->keyBits, h->sort, st);
+    }
+    launch_fill_i32(D.sceneStart, -1, D.nscenes, st);
+    launch_fill_i32(D.sceneCount, 0, D.nscenes, st);
+    launch_scene_bounds(D, h->sk, npad, st);
+    launch_scene_layout(D, h->layoutOut, st);
+    {
+        ProfScope ps(h, "ns_reorder", nf_live(h) * 64.0);
+        launch_reorder(D, h->sk, h->sv, npad, npad, st);
+    }
+    std::swap(D.x, D.x2);
+    std::swap(D.v, D.v2);
+    std::swap(D.id, D.id2);
+    std::swap(D.ksum, D.ksum2);
+    std::swap(D.kvsum, D.kvsum2);
+    std::swap(D.key, D.key2);
+    launch_cell_reset(D, D.key2, npad, D.cellF, st);
+    launch_cell_build(D, D.key, npad, D.cellF, st);
+    {
+        ProfScope ps(h, "ns_neighbours", nf_live(h) * 12.0);
+        launch_neighbours(D, st);
+    }
+    return 0;
+}
+
+int sync_active(sphgpu_handle* h, bool* any) {
+    Dev& D = h->D;
+    int ns = D.nscenes;
+    HIPCHECK(h, hipMemcpyAsync(h->hActive, D.active, sizeof(int) * ns, hipMemcpyDeviceToHost, h->st));
+    HIPCHECK(h, hipMemcpyAsync(h->hIters, D.iters, sizeof(int) * ns, hipMemcpyDeviceToHost, h->st));
+    HIPCHECK(h, hipMemcpyAsync(h->hErr, D.sceneErr, sizeof(double) * ns, hipMemcpyDeviceToHost, h->st));
+    HIPCHECK(h, hipStreamSynchronize(h->st));
+    bool a = false;
+    for (int s = 0; s < ns; ++s) a = a || h->hActive[s] != 0;
+    *any = a;
+    return 0;
+}
+
+// one solver loop; records per-scene iterations/errors into field `which`
+// (0 density, 1 divergence) of sceneStats.
+int solver_loop(sphgpu_handle* h, int which) {
+    Dev& D = h->D;
+    int ns = D.nscenes;
+    launch_fill_i32(D.active, 1, ns, h->st);
+    launch_fill_i32(D.iters, 0, ns, h->st);
+    const sphgpu_params& P = h->P;
+    for (;;) {
+        if (P.solver == SPHGPU_DFSPH) {
+            if (which == 0) {
+                {
+                    ProfScope ps(h, "df_pass_density", nf_live(h) * 36.0);
+                    launch_df_pass_density(D, h->st);
+                }
+                {
+                    ProfScope ps(h, "df_pass_velocity", nf_live(h) * 44.0);
+                    launch_df_pass_velocity(D, D.kr, h->st);
+                }
+                launch_scene_reduce(D, 0, P.density_tol, P.min_iter_density,
+                                    P.max_iter_density > 0 ? P.max_iter_density : 1 << 30, h->st);
+            } else {
+                {
+                    ProfScope ps(h, "df_pass_divergence", nf_live(h) * 36.0);
+                    launch_df_pass_divergence(D, h->st);
+                }
+                {
+                    ProfScope ps(h, "df_pass_velocity", nf_live(h) * 44.0);
+                    launch_df_pass_velocity(D, D.kr, h->st);
+                }
+                launch_scene_reduce(D, 1, P.divergence_tol, P.min_iter_div, P.max_iter_div + 1, h->st);
+            }
+        } else {  // IISPH
+            {
+                ProfScope ps(h, "ii_sumdij", nf_live(h) * 32.0);
+                launch_ii_sumdij(D, h->st);
+            }
+            {
+                ProfScope ps(h, "ii_pressure", nf_live(h) * 56.0);
+                launch_ii_pressure(D, h->st);
+            }
+            std::swap(D.kr, D.f1);
+            launch_scene_reduce(D, 1, P.density_tol, P.min_iter_density,
+                                P.max_iter_density > 0 ? P.max_iter_density : 1 << 30, h->st);
+        }
+        bool any = false;
+        int rc = sync_active(h, &any);
+        if (rc) return rc;
+        if (!any) break;
+    }
+    for (int s = 0; s < ns; ++s) {
+        if (which == 0) {
+            h->sceneStats[s].iter_density = h->hIters[s];
+            h->sceneStats[s].density_err = h->hErr[s];
+        } else {
+            h->sceneStats[s].iter_divergence = h->hIters[s];
+            h->sceneStats[s].divergence_err = h->hErr[s];
+        }
+    }
+    return 0;
+}
+
+int finish_step(sphgpu_handle* h, sphgpu_stats* out) {
+    Dev& D = h->D;
+    int ns = D.nscenes;
+    launch_stats(D, h->statsPart, h->statsOut, h->st);
+    HIPCHECK(h, hipMemcpyAsync(h->hStats, h->statsOut, sizeof(double) * 4 * ns, hipMemcpyDeviceToHost, h->st));
+    HIPCHECK(h, hipMemcpyAsync(h->hCounters, D.counters, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, h->st));
+    HIPCHECK(h, hipMemcpyAsync(h->hLayout, h->layoutOut, sizeof(int) * 2, hipMemcpyDeviceToHost, h->st));
+    HIPCHECK(h, hipMemcpyAsync(h->hDt, D.dt, sizeof(float) * ns, hipMemcpyDeviceToHost, h->st));
+    HIPCHECK(h, hipStreamSynchronize(h->st));
+    HIPCHECK(h, hipGetLastError());
+    // shrink the slot bound to the new padded layout
+    if (h->hLayout[0] > 0 && h->hLayout[0] <= D.npad) D.npad = h->hLayout[0];
+    sphgpu_stats agg{};
+    double seSum = 0, ke = 0, emax = -1e300;
+    long long nsum = 0;
+    for (int s = 0; s < ns; ++s) {
+        sphgpu_stats& ss = h->sceneStats[s];
+        double n = h->hStats[4 * s + 3];
+        ss.rho_mean_err = h->hStats[4 * s];
+        ss.rho_max_err = h->hStats[4 * s + 1];
+        ss.kinetic_energy = h->hStats[4 * s + 2];
+        ss.dt = h->hDt[s];
+        ss.num_fluid = (long long)n;
+        seSum += ss.rho_mean_err * n;
+        nsum += (long long)n;
+        ke += ss.kinetic_energy;
+        if (n > 0) emax = std::max(emax, ss.rho_max_err);
+        agg.iter_density = std::max(agg.iter_density, ss.iter_density);
+        agg.iter_divergence = std::max(agg.iter_divergence, ss.iter_divergence);
+        agg.density_err = std::max(agg.density_err, ss.density_err);
+        agg.divergence_err = std::max(agg.divergence_err, ss.divergence_err);
+    }
+    agg.rho_mean_err = nsum ? seSum / nsum : 0.0;
+    agg.rho_max_err = nsum ? emax : 0.0;
+    agg.kinetic_energy = ke;
+    agg.dt = h->hDt[0];
+    agg.zero_dis = h->hCounters[C_ZERO_DIS];
+    agg.too_many_neigb = h->hCounters[C_TOO_MANY];
+    agg.list_overflow = h->hCounters[C_LIST_OVF];
+    agg.removed = h->hCounters[C_REMOVED];
+    agg.nan_count = h->hCounters[C_NAN];
+    agg.max_iter_hit = h->hCounters[C_COUNT - 1];
+    h->nLive = (long long)nsum;
+    agg.num_fluid = h->nLive;
+    launch_fill_i32(D.counters, 0, C_COUNT, h->st);
+    if (out) *out = agg;
+    ++h->frame;
+    if (agg.nan_count > 0) return fail(h, SPHGPU_E_DIVERGED, "%d non-finite fluid particles", agg.nan_count);
+    return 0;
+}
+
+int step_wcsph(sphgpu_handle* h, sphgpu_stats* st) {
+    Dev& D = h->D;
+    run_ns(h);
+    launch_cfl(D, h->P.cfl, h->P.spacing_r, h->P.dt, h->P.dt_min, h->st);
+    {
+        ProfScope ps(h, "wc_pressure", nf_live(h) * 20.0);
+        launch_wc_pressure(D, h->st);
+    }
+    {
+        ProfScope ps(h, "wc_force", nf_live(h) * 56.0);
+        launch_wc_force_integrate(D, h->st);
+    }
+    std::swap(D.x, D.x2);
+    std::swap(D.v, D.v2);
+    return finish_step(h, st);
+}
+
+int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
+    Dev& D = h->D;
+    int rc;
+    if (h->frame == 0) {
+        run_ns(h);
+        launch_df_density_alpha(D, h->st);
+    }
+    launch_cfl(D, h->P.cfl, h->P.spacing_r, h->P.dt, h->P.dt_min, h->st);
+    int warm = h->P.warm_start == SPHGPU_WARM_PREVIOUS ? 1 : 0;
+    {
+        ProfScope ps(h, "df_aadv", nf_live(h) * 40.0);
+        launch_df_aadv(D, warm, h->st);
+    }
+    if (warm) {
+        launch_fill_i32(D.active, 1, D.nscenes, h->st);
+        launch_df_pass_velocity(D, D.kr0, h->st);
+    }
+    if ((rc = solver_loop(h, 0))) return rc;
+    launch_df_integrate(D, h->st);
+    run_ns(h);
+    {
+        ProfScope ps(h, "df_density_alpha", nf_live(h) * 20.0);
+        launch_df_density_alpha(D, h->st);
+    }
+    HIPCHECK(h, hipMemcpyAsync(D.vs, D.v, sizeof(float4) * D.npad, hipMemcpyDeviceToDevice, h->st));
+    if ((rc = solver_loop(h, 1))) return rc;
+    std::swap(D.v, D.vs);
+    return finish_step(h, st);
+}
+
+int step_iisph(sphgpu_handle* h, sphgpu_stats* st) {
+    Dev& D = h->D;
+    int rc;
+    run_ns(h);
+    launch_cfl(D, h->P.cfl, h->P.spacing_r, h->P.dt, h->P.dt_min, h->st);
+    launch_ii_density(D, h->st);
+    {
+        ProfScope ps(h, "ii_aadv_dii", nf_live(h) * 64.0);
+        launch_ii_aadv_dii(D, h->st);
+    }
+    {
+        ProfScope ps(h, "ii_aii", nf_live(h) * 48.0);
+        launch_ii_aii(D, h->st);
+    }
+    if ((rc = solver_loop(h, 0))) return rc;
+    {
+        ProfScope ps(h, "ii_force", nf_live(h) * 56.0);
+        launch_ii_force_integrate(D, h->st);
+    }
+    std::swap(D.x, D.x2);
+    std::swap(D.v, D.v2);
+    return finish_step(h, st);
+}
+
+// host: uniform grid of a scene (SphBase::SpaceGridStruct::initialize, SphBase.cpp:963-988)
+void build_grids(sphgpu_handle* h, std::vector<int>& rowRank) {
+    const float H = h->P.smooth_radius_h;
+    uint32_t keyBase = 0;
+    int rowBase = 0;
+    h->hgrids.clear();
+    h->hKeyBase.clear();
+    rowRank.clear();
+    for (int s = 0; s < h->P.num_scenes; ++s) {
+        const SceneDesc& d = h->scenes[s];
+        SceneGrid g{};
+        volatile float twoH = H * 2.0f;
+        g.minx = d.mn[0] - twoH;
+        g.miny = d.mn[1] - twoH;
+        g.minz = d.mn[2] - twoH;
+        g.divisor = 1.0f / H;
+        int num[3];
+        for (int a = 0; a < 3; ++a) {
+            volatile float ext = d.mx[a] - d.mn[a];
+            volatile float e2 = ext + twoH;
+            volatile float e3 = e2 * g.divisor;
+            num[a] = int(e3) + 2;
+            g.smin[a] = d.mn[a];
+            g.smax[a] = d.mx[a];
+        }
+        g.nx = num[0]; g.ny = num[1]; g.nz = num[2];
+        g.keyBase = keyBase;
+        g.rowBase = rowBase;
+        // rows (cy, cz) ranked by 2-D Morton code
+        std::vector<std::pair<uint32_t, int>> rows;
+        rows.reserve((size_t)g.ny * g.nz);
+        for (int cz = 0; cz < g.nz; ++cz)
+            for (int cy = 0; cy < g.ny; ++cy)
+                rows.push_back({spread2(uint32_t(cy)) | (spread2(uint32_t(cz)) << 1), cz * g.ny + cy});
+        std::sort(rows.begin(), rows.end());
+        rowRank.resize((size_t)rowBase + (size_t)g.ny * g.nz);
+        for (size_t r = 0; r < rows.size(); ++r) rowRank[(size_t)rowBase + rows[r].second] = int(r);
+        h->hgrids.push_back(g);
+        h->hKeyBase.push_back(keyBase);
+        keyBase += uint32_t((uint64_t)g.nx * g.ny * g.nz);
+        rowBase += g.ny * g.nz;
+    }
+    h->hKeyBase.push_back(keyBase);
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+int sphgpu_abi_version(void) { return SPHGPU_ABI_VERSION; }
+
+void sphgpu_default_params(sphgpu_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->solver = SPHGPU_DFSPH;
+    p->spacing_r = 0.1f;                  // SphBase.cpp:214
+    p->smooth_radius_h = 0.2f;            // SphBase.cpp:215
+    p->space_max[0] = p->space_max[1] = p->space_max[2] = 1.0f;
+    p->dt = 0.001f;                       // SphBase.cpp:219
+    p->gravity[1] = -9.8f;                // SphBase.cpp:220
+    p->sound_speed = 100.0f;              // SphBase.cpp:222
+    p->gamma = 7.0f;                      // WcSph.h:34
+    p->nu = 0.001f;
+    p->cfl = 0.0f;
+    p->dt_min = 1e-6f;
+    p->density_tol = 1e-4f;               // DFSPH.cpp:225
+    p->divergence_tol = 2.5f;             // DFSPH.cpp:762
+    p->min_iter_density = 2;              // DFSPH.cpp:229
+    p->max_iter_density = 100;
+    p->min_iter_div = 1;                  // DFSPH.cpp:765
+    p->max_iter_div = 30;                 // DFSPH.cpp:764
+    p->omega = 0.5f;                      // IISph_M.cpp:341
+    p->warm_start = SPHGPU_WARM_NONE;
+    p->num_scenes = 1;
+}
+
+int sphgpu_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+    return std::strncmp(prop.gcnArchName, "gfx9", 4) == 0 ? 1 : 0;
+}
+
+int sphgpu_create(const sphgpu_params* p, sphgpu_handle** out) {
+    if (!p || !out) return SPHGPU_E_ARG;
+    *out = nullptr;
+    if (p->solver < SPHGPU_WCSPH || p->solver > SPHGPU_IISPH) return SPHGPU_E_ARG;
+    if (!(p->spacing_r > 0) || !(p->smooth_radius_h > 0) || !(p->dt > 0)) return SPHGPU_E_ARG;
+    if (p->num_scenes < 1 || p->num_scenes > 4096) return SPHGPU_E_ARG;
+    if (p->warm_start < 0 || p->warm_start > 2) return SPHGPU_E_ARG;
+    sphgpu_handle* h = new sphgpu_handle();
+    h->P = *p;
+    h->scenes.resize(p->num_scenes);
+    std::memcpy(h->scenes[0].mn, p->space_min, sizeof(float) * 3);
+    std::memcpy(h->scenes[0].mx, p->space_max, sizeof(float) * 3);
+    h->scenes[0].set = true;
+    h->sceneStats.resize(p->num_scenes);
+    *out = h;
+    return SPHGPU_OK;
+}
+
+void sphgpu_destroy(sphgpu_handle* h) {
+    if (!h) return;
+    if (h->initialized) {
+        hipStreamSynchronize(h->st);
+        for (auto& e : h->prof) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+        for (void* m : h->allocs) hipFree(m);
+        hipHostFree(h->hActive);
+        hipStreamDestroy(h->st);
+    }
+    delete h;
+}
+
+const char* sphgpu_last_error(const sphgpu_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int sphgpu_set_scene_domain(sphgpu_handle* h, int scene, const float mn[3], const float mx[3]) {
+    if (!h || !mn || !mx) return SPHGPU_E_ARG;
+    if (h->initialized) return fail(h, SPHGPU_E_STATE, "scene domain after init");
+    if (scene < 0 || scene >= h->P.num_scenes) return fail(h, SPHGPU_E_ARG, "scene %d out of range", scene);
+    for (int a = 0; a < 3; ++a)
+        if (!(mx[a] > mn[a])) return fail(h, SPHGPU_E_ARG, "empty domain on axis %d", a);
+    std::memcpy(h->scenes[scene].mn, mn, sizeof(float) * 3);
+    std::memcpy(h->scenes[scene].mx, mx, sizeof(float) * 3);
+    h->scenes[scene].set = true;
+    if (scene == 0) {
+        std::memcpy(h->P.space_min, mn, sizeof(float) * 3);
+        std::memcpy(h->P.space_max, mx, sizeof(float) * 3);
+    }
+    return SPHGPU_OK;
+}
+
+int sphgpu_add_phase(sphgpu_handle* h, float rest_density, float visc_alpha, int* phase) {
+    if (!h) return SPHGPU_E_ARG;
+    if (h->initialized) return fail(h, SPHGPU_E_STATE, "phase after init");
+    if (!(rest_density > 0)) return fail(h, SPHGPU_E_ARG, "rest density must be > 0");
+    if ((int)h->phases.size() >= MAX_PHASES) return fail(h, SPHGPU_E_CAPACITY, "at most %d phases", MAX_PHASES);
+    h->phases.push_back({rest_density, visc_alpha});
+    if (phase) *phase = int(h->phases.size()) - 1;
+    return SPHGPU_OK;
+}
+
+int sphgpu_add_fluid(sphgpu_handle* h, int scene, int phase, long long n, const float* xyz,
+                     const float* vxyz, long long* first_id) {
+    if (!h) return SPHGPU_E_ARG;
+    if (h->initialized) return fail(h, SPHGPU_E_STATE, "fluid after init (append is not supported yet)");
+    if (scene < 0 || scene >= h->P.num_scenes) return fail(h, SPHGPU_E_ARG, "scene %d out of range", scene);
+    if (phase < 0 || phase >= (int)h->phases.size()) return fail(h, SPHGPU_E_ARG, "phase %d not defined", phase);
+    if (n < 0 || (n > 0 && !xyz)) return fail(h, SPHGPU_E_ARG, "bad particle array");
+    if (h->nextId + n > (1LL << 30)) return fail(h, SPHGPU_E_CAPACITY, "too many fluid particles");
+    if (first_id) *first_id = h->nextId;
+    size_t o = h->fx.size();
+    h->fx.insert(h->fx.end(), xyz, xyz + 3 * n);
+    if (vxyz) h->fv.insert(h->fv.end(), vxyz, vxyz + 3 * n);
+    else h->fv.resize(o + 3 * n, 0.0f);
+    h->fphase.insert(h->fphase.end(), (size_t)n, phase);
+    h->fscene.insert(h->fscene.end(), (size_t)n, scene);
+    h->nextId += n;
+    return SPHGPU_OK;
+}
+
+int sphgpu_add_solid(sphgpu_handle* h, int scene, long long n, const float* xyz, float visc_alpha,
+                     int* solid) {
+    if (!h) return SPHGPU_E_ARG;
+    if (h->initialized) return fail(h, SPHGPU_E_STATE, "solid after init");
+    if (scene < 0 || scene >= h->P.num_scenes) return fail(h, SPHGPU_E_ARG, "scene %d out of range", scene);
+    if (n < 0 || (n > 0 && !xyz)) return fail(h, SPHGPU_E_ARG, "bad particle array");
+    int sid = int(h->solidVisc.size());
+    h->solidVisc.push_back(visc_alpha);
+    h->bx.insert(h->bx.end(), xyz, xyz + 3 * n);
+    h->bsolid.insert(h->bsolid.end(), (size_t)n, sid);
+    h->bscene.insert(h->bscene.end(), (size_t)n, scene);
+    if (solid) *solid = sid;
+    return SPHGPU_OK;
+}
+
+int sphgpu_init(sphgpu_handle* h) {
+    if (!h) return SPHGPU_E_ARG;
+    if (h->initialized) return SPHGPU_OK;
+    const sphgpu_params& P = h->P;
+    if (h->phases.empty()) return fail(h, SPHGPU_E_STATE, "no fluid phase defined");
+    for (int s = 0; s < P.num_scenes; ++s)
+        if (!h->scenes[s].set) return fail(h, SPHGPU_E_STATE, "scene %d has no domain", s);
+    if (!sphgpu_device_available()) return fail(h, SPHGPU_E_DEVICE, "no gfx9 HIP device available");
+    // every particle must lie inside its scene domain (SphBase grows the domain to contain them)
+    for (size_t i = 0; i < h->fscene.size(); ++i) {
+        const SceneDesc& d = h->scenes[h->fscene[i]];
+        for (int a = 0; a < 3; ++a)
+            if (!(h->fx[3 * i + a] >= d.mn[a] && h->fx[3 * i + a] <= d.mx[a]))
+                return fail(h, SPHGPU_E_ARG, "fluid particle %zu outside its scene domain", i);
+    }
+    for (size_t i = 0; i < h->bscene.size(); ++i) {
+        const SceneDesc& d = h->scenes[h->bscene[i]];
+        for (int a = 0; a < 3; ++a)
+            if (!(h->bx[3 * i + a] >= d.mn[a] && h->bx[3 * i + a] <= d.mx[a]))
+                return fail(h, SPHGPU_E_ARG, "boundary particle %zu outside its scene domain", i);
+    }
+    for (float sv : h->solidVisc)
+        if (sv != h->solidVisc[0]) return fail(h, SPHGPU_E_ARG, "all solids must share viscosity_alpha");
+
+    HIPCHECK(h, hipSetDevice(0));
+    HIPCHECK(h, hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+    h->initialized = true;
+
+    std::vector<int> rowRank;
+    build_grids(h, rowRank);
+    uint64_t totalKeys = h->hKeyBase.back();
+    if (totalKeys >= 0xFFFFFFF0ull) return fail(h, SPHGPU_E_CAPACITY, "grid too large (%llu cells)", (unsigned long long)totalKeys);
+    int bits = 1;
+    while ((1ull << bits) <= totalKeys) ++bits;   // KEY_INVALID & mask must exceed every key
+    h->keyBits = bits;
+
+    const int ns = P.num_scenes;
+    // initial padded layout: scenes contiguous in add order
+    std::vector<int> count(ns, 0), padStart(ns + 1, 0);
+    for (int s : h->fscene) ++count[s];
+    for (int s = 0; s < ns; ++s) padStart[s + 1] = padStart[s] + (count[s] + TILE - 1) / TILE * TILE;
+    int npad = padStart[ns];
+    int ncap = std::max(npad, TILE);
+    long long nb = (long long)h->bscene.size();
+
+    Dev& D = h->D;
+    D.ncap = ncap;
+    D.npad = npad;
+    D.nb = int(nb);
+    D.nscenes = ns;
+    D.nphases = int(h->phases.size());
+    D.solver = P.solver;
+    const float s = P.spacing_r, H = P.smooth_radius_h;
+    D.kh = H / 2.0f;
+    D.inv_kh = 1.0f / D.kh;
+    D.sigma = 1.0f / float(M_PI) / (D.kh * D.kh * D.kh);
+    D.sigma_h = 1.0f / float(M_PI) / (D.kh * D.kh * D.kh) / D.kh;
+    D.W0 = D.sigma;
+    {
+        volatile float hh = s + s;
+        volatile float h2 = hh * hh;
+        D.h2 = h2;
+    }
+    D.Hs = H;
+    D.visc_eps = 0.01f * H * H;
+    D.v0 = s * s * s;
+    D.gx = P.gravity[0]; D.gy = P.gravity[1]; D.gz = P.gravity[2];
+    D.cs = P.sound_speed;
+    D.gamma = P.gamma;
+    D.nu = P.nu;
+    D.omega = P.omega;
+    for (int k = 0; k < MAX_PHASES; ++k) {
+        const Phase ph = k < (int)h->phases.size() ? h->phases[k] : h->phases[0];
+        D.rho0[k] = ph.rho0;
+        D.mass[k] = ph.rho0 * D.v0;
+        D.visc[k] = ph.visc;
+        D.tait_B[k] = ph.rho0 * P.sound_speed * P.sound_speed / P.gamma;
+    }
+    D.solid_visc = h->solidVisc.empty() ? 0.0f : h->solidVisc[0];
+    D.totalKeys = uint32_t(totalKeys);
+
+    int rc = 0;
+#define ALLOC(ptr, n) if ((rc = dalloc(h, &(ptr), (n)))) return rc
+    ALLOC(D.x, ncap); ALLOC(D.v, ncap); ALLOC(D.id, ncap); ALLOC(D.ksum, ncap); ALLOC(D.kvsum, ncap); ALLOC(D.key, ncap);
+    ALLOC(D.x2, ncap); ALLOC(D.v2, ncap); ALLOC(D.id2, ncap); ALLOC(D.ksum2, ncap); ALLOC(D.kvsum2, ncap); ALLOC(D.key2, ncap);
+    ALLOC(D.vs, ncap); ALLOC(D.f4a, ncap); ALLOC(D.f4b, ncap);
+    ALLOC(D.rho, ncap); ALLOC(D.alpha, ncap); ALLOC(D.kr, ncap); ALLOC(D.kr0, ncap); ALLOC(D.f1, ncap); ALLOC(D.f2, ncap);
+    ALLOC(D.nbF, (size_t)NBF_CAP * ncap); ALLOC(D.nbB, (size_t)NBB_CAP * ncap); ALLOC(D.cnt, ncap);
+    ALLOC(D.xb, nb); ALLOC(D.vb, nb); ALLOC(D.bid, nb); ALLOC(D.bsolid, nb); ALLOC(D.bscene, nb); ALLOC(D.bkself, nb); ALLOC(D.bkey, nb);
+    SceneGrid* dgrids; int* drow; uint32_t* dkb;
+    ALLOC(dgrids, ns); ALLOC(drow, rowRank.size()); ALLOC(dkb, ns + 1);
+    D.grids = dgrids; D.rowRank = drow; D.sceneKeyBase = dkb;
+    ALLOC(D.cellF, totalKeys); ALLOC(D.cellB, totalKeys);
+    ALLOC(D.sceneStart, ns); ALLOC(D.sceneCount, ns); ALLOC(D.scenePadStart, ns + 1);
+    int numTiles = ncap / TILE;
+    ALLOC(D.tileScene, numTiles); ALLOC(D.tileValid, numTiles);
+    ALLOC(D.dt, ns); ALLOC(D.active, ns); ALLOC(D.iters, ns); ALLOC(D.sceneErr, ns);
+    ALLOC(D.partial, 2 * (size_t)numTiles); ALLOC(D.counters, C_COUNT); ALLOC(D.vmaxPartial, numTiles);
+    ALLOC(h->layoutOut, 2); ALLOC(h->statsPart, 4 * (size_t)numTiles); ALLOC(h->statsOut, 4 * (size_t)ns);
+    int sortItems = std::max<int>(ncap, int(nb));
+    ALLOC(h->sk, sortItems); ALLOC(h->sv, sortItems);
+    void* sortMem;
+    HIPCHECK(h, hipMalloc(&sortMem, sort_scratch_bytes(sortItems)));
+    h->allocs.push_back(sortMem);
+    sort_scratch_bind(h->sort, sortMem, sortItems);
+#undef ALLOC
+    // pinned host mirrors (one block)
+    {
+        size_t words = 4 * (size_t)ns + 2 * (size_t)ns + C_COUNT + 8 * (size_t)ns + 2 + ns + 16;
+        void* hm;
+        HIPCHECK(h, hipHostMalloc(&hm, words * 8));
+        char* c = static_cast<char*>(hm);
+        h->hActive = reinterpret_cast<int*>(c); c += 8 * (size_t)ns;
+        h->hIters = reinterpret_cast<int*>(c); c += 8 * (size_t)ns;
+        h->hErr = reinterpret_cast<double*>(c); c += 8 * (size_t)ns;
+        h->hStats = reinterpret_cast<double*>(c); c += 32 * (size_t)ns;
+        h->hCounters = reinterpret_cast<int*>(c); c += 8 * C_COUNT;
+        h->hLayout = reinterpret_cast<int*>(c); c += 16;
+        h->hDt = reinterpret_cast<float*>(c);
+    }
+
+    hipStream_t st = h->st;
+    HIPCHECK(h, hipMemcpyAsync(dgrids, h->hgrids.data(), sizeof(SceneGrid) * ns, hipMemcpyHostToDevice, st));
+    HIPCHECK(h, hipMemcpyAsync(drow, rowRank.data(), sizeof(int) * rowRank.size(), hipMemcpyHostToDevice, st));
+    HIPCHECK(h, hipMemcpyAsync(dkb, h->hKeyBase.data(), sizeof(uint32_t) * (ns + 1), hipMemcpyHostToDevice, st));
+    HIPCHECK(h, hipMemsetAsync(D.cellF, 0, sizeof(int2) * totalKeys, st));
+    HIPCHECK(h, hipMemsetAsync(D.cellB, 0, sizeof(int2) * totalKeys, st));
+    HIPCHECK(h, hipMemsetAsync(D.counters, 0, sizeof(int) * C_COUNT, st));
+    HIPCHECK(h, hipMemsetAsync(D.ksum, 0, sizeof(float) * ncap, st));
+    HIPCHECK(h, hipMemsetAsync(D.kvsum, 0, sizeof(float) * ncap, st));
+    HIPCHECK(h, hipMemsetAsync(D.cnt, 0, sizeof(int) * ncap, st));
+    HIPCHECK(h, hipMemsetAsync(D.key, 0xFF, sizeof(uint32_t) * ncap, st));
+    HIPCHECK(h, hipMemsetAsync(D.key2, 0xFF, sizeof(uint32_t) * ncap, st));
+
+    // fluid upload in the initial padded layout
+    {
+        std::vector<float4> hx(ncap, make_float4(0, 0, 0, 0)), hv(ncap, make_float4(0, 0, 0, 0));
+        std::vector<int> hid(ncap, -1);
+        std::vector<int> fill(ns, 0);
+        for (size_t i = 0; i < h->fscene.size(); ++i) {
+            int sc = h->fscene[i];
+            int slot = padStart[sc] + fill[sc]++;
+            hx[slot] = make_float4(h->fx[3 * i], h->fx[3 * i + 1], h->fx[3 * i + 2], float(h->fphase[i]));
+            hv[slot] = make_float4(h->fv[3 * i], h->fv[3 * i + 1], h->fv[3 * i + 2], 0.0f);
+            hid[slot] = int(i);
+        }
+        std::vector<int> tileScene(numTiles, 0), tileValid(numTiles, 0);
+        for (int sc = 0; sc < ns; ++sc)
+            for (int t = padStart[sc] / TILE; t < padStart[sc + 1] / TILE; ++t) {
+                tileScene[t] = sc;
+                tileValid[t] = std::min(TILE, std::max(0, count[sc] - (t * TILE - padStart[sc])));
+            }
+        std::vector<int> sStart(ns, 0);
+        HIPCHECK(h, hipMemcpy(D.x, hx.data(), sizeof(float4) * ncap, hipMemcpyHostToDevice));
+        HIPCHECK(h, hipMemcpy(D.v, hv.data(), sizeof(float4) * ncap, hipMemcpyHostToDevice));
+        HIPCHECK(h, hipMemcpy(D.id, hid.data(), sizeof(int) * ncap, hipMemcpyHostToDevice));
+        HIPCHECK(h, hipMemcpy(D.tileScene, tileScene.data(), sizeof(int) * numTiles, hipMemcpyHostToDevice));
+        HIPCHECK(h, hipMemcpy(D.tileValid, tileValid.data(), sizeof(int) * numTiles, hipMemcpyHostToDevice));
+        HIPCHECK(h, hipMemcpy(D.sceneCount, count.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+        HIPCHECK(h, hipMemcpy(D.scenePadStart, padStart.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice));
+        HIPCHECK(h, hipMemcpy(D.sceneStart, sStart.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+        std::vector<float> dt0(ns, P.dt);
+        HIPCHECK(h, hipMemcpy(D.dt, dt0.data(), sizeof(float) * ns, hipMemcpyHostToDevice));
+        h->nLive = (long long)h->fscene.size();
+    }
+    // boundary: sort by cell once, Akinci volumes (prepareSolidPartWeight)
+    if (nb > 0) {
+        std::vector<float4> hb(nb);
+        for (long long i = 0; i < nb; ++i) hb[i] = make_float4(h->bx[3 * i], h->bx[3 * i + 1], h->bx[3 * i + 2], 0.0f);
+        float4* tx; int* tsolid; int* tscene;
+        if ((rc = dalloc(h, &tx, nb))) return rc;
+        if ((rc = dalloc(h, &tsolid, nb))) return rc;
+        if ((rc = dalloc(h, &tscene, nb))) return rc;
+        HIPCHECK(h, hipMemcpy(tx, hb.data(), sizeof(float4) * nb, hipMemcpyHostToDevice));
+        HIPCHECK(h, hipMemcpy(tsolid, h->bsolid.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+        HIPCHECK(h, hipMemcpy(tscene, h->bscene.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+        launch_boundary_keys_from(D, tx, tscene, h->sk, h->sv, st);
+        radix_sort_pairs(h->sk, h->sv, int(nb), h->keyBits, h->sort, st);
+        launch_boundary_reorder(D, tx, tsolid, tscene, h->sk, h->sv, st);
+        launch_cell_build(D, D.bkey, int(nb), D.cellB, st);
+        launch_boundary_volume(D, st);
+    }
+    HIPCHECK(h, hipStreamSynchronize(st));
+    HIPCHECK(h, hipGetLastError());
+    return SPHGPU_OK;
+}
+
+int sphgpu_step(sphgpu_handle* h, sphgpu_stats* st) {
+    if (!h) return SPHGPU_E_ARG;
+    int rc = sphgpu_init(h);
+    if (rc) return rc;
+    if (h->nLive == 0 && h->frame > 0) return fail(h, SPHGPU_E_STATE, "no fluid particles left");
+    for (auto& s : h->sceneStats) s = sphgpu_stats{};
+    switch (h->P.solver) {
+        case SPHGPU_WCSPH: return step_wcsph(h, st);
+        case SPHGPU_DFSPH: return step_dfsph(h, st);
+        default: return step_iisph(h, st);
+    }
+}
+
+int sphgpu_scene_stats(const sphgpu_handle* h, int scene, sphgpu_stats* st) {
+    if (!h || !st || scene < 0 || scene >= (int)h->sceneStats.size()) return SPHGPU_E_ARG;
+    *st = h->sceneStats[scene];
+    return SPHGPU_OK;
+}
+
+int sphgpu_synchronize(sphgpu_handle* h) {
+    if (!h) return SPHGPU_E_ARG;
+    if (!h->initialized) return SPHGPU_OK;
+    HIPCHECK(h, hipStreamSynchronize(h->st));
+    return SPHGPU_OK;
+}
+
+long long sphgpu_num_fluid(const sphgpu_handle* h) {
+    if (!h) return SPHGPU_E_ARG;
+    return h->initialized ? h->nLive : (long long)h->fscene.size();
+}
+
+namespace {
+// live slots in ascending id order: returns (slot, id) pairs
+int live_slots(sphgpu_handle* h, std::vector<std::pair<int, int>>& out) {
+    Dev& D = h->D;
+    std::vector<int> ids(D.npad);
+    HIPCHECK(h, hipStreamSynchronize(h->st));
+    HIPCHECK(h, hipMemcpy(ids.data(), D.id, sizeof(int) * D.npad, hipMemcpyDeviceToHost));
+    std::vector<int> tv(D.npad / TILE);
+    HIPCHECK(h, hipMemcpy(tv.data(), D.tileValid, sizeof(int) * tv.size(), hipMemcpyDeviceToHost));
+    out.clear();
+    for (int i = 0; i < D.npad; ++i) {
+        int t = i / TILE;
+        if ((i - t * TILE) < tv[t] && ids[i] >= 0) out.push_back({ids[i], i});
+    }
+    std::sort(out.begin(), out.end());
+    return 0;
+}
+}  // namespace
+
+int sphgpu_get_fluid(sphgpu_handle* h, int field, float* out) {
+    if (!h || !out) return SPHGPU_E_ARG;
+    if (!h->initialized) return fail(h, SPHGPU_E_STATE, "not initialised");
+    Dev& D = h->D;
+    std::vector<std::pair<int, int>> ls;
+    int rc = live_slots(h, ls);
+    if (rc) return rc;
+    int comp = 1;
+    const void* src = nullptr;
+    bool f4 = false;
+    switch (field) {
+        case SPHGPU_F_POS: src = D.x; f4 = true; comp = 3; break;
+        case SPHGPU_F_VEL: src = D.v; f4 = true; comp = 3; break;
+        case SPHGPU_F_RHO: src = D.rho; break;
+        case SPHGPU_F_PRESSURE: src = (h->P.solver == SPHGPU_IISPH) ? D.kr : D.f1; break;
+        case SPHGPU_F_ACC: src = (h->P.solver == SPHGPU_IISPH) ? D.f4b : D.f4a; f4 = true; comp = 3; break;
+        case SPHGPU_F_ALPHA: src = D.alpha; break;
+        case SPHGPU_F_KAPPA_SUM: src = D.ksum; break;
+        case SPHGPU_F_PHASE: src = D.x; f4 = true; comp = 1; break;
+        default: return fail(h, SPHGPU_E_ARG, "unknown field %d", field);
+    }
+    if (f4) {
+        std::vector<float4> buf(D.npad);
+        HIPCHECK(h, hipMemcpy(buf.data(), src, sizeof(float4) * D.npad, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < ls.size(); ++k) {
+            const float4& q = buf[ls[k].second];
+            if (field == SPHGPU_F_PHASE) out[k] = q.w;
+            else { out[3 * k] = q.x; out[3 * k + 1] = q.y; out[3 * k + 2] = q.z; }
+        }
+    } else {
+        std::vector<float> buf(D.npad);
+        HIPCHECK(h, hipMemcpy(buf.data(), src, sizeof(float) * D.npad, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < ls.size(); ++k) out[k] = buf[ls[k].second];
+    }
+    (void)comp;
+    return SPHGPU_OK;
+}
+
+int sphgpu_get_fluid_ids(sphgpu_handle* h, long long* out) {
+    if (!h || !out) return SPHGPU_E_ARG;
+    if (!h->initialized) return fail(h, SPHGPU_E_STATE, "not initialised");
+    std::vector<std::pair<int, int>> ls;
+    int rc = live_slots(h, ls);
+    if (rc) return rc;
+    for (size_t k = 0; k < ls.size(); ++k) out[k] = ls[k].first;
+    return SPHGPU_OK;
+}
+
+int sphgpu_get_boundary_volume(sphgpu_handle* h, float* out) {
+    if (!h || !out) return SPHGPU_E_ARG;
+    int rc = sphgpu_init(h);
+    if (rc) return rc;
+    Dev& D = h->D;
+    std::vector<float4> xb(D.nb);
+    std::vector<int> bid(D.nb);
+    HIPCHECK(h, hipStreamSynchronize(h->st));
+    HIPCHECK(h, hipMemcpy(xb.data(), D.xb, sizeof(float4) * D.nb, hipMemcpyDeviceToHost));
+    HIPCHECK(h, hipMemcpy(bid.data(), D.bid, sizeof(int) * D.nb, hipMemcpyDeviceToHost));
+    for (int i = 0; i < D.nb; ++i) out[bid[i]] = xb[i].w;
+    return SPHGPU_OK;
+}
+
+int sphgpu_get_neighbours(sphgpu_handle* h, int cap, int* counts, int* kinds, long long* ids, float* dis) {
+    if (!h || cap <= 0 || !counts || !kinds || !ids || !dis) return SPHGPU_E_ARG;
+    int rc = sphgpu_init(h);
+    if (rc) return rc;
+    Dev& D = h->D;
+    run_ns(h);
+    if (h->P.solver == SPHGPU_DFSPH && h->frame > 0) launch_df_density_alpha(D, h->st);
+    std::vector<std::pair<int, int>> ls;
+    if ((rc = live_slots(h, ls))) return rc;
+    std::vector<int> rankOf(D.npad, -1);
+    for (size_t k = 0; k < ls.size(); ++k) rankOf[ls[k].second] = int(k);
+    size_t n = ls.size();
+    int *dRank, *dCounts, *dKinds;
+    long long* dIds;
+    float* dDis;
+    HIPCHECK(h, hipMalloc(&dRank, sizeof(int) * std::max<size_t>(D.npad, 1)));
+    HIPCHECK(h, hipMalloc(&dCounts, sizeof(int) * std::max<size_t>(n, 1)));
+    HIPCHECK(h, hipMalloc(&dKinds, sizeof(int) * std::max<size_t>(n * cap, 1)));
+    HIPCHECK(h, hipMalloc(&dIds, sizeof(long long) * std::max<size_t>(n * cap, 1)));
+    HIPCHECK(h, hipMalloc(&dDis, sizeof(float) * std::max<size_t>(n * cap, 1)));
+    HIPCHECK(h, hipMemcpy(dRank, rankOf.data(), sizeof(int) * D.npad, hipMemcpyHostToDevice));
+    launch_neighbour_dump_ranked(D, cap, dCounts, dKinds, dIds, dDis, dRank, h->st);
+    HIPCHECK(h, hipStreamSynchronize(h->st));
+    HIPCHECK(h, hipGetLastError());
+    std::vector<int> hk(n * cap);
+    std::vector<long long> hi(n * cap);
+    std::vector<float> hd(n * cap);
+    HIPCHECK(h, hipMemcpy(counts, dCounts, sizeof(int) * n, hipMemcpyDeviceToHost));
+    HIPCHECK(h, hipMemcpy(hk.data(), dKinds, sizeof(int) * n * cap, hipMemcpyDeviceToHost));
+    HIPCHECK(h, hipMemcpy(hi.data(), dIds, sizeof(long long) * n * cap, hipMemcpyDeviceToHost));
+    HIPCHECK(h, hipMemcpy(hd.data(), dDis, sizeof(float) * n * cap, hipMemcpyDeviceToHost));
+    hipFree(dRank); hipFree(dCounts); hipFree(dKinds); hipFree(dIds); hipFree(dDis);
+    for (size_t k = 0; k < n; ++k) {
+        int c = std::min(counts[k], cap);
+        std::vector<int> order(c);
+        std::iota(order.begin(), order.end(), 0);
+        size_t b = k * cap;
+        std::sort(order.begin(), order.end(), [&](int a, int z) {
+            return hk[b + a] != hk[b + z] ? hk[b + a] < hk[b + z] : hi[b + a] < hi[b + z];
+        });
+        for (int q = 0; q < c; ++q) {
+            kinds[b + q] = hk[b + order[q]];
+            ids[b + q] = hi[b + order[q]];
+            dis[b + q] = hd[b + order[q]];
+        }
+    }
+    return SPHGPU_OK;
+}
+
+int sphgpu_profile_kernel(sphgpu_handle* h, const char* name) {
+    if (!h) return SPHGPU_E_ARG;
+    h->profName = name ? name : "";
+    h->profUsed = 0;
+    return SPHGPU_OK;
+}
+
+int sphgpu_kernel_timing(sphgpu_handle* h, long long* launches, double* total_ms, double* total_bytes) {
+    if (!h) return SPHGPU_E_ARG;
+    if (h->initialized) HIPCHECK(h, hipStreamSynchronize(h->st));
+    double ms = 0, bytes = 0;
+    for (size_t k = 0; k < h->profUsed; ++k) {
+        float t = 0;
+        HIPCHECK(h, hipEventElapsedTime(&t, h->prof[k].a, h->prof[k].b));
+        ms += t;
+        bytes += h->prof[k].bytes;
+    }
+    if (launches) *launches = (long long)h->profUsed;
+    if (total_ms) *total_ms = ms;
+    if (total_bytes) *total_bytes = bytes;
+    return SPHGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,388 @@
+// Uniform-grid neighbour search on the GPU (the reference's SphBase::gridSetup
+// + neighbourSearch, SphBase.cpp:849-1089, re-designed): cell keys -> radix
+// sort -> scene layout padded to whole tiles -> SoA reorder -> per-cell
+// [start,end) tables -> ELL neighbour lists built from 9 contiguous x-runs of
+// three cells each.  The pair test keeps the reference's FP32 rounding so
+// that the neighbour sets are bit-identical.
+#include "sph_common.h"
+
+namespace sphgpu {
+
+namespace {
+
+__device__ __forceinline__ bool slot_valid(const Dev& D, int i, int* scene) {
+    int tile = i / TILE;
+    *scene = D.tileScene[tile];
+    return (i - tile * TILE) < D.tileValid[tile];
+}
+
+// cell coordinates of a point inside scene grid G, clamped into the grid
+__device__ __forceinline__ void cell_of(const SceneGrid& G, float4 p, int& cx, int& cy, int& cz) {
+    cx = cell_coord(p.x, G.minx, G.divisor);
+    cy = cell_coord(p.y, G.miny, G.divisor);
+    cz = cell_coord(p.z, G.minz, G.divisor);
+    cx = min(max(cx, 0), G.nx - 1);
+    cy = min(max(cy, 0), G.ny - 1);
+    cz = min(max(cz, 0), G.nz - 1);
+}
+
+__device__ __forceinline__ uint32_t key_of(const SceneGrid& G, const int* rowRank, int cx, int cy, int cz) {
+    int row = rowRank[G.rowBase + cz * G.ny + cy];
+    return G.keyBase + (uint32_t)row * (uint32_t)G.nx + (uint32_t)cx;
+}
+
+__global__ __launch_bounds__(256) void k_fluid_keys(Dev D, uint32_t* keys, uint32_t* vals) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= D.npad) return;
+    uint32_t k = KEY_INVALID;
+    if (D.id[i] >= 0) {
+        float4 p = D.x[i];
+        int s = D.tileScene[i / TILE];
+        const SceneGrid G = D.grids[s];
+        int cx, cy, cz;
+        cell_of(G, p, cx, cy, cz);
+        k = key_of(G, D.rowRank, cx, cy, cz);
+    }
+    keys[i] = k;
+    vals[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_scene_bounds(Dev D, const uint32_t* __restrict__ sk, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = sk[i];
+    if (k == KEY_INVALID) return;
+    int s = scene_of_key(k, D.sceneKeyBase, D.nscenes);
+    bool first = (i == 0) || sk[i - 1] == KEY_INVALID ||
+                 scene_of_key(sk[i - 1], D.sceneKeyBase, D.nscenes) != s;
+    bool last = (i == n - 1) || sk[i + 1] == KEY_INVALID ||
+                scene_of_key(sk[i + 1], D.sceneKeyBase, D.nscenes) != s;
+    if (first) D.sceneStart[s] = i;
+    if (last) D.sceneCount[s] = i + 1;   // end for now; the layout kernel turns it into a count
+}
+
+// single thread: counts, padded starts, tile tables.  hostOut (mapped to the
+// host by the caller via a device buffer) receives [npad_new, n_live].
+__global__ void k_scene_layout(Dev D, int* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int pad = 0, live = 0;
+    for (int s = 0; s < D.nscenes; ++s) {
+        int st = D.sceneStart[s], en = D.sceneCount[s];
+        int c = (en > st) ? en - st : 0;
+        if (st < 0) c = 0;
+        D.sceneCount[s] = c;
+        D.scenePadStart[s] = pad;
+        pad += (c + TILE - 1) / TILE * TILE;
+        live += c;
+    }
+    D.scenePadStart[D.nscenes] = pad;
+    out[0] = pad;
+    out[1] = live;
+}
+
+__global__ __launch_bounds__(256) void k_tiles(Dev D, int numTiles) {
+    int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= numTiles) return;
+    int slot = t * TILE;
+    int s = 0;
+    // largest s with padStart[s] <= slot
+    int lo = 0, hi = D.nscenes - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (D.scenePadStart[mid] <= slot) lo = mid; else hi = mid - 1;
+    }
+    s = lo;
+    int valid = 0;
+    if (slot < D.scenePadStart[D.nscenes]) {
+        int local = slot - D.scenePadStart[s];
+        valid = min(max(D.sceneCount[s] - local, 0), TILE);
+    }
+    D.tileScene[t] = s;
+    D.tileValid[t] = valid;
+}
+
+// scatter sorted rank r to its padded slot; also marks padding slots
+__global__ __launch_bounds__(256) void k_reorder(Dev D, const uint32_t* __restrict__ sk,
+                                               const uint32_t* __restrict__ sv, int n) {
+    int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    uint32_t k = sk[r];
+    if (k == KEY_INVALID) return;
+    int s = scene_of_key(k, D.sceneKeyBase, D.nscenes);
+    int p = r - D.sceneStart[s] + D.scenePadStart[s];
+    int src = (int)sv[r];
+    D.x2[p] = D.x[src];
+    D.v2[p] = D.v[src];
+    D.id2[p] = D.id[src];
+    D.ksum2[p] = D.ksum[src];
+    D.kvsum2[p] = D.kvsum[src];
+    D.key2[p] = k;
+}
+
+__global__ __launch_bounds__(256) void k_mark_padding(Dev D, int limit) {
+    int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= limit) return;
+    int tile = t / TILE;
+    if ((t - tile * TILE) >= D.tileValid[tile]) {
+        D.id2[t] = -1;
+        D.key2[t] = KEY_INVALID;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cell_reset(const uint32_t* __restrict__ keys, int n,
+                                                  int2* __restrict__ table) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = keys[i];
+    if (k == KEY_INVALID) return;
+    if (i == 0 || keys[i - 1] != k) table[k] = make_int2(0, 0);
+}
+
+__global__ __launch_bounds__(256) void k_cell_build(const uint32_t* __restrict__ keys, int n,
+                                                  int2* __restrict__ table) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = keys[i];
+    if (k == KEY_INVALID) return;
+    if (i == 0 || keys[i - 1] != k) table[k].x = i;
+    if (i == n - 1 || keys[i + 1] != k) table[k].y = i + 1;
+}
+
+// union of the [start,end) ranges of three consecutive keys (empty = start >= end)
+__device__ __forceinline__ int2 run3(const int2* __restrict__ table, uint32_t k0, int cx, int nx) {
+    int s = 0x7fffffff, e = 0;
+    for (int dx = -1; dx <= 1; ++dx) {
+        int c = cx + dx;
+        if (c < 0 || c >= nx) continue;
+        int2 r = table[k0 + dx];
+        if (r.y > r.x) {
+            s = min(s, r.x);
+            e = max(e, r.y);
+        }
+    }
+    return make_int2(s, e);
+}
+
+__global__ __launch_bounds__(256) void k_neighbours(Dev D) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    int scene;
+    if (i >= D.npad) return;
+    if (!slot_valid(D, i, &scene)) {
+        D.cnt[i] = 0;
+        return;
+    }
+    const SceneGrid G = D.grids[scene];
+    float4 pa = D.x[i];
+    int cx, cy, cz;
+    cell_of(G, pa, cx, cy, cz);
+    int nf = 0, nbn = 0, over = 0;
+    const int stride = D.ncap;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int z = cz + dz;
+        if (z < 0 || z >= G.nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int y = cy + dy;
+            if (y < 0 || y >= G.ny) continue;
+            uint32_t k0 = key_of(G, D.rowRank, cx, y, z);
+            int2 rf = run3(D.cellF, k0, cx, G.nx);
+            for (int j = rf.x; j < rf.y; ++j) {
+                if (j == i) continue;
+                float4 pb = D.x[j];
+                if (dis_square_exact(pa, pb) < D.h2) {
+                    if (nf < NBF_CAP) D.nbF[nf * stride + i] = j;
+                    else ++over;
+                    ++nf;
+                }
+            }
+            int2 rb = run3(D.cellB, k0, cx, G.nx);
+            for (int j = rb.x; j < rb.y; ++j) {
+                float4 pb = D.xb[j];
+                if (dis_square_exact(pa, pb) < D.h2) {
+                    if (nbn < NBB_CAP) D.nbB[nbn * stride + i] = j;
+                    else ++over;
+                    ++nbn;
+                }
+            }
+        }
+    }
+    D.cnt[i] = min(nf, NBF_CAP) | (min(nbn, NBB_CAP) << 16);
+    int tm = nf + nbn - REF_NEIGHBOUR_CAP;
+    if (tm > 0) atomicAdd(&D.counters[C_TOO_MANY], tm);
+    if (over > 0) atomicAdd(&D.counters[C_LIST_OVF], over);
+}
+
+// ---- boundary --------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_boundary_keys(Dev D, const float4* __restrict__ xs,
+                                                     const int* __restrict__ scene, uint32_t* keys,
+                                                     uint32_t* vals) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= D.nb) return;
+    const SceneGrid G = D.grids[scene[i]];
+    int cx, cy, cz;
+    cell_of(G, xs[i], cx, cy, cz);
+    keys[i] = key_of(G, D.rowRank, cx, cy, cz);
+    vals[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_boundary_reorder(Dev D, const float4* __restrict__ xs,
+                                                        const int* __restrict__ solid,
+                                                        const int* __restrict__ scene,
+                                                        const uint32_t* __restrict__ sk,
+                                                        const uint32_t* __restrict__ sv) {
+    int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= D.nb) return;
+    int src = (int)sv[r];
+    D.xb[r] = xs[src];
+    D.vb[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+    D.bid[r] = src;
+    D.bsolid[r] = solid[src];
+    D.bscene[r] = scene[src];
+    D.bkey[r] = sk[r];
+}
+
+// SphBase::prepareSolidPartWeight (SphBase.cpp:766-813): V_b = 1/(W(0) + sum over
+// boundary neighbours W(r)); kerSumFromSelf = W(0) + same-solid part.
+__global__ __launch_bounds__(256) void k_boundary_volume(Dev D) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= D.nb) return;
+    const SceneGrid G = D.grids[D.bscene[i]];
+    float4 pa = D.xb[i];
+    int sa = D.bsolid[i];
+    int cx, cy, cz;
+    cell_of(G, pa, cx, cy, cz);
+    float sumAll = D.W0, sumSelf = D.W0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int z = cz + dz;
+        if (z < 0 || z >= G.nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int y = cy + dy;
+            if (y < 0 || y >= G.ny) continue;
+            uint32_t k0 = key_of(G, D.rowRank, cx, y, z);
+            int2 rb = run3(D.cellB, k0, cx, G.nx);
+            for (int j = rb.x; j < rb.y; ++j) {
+                if (j == i) continue;
+                float4 pb = D.xb[j];
+                float l2 = dis_square_exact(pa, pb);
+                if (l2 < D.h2) {
+                    float w = kernel_W(sqrtf(l2), D);
+                    sumAll += w;
+                    if (D.bsolid[j] == sa) sumSelf += w;
+                }
+            }
+        }
+    }
+    pa.w = 1.0f / sumAll;
+    D.xb[i] = pa;
+    D.bkself[i] = sumSelf;
+}
+
+// test dump: neighbour sets with original ids and correctly rounded distances
+__global__ __launch_bounds__(256) void k_neighbour_dump(Dev D, int cap, int* counts, int* kinds,
+                                                      long long* ids, float* dis, const int* rankOf) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    int scene;
+    if (i >= D.npad || !slot_valid(D, i, &scene)) return;
+    int o = rankOf[i];
+    if (o < 0) return;
+    const SceneGrid G = D.grids[scene];
+    float4 pa = D.x[i];
+    int cx, cy, cz;
+    cell_of(G, pa, cx, cy, cz);
+    int c = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int z = cz + dz;
+        if (z < 0 || z >= G.nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int y = cy + dy;
+            if (y < 0 || y >= G.ny) continue;
+            uint32_t k0 = key_of(G, D.rowRank, cx, y, z);
+            int2 rf = run3(D.cellF, k0, cx, G.nx);
+            for (int j = rf.x; j < rf.y; ++j) {
+                if (j == i) continue;
+                float l2 = dis_square_exact(pa, D.x[j]);
+                if (l2 < D.h2) {
+                    if (c < cap) {
+                        size_t q = (size_t)o * cap + c;
+                        kinds[q] = 0; ids[q] = D.id[j]; dis[q] = sqrtf(l2);
+                    }
+                    ++c;
+                }
+            }
+            int2 rb = run3(D.cellB, k0, cx, G.nx);
+            for (int j = rb.x; j < rb.y; ++j) {
+                float l2 = dis_square_exact(pa, D.xb[j]);
+                if (l2 < D.h2) {
+                    if (c < cap) {
+                        size_t q = (size_t)o * cap + c;
+                        kinds[q] = 1; ids[q] = D.bid[j]; dis[q] = sqrtf(l2);
+                    }
+                    ++c;
+                }
+            }
+        }
+    }
+    counts[o] = c;
+}
+
+inline int blocks_for(int n) { return (n + 255) / 256; }
+
+}  // namespace
+
+void launch_fluid_keys_into(const Dev& D, uint32_t* keys, uint32_t* vals, hipStream_t st) {
+    hipLaunchKernelGGL(k_fluid_keys, dim3(blocks_for(D.npad)), dim3(256), 0, st, D, keys, vals);
+}
+
+void launch_scene_bounds(const Dev& D, const uint32_t* sk, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_scene_bounds, dim3(blocks_for(n)), dim3(256), 0, st, D, sk, n);
+}
+
+void launch_scene_layout(const Dev& D, int* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_scene_layout, dim3(1), dim3(64), 0, st, D, out);
+    int numTiles = D.ncap / TILE;
+    hipLaunchKernelGGL(k_tiles, dim3(blocks_for(numTiles)), dim3(256), 0, st, D, numTiles);
+}
+
+void launch_reorder(const Dev& D, const uint32_t* sk, const uint32_t* sv, int n, int oldPad,
+                    hipStream_t st) {
+    int limit = oldPad > 0 ? oldPad : n;
+    hipLaunchKernelGGL(k_mark_padding, dim3(blocks_for(limit)), dim3(256), 0, st, D, limit);
+    hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n)), dim3(256), 0, st, D, sk, sv, n);
+}
+
+void launch_cell_reset(const Dev& D, const uint32_t* keys, int n, int2* table, hipStream_t st) {
+    (void)D;
+    if (n > 0) hipLaunchKernelGGL(k_cell_reset, dim3(blocks_for(n)), dim3(256), 0, st, keys, n, table);
+}
+
+void launch_cell_build(const Dev& D, const uint32_t* keys, int n, int2* table, hipStream_t st) {
+    (void)D;
+    if (n > 0) hipLaunchKernelGGL(k_cell_build, dim3(blocks_for(n)), dim3(256), 0, st, keys, n, table);
+}
+
+void launch_neighbours(const Dev& D, hipStream_t st) {
+    hipLaunchKernelGGL(k_neighbours, dim3(blocks_for(D.npad)), dim3(256), 0, st, D);
+}
+
+void launch_boundary_keys_from(const Dev& D, const float4* xs, const int* scene, uint32_t* keys,
+                               uint32_t* vals, hipStream_t st) {
+    hipLaunchKernelGGL(k_boundary_keys, dim3(blocks_for(D.nb)), dim3(256), 0, st, D, xs, scene, keys, vals);
+}
+
+void launch_boundary_reorder(const Dev& D, const float4* srcX, const int* srcSolid, const int* srcScene,
+                             const uint32_t* sk, const uint32_t* sv, hipStream_t st) {
+    hipLaunchKernelGGL(k_boundary_reorder, dim3(blocks_for(D.nb)), dim3(256), 0, st, D, srcX, srcSolid,
+                       srcScene, sk, sv);
+}
+
+void launch_boundary_volume(const Dev& D, hipStream_t st) {
+    hipLaunchKernelGGL(k_boundary_volume, dim3(blocks_for(D.nb)), dim3(256), 0, st, D);
+}
+
+void launch_neighbour_dump_ranked(const Dev& D, int cap, int* counts, int* kinds, long long* ids,
+                                  float* dis, const int* rankOf, hipStream_t st) {
+    hipLaunchKernelGGL(k_neighbour_dump, dim3(blocks_for(D.npad)), dim3(256), 0, st, D, cap, counts,
+                       kinds, ids, dis, rankOf);
+}
+
+}  // namespace sphgpu

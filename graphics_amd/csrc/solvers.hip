@@ -1,0 +1,841 @@
+// Per-particle neighbour passes of the three solvers.  Each kernel is one
+// loop over particles of the reference (cited per kernel), one thread per
+// particle, neighbours read through the ELL lists built by grid.hip.  Sums
+// that decide convergence are reduced in FP64: a fixed-order block tree per
+// tile (deterministic), then a fixed-order per-scene reduction.
+#include "sph_common.h"
+
+namespace sphgpu {
+
+namespace {
+
+struct Slot {
+    int i, tile, scene;
+    bool live;
+};
+
+__device__ __forceinline__ Slot slot_of(const Dev& D, bool gated) {
+    Slot s;
+    s.tile = blockIdx.x;
+    s.i = s.tile * TILE + threadIdx.x;
+    s.scene = D.tileScene[s.tile];
+    s.live = (int)threadIdx.x < D.tileValid[s.tile];
+    if (gated) s.live = s.live && D.active[s.scene] != 0;
+    return s;
+}
+
+__device__ __forceinline__ int phase_of(float4 x) { return (int)x.w; }
+
+__device__ __forceinline__ void write_partial(const Dev& D, int tile, int q, double v, double* lds) {
+    double t = block_sum_256(v, lds);
+    if (threadIdx.x == 0) D.partial[2 * tile + q] = t;
+}
+
+// ---------------------------------------------------------------- CFL ----
+__global__ __launch_bounds__(256) void k_cfl_partial(Dev D) {
+    __shared__ float lds[4];
+    Slot s = slot_of(D, false);
+    float v2 = 0.0f;
+    if (s.live) {
+        float4 v = D.v[s.i];
+        v2 = v.x * v.x + v.y * v.y + v.z * v.z;
+    }
+    float m = block_max_256(v2, lds);
+    if (threadIdx.x == 0) D.vmaxPartial[s.tile] = m;
+}
+
+__global__ __launch_bounds__(256) void k_cfl_reduce(Dev D, float cfl, float spacing, float dtMax,
+                                                  float dtMin) {
+    __shared__ float lds[4];
+    int sc = blockIdx.x;
+    int t0 = D.scenePadStart[sc] / TILE, t1 = D.scenePadStart[sc + 1] / TILE;
+    float m = 0.0f;
+    for (int t = t0 + threadIdx.x; t < t1; t += 256) m = fmaxf(m, D.vmaxPartial[t]);
+    m = block_max_256(m, lds);
+    if (threadIdx.x == 0) {
+        float dt = dtMax;
+        if (cfl > 0.0f) {
+            float vmax = sqrtf(m);
+            if (vmax > 0.0f) dt = fminf(dtMax, cfl * spacing / vmax);
+            dt = fmaxf(dt, dtMin);
+        }
+        D.dt[sc] = dt;
+    }
+}
+
+// ------------------------------------------------------- scene reduce ----
+// mode 0: DFSPH density  err = sum(rho*/rho0)/n - 1          (DFSPH.cpp:696)
+// mode 1: DFSPH divergence / IISPH error   err = sum/n      (DFSPH.cpp:896, IISph_M.cpp:353)
+// Loop continuation (while err > tol || it < minIt), stopping at `cap`.
+__global__ __launch_bounds__(256) void k_scene_reduce(Dev D, int mode, float tol, int minIt, int cap) {
+    __shared__ double lds[4];
+    int sc = blockIdx.x;
+    if (!D.active[sc]) return;   // uniform per block
+    int t0 = D.scenePadStart[sc] / TILE, t1 = D.scenePadStart[sc + 1] / TILE;
+    double s = 0.0;
+    for (int t = t0 + threadIdx.x; t < t1; t += 256) s += D.partial[2 * t];
+    s = block_sum_256(s, lds);
+    if (threadIdx.x == 0) {
+        int n = D.sceneCount[sc];
+        double err = n > 0 ? s / n : 0.0;
+        if (mode == 0) err = n > 0 ? err - 1.0 : 0.0;
+        int it = D.iters[sc] + 1;
+        D.iters[sc] = it;
+        D.sceneErr[sc] = err;
+        bool cont = (err > (double)tol) || (it < minIt);
+        if (cont && it >= cap) {
+            cont = false;
+            D.counters[C_COUNT - 1] = 1;  // max-iteration flag
+        }
+        D.active[sc] = cont ? 1 : 0;
+    }
+}
+
+// ------------------------------------------------------------- stats -----
+__global__ __launch_bounds__(256) void k_stats_partial(Dev D, double* part) {
+    __shared__ double lds[4];
+    __shared__ float ldf[4];
+    Slot s = slot_of(D, false);
+    double e = 0.0, ke = 0.0, n = 0.0;
+    float emax = -1e30f;
+    int nan = 0;
+    if (s.live && D.id[s.i] >= 0) {
+        float4 x = D.x[s.i], v = D.v[s.i];
+        int ph = phase_of(x);
+        float r = D.rho[s.i];
+        float ee = r / D.rho0[ph] - 1.0f;
+        e = (double)ee;
+        emax = ee;
+        n = 1.0;
+        ke = 0.5 * (double)D.mass[ph] * (double)(v.x * v.x + v.y * v.y + v.z * v.z);
+        if (!isfinite(x.x + x.y + x.z + v.x + v.y + v.z)) nan = 1;
+    }
+    double se = block_sum_256(e, lds);
+    double sk = block_sum_256(ke, lds);
+    double sn = block_sum_256(n, lds);
+    float mx = block_max_256(emax, ldf);
+    if (nan) atomicAdd(&D.counters[C_NAN], 1);
+    if (threadIdx.x == 0) {
+        part[4 * s.tile] = se;
+        part[4 * s.tile + 1] = sk;
+        part[4 * s.tile + 2] = (double)mx;
+        part[4 * s.tile + 3] = sn;
+    }
+}
+
+// per scene: [mean(rho/rho0-1), max(rho/rho0-1), kinetic energy, count]
+__global__ __launch_bounds__(256) void k_stats_reduce(Dev D, const double* part, double* out) {
+    __shared__ double lds[4];
+    __shared__ float ldf[4];
+    int sc = blockIdx.x;
+    int t0 = D.scenePadStart[sc] / TILE, t1 = D.scenePadStart[sc + 1] / TILE;
+    double se = 0.0, sk = 0.0, sn = 0.0;
+    float mx = -1e30f;
+    for (int t = t0 + threadIdx.x; t < t1; t += 256) {
+        se += part[4 * t];
+        sk += part[4 * t + 1];
+        mx = fmaxf(mx, (float)part[4 * t + 2]);
+        sn += part[4 * t + 3];
+    }
+    se = block_sum_256(se, lds);
+    sk = block_sum_256(sk, lds);
+    sn = block_sum_256(sn, lds);
+    mx = block_max_256(mx, ldf);
+    if (threadIdx.x == 0) {
+        out[4 * sc] = sn > 0 ? se / sn : 0.0;
+        out[4 * sc + 1] = sn > 0 ? (double)mx : 0.0;
+        out[4 * sc + 2] = sk;
+        out[4 * sc + 3] = sn;
+    }
+}
+
+// ============================================================ WCSPH ======
+// WcSph::wc_computePressure (WcSph.cpp:99-153): rho (Kahan), Tait p >= 0.
+__global__ __launch_bounds__(256) void k_wc_pressure(Dev D) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i, stride = D.ncap;
+    float4 pa = D.x[i];
+    int ph = phase_of(pa);
+    float m = D.mass[ph], rho0 = D.rho0[ph];
+    Kahan rho;
+    rho.init(m * D.W0);
+    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
+    for (int k = 0; k < nf; ++k) {
+        float4 pb = D.x[D.nbF[k * stride + i]];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
+        rho.add(m * kernel_W(r, D));
+    }
+    for (int k = 0; k < nbn; ++k) {
+        float4 pb = D.xb[D.nbB[k * stride + i]];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
+        rho.add(rho0 * pb.w * kernel_W(r, D));
+    }
+    float density = rho.s;
+    float p = D.tait_B[ph] * (powf(density / rho0, D.gamma) - 1.0f);
+    D.rho[i] = density;
+    D.f1[i] = p < 0.0f ? 0.0f : p;
+}
+
+// WcSph::wc_computeForce (WcSph.cpp:156-312, 364-436) fused with the
+// Euler-Cromer update and domain removal of SphBase::updateFluids
+// (SphBase.cpp:377-402): writes x2/v2, the caller swaps buffers.
+__global__ __launch_bounds__(256) void k_wc_force(Dev D) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i, stride = D.ncap;
+    float4 pa = D.x[i], va = D.v[i];
+    int ph = phase_of(pa);
+    float m = D.mass[ph], rho0 = D.rho0[ph], alpha = D.visc[ph];
+    float rhoa = D.rho[i], pra = D.f1[i];
+    float pa_r2 = pra / (rhoa * rhoa);
+    const float Hcs2 = 2.0f * D.Hs * D.cs;
+    float ax = D.gx, ay = D.gy, az = D.gz;
+    int zd = 0;
+    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
+    for (int k = 0; k < nf; ++k) {
+        int j = D.nbF[k * stride + i];
+        float4 pb = D.x[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) { ++zd; continue; }
+        float r = fast_sqrt(r2);
+        float g = kernel_gradfac(r, D);
+        float4 vb = D.v[j];
+        float rhob = D.rho[j], prb = D.f1[j];
+        float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
+        int phb = phase_of(pb);
+        float acce;
+        if (phb == ph) {
+            acce = g * (-m * (pa_r2 + prb / (rhob * rhob)));
+            if (pro < 0.0f) {
+                float nu = alpha * Hcs2 / (rhoa + rhob);
+                float pi = -nu * pro / (r2 + D.visc_eps);
+                acce += g * (-m * pi);
+            }
+        } else {                                      /* SP08 multiphase, WcSph.cpp:220-260 */
+            float mb = D.mass[phb], al = 0.5f * (alpha + D.visc[phb]);
+            float da = rhoa / m, db = rhob / mb;
+            acce = g * (-1.0f / m * (pra / (da * da) + prb / (db * db)));
+            if (pro < 0.0f) {
+                float nu = al * Hcs2 / (rhoa + rhob);
+                float pi = -nu * pro / (r2 + D.visc_eps);
+                acce += g * (-0.5f * (m + mb) * pi);
+            }
+        }
+        ax += dx * acce; ay += dy * acce; az += dz * acce;
+    }
+    for (int k = 0; k < nbn; ++k) {
+        int j = D.nbB[k * stride + i];
+        float4 pb = D.xb[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) { ++zd; continue; }
+        float r = fast_sqrt(r2);
+        float g = kernel_gradfac(r, D);
+        float4 vb = D.vb[j];
+        float acce = 0.0f;
+        if (pra > 0.0f) acce += g * (-rho0 * pb.w * (pa_r2 * 2.0f));
+        float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
+        if (pro < 0.0f) {
+            float nu = D.solid_visc * Hcs2 / (rhoa * 2.0f);
+            float pi = -nu * pro / (r2 + D.visc_eps);
+            acce += g * (-rho0 * pb.w * pi);
+        }
+        ax += dx * acce; ay += dy * acce; az += dz * acce;
+    }
+    if (zd) atomicAdd(&D.counters[C_ZERO_DIS], zd);
+    float dt = D.dt[s.scene];
+    D.f4a[i] = make_float4(ax, ay, az, 0.0f);
+    float4 vn = make_float4(va.x + ax * dt, va.y + ay * dt, va.z + az * dt, 0.0f);
+    float4 xn = make_float4(pa.x + vn.x * dt, pa.y + vn.y * dt, pa.z + vn.z * dt, pa.w);
+    D.v2[i] = vn;
+    D.x2[i] = xn;
+    const SceneGrid& G = D.grids[s.scene];
+    bool inside = G.smin[0] <= xn.x && G.smin[1] <= xn.y && G.smin[2] <= xn.z &&
+                  xn.x <= G.smax[0] && xn.y <= G.smax[1] && xn.z <= G.smax[2];
+    if (!inside) {
+        D.id[i] = -1;
+        atomicAdd(&D.counters[C_REMOVED], 1);
+    }
+}
+
+// ============================================================ DFSPH ======
+// DFSPH::DF_constantDensitySolverFirst (DFSPH.cpp:84-193): rho (Kahan) and alpha.
+__global__ __launch_bounds__(256) void k_df_density_alpha(Dev D) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i, stride = D.ncap;
+    float4 pa = D.x[i];
+    int ph = phase_of(pa);
+    float m = D.mass[ph], rho0 = D.rho0[ph];
+    Kahan rho;
+    rho.init(m * D.W0);
+    float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2 = 0.f;
+    int zd = 0;
+    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
+    for (int k = 0; k < nf; ++k) {
+        float4 pb = D.x[D.nbF[k * stride + i]];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) { ++zd; continue; }
+        float r = fast_sqrt(r2);
+        rho.add(m * kernel_W(r, D));
+        float g = kernel_gradfac(r, D);
+        float gm = g * m;
+        a1x += dx * gm; a1y += dy * gm; a1z += dz * gm;
+        float gs = g * r * m;                  // |dW/dr| * m
+        a2 += gs * gs;
+    }
+    for (int k = 0; k < nbn; ++k) {
+        float4 pb = D.xb[D.nbB[k * stride + i]];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) { ++zd; continue; }
+        float r = fast_sqrt(r2);
+        float psi = rho0 * pb.w;
+        rho.add(psi * kernel_W(r, D));
+        float gm = kernel_gradfac(r, D) * psi;
+        a1x += dx * gm; a1y += dy * gm; a1z += dz * gm;
+    }
+    if (zd) atomicAdd(&D.counters[C_ZERO_DIS], zd);
+    float density = rho.s;
+    float den = a1x * a1x + a1y * a1y + a1z * a1z + a2;
+    if (den < 1.0e-6f) den = 1.0e-6f;
+    D.rho[i] = density;
+    D.alpha[i] = density / den;
+}
+
+// DFSPH::DF_forceExceptPressure (DFSPH.cpp:1419-1475; helpers 1246-1359) and
+// v* = v + dt a_adv (DFSPH.cpp:557-563).  With a warm start, also the
+// initial kappa/rho of every particle (kr0) for the first correction.
+__global__ __launch_bounds__(256) void k_df_aadv(Dev D, int warm) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i, stride = D.ncap;
+    float4 pa = D.x[i], va = D.v[i];
+    int ph = phase_of(pa);
+    float m = D.mass[ph], rho0 = D.rho0[ph], alpha = D.visc[ph];
+    float rhoa = D.rho[i];
+    const float nu10 = 10.0f * D.nu;
+    const float Hcs2 = 2.0f * D.Hs * D.cs;
+    float ax = D.gx, ay = D.gy, az = D.gz;
+    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
+    for (int k = 0; k < nf; ++k) {
+        int j = D.nbF[k * stride + i];
+        float4 pb = D.x[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) continue;
+        float4 vb = D.v[j];
+        float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
+        if (pro >= 0.0f) continue;
+        float r = fast_sqrt(r2);
+        float g = kernel_gradfac(r, D);
+        int phb = phase_of(pb);
+        float f;
+        if (phb == ph) {
+            f = g * (pro / (r2 + D.visc_eps) * m / D.rho[j] * nu10);
+        } else {                                         /* DFSPH.cpp:1278-1303 */
+            float mb = D.mass[phb], al = 0.5f * (alpha + D.visc[phb]);
+            float nuv = al * Hcs2 / (rhoa + D.rho[j]);
+            float pi = -nuv * pro / (r2 + D.visc_eps);
+            f = g * (-0.5f * (m + mb) * pi);
+        }
+        ax += dx * f; ay += dy * f; az += dz * f;
+    }
+    for (int k = 0; k < nbn; ++k) {
+        int j = D.nbB[k * stride + i];
+        float4 pb = D.xb[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) continue;
+        float4 vb = D.vb[j];
+        float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
+        if (pro >= 0.0f) continue;
+        float r = fast_sqrt(r2);
+        float g = kernel_gradfac(r, D);
+        float f = g * (pro / (r2 + D.visc_eps) * (rho0 * pb.w) / rhoa * nu10);
+        ax += dx * f; ay += dy * f; az += dz * f;
+    }
+    float dt = D.dt[s.scene];
+    D.f4a[i] = make_float4(ax, ay, az, 0.0f);
+    D.vs[i] = make_float4(va.x + ax * dt, va.y + ay * dt, va.z + az * dt, 0.0f);
+    if (warm) {
+        D.kr0[i] = D.ksum[i] / (dt * dt) / rhoa;
+    } else {
+        D.ksum[i] = 0.0f;
+    }
+}
+
+// pass A of the density loop (DFSPH.cpp:572-643): rho*, clamp rho* >= rho0,
+// kappa = (rho*-rho0)/dt^2 alpha -> kr = kappa/rho; partial sum of rho*/rho0.
+__global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
+    __shared__ double lds[4];
+    Slot s = slot_of(D, true);
+    double contrib = 0.0;
+    if (s.live) {
+        const int i = s.i, stride = D.ncap;
+        float4 pa = D.x[i], va = D.vs[i];
+        int ph = phase_of(pa);
+        float m = D.mass[ph], rho0 = D.rho0[ph];
+        float sf = 0.0f, sb = 0.0f;
+        int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
+        for (int k = 0; k < nf; ++k) {
+            int j = D.nbF[k * stride + i];
+            float4 pb = D.x[j];
+            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0f) continue;
+            float4 vb = D.vs[j];
+            float g = kernel_gradfac(fast_sqrt(r2), D);
+            sf += g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
+        }
+        for (int k = 0; k < nbn; ++k) {
+            int j = D.nbB[k * stride + i];
+            float4 pb = D.xb[j];
+            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0f) continue;
+            float4 vb = D.vb[j];
+            float g = kernel_gradfac(fast_sqrt(r2), D);
+            sb += pb.w * g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
+        }
+        float dt = D.dt[s.scene];
+        float rhoa = D.rho[i];
+        float rs = rhoa + dt * (m * sf + rho0 * sb);
+        if (rs < rho0) rs = rho0;
+        float dev = (rs - rho0) * D.alpha[i];     // kappa * dt^2
+        D.kr[i] = dev / (dt * dt) / rhoa;
+        D.ksum[i] += dev;
+        contrib = (double)rs / (double)rho0;
+    }
+    write_partial(D, s.tile, 0, contrib, lds);
+}
+
+// pass 1 of the divergence loop (DFSPH.cpp:780-836): Drho/Dt, clamp,
+// kappaV = Drho/Dt alpha / dt -> kr = kappaV/rho; partial sum of Drho/Dt.
+__global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D) {
+    __shared__ double lds[4];
+    Slot s = slot_of(D, true);
+    double contrib = 0.0;
+    if (s.live) {
+        const int i = s.i, stride = D.ncap;
+        float4 pa = D.x[i], va = D.vs[i];
+        int ph = phase_of(pa);
+        float m = D.mass[ph], rho0 = D.rho0[ph];
+        float sf = 0.0f, sb = 0.0f;
+        int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
+        for (int k = 0; k < nf; ++k) {
+            int j = D.nbF[k * stride + i];
+            float4 pb = D.x[j];
+            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0f) continue;
+            float4 vb = D.vs[j];
+            float g = kernel_gradfac(fast_sqrt(r2), D);
+            sf += g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
+        }
+        for (int k = 0; k < nbn; ++k) {
+            int j = D.nbB[k * stride + i];
+            float4 pb = D.xb[j];
+            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0f) continue;
+            float4 vb = D.vb[j];
+            float g = kernel_gradfac(fast_sqrt(r2), D);
+            sb += pb.w * g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
+        }
+        float dt = D.dt[s.scene];
+        float rhoa = D.rho[i];
+        float dd = m * sf + rho0 * sb;
+        if (dd < 0.0f || (rhoa + dd * dt) < rho0) dd = 0.0f;
+        float kv = dd * D.alpha[i] / dt;
+        D.kr[i] = kv / rhoa;
+        D.kvsum[i] += kv * dt;
+        contrib = (double)dd;
+    }
+    write_partial(D, s.tile, 0, contrib, lds);
+}
+
+// pass B of both loops (DFSPH.cpp:647-693, 840-883):
+// v* -= dt (m sum (kr_a + kr_b) gradW + rho0 sum V_b kr_a gradW)
+__global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D, const float* __restrict__ kr) {
+    Slot s = slot_of(D, true);
+    if (!s.live) return;
+    const int i = s.i, stride = D.ncap;
+    float4 pa = D.x[i];
+    int ph = phase_of(pa);
+    float m = D.mass[ph], rho0 = D.rho0[ph];
+    float ka = kr[i];
+    float fx = 0.f, fy = 0.f, fz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
+    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
+    for (int k = 0; k < nf; ++k) {
+        int j = D.nbF[k * stride + i];
+        float4 pb = D.x[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) continue;
+        float g = kernel_gradfac(fast_sqrt(r2), D) * (ka + kr[j]);
+        fx += dx * g; fy += dy * g; fz += dz * g;
+    }
+    for (int k = 0; k < nbn; ++k) {
+        float4 pb = D.xb[D.nbB[k * stride + i]];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) continue;
+        float g = kernel_gradfac(fast_sqrt(r2), D) * pb.w;
+        bx += dx * g; by += dy * g; bz += dz * g;
+    }
+    float dt = D.dt[s.scene];
+    float cf = dt * m, cb = dt * rho0 * ka;
+    float4 v = D.vs[i];
+    v.x -= cf * fx + cb * bx;
+    v.y -= cf * fy + cb * by;
+    v.z -= cf * fz + cb * bz;
+    D.vs[i] = v;
+}
+
+// v = v*, x += dt v, removal outside [spaceMin, spaceMax] (DFSPH.cpp:705-718)
+__global__ __launch_bounds__(256) void k_df_integrate(Dev D) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i;
+    float dt = D.dt[s.scene];
+    float4 v = D.vs[i];
+    float4 x = D.x[i];
+    x.x += v.x * dt; x.y += v.y * dt; x.z += v.z * dt;
+    D.v[i] = v;
+    D.x[i] = x;
+    const SceneGrid& G = D.grids[s.scene];
+    bool inside = G.smin[0] <= x.x && G.smin[1] <= x.y && G.smin[2] <= x.z &&
+                  x.x <= G.smax[0] && x.y <= G.smax[1] && x.z <= G.smax[2];
+    if (!inside) {
+        D.id[i] = -1;
+        atomicAdd(&D.counters[C_REMOVED], 1);
+    }
+}
+
+// ============================================================ IISPH ======
+// density (IISph_M.cpp:101-112), computed before a_adv so that a_adv reads
+// this step's densities.
+__global__ __launch_bounds__(256) void k_ii_density(Dev D) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i, stride = D.ncap;
+    float4 pa = D.x[i];
+    int ph = phase_of(pa);
+    float m = D.mass[ph], rho0 = D.rho0[ph];
+    Kahan rho;
+    rho.init(m * D.W0);
+    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
+    for (int k = 0; k < nf; ++k) {
+        float4 pb = D.x[D.nbF[k * stride + i]];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        rho.add(m * kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D));
+    }
+    for (int k = 0; k < nbn; ++k) {
+        float4 pb = D.xb[D.nbB[k * stride + i]];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        rho.add(rho0 * pb.w * kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D));
+    }
+    D.rho[i] = rho.s;
+}
+
+// a_adv (IISph_M.cpp:364-476), d_ii (114-130) and v_adv (135)
+__global__ __launch_bounds__(256) void k_ii_aadv_dii(Dev D) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i, stride = D.ncap;
+    float4 pa = D.x[i], va = D.v[i];
+    int ph = phase_of(pa);
+    float m = D.mass[ph], rho0 = D.rho0[ph], alpha = D.visc[ph];
+    float rhoa = D.rho[i];
+    const float nu10 = 10.0f * D.nu, Hcs2 = 2.0f * D.Hs * D.cs;
+    float ax = D.gx, ay = D.gy, az = D.gz;
+    float dfx = 0.f, dfy = 0.f, dfz = 0.f, dbx = 0.f, dby = 0.f, dbz = 0.f;
+    int zd = 0;
+    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
+    for (int k = 0; k < nf; ++k) {
+        int j = D.nbF[k * stride + i];
+        float4 pb = D.x[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) { ++zd; continue; }
+        float4 vb = D.v[j];
+        float g = kernel_gradfac(fast_sqrt(r2), D);
+        float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
+        int phb = phase_of(pb);
+        float f = 0.0f;
+        if (phb == ph) {
+            f = g * (pro / (r2 + D.visc_eps) * m / D.rho[j] * nu10);   // no pro<0 gate (IISph_M.cpp:377)
+        } else if (pro < 0.0f) {
+            float mb = D.mass[phb], al = 0.5f * (alpha + D.visc[phb]);
+            float nuv = al * Hcs2 / (rhoa + D.rho[j]);
+            float pi = -nuv * pro / (r2 + D.visc_eps);
+            f = g * (-0.5f * (m + mb) * pi);
+        }
+        ax += dx * f; ay += dy * f; az += dz * f;
+        dfx += dx * g; dfy += dy * g; dfz += dz * g;
+    }
+    for (int k = 0; k < nbn; ++k) {
+        int j = D.nbB[k * stride + i];
+        float4 pb = D.xb[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) { ++zd; continue; }
+        float4 vb = D.vb[j];
+        float g = kernel_gradfac(fast_sqrt(r2), D);
+        float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
+        float f = g * (pro / (r2 + D.visc_eps) * (rho0 * pb.w) / rhoa * nu10);
+        ax += dx * f; ay += dy * f; az += dz * f;
+        float gw = g * pb.w;
+        dbx += dx * gw; dby += dy * gw; dbz += dz * gw;
+    }
+    if (zd) atomicAdd(&D.counters[C_ZERO_DIS], zd);
+    float dt = D.dt[s.scene];
+    float cf = -m / (rhoa * rhoa) * dt * dt, cb = -rho0 / (rhoa * rhoa) * dt * dt;
+    D.f4a[i] = make_float4(cf * dfx + cb * dbx, cf * dfy + cb * dby, cf * dfz + cb * dbz, 0.0f);
+    D.vs[i] = make_float4(va.x + ax * dt, va.y + ay * dt, va.z + az * dt, 0.0f);
+}
+
+// a_ii, rho_adv, p_l = 0.5 p(t-dt) (IISph_M.cpp:243-278)
+__global__ __launch_bounds__(256) void k_ii_aii(Dev D) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i, stride = D.ncap;
+    float4 pa = D.x[i], va = D.vs[i], dii = D.f4a[i];
+    int ph = phase_of(pa);
+    float m = D.mass[ph], rho0 = D.rho0[ph];
+    float rhoa = D.rho[i];
+    float dt = D.dt[s.scene];
+    float cji = m * dt * dt / (rhoa * rhoa);
+    float t1f = 0.f, t1b = 0.f, t2f = 0.f, t2b = 0.f;
+    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
+    for (int k = 0; k < nf; ++k) {
+        int j = D.nbF[k * stride + i];
+        float4 pb = D.x[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) continue;
+        float4 vb = D.vs[j];
+        float g = kernel_gradfac(fast_sqrt(r2), D);
+        float gx = dx * g, gy = dy * g, gz = dz * g;
+        t1f += gx * (va.x - vb.x) + gy * (va.y - vb.y) + gz * (va.z - vb.z);
+        t2f += (dii.x - cji * gx) * gx + (dii.y - cji * gy) * gy + (dii.z - cji * gz) * gz;
+    }
+    for (int k = 0; k < nbn; ++k) {
+        int j = D.nbB[k * stride + i];
+        float4 pb = D.xb[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) continue;
+        float4 vb = D.vb[j];
+        float g = kernel_gradfac(fast_sqrt(r2), D) * pb.w;
+        float gx = dx * g, gy = dy * g, gz = dz * g;
+        t1b += gx * (va.x - vb.x) + gy * (va.y - vb.y) + gz * (va.z - vb.z);   // v_adv (IISph_M.cpp:269 uses v)
+        t2b += dii.x * gx + dii.y * gy + dii.z * gz;
+    }
+    float aii = m * t2f + rho0 * t2b;
+    if (aii == 0.0f) aii = 1.0f;
+    D.alpha[i] = aii;
+    D.kr0[i] = rhoa + dt * (m * t1f + rho0 * t1b);
+    D.kr[i] = 0.5f * D.ksum[i];
+}
+
+// sum_j d_ij p_j = dt^2 sum_f -m p_j/rho_j^2 gradW (IISph_M.cpp:289-309)
+__global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
+    Slot s = slot_of(D, true);
+    if (!s.live) return;
+    const int i = s.i, stride = D.ncap;
+    float4 pa = D.x[i];
+    int ph = phase_of(pa);
+    float m = D.mass[ph];
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    int c = D.cnt[i], nf = c & 0xffff;
+    for (int k = 0; k < nf; ++k) {
+        int j = D.nbF[k * stride + i];
+        float4 pb = D.x[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) continue;
+        float rb = D.rho[j];
+        float g = kernel_gradfac(fast_sqrt(r2), D) * (D.kr[j] / (rb * rb));
+        sx += dx * g; sy += dy * g; sz += dz * g;
+    }
+    float dt = D.dt[s.scene];
+    float cf = -m * dt * dt;
+    D.f4b[i] = make_float4(cf * sx, cf * sy, cf * sz, 0.0f);
+}
+
+// relaxed Jacobi update (IISph_M.cpp:312-351): writes the new p_l into f1,
+// partial sum of the (clamped-excluded) relative density error.
+__global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
+    __shared__ double lds[4];
+    Slot s = slot_of(D, false);
+    double contrib = 0.0;
+    if (s.live && !D.active[s.scene]) {
+        D.f1[s.i] = D.kr[s.i];          // converged scene: keep p_l across the buffer swap
+    } else if (s.live) {
+        const int i = s.i, stride = D.ncap;
+        float4 pa = D.x[i], sa = D.f4b[i];
+        int ph = phase_of(pa);
+        float m = D.mass[ph], rho0 = D.rho0[ph];
+        float rhoa = D.rho[i], pla = D.kr[i];
+        float dt = D.dt[s.scene];
+        float cji = dt * dt * m / (rhoa * rhoa);
+        float ff = 0.f, fb = 0.f;
+        int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
+        for (int k = 0; k < nf; ++k) {
+            int j = D.nbF[k * stride + i];
+            float4 pb = D.x[j];
+            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0f) continue;
+            float g = kernel_gradfac(fast_sqrt(r2), D);
+            float gx = dx * g, gy = dy * g, gz = dz * g;
+            float4 db = D.f4a[j], sb = D.f4b[j];
+            float plb = D.kr[j];
+            float tx = (sa.x - db.x * plb) - (sb.x - cji * gx * pla);
+            float ty = (sa.y - db.y * plb) - (sb.y - cji * gy * pla);
+            float tz = (sa.z - db.z * plb) - (sb.z - cji * gz * pla);
+            ff += tx * gx + ty * gy + tz * gz;
+        }
+        for (int k = 0; k < nbn; ++k) {
+            float4 pb = D.xb[D.nbB[k * stride + i]];
+            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0f) continue;
+            float g = kernel_gradfac(fast_sqrt(r2), D) * pb.w;
+            fb += (sa.x * dx + sa.y * dy + sa.z * dz) * g;
+        }
+        float fin = m * ff + rho0 * fb;
+        float aii = D.alpha[i], radv = D.kr0[i];
+        float t = (1.0f - D.omega) * pla + (D.omega / aii) * (rho0 - radv - fin);
+        if (t < 0.0f) {
+            t = 0.0f;
+        } else {
+            contrib = ((double)radv + (double)aii * (double)pla + (double)fin - (double)rho0) / (double)rho0;
+        }
+        D.f1[i] = t;
+    }
+    write_partial(D, s.tile, 0, contrib, lds);
+}
+
+// IISph_M::ii_computeForcePressure (IISph_M.cpp:478-684) fused with the
+// update v = v_adv + dt a_p, x += dt v and domain removal.
+__global__ __launch_bounds__(256) void k_ii_force(Dev D) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i, stride = D.ncap;
+    float4 pa = D.x[i], va = D.v[i];
+    int ph = phase_of(pa);
+    float m = D.mass[ph], rho0 = D.rho0[ph], alpha = D.visc[ph];
+    float rhoa = D.rho[i], pra = D.kr[i];
+    float pa_r2 = pra / (rhoa * rhoa);
+    const float Hcs2 = 2.0f * D.Hs * D.cs;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
+    for (int k = 0; k < nf; ++k) {
+        int j = D.nbF[k * stride + i];
+        float4 pb = D.x[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) continue;
+        float r = fast_sqrt(r2);
+        float g = kernel_gradfac(r, D);
+        float rhob = D.rho[j], prb = D.kr[j];
+        int phb = phase_of(pb);
+        float acce;
+        if (phb == ph) {
+            acce = g * (-m * (pa_r2 + prb / (rhob * rhob)));
+        } else {
+            float4 vb = D.v[j];
+            float mb = D.mass[phb], al = 0.5f * (alpha + D.visc[phb]);
+            float da = rhoa / m, db = rhob / mb;
+            acce = g * (-1.0f / m * (pra / (da * da) + prb / (db * db)));
+            float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
+            if (pro < 0.0f) {
+                float nuv = al * Hcs2 / (rhoa + rhob);
+                float pi = -nuv * pro / (r2 + D.visc_eps);
+                acce += g * (-0.5f * (m + mb) * pi);
+            }
+        }
+        ax += dx * acce; ay += dy * acce; az += dz * acce;
+    }
+    if (pra > 0.0f) {
+        for (int k = 0; k < nbn; ++k) {
+            float4 pb = D.xb[D.nbB[k * stride + i]];
+            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0f) continue;
+            float g = kernel_gradfac(fast_sqrt(r2), D);
+            float acce = g * (-rho0 * pb.w * (pa_r2 * 2.0f));
+            ax += dx * acce; ay += dy * acce; az += dz * acce;
+        }
+    }
+    float dt = D.dt[s.scene];
+    float4 vadv = D.vs[i];
+    D.f4b[i] = make_float4(ax, ay, az, 0.0f);
+    float4 vn = make_float4(vadv.x + ax * dt, vadv.y + ay * dt, vadv.z + az * dt, 0.0f);
+    float4 xn = make_float4(pa.x + vn.x * dt, pa.y + vn.y * dt, pa.z + vn.z * dt, pa.w);
+    D.v2[i] = vn;
+    D.x2[i] = xn;
+    D.ksum[i] = pra;
+    const SceneGrid& G = D.grids[s.scene];
+    bool inside = G.smin[0] <= xn.x && G.smin[1] <= xn.y && G.smin[2] <= xn.z &&
+                  xn.x <= G.smax[0] && xn.y <= G.smax[1] && xn.z <= G.smax[2];
+    if (!inside) {
+        D.id[i] = -1;
+        atomicAdd(&D.counters[C_REMOVED], 1);
+    }
+}
+
+__global__ void k_fill_i32(int* p, int v, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+inline int tiles_of(const Dev& D) { return D.npad / TILE; }
+
+}  // namespace
+
+void launch_cfl(const Dev& D, float cfl, float spacing, float dtMax, float dtMin, hipStream_t st) {
+    if (tiles_of(D) > 0) hipLaunchKernelGGL(k_cfl_partial, dim3(tiles_of(D)), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_cfl_reduce, dim3(D.nscenes), dim3(256), 0, st, D, cfl, spacing, dtMax, dtMin);
+}
+void launch_scene_reduce(const Dev& D, int mode, float tol, int minIt, int cap, hipStream_t st) {
+    hipLaunchKernelGGL(k_scene_reduce, dim3(D.nscenes), dim3(256), 0, st, D, mode, tol, minIt, cap);
+}
+void launch_stats(const Dev& D, double* part, double* out, hipStream_t st) {
+    if (tiles_of(D) > 0) hipLaunchKernelGGL(k_stats_partial, dim3(tiles_of(D)), dim3(256), 0, st, D, part);
+    hipLaunchKernelGGL(k_stats_reduce, dim3(D.nscenes), dim3(256), 0, st, D, (const double*)part, out);
+}
+void launch_fill_i32(int* p, int v, int n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
+}
+
+#define SPH_LAUNCH(kern, ...)                                                              \
+    do {                                                                                   \
+        if (tiles_of(D) > 0)                                                               \
+            hipLaunchKernelGGL(kern, dim3(tiles_of(D)), dim3(256), 0, st, D, ##__VA_ARGS__); \
+    } while (0)
+
+void launch_wc_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_wc_pressure); }
+void launch_wc_force_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_wc_force); }
+void launch_df_density_alpha(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_density_alpha); }
+void launch_df_aadv(const Dev& D, int warm, hipStream_t st) { SPH_LAUNCH(k_df_aadv, warm); }
+void launch_df_pass_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_pass_density); }
+void launch_df_pass_divergence(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_pass_divergence); }
+void launch_df_pass_velocity(const Dev& D, const float* kr, hipStream_t st) { SPH_LAUNCH(k_df_pass_velocity, kr); }
+void launch_df_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_integrate); }
+void launch_ii_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_density); }
+void launch_ii_aadv_dii(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_aadv_dii); }
+void launch_ii_aii(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_aii); }
+void launch_ii_sumdij(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_sumdij); }
+void launch_ii_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_pressure); }
+void launch_ii_force_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_force); }
+
+}  // namespace sphgpu

@@ -1,0 +1,286 @@
+// Internal definitions shared by the HIP translation units of libsphgpu.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace sphgpu {
+
+constexpr int TILE = 256;            // particles per tile; every particle kernel uses 256 threads
+constexpr int NBF_CAP = 64;          // fluid neighbours kept per particle (ELL list width)
+constexpr int NBB_CAP = 48;          // boundary neighbours kept per particle
+constexpr int REF_NEIGHBOUR_CAP = 48;  // NeigbStr::maxNeigbNum (SphBase.h:630)
+constexpr int MAX_PHASES = 8;
+constexpr uint32_t KEY_INVALID = 0xFFFFFFFFu;
+
+// Uniform grid of one scene (SphBase::SpaceGridStruct, SphBase.h:682-705).
+// Cells are numbered row-major along x inside "rows" (cy, cz); rows are
+// ranked in 2-D Z-order (Morton on cy, cz) so that a run of three x-adjacent
+// cells is contiguous in the sorted particle order while rows that are
+// neighbours in y/z stay close in memory.
+struct SceneGrid {
+    float minx, miny, minz;  // spaceMin - 2*smoothRadius_h (SphBase.cpp:968)
+    float divisor;           // 1/smoothRadius_h (SphBase.cpp:969)
+    float smin[3], smax[3];  // spaceMin / spaceMax: particles outside are removed
+    int nx, ny, nz;          // cell counts (SphBase.cpp:970)
+    uint32_t keyBase;        // first key of this scene
+    int rowBase;             // offset of this scene's row-rank table
+    int pad;
+};
+
+// Indices of device counters (int32, reset per step).
+enum Counter {
+    C_ZERO_DIS = 0,
+    C_TOO_MANY = 1,
+    C_LIST_OVF = 2,
+    C_REMOVED = 3,
+    C_NAN = 4,
+    C_COUNT = 8
+};
+
+// Everything a kernel may need, passed by value as the kernel argument.
+struct Dev {
+    // ---- sizes
+    int ncap;          // fluid slot capacity (padded), ELL stride
+    int npad;          // fluid slots in use (upper bound, multiple of TILE)
+    int nb;            // boundary particles
+    int nscenes;
+    int nphases;
+    int solver;
+
+    // ---- kernel constants (SphBase.h:808-829, h = smoothRadius_h / 2)
+    float kh, inv_kh, sigma, sigma_h, W0;
+    float h2;          // neighbour cutoff fl((d+d)^2), SphBase.cpp:883-884
+    float Hs;          // smoothRadius_h (artificial viscosity length, WcSph.cpp:182)
+    float visc_eps;    // 0.01*Hs*Hs (WcSph.cpp:184)
+    float v0;          // spacing^3 (DFSPH.cpp:88)
+    float gx, gy, gz;
+    float cs, gamma, nu, omega;
+    float rho0[MAX_PHASES], mass[MAX_PHASES], visc[MAX_PHASES], tait_B[MAX_PHASES];
+    float solid_visc;  // viscosity_alpha of the boundary (one value for all static solids)
+
+    // ---- fluid state (sorted, padded slots)
+    float4* x;         // xyz, w = phase
+    float4* v;
+    int* id;           // original id, -1 for padding / removed
+    float* ksum;       // DFSPH accumulated kappa*dt^2; IISPH pressure (warm start)
+    float* kvsum;      // DFSPH accumulated kappaV*dt
+    uint32_t* key;     // cell key of each slot (KEY_INVALID for padding)
+    // alternate buffers (reorder / integration targets)
+    float4* x2;
+    float4* v2;
+    int* id2;
+    float* ksum2;
+    float* kvsum2;
+    uint32_t* key2;
+
+    // ---- per-step fields
+    float4* vs;        // DFSPH v*, IISPH v_adv
+    float4* f4a;       // WCSPH acceleration, IISPH d_ii, DFSPH a_adv
+    float4* f4b;       // IISPH sum_j d_ij p_j
+    float* rho;
+    float* alpha;      // DFSPH alpha; IISPH a_ii
+    float* kr;         // kappa/rho (pass B input); IISPH p_l
+    float* kr0;        // warm-start kappa/rho; IISPH rho_adv
+    float* f1;         // WCSPH pressure; IISPH p_l (next)
+    float* f2;         // per-particle scratch (divergence deviation, errors)
+
+    // ---- neighbour lists (ELL, column k of particle i at k*ncap + i)
+    int* nbF;
+    int* nbB;
+    int* cnt;          // nf | nb << 16
+
+    // ---- boundary (sorted by cell once)
+    float4* xb;        // xyz, w = Akinci volume V_b
+    float4* vb;        // boundary velocity
+    int* bid;          // original boundary index
+    int* bsolid;       // solid index
+    int* bscene;
+    float* bkself;     // kerSumFromSelf
+    uint32_t* bkey;
+
+    // ---- grid
+    const SceneGrid* grids;
+    const int* rowRank;
+    int2* cellF;       // [start, end) of fluid slots per key
+    int2* cellB;       // [start, end) of boundary particles per key
+    uint32_t totalKeys;
+
+    // ---- scene layout
+    int* sceneStart;     // first sorted rank of each scene (unpadded)
+    int* sceneCount;     // live particles per scene
+    int* scenePadStart;  // first padded slot of each scene (multiple of TILE), [nscenes+1]
+    int* tileScene;
+    int* tileValid;
+    const uint32_t* sceneKeyBase;  // [nscenes+1]
+
+    // ---- per-scene solver scalars
+    float* dt;           // per scene
+    int* active;         // per scene: 1 while its solver loop runs
+    int* iters;          // per scene: iterations of the current loop
+    double* sceneErr;    // per scene
+    double* partial;     // per tile partial sums (2 per tile)
+    int* counters;       // Counter
+    float* vmaxPartial;  // per tile
+};
+
+// ---------------------------------------------------------------------------
+// device helpers
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// cell coordinate exactly as SphBase.h:700-701: int((p - min) * divisor), FP32, truncation.
+__device__ __forceinline__ int cell_coord(float p, float mn, float div) {
+#pragma clang fp contract(off)
+    float t = (p - mn) * div;
+    return int(t);
+}
+
+// squared distance exactly as Vec23.h:170-171 ((dx*dx + dy*dy) + dz*dz, no FMA)
+__device__ __forceinline__ float dis_square_exact(float4 a, float4 b) {
+#pragma clang fp contract(off)
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+__device__ __forceinline__ uint32_t morton2(uint32_t a, uint32_t b) {
+    auto spread = [](uint32_t v) {
+        v &= 0xFFFFu;
+        v = (v | (v << 8)) & 0x00FF00FFu;
+        v = (v | (v << 4)) & 0x0F0F0F0Fu;
+        v = (v | (v << 2)) & 0x33333333u;
+        v = (v | (v << 1)) & 0x55555555u;
+        return v;
+    };
+    return spread(a) | (spread(b) << 1);
+}
+
+// cubic spline W(r), SphBase.h:855-871, with q = r/h.
+__device__ __forceinline__ float kernel_W(float r, const Dev& D) {
+    float q = r * D.inv_kh;
+    float qm2 = 2.0f - q;
+    float w1 = D.sigma * (1.0f - 1.5f * q * q + 0.75f * q * q * q);
+    float w2 = D.sigma * (0.25f * qm2 * qm2 * qm2);
+    float w = q <= 1.0f ? w1 : w2;
+    return q <= 2.0f ? w : 0.0f;
+}
+
+// -|dW/dr| / r  (SphBase.h:890-906 divided by r), the scalar that turns x_ab
+// into grad W_ab (DFSPH.cpp:156-157).  For q <= 1 the 1/r cancels:
+// -(sigma_h/h)(3 - 2.25 q); for 1 < q <= 2: -0.75 sigma_h (2-q)^2 / r.
+__device__ __forceinline__ float kernel_gradfac(float r, const Dev& D) {
+    float q = r * D.inv_kh;
+    float qm2 = 2.0f - q;
+    float g1 = -(D.sigma_h * D.inv_kh) * (3.0f - 2.25f * q);
+    float g2 = -0.75f * D.sigma_h * qm2 * qm2 / r;
+    float g = q <= 1.0f ? g1 : g2;
+    return q <= 2.0f ? g : 0.0f;
+}
+
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_sqrtf(x); }
+
+// Kahan-compensated FP32 accumulator.
+struct Kahan {
+    float s, c;
+    __device__ __forceinline__ void init(float v) { s = v; c = 0.0f; }
+    __device__ __forceinline__ void add(float t) {
+#pragma clang fp contract(off)
+        float y = t - c;
+        float u = s + y;
+        c = (u - s) - y;
+        s = u;
+    }
+};
+
+// Deterministic FP64 block sum over 256 threads (4 waves): fixed shuffle tree
+// per wave, then waves in order.  Every thread of the block must call it.
+__device__ __forceinline__ double block_sum_256(double v, double* lds) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane_id() == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ float block_max_256(float v, float* lds) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    if (lane_id() == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = 0.0f;
+    if (threadIdx.x == 0) r = fmaxf(fmaxf(lds[0], lds[1]), fmaxf(lds[2], lds[3]));
+    __syncthreads();
+    return r;
+}
+
+// scene of a key: largest s with keyBase[s] <= key (binary search over nscenes+1 bases)
+__device__ __forceinline__ int scene_of_key(uint32_t key, const uint32_t* base, int ns) {
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= key) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers implemented in the .hip files
+
+// sort.hip
+struct SortScratch {
+    uint32_t* keys2;
+    uint32_t* vals2;
+    uint32_t* hist;       // 256 * maxTiles
+    uint32_t* scanTmp;    // scan block sums
+    int maxItems;
+};
+size_t sort_scratch_bytes(int maxItems);
+void sort_scratch_bind(SortScratch& s, void* mem, int maxItems);
+// Stable LSD radix sort of (keys, vals) on the lowest `bits` key bits.
+// Result ends in keys/vals (ping-pong through the scratch buffers).
+void radix_sort_pairs(uint32_t* keys, uint32_t* vals, int n, int bits, SortScratch& s,
+                      hipStream_t st);
+// exclusive scan of n uint32 in place; tmp needs scan_tmp_words(n) words.
+int scan_tmp_words(int n);
+void exclusive_scan_u32(uint32_t* data, int n, uint32_t* tmp, hipStream_t st);
+
+// grid.hip
+void launch_fluid_keys_into(const Dev& D, uint32_t* keys, uint32_t* vals, hipStream_t st);
+void launch_scene_bounds(const Dev& D, const uint32_t* sortedKeys, int n, hipStream_t st);
+void launch_scene_layout(const Dev& D, int* hostOut, hipStream_t st);
+void launch_reorder(const Dev& D, const uint32_t* sortedKeys, const uint32_t* sortedVals, int n,
+                    int oldPad, hipStream_t st);
+void launch_cell_reset(const Dev& D, const uint32_t* keys, int n, int2* table, hipStream_t st);
+void launch_cell_build(const Dev& D, const uint32_t* keys, int n, int2* table, hipStream_t st);
+void launch_neighbours(const Dev& D, hipStream_t st);
+void launch_boundary_keys_from(const Dev& D, const float4* xs, const int* scene, uint32_t* keys,
+                               uint32_t* vals, hipStream_t st);
+void launch_boundary_reorder(const Dev& D, const float4* srcX, const int* srcSolid,
+                             const int* srcScene, const uint32_t* sortedKeys,
+                             const uint32_t* sortedVals, hipStream_t st);
+void launch_boundary_volume(const Dev& D, hipStream_t st);
+void launch_neighbour_dump_ranked(const Dev& D, int cap, int* counts, int* kinds, long long* ids,
+                                  float* dis, const int* rankOf, hipStream_t st);
+
+// solvers.hip
+void launch_cfl(const Dev& D, float cfl, float spacing, float dtMax, float dtMin, hipStream_t st);
+void launch_wc_pressure(const Dev& D, hipStream_t st);
+void launch_wc_force_integrate(const Dev& D, hipStream_t st);
+void launch_df_density_alpha(const Dev& D, hipStream_t st);
+void launch_df_aadv(const Dev& D, int warm, hipStream_t st);
+void launch_df_pass_density(const Dev& D, hipStream_t st);
+void launch_df_pass_divergence(const Dev& D, hipStream_t st);
+void launch_df_pass_velocity(const Dev& D, const float* krSrc, hipStream_t st);
+void launch_df_integrate(const Dev& D, hipStream_t st);
+void launch_ii_density(const Dev& D, hipStream_t st);
+void launch_ii_aadv_dii(const Dev& D, hipStream_t st);
+void launch_ii_aii(const Dev& D, hipStream_t st);
+void launch_ii_sumdij(const Dev& D, hipStream_t st);
+void launch_ii_pressure(const Dev& D, hipStream_t st);
+void launch_ii_force_integrate(const Dev& D, hipStream_t st);
+void launch_scene_reduce(const Dev& D, int mode, float tol, int minIt, int cap, hipStream_t st);
+void launch_stats(const Dev& D, double* part, double* out, hipStream_t st);
+void launch_fill_i32(int* p, int v, int n, hipStream_t st);
+
+}  // namespace sphgpu

@@ -1,0 +1,162 @@
+"""Synthetic dam-break workloads (identical bytes for the GPU path and the oracle).
+
+The reference builds its scenes in C++ (SphWithIO::addFluidCuboid,
+SphWithIO.cpp:149-194, and SphWithIO::addRigidCuboid -> makeCubeSampleParts,
+SphWithIO.cpp:489-522, 642-659).  The same samplers live in libsphgpu.so
+(sphgpu_scene_lattice / sphgpu_scene_box); this module only lays scenes out.
+Jitter is uniform in +-1 % of the spacing per axis from PCG32 seeded with
+1000 + workload number.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import sphgpu as sg
+
+
+@dataclass
+class Scene:
+    name: str
+    params: dict
+    phases: list = field(default_factory=list)         # (rest_density, visc_alpha)
+    fluids: list = field(default_factory=list)         # (scene, phase, xyz[n,3])
+    solids: list = field(default_factory=list)         # (scene, xyz[n,3], visc_alpha)
+    domains: list = field(default_factory=list)        # (min[3], max[3]) per scene
+
+    @property
+    def num_fluid(self) -> int:
+        return sum(len(f[2]) for f in self.fluids)
+
+    @property
+    def num_boundary(self) -> int:
+        return sum(len(s[1]) for s in self.solids)
+
+    @property
+    def num_scenes(self) -> int:
+        return len(self.domains)
+
+
+def make_simulation(sc: Scene, **overrides) -> sg.Simulation:
+    p = dict(sc.params)
+    p.update(overrides)
+    p["num_scenes"] = sc.num_scenes
+    p["space_min"], p["space_max"] = sc.domains[0]
+    sim = sg.Simulation(sg.default_params(**p))
+    for s, (mn, mx) in enumerate(sc.domains):
+        sim.set_scene_domain(s, mn, mx)
+    for rho0, visc in sc.phases:
+        sim.add_phase(rho0, visc)
+    for s, ph, xyz in sc.fluids:
+        sim.add_fluid(xyz, phase=ph, scene=s)
+    for s, xyz, visc in sc.solids:
+        sim.add_solid(xyz, visc_alpha=visc, scene=s)
+    return sim
+
+
+def _tank(scene: Scene, s: int, size, spacing, have_top, visc, origin=(0.0, 0.0, 0.0)):
+    half = [0.5 * float(t) for t in size]
+    center = [origin[k] + half[k] for k in range(3)]
+    walls = sg.box_walls(center, half, spacing, have_top)
+    scene.solids.append((s, walls, visc))
+    return walls
+
+
+def small_dam(solver=sg.DFSPH, n=(10, 10, 10), box=(0.4, 0.3, 0.3), spacing=0.02, jitter=0.01,
+              seed=1001, have_top=True, **kw) -> Scene:
+    """A small dam break for parity tests (seconds on the oracle)."""
+    p = dict(solver=solver, spacing_r=spacing, smooth_radius_h=2 * spacing, dt=1e-3,
+             gravity=(0.0, -9.81, 0.0), sound_speed=20.0, gamma=7.0, nu=1e-3, cfl=0.0,
+             density_tol=1e-4, divergence_tol=1.0, min_iter_density=2, max_iter_density=100,
+             min_iter_div=1, max_iter_div=30, omega=0.5, warm_start=0)
+    if solver == sg.WCSPH:
+        p["dt"] = 1e-4
+    if solver == sg.IISPH:
+        p["density_tol"] = 1e-3
+    p.update(kw)
+    sc = Scene(f"small_dam_{solver}", p)
+    sc.phases.append((1000.0, 0.01))
+    _tank(sc, 0, box, spacing, have_top, 0.01)
+    top = box[1] if have_top else box[1] + 0.2
+    sc.domains.append(((0.0, 0.0, 0.0), (box[0], top, box[2])))
+    xyz = sg.lattice(n, (spacing, spacing, spacing), spacing, jitter, seed, 0)
+    sc.fluids.append((0, 0, xyz))
+    return sc
+
+
+def wcsph_c1(seed=1001) -> Scene:
+    """WCSPH 3-D dam break: 25^3 fluid at 0.02 m in a 1.0x0.6x0.6 m open-top box."""
+    s = 0.02
+    p = dict(solver=sg.WCSPH, spacing_r=s, smooth_radius_h=2 * s, dt=1e-4,
+             gravity=(0.0, -9.81, 0.0), sound_speed=20.0, gamma=7.0, cfl=0.0)
+    sc = Scene("wcsph_dam_15625", p)
+    sc.phases.append((1000.0, 0.01))
+    _tank(sc, 0, (1.0, 0.6, 0.6), s, False, 0.01)
+    sc.domains.append(((0.0, 0.0, 0.0), (1.0, 1.0, 0.6)))
+    sc.fluids.append((0, 0, sg.lattice((25, 25, 25), (s, s, s), s, 0.01, seed, 0)))
+    return sc
+
+
+def dfsph_c2(seed=1002, n=100) -> Scene:
+    """DFSPH dam break: n^3 fluid (1,000,000 at n=100) at 0.01 m in a closed 4x2x2 m box."""
+    s = 0.01
+    p = dict(solver=sg.DFSPH, spacing_r=s, smooth_radius_h=2 * s, dt=1e-3, cfl=0.4, dt_min=1e-5,
+             gravity=(0.0, -9.81, 0.0), sound_speed=20.0, nu=1e-3,
+             density_tol=1e-4, divergence_tol=1.0, min_iter_density=2, max_iter_density=100,
+             min_iter_div=1, max_iter_div=30)
+    sc = Scene(f"dfsph_dam_{n ** 3}", p)
+    sc.phases.append((1000.0, 0.01))
+    _tank(sc, 0, (4.0, 2.0, 2.0), s, True, 0.01)
+    sc.domains.append(((0.0, 0.0, 0.0), (4.0, 2.0, 2.0)))
+    sc.fluids.append((0, 0, sg.lattice((n, n, n), (s, s, s), s, 0.01, seed, 0)))
+    return sc
+
+
+def dfsph_c3(seed=1003, n=126) -> Scene:
+    """DFSPH, two facing n^3 blocks (4,000,752 at n=126) at 0.01 m in a closed 6x3x3 m box."""
+    s = 0.01
+    sc = dfsph_c2(seed, 1)
+    sc.name = f"dfsph_two_blocks_{2 * n ** 3}"
+    sc.solids.clear()
+    sc.fluids.clear()
+    sc.domains.clear()
+    _tank(sc, 0, (6.0, 3.0, 3.0), s, True, 0.01)
+    sc.domains.append(((0.0, 0.0, 0.0), (6.0, 3.0, 3.0)))
+    ext = (n - 1) * s
+    a = sg.lattice((n, n, n), (s, s, s), s, 0.01, seed, 0)
+    b = sg.lattice((n, n, n), (6.0 - s - ext, s, 3.0 - s - ext), s, 0.01, seed, 1)
+    sc.fluids.append((0, 0, a))
+    sc.fluids.append((0, 0, b))
+    return sc
+
+
+def iisph_batched_c4(seed=1004, scenes=64, nmin=10_000, nmax=500_000, scale=1.0) -> Scene:
+    """Batched IISPH: `scenes` dam breaks, fluid counts log-uniform in [nmin, nmax],
+    box aspect ratio uniform in [1, 3] (`scale` shrinks counts for tests)."""
+    s = 0.01
+    rng = np.random.default_rng(seed)
+    p = dict(solver=sg.IISPH, spacing_r=s, smooth_radius_h=2 * s, dt=1e-3, cfl=0.4, dt_min=1e-5,
+             gravity=(0.0, -9.81, 0.0), sound_speed=20.0, nu=1e-3, omega=0.5,
+             density_tol=1e-3, min_iter_density=2, max_iter_density=100)
+    sc = Scene(f"iisph_batched_{scenes}", p)
+    sc.phases.append((1000.0, 0.01))
+    for k in range(scenes):
+        nk = math.exp(rng.uniform(math.log(nmin), math.log(nmax))) * scale
+        aspect = rng.uniform(1.0, 3.0)
+        m = max(2, int(round(nk ** (1.0 / 3.0))))
+        width = (m + 3) * s
+        size = (aspect * width + 2 * s, 1.5 * width, width)
+        _tank(sc, k, size, s, True, 0.01)
+        sc.domains.append(((0.0, 0.0, 0.0), size))
+        sc.fluids.append((k, 0, sg.lattice((m, m, m), (s, s, s), s, 0.01, seed, k)))
+    return sc
+
+
+WORKLOADS = {
+    "wcsph_c1": wcsph_c1,
+    "dfsph_c2": dfsph_c2,
+    "dfsph_c3": dfsph_c3,
+    "iisph_c4": iisph_batched_c4,
+}

@@ -1,0 +1,291 @@
+"""ctypes binding of libsphgpu.so (the C-ABI declared in include/sphgpu.h).
+
+The product path is the HIP library; this module only marshals numpy arrays
+across the C-ABI.  Loading fails loudly when the library is missing, and the
+compute entry points return SPHGPU_E_DEVICE when no gfx9 device is present:
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libsphgpu.so"
+
+WCSPH, DFSPH, IISPH = 0, 1, 2
+WARM_NONE, WARM_PREVIOUS, WARM_LEARNED = 0, 1, 2
+F_POS, F_VEL, F_RHO, F_PRESSURE, F_ACC, F_ALPHA, F_KAPPA_SUM, F_PHASE = range(8)
+FIELD_COMPONENTS = {F_POS: 3, F_VEL: 3, F_RHO: 1, F_PRESSURE: 1, F_ACC: 3, F_ALPHA: 1,
+                    F_KAPPA_SUM: 1, F_PHASE: 1}
+
+ERRORS = {0: "OK", -1: "E_ARG", -2: "E_STATE", -3: "E_DEVICE", -4: "E_CAPACITY", -5: "E_DIVERGED"}
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("solver", C.c_int),
+        ("spacing_r", C.c_float),
+        ("smooth_radius_h", C.c_float),
+        ("space_min", C.c_float * 3),
+        ("space_max", C.c_float * 3),
+        ("dt", C.c_float),
+        ("gravity", C.c_float * 3),
+        ("sound_speed", C.c_float),
+        ("gamma", C.c_float),
+        ("nu", C.c_float),
+        ("cfl", C.c_float),
+        ("dt_min", C.c_float),
+        ("density_tol", C.c_float),
+        ("divergence_tol", C.c_float),
+        ("min_iter_density", C.c_int),
+        ("max_iter_density", C.c_int),
+        ("min_iter_div", C.c_int),
+        ("max_iter_div", C.c_int),
+        ("omega", C.c_float),
+        ("warm_start", C.c_int),
+        ("num_scenes", C.c_int),
+        ("reserved", C.c_int * 8),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("iter_density", C.c_int),
+        ("iter_divergence", C.c_int),
+        ("density_err", C.c_double),
+        ("divergence_err", C.c_double),
+        ("rho_mean_err", C.c_double),
+        ("rho_max_err", C.c_double),
+        ("kinetic_energy", C.c_double),
+        ("dt", C.c_float),
+        ("zero_dis", C.c_int),
+        ("too_many_neigb", C.c_int),
+        ("removed", C.c_int),
+        ("max_iter_hit", C.c_int),
+        ("nan_count", C.c_int),
+        ("num_fluid", C.c_longlong),
+        ("list_overflow", C.c_int),
+        ("reserved", C.c_int * 5),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int)
+_LP = C.POINTER(C.c_longlong)
+_DP = C.POINTER(C.c_double)
+
+# exported symbol -> (restype, argtypes); every declaration of include/sphgpu.h
+SIGNATURES = {
+    "sphgpu_default_params": (None, [C.POINTER(Params)]),
+    "sphgpu_abi_version": (C.c_int, []),
+    "sphgpu_create": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "sphgpu_destroy": (None, [C.c_void_p]),
+    "sphgpu_last_error": (C.c_char_p, [C.c_void_p]),
+    "sphgpu_device_available": (C.c_int, []),
+    "sphgpu_set_scene_domain": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP]),
+    "sphgpu_add_phase": (C.c_int, [C.c_void_p, C.c_float, C.c_float, _IP]),
+    "sphgpu_add_fluid": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_longlong, _FP, _FP, _LP]),
+    "sphgpu_add_solid": (C.c_int, [C.c_void_p, C.c_int, C.c_longlong, _FP, C.c_float, _IP]),
+    "sphgpu_init": (C.c_int, [C.c_void_p]),
+    "sphgpu_step": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "sphgpu_scene_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Stats)]),
+    "sphgpu_synchronize": (C.c_int, [C.c_void_p]),
+    "sphgpu_num_fluid": (C.c_longlong, [C.c_void_p]),
+    "sphgpu_get_fluid": (C.c_int, [C.c_void_p, C.c_int, _FP]),
+    "sphgpu_get_fluid_ids": (C.c_int, [C.c_void_p, _LP]),
+    "sphgpu_get_boundary_volume": (C.c_int, [C.c_void_p, _FP]),
+    "sphgpu_get_neighbours": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, _LP, _FP]),
+    "sphgpu_profile_kernel": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "sphgpu_kernel_timing": (C.c_int, [C.c_void_p, _LP, _DP, _DP]),
+    "sphgpu_scene_lattice": (C.c_longlong, [_IP, _FP, C.c_float, C.c_float, C.c_ulonglong,
+                                            C.c_ulonglong, _FP]),
+    "sphgpu_scene_box": (C.c_longlong, [_FP, _FP, C.c_float, C.c_int, _FP]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads the in-tree libsphgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                               "(the HIP extension is required; there is no CPU fallback)")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(_FP)
+
+
+class SphError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    load().sphgpu_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k in ("space_min", "space_max", "gravity"):
+            getattr(p, k)[:] = [float(t) for t in v]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def device_available() -> bool:
+    return bool(load().sphgpu_device_available())
+
+
+def lattice(n, origin, spacing, jitter=0.0, seed=0, stream=0) -> np.ndarray:
+    lib = load()
+    n3 = (C.c_int * 3)(*[int(t) for t in n])
+    o3 = (C.c_float * 3)(*[float(t) for t in origin])
+    total = int(n[0]) * int(n[1]) * int(n[2])
+    out = np.empty((total, 3), np.float32)
+    got = lib.sphgpu_scene_lattice(n3, o3, float(spacing), float(jitter), int(seed), int(stream), _fp(out))
+    assert got == total
+    return out
+
+
+def box_walls(center, half, spacing, have_top=True) -> np.ndarray:
+    lib = load()
+    c3 = (C.c_float * 3)(*[float(t) for t in center])
+    h3 = (C.c_float * 3)(*[float(t) for t in half])
+    n = lib.sphgpu_scene_box(c3, h3, float(spacing), int(bool(have_top)), None)
+    out = np.empty((n, 3), np.float32)
+    lib.sphgpu_scene_box(c3, h3, float(spacing), int(bool(have_top)), _fp(out))
+    return out
+
+
+class Simulation:
+    """One handle of the C-ABI (a SphBase-derived solver object)."""
+
+    def __init__(self, params: Params):
+        self.lib = load()
+        self.params = params
+        h = C.c_void_p()
+        rc = self.lib.sphgpu_create(C.byref(params), C.byref(h))
+        if rc != 0:
+            raise SphError(rc, "sphgpu_create rejected the parameters")
+        self.h = h
+
+    # -- helpers
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            raise SphError(rc, f"{what}: {self.lib.sphgpu_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.lib.sphgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- scene
+    def set_scene_domain(self, scene, mn, mx):
+        a, b = _f32(mn), _f32(mx)
+        self._check(self.lib.sphgpu_set_scene_domain(self.h, scene, _fp(a), _fp(b)), "set_scene_domain")
+
+    def add_phase(self, rest_density, visc_alpha) -> int:
+        ph = C.c_int()
+        self._check(self.lib.sphgpu_add_phase(self.h, rest_density, visc_alpha, C.byref(ph)), "add_phase")
+        return ph.value
+
+    def add_fluid(self, xyz, vel=None, phase=0, scene=0) -> int:
+        x = _f32(xyz).reshape(-1, 3)
+        v = None if vel is None else _f32(vel).reshape(-1, 3)
+        first = C.c_longlong()
+        self._check(self.lib.sphgpu_add_fluid(self.h, scene, phase, len(x), _fp(x),
+                                              None if v is None else _fp(v), C.byref(first)),
+                    "add_fluid")
+        return first.value
+
+    def add_solid(self, xyz, visc_alpha=0.0, scene=0) -> int:
+        x = _f32(xyz).reshape(-1, 3)
+        sid = C.c_int()
+        self._check(self.lib.sphgpu_add_solid(self.h, scene, len(x), _fp(x), visc_alpha, C.byref(sid)),
+                    "add_solid")
+        return sid.value
+
+    def init(self):
+        self._check(self.lib.sphgpu_init(self.h), "init")
+
+    # -- stepping
+    def step(self) -> dict:
+        st = Stats()
+        self._check(self.lib.sphgpu_step(self.h, C.byref(st)), "step")
+        return st.as_dict()
+
+    def scene_stats(self, scene) -> dict:
+        st = Stats()
+        self._check(self.lib.sphgpu_scene_stats(self.h, scene, C.byref(st)), "scene_stats")
+        return st.as_dict()
+
+    def synchronize(self):
+        self._check(self.lib.sphgpu_synchronize(self.h), "synchronize")
+
+    # -- readback
+    def num_fluid(self) -> int:
+        return int(self.lib.sphgpu_num_fluid(self.h))
+
+    def get(self, field) -> np.ndarray:
+        n = self.num_fluid()
+        comp = FIELD_COMPONENTS[field]
+        out = np.empty((n, comp), np.float32)
+        self._check(self.lib.sphgpu_get_fluid(self.h, field, _fp(out)), "get_fluid")
+        return out if comp > 1 else out[:, 0]
+
+    def ids(self) -> np.ndarray:
+        out = np.empty(self.num_fluid(), np.int64)
+        self._check(self.lib.sphgpu_get_fluid_ids(self.h, out.ctypes.data_as(_LP)), "get_fluid_ids")
+        return out
+
+    def boundary_volume(self, nb) -> np.ndarray:
+        out = np.empty(nb, np.float32)
+        self._check(self.lib.sphgpu_get_boundary_volume(self.h, _fp(out)), "get_boundary_volume")
+        return out
+
+    def neighbours(self, cap=96):
+        n = self.num_fluid()
+        counts = np.zeros(n, np.int32)
+        kinds = np.zeros((n, cap), np.int32)
+        ids = np.zeros((n, cap), np.int64)
+        dis = np.zeros((n, cap), np.float32)
+        self._check(self.lib.sphgpu_get_neighbours(
+            self.h, cap, counts.ctypes.data_as(_IP), kinds.ctypes.data_as(_IP),
+            ids.ctypes.data_as(_LP), _fp(dis)), "get_neighbours")
+        return counts, kinds, ids, dis
+
+    # -- measurement
+    def profile_kernel(self, name):
+        self._check(self.lib.sphgpu_profile_kernel(self.h, None if name is None else name.encode()),
+                    "profile_kernel")
+
+    def kernel_timing(self):
+        n, ms, b = C.c_longlong(), C.c_double(), C.c_double()
+        self._check(self.lib.sphgpu_kernel_timing(self.h, C.byref(n), C.byref(ms), C.byref(b)),
+                    "kernel_timing")
+        return n.value, ms.value, b.value

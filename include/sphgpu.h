@@ -1,0 +1,196 @@
+/*
+ * sphgpu.h -- C-ABI of the MI355X SPH particle loops (libsphgpu.so).
+ *
+ * The library replaces the hot protected methods of the reference's solver
+ * classes (Artemisaturn/GRAPHICS, Source/):
+ *   SphBase::neighbourSearch / gridSetup        SphBase.h:673, SphBase.cpp:849-1089
+ *   SphBase::prepare/updateSolidPartWeight      SphBase.h:477-481, SphBase.cpp:766-841
+ *   SphBase::updateFluids                       SphBase.h:414, SphBase.cpp:377-402
+ *   WcSph::wc_computePressure / wc_computeForce WcSph.h:72, 91
+ *   DFSPH::DF_constantDensitySolverFirst/Second, DF_divergenceFreeSolver,
+ *         DF_forceExceptPressure                DFSPH.h:58-68
+ *   IISph_M::ii_computePressure, ii_forceExceptPressure,
+ *         ii_computeForcePressure               IISph_M.h:56-90
+ * and the per-step driver SphBase::runOneStep -> sphStep (SphBase.cpp:348-354).
+ * A host class (graphics_amd/host/SphGpu.h) keeps the reference class names
+ * and calls these entry points from its sphStep() override.
+ *
+ * Conventions: plain C types only; vec3 arrays are AoS float[3*n] like vec_t
+ * (SphBase.h:41); the caller owns host buffers (copied in/out); the library
+ * owns device memory.  Every entry point returns 0 on success and a negative
+ * SPHGPU_E* code on error, with a message in sphgpu_last_error().  A handle
+ * is not thread-safe (like SphBase, which is not re-entrant); it owns one HIP
+ * stream.  No entry point here falls back to a CPU implementation: without a
+ * usable gfx950 device the compute entry points return SPHGPU_E_DEVICE.
+ */
+#ifndef SPHGPU_H_
+#define SPHGPU_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPHGPU_ABI_VERSION 1
+
+enum sphgpu_status {
+    SPHGPU_OK = 0,
+    SPHGPU_E_ARG = -1,      /* bad argument (null pointer, bad index, bad size) */
+    SPHGPU_E_STATE = -2,    /* call not valid in the current state */
+    SPHGPU_E_DEVICE = -3,   /* HIP runtime / device error, or no device */
+    SPHGPU_E_CAPACITY = -4, /* a fixed capacity would be exceeded */
+    SPHGPU_E_DIVERGED = -5  /* non-finite particle state detected */
+};
+
+enum sphgpu_solver {
+    SPHGPU_WCSPH = 0,       /* WcSph::sphStep            WcSph.cpp:23-96   */
+    SPHGPU_DFSPH = 1,       /* DFSPH::sphStep            DFSPH.cpp:9-75    */
+    SPHGPU_IISPH = 2        /* IISph_M::sphStep (Ihmsen)  IISph_M.cpp:10-75 */
+};
+
+enum sphgpu_warm_start {
+    SPHGPU_WARM_NONE = 0,     /* plain solver (the reference) */
+    SPHGPU_WARM_PREVIOUS = 1, /* previous step's accumulated kappa (Bender & Koschier) */
+    SPHGPU_WARM_LEARNED = 2   /* per-particle MLP prediction (sphgpu_mlp_*) */
+};
+
+/* Mirrors ThresholdsBase m_TH (SphBase.h:546-583) plus the solver settings the
+ * reference hard-codes.  Field meanings and defaults: see sphgpu_default_params. */
+typedef struct sphgpu_params {
+    int   solver;              /* sphgpu_solver */
+    float spacing_r;           /* m_TH.spacing_r: particle spacing and diameter d */
+    float smooth_radius_h;     /* m_TH.smoothRadius_h: cell size; kernel h = smooth_radius_h/2 */
+    float space_min[3];        /* m_TH.spaceMin: scene 0 domain (particles outside are removed) */
+    float space_max[3];        /* m_TH.spaceMax */
+    float dt;                  /* m_TH.dt: fixed step, or the upper bound when cfl > 0 */
+    float gravity[3];          /* m_TH.gravity_g */
+    float sound_speed;         /* m_TH.soundSpeed_cs (WCSPH EOS and artificial viscosity) */
+    float gamma;               /* WcSph::m_DensityPower_gamma (WcSph.h:34) */
+    float nu;                  /* m_TH.nu: DFSPH/IISPH viscosity (DFSPH.cpp:1265) */
+    float cfl;                 /* 0: fixed dt; >0: dt = clamp(cfl*spacing/max|v|, dt_min, dt) */
+    float dt_min;
+    float density_tol;         /* DFSPH avg (rho*-rho0)/rho0 (DFSPH.cpp:225); IISPH eta/rho0 (IISph_M.cpp:285) */
+    float divergence_tol;      /* DFSPH avg Drho/Dt in kg m^-3 s^-1 (DFSPH.cpp:762) */
+    int   min_iter_density;    /* 2 (DFSPH.cpp:229, IISph_M.cpp:286) */
+    int   max_iter_density;    /* cap (the reference DFSPH loop has none) */
+    int   min_iter_div;        /* 1 (DFSPH.cpp:765) */
+    int   max_iter_div;        /* 30: the loop breaks when it > max (DFSPH.cpp:772) */
+    float omega;               /* IISPH relaxation, 0.5 (IISph_M.cpp:341) */
+    int   warm_start;          /* sphgpu_warm_start */
+    int   num_scenes;          /* >= 1 independent scenes advanced together */
+    int   reserved[8];
+} sphgpu_params;
+
+typedef struct sphgpu_stats {
+    int    iter_density;       /* DFSPH density / IISPH pressure iterations (max over scenes) */
+    int    iter_divergence;    /* DFSPH divergence iterations (max over scenes) */
+    double density_err;        /* last solver density error, relative (max over scenes) */
+    double divergence_err;     /* last divergence error, kg m^-3 s^-1 (max over scenes) */
+    double rho_mean_err;       /* mean(rho/rho0 - 1) over fluid */
+    double rho_max_err;        /* max(rho/rho0 - 1) */
+    double kinetic_energy;     /* sum 1/2 m |v|^2 */
+    float  dt;                 /* dt of this step (scene 0) */
+    int    zero_dis;           /* m_EC.zeroDis increments (WcSph.cpp:160) */
+    int    too_many_neigb;     /* m_EC.tooManyNeigb: neighbours beyond the 48 of the reference (SphBase.h:630) */
+    int    removed;            /* fluid particles removed outside [spaceMin,spaceMax] (SphBase.cpp:397) */
+    int    max_iter_hit;       /* a capped solver loop reached its cap */
+    int    nan_count;          /* non-finite fluid positions/velocities */
+    long long num_fluid;       /* live fluid particles after the step */
+    int    list_overflow;      /* neighbours dropped because a list was full (GPU capacity) */
+    int    reserved[5];
+} sphgpu_stats;
+
+/* fluid fields for sphgpu_get_fluid (values in ascending original-id order) */
+enum sphgpu_field {
+    SPHGPU_F_POS = 0,          /* 3 floats  position */
+    SPHGPU_F_VEL = 1,          /* 3  velocity */
+    SPHGPU_F_RHO = 2,          /* 1  density of the last density pass */
+    SPHGPU_F_PRESSURE = 3,     /* 1  WCSPH / IISPH pressure */
+    SPHGPU_F_ACC = 4,          /* 3  WCSPH acceleration / IISPH pressure acceleration */
+    SPHGPU_F_ALPHA = 5,        /* 1  DFSPH alpha factor */
+    SPHGPU_F_KAPPA_SUM = 6,    /* 1  accumulated kappa*dt^2 of the last density solve */
+    SPHGPU_F_PHASE = 7         /* 1  fluid phase index (as float) */
+};
+
+typedef struct sphgpu_handle sphgpu_handle;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+void        sphgpu_default_params(sphgpu_params* p);
+int         sphgpu_abi_version(void);
+/* Replaces SphBase::SphBase (SphBase.cpp:210-248): parameter defaults + state. */
+int         sphgpu_create(const sphgpu_params* p, sphgpu_handle** out);
+void        sphgpu_destroy(sphgpu_handle* h);
+const char* sphgpu_last_error(const sphgpu_handle* h);
+/* 1 when a gfx9 HIP device is visible to this process, else 0 (never initialises a context twice). */
+int         sphgpu_device_available(void);
+
+/* ---- scene state (before the first step) ------------------------------- */
+/* Domain of scene s (m_TH.spaceMin/spaceMax).  Scene 0 defaults to params. */
+int sphgpu_set_scene_domain(sphgpu_handle* h, int scene, const float mn[3], const float mx[3]);
+/* SphFluid(restDensity, viscosity, tension) (SphBase.h:284): a fluid phase.
+ * Returns the phase index in *phase (at most 8 phases). */
+int sphgpu_add_phase(sphgpu_handle* h, float rest_density, float visc_alpha, int* phase);
+/* Appends n fluid particles of `phase` to scene `scene`
+ * (SphWithIO::addFluidCuboid output, SphWithIO.cpp:149-194).  Original ids are
+ * assigned consecutively in call order; *first_id receives the first. */
+int sphgpu_add_fluid(sphgpu_handle* h, int scene, int phase, long long n,
+                     const float* xyz, const float* vxyz, long long* first_id);
+/* Appends a static solid of n boundary particles to scene `scene`
+ * (SphWithBullet::addSolid -> SphSolid, SphBase.h:315-365, rigid type).
+ * visc_alpha is SphSolid::viscosity_alpha. */
+int sphgpu_add_solid(sphgpu_handle* h, int scene, long long n, const float* xyz,
+                     float visc_alpha, int* solid);
+/* Uploads, sorts the boundary, and computes the Akinci boundary volumes
+ * (SphBase::prepareSolidPartWeight, SphBase.cpp:766-813).  Called by the first
+ * step if not called before. */
+int sphgpu_init(sphgpu_handle* h);
+
+/* ---- stepping ---------------------------------------------------------- */
+/* One SphBase::runOneStep (SphBase.cpp:348-354) of the configured solver. */
+int sphgpu_step(sphgpu_handle* h, sphgpu_stats* st);
+/* Per-scene statistics of the last step (batched scenes). */
+int sphgpu_scene_stats(const sphgpu_handle* h, int scene, sphgpu_stats* st);
+/* Waits for all work of the handle's stream. */
+int sphgpu_synchronize(sphgpu_handle* h);
+
+/* ---- readback ---------------------------------------------------------- */
+long long sphgpu_num_fluid(const sphgpu_handle* h);
+/* Copies a field of all live fluid particles, ascending original id.
+ * `out` holds num_fluid * components floats. */
+int sphgpu_get_fluid(sphgpu_handle* h, int field, float* out);
+/* Original ids of the live fluid particles, ascending. */
+int sphgpu_get_fluid_ids(sphgpu_handle* h, long long* out);
+/* Akinci boundary volume of every boundary particle, in add order. */
+int sphgpu_get_boundary_volume(sphgpu_handle* h, float* out);
+/* Test dump of the neighbour sets at the current positions (re-runs the grid
+ * search).  For live fluid particle k (ascending id) writes counts[k] and up
+ * to `cap` entries (kind 0 fluid / 1 boundary, original id, FP32 distance
+ * computed as SphBase.cpp:885-886), sorted by (kind, id). */
+int sphgpu_get_neighbours(sphgpu_handle* h, int cap, int* counts, int* kinds,
+                          long long* ids, float* dis);
+
+/* ---- measurement ------------------------------------------------------- */
+/* Selects one kernel (by name, NULL to stop) whose launches are bracketed by
+ * HIP events on the handle's stream; sphgpu_kernel_timing returns the launch
+ * count, total milliseconds and the algorithmic bytes per launch summed over
+ * launches. */
+int sphgpu_profile_kernel(sphgpu_handle* h, const char* name);
+int sphgpu_kernel_timing(sphgpu_handle* h, long long* launches, double* total_ms,
+                         double* total_alg_bytes);
+
+/* ---- scene generation (host only; the reference's samplers) ------------ */
+/* Cubic lattice block of nx*ny*nz particles, x fastest (SphWithIO.cpp:164-192),
+ * origin + i*spacing, plus uniform jitter in [-jitter, +jitter]*spacing per axis
+ * drawn from PCG32(seed, stream) in lattice order.  Returns the count. */
+long long sphgpu_scene_lattice(const int n[3], const float origin[3], float spacing,
+                               float jitter, unsigned long long seed,
+                               unsigned long long stream, float* xyz);
+/* Surface samples of an axis-aligned box (SphWithIO::makeCubeSampleParts,
+ * SphWithIO.cpp:489-522) centred at `center`; have_top = 0 leaves the +y face
+ * open.  xyz may be NULL to count.  Returns the count. */
+long long sphgpu_scene_box(const float center[3], const float half[3], float spacing,
+                           int have_top, float* xyz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPHGPU_H_ */

@@ -1,0 +1,1192 @@
+/*
+ * sph_oracle.cpp -- CPU restatement of the reference SPH particle loops.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker of the HIP path, and the timed CPU
+ * baseline).  It keeps the reference's loop structure on purpose:
+ *   - AoS particles (FluidParticle / BoundaryParticle, SphBase.h:66-199);
+ *   - a uniform grid of cell size smoothRadius_h with per-cell linked lists
+ *     filled by a SERIAL head insertion, solids before fluids
+ *     (SphBase.cpp:991-1089);
+ *   - per-particle neighbour lists capped at 48 (SphBase.h:628-646);
+ *   - OpenMP `schedule(dynamic, max(100, N/160))` (SphBase.h:675-676);
+ *   - serial convergence sums (DFSPH.cpp:626-631, 888-896).
+ * Deviations from the reference (each one is a reference defect; see the
+ * comments at the use site):
+ *   - sums that decide convergence accumulate in double (an FP32 serial sum
+ *     of 1e6 values near 1000 carries ~1e-4 relative error, the size of the
+ *     tolerance it is compared with);
+ *   - the DFSPH density loop has a cap (none at DFSPH.cpp:228);
+ *   - IISPH: per-iteration reset of the error sum and iteration counter,
+ *     boundary rho_adv uses v_adv, the warm start p_l = 0.5 p(t-dt) is kept,
+ *     a_adv is integrated, a_adv reads this step's densities
+ *     (IISph_M.cpp:81, 269, 277, 283-284, 350; SphBase.cpp:387-388);
+ *   - a zero distance pair is skipped and counted in every pass (the
+ *     reference only checks in the force helpers, WcSph.cpp:160).
+ * Surface tension / adhesion (m_TH.r, m_TH.bt) are zero in every workload
+ * and are not restated.  The one-argument kernels use h = smoothRadius_h/2
+ * (SphBase.h:808-829), the value ker_update_numbers() would set.
+ *
+ * Build with -ffp-contract=off so that dis_square() keeps the reference's
+ * rounding ((dx*dx + dy*dy) + dz*dz, Vec23.h:170-171).
+ */
+#include "sph_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+template <class T>
+struct V3 {
+    T x, y, z;
+    V3() : x(0), y(0), z(0) {}
+    V3(T a, T b, T c) : x(a), y(b), z(c) {}
+    V3 operator+(const V3& o) const { return V3(x + o.x, y + o.y, z + o.z); }
+    V3 operator-(const V3& o) const { return V3(x - o.x, y - o.y, z - o.z); }
+    V3 operator*(T s) const { return V3(x * s, y * s, z * s); }
+    V3& operator+=(const V3& o) { x += o.x; y += o.y; z += o.z; return *this; }
+    V3& operator-=(const V3& o) { x -= o.x; y -= o.y; z -= o.z; return *this; }
+    T dot(const V3& o) const { return x * o.x + y * o.y + z * o.z; }      /* Vec23.h:139-140 */
+    T len_square() const { return x * x + y * y + z * z; }
+    T dis_square(const V3& o) const {                                      /* Vec23.h:170-171 */
+        return (x - o.x) * (x - o.x) + (y - o.y) * (y - o.y) + (z - o.z) * (z - o.z);
+    }
+    V3 cross(const V3& a) const { return V3(y * a.z - z * a.y, z * a.x - x * a.z, x * a.y - y * a.x); }
+    bool inside(const V3& lo, const V3& hi) const {                        /* Vec23.h:193-194 */
+        return lo.x <= x && lo.y <= y && lo.z <= z && x <= hi.x && y <= hi.y && z <= hi.z;
+    }
+};
+
+template <class T>
+struct FluidPart {           /* the fields of FluidParticle (SphBase.h:66-165) the loops touch */
+    V3<T> x, v, a, a_adv, v_adv, dii, sum_dijpj, a_p;
+    T rho = 0, p = 0, p_l = 0, aii = 0, rho_adv = 0;
+    T alpha = 0, kappa = 0, kappaV = 0, divdev = 0;
+    T kappa_sum = 0, kappaV_sum = 0;   /* accumulated kappa*dt^2 (warm start state) */
+    T d = 0;
+    int id = 0;
+};
+
+template <class T>
+struct BoundPart {           /* BoundaryParticle (SphBase.h:168-199) */
+    V3<T> x, v, force;
+    T volume = 0, kerSumFromSelf = 0, d = 0;
+};
+
+template <class T>
+struct Fluid {
+    T rho0, visc_alpha;
+    std::vector<FluidPart<T>> parts;
+};
+
+template <class T>
+struct Solid {
+    bool dynamic, rigid;
+    T visc_alpha;
+    std::vector<BoundPart<T>> parts;
+};
+
+struct PartIdx {             /* SphBase.h:596-617 */
+    int t, i;
+    bool valid() const { return t != 0; }
+    bool isFluid() const { return t < 0; }
+    int toFluidI() const { return -t - 1; }
+    int toSolidI() const { return t - 1; }
+    bool operator==(const PartIdx& o) const { return t == o.t && i == o.i; }
+};
+
+template <class T>
+struct Neigb { PartIdx pidx; T dis; };
+
+template <class T>
+struct NeigbStr {            /* SphBase.h:628-646, cap 48 */
+    static const int maxNeigbNum = 48;
+    std::array<Neigb<T>, maxNeigbNum> neigs;
+    int num = 0;
+    void push(const PartIdx& j, T dis, int& overflow) {
+        if (num < maxNeigbNum) { neigs[num].pidx = j; neigs[num].dis = dis; ++num; }
+        else ++overflow;
+    }
+};
+
+static int dyn_chunk(int n) { return std::max(100, n / 160); }  /* SphBase.h:675-676 */
+
+template <class T>
+class Sim {
+public:
+    orc_params P;
+    T r, H, kh, sigma, sigma_h, W0;
+    V3<T> spMin, spMax, grav;
+    std::vector<Fluid<T>> fluids;
+    std::vector<Solid<T>> solids;
+    std::vector<std::vector<NeigbStr<T>>> nbF, nbS;
+    int frame = 0;
+    T dt_cur, dt_prev = 0;
+    int zeroDis = 0, tooMany = 0;
+
+    /* SpaceGridStruct (SphBase.h:682-705) */
+    struct Grid {
+        V3<T> min; T divisor = 0; int num[3] = {0, 0, 0}; long long factor[3] = {1, 1, 1};
+        std::vector<long long> offset;
+        std::vector<PartIdx> first;
+        std::vector<std::vector<PartIdx>> nextF, nextS;
+        bool init = false;
+        long long cellOf(const V3<T>& p) const {                          /* SphBase.h:700-701 */
+            int cx = int((p.x - min.x) * divisor), cy = int((p.y - min.y) * divisor),
+                cz = int((p.z - min.z) * divisor);
+            return cx * factor[0] + cy * factor[1] + cz * factor[2];
+        }
+    } G;
+
+    explicit Sim(const orc_params& p) : P(p) {
+        r = T(P.spacing_r);
+        H = T(P.smooth_radius_h);
+        kh = H / 2;                                                        /* SphBase.h:812 */
+        sigma = T(1) / T(M_PI) / (kh * kh * kh);                           /* SphBase.h:814-816 */
+        sigma_h = T(1) / T(M_PI) / (kh * kh * kh) / kh;                    /* SphBase.h:817-819 */
+        W0 = sigma;
+        spMin = V3<T>(P.space_min[0], P.space_min[1], P.space_min[2]);
+        spMax = V3<T>(P.space_max[0], P.space_max[1], P.space_max[2]);
+        grav = V3<T>(P.gravity[0], P.gravity[1], P.gravity[2]);
+        dt_cur = T(P.dt);
+    }
+
+    /* cubic spline, SphBase.h:840-853 with h = smoothRadius_h/2 */
+    T W(T rr) const {
+        T q = rr / kh, qm2 = 2 - q;
+        if (!(q >= 0 && q <= 2)) return 0;
+        return q <= 1 ? sigma * (1 - T(1.5) * q * q + T(0.75) * q * q * q)
+                      : sigma * (T(0.25) * qm2 * qm2 * qm2);
+    }
+    /* |dW/dr|, SphBase.h:876-889 */
+    T Wg(T rr) const {
+        T q = rr / kh, qm2 = 2 - q;
+        if (!(q >= 0 && q <= 2)) return 0;
+        return q <= 1 ? sigma_h * (3 * q - T(2.25) * q * q) : sigma_h * T(0.75) * qm2 * qm2;
+    }
+
+    T mass(int k) const { return fluids[k].rho0 * (r * r * r); }         /* fm0 = rho0*v0, DFSPH.cpp:88,119 */
+
+    const FluidPart<T>& fp(const PartIdx& j) const { return fluids[j.toFluidI()].parts[j.i]; }
+    BoundPart<T>& bp(const PartIdx& j) { return solids[j.toSolidI()].parts[j.i]; }
+    const BoundPart<T>& bp(const PartIdx& j) const { return solids[j.toSolidI()].parts[j.i]; }
+    const V3<T>& posOf(const PartIdx& j) const { return j.isFluid() ? fp(j).x : bp(j).x; }
+    T dOf(const PartIdx& j) const { return j.isFluid() ? fp(j).d : bp(j).d; }
+    const PartIdx& nextOf(const PartIdx& j) const {
+        return j.isFluid() ? G.nextF[j.toFluidI()][j.i] : G.nextS[j.toSolidI()][j.i];
+    }
+
+    /* SphBase::SpaceGridStruct::initialize, SphBase.cpp:963-988 */
+    void gridInit() {
+        const int gridt = 1;
+        T rh = H;
+        G.min = V3<T>(spMin.x - rh * 2, spMin.y - rh * 2, spMin.z - rh * 2);
+        G.divisor = gridt / rh;
+        V3<T> ext((spMax.x - spMin.x + rh * 2) * G.divisor, (spMax.y - spMin.y + rh * 2) * G.divisor,
+                  (spMax.z - spMin.z + rh * 2) * G.divisor);
+        G.num[0] = int(ext.x) + 2; G.num[1] = int(ext.y) + 2; G.num[2] = int(ext.z) + 2;
+        G.factor[0] = 1; G.factor[1] = G.num[0]; G.factor[2] = (long long)G.num[0] * G.num[1];
+        G.first.assign((size_t)G.num[0] * G.num[1] * G.num[2], PartIdx{0, -1});
+        G.offset.clear();
+        for (int dz = -gridt; dz <= gridt; ++dz)
+            for (int dy = -gridt; dy <= gridt; ++dy)
+                for (int dx = -gridt; dx <= gridt; ++dx)
+                    G.offset.push_back(dx * G.factor[0] + dy * G.factor[1] + dz * G.factor[2]);
+        G.init = true;
+    }
+
+    /* SphBase::gridSetup, SphBase.cpp:991-1089 (serial head insertion) */
+    void gridSetup() {
+        if (!G.init) gridInit();
+        int nf = int(fluids.size()), ns = int(solids.size());
+        G.nextF.resize(nf); nbF.resize(nf);
+        for (int k = 0; k < nf; ++k) { G.nextF[k].resize(fluids[k].parts.size()); nbF[k].resize(fluids[k].parts.size()); }
+        G.nextS.resize(ns); nbS.resize(ns);
+        for (int k = 0; k < ns; ++k) { G.nextS[k].resize(solids[k].parts.size()); nbS[k].resize(solids[k].parts.size()); }
+        std::fill(G.first.begin(), G.first.end(), PartIdx{0, -1});
+        std::vector<long long> cell;
+        for (int k = 0; k < ns; ++k) {
+            auto& ps = solids[k].parts; int n = int(ps.size()); cell.resize(n);
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) cell[i] = G.cellOf(ps[i].x);
+            for (int i = 0; i < n; ++i) {
+                G.nextS[k][i] = G.first[cell[i]];
+                G.first[cell[i]] = PartIdx{k + 1, i};
+            }
+        }
+        for (int k = 0; k < nf; ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size()); cell.resize(n);
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) cell[i] = G.cellOf(ps[i].x);
+            for (int i = 0; i < n; ++i) {
+                G.nextF[k][i] = G.first[cell[i]];
+                G.first[cell[i]] = PartIdx{-(k + 1), i};
+            }
+        }
+    }
+
+    /* SphBase::neighbourSearch, SphBase.cpp:849-960 */
+    void neighbourSearch(bool fluidPart = true, bool solidPart = true, bool RR_same = false) {
+        gridSetup();
+        int n_off = int(G.offset.size());
+        int overflow = 0;
+        if (fluidPart)
+            for (int k = 0; k < int(fluids.size()); ++k) {
+                auto& ps = fluids[k].parts; int n = int(ps.size());
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n)) reduction(+ : overflow)
+                for (int ii = 0; ii < n; ++ii) {
+                    PartIdx i{-(k + 1), ii};
+                    NeigbStr<T>& L = nbF[k][ii]; L.num = 0;
+                    const V3<T>& pa = ps[ii].x;
+                    long long jo = G.cellOf(pa);
+                    for (int o = 0; o < n_off; ++o)
+                        for (PartIdx j = G.first[jo + G.offset[o]]; j.valid(); j = nextOf(j)) {
+                            if (i == j) continue;
+                            T h = dOf(j) + ps[ii].d;
+                            T h2 = h * h;                  /* (real_t)pow(h,2): exact square rounded once */
+                            T len2 = pa.dis_square(posOf(j));
+                            if (len2 < h2) L.push(j, std::sqrt(len2), overflow);
+                        }
+                }
+            }
+        if (solidPart)
+            for (int k = 0; k < int(solids.size()); ++k) {
+                auto& ps = solids[k].parts; int n = int(ps.size());
+                bool rigid = solids[k].rigid;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n)) reduction(+ : overflow)
+                for (int ii = 0; ii < n; ++ii) {
+                    PartIdx i{k + 1, ii};
+                    NeigbStr<T>& L = nbS[k][ii]; L.num = 0;
+                    const V3<T>& pa = ps[ii].x;
+                    long long jo = G.cellOf(pa);
+                    for (int o = 0; o < n_off; ++o)
+                        for (PartIdx j = G.first[jo + G.offset[o]]; j.valid(); j = nextOf(j)) {
+                            if (i == j) continue;
+                            if (!RR_same && rigid && i.t == j.t) continue;
+                            T h = dOf(j) + ps[ii].d;
+                            T h2 = h * h;
+                            T len2 = pa.dis_square(posOf(j));
+                            if (len2 < h2) L.push(j, std::sqrt(len2), overflow);
+                        }
+                }
+            }
+        tooMany += overflow;
+    }
+
+    /* SphBase::prepareSolidPartWeight, SphBase.cpp:766-813 */
+    void prepareSolidPartWeight() {
+        neighbourSearch(false, true, true);
+        for (int k = 0; k < int(solids.size()); ++k) {
+            auto& ps = solids[k].parts;
+            bool rigid = solids[k].rigid;
+            for (size_t i = 0; i < ps.size(); ++i) {
+                const NeigbStr<T>& L = nbS[k][i];
+                T sum = W0, sum2 = W0;
+                for (int j = 0; j < L.num; ++j) {
+                    const PartIdx& pj = L.neigs[j].pidx;
+                    if (rigid && pj.t == k + 1) sum += W(L.neigs[j].dis);
+                    if (!pj.isFluid()) sum2 += W(L.neigs[j].dis);
+                }
+                ps[i].kerSumFromSelf = rigid ? sum : W0;
+                ps[i].volume = 1 / sum2;
+            }
+        }
+    }
+
+    /* SphBase::updateSolidPartWeight, SphBase.cpp:817-841 */
+    void updateSolidPartWeight() {
+        for (int k = 0; k < int(solids.size()); ++k) {
+            auto& ps = solids[k].parts; int n = int(ps.size());
+#pragma omp parallel for
+            for (int i = 0; i < n; ++i) {
+                const NeigbStr<T>& L = nbS[k][i];
+                T vol = ps[i].kerSumFromSelf;
+                for (int j = 0; j < L.num; ++j)
+                    if (!L.neigs[j].pidx.isFluid()) vol += W(L.neigs[j].dis);
+                ps[i].volume = 1 / vol;
+            }
+        }
+    }
+
+    int numFluid() const { int n = 0; for (auto& f : fluids) n += int(f.parts.size()); return n; }
+
+    /* SphFluid::removeOutsideParts, SphBase.cpp:26-48 (serial swap compaction) */
+    int removeOutside(std::vector<FluidPart<T>>& fp_) {
+        if (fp_.empty()) return 0;
+        int num = int(fp_.size()), f = 0, b = num - 1;
+        while (f < b) {
+            while (f < b && fp_[f].x.inside(spMin, spMax)) ++f;
+            while (b > f && !fp_[b].x.inside(spMin, spMax)) --b;
+            if (f < b) { fp_[f] = fp_[b]; ++f; --b; }
+        }
+        if (!fp_[f].x.inside(spMin, spMax)) --f;
+        fp_.resize(f + 1);
+        return num - (f + 1);
+    }
+
+    /* ------------------------------ WCSPH ------------------------------ */
+    /* WcSph::wc_computePressure, WcSph.cpp:99-153 (normals omitted: surface tension is off) */
+    void wc_computePressure() {
+        T v0 = r * r * r;
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0, wm0 = W0 * fm0;
+            T B = rho0 * T(P.sound_speed) * T(P.sound_speed) / T(P.gamma);
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                T density = wm0;
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    if (nb.pidx.isFluid()) density += fm0 * W(nb.dis);
+                    else density += rho0 * bp(nb.pidx).volume * W(nb.dis);
+                }
+                a.rho = density;
+                a.p = B * (std::pow(density / rho0, T(P.gamma)) - 1);
+                if (a.p < 0) a.p = 0;                                      /* SphBase.h:32 */
+            }
+        }
+    }
+
+    /* WcSph::wc_computeForce + wc_fluidPartForce_fsame/_fdiff/_bound, WcSph.cpp:156-312, 364-436 */
+    void wc_computeForce() {
+        T v0 = r * r * r;
+        T Hs = H, cs = T(P.sound_speed);
+        int zd = 0;
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0, alpha = fluids[k].visc_alpha;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n)) reduction(+ : zd)
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                a.a = grav;
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    T dis = nb.dis;
+                    if (dis == 0) { ++zd; continue; }
+                    if (nb.pidx.isFluid()) {
+                        const FluidPart<T>& b = fp(nb.pidx);
+                        int kb = nb.pidx.toFluidI();
+                        V3<T> xab = a.x - b.x;
+                        T grad = -Wg(dis) / dis;
+                        T acce;
+                        T pro = (a.v - b.v).dot(xab);
+                        if (kb == k) {
+                            acce = grad * (-fm0 * (a.p / (a.rho * a.rho) + b.p / (b.rho * b.rho)));
+                            if (pro < 0) {
+                                T nu = 2 * alpha * Hs * cs / (a.rho + b.rho);
+                                T pi = -nu * pro / (dis * dis + T(0.01) * Hs * Hs);
+                                acce += grad * (-fm0 * pi);
+                            }
+                        } else {
+                            T fmb = v0 * fluids[kb].rho0;
+                            T al = (alpha + fluids[kb].visc_alpha) / 2;
+                            T da = a.rho / fm0, db = b.rho / fmb;
+                            acce = grad * (-1 / fm0 * (a.p / (da * da) + b.p / (db * db)));
+                            if (pro < 0) {
+                                T nu = 2 * al * Hs * cs / (a.rho + b.rho);
+                                T pi = -nu * pro / (dis * dis + T(0.01) * Hs * Hs);
+                                acce += grad * (-(fm0 + fmb) / 2 * pi);
+                            }
+                        }
+                        a.a += xab * acce;
+                    } else {
+                        const BoundPart<T>& b = bp(nb.pidx);
+                        T ralpha = solids[nb.pidx.toSolidI()].visc_alpha;
+                        V3<T> xab = a.x - b.x;
+                        T grad = -Wg(dis) / dis, acce = 0;
+                        if (a.p > 0) acce += grad * (-rho0 * b.volume * (a.p / (a.rho * a.rho) * 2));
+                        T pro = (a.v - b.v).dot(xab);
+                        if (pro < 0) {
+                            T nu = 2 * ralpha * Hs * cs / (a.rho * 2);
+                            T pi = -nu * pro / (dis * dis + T(0.01) * Hs * Hs);
+                            acce += grad * (-rho0 * b.volume * pi);
+                        }
+                        a.a += xab * acce;
+                    }
+                }
+            }
+        }
+        zeroDis += zd;
+    }
+
+    /* SphBase::updateFluids, SphBase.cpp:377-402 (WCSPH branch) */
+    int updateFluidsWC() {
+        int removed = 0;
+        T dt = dt_cur;
+        for (auto& f : fluids) {
+            int n = int(f.parts.size());
+#pragma omp parallel for
+            for (int i = 0; i < n; ++i) {
+                f.parts[i].v += f.parts[i].a * dt;
+                f.parts[i].x += f.parts[i].v * dt;
+            }
+            removed += removeOutside(f.parts);
+        }
+        return removed;
+    }
+
+    /* ------------------------------ DFSPH ------------------------------ */
+    /* DFSPH::DF_constantDensitySolverFirst, DFSPH.cpp:84-193 */
+    void df_first() {
+        T v0 = r * r * r;
+        int zd = 0;
+        for (auto& s : solids) for (auto& b : s.parts) b.force = V3<T>();
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0, wm0 = W0 * fm0;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n)) reduction(+ : zd)
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                T density = wm0, a2 = 0;
+                V3<T> a1;
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    if (nb.dis == 0) { ++zd; continue; }
+                    T gs = -Wg(nb.dis);
+                    if (nb.pidx.isFluid()) {
+                        density += fm0 * W(nb.dis);
+                        V3<T> g = (a.x - fp(nb.pidx).x) * (gs / nb.dis);
+                        a1 += g * fm0;
+                        a2 += (gs * fm0) * (gs * fm0);
+                    } else {
+                        const BoundPart<T>& b = bp(nb.pidx);
+                        density += rho0 * b.volume * W(nb.dis);
+                        V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                        a1 += g * (rho0 * b.volume);
+                    }
+                }
+                a.rho = density;
+                T den = a1.len_square() + a2;
+                if (den < T(1.0e-6)) den = T(1.0e-6);
+                a.alpha = a.rho / den;
+            }
+        }
+        zeroDis += zd;
+    }
+
+    /* DFSPH::DF_forceExceptPressure + helpers, DFSPH.cpp:1246-1359, 1419-1475 */
+    void df_forceExceptPressure() {
+        T v0 = r * r * r, Hs = H, nu = T(P.nu), cs = T(P.sound_speed);
+        int zd = 0;
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0, alpha = fluids[k].visc_alpha;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n)) reduction(+ : zd)
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                a.p = 0;
+                a.a_adv = grav;
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    T dis = nb.dis;
+                    if (dis == 0) { ++zd; continue; }
+                    if (nb.pidx.isFluid()) {
+                        const FluidPart<T>& b = fp(nb.pidx);
+                        int kb = nb.pidx.toFluidI();
+                        V3<T> xab = a.x - b.x;
+                        T pro = (a.v - b.v).dot(xab);
+                        if (kb == k) {                                     /* DFSPH.cpp:1246-1275 */
+                            if (pro < 0) {
+                                V3<T> g2 = xab * (-Wg(dis) / dis);
+                                a.a_adv += g2 * (pro / (dis * dis + T(0.01) * Hs * Hs) * fm0 / b.rho * 10 * nu);
+                            }
+                        } else {                                           /* DFSPH.cpp:1278-1303 */
+                            T fmb = v0 * fluids[kb].rho0, al = (alpha + fluids[kb].visc_alpha) / 2;
+                            if (pro < 0) {
+                                T nuv = 2 * al * Hs * cs / (a.rho + b.rho);
+                                T pi = -nuv * pro / (dis * dis + T(0.01) * Hs * Hs);
+                                T acce = (-Wg(dis) / dis) * (-(fm0 + fmb) / 2 * pi);
+                                a.a_adv += xab * acce;
+                            }
+                        }
+                    } else {                                               /* DFSPH.cpp:1334-1359 */
+                        BoundPart<T>& b = bp(nb.pidx);
+                        V3<T> xab = a.x - b.x;
+                        T pro = (a.v - b.v).dot(xab);
+                        if (pro < 0) {
+                            V3<T> g2 = xab * (-Wg(dis) / dis);
+                            V3<T> tmp = g2 * (pro / (dis * dis + T(0.01) * Hs * Hs) * (rho0 * b.volume) / a.rho * 10 * nu);
+                            a.a_adv += tmp;
+                        }
+                    }
+                }
+            }
+        }
+        zeroDis += zd;
+    }
+
+    /* pass B of either DFSPH loop: v* -= dt * sum m (k_a/rho_a + k_b/rho_b) gradW (+ boundary),
+     * DFSPH.cpp:647-693 and 840-883.  `which` selects the kappa field. */
+    void df_velocityCorrection(int which, T dt, const std::vector<std::vector<T>>* kapOverride = nullptr) {
+        T v0 = r * r * r;
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0;
+            auto kap = [&](const FluidPart<T>& q, int kk, int idx) -> T {
+                if (kapOverride) return (*kapOverride)[kk][idx];
+                return which == 0 ? q.kappa : q.kappaV;
+            };
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                T ka = kap(a, k, i);
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    if (nb.dis == 0) continue;
+                    T gs = -Wg(nb.dis);
+                    if (nb.pidx.isFluid()) {
+                        const FluidPart<T>& b = fp(nb.pidx);
+                        T kb = kap(b, nb.pidx.toFluidI(), nb.pidx.i);
+                        V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                        a.v_adv = a.v_adv - g * (fm0 * ((ka / a.rho) + (kb / b.rho)) * dt);
+                    } else {
+                        const BoundPart<T>& b = bp(nb.pidx);
+                        V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                        V3<T> acc_p = g * (rho0 * b.volume * (ka / a.rho));
+                        a.v_adv = a.v_adv - acc_p * dt;
+                    }
+                }
+            }
+        }
+    }
+
+    /* pass A of the density loop, DFSPH.cpp:572-643: rho*, clamp, kappa.  Returns sum rho* (double). */
+    double df_densityAdv(T dt, int& nOut) {
+        T v0 = r * r * r;
+        double sum = 0; nOut = 0;
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                T dadv = a.rho;
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    if (nb.dis == 0) continue;
+                    T gs = -Wg(nb.dis);
+                    if (nb.pidx.isFluid()) {
+                        const FluidPart<T>& b = fp(nb.pidx);
+                        V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                        dadv += g.dot(a.v_adv - b.v_adv) * fm0 * dt;
+                    } else {
+                        const BoundPart<T>& b = bp(nb.pidx);
+                        V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                        dadv += g.dot(a.v_adv - b.v) * rho0 * b.volume * dt;
+                    }
+                }
+                if (dadv < rho0) dadv = rho0;
+                a.rho_adv = dadv;
+            }
+            /* serial sum (DFSPH.cpp:626-631), accumulated in double */
+            for (int i = 0; i < n; ++i) sum += double(ps[i].rho_adv) / double(rho0);
+            /* kappa (DFSPH.cpp:636-643) */
+            for (int i = 0; i < n; ++i) {
+                ps[i].kappa = (ps[i].rho_adv - rho0) / (dt * dt) * ps[i].alpha;
+                ps[i].kappa_sum += ps[i].kappa * dt * dt;
+            }
+            nOut += n;
+        }
+        return sum;
+    }
+
+    /* DFSPH::DF_constantDensitySolverSecond, DFSPH.cpp:195-738 (vorticity options omitted) */
+    void df_second(orc_stats* st) {
+        T dt = dt_cur;
+        df_forceExceptPressure();
+        for (auto& f : fluids) {
+            int n = int(f.parts.size());
+#pragma omp parallel for
+            for (int i = 0; i < n; ++i) f.parts[i].v_adv = f.parts[i].v + f.parts[i].a_adv * dt;
+        }
+        /* warm start (Bender & Koschier): apply the previous step's accumulated
+         * kappa*dt^2 as the first correction.  Not in the reference. */
+        if (P.warm_start == 1 && frame > 0) {
+            std::vector<std::vector<T>> k0(fluids.size());
+            for (size_t k = 0; k < fluids.size(); ++k) {
+                k0[k].resize(fluids[k].parts.size());
+                for (size_t i = 0; i < fluids[k].parts.size(); ++i) k0[k][i] = fluids[k].parts[i].kappa_sum / (dt * dt);
+            }
+            df_velocityCorrection(0, dt, &k0);
+        }
+        for (auto& f : fluids) for (auto& p : f.parts) p.kappa_sum = (P.warm_start == 1 && frame > 0) ? p.kappa_sum : 0;
+        double err = 1; int it = 0;
+        int maxIt = P.max_iter_density > 0 ? P.max_iter_density : std::numeric_limits<int>::max();
+        while (err > double(P.density_tol) || it < P.min_iter_density) {
+            if (it >= maxIt) { st->max_iter_hit = 1; break; }
+            int n = 0;
+            double s = df_densityAdv(dt, n);
+            df_velocityCorrection(0, dt);
+            err = n > 0 ? s / n - 1.0 : 0.0;                              /* DFSPH.cpp:696 */
+            ++it;
+        }
+        st->iter_density = it;
+        st->density_err = err;
+        for (auto& f : fluids) {
+            int n = int(f.parts.size());
+#pragma omp parallel for
+            for (int i = 0; i < n; ++i) {
+                f.parts[i].v = f.parts[i].v_adv;
+                f.parts[i].x += f.parts[i].v * dt;
+            }
+            st->removed += removeOutside(f.parts);
+        }
+    }
+
+    /* DFSPH::DF_divergenceFreeSolver, DFSPH.cpp:740-911 (stream correction omitted) */
+    void df_divergence(orc_stats* st) {
+        T v0 = r * r * r, dt = dt_cur;
+        for (auto& f : fluids) for (auto& p : f.parts) p.v_adv = p.v;
+        double avg = 0; int it = 0;
+        while (avg > double(P.divergence_tol) || it < P.min_iter_div) {
+            if (it > P.max_iter_div) { st->max_iter_hit = 1; break; }   /* DFSPH.cpp:772-773 */
+            double sum = 0; int ntot = 0;
+            for (int k = 0; k < int(fluids.size()); ++k) {
+                auto& ps = fluids[k].parts; int n = int(ps.size());
+                T rho0 = fluids[k].rho0, fm0 = rho0 * v0;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+                for (int i = 0; i < n; ++i) {
+                    FluidPart<T>& a = ps[i];
+                    T dd = 0;
+                    const NeigbStr<T>& L = nbF[k][i];
+                    for (int j = 0; j < L.num; ++j) {
+                        const auto& nb = L.neigs[j];
+                        if (nb.dis == 0) continue;
+                        T gs = -Wg(nb.dis);
+                        if (nb.pidx.isFluid()) {
+                            const FluidPart<T>& b = fp(nb.pidx);
+                            V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                            dd += g.dot(a.v_adv - b.v_adv) * fm0;
+                        } else {
+                            const BoundPart<T>& b = bp(nb.pidx);
+                            V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                            dd += g.dot(a.v_adv - b.v) * (rho0 * b.volume);
+                        }
+                    }
+                    if (dd < 0 || (a.rho + dd * dt) < rho0) dd = 0;       /* DFSPH.cpp:829-832 */
+                    a.divdev = dd;
+                    a.kappaV = dd * a.alpha / dt;                          /* DFSPH.cpp:835 */
+                }
+                for (int i = 0; i < n; ++i) sum += double(ps[i].divdev);
+                ntot += n;
+            }
+            df_velocityCorrection(1, dt);
+            avg = ntot > 0 ? sum / ntot : 0.0;
+            ++it;
+        }
+        st->iter_divergence = it;
+        st->divergence_err = avg;
+        for (auto& f : fluids) for (auto& p : f.parts) p.v = p.v_adv;
+    }
+
+    /* ------------------------------ IISPH ------------------------------ */
+    /* IISph_M::ii_forceExceptPressure + helpers, IISph_M.cpp:364-476 */
+    void ii_forceExceptPressure() {
+        T v0 = r * r * r, Hs = H, nu = T(P.nu), cs = T(P.sound_speed);
+        int zd = 0;
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0, alpha = fluids[k].visc_alpha;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n)) reduction(+ : zd)
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                a.a_adv = grav;
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    T dis = nb.dis;
+                    if (dis == 0) { ++zd; continue; }
+                    if (nb.pidx.isFluid()) {
+                        const FluidPart<T>& b = fp(nb.pidx);
+                        int kb = nb.pidx.toFluidI();
+                        V3<T> xab = a.x - b.x;
+                        T pro = (a.v - b.v).dot(xab);
+                        if (kb == k) {                                     /* IISph_M.cpp:364-385: no pro<0 gate */
+                            V3<T> g2 = xab * (-Wg(dis) / dis);
+                            a.a_adv += g2 * (pro / (dis * dis + T(0.01) * Hs * Hs) * fm0 / b.rho * 10 * nu);
+                        } else {                                           /* IISph_M.cpp:388-405 */
+                            T fmb = v0 * fluids[kb].rho0, al = (alpha + fluids[kb].visc_alpha) / 2;
+                            if (pro < 0) {
+                                T nuv = 2 * al * Hs * cs / (a.rho + b.rho);
+                                T pi = -nuv * pro / (dis * dis + T(0.01) * Hs * Hs);
+                                a.a_adv += xab * ((-Wg(dis) / dis) * (-(fm0 + fmb) / 2 * pi));
+                            }
+                        }
+                    } else {                                               /* IISph_M.cpp:408-428 */
+                        const BoundPart<T>& b = bp(nb.pidx);
+                        V3<T> xab = a.x - b.x;
+                        T pro = (a.v - b.v).dot(xab);
+                        V3<T> g2 = xab * (-Wg(dis) / dis);
+                        a.a_adv += g2 * (pro / (dis * dis + T(0.01) * Hs * Hs) * (rho0 * b.volume) / a.rho * 10 * nu);
+                    }
+                }
+            }
+        }
+        zeroDis += zd;
+    }
+
+    /* IISph_M::ii_computePressure, IISph_M.cpp:78-360, all phases solved jointly */
+    void ii_computePressure(orc_stats* st) {
+        T v0 = r * r * r, dt = dt_cur;
+        /* density first (IISph_M.cpp:101-112), so that a_adv reads this step's rho */
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0, wm0 = W0 * fm0;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                T density = wm0;
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    if (nb.pidx.isFluid()) density += fm0 * W(nb.dis);
+                    else density += rho0 * bp(nb.pidx).volume * W(nb.dis);
+                }
+                a.rho = density;
+            }
+        }
+        ii_forceExceptPressure();
+        /* d_ii and v_adv, IISph_M.cpp:114-135 */
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                V3<T> t1;
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    if (nb.dis == 0) continue;
+                    if (nb.pidx.isFluid()) {
+                        V3<T> g = (a.x - fp(nb.pidx).x) * (-Wg(nb.dis) / nb.dis);
+                        t1 += g * (-fm0 / (a.rho * a.rho));
+                    } else {
+                        const BoundPart<T>& b = bp(nb.pidx);
+                        V3<T> g = (a.x - b.x) * (-Wg(nb.dis) / nb.dis);
+                        t1 += g * (-(rho0 * b.volume) / (a.rho * a.rho));
+                    }
+                }
+                a.dii = t1 * (dt * dt);
+                a.v_adv = a.v + a.a_adv * dt;
+            }
+        }
+        /* a_ii, rho_adv, p_l, IISph_M.cpp:243-278 */
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                T t1 = 0, t2 = 0;
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    if (nb.dis == 0) continue;
+                    if (nb.pidx.isFluid()) {
+                        const FluidPart<T>& b = fp(nb.pidx);
+                        V3<T> g = (a.x - b.x) * (-Wg(nb.dis) / nb.dis);
+                        t1 += g.dot(a.v_adv - b.v_adv) * fm0 * dt;
+                        V3<T> dji = g * (fm0 * dt * dt / (a.rho * a.rho));
+                        t2 += (a.dii - dji).dot(g) * fm0;
+                    } else {
+                        const BoundPart<T>& b = bp(nb.pidx);
+                        V3<T> g = (a.x - b.x) * (-Wg(nb.dis) / nb.dis);
+                        t1 += g.dot(a.v_adv - b.v) * (rho0 * b.volume) * dt;  /* v_adv: IISph_M.cpp:269 uses v */
+                        t2 += a.dii.dot(g) * (rho0 * b.volume);
+                    }
+                }
+                if (t2 == 0) t2 = 1;
+                a.aii = t2;
+                a.rho_adv = a.rho + t1;
+                a.p_l = T(0.5) * a.p;          /* warm start from p(t-dt), IISph_M.cpp:277 */
+            }
+        }
+        /* relaxed Jacobi, IISph_M.cpp:280-358 */
+        T omega = T(P.omega);
+        int l = 0; double avgErr = 1e10;
+        int maxIt = P.max_iter_density > 0 ? P.max_iter_density : std::numeric_limits<int>::max();
+        int ntot = numFluid();
+        while (avgErr > double(P.density_tol) || l < P.min_iter_density) {
+            if (l >= maxIt) { st->max_iter_hit = 1; break; }
+            for (int k = 0; k < int(fluids.size()); ++k) {
+                auto& ps = fluids[k].parts; int n = int(ps.size());
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+                for (int i = 0; i < n; ++i) {
+                    FluidPart<T>& a = ps[i];
+                    V3<T> t1;
+                    const NeigbStr<T>& L = nbF[k][i];
+                    for (int j = 0; j < L.num; ++j) {
+                        const auto& nb = L.neigs[j];
+                        if (nb.dis == 0 || !nb.pidx.isFluid()) continue;
+                        const FluidPart<T>& b = fp(nb.pidx);
+                        T fmb = v0 * fluids[nb.pidx.toFluidI()].rho0;
+                        (void)fmb;
+                        V3<T> g = (a.x - b.x) * (-Wg(nb.dis) / nb.dis);
+                        t1 += g * (-(v0 * fluids[k].rho0 * b.p_l) / (b.rho * b.rho));
+                    }
+                    a.sum_dijpj = t1 * (dt * dt);
+                }
+            }
+            double sumErr = 0;
+            for (int k = 0; k < int(fluids.size()); ++k) {
+                auto& ps = fluids[k].parts; int n = int(ps.size());
+                T rho0 = fluids[k].rho0, fm0 = rho0 * v0;
+                std::vector<T> newp(n), errs(n);
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+                for (int i = 0; i < n; ++i) {
+                    FluidPart<T>& a = ps[i];
+                    T fin = 0;
+                    const NeigbStr<T>& L = nbF[k][i];
+                    for (int j = 0; j < L.num; ++j) {
+                        const auto& nb = L.neigs[j];
+                        if (nb.dis == 0) continue;
+                        if (nb.pidx.isFluid()) {
+                            const FluidPart<T>& b = fp(nb.pidx);
+                            V3<T> g = (a.x - b.x) * (-Wg(nb.dis) / nb.dis);
+                            V3<T> dji = g * (dt * dt * fm0 / (a.rho * a.rho));
+                            V3<T> t1 = (a.sum_dijpj - b.dii * b.p_l) - (b.sum_dijpj - dji * a.p_l);
+                            fin += fm0 * t1.dot(g);
+                        } else {
+                            const BoundPart<T>& b = bp(nb.pidx);
+                            V3<T> g = (a.x - b.x) * (-Wg(nb.dis) / nb.dis);
+                            fin += rho0 * b.volume * a.sum_dijpj.dot(g);
+                        }
+                    }
+                    T t = (1 - omega) * a.p_l + (omega / a.aii) * (rho0 - a.rho_adv - fin);
+                    if (t < 0) { t = 0; errs[i] = 0; }
+                    else errs[i] = (a.rho_adv + a.aii * a.p_l + fin - rho0) / rho0;
+                    newp[i] = t;
+                }
+                for (int i = 0; i < n; ++i) {             /* Jacobi: publish after the sweep */
+                    ps[i].p_l = newp[i]; ps[i].p = newp[i];
+                    sumErr += double(errs[i]);
+                }
+            }
+            avgErr = ntot > 0 ? sumErr / ntot : 0.0;
+            ++l;
+        }
+        st->iter_density = l;
+        st->density_err = avgErr;
+    }
+
+    /* IISph_M::ii_computeForcePressure, IISph_M.cpp:478-684 */
+    void ii_computeForcePressure() {
+        T v0 = r * r * r, Hs = H, cs = T(P.sound_speed);
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0, alpha = fluids[k].visc_alpha;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                a.a_p = V3<T>();
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    T dis = nb.dis;
+                    if (dis == 0) continue;
+                    if (nb.pidx.isFluid()) {
+                        const FluidPart<T>& b = fp(nb.pidx);
+                        int kb = nb.pidx.toFluidI();
+                        V3<T> xab = a.x - b.x;
+                        T grad = -Wg(dis) / dis, acce;
+                        if (kb == k) {
+                            acce = grad * (-fm0 * (a.p / (a.rho * a.rho) + b.p / (b.rho * b.rho)));
+                        } else {
+                            T fmb = v0 * fluids[kb].rho0, al = (alpha + fluids[kb].visc_alpha) / 2;
+                            T da = a.rho / fm0, db = b.rho / fmb;
+                            acce = grad * (-1 / fm0 * (a.p / (da * da) + b.p / (db * db)));
+                            T pro = (a.v - b.v).dot(xab);
+                            if (pro < 0) {
+                                T nuv = 2 * al * Hs * cs / (a.rho + b.rho);
+                                T pi = -nuv * pro / (dis * dis + T(0.01) * Hs * Hs);
+                                acce += grad * (-(fm0 + fmb) / 2 * pi);
+                            }
+                        }
+                        a.a_p += xab * acce;
+                    } else {
+                        const BoundPart<T>& b = bp(nb.pidx);
+                        V3<T> xab = a.x - b.x;
+                        T grad = -Wg(dis) / dis, acce = 0;
+                        if (a.p > 0) acce += grad * (-rho0 * b.volume * (a.p / (a.rho * a.rho) * 2));
+                        a.a_p += xab * acce;
+                    }
+                }
+            }
+        }
+    }
+
+    int updateFluidsII() {
+        int removed = 0;
+        T dt = dt_cur;
+        for (auto& f : fluids) {
+            int n = int(f.parts.size());
+#pragma omp parallel for
+            for (int i = 0; i < n; ++i) {
+                /* v(t+dt) = v_adv + dt a_p  (SphBase.cpp:387-388 drops a_adv) */
+                f.parts[i].v = f.parts[i].v_adv + f.parts[i].a_p * dt;
+                f.parts[i].x += f.parts[i].v * dt;
+            }
+            removed += removeOutside(f.parts);
+        }
+        return removed;
+    }
+
+    /* ------------------------------ step ------------------------------ */
+    void cflDt() {
+        if (P.cfl <= 0) { dt_cur = T(P.dt); return; }
+        T vmax = 0;
+        for (auto& f : fluids) for (auto& p : f.parts) vmax = std::max(vmax, std::sqrt(p.v.len_square()));
+        T dt = T(P.dt);
+        if (vmax > 0) dt = std::min(dt, T(P.cfl) * r / vmax);
+        dt = std::max(dt, T(P.dt_min));
+        dt_cur = dt;
+    }
+
+    void fillStats(orc_stats* st) {
+        double se = 0, me = -1e300, ke = 0; long long n = 0; int nan = 0;
+        T v0 = r * r * r;
+        for (auto& f : fluids)
+            for (auto& p : f.parts) {
+                double e = double(p.rho) / double(f.rho0) - 1.0;
+                se += e; me = std::max(me, e); ++n;
+                ke += 0.5 * double(f.rho0 * v0) * double(p.v.len_square());
+                if (!std::isfinite(double(p.x.x + p.x.y + p.x.z + p.v.x + p.v.y + p.v.z))) ++nan;
+            }
+        st->rho_mean_err = n ? se / n : 0;
+        st->rho_max_err = n ? me : 0;
+        st->kinetic_energy = ke;
+        st->nan_count = nan;
+        st->dt = float(dt_cur);
+    }
+
+    void step(orc_stats* st) {
+        std::memset(st, 0, sizeof(*st));
+        int zd0 = zeroDis, tm0 = tooMany;
+        cflDt();
+        if (P.solver == ORC_WCSPH) {                                       /* WcSph.cpp:23-96 */
+            neighbourSearch();
+            updateSolidPartWeight();
+            wc_computePressure();
+            wc_computeForce();
+            st->removed = updateFluidsWC();
+            fillStats(st);
+        } else if (P.solver == ORC_DFSPH) {                                /* DFSPH.cpp:9-75 */
+            if (frame == 0) { neighbourSearch(); updateSolidPartWeight(); df_first(); }
+            df_second(st);
+            neighbourSearch();
+            updateSolidPartWeight();
+            df_first();
+            df_divergence(st);
+            fillStats(st);
+        } else {                                                           /* IISph_M.cpp:10-75 */
+            neighbourSearch();
+            updateSolidPartWeight();
+            ii_computePressure(st);
+            ii_computeForcePressure();
+            st->removed = updateFluidsII();
+            fillStats(st);
+        }
+        st->zero_dis = zeroDis - zd0;
+        st->too_many_neigb = tooMany - tm0;
+        st->dt = float(dt_cur);
+        dt_prev = dt_cur;
+        ++frame;
+    }
+};
+
+struct SimBase {
+    int bits;
+    void* p;
+};
+
+}  // namespace
+
+template <class T>
+static void kernel_eval_t(double h, int n, const double* r, double* W, double* dW) {
+    orc_params p{};
+    p.spacing_r = float(h);
+    p.smooth_radius_h = float(2 * h);
+    p.dt = 1e-3f;
+    Sim<T> S(p);
+    S.kh = T(h);
+    S.sigma = T(1) / T(M_PI) / (S.kh * S.kh * S.kh);
+    S.sigma_h = S.sigma / S.kh;
+    for (int i = 0; i < n; ++i) {
+        W[i] = double(S.W(T(r[i])));
+        dW[i] = double(S.Wg(T(r[i])));
+    }
+}
+
+struct orc_sim {
+    int bits;
+    std::unique_ptr<Sim<float>> f;
+    std::unique_ptr<Sim<double>> d;
+};
+
+template <class F>
+static int dispatch(orc_sim* s, F&& fn) {
+    if (!s) return -1;
+    if (s->bits == 32) return fn(*s->f);
+    return fn(*s->d);
+}
+template <class F>
+static int dispatchc(const orc_sim* s, F&& fn) {
+    if (!s) return -1;
+    if (s->bits == 32) return fn(*s->f);
+    return fn(*s->d);
+}
+
+extern "C" {
+
+orc_sim* orc_create(int bits, const orc_params* p) {
+    if (!p || (bits != 32 && bits != 64)) return nullptr;
+    orc_sim* s = new orc_sim();
+    s->bits = bits;
+    if (bits == 32) s->f.reset(new Sim<float>(*p));
+    else s->d.reset(new Sim<double>(*p));
+    return s;
+}
+
+void orc_destroy(orc_sim* s) { delete s; }
+
+int orc_add_fluid(orc_sim* s, int n, const float* xyz, const float* vel, const int* ids,
+                  float rho0, float visc) {
+    return dispatch(s, [&](auto& S) {
+        using T = std::decay_t<decltype(S.r)>;
+        Fluid<T> f; f.rho0 = T(rho0); f.visc_alpha = T(visc);
+        f.parts.resize(n);
+        for (int i = 0; i < n; ++i) {
+            auto& q = f.parts[i];
+            q.x = V3<T>(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+            if (vel) q.v = V3<T>(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+            q.d = T(S.P.spacing_r);
+            q.id = ids ? ids[i] : i;
+        }
+        S.fluids.push_back(std::move(f));
+        return int(S.fluids.size()) - 1;
+    });
+}
+
+int orc_add_solid(orc_sim* s, int n, const float* xyz, int dynamic, float visc) {
+    return dispatch(s, [&](auto& S) {
+        using T = std::decay_t<decltype(S.r)>;
+        Solid<T> so; so.dynamic = dynamic != 0; so.rigid = true; so.visc_alpha = T(visc);
+        so.parts.resize(n);
+        for (int i = 0; i < n; ++i) {
+            so.parts[i].x = V3<T>(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+            so.parts[i].d = T(S.P.spacing_r);
+        }
+        S.solids.push_back(std::move(so));
+        return int(S.solids.size()) - 1;
+    });
+}
+
+int orc_init(orc_sim* s) {
+    return dispatch(s, [&](auto& S) { S.prepareSolidPartWeight(); return 0; });
+}
+
+int orc_step(orc_sim* s, orc_stats* st) {
+    return dispatch(s, [&](auto& S) { S.step(st); return 0; });
+}
+
+int orc_num_fluid(const orc_sim* s, int k) {
+    return dispatchc(s, [&](const auto& S) {
+        return (k < 0 || k >= int(S.fluids.size())) ? -1 : int(S.fluids[k].parts.size());
+    });
+}
+
+int orc_get_fluid(const orc_sim* s, int k, int field, double* out) {
+    return dispatchc(s, [&](const auto& S) {
+        if (k < 0 || k >= int(S.fluids.size())) return -1;
+        const auto& ps = S.fluids[k].parts;
+        size_t n = ps.size();
+        for (size_t i = 0; i < n; ++i) {
+            const auto& q = ps[i];
+            switch (field) {
+                case ORC_F_POS: out[3 * i] = q.x.x; out[3 * i + 1] = q.x.y; out[3 * i + 2] = q.x.z; break;
+                case ORC_F_VEL: out[3 * i] = q.v.x; out[3 * i + 1] = q.v.y; out[3 * i + 2] = q.v.z; break;
+                case ORC_F_RHO: out[i] = q.rho; break;
+                case ORC_F_PRESSURE: out[i] = q.p; break;
+                case ORC_F_ACC:
+                    if (S.P.solver == ORC_IISPH) { out[3 * i] = q.a_p.x; out[3 * i + 1] = q.a_p.y; out[3 * i + 2] = q.a_p.z; }
+                    else { out[3 * i] = q.a.x; out[3 * i + 1] = q.a.y; out[3 * i + 2] = q.a.z; }
+                    break;
+                case ORC_F_ALPHA: out[i] = q.alpha; break;
+                case ORC_F_KAPPA: out[i] = q.kappa; break;
+                case ORC_F_ID: out[i] = q.id; break;
+                case ORC_F_AADV: out[3 * i] = q.a_adv.x; out[3 * i + 1] = q.a_adv.y; out[3 * i + 2] = q.a_adv.z; break;
+                case ORC_F_KAPPA_SUM: out[i] = q.kappa_sum; break;
+                case ORC_F_AII: out[i] = q.aii; break;
+                case ORC_F_RHO_ADV: out[i] = q.rho_adv; break;
+                default: return -2;
+            }
+        }
+        return int(n);
+    });
+}
+
+int orc_get_boundary_volume(const orc_sim* s, int k, double* out) {
+    return dispatchc(s, [&](const auto& S) {
+        if (k < 0 || k >= int(S.solids.size())) return -1;
+        const auto& ps = S.solids[k].parts;
+        for (size_t i = 0; i < ps.size(); ++i) out[i] = ps[i].volume;
+        return int(ps.size());
+    });
+}
+
+int orc_neighbours(orc_sim* s, int k, int cap, int* counts, int* kinds, int* ids, float* dis) {
+    return dispatch(s, [&](auto& S) {
+        int before = S.tooMany;
+        S.neighbourSearch();
+        if (k < 0 || k >= int(S.fluids.size())) return -1;
+        std::vector<int> solidBase(S.solids.size() + 1, 0);
+        for (size_t q = 0; q < S.solids.size(); ++q) solidBase[q + 1] = solidBase[q] + int(S.solids[q].parts.size());
+        const auto& L = S.nbF[k];
+        for (size_t i = 0; i < L.size(); ++i) {
+            int c = std::min(L[i].num, cap);
+            counts[i] = L[i].num;
+            for (int j = 0; j < c; ++j) {
+                const auto& nb = L[i].neigs[j];
+                size_t o = i * size_t(cap) + j;
+                if (nb.pidx.isFluid()) { kinds[o] = 0; ids[o] = S.fp(nb.pidx).id; }
+                else { kinds[o] = 1; ids[o] = solidBase[nb.pidx.toSolidI()] + nb.pidx.i; }
+                dis[o] = float(nb.dis);
+            }
+        }
+        return S.tooMany - before;
+    });
+}
+
+int orc_compute_density(orc_sim* s) {
+    return dispatch(s, [&](auto& S) {
+        S.neighbourSearch();
+        S.updateSolidPartWeight();
+        S.df_first();
+        return 0;
+    });
+}
+
+void orc_kernel_eval(int bits, double h, int n, const double* r, double* W, double* dW) {
+    if (bits == 32) kernel_eval_t<float>(h, n, r, W, dW);
+    else kernel_eval_t<double>(h, n, r, W, dW);
+}
+
+void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"

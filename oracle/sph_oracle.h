@@ -1,0 +1,109 @@
+/*
+ * sph_oracle.h -- C interface of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a CPU restatement of the reference
+ * SPH particle loops (Artemisaturn/GRAPHICS, Source/).  It is the checker for
+ * the HIP path and, in its FP32 instantiation, the timed CPU baseline
+ * ("port": the reference's own OpenMP code cannot be built, its external
+ * headers are not vendored and several of its translation units are stale).
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it; the product library never links it.
+ */
+#ifndef SPH_ORACLE_H_
+#define SPH_ORACLE_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_WCSPH = 0, ORC_DFSPH = 1, ORC_IISPH = 2 };
+
+/* Mirrors ThresholdsBase m_TH (SphBase.h:546-583) plus the solver settings
+ * that the reference hard-codes (DFSPH.cpp:225-230, 762-766; IISph_M.cpp:285,
+ * 341; WcSph.h:34). */
+typedef struct orc_params {
+    int   solver;              /* ORC_* */
+    float spacing_r;           /* SphBase.h:548 */
+    float smooth_radius_h;     /* SphBase.h:549, cell size and 2x kernel h */
+    float space_min[3];        /* SphBase.h:559 */
+    float space_max[3];        /* SphBase.h:560 */
+    float dt;                  /* fixed dt, or dt_max when cfl > 0 */
+    float gravity[3];
+    float sound_speed;         /* m_TH.soundSpeed_cs (WCSPH, multiphase viscosity) */
+    float gamma;               /* WcSph m_DensityPower_gamma (WcSph.h:34) */
+    float nu;                  /* m_TH.nu: DFSPH/IISPH viscosity (DFSPH.cpp:1265) */
+    float cfl;                 /* 0: fixed dt;  >0: dt = min(cfl*r/max|v|, dt) */
+    float dt_min;
+    float density_tol;         /* DFSPH: relative (DFSPH.cpp:225); IISPH: eta/rho0 (IISph_M.cpp:285) */
+    float divergence_tol;      /* DFSPH: absolute kg m^-3 s^-1 (DFSPH.cpp:762) */
+    int   min_iter_density;    /* DFSPH.cpp:229 (2); IISph_M.cpp:286 (2) */
+    int   max_iter_density;    /* none in the reference (DFSPH.cpp:228 unused) */
+    int   min_iter_div;        /* DFSPH.cpp:765 (1) */
+    int   max_iter_div;        /* DFSPH.cpp:764 (30, loop breaks when it > 30) */
+    float omega;               /* IISPH relaxed Jacobi (IISph_M.cpp:341) */
+    int   warm_start;          /* 0 none; 1 previous step's accumulated kappa */
+    int   reserved[6];
+} orc_params;
+
+typedef struct orc_stats {
+    int    iter_density;       /* DFSPH density / IISPH pressure iterations */
+    int    iter_divergence;    /* DFSPH divergence iterations */
+    double density_err;        /* last solver density error (relative) */
+    double divergence_err;     /* last divergence error (absolute) */
+    double rho_mean_err;       /* mean(rho/rho0 - 1) over fluid */
+    double rho_max_err;        /* max(rho/rho0 - 1) */
+    double kinetic_energy;     /* sum 1/2 m |v|^2 */
+    float  dt;                 /* dt used by this step */
+    int    zero_dis;           /* m_EC.zeroDis increments this step */
+    int    too_many_neigb;     /* m_EC.tooManyNeigb increments this step */
+    int    removed;            /* particles removed outside [spaceMin,spaceMax] */
+    int    max_iter_hit;       /* a capped solver loop hit its cap */
+    int    nan_count;          /* non-finite fluid positions/velocities */
+} orc_stats;
+
+/* fluid field selectors for orc_get_fluid */
+enum {
+    ORC_F_POS = 0,      /* 3 */
+    ORC_F_VEL = 1,      /* 3 */
+    ORC_F_RHO = 2,      /* 1  density of the last density pass */
+    ORC_F_PRESSURE = 3, /* 1  WCSPH/IISPH pressure */
+    ORC_F_ACC = 4,      /* 3  WCSPH acceleration, IISPH pressure acceleration */
+    ORC_F_ALPHA = 5,    /* 1  DFSPH alpha factor */
+    ORC_F_KAPPA = 6,    /* 1  DFSPH last kappa */
+    ORC_F_ID = 7,       /* 1  original id (as double) */
+    ORC_F_AADV = 8,     /* 3  non-pressure acceleration */
+    ORC_F_KAPPA_SUM = 9,/* 1  accumulated kappa*dt^2 of the density solve */
+    ORC_F_AII = 10,     /* 1  IISPH a_ii */
+    ORC_F_RHO_ADV = 11  /* 1  IISPH rho_adv */
+};
+
+typedef struct orc_sim orc_sim;
+
+orc_sim* orc_create(int precision_bits, const orc_params* p);
+void     orc_destroy(orc_sim* s);
+int      orc_add_fluid(orc_sim* s, int n, const float* xyz, const float* vel,
+                       const int* ids, float rest_density, float visc_alpha);
+int      orc_add_solid(orc_sim* s, int n, const float* xyz, int dynamic, float visc_alpha);
+int      orc_init(orc_sim* s);                       /* prepareSolidPartWeight */
+int      orc_step(orc_sim* s, orc_stats* st);        /* one sphStep + runOneStep bookkeeping */
+int      orc_num_fluid(const orc_sim* s, int k);
+int      orc_get_fluid(const orc_sim* s, int k, int field, double* out);
+int      orc_get_boundary_volume(const orc_sim* s, int k, double* out);
+/* Runs gridSetup + neighbourSearch on the current state and dumps the fluid
+ * neighbour lists: per fluid particle (current order) up to `cap` entries of
+ * (kind 0 fluid / 1 boundary, id, dis).  Returns total overflow count. */
+int      orc_neighbours(orc_sim* s, int k, int cap, int* counts, int* kinds,
+                        int* ids, float* dis);
+/* Runs neighbourSearch + updateSolidPartWeight + DF_constantDensitySolverFirst
+ * on the current state (density and alpha fields), without stepping. */
+int      orc_compute_density(orc_sim* s);
+/* Evaluates the cubic spline W(r) and |dW/dr| (SphBase.h:855-906) with kernel
+ * h at n radii, in the given precision. */
+void     orc_kernel_eval(int precision_bits, double h, int n, const double* r, double* W, double* dW);
+void     orc_set_threads(int n);
+int      orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
