@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Throughput of the SPH particle loops on one MI355X.
+
+A step is one SphBase::runOneStep of the solver over the whole particle set
+(neighbour search, density/alpha, non-pressure forces, the density and
+divergence solver loops, integration).  Inputs are resident in HBM before the
+timed region.  Prints ONE JSON line on stdout; progress goes to stderr.
+
+Default workload: DFSPH dam break, 1,000,000 fluid particles at 0.01 m plus
+400,002 wall particles, CFL dt (0.4, dt_max 1e-3 s), density tolerance 0.01 %
+average, divergence tolerance 1 kg m^-3 s^-1 average.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from graphics_amd import scenes, sphgpu as sg  # noqa: E402
+
+METRIC = "particle-steps/s at <=0.1% density error; solver iters/step learned vs plain"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per fluid particle of each pass (FP32 SoA, vec3 = 12 B,
+# scalar = 4 B, each array counted once per pass; neighbour lists not counted)
+ALG_BYTES = {
+    "df_pass_velocity": 44, "df_pass_density": 36, "df_pass_divergence": 36,
+    "df_density_alpha": 20, "df_aadv": 40, "wc_pressure": 20, "wc_force": 56,
+    "ii_sumdij": 32, "ii_pressure": 56, "ii_aadv_dii": 64, "ii_aii": 48, "ii_force": 56,
+}
+DOMINANT = {sg.DFSPH: "df_pass_velocity", sg.IISPH: "ii_pressure", sg.WCSPH: "wc_force"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_scene(name: str):
+    return scenes.WORKLOADS[name]()
+
+
+def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None):
+    sim = scenes.make_simulation(sc, warm_start=warm_start)
+    t0 = time.perf_counter()
+    sim.init()
+    log(f"[bench] init {time.perf_counter() - t0:.2f}s  fluid={sc.num_fluid} boundary={sc.num_boundary}")
+    warm_times, warm_stats = [], []
+    for k in range(warmup):
+        t = time.perf_counter()
+        warm_stats.append(sim.step())
+        warm_times.append(time.perf_counter() - t)
+        if k % 10 == 0:
+            s = warm_stats[-1]
+            log(f"[bench] warmup {k}: {warm_times[-1] * 1e3:.2f} ms it={s['iter_density']}/{s['iter_divergence']}")
+    if profile:
+        sim.profile_kernel(profile)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    stats = [sim.step() for _ in range(steps)]
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    timing = sim.kernel_timing() if profile else (0, 0.0, 0.0)
+    n_live = sim.num_fluid()
+    sim.close()
+    return dict(elapsed=elapsed, stats=stats, warm_times=warm_times, warm_stats=warm_stats,
+                timing=timing, n_live=n_live)
+
+
+def cpu_baseline(sc, warm_start: int, budget_s: float):
+    from oracle import oracle as orc
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    threads = max(1, min(16, ncpu))
+    orc.set_threads(threads)
+    o = orc.from_scene(sc, 32, warm_start=warm_start)
+    steps, t0 = 0, time.perf_counter()
+    per_step = []
+    while True:
+        t = time.perf_counter()
+        o.step()
+        per_step.append(time.perf_counter() - t)
+        steps += 1
+        log(f"[bench] cpu step {steps - 1}: {per_step[-1]:.2f}s")
+        if time.perf_counter() - t0 >= budget_s or steps >= 50:
+            break
+    elapsed = time.perf_counter() - t0
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    o.close()
+    return dict(steps=steps, elapsed=elapsed, threads=threads, model=model, per_step=per_step)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--workload", default="dfsph_c2", choices=sorted(scenes.WORKLOADS))
+    ap.add_argument("--warm-start", default="previous", choices=["none", "previous"])
+    ap.add_argument("--no-ab", action="store_true", help="skip the plain-solver arm")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--profile-kernel", default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no "
+              "multi-GPU path", file=sys.stderr)
+        return 2
+    if not sg.device_available():
+        print("no gfx9 HIP device visible", file=sys.stderr)
+        return 3
+
+    sc = build_scene(args.workload)
+    solver = sc.params["solver"]
+    warm = {"none": sg.WARM_NONE, "previous": sg.WARM_PREVIOUS}[args.warm_start]
+    prof = args.profile_kernel or DOMINANT[solver]
+    run = run_gpu(sc, warm, args.steps, args.warmup, prof)
+    nf = sc.num_fluid
+    stats = run["stats"]
+    value = nf * args.steps / run["elapsed"]
+    ms = run["elapsed"] / args.steps * 1e3
+    it_d = float(np.mean([s["iter_density"] for s in stats]))
+    it_v = float(np.mean([s["iter_divergence"] for s in stats]))
+    dens_err = float(max(s["density_err"] for s in stats))
+    rho_mean = float(np.mean([s["rho_mean_err"] for s in stats]))
+    log(f"[bench] timed {args.steps} steps: {ms:.3f} ms/step, {value:.4g} particle-steps/s, "
+        f"iters {it_d:.2f}/{it_v:.2f}, max density err {dens_err:.3g}")
+
+    launches, tot_ms, tot_bytes = run["timing"]
+    roofline = None
+    if launches:
+        avg_s = tot_ms / launches * 1e-3
+        alg = ALG_BYTES[prof] * nf
+        achieved = alg / avg_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "kernel": prof, "avg_us": round(avg_s * 1e6, 3), "launches": launches,
+                    "alg_bytes_per_launch": alg}
+
+    iters = {"warm_start": args.warm_start, "density": it_d, "divergence": it_v}
+    if not args.no_ab and warm != sg.WARM_NONE:
+        plain = run_gpu(sc, sg.WARM_NONE, args.steps, args.warmup, None)
+        ps = plain["stats"]
+        iters["plain"] = {"density": float(np.mean([s["iter_density"] for s in ps])),
+                          "divergence": float(np.mean([s["iter_divergence"] for s in ps])),
+                          "particle_steps_per_s": nf * args.steps / plain["elapsed"]}
+        tot_w = it_d + it_v
+        tot_p = iters["plain"]["density"] + iters["plain"]["divergence"]
+        iters["reduction_vs_plain"] = round(1.0 - tot_w / tot_p, 4) if tot_p else None
+        log(f"[bench] plain arm: iters {iters['plain']['density']:.2f}/{iters['plain']['divergence']:.2f}")
+
+    cpu = None
+    if not args.no_cpu:
+        c = cpu_baseline(sc, warm, args.cpu_seconds)
+        cpu_rate = nf * c["steps"] / c["elapsed"]
+        gpu_same = nf * c["steps"] / sum(run["warm_times"][:c["steps"]]) if len(run["warm_times"]) >= c["steps"] else None
+        cpu = {"value": round(cpu_rate, 2), "unit": "particle-steps/s", "cores": c["threads"],
+               "kind": "port",
+               "sample": (f"steps 0-{c['steps'] - 1} of the same workload from the same initial state, "
+                          f"FP32 oracle (reference loop structure, OpenMP) on {c['model']}"),
+               "gpu_same_sample": round(gpu_same, 2) if gpu_same else None,
+               "speedup_same_sample": round(gpu_same / cpu_rate, 1) if gpu_same else None}
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": sc.name, "solver": ["WCSPH", "DFSPH", "IISPH"][solver],
+                   "fluid": nf, "boundary": sc.num_boundary, "scenes": sc.num_scenes,
+                   "density_tol": sc.params.get("density_tol"),
+                   "divergence_tol": sc.params.get("divergence_tol"),
+                   "max_density_err": dens_err, "mean_rho_err": rho_mean,
+                   "live_after": run["n_live"]},
+        "iters_per_step": iters,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

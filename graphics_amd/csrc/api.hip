@@ -210,7 +210,7 @@ int solver_loop(sphgpu_handle* h, int which) {
                 }
                 {
                     ProfScope ps(h, "df_pass_velocity", nf_live(h) * 44.0);
-                    launch_df_pass_velocity(D, D.kr, h->st);
+                    launch_df_pass_velocity(D, h->st);
                 }
                 launch_scene_reduce(D, 0, P.density_tol, P.min_iter_density,
                                     P.max_iter_density > 0 ? P.max_iter_density : 1 << 30, h->st);
@@ -221,7 +221,7 @@ int solver_loop(sphgpu_handle* h, int which) {
                 }
                 {
                     ProfScope ps(h, "df_pass_velocity", nf_live(h) * 44.0);
-                    launch_df_pass_velocity(D, D.kr, h->st);
+                    launch_df_pass_velocity(D, h->st);
                 }
                 launch_scene_reduce(D, 1, P.divergence_tol, P.min_iter_div, P.max_iter_div + 1, h->st);
             }
@@ -234,7 +234,6 @@ int solver_loop(sphgpu_handle* h, int which) {
                 ProfScope ps(h, "ii_pressure", nf_live(h) * 56.0);
                 launch_ii_pressure(D, h->st);
             }
-            std::swap(D.kr, D.f1);
             launch_scene_reduce(D, 1, P.density_tol, P.min_iter_density,
                                 P.max_iter_density > 0 ? P.max_iter_density : 1 << 30, h->st);
         }
@@ -334,11 +333,7 @@ int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
     int warm = h->P.warm_start == SPHGPU_WARM_PREVIOUS ? 1 : 0;
     {
         ProfScope ps(h, "df_aadv", nf_live(h) * 40.0);
-        launch_df_aadv(D, warm, h->st);
-    }
-    if (warm) {
-        launch_fill_i32(D.active, 1, D.nscenes, h->st);
-        launch_df_pass_velocity(D, D.kr0, h->st);
+        launch_df_aadv(D, h->st);
     }
     if ((rc = solver_loop(h, 0))) return rc;
     launch_df_integrate(D, h->st);
@@ -348,6 +343,11 @@ int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
         launch_df_density_alpha(D, h->st);
     }
     HIPCHECK(h, hipMemcpyAsync(D.vs, D.v, sizeof(float4) * D.npad, hipMemcpyDeviceToDevice, h->st));
+    launch_df_warm_div(D, warm && h->frame > 0, h->st);
+    if (warm && h->frame > 0) {
+        launch_fill_i32(D.active, 1, D.nscenes, h->st);
+        launch_df_pass_velocity(D, h->st);
+    }
     if ((rc = solver_loop(h, 1))) return rc;
     std::swap(D.v, D.vs);
     return finish_step(h, st);
@@ -639,10 +639,12 @@ int sphgpu_init(sphgpu_handle* h) {
 #define ALLOC(ptr, n) if ((rc = dalloc(h, &(ptr), (n)))) return rc
     ALLOC(D.x, ncap); ALLOC(D.v, ncap); ALLOC(D.id, ncap); ALLOC(D.ksum, ncap); ALLOC(D.kvsum, ncap); ALLOC(D.key, ncap);
     ALLOC(D.x2, ncap); ALLOC(D.v2, ncap); ALLOC(D.id2, ncap); ALLOC(D.ksum2, ncap); ALLOC(D.kvsum2, ncap); ALLOC(D.key2, ncap);
-    ALLOC(D.vs, ncap); ALLOC(D.f4a, ncap); ALLOC(D.f4b, ncap);
+    ALLOC(D.xk, ncap); ALLOC(D.vs, ncap); ALLOC(D.f4a, ncap); ALLOC(D.f4b, ncap);
     ALLOC(D.rho, ncap); ALLOC(D.alpha, ncap); ALLOC(D.kr, ncap); ALLOC(D.kr0, ncap); ALLOC(D.f1, ncap); ALLOC(D.f2, ncap);
     ALLOC(D.nbF, (size_t)NBF_CAP * ncap); ALLOC(D.nbB, (size_t)NBB_CAP * ncap); ALLOC(D.cnt, ncap);
-    ALLOC(D.xb, nb); ALLOC(D.vb, nb); ALLOC(D.bid, nb); ALLOC(D.bsolid, nb); ALLOC(D.bscene, nb); ALLOC(D.bkself, nb); ALLOC(D.bkey, nb);
+    // boundary arrays carry one extra dummy particle at index nb (list padding)
+    ALLOC(D.xb, nb + 1); ALLOC(D.vb, nb + 1); ALLOC(D.bid, nb + 1); ALLOC(D.bsolid, nb + 1);
+    ALLOC(D.bscene, nb + 1); ALLOC(D.bkself, nb + 1); ALLOC(D.bkey, nb + 1);
     SceneGrid* dgrids; int* drow; uint32_t* dkb;
     ALLOC(dgrids, ns); ALLOC(drow, rowRank.size()); ALLOC(dkb, ns + 1);
     D.grids = dgrids; D.rowRank = drow; D.sceneKeyBase = dkb;
@@ -735,6 +737,11 @@ int sphgpu_init(sphgpu_handle* h) {
         launch_boundary_reorder(D, tx, tsolid, tscene, h->sk, h->sv, st);
         launch_cell_build(D, D.bkey, int(nb), D.cellB, st);
         launch_boundary_volume(D, st);
+    }
+    {
+        float4 far = make_float4(DUMMY_FAR, DUMMY_FAR, DUMMY_FAR, 0.0f), zero = make_float4(0, 0, 0, 0);
+        HIPCHECK(h, hipMemcpyAsync(D.xb + nb, &far, sizeof(float4), hipMemcpyHostToDevice, st));
+        HIPCHECK(h, hipMemcpyAsync(D.vb + nb, &zero, sizeof(float4), hipMemcpyHostToDevice, st));
     }
     HIPCHECK(h, hipStreamSynchronize(st));
     HIPCHECK(h, hipGetLastError());

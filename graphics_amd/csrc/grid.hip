@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     float4 pa = D.x[i];
     int cx, cy, cz;
     cell_of(G, pa, cx, cy, cz);
-    int nf = 0, nbn = 0, over = 0;
+    int nf = 0, nbn = 0, over = 0, zero = 0;
     const int stride = D.ncap;
     for (int dz = -1; dz <= 1; ++dz) {
         int z = cz + dz;
@@ -188,27 +188,35 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
             for (int j = rf.x; j < rf.y; ++j) {
                 if (j == i) continue;
                 float4 pb = D.x[j];
-                if (dis_square_exact(pa, pb) < D.h2) {
+                float l2 = dis_square_exact(pa, pb);
+                if (l2 < D.h2) {
                     if (nf < NBF_CAP) D.nbF[nf * stride + i] = j;
                     else ++over;
                     ++nf;
+                    zero += (l2 == 0.0f);
                 }
             }
             int2 rb = run3(D.cellB, k0, cx, G.nx);
             for (int j = rb.x; j < rb.y; ++j) {
                 float4 pb = D.xb[j];
-                if (dis_square_exact(pa, pb) < D.h2) {
+                float l2 = dis_square_exact(pa, pb);
+                if (l2 < D.h2) {
                     if (nbn < NBB_CAP) D.nbB[nbn * stride + i] = j;
                     else ++over;
                     ++nbn;
+                    zero += (l2 == 0.0f);
                 }
             }
         }
     }
-    D.cnt[i] = min(nf, NBF_CAP) | (min(nbn, NBB_CAP) << 16);
+    int cf = min(nf, NBF_CAP), cb = min(nbn, NBB_CAP);
+    for (int k = cf; k < ((cf + 3) & ~3); ++k) D.nbF[k * stride + i] = i;      // pad: self
+    for (int k = cb; k < ((cb + 3) & ~3); ++k) D.nbB[k * stride + i] = D.nb;   // pad: far dummy
+    D.cnt[i] = cf | (cb << 16);
     int tm = nf + nbn - REF_NEIGHBOUR_CAP;
     if (tm > 0) atomicAdd(&D.counters[C_TOO_MANY], tm);
     if (over > 0) atomicAdd(&D.counters[C_LIST_OVF], over);
+    if (zero > 0) atomicAdd(&D.counters[C_ZERO_DIS], zero);
 }
 
 // ---- boundary --------------------------------------------------------------

@@ -1,8 +1,11 @@
 // Per-particle neighbour passes of the three solvers.  Each kernel is one
 // loop over particles of the reference (cited per kernel), one thread per
-// particle, neighbours read through the ELL lists built by grid.hip.  Sums
-// that decide convergence are reduced in FP64: a fixed-order block tree per
-// tile (deterministic), then a fixed-order per-scene reduction.
+// particle, neighbours read through the padded ELL lists built by grid.hip.
+// What a pass reads from its neighbours is packed into one or two 16-byte
+// records (xk: position + the pass's scalar; v: velocity + density) so that
+// each neighbour costs one or two gathers.  Sums that decide convergence are
+// reduced in FP64: a fixed-order block tree per tile (deterministic), then a
+// fixed-order per-scene reduction.
 #include "sph_common.h"
 
 namespace sphgpu {
@@ -29,6 +32,14 @@ __device__ __forceinline__ int phase_of(float4 x) { return (int)x.w; }
 __device__ __forceinline__ void write_partial(const Dev& D, int tile, int q, double v, double* lds) {
     double t = block_sum_256(v, lds);
     if (threadIdx.x == 0) D.partial[2 * tile + q] = t;
+}
+
+__device__ __forceinline__ int nf_of(int c) { return c & 0xffff; }
+__device__ __forceinline__ int nb_of(int c) { return c >> 16; }
+
+__device__ __forceinline__ bool inside_domain(const SceneGrid& G, float4 x) {
+    return G.smin[0] <= x.x && G.smin[1] <= x.y && G.smin[2] <= x.z && x.x <= G.smax[0] &&
+           x.y <= G.smax[1] && x.z <= G.smax[2];
 }
 
 // ---------------------------------------------------------------- CFL ----
@@ -149,34 +160,66 @@ __global__ __launch_bounds__(256) void k_stats_reduce(Dev D, const double* part,
     }
 }
 
+// ------------------------------------------------------ shared passes ----
+// density m W(0) + sum_f m W + sum_b rho0 V_b W, Kahan-compensated
+// (WcSph.cpp:121-138, DFSPH.cpp:132-169, IISph_M.cpp:101-112); coincident
+// pairs contribute W(0), the padding (self) entries nothing.
+__device__ __forceinline__ float density_sum(const Dev& D, int i, float4 pa, float m, float rho0, int c) {
+    Kahan rho;
+    rho.init(m * D.W0);
+    each_fluid(D, i, nf_of(c), [&](int j) {
+        float4 pb = D.x[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float w = kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
+        rho.add(j != i ? m * w : 0.0f);
+    });
+    each_boundary(D, i, nb_of(c), [&](int j) {
+        float4 pb = D.xb[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        rho.add(rho0 * pb.w * kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D));
+    });
+    return rho.s;
+}
+
+// sum_f m gradW.(v*_a - v*_b) + sum_b rho0 V_b gradW.(v*_a - v_b): the
+// predicted density change rate (DFSPH.cpp:589-617, 795-825)
+__device__ __forceinline__ float density_change(const Dev& D, int i, float4 pa, float4 va, float m,
+                                                float rho0, int c) {
+    float sf = 0.0f, sb = 0.0f;
+    each_fluid(D, i, nf_of(c), [&](int j) {
+        float4 pb = D.x[j];
+        float4 vb = D.vs[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
+        sf += g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
+    });
+    each_boundary(D, i, nb_of(c), [&](int j) {
+        float4 pb = D.xb[j];
+        float4 vb = D.vb[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
+        sb += pb.w * g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
+    });
+    return m * sf + rho0 * sb;
+}
+
 // ============================================================ WCSPH ======
 // WcSph::wc_computePressure (WcSph.cpp:99-153): rho (Kahan), Tait p >= 0.
+// Publishes the force pass's gather records: xk = (x, p/rho^2), v.w = rho.
 __global__ __launch_bounds__(256) void k_wc_pressure(Dev D) {
     Slot s = slot_of(D, false);
     if (!s.live) return;
-    const int i = s.i, stride = D.ncap;
+    const int i = s.i;
     float4 pa = D.x[i];
     int ph = phase_of(pa);
     float m = D.mass[ph], rho0 = D.rho0[ph];
-    Kahan rho;
-    rho.init(m * D.W0);
-    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
-    for (int k = 0; k < nf; ++k) {
-        float4 pb = D.x[D.nbF[k * stride + i]];
-        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
-        rho.add(m * kernel_W(r, D));
-    }
-    for (int k = 0; k < nbn; ++k) {
-        float4 pb = D.xb[D.nbB[k * stride + i]];
-        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
-        rho.add(rho0 * pb.w * kernel_W(r, D));
-    }
-    float density = rho.s;
+    float density = density_sum(D, i, pa, m, rho0, D.cnt[i]);
     float p = D.tait_B[ph] * (powf(density / rho0, D.gamma) - 1.0f);
+    p = p < 0.0f ? 0.0f : p;
     D.rho[i] = density;
-    D.f1[i] = p < 0.0f ? 0.0f : p;
+    D.f1[i] = p;
+    D.xk[i] = make_float4(pa.x, pa.y, pa.z, p / (density * density));
+    D.v[i].w = density;
 }
 
 // WcSph::wc_computeForce (WcSph.cpp:156-312, 364-436) fused with the
@@ -185,78 +228,59 @@ __global__ __launch_bounds__(256) void k_wc_pressure(Dev D) {
 __global__ __launch_bounds__(256) void k_wc_force(Dev D) {
     Slot s = slot_of(D, false);
     if (!s.live) return;
-    const int i = s.i, stride = D.ncap;
+    const int i = s.i;
     float4 pa = D.x[i], va = D.v[i];
     int ph = phase_of(pa);
     float m = D.mass[ph], rho0 = D.rho0[ph], alpha = D.visc[ph];
-    float rhoa = D.rho[i], pra = D.f1[i];
+    float rhoa = va.w, pra = D.f1[i];
     float pa_r2 = pra / (rhoa * rhoa);
     const float Hcs2 = 2.0f * D.Hs * D.cs;
+    const bool multi = D.nphases > 1;
     float ax = D.gx, ay = D.gy, az = D.gz;
-    int zd = 0;
-    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
-    for (int k = 0; k < nf; ++k) {
-        int j = D.nbF[k * stride + i];
-        float4 pb = D.x[j];
+    int c = D.cnt[i];
+    each_fluid(D, i, nf_of(c), [&](int j) {
+        float4 pb = D.xk[j];           // (x, p/rho^2)
+        float4 vb = D.v[j];            // (v, rho)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) { ++zd; continue; }
-        float r = fast_sqrt(r2);
-        float g = kernel_gradfac(r, D);
-        float4 vb = D.v[j];
-        float rhob = D.rho[j], prb = D.f1[j];
+        float g = kernel_gradfac(fast_sqrt(r2), D);
         float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
-        int phb = phase_of(pb);
+        int phb = multi ? phase_of(D.x[j]) : ph;
         float acce;
         if (phb == ph) {
-            acce = g * (-m * (pa_r2 + prb / (rhob * rhob)));
-            if (pro < 0.0f) {
-                float nu = alpha * Hcs2 / (rhoa + rhob);
-                float pi = -nu * pro / (r2 + D.visc_eps);
-                acce += g * (-m * pi);
-            }
+            float nu = alpha * Hcs2 / (rhoa + vb.w);
+            float pi = pro < 0.0f ? -nu * pro * fast_rcp(r2 + D.visc_eps) : 0.0f;
+            acce = g * (-m * (pa_r2 + pb.w + pi));
         } else {                                      /* SP08 multiphase, WcSph.cpp:220-260 */
             float mb = D.mass[phb], al = 0.5f * (alpha + D.visc[phb]);
-            float da = rhoa / m, db = rhob / mb;
-            acce = g * (-1.0f / m * (pra / (da * da) + prb / (db * db)));
-            if (pro < 0.0f) {
-                float nu = al * Hcs2 / (rhoa + rhob);
-                float pi = -nu * pro / (r2 + D.visc_eps);
-                acce += g * (-0.5f * (m + mb) * pi);
-            }
+            // p/delta^2 with delta = rho/m: p m^2 / rho^2
+            acce = g * (-1.0f / m * (pa_r2 * m * m + pb.w * mb * mb));
+            float nu = al * Hcs2 / (rhoa + vb.w);
+            float pi = pro < 0.0f ? -nu * pro * fast_rcp(r2 + D.visc_eps) : 0.0f;
+            acce += g * (-0.5f * (m + mb) * pi);
         }
         ax += dx * acce; ay += dy * acce; az += dz * acce;
-    }
-    for (int k = 0; k < nbn; ++k) {
-        int j = D.nbB[k * stride + i];
+    });
+    each_boundary(D, i, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
+        float4 vb = D.vb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) { ++zd; continue; }
-        float r = fast_sqrt(r2);
-        float g = kernel_gradfac(r, D);
-        float4 vb = D.vb[j];
-        float acce = 0.0f;
-        if (pra > 0.0f) acce += g * (-rho0 * pb.w * (pa_r2 * 2.0f));
+        float g = kernel_gradfac(fast_sqrt(r2), D);
+        float acce = pra > 0.0f ? g * (-rho0 * pb.w * (pa_r2 * 2.0f)) : 0.0f;
         float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
-        if (pro < 0.0f) {
-            float nu = D.solid_visc * Hcs2 / (rhoa * 2.0f);
-            float pi = -nu * pro / (r2 + D.visc_eps);
-            acce += g * (-rho0 * pb.w * pi);
-        }
+        float nu = D.solid_visc * Hcs2 / (rhoa * 2.0f);
+        float pi = pro < 0.0f ? -nu * pro * fast_rcp(r2 + D.visc_eps) : 0.0f;
+        acce += g * (-rho0 * pb.w * pi);
         ax += dx * acce; ay += dy * acce; az += dz * acce;
-    }
-    if (zd) atomicAdd(&D.counters[C_ZERO_DIS], zd);
+    });
     float dt = D.dt[s.scene];
     D.f4a[i] = make_float4(ax, ay, az, 0.0f);
     float4 vn = make_float4(va.x + ax * dt, va.y + ay * dt, va.z + az * dt, 0.0f);
     float4 xn = make_float4(pa.x + vn.x * dt, pa.y + vn.y * dt, pa.z + vn.z * dt, pa.w);
     D.v2[i] = vn;
     D.x2[i] = xn;
-    const SceneGrid& G = D.grids[s.scene];
-    bool inside = G.smin[0] <= xn.x && G.smin[1] <= xn.y && G.smin[2] <= xn.z &&
-                  xn.x <= G.smax[0] && xn.y <= G.smax[1] && xn.z <= G.smax[2];
-    if (!inside) {
+    if (!inside_domain(D.grids[s.scene], xn)) {
         D.id[i] = -1;
         atomicAdd(&D.counters[C_REMOVED], 1);
     }
@@ -267,148 +291,133 @@ __global__ __launch_bounds__(256) void k_wc_force(Dev D) {
 __global__ __launch_bounds__(256) void k_df_density_alpha(Dev D) {
     Slot s = slot_of(D, false);
     if (!s.live) return;
-    const int i = s.i, stride = D.ncap;
+    const int i = s.i;
     float4 pa = D.x[i];
     int ph = phase_of(pa);
     float m = D.mass[ph], rho0 = D.rho0[ph];
     Kahan rho;
     rho.init(m * D.W0);
     float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2 = 0.f;
-    int zd = 0;
-    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
-    for (int k = 0; k < nf; ++k) {
-        float4 pb = D.x[D.nbF[k * stride + i]];
+    int c = D.cnt[i];
+    each_fluid(D, i, nf_of(c), [&](int j) {
+        float4 pb = D.x[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) { ++zd; continue; }
-        float r = fast_sqrt(r2);
-        rho.add(m * kernel_W(r, D));
-        float g = kernel_gradfac(r, D);
-        float gm = g * m;
+        float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
+        rho.add(j != i ? m * kernel_W(r, D) : 0.0f);
+        float gm = kernel_gradfac(r, D) * m;
         a1x += dx * gm; a1y += dy * gm; a1z += dz * gm;
-        float gs = g * r * m;                  // |dW/dr| * m
+        float gs = gm * r;                     // |dW/dr| m (zero at r = 0)
         a2 += gs * gs;
-    }
-    for (int k = 0; k < nbn; ++k) {
-        float4 pb = D.xb[D.nbB[k * stride + i]];
+    });
+    each_boundary(D, i, nb_of(c), [&](int j) {
+        float4 pb = D.xb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) { ++zd; continue; }
-        float r = fast_sqrt(r2);
+        float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
         float psi = rho0 * pb.w;
         rho.add(psi * kernel_W(r, D));
         float gm = kernel_gradfac(r, D) * psi;
         a1x += dx * gm; a1y += dy * gm; a1z += dz * gm;
-    }
-    if (zd) atomicAdd(&D.counters[C_ZERO_DIS], zd);
+    });
     float density = rho.s;
     float den = a1x * a1x + a1y * a1y + a1z * a1z + a2;
     if (den < 1.0e-6f) den = 1.0e-6f;
     D.rho[i] = density;
     D.alpha[i] = density / den;
+    D.v[i].w = density;
 }
 
 // DFSPH::DF_forceExceptPressure (DFSPH.cpp:1419-1475; helpers 1246-1359) and
-// v* = v + dt a_adv (DFSPH.cpp:557-563).  With a warm start, also the
-// initial kappa/rho of every particle (kr0) for the first correction.
-__global__ __launch_bounds__(256) void k_df_aadv(Dev D, int warm) {
+// v* = v + dt a_adv (DFSPH.cpp:557-563).  The density loop starts from zero
+// kappa: it only pushes (rho* >= rho0, DFSPH.cpp:620-623), so a warm guess
+// that overshoots would never be taken back.
+__global__ __launch_bounds__(256) void k_df_aadv(Dev D) {
     Slot s = slot_of(D, false);
     if (!s.live) return;
-    const int i = s.i, stride = D.ncap;
+    const int i = s.i;
     float4 pa = D.x[i], va = D.v[i];
     int ph = phase_of(pa);
     float m = D.mass[ph], rho0 = D.rho0[ph], alpha = D.visc[ph];
-    float rhoa = D.rho[i];
+    float rhoa = va.w;
     const float nu10 = 10.0f * D.nu;
     const float Hcs2 = 2.0f * D.Hs * D.cs;
+    const bool multi = D.nphases > 1;
     float ax = D.gx, ay = D.gy, az = D.gz;
-    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
-    for (int k = 0; k < nf; ++k) {
-        int j = D.nbF[k * stride + i];
+    int c = D.cnt[i];
+    each_fluid(D, i, nf_of(c), [&](int j) {
         float4 pb = D.x[j];
+        float4 vb = D.v[j];            // (v, rho)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) continue;
-        float4 vb = D.v[j];
         float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
-        if (pro >= 0.0f) continue;
-        float r = fast_sqrt(r2);
-        float g = kernel_gradfac(r, D);
-        int phb = phase_of(pb);
+        float g = kernel_gradfac(fast_sqrt(r2), D);
+        int phb = multi ? phase_of(pb) : ph;
         float f;
         if (phb == ph) {
-            f = g * (pro / (r2 + D.visc_eps) * m / D.rho[j] * nu10);
+            f = g * (pro * fast_rcp(r2 + D.visc_eps) * m * fast_rcp(vb.w) * nu10);
         } else {                                         /* DFSPH.cpp:1278-1303 */
             float mb = D.mass[phb], al = 0.5f * (alpha + D.visc[phb]);
-            float nuv = al * Hcs2 / (rhoa + D.rho[j]);
-            float pi = -nuv * pro / (r2 + D.visc_eps);
-            f = g * (-0.5f * (m + mb) * pi);
+            float nuv = al * Hcs2 / (rhoa + vb.w);
+            f = g * (-0.5f * (m + mb) * (-nuv * pro * fast_rcp(r2 + D.visc_eps)));
         }
+        f = pro < 0.0f ? f : 0.0f;
         ax += dx * f; ay += dy * f; az += dz * f;
-    }
-    for (int k = 0; k < nbn; ++k) {
-        int j = D.nbB[k * stride + i];
+    });
+    each_boundary(D, i, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
+        float4 vb = D.vb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) continue;
-        float4 vb = D.vb[j];
         float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
-        if (pro >= 0.0f) continue;
-        float r = fast_sqrt(r2);
-        float g = kernel_gradfac(r, D);
-        float f = g * (pro / (r2 + D.visc_eps) * (rho0 * pb.w) / rhoa * nu10);
+        float g = kernel_gradfac(fast_sqrt(r2), D);
+        float f = g * (pro * fast_rcp(r2 + D.visc_eps) * (rho0 * pb.w) / rhoa * nu10);
+        f = pro < 0.0f ? f : 0.0f;
         ax += dx * f; ay += dy * f; az += dz * f;
-    }
+    });
     float dt = D.dt[s.scene];
     D.f4a[i] = make_float4(ax, ay, az, 0.0f);
     D.vs[i] = make_float4(va.x + ax * dt, va.y + ay * dt, va.z + az * dt, 0.0f);
-    if (warm) {
-        D.kr0[i] = D.ksum[i] / (dt * dt) / rhoa;
-    } else {
-        D.ksum[i] = 0.0f;
+    D.ksum[i] = 0.0f;
+}
+
+// warm start of the divergence loop (Bender & Koschier; not in the reference):
+// half of the previous step's accumulated kappaV, only where the current
+// velocities still compress (Drho/Dt > 0); xk.w = kappaV0/rho feeds one
+// velocity correction, and the accumulator restarts from the applied guess.
+// Without warm start the accumulator restarts at zero.
+__global__ __launch_bounds__(256) void k_df_warm_div(Dev D, int warm) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i;
+    if (!warm) {
+        D.kvsum[i] = 0.0f;
+        return;
     }
+    float4 pa = D.x[i], va = D.vs[i];
+    int ph = phase_of(pa);
+    float d = density_change(D, i, pa, va, D.mass[ph], D.rho0[ph], D.cnt[i]);
+    float kv = d > 0.0f ? 0.5f * D.kvsum[i] : 0.0f;
+    D.kvsum[i] = kv;
+    D.xk[i] = make_float4(pa.x, pa.y, pa.z, kv / D.dt[s.scene] / D.rho[i]);
 }
 
 // pass A of the density loop (DFSPH.cpp:572-643): rho*, clamp rho* >= rho0,
-// kappa = (rho*-rho0)/dt^2 alpha -> kr = kappa/rho; partial sum of rho*/rho0.
+// kappa = (rho*-rho0)/dt^2 alpha -> xk.w = kappa/rho; partial sum of rho*/rho0.
 __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
     __shared__ double lds[4];
     Slot s = slot_of(D, true);
     double contrib = 0.0;
     if (s.live) {
-        const int i = s.i, stride = D.ncap;
+        const int i = s.i;
         float4 pa = D.x[i], va = D.vs[i];
         int ph = phase_of(pa);
         float m = D.mass[ph], rho0 = D.rho0[ph];
-        float sf = 0.0f, sb = 0.0f;
-        int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
-        for (int k = 0; k < nf; ++k) {
-            int j = D.nbF[k * stride + i];
-            float4 pb = D.x[j];
-            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 == 0.0f) continue;
-            float4 vb = D.vs[j];
-            float g = kernel_gradfac(fast_sqrt(r2), D);
-            sf += g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
-        }
-        for (int k = 0; k < nbn; ++k) {
-            int j = D.nbB[k * stride + i];
-            float4 pb = D.xb[j];
-            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 == 0.0f) continue;
-            float4 vb = D.vb[j];
-            float g = kernel_gradfac(fast_sqrt(r2), D);
-            sb += pb.w * g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
-        }
+        float dd = density_change(D, i, pa, va, m, rho0, D.cnt[i]);
         float dt = D.dt[s.scene];
         float rhoa = D.rho[i];
-        float rs = rhoa + dt * (m * sf + rho0 * sb);
+        float rs = rhoa + dt * dd;
         if (rs < rho0) rs = rho0;
         float dev = (rs - rho0) * D.alpha[i];     // kappa * dt^2
-        D.kr[i] = dev / (dt * dt) / rhoa;
+        D.xk[i] = make_float4(pa.x, pa.y, pa.z, dev / (dt * dt) / rhoa);
         D.ksum[i] += dev;
         contrib = (double)rs / (double)rho0;
     }
@@ -416,44 +425,22 @@ __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
 }
 
 // pass 1 of the divergence loop (DFSPH.cpp:780-836): Drho/Dt, clamp,
-// kappaV = Drho/Dt alpha / dt -> kr = kappaV/rho; partial sum of Drho/Dt.
+// kappaV = Drho/Dt alpha / dt -> xk.w = kappaV/rho; partial sum of Drho/Dt.
 __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D) {
     __shared__ double lds[4];
     Slot s = slot_of(D, true);
     double contrib = 0.0;
     if (s.live) {
-        const int i = s.i, stride = D.ncap;
+        const int i = s.i;
         float4 pa = D.x[i], va = D.vs[i];
         int ph = phase_of(pa);
         float m = D.mass[ph], rho0 = D.rho0[ph];
-        float sf = 0.0f, sb = 0.0f;
-        int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
-        for (int k = 0; k < nf; ++k) {
-            int j = D.nbF[k * stride + i];
-            float4 pb = D.x[j];
-            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 == 0.0f) continue;
-            float4 vb = D.vs[j];
-            float g = kernel_gradfac(fast_sqrt(r2), D);
-            sf += g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
-        }
-        for (int k = 0; k < nbn; ++k) {
-            int j = D.nbB[k * stride + i];
-            float4 pb = D.xb[j];
-            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 == 0.0f) continue;
-            float4 vb = D.vb[j];
-            float g = kernel_gradfac(fast_sqrt(r2), D);
-            sb += pb.w * g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
-        }
+        float dd = density_change(D, i, pa, va, m, rho0, D.cnt[i]);
         float dt = D.dt[s.scene];
         float rhoa = D.rho[i];
-        float dd = m * sf + rho0 * sb;
         if (dd < 0.0f || (rhoa + dd * dt) < rho0) dd = 0.0f;
         float kv = dd * D.alpha[i] / dt;
-        D.kr[i] = kv / rhoa;
+        D.xk[i] = make_float4(pa.x, pa.y, pa.z, kv / rhoa);
         D.kvsum[i] += kv * dt;
         contrib = (double)dd;
     }
@@ -461,34 +448,29 @@ __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D) {
 }
 
 // pass B of both loops (DFSPH.cpp:647-693, 840-883):
-// v* -= dt (m sum (kr_a + kr_b) gradW + rho0 sum V_b kr_a gradW)
-__global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D, const float* __restrict__ kr) {
+// v* -= dt (m sum (kr_a + kr_b) gradW + rho0 sum V_b kr_a gradW), kr = xk.w
+__global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
     Slot s = slot_of(D, true);
     if (!s.live) return;
-    const int i = s.i, stride = D.ncap;
-    float4 pa = D.x[i];
-    int ph = phase_of(pa);
+    const int i = s.i;
+    float4 pa = D.xk[i];
+    int ph = phase_of(D.x[i]);
     float m = D.mass[ph], rho0 = D.rho0[ph];
-    float ka = kr[i];
+    float ka = pa.w;
     float fx = 0.f, fy = 0.f, fz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
-    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
-    for (int k = 0; k < nf; ++k) {
-        int j = D.nbF[k * stride + i];
-        float4 pb = D.x[j];
+    int c = D.cnt[i];
+    each_fluid(D, i, nf_of(c), [&](int j) {
+        float4 pb = D.xk[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) continue;
-        float g = kernel_gradfac(fast_sqrt(r2), D) * (ka + kr[j]);
+        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * (ka + pb.w);
         fx += dx * g; fy += dy * g; fz += dz * g;
-    }
-    for (int k = 0; k < nbn; ++k) {
-        float4 pb = D.xb[D.nbB[k * stride + i]];
+    });
+    each_boundary(D, i, nb_of(c), [&](int j) {
+        float4 pb = D.xb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) continue;
-        float g = kernel_gradfac(fast_sqrt(r2), D) * pb.w;
+        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
         bx += dx * g; by += dy * g; bz += dz * g;
-    }
+    });
     float dt = D.dt[s.scene];
     float cf = dt * m, cb = dt * rho0 * ka;
     float4 v = D.vs[i];
@@ -509,10 +491,7 @@ __global__ __launch_bounds__(256) void k_df_integrate(Dev D) {
     x.x += v.x * dt; x.y += v.y * dt; x.z += v.z * dt;
     D.v[i] = v;
     D.x[i] = x;
-    const SceneGrid& G = D.grids[s.scene];
-    bool inside = G.smin[0] <= x.x && G.smin[1] <= x.y && G.smin[2] <= x.z &&
-                  x.x <= G.smax[0] && x.y <= G.smax[1] && x.z <= G.smax[2];
-    if (!inside) {
+    if (!inside_domain(D.grids[s.scene], x)) {
         D.id[i] = -1;
         atomicAdd(&D.counters[C_REMOVED], 1);
     }
@@ -520,92 +499,75 @@ __global__ __launch_bounds__(256) void k_df_integrate(Dev D) {
 
 // ============================================================ IISPH ======
 // density (IISph_M.cpp:101-112), computed before a_adv so that a_adv reads
-// this step's densities.
+// this step's densities; published as v.w for the a_adv gathers.
 __global__ __launch_bounds__(256) void k_ii_density(Dev D) {
     Slot s = slot_of(D, false);
     if (!s.live) return;
-    const int i = s.i, stride = D.ncap;
+    const int i = s.i;
     float4 pa = D.x[i];
     int ph = phase_of(pa);
-    float m = D.mass[ph], rho0 = D.rho0[ph];
-    Kahan rho;
-    rho.init(m * D.W0);
-    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
-    for (int k = 0; k < nf; ++k) {
-        float4 pb = D.x[D.nbF[k * stride + i]];
-        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        rho.add(m * kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D));
-    }
-    for (int k = 0; k < nbn; ++k) {
-        float4 pb = D.xb[D.nbB[k * stride + i]];
-        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        rho.add(rho0 * pb.w * kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D));
-    }
-    D.rho[i] = rho.s;
+    float density = density_sum(D, i, pa, D.mass[ph], D.rho0[ph], D.cnt[i]);
+    D.rho[i] = density;
+    D.v[i].w = density;
 }
 
 // a_adv (IISph_M.cpp:364-476), d_ii (114-130) and v_adv (135)
 __global__ __launch_bounds__(256) void k_ii_aadv_dii(Dev D) {
     Slot s = slot_of(D, false);
     if (!s.live) return;
-    const int i = s.i, stride = D.ncap;
+    const int i = s.i;
     float4 pa = D.x[i], va = D.v[i];
     int ph = phase_of(pa);
     float m = D.mass[ph], rho0 = D.rho0[ph], alpha = D.visc[ph];
-    float rhoa = D.rho[i];
+    float rhoa = va.w;
     const float nu10 = 10.0f * D.nu, Hcs2 = 2.0f * D.Hs * D.cs;
+    const bool multi = D.nphases > 1;
     float ax = D.gx, ay = D.gy, az = D.gz;
     float dfx = 0.f, dfy = 0.f, dfz = 0.f, dbx = 0.f, dby = 0.f, dbz = 0.f;
-    int zd = 0;
-    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
-    for (int k = 0; k < nf; ++k) {
-        int j = D.nbF[k * stride + i];
+    int c = D.cnt[i];
+    each_fluid(D, i, nf_of(c), [&](int j) {
         float4 pb = D.x[j];
+        float4 vb = D.v[j];            // (v, rho)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) { ++zd; continue; }
-        float4 vb = D.v[j];
         float g = kernel_gradfac(fast_sqrt(r2), D);
         float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
-        int phb = phase_of(pb);
-        float f = 0.0f;
+        int phb = multi ? phase_of(pb) : ph;
+        float f;
         if (phb == ph) {
-            f = g * (pro / (r2 + D.visc_eps) * m / D.rho[j] * nu10);   // no pro<0 gate (IISph_M.cpp:377)
-        } else if (pro < 0.0f) {
+            f = g * (pro * fast_rcp(r2 + D.visc_eps) * m * fast_rcp(vb.w) * nu10);   // no pro<0 gate (IISph_M.cpp:377)
+        } else {
             float mb = D.mass[phb], al = 0.5f * (alpha + D.visc[phb]);
-            float nuv = al * Hcs2 / (rhoa + D.rho[j]);
-            float pi = -nuv * pro / (r2 + D.visc_eps);
-            f = g * (-0.5f * (m + mb) * pi);
+            float nuv = al * Hcs2 / (rhoa + vb.w);
+            f = pro < 0.0f ? g * (-0.5f * (m + mb) * (-nuv * pro * fast_rcp(r2 + D.visc_eps))) : 0.0f;
         }
         ax += dx * f; ay += dy * f; az += dz * f;
         dfx += dx * g; dfy += dy * g; dfz += dz * g;
-    }
-    for (int k = 0; k < nbn; ++k) {
-        int j = D.nbB[k * stride + i];
+    });
+    each_boundary(D, i, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
+        float4 vb = D.vb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) { ++zd; continue; }
-        float4 vb = D.vb[j];
         float g = kernel_gradfac(fast_sqrt(r2), D);
         float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
-        float f = g * (pro / (r2 + D.visc_eps) * (rho0 * pb.w) / rhoa * nu10);
+        float f = g * (pro * fast_rcp(r2 + D.visc_eps) * (rho0 * pb.w) / rhoa * nu10);
         ax += dx * f; ay += dy * f; az += dz * f;
         float gw = g * pb.w;
         dbx += dx * gw; dby += dy * gw; dbz += dz * gw;
-    }
-    if (zd) atomicAdd(&D.counters[C_ZERO_DIS], zd);
+    });
     float dt = D.dt[s.scene];
     float cf = -m / (rhoa * rhoa) * dt * dt, cb = -rho0 / (rhoa * rhoa) * dt * dt;
     D.f4a[i] = make_float4(cf * dfx + cb * dbx, cf * dfy + cb * dby, cf * dfz + cb * dbz, 0.0f);
     D.vs[i] = make_float4(va.x + ax * dt, va.y + ay * dt, va.z + az * dt, 0.0f);
 }
 
-// a_ii, rho_adv, p_l = 0.5 p(t-dt) (IISph_M.cpp:243-278)
+// a_ii, rho_adv, p_l = 0.5 p(t-dt) (IISph_M.cpp:243-278); publishes
+// xk = (x, p_l/rho^2) for the sum_j d_ij p_j gathers.
 __global__ __launch_bounds__(256) void k_ii_aii(Dev D) {
     Slot s = slot_of(D, false);
     if (!s.live) return;
-    const int i = s.i, stride = D.ncap;
+    const int i = s.i;
     float4 pa = D.x[i], va = D.vs[i], dii = D.f4a[i];
     int ph = phase_of(pa);
     float m = D.mass[ph], rho0 = D.rho0[ph];
@@ -613,104 +575,94 @@ __global__ __launch_bounds__(256) void k_ii_aii(Dev D) {
     float dt = D.dt[s.scene];
     float cji = m * dt * dt / (rhoa * rhoa);
     float t1f = 0.f, t1b = 0.f, t2f = 0.f, t2b = 0.f;
-    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
-    for (int k = 0; k < nf; ++k) {
-        int j = D.nbF[k * stride + i];
+    int c = D.cnt[i];
+    each_fluid(D, i, nf_of(c), [&](int j) {
         float4 pb = D.x[j];
-        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) continue;
         float4 vb = D.vs[j];
-        float g = kernel_gradfac(fast_sqrt(r2), D);
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
         float gx = dx * g, gy = dy * g, gz = dz * g;
         t1f += gx * (va.x - vb.x) + gy * (va.y - vb.y) + gz * (va.z - vb.z);
         t2f += (dii.x - cji * gx) * gx + (dii.y - cji * gy) * gy + (dii.z - cji * gz) * gz;
-    }
-    for (int k = 0; k < nbn; ++k) {
-        int j = D.nbB[k * stride + i];
+    });
+    each_boundary(D, i, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
-        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) continue;
         float4 vb = D.vb[j];
-        float g = kernel_gradfac(fast_sqrt(r2), D) * pb.w;
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
         float gx = dx * g, gy = dy * g, gz = dz * g;
         t1b += gx * (va.x - vb.x) + gy * (va.y - vb.y) + gz * (va.z - vb.z);   // v_adv (IISph_M.cpp:269 uses v)
         t2b += dii.x * gx + dii.y * gy + dii.z * gz;
-    }
+    });
     float aii = m * t2f + rho0 * t2b;
     if (aii == 0.0f) aii = 1.0f;
     D.alpha[i] = aii;
     D.kr0[i] = rhoa + dt * (m * t1f + rho0 * t1b);
-    D.kr[i] = 0.5f * D.ksum[i];
+    float pl = 0.5f * D.ksum[i];
+    D.kr[i] = pl;
+    D.xk[i] = make_float4(pa.x, pa.y, pa.z, pl / (rhoa * rhoa));
 }
 
-// sum_j d_ij p_j = dt^2 sum_f -m p_j/rho_j^2 gradW (IISph_M.cpp:289-309)
+// sum_j d_ij p_j = dt^2 sum_f -m p_j/rho_j^2 gradW (IISph_M.cpp:289-309);
+// stores u = d_ii p_l + sum_j d_ij p_j, the per-particle part of the
+// Jacobi update that neighbours read.
 __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
     Slot s = slot_of(D, true);
     if (!s.live) return;
-    const int i = s.i, stride = D.ncap;
+    const int i = s.i;
     float4 pa = D.x[i];
     int ph = phase_of(pa);
     float m = D.mass[ph];
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    int c = D.cnt[i], nf = c & 0xffff;
-    for (int k = 0; k < nf; ++k) {
-        int j = D.nbF[k * stride + i];
-        float4 pb = D.x[j];
+    each_fluid(D, i, nf_of(D.cnt[i]), [&](int j) {
+        float4 pb = D.xk[j];           // (x, p_l/rho^2)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) continue;
-        float rb = D.rho[j];
-        float g = kernel_gradfac(fast_sqrt(r2), D) * (D.kr[j] / (rb * rb));
+        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
         sx += dx * g; sy += dy * g; sz += dz * g;
-    }
+    });
     float dt = D.dt[s.scene];
     float cf = -m * dt * dt;
-    D.f4b[i] = make_float4(cf * sx, cf * sy, cf * sz, 0.0f);
+    float4 dii = D.f4a[i];
+    float pl = D.kr[i];
+    D.f4b[i] = make_float4(dii.x * pl + cf * sx, dii.y * pl + cf * sy, dii.z * pl + cf * sz, 0.0f);
 }
 
-// relaxed Jacobi update (IISph_M.cpp:312-351): writes the new p_l into f1,
-// partial sum of the (clamped-excluded) relative density error.
+// relaxed Jacobi update (IISph_M.cpp:312-351) into f1; partial sum of the
+// relative density error of the particles that are not clamped.
 __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
     __shared__ double lds[4];
-    Slot s = slot_of(D, false);
+    Slot s = slot_of(D, true);
     double contrib = 0.0;
-    if (s.live && !D.active[s.scene]) {
-        D.f1[s.i] = D.kr[s.i];          // converged scene: keep p_l across the buffer swap
-    } else if (s.live) {
-        const int i = s.i, stride = D.ncap;
-        float4 pa = D.x[i], sa = D.f4b[i];
+    if (s.live) {
+        const int i = s.i;
+        float4 pa = D.x[i], ua = D.f4b[i], dia = D.f4a[i];
         int ph = phase_of(pa);
         float m = D.mass[ph], rho0 = D.rho0[ph];
         float rhoa = D.rho[i], pla = D.kr[i];
         float dt = D.dt[s.scene];
         float cji = dt * dt * m / (rhoa * rhoa);
+        // own sum_j d_ij p_j
+        float sax = ua.x - dia.x * pla, say = ua.y - dia.y * pla, saz = ua.z - dia.z * pla;
         float ff = 0.f, fb = 0.f;
-        int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
-        for (int k = 0; k < nf; ++k) {
-            int j = D.nbF[k * stride + i];
+        int c = D.cnt[i];
+        each_fluid(D, i, nf_of(c), [&](int j) {
             float4 pb = D.x[j];
+            float4 ub = D.f4b[j];
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 == 0.0f) continue;
-            float g = kernel_gradfac(fast_sqrt(r2), D);
+            float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
             float gx = dx * g, gy = dy * g, gz = dz * g;
-            float4 db = D.f4a[j], sb = D.f4b[j];
-            float plb = D.kr[j];
-            float tx = (sa.x - db.x * plb) - (sb.x - cji * gx * pla);
-            float ty = (sa.y - db.y * plb) - (sb.y - cji * gy * pla);
-            float tz = (sa.z - db.z * plb) - (sb.z - cji * gz * pla);
+            // (sum d_ij p_j|a - d_jj p_j) - (sum d_jk p_k|b - d_ji p_a)
+            float tx = sax - ub.x + cji * gx * pla;
+            float ty = say - ub.y + cji * gy * pla;
+            float tz = saz - ub.z + cji * gz * pla;
             ff += tx * gx + ty * gy + tz * gz;
-        }
-        for (int k = 0; k < nbn; ++k) {
-            float4 pb = D.xb[D.nbB[k * stride + i]];
+        });
+        each_boundary(D, i, nb_of(c), [&](int j) {
+            float4 pb = D.xb[j];
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 == 0.0f) continue;
-            float g = kernel_gradfac(fast_sqrt(r2), D) * pb.w;
-            fb += (sa.x * dx + sa.y * dy + sa.z * dz) * g;
-        }
+            float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
+            fb += (sax * dx + say * dy + saz * dz) * g;
+        });
         float fin = m * ff + rho0 * fb;
         float aii = D.alpha[i], radv = D.kr0[i];
         float t = (1.0f - D.omega) * pla + (D.omega / aii) * (rho0 - radv - fin);
@@ -724,57 +676,61 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
     write_partial(D, s.tile, 0, contrib, lds);
 }
 
+// publish the new p_l of the Jacobi sweep (own value and the gather record)
+__global__ __launch_bounds__(256) void k_ii_publish(Dev D) {
+    Slot s = slot_of(D, true);
+    if (!s.live) return;
+    const int i = s.i;
+    float t = D.f1[i], r = D.rho[i];
+    D.kr[i] = t;
+    D.xk[i].w = t / (r * r);
+}
+
 // IISph_M::ii_computeForcePressure (IISph_M.cpp:478-684) fused with the
 // update v = v_adv + dt a_p, x += dt v and domain removal.
 __global__ __launch_bounds__(256) void k_ii_force(Dev D) {
     Slot s = slot_of(D, false);
     if (!s.live) return;
-    const int i = s.i, stride = D.ncap;
-    float4 pa = D.x[i], va = D.v[i];
+    const int i = s.i;
+    float4 pa = D.x[i];
     int ph = phase_of(pa);
     float m = D.mass[ph], rho0 = D.rho0[ph], alpha = D.visc[ph];
     float rhoa = D.rho[i], pra = D.kr[i];
     float pa_r2 = pra / (rhoa * rhoa);
     const float Hcs2 = 2.0f * D.Hs * D.cs;
+    const bool multi = D.nphases > 1;
     float ax = 0.f, ay = 0.f, az = 0.f;
-    int c = D.cnt[i], nf = c & 0xffff, nbn = c >> 16;
-    for (int k = 0; k < nf; ++k) {
-        int j = D.nbF[k * stride + i];
-        float4 pb = D.x[j];
+    int c = D.cnt[i];
+    each_fluid(D, i, nf_of(c), [&](int j) {
+        float4 pb = D.xk[j];           // (x, p/rho^2)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 == 0.0f) continue;
-        float r = fast_sqrt(r2);
-        float g = kernel_gradfac(r, D);
-        float rhob = D.rho[j], prb = D.kr[j];
-        int phb = phase_of(pb);
+        float g = kernel_gradfac(fast_sqrt(r2), D);
+        int phb = multi ? phase_of(D.x[j]) : ph;
         float acce;
         if (phb == ph) {
-            acce = g * (-m * (pa_r2 + prb / (rhob * rhob)));
+            acce = g * (-m * (pa_r2 + pb.w));
         } else {
-            float4 vb = D.v[j];
+            float4 va = D.v[i], vb = D.v[j];
             float mb = D.mass[phb], al = 0.5f * (alpha + D.visc[phb]);
-            float da = rhoa / m, db = rhob / mb;
-            acce = g * (-1.0f / m * (pra / (da * da) + prb / (db * db)));
+            acce = g * (-1.0f / m * (pa_r2 * m * m + pb.w * mb * mb));
             float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
-            if (pro < 0.0f) {
-                float nuv = al * Hcs2 / (rhoa + rhob);
-                float pi = -nuv * pro / (r2 + D.visc_eps);
-                acce += g * (-0.5f * (m + mb) * pi);
-            }
+            float nuv = al * Hcs2 / (rhoa + vb.w);
+            float pi = pro < 0.0f ? -nuv * pro * fast_rcp(r2 + D.visc_eps) : 0.0f;
+            acce += g * (-0.5f * (m + mb) * pi);
         }
         ax += dx * acce; ay += dy * acce; az += dz * acce;
-    }
+    });
     if (pra > 0.0f) {
-        for (int k = 0; k < nbn; ++k) {
-            float4 pb = D.xb[D.nbB[k * stride + i]];
+        each_boundary(D, i, nb_of(c), [&](int j) {
+            float4 pb = D.xb[j];
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 == 0.0f) continue;
-            float g = kernel_gradfac(fast_sqrt(r2), D);
-            float acce = g * (-rho0 * pb.w * (pa_r2 * 2.0f));
+            float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
+            // p_a/rho_a^2: the boundary term the pressure solve predicts
+            // (IISph_M.cpp:127, 271, 338), not the 2 p_a/rho_a^2 of IISph_M.cpp:558-559
+            float acce = g * (-rho0 * pb.w * pa_r2);
             ax += dx * acce; ay += dy * acce; az += dz * acce;
-        }
+        });
     }
     float dt = D.dt[s.scene];
     float4 vadv = D.vs[i];
@@ -784,10 +740,7 @@ __global__ __launch_bounds__(256) void k_ii_force(Dev D) {
     D.v2[i] = vn;
     D.x2[i] = xn;
     D.ksum[i] = pra;
-    const SceneGrid& G = D.grids[s.scene];
-    bool inside = G.smin[0] <= xn.x && G.smin[1] <= xn.y && G.smin[2] <= xn.z &&
-                  xn.x <= G.smax[0] && xn.y <= G.smax[1] && xn.z <= G.smax[2];
-    if (!inside) {
+    if (!inside_domain(D.grids[s.scene], xn)) {
         D.id[i] = -1;
         atomicAdd(&D.counters[C_REMOVED], 1);
     }
@@ -826,16 +779,20 @@ void launch_fill_i32(int* p, int v, int n, hipStream_t st) {
 void launch_wc_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_wc_pressure); }
 void launch_wc_force_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_wc_force); }
 void launch_df_density_alpha(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_density_alpha); }
-void launch_df_aadv(const Dev& D, int warm, hipStream_t st) { SPH_LAUNCH(k_df_aadv, warm); }
+void launch_df_aadv(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_aadv); }
+void launch_df_warm_div(const Dev& D, int warm, hipStream_t st) { SPH_LAUNCH(k_df_warm_div, warm); }
 void launch_df_pass_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_pass_density); }
 void launch_df_pass_divergence(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_pass_divergence); }
-void launch_df_pass_velocity(const Dev& D, const float* kr, hipStream_t st) { SPH_LAUNCH(k_df_pass_velocity, kr); }
+void launch_df_pass_velocity(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_pass_velocity); }
 void launch_df_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_integrate); }
 void launch_ii_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_density); }
 void launch_ii_aadv_dii(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_aadv_dii); }
 void launch_ii_aii(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_aii); }
 void launch_ii_sumdij(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_sumdij); }
-void launch_ii_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_pressure); }
+void launch_ii_pressure(const Dev& D, hipStream_t st) {
+    SPH_LAUNCH(k_ii_pressure);
+    SPH_LAUNCH(k_ii_publish);
+}
 void launch_ii_force_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_force); }
 
 }  // namespace sphgpu

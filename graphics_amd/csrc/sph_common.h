@@ -61,7 +61,7 @@ struct Dev {
 
     // ---- fluid state (sorted, padded slots)
     float4* x;         // xyz, w = phase
-    float4* v;
+    float4* v;         // xyz, w = density of the last density pass (gathered with v)
     int* id;           // original id, -1 for padding / removed
     float* ksum;       // DFSPH accumulated kappa*dt^2; IISPH pressure (warm start)
     float* kvsum;      // DFSPH accumulated kappaV*dt
@@ -75,9 +75,11 @@ struct Dev {
     uint32_t* key2;
 
     // ---- per-step fields
+    float4* xk;        // packed gather record: xyz + per-pass scalar of the neighbour
+                       // (DFSPH kappa/rho, WCSPH p/rho^2, IISPH p_l/rho^2)
     float4* vs;        // DFSPH v*, IISPH v_adv
     float4* f4a;       // WCSPH acceleration, IISPH d_ii, DFSPH a_adv
-    float4* f4b;       // IISPH sum_j d_ij p_j
+    float4* f4b;       // IISPH u = d_ii p_l + sum_j d_ij p_j; pressure acceleration
     float* rho;
     float* alpha;      // DFSPH alpha; IISPH a_ii
     float* kr;         // kappa/rho (pass B input); IISPH p_l
@@ -165,19 +167,24 @@ __device__ __forceinline__ float kernel_W(float r, const Dev& D) {
     return q <= 2.0f ? w : 0.0f;
 }
 
+// Hardware square root and reciprocal (v_sqrt_f32 / v_rcp_f32, 1 ulp) for
+// the solver passes, whose bar is 1e-5 relative; the neighbour predicate and
+// the test dump keep correctly rounded arithmetic.
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // -|dW/dr| / r  (SphBase.h:890-906 divided by r), the scalar that turns x_ab
 // into grad W_ab (DFSPH.cpp:156-157).  For q <= 1 the 1/r cancels:
 // -(sigma_h/h)(3 - 2.25 q); for 1 < q <= 2: -0.75 sigma_h (2-q)^2 / r.
+// At r = 0 the q <= 1 branch is selected (finite).
 __device__ __forceinline__ float kernel_gradfac(float r, const Dev& D) {
     float q = r * D.inv_kh;
     float qm2 = 2.0f - q;
     float g1 = -(D.sigma_h * D.inv_kh) * (3.0f - 2.25f * q);
-    float g2 = -0.75f * D.sigma_h * qm2 * qm2 / r;
+    float g2 = -0.75f * D.sigma_h * qm2 * qm2 * fast_rcp(r);
     float g = q <= 1.0f ? g1 : g2;
     return q <= 2.0f ? g : 0.0f;
 }
-
-__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_sqrtf(x); }
 
 // Kahan-compensated FP32 accumulator.
 struct Kahan {
@@ -224,6 +231,41 @@ __device__ __forceinline__ int scene_of_key(uint32_t key, const uint32_t* base, 
     return lo;
 }
 
+// Neighbour lists are padded to a multiple of 4 entries: fluid lists with the
+// particle itself (distance 0: no gradient contribution; densities skip it by
+// index), boundary lists with the dummy boundary particle D.nb placed far
+// away (W = gradW = 0).  The loops below read four indices, then issue the
+// four gathers before using them.
+template <class F>
+__device__ __forceinline__ void each_fluid(const Dev& D, int i, int nf, F&& f) {
+    const int* __restrict__ col = D.nbF + i;
+    const int st = D.ncap;
+    const int n4 = (nf + 3) & ~3;
+    for (int k = 0; k < n4; k += 4) {
+        int j0 = col[(k + 0) * st], j1 = col[(k + 1) * st], j2 = col[(k + 2) * st], j3 = col[(k + 3) * st];
+        f(j0);
+        f(j1);
+        f(j2);
+        f(j3);
+    }
+}
+
+template <class F>
+__device__ __forceinline__ void each_boundary(const Dev& D, int i, int nb, F&& f) {
+    const int* __restrict__ col = D.nbB + i;
+    const int st = D.ncap;
+    const int n4 = (nb + 3) & ~3;
+    for (int k = 0; k < n4; k += 4) {
+        int j0 = col[(k + 0) * st], j1 = col[(k + 1) * st], j2 = col[(k + 2) * st], j3 = col[(k + 3) * st];
+        f(j0);
+        f(j1);
+        f(j2);
+        f(j3);
+    }
+}
+
+constexpr float DUMMY_FAR = 1.0e15f;   // coordinate of the dummy boundary particle
+
 // ---------------------------------------------------------------------------
 // host-side launchers implemented in the .hip files
 
@@ -268,10 +310,12 @@ void launch_cfl(const Dev& D, float cfl, float spacing, float dtMax, float dtMin
 void launch_wc_pressure(const Dev& D, hipStream_t st);
 void launch_wc_force_integrate(const Dev& D, hipStream_t st);
 void launch_df_density_alpha(const Dev& D, hipStream_t st);
-void launch_df_aadv(const Dev& D, int warm, hipStream_t st);
+void launch_df_aadv(const Dev& D, hipStream_t st);
 void launch_df_pass_density(const Dev& D, hipStream_t st);
+void launch_df_warm_div(const Dev& D, int warm, hipStream_t st);
 void launch_df_pass_divergence(const Dev& D, hipStream_t st);
-void launch_df_pass_velocity(const Dev& D, const float* krSrc, hipStream_t st);
+void launch_df_pass_velocity(const Dev& D, hipStream_t st);
+void launch_df_warm_apply(const Dev& D, hipStream_t st);
 void launch_df_integrate(const Dev& D, hipStream_t st);
 void launch_ii_density(const Dev& D, hipStream_t st);
 void launch_ii_aadv_dii(const Dev& D, hipStream_t st);

@@ -175,7 +175,8 @@ class Oracle:
                                       int(dynamic), visc)
 
     def init(self):
-        self.lib.orc_init(self.h)
+        if self.lib.orc_init(self.h) != 0:
+            raise ValueError("a particle lies outside [space_min, space_max]")
 
     def compute_density(self):
         self.lib.orc_compute_density(self.h)

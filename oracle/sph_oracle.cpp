@@ -20,8 +20,13 @@
  *     boundary rho_adv uses v_adv, the warm start p_l = 0.5 p(t-dt) is kept,
  *     a_adv is integrated, a_adv reads this step's densities
  *     (IISph_M.cpp:81, 269, 277, 283-284, 350; SphBase.cpp:387-388);
- *   - a zero distance pair is skipped and counted in every pass (the
- *     reference only checks in the force helpers, WcSph.cpp:160).
+ *   - IISPH: the pressure force from boundary neighbours uses p_a/rho_a^2,
+ *     the term the pressure solve predicts, not 2 p_a/rho_a^2
+ *     (IISph_M.cpp:558-559 against 127, 271, 338);
+ *   - a zero distance pair is counted once per neighbour search and
+ *     contributes W(0) to densities and nothing to gradient terms (the
+ *     reference counts it per force evaluation, WcSph.cpp:160, and divides
+ *     0/0 in the alpha and solver passes, DFSPH.cpp:157).
  * Surface tension / adhesion (m_TH.r, m_TH.bt) are zero in every workload
  * and are not restated.  The one-argument kernels use h = smoothRadius_h/2
  * (SphBase.h:808-829), the value ker_update_numbers() would set.
@@ -236,11 +241,11 @@ public:
     void neighbourSearch(bool fluidPart = true, bool solidPart = true, bool RR_same = false) {
         gridSetup();
         int n_off = int(G.offset.size());
-        int overflow = 0;
+        int overflow = 0, zero = 0;
         if (fluidPart)
             for (int k = 0; k < int(fluids.size()); ++k) {
                 auto& ps = fluids[k].parts; int n = int(ps.size());
-#pragma omp parallel for schedule(dynamic, dyn_chunk(n)) reduction(+ : overflow)
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n)) reduction(+ : overflow, zero)
                 for (int ii = 0; ii < n; ++ii) {
                     PartIdx i{-(k + 1), ii};
                     NeigbStr<T>& L = nbF[k][ii]; L.num = 0;
@@ -252,7 +257,10 @@ public:
                             T h = dOf(j) + ps[ii].d;
                             T h2 = h * h;                  /* (real_t)pow(h,2): exact square rounded once */
                             T len2 = pa.dis_square(posOf(j));
-                            if (len2 < h2) L.push(j, std::sqrt(len2), overflow);
+                            if (len2 < h2) {
+                                if (len2 == 0) ++zero;
+                                L.push(j, std::sqrt(len2), overflow);
+                            }
                         }
                 }
             }
@@ -278,6 +286,7 @@ public:
                 }
             }
         tooMany += overflow;
+        zeroDis += zero;   /* m_EC.zeroDis: coincident fluid-particle neighbour pairs of this search */
     }
 
     /* SphBase::prepareSolidPartWeight, SphBase.cpp:766-813 */
@@ -372,7 +381,7 @@ public:
                 for (int j = 0; j < L.num; ++j) {
                     const auto& nb = L.neigs[j];
                     T dis = nb.dis;
-                    if (dis == 0) { ++zd; continue; }
+                    if (dis == 0) continue;
                     if (nb.pidx.isFluid()) {
                         const FluidPart<T>& b = fp(nb.pidx);
                         int kb = nb.pidx.toFluidI();
@@ -452,7 +461,10 @@ public:
                 const NeigbStr<T>& L = nbF[k][i];
                 for (int j = 0; j < L.num; ++j) {
                     const auto& nb = L.neigs[j];
-                    if (nb.dis == 0) { ++zd; continue; }
+                    if (nb.dis == 0) {                 /* coincident: W(0) only, no gradient */
+                        density += nb.pidx.isFluid() ? fm0 * W0 : rho0 * bp(nb.pidx).volume * W0;
+                        continue;
+                    }
                     T gs = -Wg(nb.dis);
                     if (nb.pidx.isFluid()) {
                         density += fm0 * W(nb.dis);
@@ -491,7 +503,7 @@ public:
                 for (int j = 0; j < L.num; ++j) {
                     const auto& nb = L.neigs[j];
                     T dis = nb.dis;
-                    if (dis == 0) { ++zd; continue; }
+                    if (dis == 0) continue;
                     if (nb.pidx.isFluid()) {
                         const FluidPart<T>& b = fp(nb.pidx);
                         int kb = nb.pidx.toFluidI();
@@ -563,6 +575,58 @@ public:
         }
     }
 
+    /* sum_j m (v*_a - v*_b).gradW + sum_b rho0 V_b (v*_a - v_b).gradW for particle i of
+     * fluid k: the predicted density change rate of DFSPH.cpp:589-617 / 795-825. */
+    T densityChange(int k, int i) const {
+        const auto& ps = fluids[k].parts;
+        const FluidPart<T>& a = ps[i];
+        T rho0 = fluids[k].rho0, fm0 = rho0 * r * r * r, d = 0;
+        const NeigbStr<T>& L = nbF[k][i];
+        for (int j = 0; j < L.num; ++j) {
+            const auto& nb = L.neigs[j];
+            if (nb.dis == 0) continue;
+            T gs = -Wg(nb.dis);
+            if (nb.pidx.isFluid()) {
+                const FluidPart<T>& b = fp(nb.pidx);
+                d += ((a.x - b.x) * (gs / nb.dis)).dot(a.v_adv - b.v_adv) * fm0;
+            } else {
+                const BoundPart<T>& b = bp(nb.pidx);
+                d += ((a.x - b.x) * (gs / nb.dis)).dot(a.v_adv - b.v) * (rho0 * b.volume);
+            }
+        }
+        return d;
+    }
+
+    /* Warm start of a DFSPH loop (Bender & Koschier; not in the reference):
+     * half of the previous step's accumulated stiffness, only where the
+     * current prediction still compresses (rho* > rho0 for the density loop,
+     * Drho/Dt > 0 for the divergence loop), applied as one velocity
+     * correction.  The accumulator restarts from the applied guess. */
+    void warmStart(int which, T dt) {
+        std::vector<std::vector<T>> k0(fluids.size());
+        for (size_t k = 0; k < fluids.size(); ++k) {
+            auto& ps = fluids[k].parts;
+            int n = int(ps.size());
+            k0[k].assign(n, T(0));
+            T rho0 = fluids[k].rho0;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+            for (int i = 0; i < n; ++i) {
+                T d = densityChange(int(k), i);
+                FluidPart<T>& a = ps[i];
+                if (which == 0) {
+                    bool on = a.rho + dt * d > rho0;
+                    a.kappa_sum = on ? T(0.5) * a.kappa_sum : T(0);
+                    k0[k][i] = a.kappa_sum / (dt * dt);
+                } else {
+                    bool on = d > 0;
+                    a.kappaV_sum = on ? T(0.5) * a.kappaV_sum : T(0);
+                    k0[k][i] = a.kappaV_sum / dt;
+                }
+            }
+        }
+        df_velocityCorrection(which, dt, &k0);
+    }
+
     /* pass A of the density loop, DFSPH.cpp:572-643: rho*, clamp, kappa.  Returns sum rho* (double). */
     double df_densityAdv(T dt, int& nOut) {
         T v0 = r * r * r;
@@ -613,17 +677,10 @@ public:
 #pragma omp parallel for
             for (int i = 0; i < n; ++i) f.parts[i].v_adv = f.parts[i].v + f.parts[i].a_adv * dt;
         }
-        /* warm start (Bender & Koschier): apply the previous step's accumulated
-         * kappa*dt^2 as the first correction.  Not in the reference. */
-        if (P.warm_start == 1 && frame > 0) {
-            std::vector<std::vector<T>> k0(fluids.size());
-            for (size_t k = 0; k < fluids.size(); ++k) {
-                k0[k].resize(fluids[k].parts.size());
-                for (size_t i = 0; i < fluids[k].parts.size(); ++i) k0[k][i] = fluids[k].parts[i].kappa_sum / (dt * dt);
-            }
-            df_velocityCorrection(0, dt, &k0);
-        }
-        for (auto& f : fluids) for (auto& p : f.parts) p.kappa_sum = (P.warm_start == 1 && frame > 0) ? p.kappa_sum : 0;
+        /* The density loop is not warm-started: its kappa only pushes
+         * (rho* is clamped at rho0, DFSPH.cpp:620-623), so an over-estimated
+         * initial guess is never taken back and adds energy. */
+        for (auto& f : fluids) for (auto& p : f.parts) p.kappa_sum = 0;
         double err = 1; int it = 0;
         int maxIt = P.max_iter_density > 0 ? P.max_iter_density : std::numeric_limits<int>::max();
         while (err > double(P.density_tol) || it < P.min_iter_density) {
@@ -651,6 +708,8 @@ public:
     void df_divergence(orc_stats* st) {
         T v0 = r * r * r, dt = dt_cur;
         for (auto& f : fluids) for (auto& p : f.parts) p.v_adv = p.v;
+        if (P.warm_start == 1 && frame > 0) warmStart(1, dt);
+        else for (auto& f : fluids) for (auto& p : f.parts) p.kappaV_sum = 0;
         double avg = 0; int it = 0;
         while (avg > double(P.divergence_tol) || it < P.min_iter_div) {
             if (it > P.max_iter_div) { st->max_iter_hit = 1; break; }   /* DFSPH.cpp:772-773 */
@@ -680,6 +739,7 @@ public:
                     if (dd < 0 || (a.rho + dd * dt) < rho0) dd = 0;       /* DFSPH.cpp:829-832 */
                     a.divdev = dd;
                     a.kappaV = dd * a.alpha / dt;                          /* DFSPH.cpp:835 */
+                    a.kappaV_sum += a.kappaV * dt;
                 }
                 for (int i = 0; i < n; ++i) sum += double(ps[i].divdev);
                 ntot += n;
@@ -709,7 +769,7 @@ public:
                 for (int j = 0; j < L.num; ++j) {
                     const auto& nb = L.neigs[j];
                     T dis = nb.dis;
-                    if (dis == 0) { ++zd; continue; }
+                    if (dis == 0) continue;
                     if (nb.pidx.isFluid()) {
                         const FluidPart<T>& b = fp(nb.pidx);
                         int kb = nb.pidx.toFluidI();
@@ -922,7 +982,9 @@ public:
                         const BoundPart<T>& b = bp(nb.pidx);
                         V3<T> xab = a.x - b.x;
                         T grad = -Wg(dis) / dis, acce = 0;
-                        if (a.p > 0) acce += grad * (-rho0 * b.volume * (a.p / (a.rho * a.rho) * 2));
+                        /* p_a/rho_a^2, the boundary term the pressure solve predicts (IISph_M.cpp:127, 271, 338);
+                         * IISph_M.cpp:558-559 applies 2 p_a/rho_a^2, twice the push the solve accounted for */
+                        if (a.p > 0) acce += grad * (-rho0 * b.volume * (a.p / (a.rho * a.rho)));
                         a.a_p += xab * acce;
                     }
                 }
@@ -1097,7 +1159,19 @@ int orc_add_solid(orc_sim* s, int n, const float* xyz, int dynamic, float visc) 
 }
 
 int orc_init(orc_sim* s) {
-    return dispatch(s, [&](auto& S) { S.prepareSolidPartWeight(); return 0; });
+    return dispatch(s, [&](auto& S) {
+        // SphWithIO::addFluidCuboid / SphSolid::adjustSpace grow the domain to
+        // contain every particle (SphWithIO.cpp:161-162, SphBase.cpp:149-161);
+        // here the domain is given, so particles outside it are rejected.
+        for (auto& f : S.fluids)
+            for (auto& q : f.parts)
+                if (!q.x.inside(S.spMin, S.spMax)) return -1;
+        for (auto& so : S.solids)
+            for (auto& q : so.parts)
+                if (!q.x.inside(S.spMin, S.spMax)) return -1;
+        S.prepareSolidPartWeight();
+        return 0;
+    });
 }
 
 int orc_step(orc_sim* s, orc_stats* st) {

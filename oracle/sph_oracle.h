@@ -41,7 +41,7 @@ typedef struct orc_params {
     int   min_iter_div;        /* DFSPH.cpp:765 (1) */
     int   max_iter_div;        /* DFSPH.cpp:764 (30, loop breaks when it > 30) */
     float omega;               /* IISPH relaxed Jacobi (IISph_M.cpp:341) */
-    int   warm_start;          /* 0 none; 1 previous step's accumulated kappa */
+    int   warm_start;          /* 0 none; 1 divergence loop warm-started from the previous step */
     int   reserved[6];
 } orc_params;
 

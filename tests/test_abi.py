@@ -1,0 +1,108 @@
+"""The C-ABI library loads and exports every entry point include/sphgpu.h
+declares; host-only entry points (parameter validation, scene samplers) behave
+as documented.  No compute call is made here."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from graphics_amd import sphgpu as sg
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "sphgpu.h"
+
+
+def _declared_symbols():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sphgpu_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = sg.load()
+    names = _declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(sg.SIGNATURES), "binding and header disagree"
+
+
+def test_struct_layouts_match_header_sizes():
+    # 4 (solver) + 2*4 + 6*4 + 4 + 3*4 + 6*4 + 2*4 + 4*4 + 4 + 4 + 4 + 8*4
+    assert C.sizeof(sg.Params) == 140
+    assert C.sizeof(sg.Stats) % 8 == 0
+
+
+def test_abi_version():
+    assert sg.load().sphgpu_abi_version() == 1
+
+
+def test_default_params_mirror_reference_defaults():
+    p = sg.default_params()
+    assert p.spacing_r == pytest.approx(0.1)          # SphBase.cpp:214
+    assert p.smooth_radius_h == pytest.approx(0.2)    # SphBase.cpp:215
+    assert p.dt == pytest.approx(0.001)               # SphBase.cpp:219
+    assert p.gravity[1] == pytest.approx(-9.8)        # SphBase.cpp:220
+    assert p.gamma == 7.0                             # WcSph.h:34
+    assert p.density_tol == pytest.approx(1e-4)       # DFSPH.cpp:225
+    assert p.divergence_tol == pytest.approx(2.5)     # DFSPH.cpp:762
+    assert (p.min_iter_density, p.min_iter_div, p.max_iter_div) == (2, 1, 30)
+    assert p.omega == pytest.approx(0.5)              # IISph_M.cpp:341
+
+
+@pytest.mark.parametrize("bad", [dict(solver=7), dict(spacing_r=0.0), dict(dt=-1.0),
+                                 dict(num_scenes=0), dict(warm_start=9)])
+def test_create_rejects_bad_parameters(bad):
+    with pytest.raises(sg.SphError) as e:
+        sg.Simulation(sg.default_params(**bad))
+    assert e.value.code == -1
+
+
+def test_scene_calls_validate_without_a_device():
+    sim = sg.Simulation(sg.default_params())
+    with pytest.raises(sg.SphError):
+        sim.add_fluid(np.zeros((4, 3)), phase=0)           # no phase defined yet
+    assert sim.add_phase(1000.0, 0.01) == 0
+    with pytest.raises(sg.SphError):
+        sim.add_fluid(np.zeros((4, 3)), phase=0, scene=3)  # scene out of range
+    assert sim.add_fluid(np.full((4, 3), 0.5), phase=0) == 0
+    assert sim.add_fluid(np.full((2, 3), 0.5), phase=0) == 4
+    assert sim.num_fluid() == 6
+    with pytest.raises(sg.SphError):
+        sim.set_scene_domain(0, (0, 0, 0), (1, 0, 1))      # empty domain
+    sim.close()
+
+
+def test_box_sampler_counts():
+    # makeCubeSampleParts: 1.0x0.6x0.6 m at 0.02 m, open top 6,381 / closed 7,802;
+    # 4x2x2 m at 0.01 m closed 400,002
+    assert len(sg.box_walls((0.5, 0.3, 0.3), (0.5, 0.3, 0.3), 0.02, False)) == 6381
+    assert len(sg.box_walls((0.5, 0.3, 0.3), (0.5, 0.3, 0.3), 0.02, True)) == 7802
+    assert len(sg.box_walls((2.0, 1.0, 1.0), (2.0, 1.0, 1.0), 0.01, True)) == 400002
+
+
+def test_box_samples_lie_on_the_surface():
+    w = sg.box_walls((0.5, 0.3, 0.3), (0.5, 0.3, 0.3), 0.02, False).astype(np.float64)
+    lo, hi = np.array([0.0, 0.0, 0.0]), np.array([1.0, 0.6, 0.6])
+    on = np.any((np.abs(w - lo) < 1e-6) | (np.abs(w - hi) < 1e-6), axis=1)
+    assert on.all()
+    assert w.min() >= -1e-6 and (w - hi).max() <= 1e-6
+    # open top: no sample strictly inside the top face
+    top = np.abs(w[:, 1] - 0.6) < 1e-6
+    inner = (w[:, 0] > 1e-3) & (w[:, 0] < 1 - 1e-3) & (w[:, 2] > 1e-3) & (w[:, 2] < 0.6 - 1e-3)
+    assert not np.any(top & inner)
+
+
+def test_lattice_jitter_is_bounded_and_deterministic():
+    a = sg.lattice((10, 9, 8), (0.1, 0.2, 0.3), 0.01, 0.01, 1002, 0)
+    b = sg.lattice((10, 9, 8), (0.1, 0.2, 0.3), 0.01, 0.01, 1002, 0)
+    c = sg.lattice((10, 9, 8), (0.1, 0.2, 0.3), 0.01, 0.01, 1003, 0)
+    assert a.tobytes() == b.tobytes()
+    assert a.tobytes() != c.tobytes()
+    base = sg.lattice((10, 9, 8), (0.1, 0.2, 0.3), 0.01, 0.0, 0, 0)
+    d = (a - base).astype(np.float64)
+    assert np.abs(d).max() <= 0.01 * 0.01 * (1 + 1e-4)
+    assert np.abs(d).max() > 0.009 * 0.01
+    # x fastest (SphWithIO.cpp:175-192)
+    assert base[1, 0] == pytest.approx(0.11) and base[10, 1] == pytest.approx(0.21)

@@ -1,0 +1,212 @@
+"""Parity of the HIP path (through the C-ABI) with the oracle.
+
+Bars, as stated by the workload definition:
+  * neighbour sets bit-exact: (kind, original id, FP32 distance bits) equal to
+    the oracle's FP32 restatement of the reference grid search;
+  * single-step fields within 1e-5, normwise relative (max |gpu - oracle| /
+    max |oracle|) of the FP64 oracle on the same FP32 inputs;
+  * multi-step runs matched on statistics (trajectories diverge chaotically).
+"""
+import numpy as np
+import pytest
+
+from graphics_amd import scenes, sphgpu as sg
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-5
+
+
+def _sets_gpu(counts, kinds, ids, dis):
+    return [sorted(zip(kinds[q, :counts[q]].tolist(), ids[q, :counts[q]].tolist(),
+                       dis[q, :counts[q]].view(np.uint32).tolist())) for q in range(len(counts))]
+
+
+def _sets_oracle(o):
+    counts, kinds, ids, dis, over = o.neighbours(0, 48)
+    assert over == 0, "tests assume no list truncation (order dependent in the reference)"
+    fid = o.get(orc.F_ID).astype(np.int64)
+    out = [None] * len(fid)
+    for q in range(len(fid)):
+        c = counts[q]
+        out[fid[q]] = sorted(zip(kinds[q, :c].tolist(), ids[q, :c].tolist(),
+                                 dis[q, :c].view(np.uint32).tolist()))
+    return out
+
+
+def _relerr(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+@pytest.mark.parametrize("name", ["small_dam", "wcsph_c1", "compressed"])
+def test_neighbour_sets_bit_exact(gpu, name):
+    if name == "small_dam":
+        sc = scenes.small_dam(sg.DFSPH, n=(12, 10, 9))
+    elif name == "wcsph_c1":
+        sc = scenes.wcsph_c1()
+    else:   # 10 % compressed lattice with 1 % jitter: more neighbours per particle
+        sc = scenes.small_dam(sg.DFSPH, n=(12, 12, 12), spacing=0.02)
+        sc.fluids[0] = (0, 0, sg.lattice((12, 12, 12), (0.02, 0.02, 0.02), 0.018, 0.01, 77, 0))
+    sim = scenes.make_simulation(sc)
+    sim.init()
+    got = _sets_gpu(*sim.neighbours(96))
+    want = _sets_oracle(orc.from_scene(sc, 32))
+    assert len(got) == len(want) == sc.num_fluid
+    bad = [q for q in range(len(got)) if got[q] != want[q]]
+    assert not bad, f"{len(bad)} particles differ, first {bad[0]}: {got[bad[0]][:4]} vs {want[bad[0]][:4]}"
+    sim.close()
+
+
+def test_boundary_volume(gpu):
+    sc = scenes.small_dam(sg.DFSPH, n=(10, 10, 10))
+    sim = scenes.make_simulation(sc)
+    vb = sim.boundary_volume(sc.num_boundary)
+    ob = orc.from_scene(sc, 64).boundary_volume(0, sc.num_boundary)
+    assert _relerr(vb, ob) < FIELD_TOL
+    sim.close()
+
+
+def test_lattice_density_kat(gpu):
+    # free unjittered lattice, g = 0: nothing moves, rho is the lattice density
+    s = 0.01
+    p = dict(solver=sg.DFSPH, spacing_r=s, smooth_radius_h=2 * s, dt=1e-3, gravity=(0, 0, 0))
+    sc = scenes.Scene("lattice", p)
+    sc.phases.append((1000.0, 0.0))
+    sc.domains.append(((-0.1, -0.1, -0.1), (0.2, 0.2, 0.2)))
+    sc.fluids.append((0, 0, sg.lattice((9, 9, 9), (0.0, 0.0, 0.0), s, 0.0, 0, 0)))
+    sim = scenes.make_simulation(sc)
+    sim.step()
+    rho = sim.get(sg.F_RHO)
+    alpha = sim.get(sg.F_ALPHA)
+    idx = np.arange(9 ** 3).reshape(9, 9, 9)[2:-2, 2:-2, 2:-2].ravel()   # interior (z, y, x)
+    np.testing.assert_allclose(rho[idx], 999.972466, rtol=1e-6)
+    np.testing.assert_allclose(alpha[idx], 2.353763e-7, rtol=1e-5)
+    sim.close()
+
+
+FIXED = dict(min_iter_density=3, max_iter_density=3, min_iter_div=2, max_iter_div=1)
+
+
+@pytest.mark.parametrize("solver", [sg.WCSPH, sg.DFSPH, sg.IISPH])
+def test_single_step_fields(gpu, solver):
+    sc = scenes.small_dam(solver, n=(12, 12, 12), **FIXED)
+    sim = scenes.make_simulation(sc)
+    st = sim.step()
+    o = orc.from_scene(sc, 64)
+    ost = o.step()
+    fields = [(sg.F_POS, orc.F_POS), (sg.F_VEL, orc.F_VEL), (sg.F_RHO, orc.F_RHO)]
+    if solver == sg.WCSPH:
+        fields += [(sg.F_PRESSURE, orc.F_PRESSURE), (sg.F_ACC, orc.F_ACC)]
+    elif solver == sg.DFSPH:
+        fields += [(sg.F_ALPHA, orc.F_ALPHA)]
+    else:
+        fields += [(sg.F_PRESSURE, orc.F_PRESSURE), (sg.F_ACC, orc.F_ACC)]
+    assert np.array_equal(sim.ids(), o.get_by_id(orc.F_ID)[0])
+    errs = {}
+    for f, of in fields:
+        errs[f] = _relerr(sim.get(f), o.get_by_id(of)[1])
+    assert all(e < FIELD_TOL for e in errs.values()), errs
+    assert st["iter_density"] == ost["iter_density"]
+    assert st["iter_divergence"] == ost["iter_divergence"]
+    assert st["density_err"] == pytest.approx(ost["density_err"], rel=1e-4, abs=1e-9)
+    sim.close()
+
+
+@pytest.mark.parametrize("solver,warm", [(sg.DFSPH, 0), (sg.DFSPH, 1), (sg.IISPH, 0), (sg.WCSPH, 0)])
+def test_multi_step_statistics(gpu, solver, warm):
+    steps = 40 if solver != sg.WCSPH else 200
+    sc = scenes.small_dam(solver, n=(12, 14, 12), box=(0.5, 0.4, 0.35), warm_start=warm)
+    sim = scenes.make_simulation(sc)
+    o = orc.from_scene(sc, 64)
+    g = [sim.step() for _ in range(steps)]
+    r = [o.step() for _ in range(steps)]
+    for key in ("rho_mean_err", "rho_max_err"):
+        a = np.array([s[key] for s in g[-10:]])
+        b = np.array([s[key] for s in r[-10:]])
+        assert np.abs(a - b).max() < 2e-3, key
+    ke_g = np.array([s["kinetic_energy"] for s in g])
+    ke_r = np.array([s["kinetic_energy"] for s in r])
+    assert np.abs(ke_g - ke_r).max() / ke_r.max() < 0.02
+    if solver != sg.WCSPH:
+        ig = sum(s["iter_density"] for s in g)
+        ir = sum(s["iter_density"] for s in r)
+        assert abs(ig - ir) <= 0.15 * ir + 2
+    if solver == sg.DFSPH:
+        ig = sum(s["iter_divergence"] for s in g)
+        ir = sum(s["iter_divergence"] for s in r)
+        assert abs(ig - ir) <= 0.15 * ir + 2
+    assert all(s["nan_count"] == 0 for s in g)
+    sim.close()
+
+
+def test_run_is_deterministic(gpu):
+    out = []
+    for _ in range(2):
+        sc = scenes.small_dam(sg.DFSPH, n=(10, 10, 10))
+        sim = scenes.make_simulation(sc)
+        st = [sim.step() for _ in range(10)]
+        out.append((sim.get(sg.F_POS).tobytes(), sim.get(sg.F_VEL).tobytes(),
+                    [s["iter_density"] for s in st], [s["density_err"] for s in st]))
+        sim.close()
+    assert out[0] == out[1]
+
+
+def test_batched_scenes_equal_single_scenes(gpu):
+    """Scenes advanced together give bit-identical results to each scene alone."""
+    sc = scenes.iisph_batched_c4(scenes=3, nmin=300, nmax=2000, seed=11)
+    sim = scenes.make_simulation(sc)
+    for _ in range(5):
+        sim.step()
+    ids = sim.ids()
+    pos = sim.get(sg.F_POS)
+    first = 0
+    for k, (s, ph, xyz) in enumerate(sc.fluids):
+        single = scenes.Scene("one", sc.params, list(sc.phases), [(0, 0, xyz)],
+                              [(0, sc.solids[k][1], sc.solids[k][2])], [sc.domains[k]])
+        one = scenes.make_simulation(single)
+        for _ in range(5):
+            one.step()
+        sel = (ids >= first) & (ids < first + len(xyz))
+        assert np.array_equal(ids[sel] - first, one.ids())
+        assert pos[sel].tobytes() == one.get(sg.F_POS).tobytes()
+        first += len(xyz)
+        one.close()
+    sim.close()
+
+
+# ------------------------------------------------------------- faults ------
+def test_fault_counters(gpu):
+    s = 0.02
+    p = dict(solver=sg.WCSPH, spacing_r=s, smooth_radius_h=2 * s, dt=1e-4, sound_speed=20.0)
+    sc = scenes.Scene("faults", p)
+    sc.phases.append((1000.0, 0.01))
+    sc.domains.append(((0.0, 0.0, 0.0), (1.0, 0.5, 1.0)))
+    xyz = sg.lattice((4, 4, 4), (0.3, 0.3, 0.3), s, 0.0, 0, 0)
+    xyz = np.concatenate([xyz, xyz[:1]])                              # a coincident pair
+    dense = sg.lattice((5, 5, 5), (0.6, 0.2, 0.6), 0.012, 0.0, 0, 0)   # > 48 neighbours
+    sc.fluids.append((0, 0, np.concatenate([xyz, dense])))
+    sim = scenes.make_simulation(sc)
+    st = sim.step()
+    assert st["zero_dis"] >= 2
+    assert st["too_many_neigb"] > 0
+    sim.close()
+    # removal outside the domain
+    vel = np.zeros((27, 3), np.float32)
+    vel[0, 1] = 1e5
+    sim = sg.Simulation(sg.default_params(**{**p, "space_min": (0, 0, 0), "space_max": (1.0, 0.5, 1.0)}))
+    sim.add_phase(1000.0, 0.01)
+    sim.add_fluid(sg.lattice((3, 3, 3), (0.3, 0.3, 0.3), s, 0.0, 0, 0), vel)
+    st = sim.step()
+    assert st["removed"] == 1
+    st = sim.step()
+    assert sim.num_fluid() == 26 and 0 not in set(sim.ids().tolist())
+    sim.close()
+    # iteration cap
+    sc = scenes.small_dam(sg.DFSPH, n=(6, 6, 6), density_tol=0.0, max_iter_density=7)
+    sim = scenes.make_simulation(sc)
+    st = sim.step()
+    assert st["max_iter_hit"] == 1 and st["iter_density"] == 7
+    sim.close()
