@@ -112,7 +112,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", default="dfsph_c2", choices=sorted(scenes.WORKLOADS))
-    ap.add_argument("--warm-start", default="previous", choices=["none", "previous"])
+    ap.add_argument("--warm-start", default="previous", choices=["none", "previous", "learned"])
     ap.add_argument("--no-ab", action="store_true", help="skip the plain-solver arm")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -128,7 +128,7 @@ def main() -> int:
 
     sc = build_scene(args.workload)
     solver = sc.params["solver"]
-    warm = {"none": sg.WARM_NONE, "previous": sg.WARM_PREVIOUS}[args.warm_start]
+    warm = {"none": sg.WARM_NONE, "previous": sg.WARM_PREVIOUS, "learned": sg.WARM_LEARNED}[args.warm_start]
     prof = args.profile_kernel or DOMINANT[solver]
     run = run_gpu(sc, warm, args.steps, args.warmup, prof)
     nf = sc.num_fluid
@@ -163,11 +163,14 @@ def main() -> int:
         tot_w = it_d + it_v
         tot_p = iters["plain"]["density"] + iters["plain"]["divergence"]
         iters["reduction_vs_plain"] = round(1.0 - tot_w / tot_p, 4) if tot_p else None
+        pv = iters["plain"]["divergence"]
+        iters["divergence_reduction_vs_plain"] = round(1.0 - it_v / pv, 4) if pv else None
         log(f"[bench] plain arm: iters {iters['plain']['density']:.2f}/{iters['plain']['divergence']:.2f}")
 
     cpu = None
     if not args.no_cpu:
-        c = cpu_baseline(sc, warm, args.cpu_seconds)
+        # the oracle has no learned model: its divergence solve warm-starts from the previous step
+        c = cpu_baseline(sc, min(warm, sg.WARM_PREVIOUS), args.cpu_seconds)
         cpu_rate = nf * c["steps"] / c["elapsed"]
         gpu_same = nf * c["steps"] / sum(run["warm_times"][:c["steps"]]) if len(run["warm_times"]) >= c["steps"] else None
         cpu = {"value": round(cpu_rate, 2), "unit": "particle-steps/s", "cores": c["threads"],

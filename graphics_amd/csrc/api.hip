@@ -1,6 +1,7 @@
 // C-ABI of libsphgpu.so: handle lifecycle, scene upload, the per-step
 // drivers of the three solvers (the reference's sphStep overrides) and
 // readback.  All device work runs on the handle's single HIP stream.
+#include "mlp.h"
 #include "sph_common.h"
 #include "sphgpu.h"
 
@@ -72,6 +73,18 @@ struct sphgpu_handle {
     int* hLayout = nullptr;
     float* hDt = nullptr;
     std::vector<sphgpu_stats> sceneStats;
+    // learned warm start of the divergence solve
+    MlpDev mlp{};
+    bool mlpLive = false;
+    std::vector<float> mlpHost;        // parameters before init (or set by the caller)
+    bool mlpPreset = false;            // caller-provided parameters: use them from the first step
+    float* feat = nullptr;             // ncap x 8 features, per slot
+    float* tgt = nullptr;              // ncap converged stiffness, per slot
+    float* ypred = nullptr;            // ncap predictions
+    float* bfeat = nullptr;            // learn_batch x 8 minibatch
+    float* btgt = nullptr;
+    int learned = 0;                   // training steps done
+    uint32_t gatherSeed = 1;
     // profiling
     std::string profName;
     std::vector<ProfEvent> prof;
@@ -217,7 +230,7 @@ int solver_loop(sphgpu_handle* h, int which) {
             } else {
                 {
                     ProfScope ps(h, "df_pass_divergence", nf_live(h) * 36.0);
-                    launch_df_pass_divergence(D, h->st);
+                    launch_df_pass_divergence(D, P.warm_start != SPHGPU_WARM_NONE, h->st);
                 }
                 {
                     ProfScope ps(h, "df_pass_velocity", nf_live(h) * 44.0);
@@ -343,12 +356,36 @@ int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
         launch_df_density_alpha(D, h->st);
     }
     HIPCHECK(h, hipMemcpyAsync(D.vs, D.v, sizeof(float4) * D.npad, hipMemcpyDeviceToDevice, h->st));
-    launch_df_warm_div(D, warm && h->frame > 0, h->st);
-    if (warm && h->frame > 0) {
+    // divergence warm start: 0 none, 1 half the previous stiffness, 2 model
+    const bool learnedMode = h->P.warm_start == SPHGPU_WARM_LEARNED;
+    const bool training = learnedMode && !h->mlpPreset && h->learned < h->P.learn_steps;
+    int mode = 0;
+    if (h->frame > 0) {
+        if (warm || (learnedMode && training)) mode = 1;
+        else if (learnedMode) mode = 2;
+    }
+    launch_df_warm_features(D, learnedMode ? h->feat : nullptr, h->st);
+    if (mode == 2) {
+        ProfScope ps(h, "mlp_forward", nf_live(h) * 36.0);
+        mlp_forward(h->mlp, h->feat, h->ypred, D.npad, nullptr, h->st);
+    }
+    launch_df_warm_apply(D, h->ypred, mode, h->st);
+    if (mode != 0) {
         launch_fill_i32(D.active, 1, D.nscenes, h->st);
         launch_df_pass_velocity(D, h->st);
     }
     if ((rc = solver_loop(h, 1))) return rc;
+    if (training && h->frame > 0) {
+        // online training on this step's converged stiffness
+        launch_df_targets(D, h->tgt, h->st);
+        const int b = h->P.learn_batch;
+        for (int it = 0; it < h->P.learn_iters; ++it) {
+            launch_mlp_gather(D, h->feat, h->tgt, h->bfeat, h->btgt, b, h->gatherSeed++, h->st);
+            mlp_grad(h->mlp, h->bfeat, h->btgt, b, h->st);
+            mlp_adam(h->mlp, h->P.learn_rate, h->st);
+        }
+        ++h->learned;
+    }
     std::swap(D.v, D.vs);
     return finish_step(h, st);
 }
@@ -452,6 +489,11 @@ void sphgpu_default_params(sphgpu_params* p) {
     p->omega = 0.5f;                      // IISph_M.cpp:341
     p->warm_start = SPHGPU_WARM_NONE;
     p->num_scenes = 1;
+    p->learn_steps = 40;
+    p->learn_iters = 16;
+    p->learn_batch = 16384;
+    p->learn_rate = 1e-3f;
+    p->learn_over_weight = 1.0f;
 }
 
 int sphgpu_device_available(void) {
@@ -469,6 +511,11 @@ int sphgpu_create(const sphgpu_params* p, sphgpu_handle** out) {
     if (!(p->spacing_r > 0) || !(p->smooth_radius_h > 0) || !(p->dt > 0)) return SPHGPU_E_ARG;
     if (p->num_scenes < 1 || p->num_scenes > 4096) return SPHGPU_E_ARG;
     if (p->warm_start < 0 || p->warm_start > 2) return SPHGPU_E_ARG;
+    if (p->warm_start == SPHGPU_WARM_LEARNED &&
+        (p->solver != SPHGPU_DFSPH || p->learn_steps < 0 || p->learn_iters < 0 || p->learn_batch < 16 ||
+         p->learn_batch > (1 << 22) || !(p->learn_rate >= 0) || !(p->learn_over_weight > 0) ||
+         !std::isfinite(p->learn_over_weight)))
+        return SPHGPU_E_ARG;
     sphgpu_handle* h = new sphgpu_handle();
     h->P = *p;
     h->scenes.resize(p->num_scenes);
@@ -486,6 +533,7 @@ void sphgpu_destroy(sphgpu_handle* h) {
         hipStreamSynchronize(h->st);
         for (auto& e : h->prof) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
         for (void* m : h->allocs) hipFree(m);
+        if (h->mlpLive) mlp_free(h->mlp);
         hipHostFree(h->hActive);
         hipStreamDestroy(h->st);
     }
@@ -620,6 +668,26 @@ int sphgpu_init(sphgpu_handle* h) {
     D.Hs = H;
     D.visc_eps = 0.01f * H * H;
     D.v0 = s * s * s;
+    D.spacing = s;
+    D.dt_max = P.dt;
+    {
+        // alpha of a particle inside a lattice of the sampling spacing
+        // (DFSPH.cpp:541-566 on a full neighbourhood): rho / sum (m |grad W|)^2
+        const double kh = D.kh, sig = 1.0 / M_PI / (kh * kh * kh), m = double(h->phases[0].rho0) * s * s * s;
+        double rho = 0.0, g2 = 0.0;
+        const int R = int(std::ceil(2.0 * kh / s)) + 1;
+        for (int a = -R; a <= R; ++a)
+            for (int b = -R; b <= R; ++b)
+                for (int c = -R; c <= R; ++c) {
+                    double r = s * std::sqrt(double(a * a + b * b + c * c)), q = r / kh;
+                    if (q >= 2.0) continue;
+                    rho += m * sig * (q <= 1.0 ? 1.0 - 1.5 * q * q + 0.75 * q * q * q : 0.25 * (2 - q) * (2 - q) * (2 - q));
+                    if (r == 0.0) continue;
+                    double dw = sig / kh * (q <= 1.0 ? -3.0 * q + 2.25 * q * q : -0.75 * (2 - q) * (2 - q));
+                    g2 += (m * dw) * (m * dw);
+                }
+        D.alpha_lat = float(rho / std::max(g2, 1e-6));
+    }
     D.gx = P.gravity[0]; D.gy = P.gravity[1]; D.gz = P.gravity[2];
     D.cs = P.sound_speed;
     D.gamma = P.gamma;
@@ -661,6 +729,18 @@ int sphgpu_init(sphgpu_handle* h) {
     HIPCHECK(h, hipMalloc(&sortMem, sort_scratch_bytes(sortItems)));
     h->allocs.push_back(sortMem);
     sort_scratch_bind(h->sort, sortMem, sortItems);
+    if (P.solver == SPHGPU_DFSPH && P.warm_start == SPHGPU_WARM_LEARNED) {
+        ALLOC(h->feat, 8 * (size_t)ncap); ALLOC(h->tgt, ncap); ALLOC(h->ypred, ncap);
+        ALLOC(h->bfeat, 8 * (size_t)P.learn_batch); ALLOC(h->btgt, P.learn_batch);
+        HIPCHECK(h, hipMemset(h->feat, 0, sizeof(float) * 8 * ncap));
+        HIPCHECK(h, hipMemset(h->tgt, 0, sizeof(float) * ncap));
+        if (mlp_alloc(h->mlp, P.learn_batch)) return fail(h, SPHGPU_E_DEVICE, "MLP allocation failed");
+        h->mlpLive = true;
+        h->mlp.overWeight = P.learn_over_weight;
+        if (h->mlpHost.empty()) mlp_init_params(h->mlpHost, 0x5eedULL);
+        HIPCHECK(h, hipMemcpy(h->mlp.params, h->mlpHost.data(), sizeof(float) * MLP_PARAMS, hipMemcpyHostToDevice));
+        mlp_pack(h->mlp, h->st);
+    }
 #undef ALLOC
     // pinned host mirrors (one block)
     {
@@ -929,6 +1009,35 @@ int sphgpu_kernel_timing(sphgpu_handle* h, long long* launches, double* total_ms
     if (launches) *launches = (long long)h->profUsed;
     if (total_ms) *total_ms = ms;
     if (total_bytes) *total_bytes = bytes;
+    return SPHGPU_OK;
+}
+
+int sphgpu_get_mlp_params(sphgpu_handle* h, float* params) {
+    if (!h || !params) return SPHGPU_E_ARG;
+    if (h->P.warm_start != SPHGPU_WARM_LEARNED) return fail(h, SPHGPU_E_STATE, "handle has no learned warm start");
+    if (h->mlpLive) {
+        HIPCHECK(h, hipMemcpyAsync(params, h->mlp.params, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost, h->st));
+        HIPCHECK(h, hipStreamSynchronize(h->st));
+        return SPHGPU_OK;
+    }
+    if (h->mlpHost.empty()) mlp_init_params(h->mlpHost, 0x5eedULL);
+    std::memcpy(params, h->mlpHost.data(), sizeof(float) * MLP_PARAMS);
+    return SPHGPU_OK;
+}
+
+int sphgpu_set_mlp_params(sphgpu_handle* h, const float* params) {
+    if (!h || !params) return SPHGPU_E_ARG;
+    if (h->P.warm_start != SPHGPU_WARM_LEARNED) return fail(h, SPHGPU_E_STATE, "handle has no learned warm start");
+    for (int k = 0; k < MLP_PARAMS; ++k)
+        if (!std::isfinite(params[k])) return fail(h, SPHGPU_E_ARG, "parameter %d is not finite", k);
+    h->mlpHost.assign(params, params + MLP_PARAMS);
+    h->mlpPreset = true;
+    if (h->mlpLive) {
+        HIPCHECK(h, hipMemcpyAsync(h->mlp.params, h->mlpHost.data(), sizeof(float) * MLP_PARAMS, hipMemcpyHostToDevice,
+                                   h->st));
+        mlp_pack(h->mlp, h->st);
+        HIPCHECK(h, hipStreamSynchronize(h->st));
+    }
     return SPHGPU_OK;
 }
 

@@ -1,0 +1,531 @@
+// Per-particle MLP 8 -> 64 -> 64 -> 64 -> 1 (tanh-form GELU) on the gfx950
+// matrix cores: BF16 operands, FP32 accumulation (v_mfma_f32_16x16x32_bf16).
+//
+// Formulation: activations are kept "transposed", units x particles.  A layer
+// z = W h + b is the MFMA D = A*B with A = W (out x in, from weight fragments)
+// and B = h (in x 16 particles).  The D layout of a 16x16 tile is
+//   lane l = 16*g + c holds D[4g + j][c], j = 0..3,
+// so two consecutive 16-row tiles of one layer's output, rounded to BF16, are
+// exactly the B operand of the next layer when K slot (g, j) of the 32-deep
+// product carries logical index kmap(g, j) = j < 4 ? 4g + j : 16 + 4g + j - 4.
+// A and B place the same logical index in the same slot, so the product does
+// not depend on the hardware's own K order.
+//
+// Kernels: fragment packing (FP32 master weights -> BF16 A fragments),
+// forward (inference; in training also stores pre/post activations),
+// backward through the activations (dh = W^T dz, same chaining with the
+// transposed fragments), weight/bias gradients (MFMA with K = particles),
+// and Adam.
+#include "sph_common.h"
+#include "mlp.h"
+#include "sphgpu.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace sphgpu {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int kmap(int g, int j) { return j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4); }
+
+__device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }   // round to nearest even
+
+// GELU, tanh form: 0.5 x (1 + tanh(c (x + 0.044715 x^3))), c = sqrt(2/pi)
+__device__ __forceinline__ float gelu(float x) {
+    const float c = 0.7978845608028654f;
+    float u = c * (x + 0.044715f * x * x * x);
+    float t = 1.0f - 2.0f / (__expf(2.0f * u) + 1.0f);
+    return 0.5f * x * (1.0f + t);
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+    const float c = 0.7978845608028654f;
+    float u = c * (x + 0.044715f * x * x * x);
+    float t = 1.0f - 2.0f / (__expf(2.0f * u) + 1.0f);
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * x * x);
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// --------------------------------------------------------------- layout ----
+// parameter block (FP32 master): W1[64][8] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[1][64] b4[1]
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + 64 * 8;
+constexpr int OFF_W2 = OFF_B1 + 64, OFF_B2 = OFF_W2 + 64 * 64;
+constexpr int OFF_W3 = OFF_B2 + 64, OFF_B3 = OFF_W3 + 64 * 64;
+constexpr int OFF_W4 = OFF_B3 + 64, OFF_B4 = OFF_W4 + 64;
+static_assert(OFF_B4 + 1 == MLP_PARAMS, "parameter count");
+
+struct LayerDim { int off_w, in, out; };
+__constant__ LayerDim kLayers[4] = {{OFF_W1, 8, 64}, {OFF_W2, 64, 64}, {OFF_W3, 64, 64}, {OFF_W4, 64, 1}};
+
+// fragment counts: forward A = W (out x in): [ceil(out/16) M tiles] x [ceil(in/32) K steps]
+//   L1 4x1 (ids 0-3), L2 4x2 (4-11), L3 4x2 (12-19), L4 1x2 (20-21);
+// backward A = W^T (in x out): W4^T 4x1 (0-3), W3^T 4x2 (4-11), W2^T 4x2 (12-19)
+// (the first layer needs no input gradient).
+
+// one thread per (fragment, lane): 64 lanes x 8 BF16 values
+__global__ void k_mlp_pack(const float* __restrict__ P, bf16x8* __restrict__ fwd, bf16x8* __restrict__ bwd) {
+    int fid = blockIdx.x, lane = threadIdx.x;
+    int g = lane >> 4, c = lane & 15;
+    bf16x8 v;
+    if (fid < MLP_FWD_FRAGS) {
+        int layer = fid < 4 ? 0 : fid < 12 ? 1 : fid < 20 ? 2 : 3;
+        const int base[4] = {0, 4, 12, 20};
+        const int ks[4] = {1, 2, 2, 2};
+        int rel = fid - base[layer], t = rel / ks[layer], s = rel % ks[layer];
+        LayerDim L = kLayers[layer];
+        int out = 16 * t + c;
+        for (int j = 0; j < 8; ++j) {
+            int in = 32 * s + kmap(g, j);
+            float w = (out < L.out && in < L.in) ? P[L.off_w + out * L.in + in] : 0.0f;
+            v[j] = to_bf16(w);
+        }
+        fwd[fid * 64 + lane] = v;
+    } else {
+        int b = fid - MLP_FWD_FRAGS;
+        // backward fragments: layer 3 (W4^T: 64 x 1, 4 tiles x 1 kstep) -> 0..3,
+        // layer 2 (W3^T: 4 x 2) -> 4..11, layer 1 (W2^T: 4 x 2) -> 12..19
+        int layer = b < 4 ? 3 : b < 12 ? 2 : 1;
+        int base = b < 4 ? 0 : b < 12 ? 4 : 12;
+        int ks = b < 4 ? 1 : 2;
+        int rel = b - base, t = rel / ks, s = rel % ks;
+        LayerDim L = kLayers[layer];   // W is out x in; A = W^T is in x out
+        int in = 16 * t + c;
+        for (int j = 0; j < 8; ++j) {
+            int out = 32 * s + kmap(g, j);
+            float w = (out < L.out && in < L.in) ? P[L.off_w + out * L.in + in] : 0.0f;
+            v[j] = to_bf16(w);
+        }
+        bwd[b * 64 + lane] = v;
+    }
+}
+
+// B operand for K step s from two D tiles (t = 2s, 2s+1) of activations
+__device__ __forceinline__ bf16x8 chain(const float (&h)[4][4], int s) {
+    bf16x8 b;
+    for (int j = 0; j < 4; ++j) {
+        b[j] = to_bf16(h[2 * s][j]);
+        b[j + 4] = to_bf16(h[2 * s + 1][j]);
+    }
+    return b;
+}
+
+// forward of 16 particles [p0, p0+16); optionally stores z/h rows for training
+struct FwdOut {
+    float z[3][4][4];   // pre-activations (layer, tile, j)
+    float y;
+};
+
+__device__ __forceinline__ void forward_tile(const bf16x8* __restrict__ F, const float* __restrict__ P,
+                                             const float* __restrict__ feat, int p0, int n, FwdOut& o,
+                                             float* __restrict__ store) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const int p = p0 + c;
+    bf16x8 x;
+    for (int j = 0; j < 8; ++j) x[j] = to_bf16(0.0f);
+    if (g < 2 && p < n) {
+        const float4 f = reinterpret_cast<const float4*>(feat)[p * 2 + g];
+        x[0] = to_bf16(f.x); x[1] = to_bf16(f.y); x[2] = to_bf16(f.z); x[3] = to_bf16(f.w);
+    }
+    float h[4][4];
+    // layer 1
+    for (int t = 0; t < 4; ++t) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma(F[(0 + t) * 64 + lane], x, acc);
+        for (int j = 0; j < 4; ++j) {
+            int u = 16 * t + 4 * g + j;
+            float z = acc[j] + P[OFF_B1 + u];
+            o.z[0][t][j] = z;
+            h[t][j] = gelu(z);
+        }
+    }
+    // layers 2, 3
+    const int fbase[2] = {4, 12}, boff[2] = {OFF_B2, OFF_B3};
+    for (int L = 0; L < 2; ++L) {
+        bf16x8 b0 = chain(h, 0), b1 = chain(h, 1);
+        for (int t = 0; t < 4; ++t) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma(F[(fbase[L] + 2 * t) * 64 + lane], b0, acc);
+            acc = mfma(F[(fbase[L] + 2 * t + 1) * 64 + lane], b1, acc);
+            for (int j = 0; j < 4; ++j) {
+                int u = 16 * t + 4 * g + j;
+                float z = acc[j] + P[boff[L] + u];
+                o.z[L + 1][t][j] = z;
+                h[t][j] = gelu(z);
+            }
+        }
+    }
+    // layer 4 (1 output row)
+    {
+        bf16x8 b0 = chain(h, 0), b1 = chain(h, 1);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma(F[20 * 64 + lane], b0, acc);
+        acc = mfma(F[21 * 64 + lane], b1, acc);
+        o.y = acc[0] + P[OFF_B4];       // row 0: lanes g == 0, j == 0
+    }
+    if (store && p < n) {
+        // rows [p][layer*64 + unit]: z1 z2 z3, then y at 192
+        for (int L = 0; L < 3; ++L)
+            for (int t = 0; t < 4; ++t)
+                for (int j = 0; j < 4; ++j) store[(size_t)p * MLP_STORE + L * 64 + 16 * t + 4 * g + j] = o.z[L][t][j];
+        if (g == 0) store[(size_t)p * MLP_STORE + 192] = o.y;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_forward(const bf16x8* __restrict__ F, const float* __restrict__ P,
+                                                   const float* __restrict__ feat, float* __restrict__ y, int n,
+                                                   float* __restrict__ store) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int ntile = (n + 15) / 16;
+    for (int tile = wave; tile < ntile; tile += nw) {
+        FwdOut o;
+        forward_tile(F, P, feat, tile * 16, n, o, store);
+        int p = tile * 16 + (lane & 15);
+        if ((lane >> 4) == 0 && p < n) y[p] = o.y;
+    }
+}
+
+// backward through the activations for 16 particles: dL/dy -> dz3, dz2, dz1,
+// stored as rows [p][layer*64 + unit] in dz (FP32).  Pre-activations come
+// from the forward store.
+__global__ __launch_bounds__(256) void k_mlp_backward(const bf16x8* __restrict__ B, const float* __restrict__ store,
+                                                    const float* __restrict__ dy, float* __restrict__ dz, int n) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const int ntile = (n + 15) / 16;
+    for (int tile = wave; tile < ntile; tile += nw) {
+        const int p = tile * 16 + c;
+        const bool ok = p < n;
+        // B operand: dz4 (1 x P) in K slot (0, 0)
+        bf16x8 b;
+        for (int j = 0; j < 8; ++j) b[j] = to_bf16(0.0f);
+        if (g == 0 && ok) b[0] = to_bf16(dy[p]);
+        float d[4][4];
+        // dh3 = W4^T dz4 ; dz3 = dh3 * gelu'(z3)
+        for (int t = 0; t < 4; ++t) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma(B[(0 + t) * 64 + lane], b, acc);
+            for (int j = 0; j < 4; ++j) {
+                int u = 16 * t + 4 * g + j;
+                float z = ok ? store[(size_t)p * MLP_STORE + 2 * 64 + u] : 0.0f;
+                d[t][j] = acc[j] * gelu_grad(z);
+                if (ok) dz[(size_t)p * MLP_STORE + 2 * 64 + u] = d[t][j];
+            }
+        }
+        // dh2 = W3^T dz3 -> dz2 ; dh1 = W2^T dz2 -> dz1
+        const int fb[2] = {4, 12}, zl[2] = {1, 0};
+        for (int L = 0; L < 2; ++L) {
+            bf16x8 b0 = chain(d, 0), b1 = chain(d, 1);
+            float e[4][4];
+            for (int t = 0; t < 4; ++t) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                acc = mfma(B[(fb[L] + 2 * t) * 64 + lane], b0, acc);
+                acc = mfma(B[(fb[L] + 2 * t + 1) * 64 + lane], b1, acc);
+                for (int j = 0; j < 4; ++j) {
+                    int u = 16 * t + 4 * g + j;
+                    float z = ok ? store[(size_t)p * MLP_STORE + zl[L] * 64 + u] : 0.0f;
+                    e[t][j] = acc[j] * gelu_grad(z);
+                    if (ok) dz[(size_t)p * MLP_STORE + zl[L] * 64 + u] = e[t][j];
+                }
+            }
+            for (int t = 0; t < 4; ++t)
+                for (int j = 0; j < 4; ++j) d[t][j] = e[t][j];
+        }
+        if (g == 0 && ok) dz[(size_t)p * MLP_STORE + 192] = dy[p];
+    }
+}
+
+// value of the layer input h_L[unit] for sample p (h_0 = features)
+__device__ __forceinline__ float layer_input(int L, const float* __restrict__ feat, const float* __restrict__ store,
+                                             int p, int unit) {
+    if (L == 0) return unit < 8 ? feat[(size_t)p * 8 + unit] : 0.0f;
+    return gelu(store[(size_t)p * MLP_STORE + (L - 1) * 64 + unit]);
+}
+
+// weight and bias gradients: one wave per 16x16 output tile of dW (out x in),
+// plus bias tiles; K = samples in chunks of 32 (fixed order: deterministic).
+// tile list: L1 4x1, L2 4x4, L3 4x4, L4 1x4 (W), then bias tiles 4+4+4+1.
+__global__ __launch_bounds__(64) void k_mlp_wgrad(const float* __restrict__ feat, const float* __restrict__ store,
+                                                const float* __restrict__ dz, float* __restrict__ grad, int n,
+                                                float scale) {
+    const int tid = blockIdx.x;
+    const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+    int L, to, ti, bias = 0;
+    {
+        int k = tid;
+        const int wt[4] = {4, 16, 16, 4};
+        L = 0;
+        while (L < 4 && k >= wt[L]) { k -= wt[L]; ++L; }
+        if (L < 4) {
+            int nin = (L == 0) ? 1 : 4;
+            to = k / nin;
+            ti = k % nin;
+        } else {
+            bias = 1;
+            const int bt[4] = {4, 4, 4, 1};
+            L = 0;
+            while (L < 3 && k >= bt[L]) { k -= bt[L]; ++L; }
+            to = k;
+            ti = 0;
+        }
+    }
+    const int nin = (L == 0) ? 8 : 64, nout = (L == 3) ? 1 : 64;
+    // dz of layer L output lives at store row offset: L<3 -> L*64, L==3 -> 192
+    const int zoff = (L < 3) ? L * 64 : 192;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int orow = 16 * to + c;          // A row (output unit)
+    const int icol = 16 * ti + c;          // B column (input unit)
+    for (int q = 0; q < n; q += 32) {
+        bf16x8 a, b;
+        for (int j = 0; j < 8; ++j) {
+            int p = q + kmap(g, j);
+            bool ok = p < n;
+            float dv = (ok && orow < nout) ? dz[(size_t)p * MLP_STORE + zoff + orow] : 0.0f;
+            float hv;
+            if (bias) hv = ok ? 1.0f : 0.0f;
+            else hv = (ok && icol < nin) ? layer_input(L, feat, store, p, icol) : 0.0f;
+            a[j] = to_bf16(dv);
+            b[j] = to_bf16(hv);
+        }
+        acc = mfma(a, b, acc);
+    }
+    // D[4g+j][c] = dW[out = 16 to + 4g + j][in = 16 ti + c]
+    const int woff[4] = {OFF_W1, OFF_W2, OFF_W3, OFF_W4}, boff[4] = {OFF_B1, OFF_B2, OFF_B3, OFF_B4};
+    for (int j = 0; j < 4; ++j) {
+        int out = 16 * to + 4 * g + j;
+        if (out >= nout) continue;
+        if (bias) {
+            if (c == 0) grad[boff[L] + out] = acc[j] * scale;
+        } else {
+            int in = 16 * ti + c;
+            if (in < nin) grad[woff[L] + out * nin + in] = acc[j] * scale;
+        }
+    }
+}
+
+// MSE loss gradient dL/dy = 2 (y - t) / n (times `scale` for normalised targets)
+// loss = mean(w e^2), e = y - target, w = over where e > 0 else 1
+__global__ void k_mlp_loss(const float* __restrict__ store, const float* __restrict__ target,
+                           float* __restrict__ dy, double* __restrict__ lossPart, int n, float over) {
+    __shared__ double lds[4];
+    int i = blockIdx.x * 256 + threadIdx.x;
+    double l = 0.0;
+    if (i < n) {
+        float y = store[(size_t)i * MLP_STORE + 192];
+        float e = y - target[i];
+        float w = e > 0.0f ? over : 1.0f;
+        dy[i] = 2.0f * w * e / (float)n;
+        l = (double)w * (double)e * (double)e;
+    }
+    double s = block_sum_256(l, lds);
+    if (threadIdx.x == 0) lossPart[blockIdx.x] = s;
+}
+
+__global__ void k_mlp_adam(float* __restrict__ P, float* __restrict__ M, float* __restrict__ V,
+                           const float* __restrict__ grad, float lr, float b1, float b2, float eps, float c1,
+                           float c2) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= MLP_PARAMS) return;
+    float gr = grad[i];
+    float m = b1 * M[i] + (1.0f - b1) * gr;
+    float v = b2 * V[i] + (1.0f - b2) * gr * gr;
+    M[i] = m;
+    V[i] = v;
+    P[i] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host ----
+void mlp_pack(MlpDev& m, hipStream_t st) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3(MLP_FWD_FRAGS + MLP_BWD_FRAGS), dim3(64), 0, st, (const float*)m.params,
+                       (bf16x8*)m.fwd, (bf16x8*)m.bwd);
+}
+
+void mlp_forward(const MlpDev& m, const float* feat, float* y, int n, float* store, hipStream_t st) {
+    if (n <= 0) return;
+    int waves = (n + 15) / 16;
+    int blocks = std::min((waves + 3) / 4, 8192);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(256), 0, st, (const bf16x8*)m.fwd,
+                       (const float*)m.params, feat, y, n, store);
+}
+
+void mlp_grad(MlpDev& m, const float* feat, const float* target, int n, hipStream_t st) {
+    mlp_forward(m, feat, m.scratchY, n, m.store, st);
+    int lb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_mlp_loss, dim3(lb), dim3(256), 0, st, (const float*)m.store, target, m.dy, m.lossPart, n,
+                       m.overWeight);
+    int waves = (n + 15) / 16;
+    int blocks = std::min((waves + 3) / 4, 8192);
+    hipLaunchKernelGGL(k_mlp_backward, dim3(blocks), dim3(256), 0, st, (const bf16x8*)m.bwd,
+                       (const float*)m.store, (const float*)m.dy, m.dz, n);
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3(MLP_WGRAD_TILES), dim3(64), 0, st, feat, (const float*)m.store,
+                       (const float*)m.dz, m.grad, n, 1.0f);
+}
+
+void mlp_adam(MlpDev& m, float lr, hipStream_t st) {
+    m.adamStep += 1;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    float c1 = 1.0f - std::pow(b1, (float)m.adamStep), c2 = 1.0f - std::pow(b2, (float)m.adamStep);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_PARAMS + 255) / 256), dim3(256), 0, st, m.params, m.adamM, m.adamV,
+                       (const float*)m.grad, lr, b1, b2, eps, c1, c2);
+    mlp_pack(m, st);
+}
+
+// Glorot-uniform weights, zero biases, from a 64-bit LCG stream
+void mlp_init_params(std::vector<float>& p, unsigned long long seed) {
+    p.assign(MLP_PARAMS, 0.0f);
+    unsigned long long s = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+    auto uni = [&]() {
+        s = s * 6364136223846793005ULL + 1442695040888963407ULL;
+        return double(s >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    };
+    const int off[4] = {OFF_W1, OFF_W2, OFF_W3, OFF_W4}, in[4] = {8, 64, 64, 64}, out[4] = {64, 64, 64, 1};
+    for (int L = 0; L < 4; ++L) {
+        double a = std::sqrt(6.0 / double(in[L] + out[L]));
+        for (int k = 0; k < in[L] * out[L]; ++k) p[off[L] + k] = float(a * uni());
+    }
+}
+
+int mlp_alloc(MlpDev& m, int cap) {
+    m.cap = std::max(cap, 16);
+    auto al = [](void** p, size_t b) { return hipMalloc(p, std::max<size_t>(b, 256)) == hipSuccess; };
+    bool ok = al((void**)&m.params, sizeof(float) * MLP_PARAMS) && al((void**)&m.adamM, sizeof(float) * MLP_PARAMS) &&
+              al((void**)&m.adamV, sizeof(float) * MLP_PARAMS) && al((void**)&m.grad, sizeof(float) * MLP_PARAMS) &&
+              al(&m.fwd, 16 * 64 * MLP_FWD_FRAGS) && al(&m.bwd, 16 * 64 * MLP_BWD_FRAGS) &&
+              al((void**)&m.store, sizeof(float) * MLP_STORE * (size_t)m.cap) &&
+              al((void**)&m.dz, sizeof(float) * MLP_STORE * (size_t)m.cap) &&
+              al((void**)&m.dy, sizeof(float) * m.cap) && al((void**)&m.scratchY, sizeof(float) * m.cap) &&
+              al((void**)&m.lossPart, sizeof(double) * (m.cap / 256 + 2));
+    if (!ok) return -1;
+    hipMemset(m.adamM, 0, sizeof(float) * MLP_PARAMS);
+    hipMemset(m.adamV, 0, sizeof(float) * MLP_PARAMS);
+    hipMemset(m.dz, 0, sizeof(float) * MLP_STORE * (size_t)m.cap);
+    m.adamStep = 0;
+    return 0;
+}
+
+void mlp_free(MlpDev& m) {
+    void* ps[] = {m.params, m.adamM, m.adamV, m.grad, m.fwd, m.bwd, m.store, m.dz, m.dy, m.scratchY, m.lossPart};
+    for (void* p : ps)
+        if (p) hipFree(p);
+    m = MlpDev{};
+}
+
+double mlp_loss_host(const MlpDev& m, int n, hipStream_t st) {
+    int lb = (n + 255) / 256;
+    std::vector<double> part(lb);
+    hipMemcpyAsync(part.data(), m.lossPart, sizeof(double) * lb, hipMemcpyDeviceToHost, st);
+    hipStreamSynchronize(st);
+    double s = 0.0;
+    for (double v : part) s += v;
+    return n > 0 ? s / n : 0.0;
+}
+
+}  // namespace sphgpu
+
+// ------------------------------------------------------------- C-ABI ----
+struct sphgpu_mlp {
+    sphgpu::MlpDev m;
+    hipStream_t st = nullptr;
+    float* feat = nullptr;
+    float* target = nullptr;
+    float* y = nullptr;
+};
+
+extern "C" int sphgpu_mlp_create(int capacity, unsigned long long seed, sphgpu_mlp** out) {
+    if (!out || capacity <= 0) return SPHGPU_E_ARG;
+    *out = nullptr;
+    if (!sphgpu_device_available()) return SPHGPU_E_DEVICE;
+    sphgpu_mlp* h = new sphgpu_mlp();
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess || sphgpu::mlp_alloc(h->m, capacity) ||
+        hipMalloc((void**)&h->feat, sizeof(float) * 8 * (size_t)h->m.cap) != hipSuccess ||
+        hipMalloc((void**)&h->target, sizeof(float) * (size_t)h->m.cap) != hipSuccess ||
+        hipMalloc((void**)&h->y, sizeof(float) * (size_t)h->m.cap) != hipSuccess) {
+        sphgpu_mlp_destroy(h);
+        return SPHGPU_E_DEVICE;
+    }
+    std::vector<float> p;
+    sphgpu::mlp_init_params(p, seed);
+    return sphgpu_mlp_set_params(h, p.data()) == 0 ? (*out = h, SPHGPU_OK) : (sphgpu_mlp_destroy(h), SPHGPU_E_DEVICE);
+}
+
+extern "C" void sphgpu_mlp_destroy(sphgpu_mlp* h) {
+    if (!h) return;
+    if (h->st) hipStreamSynchronize(h->st);
+    sphgpu::mlp_free(h->m);
+    if (h->feat) hipFree(h->feat);
+    if (h->target) hipFree(h->target);
+    if (h->y) hipFree(h->y);
+    if (h->st) hipStreamDestroy(h->st);
+    delete h;
+}
+
+extern "C" int sphgpu_mlp_set_params(sphgpu_mlp* h, const float* params) {
+    if (!h || !params) return SPHGPU_E_ARG;
+    if (hipMemcpyAsync(h->m.params, params, sizeof(float) * sphgpu::MLP_PARAMS, hipMemcpyHostToDevice, h->st) != hipSuccess)
+        return SPHGPU_E_DEVICE;
+    sphgpu::mlp_pack(h->m, h->st);
+    return hipStreamSynchronize(h->st) == hipSuccess ? SPHGPU_OK : SPHGPU_E_DEVICE;
+}
+
+extern "C" int sphgpu_mlp_get_params(sphgpu_mlp* h, float* params) {
+    if (!h || !params) return SPHGPU_E_ARG;
+    hipMemcpyAsync(params, h->m.params, sizeof(float) * sphgpu::MLP_PARAMS, hipMemcpyDeviceToHost, h->st);
+    return hipStreamSynchronize(h->st) == hipSuccess ? SPHGPU_OK : SPHGPU_E_DEVICE;
+}
+
+extern "C" int sphgpu_mlp_set_loss(sphgpu_mlp* h, float over_weight) {
+    if (!h || !(over_weight > 0.0f) || !std::isfinite(over_weight)) return SPHGPU_E_ARG;
+    h->m.overWeight = over_weight;
+    return SPHGPU_OK;
+}
+
+extern "C" int sphgpu_mlp_forward(sphgpu_mlp* h, long long n, const float* feat, float* out) {
+    if (!h || n < 0 || (n > 0 && (!feat || !out))) return SPHGPU_E_ARG;
+    for (long long o = 0; o < n; o += h->m.cap) {
+        int k = int(std::min<long long>(h->m.cap, n - o));
+        hipMemcpyAsync(h->feat, feat + 8 * o, sizeof(float) * 8 * k, hipMemcpyHostToDevice, h->st);
+        sphgpu::mlp_forward(h->m, h->feat, h->y, k, nullptr, h->st);
+        hipMemcpyAsync(out + o, h->y, sizeof(float) * k, hipMemcpyDeviceToHost, h->st);
+    }
+    return hipStreamSynchronize(h->st) == hipSuccess && hipGetLastError() == hipSuccess ? SPHGPU_OK : SPHGPU_E_DEVICE;
+}
+
+extern "C" int sphgpu_mlp_gradient(sphgpu_mlp* h, long long n, const float* feat, const float* target, float* grad,
+                                   double* loss) {
+    if (!h || n <= 0 || n > h->m.cap || !feat || !target || !grad) return SPHGPU_E_ARG;
+    int k = int(n);
+    hipMemcpyAsync(h->feat, feat, sizeof(float) * 8 * k, hipMemcpyHostToDevice, h->st);
+    hipMemcpyAsync(h->target, target, sizeof(float) * k, hipMemcpyHostToDevice, h->st);
+    sphgpu::mlp_grad(h->m, h->feat, h->target, k, h->st);
+    hipMemcpyAsync(grad, h->m.grad, sizeof(float) * sphgpu::MLP_PARAMS, hipMemcpyDeviceToHost, h->st);
+    double l = sphgpu::mlp_loss_host(h->m, k, h->st);
+    if (loss) *loss = l;
+    return hipGetLastError() == hipSuccess ? SPHGPU_OK : SPHGPU_E_DEVICE;
+}
+
+extern "C" int sphgpu_mlp_train(sphgpu_mlp* h, long long n, const float* feat, const float* target, int iters, float lr,
+                                double* loss) {
+    if (!h || n <= 0 || n > h->m.cap || !feat || !target || iters < 0) return SPHGPU_E_ARG;
+    int k = int(n);
+    hipMemcpyAsync(h->feat, feat, sizeof(float) * 8 * k, hipMemcpyHostToDevice, h->st);
+    hipMemcpyAsync(h->target, target, sizeof(float) * k, hipMemcpyHostToDevice, h->st);
+    for (int it = 0; it < iters; ++it) {
+        sphgpu::mlp_grad(h->m, h->feat, h->target, k, h->st);
+        sphgpu::mlp_adam(h->m, lr, h->st);
+    }
+    sphgpu::mlp_grad(h->m, h->feat, h->target, k, h->st);
+    double l = sphgpu::mlp_loss_host(h->m, k, h->st);
+    if (loss) *loss = l;
+    return hipGetLastError() == hipSuccess ? SPHGPU_OK : SPHGPU_E_DEVICE;
+}

@@ -379,25 +379,112 @@ __global__ __launch_bounds__(256) void k_df_aadv(Dev D) {
     D.ksum[i] = 0.0f;
 }
 
-// warm start of the divergence loop (Bender & Koschier; not in the reference):
-// half of the previous step's accumulated kappaV, only where the current
-// velocities still compress (Drho/Dt > 0); xk.w = kappaV0/rho feeds one
-// velocity correction, and the accumulator restarts from the applied guess.
-// Without warm start the accumulator restarts at zero.
-__global__ __launch_bounds__(256) void k_df_warm_div(Dev D, int warm) {
+// Warm start of the divergence loop (not in the reference).  The stiffness
+// the loop accumulates, kappaV*dt summed over iterations, is carried per
+// particle in kvsum; its dimensionless form
+//     s = kvsum dt / (alpha rho0) * 1000
+// (the relative density change per step the solve removed, in 1e-3) is what
+// the learned model predicts.
+//
+// Features of the learned model, per particle, at the start of the solve:
+//   0 (rho/rho0 - 1) * 100            density deviation
+//   1 Drho/Dt dt / rho0 * 1000        velocity divergence (with the current v)
+//   2 (fluid + boundary neighbours)/30
+//   3 previous step's s               previous stiffness
+//   4 alpha / alpha_lattice           alpha factor
+//   5 |v| dt / spacing
+//   6 distance to the nearest boundary particle / spacing (2 if none)
+//   7 dt / dt_max
+// The kernel always stores Drho/Dt in f2 for the gate of k_df_warm_apply.
+__global__ __launch_bounds__(256) void k_df_warm_features(Dev D, float* __restrict__ feat) {
     Slot s = slot_of(D, false);
     if (!s.live) return;
     const int i = s.i;
-    if (!warm) {
+    float4 pa = D.x[i], va = D.vs[i];
+    int ph = phase_of(pa);
+    float rho0 = D.rho0[ph];
+    int c = D.cnt[i];
+    float d = density_change(D, i, pa, va, D.mass[ph], rho0, c);
+    D.f2[i] = d;
+    if (!feat) return;
+    const float sp = D.spacing;
+    float bmin2 = 4.0f * sp * sp;
+    each_boundary(D, i, nb_of(c), [&](int j) {
+        float4 pb = D.xb[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        bmin2 = fminf(bmin2, dx * dx + dy * dy + dz * dz);
+    });
+    float dt = D.dt[s.scene], rho = D.rho[i], alpha = D.alpha[i];
+    float4* f = reinterpret_cast<float4*>(feat) + 2 * (size_t)i;
+    f[0] = make_float4((rho / rho0 - 1.0f) * 100.0f, d * dt / rho0 * 1000.0f,
+                       (float)(nf_of(c) + nb_of(c)) * (1.0f / 30.0f),
+                       D.kvsum[i] * dt / (alpha * rho0) * 1000.0f);
+    f[1] = make_float4(alpha / D.alpha_lat, sqrtf(va.x * va.x + va.y * va.y + va.z * va.z) * dt / sp,
+                       sqrtf(bmin2) / sp, dt / D.dt_max);
+}
+
+// mode 0: no warm start (accumulator restarts at zero); mode 1: half of the
+// previous step's accumulated stiffness (Bender & Koschier); mode 2: the
+// model's prediction y (the dimensionless s above).  Modes 1 and 2 apply
+// only where the current velocities still compress (Drho/Dt > 0); the guess
+// becomes xk.w = kappaV0/rho for one velocity correction, and the
+// accumulator restarts from it.
+__global__ __launch_bounds__(256) void k_df_warm_apply(Dev D, const float* __restrict__ y, int mode) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i;
+    if (mode == 0) {
         D.kvsum[i] = 0.0f;
         return;
     }
-    float4 pa = D.x[i], va = D.vs[i];
+    float4 pa = D.x[i];
     int ph = phase_of(pa);
-    float d = density_change(D, i, pa, va, D.mass[ph], D.rho0[ph], D.cnt[i]);
-    float kv = d > 0.0f ? 0.5f * D.kvsum[i] : 0.0f;
+    float dt = D.dt[s.scene];
+    float kv = 0.0f;
+    if (D.f2[i] > 0.0f) {
+        if (mode == 1) kv = 0.5f * D.kvsum[i];
+        else kv = fmaxf(y[i], 0.0f) * 1.0e-3f * D.alpha[i] * D.rho0[ph] / dt;
+    }
     D.kvsum[i] = kv;
-    D.xk[i] = make_float4(pa.x, pa.y, pa.z, kv / D.dt[s.scene] / D.rho[i]);
+    D.xk[i] = make_float4(pa.x, pa.y, pa.z, kv / dt / D.rho[i]);
+}
+
+// converged stiffness of the divergence solve in the model's units
+__global__ __launch_bounds__(256) void k_df_targets(Dev D, float* __restrict__ tgt) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i;
+    int ph = phase_of(D.x[i]);
+    tgt[i] = D.kvsum[i] * D.dt[s.scene] / (D.alpha[i] * D.rho0[ph]) * 1000.0f;
+}
+
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// minibatch of `n` live particles drawn uniformly (hashed slots, retried
+// until a live slot is hit); copies features and targets.
+__global__ __launch_bounds__(256) void k_mlp_gather(Dev D, const float* __restrict__ feat,
+                                                  const float* __restrict__ tgt, float* __restrict__ bfeat,
+                                                  float* __restrict__ btgt, int n, uint32_t seed) {
+    int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    int slot = -1;
+    for (int tries = 0; tries < 16 && slot < 0; ++tries) {
+        uint32_t r = hash32(seed * 0x9e3779b9u + (uint32_t)k * 16u + (uint32_t)tries);
+        int q = (int)(r % (uint32_t)D.npad);
+        int t = q / TILE;
+        if ((q - t * TILE) < D.tileValid[t] && D.id[q] >= 0) slot = q;
+    }
+    if (slot < 0) slot = 0;
+    const float4* f = reinterpret_cast<const float4*>(feat) + 2 * (size_t)slot;
+    float4* o = reinterpret_cast<float4*>(bfeat) + 2 * (size_t)k;
+    o[0] = f[0];
+    o[1] = f[1];
+    btgt[k] = tgt[slot];
 }
 
 // pass A of the density loop (DFSPH.cpp:572-643): rho*, clamp rho* >= rho0,
@@ -426,7 +513,11 @@ __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
 
 // pass 1 of the divergence loop (DFSPH.cpp:780-836): Drho/Dt, clamp,
 // kappaV = Drho/Dt alpha / dt -> xk.w = kappaV/rho; partial sum of Drho/Dt.
-__global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D) {
+// projected (warm-started solves): the accumulated stiffness kvsum is
+// clamped at zero instead of each increment, so a guess that was too large
+// is taken back where the fluid now expands; the error is the mean applied
+// |increment| / alpha.
+__global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected) {
     __shared__ double lds[4];
     Slot s = slot_of(D, true);
     double contrib = 0.0;
@@ -438,11 +529,22 @@ __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D) {
         float dd = density_change(D, i, pa, va, m, rho0, D.cnt[i]);
         float dt = D.dt[s.scene];
         float rhoa = D.rho[i];
-        if (dd < 0.0f || (rhoa + dd * dt) < rho0) dd = 0.0f;
-        float kv = dd * D.alpha[i] / dt;
-        D.xk[i] = make_float4(pa.x, pa.y, pa.z, kv / rhoa);
-        D.kvsum[i] += kv * dt;
-        contrib = (double)dd;
+        if (!projected) {
+            if (dd < 0.0f || (rhoa + dd * dt) < rho0) dd = 0.0f;
+            float kv = dd * D.alpha[i] / dt;
+            D.xk[i] = make_float4(pa.x, pa.y, pa.z, kv / rhoa);
+            D.kvsum[i] += kv * dt;
+            contrib = (double)dd;
+        } else {
+            float alpha = D.alpha[i], K = D.kvsum[i];
+            if (dd >= 0.0f && (rhoa + dd * dt) < rho0) dd = 0.0f;
+            float inc = dd * alpha;
+            if (inc < -K) inc = -K;
+            float kv = inc / dt;
+            D.xk[i] = make_float4(pa.x, pa.y, pa.z, kv / rhoa);
+            D.kvsum[i] = K + inc;
+            contrib = (double)(fabsf(inc) / alpha);
+        }
     }
     write_partial(D, s.tile, 0, contrib, lds);
 }
@@ -780,9 +882,20 @@ void launch_wc_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_wc_pressure
 void launch_wc_force_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_wc_force); }
 void launch_df_density_alpha(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_density_alpha); }
 void launch_df_aadv(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_aadv); }
-void launch_df_warm_div(const Dev& D, int warm, hipStream_t st) { SPH_LAUNCH(k_df_warm_div, warm); }
+void launch_df_warm_features(const Dev& D, float* feat, hipStream_t st) { SPH_LAUNCH(k_df_warm_features, feat); }
+void launch_df_warm_apply(const Dev& D, const float* y, int mode, hipStream_t st) {
+    SPH_LAUNCH(k_df_warm_apply, y, mode);
+}
+void launch_df_targets(const Dev& D, float* tgt, hipStream_t st) { SPH_LAUNCH(k_df_targets, tgt); }
+void launch_mlp_gather(const Dev& D, const float* feat, const float* tgt, float* bfeat, float* btgt, int n,
+                       uint32_t seed, hipStream_t st) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_mlp_gather, dim3((n + 255) / 256), dim3(256), 0, st, D, feat, tgt, bfeat, btgt, n, seed);
+}
 void launch_df_pass_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_pass_density); }
-void launch_df_pass_divergence(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_pass_divergence); }
+void launch_df_pass_divergence(const Dev& D, int projected, hipStream_t st) {
+    SPH_LAUNCH(k_df_pass_divergence, projected);
+}
 void launch_df_pass_velocity(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_pass_velocity); }
 void launch_df_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_integrate); }
 void launch_ii_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_density); }

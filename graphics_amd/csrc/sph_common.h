@@ -54,6 +54,9 @@ struct Dev {
     float Hs;          // smoothRadius_h (artificial viscosity length, WcSph.cpp:182)
     float visc_eps;    // 0.01*Hs*Hs (WcSph.cpp:184)
     float v0;          // spacing^3 (DFSPH.cpp:88)
+    float spacing;     // m_TH.spacing_r
+    float alpha_lat;   // DFSPH alpha of an interior lattice particle (feature scale)
+    float dt_max;
     float gx, gy, gz;
     float cs, gamma, nu, omega;
     float rho0[MAX_PHASES], mass[MAX_PHASES], visc[MAX_PHASES], tait_B[MAX_PHASES];
@@ -312,10 +315,13 @@ void launch_wc_force_integrate(const Dev& D, hipStream_t st);
 void launch_df_density_alpha(const Dev& D, hipStream_t st);
 void launch_df_aadv(const Dev& D, hipStream_t st);
 void launch_df_pass_density(const Dev& D, hipStream_t st);
-void launch_df_warm_div(const Dev& D, int warm, hipStream_t st);
-void launch_df_pass_divergence(const Dev& D, hipStream_t st);
+void launch_df_warm_features(const Dev& D, float* feat, hipStream_t st);
+void launch_df_warm_apply(const Dev& D, const float* y, int mode, hipStream_t st);
+void launch_df_targets(const Dev& D, float* tgt, hipStream_t st);
+void launch_mlp_gather(const Dev& D, const float* feat, const float* tgt, float* bfeat, float* btgt, int n,
+                       uint32_t seed, hipStream_t st);
+void launch_df_pass_divergence(const Dev& D, int projected, hipStream_t st);
 void launch_df_pass_velocity(const Dev& D, hipStream_t st);
-void launch_df_warm_apply(const Dev& D, hipStream_t st);
 void launch_df_integrate(const Dev& D, hipStream_t st);
 void launch_ii_density(const Dev& D, hipStream_t st);
 void launch_ii_aadv_dii(const Dev& D, hipStream_t st);

@@ -46,7 +46,12 @@ class Params(C.Structure):
         ("omega", C.c_float),
         ("warm_start", C.c_int),
         ("num_scenes", C.c_int),
-        ("reserved", C.c_int * 8),
+        ("learn_steps", C.c_int),
+        ("learn_iters", C.c_int),
+        ("learn_batch", C.c_int),
+        ("learn_rate", C.c_float),
+        ("learn_over_weight", C.c_float),
+        ("reserved", C.c_int * 3),
     ]
 
 
@@ -102,6 +107,16 @@ SIGNATURES = {
     "sphgpu_get_neighbours": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, _LP, _FP]),
     "sphgpu_profile_kernel": (C.c_int, [C.c_void_p, C.c_char_p]),
     "sphgpu_kernel_timing": (C.c_int, [C.c_void_p, _LP, _DP, _DP]),
+    "sphgpu_mlp_create": (C.c_int, [C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
+    "sphgpu_mlp_destroy": (None, [C.c_void_p]),
+    "sphgpu_mlp_set_params": (C.c_int, [C.c_void_p, _FP]),
+    "sphgpu_mlp_get_params": (C.c_int, [C.c_void_p, _FP]),
+    "sphgpu_mlp_forward": (C.c_int, [C.c_void_p, C.c_longlong, _FP, _FP]),
+    "sphgpu_mlp_gradient": (C.c_int, [C.c_void_p, C.c_longlong, _FP, _FP, _FP, _DP]),
+    "sphgpu_mlp_set_loss": (C.c_int, [C.c_void_p, C.c_float]),
+    "sphgpu_mlp_train": (C.c_int, [C.c_void_p, C.c_longlong, _FP, _FP, C.c_int, C.c_float, _DP]),
+    "sphgpu_get_mlp_params": (C.c_int, [C.c_void_p, _FP]),
+    "sphgpu_set_mlp_params": (C.c_int, [C.c_void_p, _FP]),
     "sphgpu_scene_lattice": (C.c_longlong, [_IP, _FP, C.c_float, C.c_float, C.c_ulonglong,
                                             C.c_ulonglong, _FP]),
     "sphgpu_scene_box": (C.c_longlong, [_FP, _FP, C.c_float, C.c_int, _FP]),
@@ -174,6 +189,72 @@ def box_walls(center, half, spacing, have_top=True) -> np.ndarray:
     out = np.empty((n, 3), np.float32)
     lib.sphgpu_scene_box(c3, h3, float(spacing), int(bool(have_top)), _fp(out))
     return out
+
+
+MLP_PARAMS = 8961
+
+
+class Mlp:
+    """Standalone per-particle MLP of the C-ABI (8 -> 64 -> 64 -> 64 -> 1)."""
+
+    def __init__(self, capacity: int, seed: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.sphgpu_mlp_create(int(capacity), int(seed), C.byref(h))
+        if rc != 0:
+            raise SphError(rc, "sphgpu_mlp_create failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.sphgpu_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise SphError(rc, what)
+
+    def set_params(self, p):
+        a = _f32(p)
+        assert a.size == MLP_PARAMS
+        self._check(self.lib.sphgpu_mlp_set_params(self.h, _fp(a)), "mlp_set_params")
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(MLP_PARAMS, np.float32)
+        self._check(self.lib.sphgpu_mlp_get_params(self.h, _fp(out)), "mlp_get_params")
+        return out
+
+    def forward(self, feat) -> np.ndarray:
+        f = _f32(feat).reshape(-1, 8)
+        out = np.empty(len(f), np.float32)
+        self._check(self.lib.sphgpu_mlp_forward(self.h, len(f), _fp(f), _fp(out)), "mlp_forward")
+        return out
+
+    def gradient(self, feat, target):
+        f = _f32(feat).reshape(-1, 8)
+        t = _f32(target)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        self._check(self.lib.sphgpu_mlp_gradient(self.h, len(f), _fp(f), _fp(t), _fp(g), C.byref(loss)),
+                    "mlp_gradient")
+        return g, loss.value
+
+    def set_loss(self, over_weight: float):
+        self._check(self.lib.sphgpu_mlp_set_loss(self.h, float(over_weight)), "mlp_set_loss")
+
+    def train(self, feat, target, iters, lr):
+        f = _f32(feat).reshape(-1, 8)
+        t = _f32(target)
+        loss = C.c_double()
+        self._check(self.lib.sphgpu_mlp_train(self.h, len(f), _fp(f), _fp(t), int(iters), float(lr),
+                                              C.byref(loss)), "mlp_train")
+        return loss.value
 
 
 class Simulation:
@@ -278,6 +359,16 @@ class Simulation:
             self.h, cap, counts.ctypes.data_as(_IP), kinds.ctypes.data_as(_IP),
             ids.ctypes.data_as(_LP), _fp(dis)), "get_neighbours")
         return counts, kinds, ids, dis
+
+    # -- learned warm start
+    def mlp_params(self) -> np.ndarray:
+        out = np.empty(MLP_PARAMS, np.float32)
+        self._check(self.lib.sphgpu_get_mlp_params(self.h, _fp(out)), "get_mlp_params")
+        return out
+
+    def set_mlp_params(self, p):
+        a = _f32(p)
+        self._check(self.lib.sphgpu_set_mlp_params(self.h, _fp(a)), "set_mlp_params")
 
     # -- measurement
     def profile_kernel(self, name):

@@ -77,7 +77,12 @@ typedef struct sphgpu_params {
     float omega;               /* IISPH relaxation, 0.5 (IISph_M.cpp:341) */
     int   warm_start;          /* sphgpu_warm_start */
     int   num_scenes;          /* >= 1 independent scenes advanced together */
-    int   reserved[8];
+    int   learn_steps;         /* learned warm start: steps of online training before use */
+    int   learn_iters;         /* Adam iterations per training step */
+    int   learn_batch;         /* samples per Adam iteration */
+    float learn_rate;          /* Adam learning rate */
+    float learn_over_weight;   /* loss weight of over-predictions (>= 1 biases the guess low) */
+    int   reserved[3];
 } sphgpu_params;
 
 typedef struct sphgpu_stats {
@@ -176,6 +181,38 @@ int sphgpu_get_neighbours(sphgpu_handle* h, int cap, int* counts, int* kinds,
 int sphgpu_profile_kernel(sphgpu_handle* h, const char* name);
 int sphgpu_kernel_timing(sphgpu_handle* h, long long* launches, double* total_ms,
                          double* total_alg_bytes);
+
+/* ---- learned warm start ------------------------------------------------ */
+/* Per-particle MLP 8 -> 64 -> 64 -> 64 -> 1 (tanh-form GELU), BF16 weights on
+ * the matrix cores with FP32 accumulation.  In a DFSPH handle with
+ * warm_start = SPHGPU_WARM_LEARNED it predicts the converged stiffness of the
+ * divergence-free solve from 8 local features (density deviation, velocity
+ * divergence, neighbour count, previous stiffness, alpha factor, |v|,
+ * boundary distance, dt); it is trained online on the solver's own converged
+ * values during the first learn_steps steps.  It is only the initial guess:
+ * the solver still iterates to the same tolerance.  No counterpart in the
+ * reference. */
+#define SPHGPU_MLP_PARAMS 8961   /* W1[64][8] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[1][64] b4[1] */
+typedef struct sphgpu_mlp sphgpu_mlp;
+/* standalone network (tests, offline training); capacity = max samples per call */
+int  sphgpu_mlp_create(int capacity, unsigned long long seed, sphgpu_mlp** out);
+void sphgpu_mlp_destroy(sphgpu_mlp* m);
+int  sphgpu_mlp_set_params(sphgpu_mlp* m, const float* params);
+int  sphgpu_mlp_get_params(sphgpu_mlp* m, float* params);
+/* out[i] = MLP(feat[8i .. 8i+7]) */
+int  sphgpu_mlp_forward(sphgpu_mlp* m, long long n, const float* feat, float* out);
+/* gradient of mean((MLP(feat) - target)^2) w.r.t. the parameters; *loss = that mean */
+int  sphgpu_mlp_gradient(sphgpu_mlp* m, long long n, const float* feat, const float* target,
+                         float* grad, double* loss);
+/* loss of sphgpu_mlp_gradient / _train: mean(w e^2), e = out - target,
+ * w = over_weight where e > 0 and 1 elsewhere (default 1: mean squared error) */
+int  sphgpu_mlp_set_loss(sphgpu_mlp* m, float over_weight);
+/* `iters` Adam steps on the full set (n <= capacity); *loss = final mean squared error */
+int  sphgpu_mlp_train(sphgpu_mlp* m, long long n, const float* feat, const float* target,
+                      int iters, float lr, double* loss);
+/* network of a simulation handle (SPHGPU_MLP_PARAMS floats) */
+int  sphgpu_get_mlp_params(sphgpu_handle* h, float* params);
+int  sphgpu_set_mlp_params(sphgpu_handle* h, const float* params);
 
 /* ---- scene generation (host only; the reference's samplers) ------------ */
 /* Cubic lattice block of nx*ny*nz particles, x fastest (SphWithIO.cpp:164-192),

@@ -710,6 +710,10 @@ public:
         for (auto& f : fluids) for (auto& p : f.parts) p.v_adv = p.v;
         if (P.warm_start == 1 && frame > 0) warmStart(1, dt);
         else for (auto& f : fluids) for (auto& p : f.parts) p.kappaV_sum = 0;
+        /* a warm-started solve is a projected Jacobi iteration on the
+         * accumulated stiffness; the plain solve keeps the reference's
+         * clamp of each increment */
+        const bool projected = P.warm_start != 0;
         double avg = 0; int it = 0;
         while (avg > double(P.divergence_tol) || it < P.min_iter_div) {
             if (it > P.max_iter_div) { st->max_iter_hit = 1; break; }   /* DFSPH.cpp:772-773 */
@@ -736,10 +740,23 @@ public:
                             dd += g.dot(a.v_adv - b.v) * (rho0 * b.volume);
                         }
                     }
-                    if (dd < 0 || (a.rho + dd * dt) < rho0) dd = 0;       /* DFSPH.cpp:829-832 */
-                    a.divdev = dd;
-                    a.kappaV = dd * a.alpha / dt;                          /* DFSPH.cpp:835 */
-                    a.kappaV_sum += a.kappaV * dt;
+                    if (!projected) {
+                        if (dd < 0 || (a.rho + dd * dt) < rho0) dd = 0;   /* DFSPH.cpp:829-832 */
+                        a.divdev = dd;
+                        a.kappaV = dd * a.alpha / dt;                      /* DFSPH.cpp:835 */
+                        a.kappaV_sum += a.kappaV * dt;
+                    } else {
+                        /* clamp the accumulated stiffness at zero instead of
+                         * each increment: an over-large initial guess is
+                         * taken back where the fluid now expands */
+                        T rr = dd;
+                        if (dd >= 0 && (a.rho + dd * dt) < rho0) rr = 0;
+                        T inc = rr * a.alpha;
+                        if (inc < -a.kappaV_sum) inc = -a.kappaV_sum;
+                        a.kappaV = inc / dt;
+                        a.kappaV_sum += inc;
+                        a.divdev = std::abs(inc) / a.alpha;
+                    }
                 }
                 for (int i = 0; i < n; ++i) sum += double(ps[i].divdev);
                 ntot += n;
