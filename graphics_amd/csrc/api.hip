@@ -84,6 +84,7 @@ struct sphgpu_handle {
     float* bfeat = nullptr;            // learn_batch x 8 minibatch
     float* btgt = nullptr;
     int learned = 0;                   // training steps done
+    int lastIters[2] = {0, 0};         // iterations of the previous solve (density, divergence)
     uint32_t gatherSeed = 1;
     // profiling
     std::string profName;
@@ -157,6 +158,13 @@ struct ProfScope {
 
 double nf_live(const sphgpu_handle* h) { return double(h->nLive); }
 
+// Glorot weights with a zero output layer: the warm-start model predicts a
+// change from the previous stiffness, so an untrained network predicts none.
+void init_handle_mlp(std::vector<float>& p) {
+    mlp_init_params(p, 0x5eedULL);
+    for (int k = MLP_PARAMS - 65; k < MLP_PARAMS; ++k) p[k] = 0.0f;   // W4[1][64], b4
+}
+
 // ---- neighbour search pipeline ------------------------------------------
 int run_ns(sphgpu_handle* h) {
     Dev& D = h->D;
@@ -208,13 +216,16 @@ int sync_active(sphgpu_handle* h, bool* any) {
 
 // one solver loop; records per-scene iterations/errors into field `which`
 // (0 density, 1 divergence) of sceneStats.
-int solver_loop(sphgpu_handle* h, int which) {
+//
+// Iterations are launched speculatively: every pass and the reduction skip
+// scenes whose device-side `active` flag is cleared, so iterations launched
+// past convergence cost only their launches.  The host reads the flags
+// after a first batch sized from the previous step's count, then after
+// every second iteration.
+void launch_iteration(sphgpu_handle* h, int which, int it) {
     Dev& D = h->D;
-    int ns = D.nscenes;
-    launch_fill_i32(D.active, 1, ns, h->st);
-    launch_fill_i32(D.iters, 0, ns, h->st);
     const sphgpu_params& P = h->P;
-    for (;;) {
+    {
         if (P.solver == SPHGPU_DFSPH) {
             if (which == 0) {
                 {
@@ -230,7 +241,7 @@ int solver_loop(sphgpu_handle* h, int which) {
             } else {
                 {
                     ProfScope ps(h, "df_pass_divergence", nf_live(h) * 36.0);
-                    launch_df_pass_divergence(D, P.warm_start != SPHGPU_WARM_NONE, h->st);
+                    launch_df_pass_divergence(D, P.warm_start != SPHGPU_WARM_NONE && it == 0, h->st);
                 }
                 {
                     ProfScope ps(h, "df_pass_velocity", nf_live(h) * 44.0);
@@ -250,11 +261,27 @@ int solver_loop(sphgpu_handle* h, int which) {
             launch_scene_reduce(D, 1, P.density_tol, P.min_iter_density,
                                 P.max_iter_density > 0 ? P.max_iter_density : 1 << 30, h->st);
         }
+    }
+}
+
+int solver_loop(sphgpu_handle* h, int which) {
+    Dev& D = h->D;
+    int ns = D.nscenes;
+    launch_fill_i32(D.active, 1, ns, h->st);
+    launch_fill_i32(D.iters, 0, ns, h->st);
+    const int minIt = which == 0 || h->P.solver != SPHGPU_DFSPH ? h->P.min_iter_density : h->P.min_iter_div;
+    int batch = std::max({1, minIt, h->lastIters[which] - 1});
+    int maxIt = 0, launched = 0;
+    for (;;) {
+        for (int q = 0; q < batch; ++q) launch_iteration(h, which, launched++);
         bool any = false;
         int rc = sync_active(h, &any);
         if (rc) return rc;
         if (!any) break;
+        batch = 2;
     }
+    for (int s = 0; s < ns; ++s) maxIt = std::max(maxIt, h->hIters[s]);
+    h->lastIters[which] = maxIt;
     for (int s = 0; s < ns; ++s) {
         if (which == 0) {
             h->sceneStats[s].iter_density = h->hIters[s];
@@ -369,7 +396,7 @@ int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
         ProfScope ps(h, "mlp_forward", nf_live(h) * 36.0);
         mlp_forward(h->mlp, h->feat, h->ypred, D.npad, nullptr, h->st);
     }
-    launch_df_warm_apply(D, h->ypred, mode, h->st);
+    launch_df_warm_apply(D, h->ypred, h->feat, mode, h->st);
     if (mode != 0) {
         launch_fill_i32(D.active, 1, D.nscenes, h->st);
         launch_df_pass_velocity(D, h->st);
@@ -377,7 +404,7 @@ int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
     if ((rc = solver_loop(h, 1))) return rc;
     if (training && h->frame > 0) {
         // online training on this step's converged stiffness
-        launch_df_targets(D, h->tgt, h->st);
+        launch_df_targets(D, h->feat, h->tgt, h->st);
         const int b = h->P.learn_batch;
         for (int it = 0; it < h->P.learn_iters; ++it) {
             launch_mlp_gather(D, h->feat, h->tgt, h->bfeat, h->btgt, b, h->gatherSeed++, h->st);
@@ -490,10 +517,11 @@ void sphgpu_default_params(sphgpu_params* p) {
     p->warm_start = SPHGPU_WARM_NONE;
     p->num_scenes = 1;
     p->learn_steps = 40;
-    p->learn_iters = 16;
+    p->learn_iters = 32;
     p->learn_batch = 16384;
     p->learn_rate = 1e-3f;
     p->learn_over_weight = 1.0f;
+    p->warm_scale = 0.5f;
 }
 
 int sphgpu_device_available(void) {
@@ -516,6 +544,7 @@ int sphgpu_create(const sphgpu_params* p, sphgpu_handle** out) {
          p->learn_batch > (1 << 22) || !(p->learn_rate >= 0) || !(p->learn_over_weight > 0) ||
          !std::isfinite(p->learn_over_weight)))
         return SPHGPU_E_ARG;
+    if (!(p->warm_scale >= 0.0f) || !(p->warm_scale <= 4.0f)) return SPHGPU_E_ARG;
     sphgpu_handle* h = new sphgpu_handle();
     h->P = *p;
     h->scenes.resize(p->num_scenes);
@@ -670,6 +699,7 @@ int sphgpu_init(sphgpu_handle* h) {
     D.v0 = s * s * s;
     D.spacing = s;
     D.dt_max = P.dt;
+    D.warm_scale = P.warm_scale;
     {
         // alpha of a particle inside a lattice of the sampling spacing
         // (DFSPH.cpp:541-566 on a full neighbourhood): rho / sum (m |grad W|)^2
@@ -737,7 +767,7 @@ int sphgpu_init(sphgpu_handle* h) {
         if (mlp_alloc(h->mlp, P.learn_batch)) return fail(h, SPHGPU_E_DEVICE, "MLP allocation failed");
         h->mlpLive = true;
         h->mlp.overWeight = P.learn_over_weight;
-        if (h->mlpHost.empty()) mlp_init_params(h->mlpHost, 0x5eedULL);
+        if (h->mlpHost.empty()) init_handle_mlp(h->mlpHost);
         HIPCHECK(h, hipMemcpy(h->mlp.params, h->mlpHost.data(), sizeof(float) * MLP_PARAMS, hipMemcpyHostToDevice));
         mlp_pack(h->mlp, h->st);
     }
@@ -897,6 +927,7 @@ int sphgpu_get_fluid(sphgpu_handle* h, int field, float* out) {
         case SPHGPU_F_ALPHA: src = D.alpha; break;
         case SPHGPU_F_KAPPA_SUM: src = D.ksum; break;
         case SPHGPU_F_PHASE: src = D.x; f4 = true; comp = 1; break;
+        case SPHGPU_F_KAPPAV_SUM: src = D.kvsum; break;
         default: return fail(h, SPHGPU_E_ARG, "unknown field %d", field);
     }
     if (f4) {
@@ -1020,7 +1051,7 @@ int sphgpu_get_mlp_params(sphgpu_handle* h, float* params) {
         HIPCHECK(h, hipStreamSynchronize(h->st));
         return SPHGPU_OK;
     }
-    if (h->mlpHost.empty()) mlp_init_params(h->mlpHost, 0x5eedULL);
+    if (h->mlpHost.empty()) init_handle_mlp(h->mlpHost);
     std::memcpy(params, h->mlpHost.data(), sizeof(float) * MLP_PARAMS);
     return SPHGPU_OK;
 }

@@ -383,14 +383,16 @@ __global__ __launch_bounds__(256) void k_df_aadv(Dev D) {
 // the loop accumulates, kappaV*dt summed over iterations, is carried per
 // particle in kvsum; its dimensionless form
 //     s = kvsum dt / (alpha rho0) * 1000
-// (the relative density change per step the solve removed, in 1e-3) is what
-// the learned model predicts.
+// (the relative density change per step the solve removed, in 1e-3) is
+// heavy-tailed, so the model works with u = log(1 + s), and it is residual:
+// it predicts the change y = u - u_prev from the previous step's u_prev
+// (feature 3), so an untrained network starts from the previous stiffness.
 //
 // Features of the learned model, per particle, at the start of the solve:
 //   0 (rho/rho0 - 1) * 100            density deviation
-//   1 Drho/Dt dt / rho0 * 1000        velocity divergence (with the current v)
+//   1 asinh(Drho/Dt dt / rho0 * 1000) velocity divergence (with the current v)
 //   2 (fluid + boundary neighbours)/30
-//   3 previous step's s               previous stiffness
+//   3 log(1 + previous step's s)      previous stiffness
 //   4 alpha / alpha_lattice           alpha factor
 //   5 |v| dt / spacing
 //   6 distance to the nearest boundary particle / spacing (2 if none)
@@ -416,20 +418,24 @@ __global__ __launch_bounds__(256) void k_df_warm_features(Dev D, float* __restri
     });
     float dt = D.dt[s.scene], rho = D.rho[i], alpha = D.alpha[i];
     float4* f = reinterpret_cast<float4*>(feat) + 2 * (size_t)i;
-    f[0] = make_float4((rho / rho0 - 1.0f) * 100.0f, d * dt / rho0 * 1000.0f,
+    f[0] = make_float4((rho / rho0 - 1.0f) * 100.0f, asinhf(d * dt / rho0 * 1000.0f),
                        (float)(nf_of(c) + nb_of(c)) * (1.0f / 30.0f),
-                       D.kvsum[i] * dt / (alpha * rho0) * 1000.0f);
+                       log1pf(fmaxf(D.kvsum[i] * dt / (alpha * rho0) * 1000.0f, 0.0f)));
     f[1] = make_float4(alpha / D.alpha_lat, sqrtf(va.x * va.x + va.y * va.y + va.z * va.z) * dt / sp,
                        sqrtf(bmin2) / sp, dt / D.dt_max);
 }
 
-// mode 0: no warm start (accumulator restarts at zero); mode 1: half of the
-// previous step's accumulated stiffness (Bender & Koschier); mode 2: the
-// model's prediction y (the dimensionless s above).  Modes 1 and 2 apply
-// only where the current velocities still compress (Drho/Dt > 0); the guess
+// mode 0: no warm start (accumulator restarts at zero); mode 1: warm_scale
+// times the previous step's accumulated stiffness K_prev (Bender & Koschier
+// use one half); mode 2: warm_scale times the model's prediction
+// (s = exp(u_prev + y) - 1), bounded by max(K_prev, K_first), K_first = Drho/Dt alpha
+// being the first increment the solve would make: the model may lower the
+// guess anywhere but raise it only as far as one of those two.  Modes 1
+// and 2 apply only where the solve itself would apply stiffness; the guess
 // becomes xk.w = kappaV0/rho for one velocity correction, and the
 // accumulator restarts from it.
-__global__ __launch_bounds__(256) void k_df_warm_apply(Dev D, const float* __restrict__ y, int mode) {
+__global__ __launch_bounds__(256) void k_df_warm_apply(Dev D, const float* __restrict__ y,
+                                                     const float* __restrict__ feat, int mode) {
     Slot s = slot_of(D, false);
     if (!s.live) return;
     const int i = s.i;
@@ -440,22 +446,32 @@ __global__ __launch_bounds__(256) void k_df_warm_apply(Dev D, const float* __res
     float4 pa = D.x[i];
     int ph = phase_of(pa);
     float dt = D.dt[s.scene];
-    float kv = 0.0f;
-    if (D.f2[i] > 0.0f) {
-        if (mode == 1) kv = 0.5f * D.kvsum[i];
-        else kv = fmaxf(y[i], 0.0f) * 1.0e-3f * D.alpha[i] * D.rho0[ph] / dt;
+    float kv = 0.0f, d = D.f2[i];
+    // only where the solve itself would apply stiffness (DFSPH.cpp:829-832)
+    if (d > 0.0f && !(D.rho[i] + d * dt < D.rho0[ph])) {
+        const float kprev = D.kvsum[i];
+        if (mode == 1) {
+            kv = D.warm_scale * kprev;
+        } else {
+            const float alpha = D.alpha[i];
+            float u = feat[8 * (size_t)i + 3] + y[i];
+            float kp = expm1f(fminf(fmaxf(u, 0.0f), 12.0f)) * 1.0e-3f * alpha * D.rho0[ph] / dt;
+            kv = D.warm_scale * fminf(kp, fmaxf(kprev, d * alpha));
+        }
     }
     D.kvsum[i] = kv;
     D.xk[i] = make_float4(pa.x, pa.y, pa.z, kv / dt / D.rho[i]);
 }
 
-// converged stiffness of the divergence solve in the model's units
-__global__ __launch_bounds__(256) void k_df_targets(Dev D, float* __restrict__ tgt) {
+// converged stiffness of the divergence solve in the model's units, as the
+// change from the previous step's (feature 3)
+__global__ __launch_bounds__(256) void k_df_targets(Dev D, const float* __restrict__ feat, float* __restrict__ tgt) {
     Slot s = slot_of(D, false);
     if (!s.live) return;
     const int i = s.i;
     int ph = phase_of(D.x[i]);
-    tgt[i] = D.kvsum[i] * D.dt[s.scene] / (D.alpha[i] * D.rho0[ph]) * 1000.0f;
+    tgt[i] = log1pf(fmaxf(D.kvsum[i] * D.dt[s.scene] / (D.alpha[i] * D.rho0[ph]) * 1000.0f, 0.0f)) -
+             feat[8 * (size_t)i + 3];
 }
 
 __device__ __forceinline__ uint32_t hash32(uint32_t x) {
@@ -513,10 +529,12 @@ __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
 
 // pass 1 of the divergence loop (DFSPH.cpp:780-836): Drho/Dt, clamp,
 // kappaV = Drho/Dt alpha / dt -> xk.w = kappaV/rho; partial sum of Drho/Dt.
-// projected (warm-started solves): the accumulated stiffness kvsum is
-// clamped at zero instead of each increment, so a guess that was too large
-// is taken back where the fluid now expands; the error is the mean applied
-// |increment| / alpha.
+// projected (first iteration of a warm-started solve): the accumulated
+// stiffness kvsum is clamped at zero instead of the increment, so a guess
+// that was too large is taken back where the fluid now expands; the error is
+// the mean applied |increment| / alpha.  Later iterations are the
+// reference's (projecting every iteration is unstable: signed Jacobi
+// oscillates).
 __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected) {
     __shared__ double lds[4];
     Slot s = slot_of(D, true);
@@ -536,10 +554,11 @@ __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected
             D.kvsum[i] += kv * dt;
             contrib = (double)dd;
         } else {
+            // below rest density the solve applies no stiffness: the whole
+            // guess is taken back
             float alpha = D.alpha[i], K = D.kvsum[i];
-            if (dd >= 0.0f && (rhoa + dd * dt) < rho0) dd = 0.0f;
             float inc = dd * alpha;
-            if (inc < -K) inc = -K;
+            if ((rhoa + dd * dt) < rho0 || inc < -K) inc = -K;
             float kv = inc / dt;
             D.xk[i] = make_float4(pa.x, pa.y, pa.z, kv / rhoa);
             D.kvsum[i] = K + inc;
@@ -883,10 +902,12 @@ void launch_wc_force_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_wc_f
 void launch_df_density_alpha(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_density_alpha); }
 void launch_df_aadv(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_aadv); }
 void launch_df_warm_features(const Dev& D, float* feat, hipStream_t st) { SPH_LAUNCH(k_df_warm_features, feat); }
-void launch_df_warm_apply(const Dev& D, const float* y, int mode, hipStream_t st) {
-    SPH_LAUNCH(k_df_warm_apply, y, mode);
+void launch_df_warm_apply(const Dev& D, const float* y, const float* feat, int mode, hipStream_t st) {
+    SPH_LAUNCH(k_df_warm_apply, y, feat, mode);
 }
-void launch_df_targets(const Dev& D, float* tgt, hipStream_t st) { SPH_LAUNCH(k_df_targets, tgt); }
+void launch_df_targets(const Dev& D, const float* feat, float* tgt, hipStream_t st) {
+    SPH_LAUNCH(k_df_targets, feat, tgt);
+}
 void launch_mlp_gather(const Dev& D, const float* feat, const float* tgt, float* bfeat, float* btgt, int n,
                        uint32_t seed, hipStream_t st) {
     if (n > 0)

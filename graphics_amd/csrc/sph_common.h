@@ -57,6 +57,7 @@ struct Dev {
     float spacing;     // m_TH.spacing_r
     float alpha_lat;   // DFSPH alpha of an interior lattice particle (feature scale)
     float dt_max;
+    float warm_scale;  // PREVIOUS warm start: scale of the previous stiffness
     float gx, gy, gz;
     float cs, gamma, nu, omega;
     float rho0[MAX_PHASES], mass[MAX_PHASES], visc[MAX_PHASES], tait_B[MAX_PHASES];
@@ -316,8 +317,8 @@ void launch_df_density_alpha(const Dev& D, hipStream_t st);
 void launch_df_aadv(const Dev& D, hipStream_t st);
 void launch_df_pass_density(const Dev& D, hipStream_t st);
 void launch_df_warm_features(const Dev& D, float* feat, hipStream_t st);
-void launch_df_warm_apply(const Dev& D, const float* y, int mode, hipStream_t st);
-void launch_df_targets(const Dev& D, float* tgt, hipStream_t st);
+void launch_df_warm_apply(const Dev& D, const float* y, const float* feat, int mode, hipStream_t st);
+void launch_df_targets(const Dev& D, const float* feat, float* tgt, hipStream_t st);
 void launch_mlp_gather(const Dev& D, const float* feat, const float* tgt, float* bfeat, float* btgt, int n,
                        uint32_t seed, hipStream_t st);
 void launch_df_pass_divergence(const Dev& D, int projected, hipStream_t st);

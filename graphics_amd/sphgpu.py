@@ -16,9 +16,9 @@ LIB_PATH = Path(__file__).resolve().parent / "libsphgpu.so"
 
 WCSPH, DFSPH, IISPH = 0, 1, 2
 WARM_NONE, WARM_PREVIOUS, WARM_LEARNED = 0, 1, 2
-F_POS, F_VEL, F_RHO, F_PRESSURE, F_ACC, F_ALPHA, F_KAPPA_SUM, F_PHASE = range(8)
+F_POS, F_VEL, F_RHO, F_PRESSURE, F_ACC, F_ALPHA, F_KAPPA_SUM, F_PHASE, F_KAPPAV_SUM = range(9)
 FIELD_COMPONENTS = {F_POS: 3, F_VEL: 3, F_RHO: 1, F_PRESSURE: 1, F_ACC: 3, F_ALPHA: 1,
-                    F_KAPPA_SUM: 1, F_PHASE: 1}
+                    F_KAPPA_SUM: 1, F_PHASE: 1, F_KAPPAV_SUM: 1}
 
 ERRORS = {0: "OK", -1: "E_ARG", -2: "E_STATE", -3: "E_DEVICE", -4: "E_CAPACITY", -5: "E_DIVERGED"}
 
@@ -51,7 +51,8 @@ class Params(C.Structure):
         ("learn_batch", C.c_int),
         ("learn_rate", C.c_float),
         ("learn_over_weight", C.c_float),
-        ("reserved", C.c_int * 3),
+        ("warm_scale", C.c_float),
+        ("reserved", C.c_int * 2),
     ]
 
 

@@ -82,7 +82,9 @@ typedef struct sphgpu_params {
     int   learn_batch;         /* samples per Adam iteration */
     float learn_rate;          /* Adam learning rate */
     float learn_over_weight;   /* loss weight of over-predictions (>= 1 biases the guess low) */
-    int   reserved[3];
+    float warm_scale;          /* the guess is warm_scale x the previous stiffness (PREVIOUS) or x the
+                                  bounded prediction (LEARNED); 0.5 as Bender & Koschier */
+    int   reserved[2];
 } sphgpu_params;
 
 typedef struct sphgpu_stats {
@@ -113,7 +115,8 @@ enum sphgpu_field {
     SPHGPU_F_ACC = 4,          /* 3  WCSPH acceleration / IISPH pressure acceleration */
     SPHGPU_F_ALPHA = 5,        /* 1  DFSPH alpha factor */
     SPHGPU_F_KAPPA_SUM = 6,    /* 1  accumulated kappa*dt^2 of the last density solve */
-    SPHGPU_F_PHASE = 7         /* 1  fluid phase index (as float) */
+    SPHGPU_F_PHASE = 7,        /* 1  fluid phase index (as float) */
+    SPHGPU_F_KAPPAV_SUM = 8    /* 1  accumulated kappaV*dt of the last divergence solve */
 };
 
 typedef struct sphgpu_handle sphgpu_handle;

@@ -598,10 +598,12 @@ public:
     }
 
     /* Warm start of a DFSPH loop (Bender & Koschier; not in the reference):
-     * half of the previous step's accumulated stiffness, only where the
-     * current prediction still compresses (rho* > rho0 for the density loop,
-     * Drho/Dt > 0 for the divergence loop), applied as one velocity
-     * correction.  The accumulator restarts from the applied guess. */
+     * a scaled copy of the previous step's accumulated stiffness (one half
+     * for the density loop, warm_scale for the divergence loop), only where
+     * the solve itself would apply stiffness (rho* > rho0 for the density
+     * loop; Drho/Dt > 0 and rho + dt Drho/Dt >= rho0 for the divergence
+     * loop), applied as one velocity correction.  The accumulator restarts
+     * from the applied guess. */
     void warmStart(int which, T dt) {
         std::vector<std::vector<T>> k0(fluids.size());
         for (size_t k = 0; k < fluids.size(); ++k) {
@@ -618,8 +620,9 @@ public:
                     a.kappa_sum = on ? T(0.5) * a.kappa_sum : T(0);
                     k0[k][i] = a.kappa_sum / (dt * dt);
                 } else {
-                    bool on = d > 0;
-                    a.kappaV_sum = on ? T(0.5) * a.kappaV_sum : T(0);
+                    /* where the solve itself would apply stiffness (DFSPH.cpp:829-832) */
+                    bool on = d > 0 && !(a.rho + d * dt < rho0);
+                    a.kappaV_sum = on ? T(P.warm_scale) * a.kappaV_sum : T(0);
                     k0[k][i] = a.kappaV_sum / dt;
                 }
             }
@@ -710,13 +713,16 @@ public:
         for (auto& f : fluids) for (auto& p : f.parts) p.v_adv = p.v;
         if (P.warm_start == 1 && frame > 0) warmStart(1, dt);
         else for (auto& f : fluids) for (auto& p : f.parts) p.kappaV_sum = 0;
-        /* a warm-started solve is a projected Jacobi iteration on the
-         * accumulated stiffness; the plain solve keeps the reference's
-         * clamp of each increment */
-        const bool projected = P.warm_start != 0;
         double avg = 0; int it = 0;
         while (avg > double(P.divergence_tol) || it < P.min_iter_div) {
             if (it > P.max_iter_div) { st->max_iter_hit = 1; break; }   /* DFSPH.cpp:772-773 */
+            /* The first iteration of a warm-started solve clamps the
+             * accumulated stiffness at zero rather than the increment, so
+             * an over-large guess is taken back where the fluid now
+             * expands; every later iteration is the reference's (increment
+             * >= 0).  Projecting every iteration is unstable (signed Jacobi
+             * oscillates), so only the guess is corrected this way. */
+            const bool projected = P.warm_start != 0 && it == 0;
             double sum = 0; int ntot = 0;
             for (int k = 0; k < int(fluids.size()); ++k) {
                 auto& ps = fluids[k].parts; int n = int(ps.size());
@@ -749,10 +755,10 @@ public:
                         /* clamp the accumulated stiffness at zero instead of
                          * each increment: an over-large initial guess is
                          * taken back where the fluid now expands */
-                        T rr = dd;
-                        if (dd >= 0 && (a.rho + dd * dt) < rho0) rr = 0;
-                        T inc = rr * a.alpha;
-                        if (inc < -a.kappaV_sum) inc = -a.kappaV_sum;
+                        /* below rest density the solve applies no stiffness:
+                         * the whole guess is taken back */
+                        T inc = dd * a.alpha;
+                        if ((a.rho + dd * dt) < rho0 || inc < -a.kappaV_sum) inc = -a.kappaV_sum;
                         a.kappaV = inc / dt;
                         a.kappaV_sum += inc;
                         a.divdev = std::abs(inc) / a.alpha;

@@ -42,7 +42,8 @@ typedef struct orc_params {
     int   max_iter_div;        /* DFSPH.cpp:764 (30, loop breaks when it > 30) */
     float omega;               /* IISPH relaxed Jacobi (IISph_M.cpp:341) */
     int   warm_start;          /* 0 none; 1 divergence loop warm-started from the previous step */
-    int   reserved[6];
+    float warm_scale;          /* scale of the previous step's stiffness in the guess */
+    int   reserved[5];
 } orc_params;
 
 typedef struct orc_stats {

@@ -41,4 +41,6 @@ def test_shim_steps_on_device(tmp_path, gpu):
     vals = dict(zip(tok[0::2], tok[1::2]))
     assert vals["frame"] == "3"
     assert vals["fluid"] == "216"
-    assert abs(float(vals["mean_rho"]) - 1000.0) < 50.0
+    # a 6^3 block: a third of its particles sit on the free surface
+    assert 800.0 < float(vals["mean_rho"]) < 1010.0
+    assert int(vals["iters"].split("/")[0]) >= 2
