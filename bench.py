@@ -43,6 +43,69 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+COPY_PROBE_BYTES = 1 << 30
+
+
+def traffic_file(workload: str) -> Path:
+    return ROOT / "profiles" / f"{workload}_traffic.txt"
+
+
+def read_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` measured by the FETCH_SIZE/WRITE_SIZE
+    passes recorded in profiles/<workload>_traffic.txt (None if absent)."""
+    path = traffic_file(workload)
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("{"):
+            rec = json.loads(line)
+            if rec.get("kernel") == "k_" + kernel:
+                return rec
+    return None
+
+
+def summarize_pmc(fetch_csv: str, write_csv: str, workload: str, command: str) -> int:
+    """Per-kernel median FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of two
+    rocprofv3 --pmc runs, scaled by what the counters read for the copy
+    probe (k_copy_probe, COPY_PROBE_BYTES read and written); writes
+    profiles/<workload>_traffic.txt."""
+    import csv
+    import collections
+
+    def load(path):
+        d = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            name = r["Kernel_Name"].replace("sphgpu::(anonymous namespace)::", "").split("(")[0]
+            dur = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            d[name].append((float(r["Counter_Value"]), dur, int(r["Grid_Size"])))
+        return d
+
+    f, w = load(fetch_csv), load(write_csv)
+    kib = COPY_PROBE_BYTES / 1024.0
+    probe_f = max(v[0] for v in f["k_copy_probe"])
+    probe_w = max(v[0] for v in w["k_copy_probe"])
+    cf, cw = kib / probe_f, kib / probe_w
+    lines = [f"# HBM traffic per dispatch, {workload}",
+             f"# command: {command}",
+             f"# copy probe {COPY_PROBE_BYTES} B: FETCH_SIZE {probe_f:.1f} KiB, WRITE_SIZE {probe_w:.1f} KiB;"
+             f" scale factors fetch {cf:.4f}, write {cw:.4f}",
+             "# dispatches that processed particles only (launches shorter than 10 us are skipped"
+             " iterations past convergence); medians over dispatches"]
+    for name in sorted(f, key=lambda k: -sum(v[0] for v in f[k])):
+        if name not in w or name == "k_copy_probe":
+            continue
+        fv = sorted(v[0] for v in f[name] if v[1] >= 10000) or sorted(v[0] for v in f[name])
+        wv = sorted(v[0] for v in w[name] if v[1] >= 10000) or sorted(v[0] for v in w[name])
+        med = lambda a: a[len(a) // 2]
+        rec = {"kernel": name, "dispatches": len(fv), "fetch_kib": round(med(fv), 1),
+               "write_kib": round(med(wv), 1),
+               "traffic_bytes": round((med(fv) * cf + med(wv) * cw) * 1024.0)}
+        lines.append(json.dumps(rec))
+    traffic_file(workload).write_text("\n".join(lines) + "\n")
+    print(traffic_file(workload))
+    return 0
+
+
 def build_scene(name: str):
     return scenes.WORKLOADS[name]()
 
@@ -117,7 +180,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-kernel", default=None)
+    ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COMMAND"),
+                    help="write profiles/<workload>_traffic.txt from two rocprofv3 --pmc runs")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        return summarize_pmc(*args.summarize_pmc[:2], build_scene(args.workload).name, args.summarize_pmc[2])
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no "
               "multi-GPU path", file=sys.stderr)
@@ -125,6 +192,12 @@ def main() -> int:
     if not sg.device_available():
         print("no gfx9 HIP device visible", file=sys.stderr)
         return 3
+
+    # plain copy of 1 GiB: counter calibration and the copy bandwidth this
+    # device reaches
+    probe_ms = sg.copy_probe(COPY_PROBE_BYTES)
+    copy_gbs = 2.0 * COPY_PROBE_BYTES / (probe_ms * 1e-3) / 1e9
+    log(f"[bench] copy probe: {COPY_PROBE_BYTES >> 20} MiB in {probe_ms:.3f} ms = {copy_gbs:.0f} GB/s (read+write)")
 
     sc = build_scene(args.workload)
     solver = sc.params["solver"]
@@ -148,10 +221,14 @@ def main() -> int:
         avg_s = tot_ms / launches * 1e-3
         alg = ALG_BYTES[prof] * nf
         achieved = alg / avg_s / 1e9
+        tr = read_traffic(sc.name, prof)
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                    "traffic": tr["traffic_bytes"] if tr else None,
                     "kernel": prof, "avg_us": round(avg_s * 1e6, 3), "launches": launches,
-                    "alg_bytes_per_launch": alg}
+                    "alg_bytes_per_launch": alg,
+                    "traffic_achieved_GBs": round(tr["traffic_bytes"] / avg_s / 1e9, 1) if tr else None,
+                    "copy_probe_GBs": round(copy_gbs, 1)}
 
     iters = {"warm_start": args.warm_start, "density": it_d, "divergence": it_v}
     if not args.no_ab and warm != sg.WARM_NONE:

@@ -30,6 +30,8 @@ struct Phase {
 struct ProfEvent {
     hipEvent_t a, b;
     double bytes;
+    int iter;      // solver iteration the launch belongs to (-1 outside loops)
+    bool skip;     // launched past convergence: no particle was processed
 };
 
 }  // namespace
@@ -85,6 +87,7 @@ struct sphgpu_handle {
     float* btgt = nullptr;
     int learned = 0;                   // training steps done
     int lastIters[2] = {0, 0};         // iterations of the previous solve (density, divergence)
+    int curIter = -1;                  // iteration being launched (profiling)
     uint32_t gatherSeed = 1;
     // profiling
     std::string profName;
@@ -146,6 +149,8 @@ struct ProfScope {
         }
         ProfEvent& e = h->prof[h->profUsed];
         e.bytes = bytes;
+        e.iter = h->curIter;
+        e.skip = false;
         hipEventRecord(e.a, h->st);
         on = true;
     }
@@ -225,6 +230,7 @@ int sync_active(sphgpu_handle* h, bool* any) {
 void launch_iteration(sphgpu_handle* h, int which, int it) {
     Dev& D = h->D;
     const sphgpu_params& P = h->P;
+    h->curIter = it;
     {
         if (P.solver == SPHGPU_DFSPH) {
             if (which == 0) {
@@ -272,6 +278,7 @@ int solver_loop(sphgpu_handle* h, int which) {
     const int minIt = which == 0 || h->P.solver != SPHGPU_DFSPH ? h->P.min_iter_density : h->P.min_iter_div;
     int batch = std::max({1, minIt, h->lastIters[which] - 1});
     int maxIt = 0, launched = 0;
+    const size_t profStart = h->profUsed;
     for (;;) {
         for (int q = 0; q < batch; ++q) launch_iteration(h, which, launched++);
         bool any = false;
@@ -282,6 +289,9 @@ int solver_loop(sphgpu_handle* h, int which) {
     }
     for (int s = 0; s < ns; ++s) maxIt = std::max(maxIt, h->hIters[s]);
     h->lastIters[which] = maxIt;
+    h->curIter = -1;
+    // launches past the last iteration of every scene processed nothing
+    for (size_t k = profStart; k < h->profUsed; ++k) h->prof[k].skip = h->prof[k].iter >= maxIt;
     for (int s = 0; s < ns; ++s) {
         if (which == 0) {
             h->sceneStats[s].iter_density = h->hIters[s];
@@ -391,7 +401,7 @@ int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
         if (warm || (learnedMode && training)) mode = 1;
         else if (learnedMode) mode = 2;
     }
-    launch_df_warm_features(D, learnedMode ? h->feat : nullptr, h->st);
+    if (mode != 0) launch_df_warm_features(D, learnedMode ? h->feat : nullptr, h->st);
     if (mode == 2) {
         ProfScope ps(h, "mlp_forward", nf_live(h) * 36.0);
         mlp_forward(h->mlp, h->feat, h->ypred, D.npad, nullptr, h->st);
@@ -1020,6 +1030,37 @@ int sphgpu_get_neighbours(sphgpu_handle* h, int cap, int* counts, int* kinds, lo
     return SPHGPU_OK;
 }
 
+int sphgpu_copy_probe(long long bytes, double* ms) {
+    if (bytes < 16 || bytes > (1LL << 32) || !ms) return SPHGPU_E_ARG;
+    if (!sphgpu_device_available()) return SPHGPU_E_DEVICE;
+    size_t n = size_t(bytes) / 16;
+    float4 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SPHGPU_E_DEVICE;
+    if (hipMalloc(&a, n * 16) == hipSuccess && hipMalloc(&b, n * 16) == hipSuccess &&
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+        hipEventCreate(&e1) == hipSuccess) {
+        launch_fill_i32(reinterpret_cast<int*>(a), 1, int(std::min<size_t>(n * 4, 1u << 30)), st);
+        launch_copy_probe(a, b, n, st);
+        hipEventRecord(e0, st);
+        launch_copy_probe(a, b, n, st);
+        hipEventRecord(e1, st);
+        float t = 0.0f;
+        if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&t, e0, e1) == hipSuccess &&
+            hipGetLastError() == hipSuccess) {
+            *ms = t;
+            rc = SPHGPU_OK;
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    return rc;
+}
+
 int sphgpu_profile_kernel(sphgpu_handle* h, const char* name) {
     if (!h) return SPHGPU_E_ARG;
     h->profName = name ? name : "";
@@ -1031,13 +1072,16 @@ int sphgpu_kernel_timing(sphgpu_handle* h, long long* launches, double* total_ms
     if (!h) return SPHGPU_E_ARG;
     if (h->initialized) HIPCHECK(h, hipStreamSynchronize(h->st));
     double ms = 0, bytes = 0;
+    long long n = 0;
     for (size_t k = 0; k < h->profUsed; ++k) {
+        if (h->prof[k].skip) continue;
         float t = 0;
         HIPCHECK(h, hipEventElapsedTime(&t, h->prof[k].a, h->prof[k].b));
         ms += t;
         bytes += h->prof[k].bytes;
+        ++n;
     }
-    if (launches) *launches = (long long)h->profUsed;
+    if (launches) *launches = n;
     if (total_ms) *total_ms = ms;
     if (total_bytes) *total_bytes = bytes;
     return SPHGPU_OK;

@@ -163,56 +163,91 @@ __device__ __forceinline__ int2 run3(const int2* __restrict__ table, uint32_t k0
     return make_int2(s, e);
 }
 
+// Lists are built in LDS, one column per thread, and written out group by
+// group (four entries, 16 bytes per particle), so each store instruction of a
+// wave writes whole cache lines.  Entries beyond the staged 48 (rare) are
+// written to global memory directly.
+constexpr int NB_STAGE_GROUPS = 12;
+
 __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    int scene;
-    if (i >= D.npad) return;
-    if (!slot_valid(D, i, &scene)) {
-        D.cnt[i] = 0;
-        return;
-    }
-    const SceneGrid G = D.grids[scene];
-    float4 pa = D.x[i];
-    int cx, cy, cz;
-    cell_of(G, pa, cx, cy, cz);
-    int nf = 0, nbn = 0, over = 0, zero = 0;
+    __shared__ int4 stage[NB_STAGE_GROUPS][256];
+    const int tid = threadIdx.x;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * 256 + tid;
+    int scene = 0;
+    const bool act = i < D.npad && slot_valid(D, i, &scene);
+    int* st = reinterpret_cast<int*>(&stage[0][0]);
     const int stride = D.ncap;
-    for (int dz = -1; dz <= 1; ++dz) {
-        int z = cz + dz;
-        if (z < 0 || z >= G.nz) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            int y = cy + dy;
-            if (y < 0 || y >= G.ny) continue;
-            uint32_t k0 = key_of(G, D.rowRank, cx, y, z);
-            int2 rf = run3(D.cellF, k0, cx, G.nx);
-            for (int j = rf.x; j < rf.y; ++j) {
-                if (j == i) continue;
-                float4 pb = D.x[j];
-                float l2 = dis_square_exact(pa, pb);
-                if (l2 < D.h2) {
-                    if (nf < NBF_CAP) D.nbF[nf * stride + i] = j;
-                    else ++over;
-                    ++nf;
-                    zero += (l2 == 0.0f);
-                }
-            }
-            int2 rb = run3(D.cellB, k0, cx, G.nx);
-            for (int j = rb.x; j < rb.y; ++j) {
-                float4 pb = D.xb[j];
-                float l2 = dis_square_exact(pa, pb);
-                if (l2 < D.h2) {
-                    if (nbn < NBB_CAP) D.nbB[nbn * stride + i] = j;
-                    else ++over;
-                    ++nbn;
-                    zero += (l2 == 0.0f);
+    int cx = 0, cy = 0, cz = 0;
+    float4 pa = make_float4(0.f, 0.f, 0.f, 0.f);
+    SceneGrid G{};
+    if (act) {
+        G = D.grids[scene];
+        pa = D.x[i];
+        cell_of(G, pa, cx, cy, cz);
+    }
+    auto put = [&](int* list, int k, int j) {
+        if (k < NB_STAGE_GROUPS * 4) st[((k >> 2) * 256 + tid) * 4 + (k & 3)] = j;
+        else list[nb_slot(k, i, stride)] = j;
+    };
+    auto flush = [&](int* list, int count) {
+        __syncthreads();
+        const int ng = min((count + 3) >> 2, NB_STAGE_GROUPS);
+        int4* out = reinterpret_cast<int4*>(list);
+        for (int g = 0; g < ng; ++g) out[(size_t)g * stride + i] = stage[g][tid];
+        __syncthreads();
+    };
+    int nf = 0, nbn = 0, over = 0, zero = 0;
+    // fluid neighbours (SphBase.cpp:867-905)
+    if (act) {
+        for (int dz = -1; dz <= 1; ++dz) {
+            int z = cz + dz;
+            if (z < 0 || z >= G.nz) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                int y = cy + dy;
+                if (y < 0 || y >= G.ny) continue;
+                int2 rf = run3(D.cellF, key_of(G, D.rowRank, cx, y, z), cx, G.nx);
+                for (int j = rf.x; j < rf.y; ++j) {
+                    if (j == i) continue;
+                    float l2 = dis_square_exact(pa, D.x[j]);
+                    if (l2 < D.h2) {
+                        if (nf < NBF_CAP) put(D.nbF, nf, j);
+                        else ++over;
+                        ++nf;
+                        zero += (l2 == 0.0f);
+                    }
                 }
             }
         }
     }
-    int cf = min(nf, NBF_CAP), cb = min(nbn, NBB_CAP);
-    for (int k = cf; k < ((cf + 3) & ~3); ++k) D.nbF[k * stride + i] = i;      // pad: self
-    for (int k = cb; k < ((cb + 3) & ~3); ++k) D.nbB[k * stride + i] = D.nb;   // pad: far dummy
-    D.cnt[i] = cf | (cb << 16);
+    const int cf = act ? min(nf, NBF_CAP) : 0;
+    for (int k = cf; k < ((cf + 3) & ~3); ++k) put(D.nbF, k, i);   // pad: self
+    flush(D.nbF, cf);
+    // boundary neighbours (SphBase.cpp:908-930)
+    if (act) {
+        for (int dz = -1; dz <= 1; ++dz) {
+            int z = cz + dz;
+            if (z < 0 || z >= G.nz) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                int y = cy + dy;
+                if (y < 0 || y >= G.ny) continue;
+                int2 rb = run3(D.cellB, key_of(G, D.rowRank, cx, y, z), cx, G.nx);
+                for (int j = rb.x; j < rb.y; ++j) {
+                    float l2 = dis_square_exact(pa, D.xb[j]);
+                    if (l2 < D.h2) {
+                        if (nbn < NBB_CAP) put(D.nbB, nbn, j);
+                        else ++over;
+                        ++nbn;
+                        zero += (l2 == 0.0f);
+                    }
+                }
+            }
+        }
+    }
+    const int cb = act ? min(nbn, NBB_CAP) : 0;
+    for (int k = cb; k < ((cb + 3) & ~3); ++k) put(D.nbB, k, D.nb);   // pad: far dummy
+    flush(D.nbB, cb);
+    if (i < D.npad) D.cnt[i] = cf | (cb << 16);
+    if (!act) return;
     int tm = nf + nbn - REF_NEIGHBOUR_CAP;
     if (tm > 0) atomicAdd(&D.counters[C_TOO_MANY], tm);
     if (over > 0) atomicAdd(&D.counters[C_LIST_OVF], over);
@@ -285,7 +320,8 @@ __global__ __launch_bounds__(256) void k_boundary_volume(Dev D) {
     D.bkself[i] = sumSelf;
 }
 
-// test dump: neighbour sets with original ids and correctly rounded distances
+// test dump: the stored neighbour lists (the ones every pass iterates), with
+// original ids and correctly rounded distances
 __global__ __launch_bounds__(256) void k_neighbour_dump(Dev D, int cap, int* counts, int* kinds,
                                                       long long* ids, float* dis, const int* rankOf) {
     int i = blockIdx.x * 256 + threadIdx.x;
@@ -293,42 +329,24 @@ __global__ __launch_bounds__(256) void k_neighbour_dump(Dev D, int cap, int* cou
     if (i >= D.npad || !slot_valid(D, i, &scene)) return;
     int o = rankOf[i];
     if (o < 0) return;
-    const SceneGrid G = D.grids[scene];
-    float4 pa = D.x[i];
-    int cx, cy, cz;
-    cell_of(G, pa, cx, cy, cz);
+    const float4 pa = D.x[i];
+    const int cnt = D.cnt[i], nf = cnt & 0xFFFF, nbn = cnt >> 16;
     int c = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-        int z = cz + dz;
-        if (z < 0 || z >= G.nz) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            int y = cy + dy;
-            if (y < 0 || y >= G.ny) continue;
-            uint32_t k0 = key_of(G, D.rowRank, cx, y, z);
-            int2 rf = run3(D.cellF, k0, cx, G.nx);
-            for (int j = rf.x; j < rf.y; ++j) {
-                if (j == i) continue;
-                float l2 = dis_square_exact(pa, D.x[j]);
-                if (l2 < D.h2) {
-                    if (c < cap) {
-                        size_t q = (size_t)o * cap + c;
-                        kinds[q] = 0; ids[q] = D.id[j]; dis[q] = sqrtf(l2);
-                    }
-                    ++c;
-                }
-            }
-            int2 rb = run3(D.cellB, k0, cx, G.nx);
-            for (int j = rb.x; j < rb.y; ++j) {
-                float l2 = dis_square_exact(pa, D.xb[j]);
-                if (l2 < D.h2) {
-                    if (c < cap) {
-                        size_t q = (size_t)o * cap + c;
-                        kinds[q] = 1; ids[q] = D.bid[j]; dis[q] = sqrtf(l2);
-                    }
-                    ++c;
-                }
-            }
+    for (int k = 0; k < nf; ++k) {
+        const int j = D.nbF[nb_slot(k, i, D.ncap)];
+        if (c < cap) {
+            size_t q = (size_t)o * cap + c;
+            kinds[q] = 0; ids[q] = D.id[j]; dis[q] = sqrtf(dis_square_exact(pa, D.x[j]));
         }
+        ++c;
+    }
+    for (int k = 0; k < nbn; ++k) {
+        const int j = D.nbB[nb_slot(k, i, D.ncap)];
+        if (c < cap) {
+            size_t q = (size_t)o * cap + c;
+            kinds[q] = 1; ids[q] = D.bid[j]; dis[q] = sqrtf(dis_square_exact(pa, D.xb[j]));
+        }
+        ++c;
     }
     counts[o] = c;
 }

@@ -19,7 +19,7 @@ struct Slot {
 
 __device__ __forceinline__ Slot slot_of(const Dev& D, bool gated) {
     Slot s;
-    s.tile = blockIdx.x;
+    s.tile = xcd_block(blockIdx.x, gridDim.x);
     s.i = s.tile * TILE + threadIdx.x;
     s.scene = D.tileScene[s.tile];
     s.live = (int)threadIdx.x < D.tileValid[s.tile];
@@ -867,6 +867,11 @@ __global__ __launch_bounds__(256) void k_ii_force(Dev D) {
     }
 }
 
+// plain streaming copy (calibration of the byte counters)
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+    for (size_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) b[k] = a[k];
+}
+
 __global__ void k_fill_i32(int* p, int v, int n) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = v;
@@ -875,6 +880,11 @@ __global__ void k_fill_i32(int* p, int v, int n) {
 inline int tiles_of(const Dev& D) { return D.npad / TILE; }
 
 }  // namespace
+
+void launch_copy_probe(const float4* a, float4* b, size_t n, hipStream_t st) {
+    int blocks = int(std::min<size_t>((n + 255) / 256, 8192));
+    if (n > 0) hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, st, a, b, n);
+}
 
 void launch_cfl(const Dev& D, float cfl, float spacing, float dtMax, float dtMin, hipStream_t st) {
     if (tiles_of(D) > 0) hipLaunchKernelGGL(k_cfl_partial, dim3(tiles_of(D)), dim3(256), 0, st, D);

@@ -149,6 +149,17 @@ __device__ __forceinline__ float dis_square_exact(float4 a, float4 b) {
     return dx * dx + dy * dy + dz * dz;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by workgroup id, and
+// each XCD has its own L2.  Tiles next to each other in key order gather the
+// same neighbour particles, so XCD x is given one contiguous run of tiles:
+// workgroup b runs on XCD b % 8 as the (b / 8)-th tile of that XCD's run.
+// A bijection on [0, n) for any n.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    const int x = b % NUM_XCD, q = n / NUM_XCD, r = n % NUM_XCD;
+    return x * q + min(x, r) + b / NUM_XCD;
+}
+
 __device__ __forceinline__ uint32_t morton2(uint32_t a, uint32_t b) {
     auto spread = [](uint32_t v) {
         v &= 0xFFFFu;
@@ -240,32 +251,41 @@ __device__ __forceinline__ int scene_of_key(uint32_t key, const uint32_t* base, 
 // index), boundary lists with the dummy boundary particle D.nb placed far
 // away (W = gradW = 0).  The loops below read four indices, then issue the
 // four gathers before using them.
+// ELL neighbour lists stored in groups of four entries: entries 4g..4g+3 of
+// particle i are the int4 at group g, (g * ncap + i).  A wave reads one group
+// of its 64 particles with one 16-byte load per lane (1 KiB contiguous).
+__device__ __forceinline__ size_t nb_slot(int k, int i, int ncap) {
+    return ((size_t)(k >> 2) * ncap + i) * 4 + (k & 3);
+}
+
+// Calls f(j) for the neighbours of i, four at a time (lists are padded to a
+// multiple of four), with the next group's indices loaded before the
+// current group is processed.
 template <class F>
-__device__ __forceinline__ void each_fluid(const Dev& D, int i, int nf, F&& f) {
-    const int* __restrict__ col = D.nbF + i;
-    const int st = D.ncap;
-    const int n4 = (nf + 3) & ~3;
-    for (int k = 0; k < n4; k += 4) {
-        int j0 = col[(k + 0) * st], j1 = col[(k + 1) * st], j2 = col[(k + 2) * st], j3 = col[(k + 3) * st];
-        f(j0);
-        f(j1);
-        f(j2);
-        f(j3);
+__device__ __forceinline__ void each_in_list(const int* __restrict__ list, int ncap, int i, int n, F&& f) {
+    const int4* __restrict__ col = reinterpret_cast<const int4*>(list) + i;
+    const int ng = (n + 3) >> 2;
+    if (ng <= 0) return;
+    int4 cur = col[0];
+    for (int g = 0; g < ng; ++g) {
+        int4 nxt = cur;
+        if (g + 1 < ng) nxt = col[(size_t)(g + 1) * ncap];
+        f(cur.x);
+        f(cur.y);
+        f(cur.z);
+        f(cur.w);
+        cur = nxt;
     }
 }
 
 template <class F>
+__device__ __forceinline__ void each_fluid(const Dev& D, int i, int nf, F&& f) {
+    each_in_list(D.nbF, D.ncap, i, nf, f);
+}
+
+template <class F>
 __device__ __forceinline__ void each_boundary(const Dev& D, int i, int nb, F&& f) {
-    const int* __restrict__ col = D.nbB + i;
-    const int st = D.ncap;
-    const int n4 = (nb + 3) & ~3;
-    for (int k = 0; k < n4; k += 4) {
-        int j0 = col[(k + 0) * st], j1 = col[(k + 1) * st], j2 = col[(k + 2) * st], j3 = col[(k + 3) * st];
-        f(j0);
-        f(j1);
-        f(j2);
-        f(j3);
-    }
+    each_in_list(D.nbB, D.ncap, i, nb, f);
 }
 
 constexpr float DUMMY_FAR = 1.0e15f;   // coordinate of the dummy boundary particle
@@ -333,5 +353,6 @@ void launch_ii_force_integrate(const Dev& D, hipStream_t st);
 void launch_scene_reduce(const Dev& D, int mode, float tol, int minIt, int cap, hipStream_t st);
 void launch_stats(const Dev& D, double* part, double* out, hipStream_t st);
 void launch_fill_i32(int* p, int v, int n, hipStream_t st);
+void launch_copy_probe(const float4* a, float4* b, size_t n, hipStream_t st);
 
 }  // namespace sphgpu

@@ -108,6 +108,7 @@ SIGNATURES = {
     "sphgpu_get_neighbours": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, _LP, _FP]),
     "sphgpu_profile_kernel": (C.c_int, [C.c_void_p, C.c_char_p]),
     "sphgpu_kernel_timing": (C.c_int, [C.c_void_p, _LP, _DP, _DP]),
+    "sphgpu_copy_probe": (C.c_int, [C.c_longlong, _DP]),
     "sphgpu_mlp_create": (C.c_int, [C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
     "sphgpu_mlp_destroy": (None, [C.c_void_p]),
     "sphgpu_mlp_set_params": (C.c_int, [C.c_void_p, _FP]),
@@ -169,6 +170,15 @@ def default_params(**kw) -> Params:
 
 def device_available() -> bool:
     return bool(load().sphgpu_device_available())
+
+
+def copy_probe(nbytes: int) -> float:
+    """Plain device-to-device copy of nbytes (counter calibration); returns ms."""
+    ms = C.c_double()
+    rc = load().sphgpu_copy_probe(int(nbytes), C.byref(ms))
+    if rc:
+        raise SphError(rc, "copy probe failed")
+    return ms.value
 
 
 def lattice(n, origin, spacing, jitter=0.0, seed=0, stream=0) -> np.ndarray:

@@ -182,6 +182,10 @@ int sphgpu_get_neighbours(sphgpu_handle* h, int cap, int* counts, int* kinds,
  * count, total milliseconds and the algorithmic bytes per launch summed over
  * launches. */
 int sphgpu_profile_kernel(sphgpu_handle* h, const char* name);
+/* Calibration of byte counters: copies `bytes` (rounded down to 16) between
+ * two device buffers with a plain streaming copy kernel (k_copy_probe), twice;
+ * *ms receives the second launch's duration.  No handle needed. */
+int sphgpu_copy_probe(long long bytes, double* ms);
 int sphgpu_kernel_timing(sphgpu_handle* h, long long* launches, double* total_ms,
                          double* total_alg_bytes);
 
