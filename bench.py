@@ -64,6 +64,43 @@ def read_traffic(workload: str, kernel: str):
     return None
 
 
+def kernel_name(raw: str) -> str:
+    """Short kernel name from a demangled or mangled (Itanium) symbol."""
+    import re
+    m = re.match(r"_ZN(?:\d+\w+?)*?\d+_GLOBAL__N_1(\d+)", raw)
+    if m:
+        k = int(m.group(1))
+        start = m.end()
+        return raw[start:start + k]
+    return raw.replace("sphgpu::(anonymous namespace)::", "").split("(")[0]
+
+
+def summarize_trace(stats_csv: str, trace_csv: str, workload: str, command: str) -> int:
+    """Kernel table of a rocprofv3 --kernel-trace --stats run: all dispatches,
+    and the dispatches that processed particles (solver iterations launched
+    past convergence return at once; launches under 10 us at this size are
+    those); writes profiles/<workload>_kernel_stats.txt."""
+    import csv
+    import collections
+
+    per = collections.defaultdict(list)
+    for r in csv.DictReader(open(trace_csv)):
+        name = kernel_name(r["Kernel_Name"])
+        per[name].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    total = sum(sum(v) for v in per.values())
+    lines = [f"# kernel durations, {workload}", f"# command: {command}",
+             f"# {'kernel':28s} {'calls':>6s} {'avg_us':>9s} {'pct':>6s} {'calls>=10us':>11s} {'avg_us>=10us':>12s}"]
+    for name, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        real = [x for x in v if x >= 10.0]
+        ra = sum(real) / len(real) if real else 0.0
+        lines.append(f"  {name[:28]:28s} {len(v):6d} {sum(v) / len(v):9.2f} {100 * sum(v) / total:6.2f} "
+                     f"{len(real):11d} {ra:12.2f}")
+    out = ROOT / "profiles" / f"{workload}_kernel_stats.txt"
+    out.write_text("\n".join(lines) + "\n")
+    print(out)
+    return 0
+
+
 def summarize_pmc(fetch_csv: str, write_csv: str, workload: str, command: str) -> int:
     """Per-kernel median FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of two
     rocprofv3 --pmc runs, scaled by what the counters read for the copy
@@ -75,7 +112,7 @@ def summarize_pmc(fetch_csv: str, write_csv: str, workload: str, command: str) -
     def load(path):
         d = collections.defaultdict(list)
         for r in csv.DictReader(open(path)):
-            name = r["Kernel_Name"].replace("sphgpu::(anonymous namespace)::", "").split("(")[0]
+            name = kernel_name(r["Kernel_Name"])
             dur = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
             d[name].append((float(r["Counter_Value"]), dur, int(r["Grid_Size"])))
         return d
@@ -173,18 +210,23 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="dfsph_c2", choices=sorted(scenes.WORKLOADS))
-    ap.add_argument("--warm-start", default="previous", choices=["none", "previous", "learned"])
+    ap.add_argument("--warm-start", default="learned", choices=["none", "previous", "learned"])
     ap.add_argument("--no-ab", action="store_true", help="skip the plain-solver arm")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-kernel", default=None)
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COMMAND"),
                     help="write profiles/<workload>_traffic.txt from two rocprofv3 --pmc runs")
+    ap.add_argument("--summarize-trace", nargs=2, metavar=("TRACE_CSV", "COMMAND"),
+                    help="write profiles/<workload>_kernel_stats.txt from a rocprofv3 --kernel-trace run")
     args = ap.parse_args()
     if args.summarize_pmc:
         return summarize_pmc(*args.summarize_pmc[:2], build_scene(args.workload).name, args.summarize_pmc[2])
+    if args.summarize_trace:
+        return summarize_trace(None, args.summarize_trace[0], build_scene(args.workload).name,
+                               args.summarize_trace[1])
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no "
               "multi-GPU path", file=sys.stderr)
@@ -232,17 +274,24 @@ def main() -> int:
 
     iters = {"warm_start": args.warm_start, "density": it_d, "divergence": it_v}
     if not args.no_ab and warm != sg.WARM_NONE:
-        plain = run_gpu(sc, sg.WARM_NONE, args.steps, args.warmup, None)
-        ps = plain["stats"]
-        iters["plain"] = {"density": float(np.mean([s["iter_density"] for s in ps])),
-                          "divergence": float(np.mean([s["iter_divergence"] for s in ps])),
-                          "particle_steps_per_s": nf * args.steps / plain["elapsed"]}
+        # same workload, same steps: the plain solver (the reference), and for
+        # the learned guess also the previous-stiffness guess it competes with
+        ke = np.array([s["kinetic_energy"] for s in run["warm_stats"] + stats])
+        arms = [("plain", sg.WARM_NONE)] + ([("previous", sg.WARM_PREVIOUS)] if warm == sg.WARM_LEARNED else [])
+        for label, w in arms:
+            arm = run_gpu(sc, w, args.steps, args.warmup, None)
+            ps = arm["stats"]
+            ke_a = np.array([s["kinetic_energy"] for s in arm["warm_stats"] + ps])
+            iters[label] = {"density": float(np.mean([s["iter_density"] for s in ps])),
+                            "divergence": float(np.mean([s["iter_divergence"] for s in ps])),
+                            "particle_steps_per_s": nf * args.steps / arm["elapsed"],
+                            "max_kinetic_energy_dev": float(np.abs(ke - ke_a).max() / max(ke_a.max(), 1e-30))}
+            log(f"[bench] {label} arm: iters {iters[label]['density']:.2f}/{iters[label]['divergence']:.2f}")
         tot_w = it_d + it_v
         tot_p = iters["plain"]["density"] + iters["plain"]["divergence"]
         iters["reduction_vs_plain"] = round(1.0 - tot_w / tot_p, 4) if tot_p else None
         pv = iters["plain"]["divergence"]
         iters["divergence_reduction_vs_plain"] = round(1.0 - it_v / pv, 4) if pv else None
-        log(f"[bench] plain arm: iters {iters['plain']['density']:.2f}/{iters['plain']['divergence']:.2f}")
 
     cpu = None
     if not args.no_cpu:

@@ -531,7 +531,7 @@ void sphgpu_default_params(sphgpu_params* p) {
     p->learn_batch = 16384;
     p->learn_rate = 1e-3f;
     p->learn_over_weight = 1.0f;
-    p->warm_scale = 0.5f;
+    p->warm_scale = 0.7f;
 }
 
 int sphgpu_device_available(void) {

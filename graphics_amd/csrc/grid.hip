@@ -165,9 +165,9 @@ __device__ __forceinline__ int2 run3(const int2* __restrict__ table, uint32_t k0
 
 // Lists are built in LDS, one column per thread, and written out group by
 // group (four entries, 16 bytes per particle), so each store instruction of a
-// wave writes whole cache lines.  Entries beyond the staged 48 (rare) are
+// wave writes whole cache lines.  Entries beyond the staged 32 are
 // written to global memory directly.
-constexpr int NB_STAGE_GROUPS = 12;
+constexpr int NB_STAGE_GROUPS = 8;
 
 __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     __shared__ int4 stage[NB_STAGE_GROUPS][256];

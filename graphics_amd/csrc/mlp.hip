@@ -251,65 +251,79 @@ __device__ __forceinline__ float layer_input(int L, const float* __restrict__ fe
     return gelu(store[(size_t)p * MLP_STORE + (L - 1) * 64 + unit]);
 }
 
-// weight and bias gradients: one wave per 16x16 output tile of dW (out x in),
-// plus bias tiles; K = samples in chunks of 32 (fixed order: deterministic).
-// tile list: L1 4x1, L2 4x4, L3 4x4, L4 1x4 (W), then bias tiles 4+4+4+1.
-__global__ __launch_bounds__(64) void k_mlp_wgrad(const float* __restrict__ feat, const float* __restrict__ store,
-                                                const float* __restrict__ dz, float* __restrict__ grad, int n,
-                                                float scale) {
-    const int tid = blockIdx.x;
-    const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
-    int L, to, ti, bias = 0;
-    {
-        int k = tid;
-        const int wt[4] = {4, 16, 16, 4};
-        L = 0;
-        while (L < 4 && k >= wt[L]) { k -= wt[L]; ++L; }
-        if (L < 4) {
-            int nin = (L == 0) ? 1 : 4;
-            to = k / nin;
-            ti = k % nin;
-        } else {
-            bias = 1;
-            const int bt[4] = {4, 4, 4, 1};
-            L = 0;
-            while (L < 3 && k >= bt[L]) { k -= bt[L]; ++L; }
-            to = k;
-            ti = 0;
-        }
+// Tile t of the weight/bias gradient: 16x16 blocks of dW (out x in) per
+// layer, then bias blocks.  L1 4x1, L2 4x4, L3 4x4, L4 1x4 (W), then bias
+// tiles 4+4+4+1.
+struct GradTile { int L, to, ti, bias; };
+__device__ __forceinline__ GradTile grad_tile(int t) {
+    GradTile r{0, 0, 0, 0};
+    int k = t;
+    const int wt[4] = {4, 16, 16, 4};
+    while (r.L < 4 && k >= wt[r.L]) { k -= wt[r.L]; ++r.L; }
+    if (r.L < 4) {
+        int nin = (r.L == 0) ? 1 : 4;
+        r.to = k / nin;
+        r.ti = k % nin;
+    } else {
+        r.bias = 1;
+        const int bt[4] = {4, 4, 4, 1};
+        r.L = 0;
+        while (r.L < 3 && k >= bt[r.L]) { k -= bt[r.L]; ++r.L; }
+        r.to = k;
     }
-    const int nin = (L == 0) ? 8 : 64, nout = (L == 3) ? 1 : 64;
+    return r;
+}
+
+// Weight and bias gradients, K = samples split into slices of
+// MLP_WGRAD_SLICE: block (tile, slice) is one wave that accumulates its 16x16
+// tile over the slice's samples in chunks of 32 on the matrix cores and
+// stores the partial tile; k_mlp_wgrad_reduce sums the slices in a fixed
+// order (deterministic).
+__global__ __launch_bounds__(64) void k_mlp_wgrad(const float* __restrict__ feat, const float* __restrict__ store,
+                                                const float* __restrict__ dz, float* __restrict__ part, int n) {
+    const int tile = blockIdx.x % MLP_WGRAD_TILES, slice = blockIdx.x / MLP_WGRAD_TILES;
+    const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+    const GradTile T = grad_tile(tile);
+    const int nin = (T.L == 0) ? 8 : 64, nout = (T.L == 3) ? 1 : 64;
     // dz of layer L output lives at store row offset: L<3 -> L*64, L==3 -> 192
-    const int zoff = (L < 3) ? L * 64 : 192;
+    const int zoff = (T.L < 3) ? T.L * 64 : 192;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    const int orow = 16 * to + c;          // A row (output unit)
-    const int icol = 16 * ti + c;          // B column (input unit)
-    for (int q = 0; q < n; q += 32) {
+    const int orow = 16 * T.to + c;          // A row (output unit)
+    const int icol = 16 * T.ti + c;          // B column (input unit)
+    const int p0 = slice * MLP_WGRAD_SLICE, p1 = min(n, p0 + MLP_WGRAD_SLICE);
+    for (int q = p0; q < p1; q += 32) {
         bf16x8 a, b;
         for (int j = 0; j < 8; ++j) {
             int p = q + kmap(g, j);
-            bool ok = p < n;
+            bool ok = p < p1;
             float dv = (ok && orow < nout) ? dz[(size_t)p * MLP_STORE + zoff + orow] : 0.0f;
             float hv;
-            if (bias) hv = ok ? 1.0f : 0.0f;
-            else hv = (ok && icol < nin) ? layer_input(L, feat, store, p, icol) : 0.0f;
+            if (T.bias) hv = ok ? 1.0f : 0.0f;
+            else hv = (ok && icol < nin) ? layer_input(T.L, feat, store, p, icol) : 0.0f;
             a[j] = to_bf16(dv);
             b[j] = to_bf16(hv);
         }
         acc = mfma(a, b, acc);
     }
-    // D[4g+j][c] = dW[out = 16 to + 4g + j][in = 16 ti + c]
+    // D[4g+j][c]
+    float* out = part + ((size_t)slice * MLP_WGRAD_TILES + tile) * 256;
+    for (int j = 0; j < 4; ++j) out[(4 * g + j) * 16 + c] = acc[j];
+}
+
+// thread per (tile, element): sum of the slices, written to its parameter
+__global__ __launch_bounds__(256) void k_mlp_wgrad_reduce(const float* __restrict__ part, float* __restrict__ grad,
+                                                        int slices, float scale) {
+    const int tile = blockIdx.x, e = threadIdx.x, row = e >> 4, col = e & 15;
+    const GradTile T = grad_tile(tile);
+    const int nin = (T.L == 0) ? 8 : 64, nout = (T.L == 3) ? 1 : 64;
+    const int out = 16 * T.to + row, in = 16 * T.ti + col;
+    if (out >= nout) return;
+    if (T.bias ? col != 0 : in >= nin) return;
+    float s = 0.0f;
+    for (int k = 0; k < slices; ++k) s += part[((size_t)k * MLP_WGRAD_TILES + tile) * 256 + e];
     const int woff[4] = {OFF_W1, OFF_W2, OFF_W3, OFF_W4}, boff[4] = {OFF_B1, OFF_B2, OFF_B3, OFF_B4};
-    for (int j = 0; j < 4; ++j) {
-        int out = 16 * to + 4 * g + j;
-        if (out >= nout) continue;
-        if (bias) {
-            if (c == 0) grad[boff[L] + out] = acc[j] * scale;
-        } else {
-            int in = 16 * ti + c;
-            if (in < nin) grad[woff[L] + out * nin + in] = acc[j] * scale;
-        }
-    }
+    if (T.bias) grad[boff[T.L] + out] = s * scale;
+    else grad[woff[T.L] + out * nin + in] = s * scale;
 }
 
 // MSE loss gradient dL/dy = 2 (y - t) / n (times `scale` for normalised targets)
@@ -368,8 +382,11 @@ void mlp_grad(MlpDev& m, const float* feat, const float* target, int n, hipStrea
     int blocks = std::min((waves + 3) / 4, 8192);
     hipLaunchKernelGGL(k_mlp_backward, dim3(blocks), dim3(256), 0, st, (const bf16x8*)m.bwd,
                        (const float*)m.store, (const float*)m.dy, m.dz, n);
-    hipLaunchKernelGGL(k_mlp_wgrad, dim3(MLP_WGRAD_TILES), dim3(64), 0, st, feat, (const float*)m.store,
-                       (const float*)m.dz, m.grad, n, 1.0f);
+    const int slices = (n + MLP_WGRAD_SLICE - 1) / MLP_WGRAD_SLICE;
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3(MLP_WGRAD_TILES * slices), dim3(64), 0, st, feat, (const float*)m.store,
+                       (const float*)m.dz, m.gradPart, n);
+    hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3(MLP_WGRAD_TILES), dim3(256), 0, st, (const float*)m.gradPart, m.grad,
+                       slices, 1.0f);
 }
 
 void mlp_adam(MlpDev& m, float lr, hipStream_t st) {
@@ -405,7 +422,9 @@ int mlp_alloc(MlpDev& m, int cap) {
               al((void**)&m.store, sizeof(float) * MLP_STORE * (size_t)m.cap) &&
               al((void**)&m.dz, sizeof(float) * MLP_STORE * (size_t)m.cap) &&
               al((void**)&m.dy, sizeof(float) * m.cap) && al((void**)&m.scratchY, sizeof(float) * m.cap) &&
-              al((void**)&m.lossPart, sizeof(double) * (m.cap / 256 + 2));
+              al((void**)&m.lossPart, sizeof(double) * (m.cap / 256 + 2)) &&
+              al((void**)&m.gradPart, sizeof(float) * 256 * MLP_WGRAD_TILES *
+                                          (size_t)((m.cap + MLP_WGRAD_SLICE - 1) / MLP_WGRAD_SLICE));
     if (!ok) return -1;
     hipMemset(m.adamM, 0, sizeof(float) * MLP_PARAMS);
     hipMemset(m.adamV, 0, sizeof(float) * MLP_PARAMS);
@@ -415,7 +434,8 @@ int mlp_alloc(MlpDev& m, int cap) {
 }
 
 void mlp_free(MlpDev& m) {
-    void* ps[] = {m.params, m.adamM, m.adamV, m.grad, m.fwd, m.bwd, m.store, m.dz, m.dy, m.scratchY, m.lossPart};
+    void* ps[] = {m.params, m.adamM, m.adamV, m.grad, m.fwd, m.bwd, m.store, m.dz, m.dy, m.scratchY, m.lossPart,
+                  m.gradPart};
     for (void* p : ps)
         if (p) hipFree(p);
     m = MlpDev{};

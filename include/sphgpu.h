@@ -83,7 +83,7 @@ typedef struct sphgpu_params {
     float learn_rate;          /* Adam learning rate */
     float learn_over_weight;   /* loss weight of over-predictions (>= 1 biases the guess low) */
     float warm_scale;          /* the guess is warm_scale x the previous stiffness (PREVIOUS) or x the
-                                  bounded prediction (LEARNED); 0.5 as Bender & Koschier */
+                                  bounded prediction (LEARNED); default 0.7 (Bender & Koschier use 0.5) */
     int   reserved[2];
 } sphgpu_params;
 

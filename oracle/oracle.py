@@ -126,7 +126,7 @@ def params_from(d: dict) -> OrcParams:
     defaults = dict(solver=DFSPH, spacing_r=0.1, smooth_radius_h=0.2, dt=1e-3, sound_speed=100.0,
                     gamma=7.0, nu=1e-3, cfl=0.0, dt_min=1e-6, density_tol=1e-4, divergence_tol=2.5,
                     min_iter_density=2, max_iter_density=100, min_iter_div=1, max_iter_div=30,
-                    omega=0.5, warm_start=0, warm_scale=0.5, gravity=(0.0, -9.8, 0.0),
+                    omega=0.5, warm_start=0, warm_scale=0.7, gravity=(0.0, -9.8, 0.0),
                     space_min=(0.0, 0.0, 0.0), space_max=(1.0, 1.0, 1.0))
     defaults.update({k: v for k, v in d.items() if k in defaults})
     for k, v in defaults.items():
