@@ -149,7 +149,8 @@ def iisph_batched_c4(seed=1004, scenes=64, nmin=10_000, nmax=500_000, scale=1.0)
         width = (m + 3) * s
         size = (aspect * width + 2 * s, 1.5 * width, width)
         _tank(sc, k, size, s, True, 0.01)
-        sc.domains.append(((0.0, 0.0, 0.0), size))
+        # the walls lie on the tank faces: the domain extends one spacing beyond
+        sc.domains.append(((-s, -s, -s), tuple(t + s for t in size)))
         sc.fluids.append((k, 0, sg.lattice((m, m, m), (s, s, s), s, 0.01, seed, k)))
     return sc
 
