@@ -147,8 +147,8 @@ def build_scene(name: str):
     return scenes.WORKLOADS[name]()
 
 
-def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None):
-    sim = scenes.make_simulation(sc, warm_start=warm_start)
+def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, graph: int = 0):
+    sim = scenes.make_simulation(sc, warm_start=warm_start, use_graph=graph)
     t0 = time.perf_counter()
     sim.init()
     log(f"[bench] init {time.perf_counter() - t0:.2f}s  fluid={sc.num_fluid} boundary={sc.num_boundary}")
@@ -217,6 +217,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-kernel", default=None)
+    ap.add_argument("--graph", action="store_true",
+                    help="replay captured HIP graphs of the WCSPH / IISPH steps (no per-kernel timing)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COMMAND"),
                     help="write profiles/<workload>_traffic.txt from two rocprofv3 --pmc runs")
     ap.add_argument("--summarize-trace", nargs=2, metavar=("TRACE_CSV", "COMMAND"),
@@ -245,7 +247,7 @@ def main() -> int:
     solver = sc.params["solver"]
     warm = {"none": sg.WARM_NONE, "previous": sg.WARM_PREVIOUS, "learned": sg.WARM_LEARNED}[args.warm_start]
     prof = args.profile_kernel or DOMINANT[solver]
-    run = run_gpu(sc, warm, args.steps, args.warmup, prof)
+    run = run_gpu(sc, warm, args.steps, args.warmup, None if args.graph else prof, int(args.graph))
     nf = sc.num_fluid
     stats = run["stats"]
     value = nf * args.steps / run["elapsed"]
@@ -316,7 +318,7 @@ def main() -> int:
                    "density_tol": sc.params.get("density_tol"),
                    "divergence_tol": sc.params.get("divergence_tol"),
                    "max_density_err": dens_err, "mean_rho_err": rho_mean,
-                   "live_after": run["n_live"]},
+                   "live_after": run["n_live"], "hip_graph": bool(args.graph)},
         "iters_per_step": iters,
         "roofline": roofline,
         "cpu_baseline": cpu,

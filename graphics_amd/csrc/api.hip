@@ -27,6 +27,13 @@ struct Phase {
     float rho0, visc;
 };
 
+struct GraphEntry {
+    int seg, iters;
+    Dev before, after;
+    hipGraphExec_t exec;
+    unsigned long long used;
+};
+
 struct ProfEvent {
     hipEvent_t a, b;
     double bytes;
@@ -88,6 +95,10 @@ struct sphgpu_handle {
     int learned = 0;                   // training steps done
     int lastIters[2] = {0, 0};         // iterations of the previous solve (density, divergence)
     int curIter = -1;                  // iteration being launched (profiling)
+    // captured step segments (use_graph)
+    std::vector<GraphEntry> graphs;
+    unsigned long long graphClock = 0;
+    bool capturing = false;
     uint32_t gatherSeed = 1;
     // profiling
     std::string profName;
@@ -141,7 +152,7 @@ struct ProfScope {
     sphgpu_handle* h;
     bool on;
     ProfScope(sphgpu_handle* hh, const char* name, double bytes) : h(hh), on(false) {
-        if (h->profName.empty() || h->profName != name) return;
+        if (h->capturing || h->profName.empty() || h->profName != name) return;
         if (h->profUsed == h->prof.size()) {
             ProfEvent e{};
             if (hipEventCreate(&e.a) != hipSuccess || hipEventCreate(&e.b) != hipSuccess) return;
@@ -270,22 +281,32 @@ void launch_iteration(sphgpu_handle* h, int which, int it) {
     }
 }
 
-int solver_loop(sphgpu_handle* h, int which) {
+// first batch of a solver loop: resets the per-scene flags and launches
+// `batch` iterations without reading anything back (capturable)
+int first_batch(const sphgpu_handle* h, int which) {
+    const int minIt = which == 0 || h->P.solver != SPHGPU_DFSPH ? h->P.min_iter_density : h->P.min_iter_div;
+    return std::max({1, minIt, h->lastIters[which] - 1});
+}
+
+void solver_loop_start(sphgpu_handle* h, int which, int batch) {
+    Dev& D = h->D;
+    launch_fill_i32(D.active, 1, D.nscenes, h->st);
+    launch_fill_i32(D.iters, 0, D.nscenes, h->st);
+    for (int q = 0; q < batch; ++q) launch_iteration(h, which, q);
+}
+
+// rest of a solver loop whose first `launched` iterations are queued:
+// reads the flags, adds iterations two at a time until every scene is done
+int solver_loop_finish(sphgpu_handle* h, int which, int launched, size_t profStart) {
     Dev& D = h->D;
     int ns = D.nscenes;
-    launch_fill_i32(D.active, 1, ns, h->st);
-    launch_fill_i32(D.iters, 0, ns, h->st);
-    const int minIt = which == 0 || h->P.solver != SPHGPU_DFSPH ? h->P.min_iter_density : h->P.min_iter_div;
-    int batch = std::max({1, minIt, h->lastIters[which] - 1});
-    int maxIt = 0, launched = 0;
-    const size_t profStart = h->profUsed;
+    int maxIt = 0;
     for (;;) {
-        for (int q = 0; q < batch; ++q) launch_iteration(h, which, launched++);
         bool any = false;
         int rc = sync_active(h, &any);
         if (rc) return rc;
         if (!any) break;
-        batch = 2;
+        for (int q = 0; q < 2; ++q) launch_iteration(h, which, launched++);
     }
     for (int s = 0; s < ns; ++s) maxIt = std::max(maxIt, h->hIters[s]);
     h->lastIters[which] = maxIt;
@@ -301,6 +322,56 @@ int solver_loop(sphgpu_handle* h, int which) {
             h->sceneStats[s].divergence_err = h->hErr[s];
         }
     }
+    return 0;
+}
+
+int solver_loop(sphgpu_handle* h, int which) {
+    const size_t profStart = h->profUsed;
+    const int batch = first_batch(h, which);
+    solver_loop_start(h, which, batch);
+    return solver_loop_finish(h, which, batch, profStart);
+}
+
+// ---- HIP graphs of whole step segments ------------------------------------
+// A segment is host code that only queues kernels (no readback).  Its first
+// run with a given device-state snapshot (the Dev struct: buffer pointers
+// after the double-buffer swaps, sizes) and a given iteration count is
+// captured and instantiated; later runs from an identical snapshot replay the
+// graph and take the host state the capture ended with.  The ping-pong
+// buffers repeat with period two, so a few graphs cover a run; a change of
+// the particle layout (removals) makes new ones.
+template <class F>
+int run_segment(sphgpu_handle* h, int seg, int iters, F&& body) {
+    for (auto& g : h->graphs)
+        if (g.seg == seg && g.iters == iters && std::memcmp(&g.before, &h->D, sizeof(Dev)) == 0) {
+            HIPCHECK(h, hipGraphLaunch(g.exec, h->st));
+            h->D = g.after;
+            g.used = ++h->graphClock;
+            return 0;
+        }
+    GraphEntry e{};
+    e.seg = seg;
+    e.iters = iters;
+    e.before = h->D;
+    HIPCHECK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+    h->capturing = true;
+    body();
+    h->capturing = false;
+    hipGraph_t graph = nullptr;
+    HIPCHECK(h, hipStreamEndCapture(h->st, &graph));
+    hipError_t ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(h, SPHGPU_E_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(ie));
+    e.after = h->D;
+    e.used = ++h->graphClock;
+    if (h->graphs.size() >= 8) {   // least recently used goes
+        auto old = std::min_element(h->graphs.begin(), h->graphs.end(),
+                                    [](const GraphEntry& a, const GraphEntry& b) { return a.used < b.used; });
+        hipGraphExecDestroy(old->exec);
+        h->graphs.erase(old);
+    }
+    h->graphs.push_back(e);
+    HIPCHECK(h, hipGraphLaunch(e.exec, h->st));
     return 0;
 }
 
@@ -356,19 +427,27 @@ int finish_step(sphgpu_handle* h, sphgpu_stats* out) {
 }
 
 int step_wcsph(sphgpu_handle* h, sphgpu_stats* st) {
-    Dev& D = h->D;
-    run_ns(h);
-    launch_cfl(D, h->P.cfl, h->P.spacing_r, h->P.dt, h->P.dt_min, h->st);
-    {
-        ProfScope ps(h, "wc_pressure", nf_live(h) * 20.0);
-        launch_wc_pressure(D, h->st);
+    auto body = [h]() {
+        Dev& D = h->D;
+        run_ns(h);
+        launch_cfl(D, h->P.cfl, h->P.spacing_r, h->P.dt, h->P.dt_min, h->st);
+        {
+            ProfScope ps(h, "wc_pressure", nf_live(h) * 20.0);
+            launch_wc_pressure(D, h->st);
+        }
+        {
+            ProfScope ps(h, "wc_force", nf_live(h) * 56.0);
+            launch_wc_force_integrate(D, h->st);
+        }
+        std::swap(D.x, D.x2);
+        std::swap(D.v, D.v2);
+    };
+    if (h->P.use_graph) {
+        int rc = run_segment(h, 0, 0, body);
+        if (rc) return rc;
+    } else {
+        body();
     }
-    {
-        ProfScope ps(h, "wc_force", nf_live(h) * 56.0);
-        launch_wc_force_integrate(D, h->st);
-    }
-    std::swap(D.x, D.x2);
-    std::swap(D.v, D.v2);
     return finish_step(h, st);
 }
 
@@ -430,18 +509,31 @@ int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
 int step_iisph(sphgpu_handle* h, sphgpu_stats* st) {
     Dev& D = h->D;
     int rc;
-    run_ns(h);
-    launch_cfl(D, h->P.cfl, h->P.spacing_r, h->P.dt, h->P.dt_min, h->st);
-    launch_ii_density(D, h->st);
-    {
-        ProfScope ps(h, "ii_aadv_dii", nf_live(h) * 64.0);
-        launch_ii_aadv_dii(D, h->st);
+    const size_t profStart = h->profUsed;
+    const int batch = first_batch(h, 0);
+    // neighbour search, the per-step passes and the first `batch` solver
+    // iterations: one captured segment in graph mode
+    auto body = [h, batch]() {
+        Dev& D = h->D;
+        run_ns(h);
+        launch_cfl(D, h->P.cfl, h->P.spacing_r, h->P.dt, h->P.dt_min, h->st);
+        launch_ii_density(D, h->st);
+        {
+            ProfScope ps(h, "ii_aadv_dii", nf_live(h) * 64.0);
+            launch_ii_aadv_dii(D, h->st);
+        }
+        {
+            ProfScope ps(h, "ii_aii", nf_live(h) * 48.0);
+            launch_ii_aii(D, h->st);
+        }
+        solver_loop_start(h, 0, batch);
+    };
+    if (h->P.use_graph) {
+        if ((rc = run_segment(h, 1, batch, body))) return rc;
+    } else {
+        body();
     }
-    {
-        ProfScope ps(h, "ii_aii", nf_live(h) * 48.0);
-        launch_ii_aii(D, h->st);
-    }
-    if ((rc = solver_loop(h, 0))) return rc;
+    if ((rc = solver_loop_finish(h, 0, batch, profStart))) return rc;
     {
         ProfScope ps(h, "ii_force", nf_live(h) * 56.0);
         launch_ii_force_integrate(D, h->st);
@@ -555,6 +647,7 @@ int sphgpu_create(const sphgpu_params* p, sphgpu_handle** out) {
          !std::isfinite(p->learn_over_weight)))
         return SPHGPU_E_ARG;
     if (!(p->warm_scale >= 0.0f) || !(p->warm_scale <= 4.0f)) return SPHGPU_E_ARG;
+    if (p->use_graph < 0 || p->use_graph > 1) return SPHGPU_E_ARG;
     sphgpu_handle* h = new sphgpu_handle();
     h->P = *p;
     h->scenes.resize(p->num_scenes);
@@ -571,6 +664,7 @@ void sphgpu_destroy(sphgpu_handle* h) {
     if (h->initialized) {
         hipStreamSynchronize(h->st);
         for (auto& e : h->prof) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+        for (auto& g : h->graphs) hipGraphExecDestroy(g.exec);
         for (void* m : h->allocs) hipFree(m);
         if (h->mlpLive) mlp_free(h->mlp);
         hipHostFree(h->hActive);

@@ -169,6 +169,15 @@ void exclusive_scan_u32(uint32_t* data, int n, uint32_t* tmp, hipStream_t st) {
     hipLaunchKernelGGL(k_scan_down, dim3(blocks), dim3(256), 0, st, data, n, (const uint32_t*)tmp);
 }
 
+__global__ __launch_bounds__(256) void k_copy_pairs(const uint32_t* __restrict__ ka, const uint32_t* __restrict__ va,
+                                                  uint32_t* __restrict__ kb, uint32_t* __restrict__ vb, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        kb[i] = ka[i];
+        vb[i] = va[i];
+    }
+}
+
 size_t sort_scratch_bytes(int maxItems) {
     int tiles = (maxItems + SORT_TILE - 1) / SORT_TILE;
     size_t words = 2 * (size_t)maxItems + 256 * (size_t)tiles + scan_tmp_words(256 * tiles) + 64;
@@ -202,8 +211,9 @@ void radix_sort_pairs(uint32_t* keys, uint32_t* vals, int n, int bits, SortScrat
         std::swap(va, vb);
     }
     if (ka != keys) {
-        hipMemcpyAsync(keys, ka, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st);
-        hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st);
+        // a kernel, not a memcpy node, so that a captured step holds kernels only
+        hipLaunchKernelGGL(k_copy_pairs, dim3((n + 255) / 256), dim3(256), 0, st, (const uint32_t*)ka,
+                           (const uint32_t*)va, keys, vals, n);
     }
 }
 

@@ -52,7 +52,8 @@ class Params(C.Structure):
         ("learn_rate", C.c_float),
         ("learn_over_weight", C.c_float),
         ("warm_scale", C.c_float),
-        ("reserved", C.c_int * 2),
+        ("use_graph", C.c_int),
+        ("reserved", C.c_int * 1),
     ]
 
 

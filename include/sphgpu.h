@@ -84,7 +84,8 @@ typedef struct sphgpu_params {
     float learn_over_weight;   /* loss weight of over-predictions (>= 1 biases the guess low) */
     float warm_scale;          /* the guess is warm_scale x the previous stiffness (PREVIOUS) or x the
                                   bounded prediction (LEARNED); default 0.7 (Bender & Koschier use 0.5) */
-    int   reserved[2];
+    int   use_graph;           /* 1: WCSPH/IISPH steps are captured once into HIP graphs and replayed */
+    int   reserved[1];
 } sphgpu_params;
 
 typedef struct sphgpu_stats {
