@@ -147,7 +147,8 @@ def build_scene(name: str):
     return scenes.WORKLOADS[name]()
 
 
-def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, graph: int = 0):
+def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, graph: int = 0,
+            stats_every: int = 1):
     sim = scenes.make_simulation(sc, warm_start=warm_start, use_graph=graph)
     t0 = time.perf_counter()
     sim.init()
@@ -164,14 +165,19 @@ def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, g
         sim.profile_kernel(profile)
     sim.synchronize()
     t0 = time.perf_counter()
-    stats = [sim.step() for _ in range(steps)]
+    stats = []
+    for k in range(steps):
+        s = sim.step(stats=(k + 1) % stats_every == 0 or k == steps - 1)
+        if s is not None:
+            stats.append(s)
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     timing = sim.kernel_timing() if profile else (0, 0.0, 0.0)
     n_live = sim.num_fluid()
+    tuning = sim.tuning()
     sim.close()
     return dict(elapsed=elapsed, stats=stats, warm_times=warm_times, warm_stats=warm_stats,
-                timing=timing, n_live=n_live)
+                timing=timing, n_live=n_live, tuning=tuning)
 
 
 def cpu_baseline(sc, warm_start: int, budget_s: float):
@@ -217,6 +223,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-kernel", default=None)
+    ap.add_argument("--stats-every", type=int, default=1,
+                    help="read statistics back every N timed steps (the others do not wait for the device)")
     ap.add_argument("--graph", action="store_true",
                     help="replay captured HIP graphs of the WCSPH / IISPH steps (no per-kernel timing)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COMMAND"),
@@ -247,7 +255,8 @@ def main() -> int:
     solver = sc.params["solver"]
     warm = {"none": sg.WARM_NONE, "previous": sg.WARM_PREVIOUS, "learned": sg.WARM_LEARNED}[args.warm_start]
     prof = args.profile_kernel or DOMINANT[solver]
-    run = run_gpu(sc, warm, args.steps, args.warmup, None if args.graph else prof, int(args.graph))
+    run = run_gpu(sc, warm, args.steps, args.warmup, None if args.graph else prof, int(args.graph),
+                  max(1, args.stats_every))
     nf = sc.num_fluid
     stats = run["stats"]
     value = nf * args.steps / run["elapsed"]
@@ -318,8 +327,10 @@ def main() -> int:
                    "density_tol": sc.params.get("density_tol"),
                    "divergence_tol": sc.params.get("divergence_tol"),
                    "max_density_err": dens_err, "mean_rho_err": rho_mean,
-                   "live_after": run["n_live"], "hip_graph": bool(args.graph)},
+                   "live_after": run["n_live"], "hip_graph": bool(args.graph),
+                   "stats_every": max(1, args.stats_every)},
         "iters_per_step": iters,
+        "autotune": run["tuning"],
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

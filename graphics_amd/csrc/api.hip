@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -99,6 +100,8 @@ struct sphgpu_handle {
     std::vector<GraphEntry> graphs;
     unsigned long long graphClock = 0;
     bool capturing = false;
+    bool tuned = false;
+    float tuneMs[4] = {0, 0, 0, 0};    // density pass: map 0, 1; list builder: 8, 12 groups
     uint32_t gatherSeed = 1;
     // profiling
     std::string profName;
@@ -332,6 +335,59 @@ int solver_loop(sphgpu_handle* h, int which) {
     return solver_loop_finish(h, which, batch, profStart);
 }
 
+// ---- autotuner --------------------------------------------------------------
+// Before the first step, on the scene itself: the tile-to-workgroup mapping
+// (contiguous runs of tiles per XCD, or round robin) is timed on the solver's
+// density pass, and the LDS staging depth of the list builder (8 or 12
+// groups of four entries) on the list builder; each candidate runs three
+// times and the fastest minimum wins.  Every timed kernel recomputes its
+// outputs from positions alone, and both settings give identical results.
+float time_launches(sphgpu_handle* h, const std::function<void()>& f) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return 0.0f;
+    float best = 1e30f;
+    for (int r = 0; r < 3; ++r) {
+        hipEventRecord(a, h->st);
+        f();
+        hipEventRecord(b, h->st);
+        float t = 0.0f;
+        if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&t, a, b) == hipSuccess)
+            best = std::min(best, t);
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return best;
+}
+
+int autotune(sphgpu_handle* h) {
+    Dev& D = h->D;
+    D.tileMap = 1;
+    D.nbStage = 8;
+    run_ns(h);   // sorted layout and lists to time on (the first step sorts again: same order)
+    auto density = [h]() {
+        const Dev& D = h->D;
+        if (h->P.solver == SPHGPU_DFSPH) launch_df_density_alpha(D, h->st);
+        else if (h->P.solver == SPHGPU_IISPH) launch_ii_density(D, h->st);
+        else launch_wc_pressure(D, h->st);
+    };
+    float tm[2];
+    for (int m = 0; m < 2; ++m) {
+        D.tileMap = m;
+        tm[m] = time_launches(h, density);
+    }
+    D.tileMap = tm[0] < tm[1] ? 0 : 1;
+    float tn[2];
+    const int depth[2] = {8, 12};
+    for (int k = 0; k < 2; ++k) tn[k] = time_launches(h, [h, &depth, k]() { launch_neighbours_variant(h->D, depth[k], h->st); });
+    D.nbStage = tn[0] <= tn[1] ? 8 : 12;
+    launch_neighbours(D, h->st);   // lists of the chosen variant (identical either way)
+    h->tuneMs[0] = tm[0]; h->tuneMs[1] = tm[1]; h->tuneMs[2] = tn[0]; h->tuneMs[3] = tn[1];
+    h->tuned = true;
+    HIPCHECK(h, hipStreamSynchronize(h->st));
+    HIPCHECK(h, hipGetLastError());
+    return 0;
+}
+
 // ---- HIP graphs of whole step segments ------------------------------------
 // A segment is host code that only queues kernels (no readback).  Its first
 // run with a given device-state snapshot (the Dev struct: buffer pointers
@@ -378,6 +434,13 @@ int run_segment(sphgpu_handle* h, int seg, int iters, F&& body) {
 int finish_step(sphgpu_handle* h, sphgpu_stats* out) {
     Dev& D = h->D;
     int ns = D.nscenes;
+    if (!out) {
+        // no statistics asked for: nothing is read back and the host does not
+        // wait; the fault counters keep accumulating until a step that reports
+        // them, and the slot bound keeps its (larger, still valid) value
+        ++h->frame;
+        return 0;
+    }
     launch_stats(D, h->statsPart, h->statsOut, h->st);
     HIPCHECK(h, hipMemcpyAsync(h->hStats, h->statsOut, sizeof(double) * 4 * ns, hipMemcpyDeviceToHost, h->st));
     HIPCHECK(h, hipMemcpyAsync(h->hCounters, D.counters, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, h->st));
@@ -967,6 +1030,7 @@ int sphgpu_step(sphgpu_handle* h, sphgpu_stats* st) {
     int rc = sphgpu_init(h);
     if (rc) return rc;
     if (h->nLive == 0 && h->frame > 0) return fail(h, SPHGPU_E_STATE, "no fluid particles left");
+    if (!h->tuned && (rc = autotune(h))) return rc;
     for (auto& s : h->sceneStats) s = sphgpu_stats{};
     switch (h->P.solver) {
         case SPHGPU_WCSPH: return step_wcsph(h, st);
@@ -1178,6 +1242,16 @@ int sphgpu_kernel_timing(sphgpu_handle* h, long long* launches, double* total_ms
     if (launches) *launches = n;
     if (total_ms) *total_ms = ms;
     if (total_bytes) *total_bytes = bytes;
+    return SPHGPU_OK;
+}
+
+int sphgpu_get_tuning(const sphgpu_handle* h, int* choice, float* ms) {
+    if (!h || !choice) return SPHGPU_E_ARG;
+    if (!h->tuned) return SPHGPU_E_STATE;
+    choice[0] = h->D.tileMap;
+    choice[1] = h->D.nbStage;
+    if (ms)
+        for (int k = 0; k < 4; ++k) ms[k] = h->tuneMs[k];
     return SPHGPU_OK;
 }
 

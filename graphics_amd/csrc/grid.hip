@@ -165,14 +165,14 @@ __device__ __forceinline__ int2 run3(const int2* __restrict__ table, uint32_t k0
 
 // Lists are built in LDS, one column per thread, and written out group by
 // group (four entries, 16 bytes per particle), so each store instruction of a
-// wave writes whole cache lines.  Entries beyond the staged 32 are
-// written to global memory directly.
-constexpr int NB_STAGE_GROUPS = 8;
-
+// wave writes whole cache lines.  Entries beyond the staged 4 x NB_STAGE_GROUPS
+// are written to global memory directly; the depth (8 or 12 groups: LDS per
+// block against blocks per CU) is chosen by the autotuner.
+template <int NB_STAGE_GROUPS>
 __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     __shared__ int4 stage[NB_STAGE_GROUPS][256];
     const int tid = threadIdx.x;
-    const int i = xcd_block(blockIdx.x, gridDim.x) * 256 + tid;
+    const int i = tile_of_block(D.tileMap) * 256 + tid;
     int scene = 0;
     const bool act = i < D.npad && slot_valid(D, i, &scene);
     int* st = reinterpret_cast<int*>(&stage[0][0]);
@@ -386,9 +386,12 @@ void launch_cell_build(const Dev& D, const uint32_t* keys, int n, int2* table, h
     if (n > 0) hipLaunchKernelGGL(k_cell_build, dim3(blocks_for(n)), dim3(256), 0, st, keys, n, table);
 }
 
-void launch_neighbours(const Dev& D, hipStream_t st) {
-    hipLaunchKernelGGL(k_neighbours, dim3(blocks_for(D.npad)), dim3(256), 0, st, D);
+void launch_neighbours_variant(const Dev& D, int groups, hipStream_t st) {
+    if (groups >= 12) hipLaunchKernelGGL(k_neighbours<12>, dim3(blocks_for(D.npad)), dim3(256), 0, st, D);
+    else hipLaunchKernelGGL(k_neighbours<8>, dim3(blocks_for(D.npad)), dim3(256), 0, st, D);
 }
+
+void launch_neighbours(const Dev& D, hipStream_t st) { launch_neighbours_variant(D, D.nbStage, st); }
 
 void launch_boundary_keys_from(const Dev& D, const float4* xs, const int* scene, uint32_t* keys,
                                uint32_t* vals, hipStream_t st) {

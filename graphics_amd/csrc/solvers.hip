@@ -19,7 +19,7 @@ struct Slot {
 
 __device__ __forceinline__ Slot slot_of(const Dev& D, bool gated) {
     Slot s;
-    s.tile = xcd_block(blockIdx.x, gridDim.x);
+    s.tile = tile_of_block(D.tileMap);
     s.i = s.tile * TILE + threadIdx.x;
     s.scene = D.tileScene[s.tile];
     s.live = (int)threadIdx.x < D.tileValid[s.tile];

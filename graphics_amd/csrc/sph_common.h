@@ -58,6 +58,8 @@ struct Dev {
     float alpha_lat;   // DFSPH alpha of an interior lattice particle (feature scale)
     float dt_max;
     float warm_scale;  // PREVIOUS warm start: scale of the previous stiffness
+    int tileMap;       // 1: contiguous tile runs per XCD, 0: round robin (autotuned)
+    int nbStage;       // neighbour-list groups staged in LDS: 8 or 12 (autotuned)
     float gx, gy, gz;
     float cs, gamma, nu, omega;
     float rho0[MAX_PHASES], mass[MAX_PHASES], visc[MAX_PHASES], tait_B[MAX_PHASES];
@@ -158,6 +160,11 @@ constexpr int NUM_XCD = 8;
 __device__ __forceinline__ int xcd_block(int b, int n) {
     const int x = b % NUM_XCD, q = n / NUM_XCD, r = n % NUM_XCD;
     return x * q + min(x, r) + b / NUM_XCD;
+}
+
+// tile of a workgroup under the mapping the autotuner chose (Dev::tileMap)
+__device__ __forceinline__ int tile_of_block(int map) {
+    return map ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
 }
 
 __device__ __forceinline__ uint32_t morton2(uint32_t a, uint32_t b) {
@@ -320,6 +327,8 @@ void launch_reorder(const Dev& D, const uint32_t* sortedKeys, const uint32_t* so
 void launch_cell_reset(const Dev& D, const uint32_t* keys, int n, int2* table, hipStream_t st);
 void launch_cell_build(const Dev& D, const uint32_t* keys, int n, int2* table, hipStream_t st);
 void launch_neighbours(const Dev& D, hipStream_t st);
+// one of the two LDS staging depths of the list builder, for the autotuner
+void launch_neighbours_variant(const Dev& D, int groups, hipStream_t st);
 void launch_boundary_keys_from(const Dev& D, const float4* xs, const int* scene, uint32_t* keys,
                                uint32_t* vals, hipStream_t st);
 void launch_boundary_reorder(const Dev& D, const float4* srcX, const int* srcSolid,

@@ -110,6 +110,7 @@ SIGNATURES = {
     "sphgpu_profile_kernel": (C.c_int, [C.c_void_p, C.c_char_p]),
     "sphgpu_kernel_timing": (C.c_int, [C.c_void_p, _LP, _DP, _DP]),
     "sphgpu_copy_probe": (C.c_int, [C.c_longlong, _DP]),
+    "sphgpu_get_tuning": (C.c_int, [C.c_void_p, _IP, _FP]),
     "sphgpu_mlp_create": (C.c_int, [C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
     "sphgpu_mlp_destroy": (None, [C.c_void_p]),
     "sphgpu_mlp_set_params": (C.c_int, [C.c_void_p, _FP]),
@@ -327,7 +328,12 @@ class Simulation:
         self._check(self.lib.sphgpu_init(self.h), "init")
 
     # -- stepping
-    def step(self) -> dict:
+    def step(self, stats: bool = True):
+        """One step; returns the statistics dict, or None (and does not wait
+        for the device) when stats is False."""
+        if not stats:
+            self._check(self.lib.sphgpu_step(self.h, None), "step")
+            return None
         st = Stats()
         self._check(self.lib.sphgpu_step(self.h, C.byref(st)), "step")
         return st.as_dict()
@@ -383,6 +389,14 @@ class Simulation:
         self._check(self.lib.sphgpu_set_mlp_params(self.h, _fp(a)), "set_mlp_params")
 
     # -- measurement
+    def tuning(self) -> dict:
+        ch = (C.c_int * 2)()
+        ms = (C.c_float * 4)()
+        self._check(self.lib.sphgpu_get_tuning(self.h, ch, ms), "get_tuning")
+        return {"tile_map_xcd": bool(ch[0]), "list_stage_groups": ch[1],
+                "ms": {"density_round_robin": ms[0], "density_xcd": ms[1],
+                       "lists_8": ms[2], "lists_12": ms[3]}}
+
     def profile_kernel(self, name):
         self._check(self.lib.sphgpu_profile_kernel(self.h, None if name is None else name.encode()),
                     "profile_kernel")

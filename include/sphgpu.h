@@ -154,7 +154,10 @@ int sphgpu_add_solid(sphgpu_handle* h, int scene, long long n, const float* xyz,
 int sphgpu_init(sphgpu_handle* h);
 
 /* ---- stepping ---------------------------------------------------------- */
-/* One SphBase::runOneStep (SphBase.cpp:348-354) of the configured solver. */
+/* One SphBase::runOneStep (SphBase.cpp:348-354) of the configured solver.
+ * With st == NULL nothing is read back at the end of the step (the host does
+ * not wait for the device unless a solver loop must); the counters then
+ * accumulate until the next step that passes a stats struct. */
 int sphgpu_step(sphgpu_handle* h, sphgpu_stats* st);
 /* Per-scene statistics of the last step (batched scenes). */
 int sphgpu_scene_stats(const sphgpu_handle* h, int scene, sphgpu_stats* st);
@@ -187,6 +190,12 @@ int sphgpu_profile_kernel(sphgpu_handle* h, const char* name);
  * two device buffers with a plain streaming copy kernel (k_copy_probe), twice;
  * *ms receives the second launch's duration.  No handle needed. */
 int sphgpu_copy_probe(long long bytes, double* ms);
+/* Choices of the autotuner, which runs before the first step on the scene
+ * itself: choice[0] = tile mapping (1: contiguous tile runs per XCD, 0: round
+ * robin), choice[1] = LDS staging depth of the list builder (8 or 12 groups);
+ * ms (optional, 4 floats) = best times of the candidates (mapping 0, 1;
+ * depth 8, 12).  SPHGPU_E_STATE before the first step. */
+int sphgpu_get_tuning(const sphgpu_handle* h, int* choice, float* ms);
 int sphgpu_kernel_timing(sphgpu_handle* h, long long* launches, double* total_ms,
                          double* total_alg_bytes);
 

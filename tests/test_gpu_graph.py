@@ -35,3 +35,32 @@ def test_graph_replay_matches_eager(gpu, case):
     for a, b in zip(eager, graph):
         assert a == b, f"step {a[0]} differs"
 
+
+
+def test_autotuner_choices_reported(gpu):
+    sc = scenes.small_dam(sg.DFSPH, n=(10, 10, 10))
+    sim = scenes.make_simulation(sc)
+    with pytest.raises(sg.SphError):
+        sim.tuning()                     # runs before the first step
+    sim.step()
+    t = sim.tuning()
+    assert t["list_stage_groups"] in (8, 12)
+    assert all(v > 0 for v in t["ms"].values())
+    sim.close()
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_steps_without_statistics_match(gpu, use_graph):
+    """Steps that read nothing back (and do not wait) give the same state."""
+    sc = scenes.small_dam(sg.WCSPH, n=(8, 10, 8), box=(0.4, 0.3, 0.3))
+    a = scenes.make_simulation(sc, use_graph=use_graph)
+    b = scenes.make_simulation(sc, use_graph=use_graph)
+    for k in range(30):
+        a.step()
+        assert b.step(stats=(k % 7 == 6)) is None or k % 7 == 6
+    st = b.step()
+    a.step()
+    assert a.get(sg.F_POS).tobytes() == b.get(sg.F_POS).tobytes()
+    assert st["num_fluid"] == a.num_fluid()
+    a.close()
+    b.close()
