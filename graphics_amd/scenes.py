@@ -86,6 +86,22 @@ def small_dam(solver=sg.DFSPH, n=(10, 10, 10), box=(0.4, 0.3, 0.3), spacing=0.02
     return sc
 
 
+def two_layer(solver=sg.IISPH, n=(12, 16, 12), box=(0.4, 0.5, 0.4), spacing=0.02, jitter=0.01,
+              seed=1005, rho=(1000.0, 3000.0), amp=0.5, **kw) -> Scene:
+    """Two immiscible layers at rest densities rho[1] (below) and rho[0] (on
+    top); the interface height is perturbed by amp * spacing * sin(2 pi x / L)."""
+    sc = small_dam(solver, n=n, box=box, spacing=spacing, jitter=jitter, seed=seed, **kw)
+    sc.name = f"two_layer_{solver}"
+    sc.phases[:] = [(rho[0], 0.01), (rho[1], 0.01)]
+    xyz = sc.fluids[0][2]
+    length = n[0] * spacing
+    level = spacing + 0.5 * n[1] * spacing
+    iface = level + amp * spacing * np.sin(2.0 * np.pi * (xyz[:, 0] - spacing) / length)
+    heavy = xyz[:, 1] < iface
+    sc.fluids[:] = [(0, 1, xyz[heavy]), (0, 0, xyz[~heavy])]
+    return sc
+
+
 def wcsph_c1(seed=1001) -> Scene:
     """WCSPH 3-D dam break: 25^3 fluid at 0.02 m in a 1.0x0.6x0.6 m open-top box."""
     s = 0.02

@@ -168,6 +168,7 @@ class Oracle:
         k = self.lib.orc_add_fluid(self.h, len(x), x.ctypes.data_as(FP),
                                    None if v is None else v.ctypes.data_as(FP),
                                    None if i is None else i.ctypes.data_as(IP), rho0, visc)
+        self.nfluid_phases = max(self.nfluid_phases, k + 1)
         return k
 
     def add_solid(self, xyz, visc=0.0, dynamic=False) -> int:
@@ -197,10 +198,13 @@ class Oracle:
         self.lib.orc_get_fluid(self.h, k, field, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out if comp > 1 else out[:, 0]
 
-    def get_by_id(self, field, k=0):
-        ids = self.get(F_ID, k).astype(np.int64)
+    def get_by_id(self, field, k=None):
+        """Field values in ascending original id; k=None: all fluids (phases)."""
+        ks = range(self.nfluid_phases) if k is None else [k]
+        ids = np.concatenate([self.get(F_ID, q).astype(np.int64) for q in ks])
+        vals = np.concatenate([self.get(field, q) for q in ks])
         order = np.argsort(ids)
-        return ids[order], self.get(field, k)[order]
+        return ids[order], vals[order]
 
     def boundary_volume(self, k=0, n=None) -> np.ndarray:
         out = np.empty(n, np.float64)

@@ -926,10 +926,14 @@ public:
                 }
             }
             double sumErr = 0;
+            std::vector<std::vector<T>> newps(fluids.size()), errss(fluids.size());
             for (int k = 0; k < int(fluids.size()); ++k) {
                 auto& ps = fluids[k].parts; int n = int(ps.size());
                 T rho0 = fluids[k].rho0, fm0 = rho0 * v0;
-                std::vector<T> newp(n), errs(n);
+                std::vector<T>& newp = newps[k];
+                std::vector<T>& errs = errss[k];
+                newp.resize(n);
+                errs.resize(n);
 #pragma omp parallel for schedule(dynamic, dyn_chunk(n))
                 for (int i = 0; i < n; ++i) {
                     FluidPart<T>& a = ps[i];
@@ -955,9 +959,13 @@ public:
                     else errs[i] = (a.rho_adv + a.aii * a.p_l + fin - rho0) / rho0;
                     newp[i] = t;
                 }
-                for (int i = 0; i < n; ++i) {             /* Jacobi: publish after the sweep */
-                    ps[i].p_l = newp[i]; ps[i].p = newp[i];
-                    sumErr += double(errs[i]);
+            }
+            /* Jacobi over all phases: publish after the whole sweep */
+            for (int k = 0; k < int(fluids.size()); ++k) {
+                auto& ps = fluids[k].parts;
+                for (size_t i = 0; i < ps.size(); ++i) {
+                    ps[i].p_l = newps[k][i]; ps[i].p = newps[k][i];
+                    sumErr += double(errss[k][i]);
                 }
             }
             avgErr = ntot > 0 ? sumErr / ntot : 0.0;

@@ -210,3 +210,42 @@ def test_fault_counters(gpu):
     st = sim.step()
     assert st["max_iter_hit"] == 1 and st["iter_density"] == 7
     sim.close()
+
+
+@pytest.mark.parametrize("solver", [sg.WCSPH, sg.DFSPH, sg.IISPH])
+def test_two_phase_single_step_fields(gpu, solver):
+    """Two immiscible phases (rest densities 1000 and 3000, the light one on
+    top): the cross-phase terms (SP08 number density, WcSph.cpp:220-260;
+    IISph_M.cpp:533-536) against the FP64 oracle, one step."""
+    sc = scenes.two_layer(solver, **FIXED)
+    sim = scenes.make_simulation(sc)
+    st = sim.step()
+    o = orc.from_scene(sc, 64)
+    ost = o.step()
+    assert np.array_equal(sim.ids(), o.get_by_id(orc.F_ID)[0])
+    fields = [(sg.F_POS, orc.F_POS), (sg.F_VEL, orc.F_VEL), (sg.F_RHO, orc.F_RHO)]
+    if solver != sg.DFSPH:
+        fields += [(sg.F_PRESSURE, orc.F_PRESSURE), (sg.F_ACC, orc.F_ACC)]
+    errs = {f: _relerr(sim.get(f), o.get_by_id(of)[1]) for f, of in fields}
+    assert all(e < FIELD_TOL for e in errs.values()), errs
+    assert st["iter_density"] == ost["iter_density"]
+    # both phases are present and keep their identity
+    ph = sim.get(sg.F_PHASE)
+    assert set(np.unique(ph)) == {0.0, 1.0}
+    sim.close()
+
+
+def test_two_phase_multi_step_statistics(gpu):
+    sc = scenes.two_layer(sg.IISPH)
+    sim = scenes.make_simulation(sc)
+    o = orc.from_scene(sc, 64)
+    g = [sim.step() for _ in range(30)]
+    r = [o.step() for _ in range(30)]
+    for key in ("rho_mean_err", "rho_max_err"):
+        a = np.array([s[key] for s in g[-10:]])
+        b = np.array([s[key] for s in r[-10:]])
+        assert np.abs(a - b).max() < 5e-3, key
+    ke_g = np.array([s["kinetic_energy"] for s in g])
+    ke_r = np.array([s["kinetic_energy"] for s in r])
+    assert np.abs(ke_g - ke_r).max() / ke_r.max() < 0.03
+    sim.close()

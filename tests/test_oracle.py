@@ -271,3 +271,27 @@ def test_fault_iteration_cap_is_flagged():
     o = orc.from_scene(sc, 64)
     st = o.step()
     assert st["max_iter_hit"] == 1 and st["iter_density"] == 7
+
+
+def test_wcsph_two_phase_pressure_forces_conserve_momentum():
+    """Cross-phase pressure term (SP08, WcSph.cpp:240-244): -(1/m_a)(p_a/d_a^2 +
+    p_b/d_b^2) gradW with d = rho/m is antisymmetric in the pair, so with
+    viscosity off the mass-weighted accelerations of two phases of different
+    rest density cancel.  A dropped mass or a wrong d breaks the balance."""
+    s = 0.02
+    p = dict(solver=orc.WCSPH, spacing_r=s, smooth_radius_h=2 * s, dt=1e-4, sound_speed=20.0,
+             gravity=(0.0, 0.0, 0.0), space_min=(-1.0, -1.0, -1.0), space_max=(1.0, 1.0, 1.0))
+    o = orc.Oracle(p, 64)
+    xyz = sg.lattice((8, 8, 8), (0.0, 0.0, 0.0), s * 0.95, 0.03, 7, 0)
+    heavy = xyz[:, 1] < 3.5 * s
+    o.add_fluid(xyz[~heavy], 1000.0, 0.0)
+    o.add_fluid(xyz[heavy], 3000.0, 0.0)
+    o.init()
+    o.step()
+    m0, m1 = 1000.0 * s ** 3, 3000.0 * s ** 3
+    f0 = m0 * o.get(orc.F_ACC, 0)
+    f1 = m1 * o.get(orc.F_ACC, 1)
+    total = np.abs(f0.sum(0) + f1.sum(0)).max()
+    scale = np.abs(f0).sum() + np.abs(f1).sum()
+    assert scale > 0
+    assert total / scale < 1e-10
