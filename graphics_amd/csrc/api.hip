@@ -336,12 +336,15 @@ int solver_loop(sphgpu_handle* h, int which) {
 }
 
 // ---- autotuner --------------------------------------------------------------
-// Before the first step, on the scene itself: the tile-to-workgroup mapping
-// (contiguous runs of tiles per XCD, or round robin) is timed on the solver's
-// density pass, and the LDS staging depth of the list builder (8 or 12
-// groups of four entries) on the list builder; each candidate runs three
-// times and the fastest minimum wins.  Every timed kernel recomputes its
-// outputs from positions alone, and both settings give identical results.
+// Before the first step, on the scene itself, two choices are timed (three
+// runs each, the fastest minimum wins): the tile-to-workgroup mapping
+// (contiguous runs of tiles per XCD, or round robin) on the solver's density
+// pass, and the LDS staging depth of the list builder (8 or 12 groups) on
+// the list builder.  Every timed kernel recomputes its outputs from
+// positions alone, and every setting gives the same results.  (Padding the
+// lists of 16 or 32 lanes to common per-row counts, so that their gathers
+// share cache lines, was measured as a third choice: the density pass took
+// 0.124-0.127 ms against 0.086 unpadded on the 1M dam, so it was removed.)
 float time_launches(sphgpu_handle* h, const std::function<void()>& f) {
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return 0.0f;
@@ -370,18 +373,18 @@ int autotune(sphgpu_handle* h) {
         else if (h->P.solver == SPHGPU_IISPH) launch_ii_density(D, h->st);
         else launch_wc_pressure(D, h->st);
     };
-    float tm[2];
     for (int m = 0; m < 2; ++m) {
         D.tileMap = m;
-        tm[m] = time_launches(h, density);
+        h->tuneMs[m] = time_launches(h, density);
     }
-    D.tileMap = tm[0] < tm[1] ? 0 : 1;
-    float tn[2];
+    D.tileMap = h->tuneMs[0] < h->tuneMs[1] ? 0 : 1;
     const int depth[2] = {8, 12};
-    for (int k = 0; k < 2; ++k) tn[k] = time_launches(h, [h, &depth, k]() { launch_neighbours_variant(h->D, depth[k], h->st); });
-    D.nbStage = tn[0] <= tn[1] ? 8 : 12;
-    launch_neighbours(D, h->st);   // lists of the chosen variant (identical either way)
-    h->tuneMs[0] = tm[0]; h->tuneMs[1] = tm[1]; h->tuneMs[2] = tn[0]; h->tuneMs[3] = tn[1];
+    for (int k = 0; k < 2; ++k)
+        h->tuneMs[2 + k] = time_launches(h, [h, &depth, k]() {
+            launch_neighbours_variant(h->D, depth[k], h->st);
+        });
+    D.nbStage = h->tuneMs[2] <= h->tuneMs[3] ? 8 : 12;
+    launch_neighbours(D, h->st);   // lists of the chosen variant
     h->tuned = true;
     HIPCHECK(h, hipStreamSynchronize(h->st));
     HIPCHECK(h, hipGetLastError());

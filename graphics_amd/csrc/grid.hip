@@ -197,29 +197,30 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
         __syncthreads();
     };
     int nf = 0, nbn = 0, over = 0, zero = 0;
+    int stored = 0;   // fluid entries written (at most NBF_CAP)
     // fluid neighbours (SphBase.cpp:867-905)
-    if (act) {
-        for (int dz = -1; dz <= 1; ++dz) {
-            int z = cz + dz;
-            if (z < 0 || z >= G.nz) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                int y = cy + dy;
-                if (y < 0 || y >= G.ny) continue;
-                int2 rf = run3(D.cellF, key_of(G, D.rowRank, cx, y, z), cx, G.nx);
-                for (int j = rf.x; j < rf.y; ++j) {
-                    if (j == i) continue;
-                    float l2 = dis_square_exact(pa, D.x[j]);
-                    if (l2 < D.h2) {
-                        if (nf < NBF_CAP) put(D.nbF, nf, j);
-                        else ++over;
-                        ++nf;
-                        zero += (l2 == 0.0f);
+    for (int r = 0; r < 9; ++r) {
+        const int dz = r / 3 - 1, dy = r % 3 - 1;
+        const int z = cz + dz, y = cy + dy;
+        if (act && z >= 0 && z < G.nz && y >= 0 && y < G.ny) {
+            int2 rf = run3(D.cellF, key_of(G, D.rowRank, cx, y, z), cx, G.nx);
+            for (int j = rf.x; j < rf.y; ++j) {
+                if (j == i) continue;
+                float l2 = dis_square_exact(pa, D.x[j]);
+                if (l2 < D.h2) {
+                    if (stored < NBF_CAP) {
+                        put(D.nbF, stored, j);
+                        ++stored;
+                    } else {
+                        ++over;
                     }
+                    ++nf;
+                    zero += (l2 == 0.0f);
                 }
             }
         }
     }
-    const int cf = act ? min(nf, NBF_CAP) : 0;
+    const int cf = act ? stored : 0;
     for (int k = cf; k < ((cf + 3) & ~3); ++k) put(D.nbF, k, i);   // pad: self
     flush(D.nbF, cf);
     // boundary neighbours (SphBase.cpp:908-930)
@@ -246,7 +247,8 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     const int cb = act ? min(nbn, NBB_CAP) : 0;
     for (int k = cb; k < ((cb + 3) & ~3); ++k) put(D.nbB, k, D.nb);   // pad: far dummy
     flush(D.nbB, cb);
-    if (i < D.npad) D.cnt[i] = cf | (cb << 16);
+    // stored fluid entries | fluid neighbours found << 8 | boundary << 16
+    if (i < D.npad) D.cnt[i] = cf | (act ? min(nf, 255) : 0) << 8 | (cb << 16);
     if (!act) return;
     int tm = nf + nbn - REF_NEIGHBOUR_CAP;
     if (tm > 0) atomicAdd(&D.counters[C_TOO_MANY], tm);
@@ -330,7 +332,7 @@ __global__ __launch_bounds__(256) void k_neighbour_dump(Dev D, int cap, int* cou
     int o = rankOf[i];
     if (o < 0) return;
     const float4 pa = D.x[i];
-    const int cnt = D.cnt[i], nf = cnt & 0xFFFF, nbn = cnt >> 16;
+    const int cnt = D.cnt[i], nf = cnt & 0xFF, nbn = cnt >> 16;
     int c = 0;
     for (int k = 0; k < nf; ++k) {
         const int j = D.nbF[nb_slot(k, i, D.ncap)];
@@ -387,8 +389,9 @@ void launch_cell_build(const Dev& D, const uint32_t* keys, int n, int2* table, h
 }
 
 void launch_neighbours_variant(const Dev& D, int groups, hipStream_t st) {
-    if (groups >= 12) hipLaunchKernelGGL(k_neighbours<12>, dim3(blocks_for(D.npad)), dim3(256), 0, st, D);
-    else hipLaunchKernelGGL(k_neighbours<8>, dim3(blocks_for(D.npad)), dim3(256), 0, st, D);
+    const dim3 g(blocks_for(D.npad)), b(256);
+    if (groups >= 12) hipLaunchKernelGGL(k_neighbours<12>, g, b, 0, st, D);
+    else hipLaunchKernelGGL(k_neighbours<8>, g, b, 0, st, D);
 }
 
 void launch_neighbours(const Dev& D, hipStream_t st) { launch_neighbours_variant(D, D.nbStage, st); }

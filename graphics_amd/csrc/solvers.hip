@@ -34,7 +34,10 @@ __device__ __forceinline__ void write_partial(const Dev& D, int tile, int q, dou
     if (threadIdx.x == 0) D.partial[2 * tile + q] = t;
 }
 
-__device__ __forceinline__ int nf_of(int c) { return c & 0xffff; }
+// packed counts: stored fluid entries | fluid neighbours found (the stored
+// ones plus any beyond the list capacity, up to 255) << 8 | boundary << 16
+__device__ __forceinline__ int nf_of(int c) { return c & 0xff; }
+__device__ __forceinline__ int nf_real(int c) { return (c >> 8) & 0xff; }
 __device__ __forceinline__ int nb_of(int c) { return c >> 16; }
 
 __device__ __forceinline__ bool inside_domain(const SceneGrid& G, float4 x) {
@@ -419,7 +422,7 @@ __global__ __launch_bounds__(256) void k_df_warm_features(Dev D, float* __restri
     float dt = D.dt[s.scene], rho = D.rho[i], alpha = D.alpha[i];
     float4* f = reinterpret_cast<float4*>(feat) + 2 * (size_t)i;
     f[0] = make_float4((rho / rho0 - 1.0f) * 100.0f, asinhf(d * dt / rho0 * 1000.0f),
-                       (float)(nf_of(c) + nb_of(c)) * (1.0f / 30.0f),
+                       (float)(nf_real(c) + nb_of(c)) * (1.0f / 30.0f),
                        log1pf(fmaxf(D.kvsum[i] * dt / (alpha * rho0) * 1000.0f, 0.0f)));
     f[1] = make_float4(alpha / D.alpha_lat, sqrtf(va.x * va.x + va.y * va.y + va.z * va.z) * dt / sp,
                        sqrtf(bmin2) / sp, dt / D.dt_max);

@@ -96,7 +96,7 @@ struct Dev {
     // ---- neighbour lists (ELL, column k of particle i at k*ncap + i)
     int* nbF;
     int* nbB;
-    int* cnt;          // nf | nb << 16
+    int* cnt;          // stored fluid entries | fluid neighbours found << 8 | boundary << 16
 
     // ---- boundary (sorted by cell once)
     float4* xb;        // xyz, w = Akinci volume V_b
