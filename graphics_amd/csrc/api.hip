@@ -55,6 +55,10 @@ struct sphgpu_handle {
     std::vector<float> bx;
     std::vector<int> bsolid, bscene;
     std::vector<float> solidVisc;
+    std::vector<SolidPose> hsolids;    // per solid: pose, viscosity, dynamic flag, id range
+    bool posesDirty = false;
+    SolidPose* dSolids = nullptr;
+    double* dSolidFT = nullptr;        // [nsolids][6] force, torque of the last step
     long long nextId = 0;
 
     bool initialized = false;
@@ -492,6 +496,39 @@ int finish_step(sphgpu_handle* h, sphgpu_stats* out) {
     return 0;
 }
 
+// Dynamic solids, at the start of a step: new poses to the device, world
+// positions and velocities, boundary re-sort into cell order, cell ranges,
+// Akinci volumes (SphSolid::updatePaticles, SphBase::updateSolidPartWeight).
+int rigid_update(sphgpu_handle* h) {
+    Dev& D = h->D;
+    hipStream_t st = h->st;
+    if (h->posesDirty) {
+        HIPCHECK(h, hipMemcpyAsync(h->dSolids, h->hsolids.data(), sizeof(SolidPose) * D.nsolids,
+                                   hipMemcpyHostToDevice, st));
+        HIPCHECK(h, hipStreamSynchronize(st));   // the host table may change again right after
+        h->posesDirty = false;
+    }
+    launch_rigid_pose(D, st);
+    launch_boundary_keys_from(D, D.xb, D.bscene, h->sk, h->sv, st);
+    radix_sort_pairs(h->sk, h->sv, D.nb, h->keyBits, h->sort, st);
+    launch_boundary_regather(D, h->sk, h->sv, st);
+    std::swap(D.xb, D.xb2); std::swap(D.vb, D.vb2); std::swap(D.bloc, D.bloc2);
+    std::swap(D.bid, D.bid2); std::swap(D.bsolid, D.bsolid2); std::swap(D.bscene, D.bscene2);
+    std::swap(D.bkself, D.bkself2); std::swap(D.bkey, D.bkey2);
+    launch_cell_reset(D, D.bkey2, D.nb, D.cellB, st);
+    launch_cell_build(D, D.bkey, D.nb, D.cellB, st);
+    launch_boundary_volume_update(D, st);
+    return 0;
+}
+
+// reaction forces on dynamic solids and their totals (SphWithBullet::updateSolids,
+// the part before the rigid-body step); launched while x, v still hold x(t), v(t)
+void rigid_reaction(sphgpu_handle* h, int iisph) {
+    if (!h->D.anyDynamic) return;
+    launch_rigid_reaction(h->D, iisph, h->st);
+    launch_solid_reduce(h->D, h->dSolidFT, h->st);
+}
+
 int step_wcsph(sphgpu_handle* h, sphgpu_stats* st) {
     auto body = [h]() {
         Dev& D = h->D;
@@ -505,10 +542,15 @@ int step_wcsph(sphgpu_handle* h, sphgpu_stats* st) {
             ProfScope ps(h, "wc_force", nf_live(h) * 56.0);
             launch_wc_force_integrate(D, h->st);
         }
+        rigid_reaction(h, 0);
         std::swap(D.x, D.x2);
         std::swap(D.v, D.v2);
     };
-    if (h->P.use_graph) {
+    if (h->D.anyDynamic) {
+        int rc = rigid_update(h);
+        if (rc) return rc;
+    }
+    if (h->P.use_graph && !h->D.anyDynamic) {
         int rc = run_segment(h, 0, 0, body);
         if (rc) return rc;
     } else {
@@ -594,7 +636,8 @@ int step_iisph(sphgpu_handle* h, sphgpu_stats* st) {
         }
         solver_loop_start(h, 0, batch);
     };
-    if (h->P.use_graph) {
+    if (h->D.anyDynamic && (rc = rigid_update(h))) return rc;
+    if (h->P.use_graph && !h->D.anyDynamic) {
         if ((rc = run_segment(h, 1, batch, body))) return rc;
     } else {
         body();
@@ -604,6 +647,7 @@ int step_iisph(sphgpu_handle* h, sphgpu_stats* st) {
         ProfScope ps(h, "ii_force", nf_live(h) * 56.0);
         launch_ii_force_integrate(D, h->st);
     }
+    rigid_reaction(h, 1);
     std::swap(D.x, D.x2);
     std::swap(D.v, D.v2);
     return finish_step(h, st);
@@ -793,12 +837,78 @@ int sphgpu_add_solid(sphgpu_handle* h, int scene, long long n, const float* xyz,
     if (scene < 0 || scene >= h->P.num_scenes) return fail(h, SPHGPU_E_ARG, "scene %d out of range", scene);
     if (n < 0 || (n > 0 && !xyz)) return fail(h, SPHGPU_E_ARG, "bad particle array");
     int sid = int(h->solidVisc.size());
+    SolidPose sp{};
+    for (int k = 0; k < 4; ++k) sp.T[5 * k] = 1.0f;   // identity
+    sp.visc = visc_alpha;
+    sp.dynamic = 0;
+    sp.firstId = int(h->bsolid.size());
+    sp.endId = int(h->bsolid.size() + n);
+    h->hsolids.push_back(sp);
     h->solidVisc.push_back(visc_alpha);
     h->bx.insert(h->bx.end(), xyz, xyz + 3 * n);
     h->bsolid.insert(h->bsolid.end(), (size_t)n, sid);
     h->bscene.insert(h->bscene.end(), (size_t)n, scene);
     if (solid) *solid = sid;
     return SPHGPU_OK;
+}
+
+int sphgpu_add_rigid(sphgpu_handle* h, int scene, long long n, const float* local_xyz, float visc_alpha,
+                     int* solid) {
+    if (!h) return SPHGPU_E_ARG;
+    if (h->P.solver == SPHGPU_DFSPH)
+        return fail(h, SPHGPU_E_ARG, "dynamic solids are coupled for WCSPH and IISPH only");
+    int sid = -1;
+    int rc = sphgpu_add_solid(h, scene, n, local_xyz, visc_alpha, &sid);
+    if (rc) return rc;
+    h->hsolids[sid].dynamic = 1;
+    if (solid) *solid = sid;
+    return SPHGPU_OK;
+}
+
+int sphgpu_set_solid_pose(sphgpu_handle* h, int solid, const float T[16], const float com[3], const float v[3],
+                          const float w[3]) {
+    if (!h || !T || !com || !v || !w) return SPHGPU_E_ARG;
+    if (solid < 0 || solid >= (int)h->hsolids.size() || !h->hsolids[solid].dynamic)
+        return fail(h, SPHGPU_E_ARG, "solid %d is not a dynamic solid", solid);
+    for (int k = 0; k < 16; ++k)
+        if (!std::isfinite(T[k])) return fail(h, SPHGPU_E_ARG, "non-finite transform");
+    SolidPose& sp = h->hsolids[solid];
+    std::memcpy(sp.T, T, sizeof(float) * 16);
+    std::memcpy(sp.com, com, sizeof(float) * 3);
+    std::memcpy(sp.v, v, sizeof(float) * 3);
+    std::memcpy(sp.w, w, sizeof(float) * 3);
+    h->posesDirty = true;
+    return SPHGPU_OK;
+}
+
+int sphgpu_get_solid_force_torque(sphgpu_handle* h, int solid, double f[3], double t[3]) {
+    if (!h || !f || !t) return SPHGPU_E_ARG;
+    if (solid < 0 || solid >= (int)h->hsolids.size()) return fail(h, SPHGPU_E_ARG, "solid %d out of range", solid);
+    if (!h->initialized || !h->dSolidFT) return fail(h, SPHGPU_E_STATE, "no step has run");
+    double buf[6];
+    HIPCHECK(h, hipMemcpyAsync(buf, h->dSolidFT + 6 * solid, sizeof(buf), hipMemcpyDeviceToHost, h->st));
+    HIPCHECK(h, hipStreamSynchronize(h->st));
+    for (int q = 0; q < 3; ++q) {
+        f[q] = buf[q];
+        t[q] = buf[3 + q];
+    }
+    return SPHGPU_OK;
+}
+
+// world position of boundary sample i at init: as given for static solids; for
+// dynamic ones the body-frame sample placed by the pose set before init
+static void boundary_world(const sphgpu_handle* h, size_t i, float w[3]) {
+    const SolidPose& sp = h->hsolids[h->bsolid[i]];
+    const float x = h->bx[3 * i], y = h->bx[3 * i + 1], z = h->bx[3 * i + 2];
+    if (!sp.dynamic) {
+        w[0] = x; w[1] = y; w[2] = z;
+        return;
+    }
+    const float* T = sp.T;
+    const float a = T[3] * x + T[7] * y + T[11] * z + T[15];
+    w[0] = (T[0] * x + T[4] * y + T[8] * z + T[12]) / a;
+    w[1] = (T[1] * x + T[5] * y + T[9] * z + T[13]) / a;
+    w[2] = (T[2] * x + T[6] * y + T[10] * z + T[14]) / a;
 }
 
 int sphgpu_init(sphgpu_handle* h) {
@@ -818,8 +928,10 @@ int sphgpu_init(sphgpu_handle* h) {
     }
     for (size_t i = 0; i < h->bscene.size(); ++i) {
         const SceneDesc& d = h->scenes[h->bscene[i]];
+        float w[3];
+        boundary_world(h, i, w);
         for (int a = 0; a < 3; ++a)
-            if (!(h->bx[3 * i + a] >= d.mn[a] && h->bx[3 * i + a] <= d.mx[a]))
+            if (!(w[a] >= d.mn[a] && w[a] <= d.mx[a]))
                 return fail(h, SPHGPU_E_ARG, "boundary particle %zu outside its scene domain", i);
     }
     for (float sv : h->solidVisc)
@@ -913,6 +1025,23 @@ int sphgpu_init(sphgpu_handle* h) {
     // boundary arrays carry one extra dummy particle at index nb (list padding)
     ALLOC(D.xb, nb + 1); ALLOC(D.vb, nb + 1); ALLOC(D.bid, nb + 1); ALLOC(D.bsolid, nb + 1);
     ALLOC(D.bscene, nb + 1); ALLOC(D.bkself, nb + 1); ALLOC(D.bkey, nb + 1);
+    // solids: pose table (all solids), dynamic-solid state and re-sort targets
+    D.nsolids = int(h->hsolids.size());
+    D.anyDynamic = 0;
+    for (const auto& sp : h->hsolids) D.anyDynamic |= sp.dynamic;
+    if (D.nsolids > 0) {
+        ALLOC(h->dSolids, D.nsolids); ALLOC(h->dSolidFT, 6 * (size_t)D.nsolids);
+        D.solids = h->dSolids;
+        HIPCHECK(h, hipMemcpy(h->dSolids, h->hsolids.data(), sizeof(SolidPose) * D.nsolids, hipMemcpyHostToDevice));
+        HIPCHECK(h, hipMemset(h->dSolidFT, 0, sizeof(double) * 6 * D.nsolids));
+        h->posesDirty = false;
+    }
+    if (D.anyDynamic) {
+        ALLOC(D.bloc, nb + 1); ALLOC(D.bforce, nb + 1); ALLOC(D.brank, nb + 1);
+        ALLOC(D.xb2, nb + 1); ALLOC(D.vb2, nb + 1); ALLOC(D.bloc2, nb + 1); ALLOC(D.bid2, nb + 1);
+        ALLOC(D.bsolid2, nb + 1); ALLOC(D.bscene2, nb + 1); ALLOC(D.bkself2, nb + 1); ALLOC(D.bkey2, nb + 1);
+        HIPCHECK(h, hipMemset(D.bforce, 0, sizeof(float4) * (nb + 1)));
+    }
     SceneGrid* dgrids; int* drow; uint32_t* dkb;
     ALLOC(dgrids, ns); ALLOC(drow, rowRank.size()); ALLOC(dkb, ns + 1);
     D.grids = dgrids; D.rowRank = drow; D.sceneKeyBase = dkb;
@@ -1004,7 +1133,11 @@ int sphgpu_init(sphgpu_handle* h) {
     // boundary: sort by cell once, Akinci volumes (prepareSolidPartWeight)
     if (nb > 0) {
         std::vector<float4> hb(nb);
-        for (long long i = 0; i < nb; ++i) hb[i] = make_float4(h->bx[3 * i], h->bx[3 * i + 1], h->bx[3 * i + 2], 0.0f);
+        for (long long i = 0; i < nb; ++i) {
+            float w[3];
+            boundary_world(h, (size_t)i, w);
+            hb[i] = make_float4(w[0], w[1], w[2], 0.0f);
+        }
         float4* tx; int* tsolid; int* tscene;
         if ((rc = dalloc(h, &tx, nb))) return rc;
         if ((rc = dalloc(h, &tsolid, nb))) return rc;
@@ -1017,12 +1150,24 @@ int sphgpu_init(sphgpu_handle* h) {
         launch_boundary_reorder(D, tx, tsolid, tscene, h->sk, h->sv, st);
         launch_cell_build(D, D.bkey, int(nb), D.cellB, st);
         launch_boundary_volume(D, st);
+        if (D.anyDynamic) {
+            // body-frame samples in sorted order, then the per-step update once
+            std::vector<float4> hl(nb);
+            for (long long i = 0; i < nb; ++i) hl[i] = make_float4(h->bx[3 * i], h->bx[3 * i + 1], h->bx[3 * i + 2], 0.0f);
+            HIPCHECK(h, hipMemcpy(tx, hl.data(), sizeof(float4) * nb, hipMemcpyHostToDevice));
+            launch_boundary_init_local(D, tx, st);
+        }
     }
     {
         float4 far = make_float4(DUMMY_FAR, DUMMY_FAR, DUMMY_FAR, 0.0f), zero = make_float4(0, 0, 0, 0);
         HIPCHECK(h, hipMemcpyAsync(D.xb + nb, &far, sizeof(float4), hipMemcpyHostToDevice, st));
         HIPCHECK(h, hipMemcpyAsync(D.vb + nb, &zero, sizeof(float4), hipMemcpyHostToDevice, st));
+        if (D.anyDynamic) {
+            HIPCHECK(h, hipMemcpyAsync(D.xb2 + nb, &far, sizeof(float4), hipMemcpyHostToDevice, st));
+            HIPCHECK(h, hipMemcpyAsync(D.vb2 + nb, &zero, sizeof(float4), hipMemcpyHostToDevice, st));
+        }
     }
+    if (D.anyDynamic && (rc = rigid_update(h))) return rc;
     HIPCHECK(h, hipStreamSynchronize(st));
     HIPCHECK(h, hipGetLastError());
     return SPHGPU_OK;

@@ -322,6 +322,165 @@ __global__ __launch_bounds__(256) void k_boundary_volume(Dev D) {
     D.bkself[i] = sumSelf;
 }
 
+// ---- dynamic solids -----------------------------------------------------------
+// SphSolid::updatePaticles (SphBase.cpp:95-128): x = T p0 (projective divide),
+// v = v_lin + w x (x - com), for the particles of dynamic solids
+__global__ __launch_bounds__(256) void k_rigid_pose(Dev D) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= D.nb) return;
+    const SolidPose& S = D.solids[D.bsolid[i]];
+    if (!S.dynamic) return;
+    const float4 p = D.bloc[i];
+    const float* T = S.T;
+    float x = T[0] * p.x + T[4] * p.y + T[8] * p.z + T[12];
+    float y = T[1] * p.x + T[5] * p.y + T[9] * p.z + T[13];
+    float z = T[2] * p.x + T[6] * p.y + T[10] * p.z + T[14];
+    const float a = T[3] * p.x + T[7] * p.y + T[11] * p.z + T[15];
+    x /= a; y /= a; z /= a;
+    const float rx = x - S.com[0], ry = y - S.com[1], rz = z - S.com[2];
+    D.xb[i] = make_float4(x, y, z, D.xb[i].w);
+    D.vb[i] = make_float4(S.v[0] + (S.w[1] * rz - S.w[2] * ry), S.v[1] + (S.w[2] * rx - S.w[0] * rz),
+                          S.v[2] + (S.w[0] * ry - S.w[1] * rx), 0.0f);
+}
+
+// body-frame samples (original order) into sorted order; slot of each original index
+__global__ __launch_bounds__(256) void k_boundary_init_local(Dev D, const float4* __restrict__ local) {
+    int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= D.nb) return;
+    const int k = D.bid[r];
+    D.bloc[r] = local[k];
+    D.brank[k] = r;
+}
+
+// boundary arrays gathered into cell-key order after the solids moved
+__global__ __launch_bounds__(256) void k_boundary_regather(Dev D, const uint32_t* __restrict__ sk,
+                                                         const uint32_t* __restrict__ sv) {
+    int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= D.nb) return;
+    const int src = (int)sv[r];
+    D.xb2[r] = D.xb[src];
+    D.vb2[r] = D.vb[src];
+    D.bloc2[r] = D.bloc[src];
+    D.bid2[r] = D.bid[src];
+    D.bsolid2[r] = D.bsolid[src];
+    D.bscene2[r] = D.bscene[src];
+    D.bkself2[r] = D.bkself[src];
+    D.bkey2[r] = sk[r];
+    D.brank[D.bid[src]] = r;
+}
+
+// SphBase::updateSolidPartWeight (SphBase.cpp:817-841): V_b = 1 /
+// (kerSumFromSelf + sum of W over boundary neighbours of other solids; a
+// rigid body's own particles keep the part fixed at the start, kerSumFromSelf)
+__global__ __launch_bounds__(256) void k_boundary_volume_update(Dev D) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= D.nb) return;
+    const SceneGrid G = D.grids[D.bscene[i]];
+    float4 pa = D.xb[i];
+    const int sa = D.bsolid[i];
+    int cx, cy, cz;
+    cell_of(G, pa, cx, cy, cz);
+    float vol = D.bkself[i];
+    for (int dz = -1; dz <= 1; ++dz) {
+        int z = cz + dz;
+        if (z < 0 || z >= G.nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int y = cy + dy;
+            if (y < 0 || y >= G.ny) continue;
+            int2 rb = run3(D.cellB, key_of(G, D.rowRank, cx, y, z), cx, G.nx);
+            for (int j = rb.x; j < rb.y; ++j) {
+                if (j == i || D.bsolid[j] == sa) continue;
+                float l2 = dis_square_exact(pa, D.xb[j]);
+                if (l2 < D.h2) vol += kernel_W(sqrtf(l2), D);
+            }
+        }
+    }
+    pa.w = 1.0f / vol;
+    D.xb[i] = pa;
+}
+
+// Reaction force on the particles of dynamic solids from their fluid
+// neighbours, summed from the boundary particle's side (a fixed order):
+// WcSph::wc_boundPartForce_f (WcSph.cpp:314-362) and
+// IISph_M::ii_boundPartForcePressure_f (IISph_M.cpp:586-607).  Pressure
+// factor 2 for WCSPH (its fluid-side boundary force, WcSph.cpp:285), 1 for
+// IISPH (the fluid-side boundary force of this implementation); fluid
+// pressure and density: WCSPH f1 and v.w, IISPH kr and rho.
+__global__ __launch_bounds__(256) void k_rigid_reaction(Dev D, int iisph) {
+    int a = blockIdx.x * 256 + threadIdx.x;
+    if (a >= D.nb) return;
+    const SolidPose& S = D.solids[D.bsolid[a]];
+    if (!S.dynamic) return;
+    const SceneGrid G = D.grids[D.bscene[a]];
+    const float4 pa = D.xb[a], va = D.vb[a];
+    const float pf = iisph ? 1.0f : 2.0f;
+    const float nuc = 2.0f * S.visc * D.Hs * D.cs;
+    int cx, cy, cz;
+    cell_of(G, pa, cx, cy, cz);
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int z = cz + dz;
+        if (z < 0 || z >= G.nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int y = cy + dy;
+            if (y < 0 || y >= G.ny) continue;
+            int2 rf = run3(D.cellF, key_of(G, D.rowRank, cx, y, z), cx, G.nx);
+            for (int j = rf.x; j < rf.y; ++j) {
+                const float4 pb = D.x[j];
+                const float l2 = dis_square_exact(pa, pb);
+                if (!(l2 < D.h2) || l2 == 0.0f) continue;
+                const float4 vb = D.v[j];
+                const int ph = (int)pb.w;
+                const float frho0 = D.rho0[ph], fm0 = D.mass[ph];
+                const float rhob = iisph ? D.rho[j] : vb.w;
+                const float pb_p = iisph ? D.kr[j] : D.f1[j];
+                const float dx = pa.x - pb.x, dy2 = pa.y - pb.y, dz2 = pa.z - pb.z;
+                const float g = kernel_gradfac(sqrtf(l2), D);
+                const float cvol = -fm0 * frho0 * pa.w;
+                float force = pb_p > 0.0f ? g * (cvol * (pb_p / (rhob * rhob) * pf)) : 0.0f;
+                const float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy2 + (va.z - vb.z) * dz2;
+                if (pro < 0.0f) {
+                    const float nu = nuc / (rhob * 2.0f);
+                    const float pi = -nu * pro / (l2 + D.visc_eps);
+                    force += g * (cvol * pi);
+                }
+                fx += dx * force; fy += dy2 * force; fz += dz2 * force;
+            }
+        }
+    }
+    D.bforce[a] = make_float4(fx, fy, fz, 0.0f);
+}
+
+// SphSolid::rigidForceJoin (SphBase.cpp:131-143) per dynamic solid: one block
+// per solid over its particles in original order, in double, fixed tree
+__global__ __launch_bounds__(256) void k_solid_reduce(Dev D, double* __restrict__ out) {
+    __shared__ double lds[6][4];
+    const int s = blockIdx.x;
+    const SolidPose& S = D.solids[s];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (S.dynamic) {
+        for (int k = S.firstId + (int)threadIdx.x; k < S.endId; k += 256) {
+            const int r = D.brank[k];
+            const float4 f = D.bforce[r], x = D.xb[r];
+            const double rx = (double)x.x - S.com[0], ry = (double)x.y - S.com[1], rz = (double)x.z - S.com[2];
+            acc[0] += f.x; acc[1] += f.y; acc[2] += f.z;
+            acc[3] += ry * f.z - rz * f.y;
+            acc[4] += rz * f.x - rx * f.z;
+            acc[5] += rx * f.y - ry * f.x;
+        }
+    }
+    for (int q = 0; q < 6; ++q) {
+        double v = acc[q];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if ((threadIdx.x & 63) == 0) lds[q][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int q = threadIdx.x;
+        out[6 * s + q] = ((lds[q][0] + lds[q][1]) + lds[q][2]) + lds[q][3];
+    }
+}
+
 // test dump: the stored neighbour lists (the ones every pass iterates), with
 // original ids and correctly rounded distances
 __global__ __launch_bounds__(256) void k_neighbour_dump(Dev D, int cap, int* counts, int* kinds,
@@ -409,6 +568,30 @@ void launch_boundary_reorder(const Dev& D, const float4* srcX, const int* srcSol
 
 void launch_boundary_volume(const Dev& D, hipStream_t st) {
     hipLaunchKernelGGL(k_boundary_volume, dim3(blocks_for(D.nb)), dim3(256), 0, st, D);
+}
+
+void launch_rigid_reaction(const Dev& D, int iisph, hipStream_t st) {
+    if (D.nb > 0) hipLaunchKernelGGL(k_rigid_reaction, dim3(blocks_for(D.nb)), dim3(256), 0, st, D, iisph);
+}
+
+void launch_solid_reduce(const Dev& D, double* out, hipStream_t st) {
+    if (D.nsolids > 0) hipLaunchKernelGGL(k_solid_reduce, dim3(D.nsolids), dim3(256), 0, st, D, out);
+}
+
+void launch_boundary_init_local(const Dev& D, const float4* local, hipStream_t st) {
+    if (D.nb > 0) hipLaunchKernelGGL(k_boundary_init_local, dim3(blocks_for(D.nb)), dim3(256), 0, st, D, local);
+}
+
+void launch_rigid_pose(const Dev& D, hipStream_t st) {
+    if (D.nb > 0) hipLaunchKernelGGL(k_rigid_pose, dim3(blocks_for(D.nb)), dim3(256), 0, st, D);
+}
+
+void launch_boundary_regather(const Dev& D, const uint32_t* sk, const uint32_t* sv, hipStream_t st) {
+    if (D.nb > 0) hipLaunchKernelGGL(k_boundary_regather, dim3(blocks_for(D.nb)), dim3(256), 0, st, D, sk, sv);
+}
+
+void launch_boundary_volume_update(const Dev& D, hipStream_t st) {
+    if (D.nb > 0) hipLaunchKernelGGL(k_boundary_volume_update, dim3(blocks_for(D.nb)), dim3(256), 0, st, D);
 }
 
 void launch_neighbour_dump_ranked(const Dev& D, int cap, int* counts, int* kinds, long long* ids,

@@ -18,6 +18,19 @@ constexpr uint32_t KEY_INVALID = 0xFFFFFFFFu;
 // ranked in 2-D Z-order (Morton on cy, cz) so that a run of three x-adjacent
 // cells is contiguous in the sorted particle order while rows that are
 // neighbours in y/z stay close in memory.
+// Pose of a solid (SphSolid::updatePaticles, SphBase.cpp:95-128): OpenGL
+// column-major 4x4 transform of the body-frame samples, centre of mass,
+// linear and angular velocity; static solids keep their samples as given.
+struct SolidPose {
+    float T[16];
+    float com[3], v[3], w[3];
+    float visc;      // SphSolid::viscosity_alpha
+    int dynamic;
+    int firstId;     // original boundary indices [firstId, endId) belong to this solid
+    int endId;
+    int pad[3];
+};
+
 struct SceneGrid {
     float minx, miny, minz;  // spaceMin - 2*smoothRadius_h (SphBase.cpp:968)
     float divisor;           // 1/smoothRadius_h (SphBase.cpp:969)
@@ -106,6 +119,17 @@ struct Dev {
     int* bscene;
     float* bkself;     // kerSumFromSelf
     uint32_t* bkey;
+    // dynamic solids (rigid coupling)
+    int nsolids;
+    int anyDynamic;
+    const SolidPose* solids;   // [nsolids]
+    float4* bloc;              // body-frame sample of a dynamic solid's particle
+    float4* bforce;            // reaction force on the particle (last step)
+    int* brank;                // slot of original boundary index k
+    float4 *xb2, *vb2, *bloc2; // re-sort targets
+    int *bid2, *bsolid2, *bscene2;
+    float* bkself2;
+    uint32_t* bkey2;
 
     // ---- grid
     const SceneGrid* grids;
@@ -335,6 +359,15 @@ void launch_boundary_reorder(const Dev& D, const float4* srcX, const int* srcSol
                              const int* srcScene, const uint32_t* sortedKeys,
                              const uint32_t* sortedVals, hipStream_t st);
 void launch_boundary_volume(const Dev& D, hipStream_t st);
+// rigid coupling: world positions/velocities from the poses; re-sort of the
+// boundary after they moved (into the *2 arrays); V_b update; reaction forces
+// on dynamic solids' particles; per-solid force and torque (double[6] each)
+void launch_rigid_pose(const Dev& D, hipStream_t st);
+void launch_boundary_init_local(const Dev& D, const float4* local, hipStream_t st);
+void launch_boundary_regather(const Dev& D, const uint32_t* sk, const uint32_t* sv, hipStream_t st);
+void launch_boundary_volume_update(const Dev& D, hipStream_t st);
+void launch_rigid_reaction(const Dev& D, int iisph, hipStream_t st);
+void launch_solid_reduce(const Dev& D, double* out, hipStream_t st);
 void launch_neighbour_dump_ranked(const Dev& D, int cap, int* counts, int* kinds, long long* ids,
                                   float* dis, const int* rankOf, hipStream_t st);
 

@@ -25,6 +25,7 @@ class Scene:
     fluids: list = field(default_factory=list)         # (scene, phase, xyz[n,3])
     solids: list = field(default_factory=list)         # (scene, xyz[n,3], visc_alpha)
     domains: list = field(default_factory=list)        # (min[3], max[3]) per scene
+    rigids: list = field(default_factory=list)         # (scene, body-frame xyz[n,3], visc_alpha, pose)
 
     @property
     def num_fluid(self) -> int:
@@ -32,7 +33,7 @@ class Scene:
 
     @property
     def num_boundary(self) -> int:
-        return sum(len(s[1]) for s in self.solids)
+        return sum(len(s[1]) for s in self.solids) + sum(len(r[1]) for r in self.rigids)
 
     @property
     def num_scenes(self) -> int:
@@ -53,7 +54,30 @@ def make_simulation(sc: Scene, **overrides) -> sg.Simulation:
         sim.add_fluid(xyz, phase=ph, scene=s)
     for s, xyz, visc in sc.solids:
         sim.add_solid(xyz, visc_alpha=visc, scene=s)
+    for s, xyz, visc, pose in sc.rigids:
+        k = sim.add_rigid(xyz, visc_alpha=visc, scene=s)
+        sim.set_solid_pose(k, *pose)
     return sim
+
+
+def pose(angle_y=0.0, translation=(0.0, 0.0, 0.0), com=None, v=(0.0, 0.0, 0.0), w=(0.0, 0.0, 0.0)):
+    """(T[16] OpenGL column-major, com, v, w) for a rotation about +y by
+    angle_y followed by a translation; com defaults to the translation."""
+    c, s = np.cos(angle_y), np.sin(angle_y)
+    T = np.array([c, 0.0, -s, 0.0,  0.0, 1.0, 0.0, 0.0,  s, 0.0, c, 0.0,
+                  translation[0], translation[1], translation[2], 1.0])
+    return (T, tuple(translation) if com is None else tuple(com), tuple(v), tuple(w))
+
+
+def dam_with_box(solver=sg.IISPH, n=(12, 10, 12), box=(0.5, 0.4, 0.4), spacing=0.02, half=(0.05, 0.05, 0.05),
+                 centre=(0.31, 0.1, 0.14), **kw) -> Scene:
+    """Small dam break with a dynamic rigid box (its surface samples in body
+    coordinates, posed at `centre`) standing in the fluid's path."""
+    sc = small_dam(solver, n=n, box=box, spacing=spacing, **kw)
+    sc.name = f"dam_with_box_{solver}"
+    local = sg.box_walls((0.0, 0.0, 0.0), half, spacing, True)
+    sc.rigids.append((0, local, 0.01, pose(0.0, centre)))
+    return sc
 
 
 def _tank(scene: Scene, s: int, size, spacing, have_top, visc, origin=(0.0, 0.0, 0.0)):

@@ -98,6 +98,9 @@ SIGNATURES = {
     "sphgpu_add_phase": (C.c_int, [C.c_void_p, C.c_float, C.c_float, _IP]),
     "sphgpu_add_fluid": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_longlong, _FP, _FP, _LP]),
     "sphgpu_add_solid": (C.c_int, [C.c_void_p, C.c_int, C.c_longlong, _FP, C.c_float, _IP]),
+    "sphgpu_add_rigid": (C.c_int, [C.c_void_p, C.c_int, C.c_longlong, _FP, C.c_float, _IP]),
+    "sphgpu_set_solid_pose": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP, _FP, _FP]),
+    "sphgpu_get_solid_force_torque": (C.c_int, [C.c_void_p, C.c_int, _DP, _DP]),
     "sphgpu_init": (C.c_int, [C.c_void_p]),
     "sphgpu_step": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "sphgpu_scene_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Stats)]),
@@ -323,6 +326,25 @@ class Simulation:
         self._check(self.lib.sphgpu_add_solid(self.h, scene, len(x), _fp(x), visc_alpha, C.byref(sid)),
                     "add_solid")
         return sid.value
+
+    def add_rigid(self, local_xyz, visc_alpha=0.0, scene=0) -> int:
+        x = _f32(local_xyz).reshape(-1, 3)
+        sid = C.c_int()
+        self._check(self.lib.sphgpu_add_rigid(self.h, scene, len(x), _fp(x), visc_alpha, C.byref(sid)),
+                    "add_rigid")
+        return sid.value
+
+    def set_solid_pose(self, solid, T16, com, v=(0.0, 0.0, 0.0), w=(0.0, 0.0, 0.0)):
+        t, c, vv, ww = (_f32(a).ravel() for a in (T16, com, v, w))
+        self._check(self.lib.sphgpu_set_solid_pose(self.h, solid, _fp(t), _fp(c), _fp(vv), _fp(ww)),
+                    "set_solid_pose")
+
+    def solid_force_torque(self, solid):
+        f = np.zeros(3)
+        t = np.zeros(3)
+        self._check(self.lib.sphgpu_get_solid_force_torque(self.h, solid, f.ctypes.data_as(_DP),
+                                                           t.ctypes.data_as(_DP)), "solid_force_torque")
+        return f, t
 
     def init(self):
         self._check(self.lib.sphgpu_init(self.h), "init")

@@ -148,6 +148,23 @@ int sphgpu_add_fluid(sphgpu_handle* h, int scene, int phase, long long n,
  * visc_alpha is SphSolid::viscosity_alpha. */
 int sphgpu_add_solid(sphgpu_handle* h, int scene, long long n, const float* xyz,
                      float visc_alpha, int* solid);
+/* Appends a dynamic rigid body of n boundary particles given in body
+ * coordinates (SphWithBullet::addSolid with a rigid body, SphBase.h:315-365;
+ * innerPositions).  WCSPH and IISPH only.  Its world positions follow the
+ * pose set with sphgpu_set_solid_pose (identity until then). */
+int sphgpu_add_rigid(sphgpu_handle* h, int scene, long long n, const float* local_xyz,
+                     float visc_alpha, int* solid);
+/* Pose of a dynamic solid for the next step, from the host's rigid-body step
+ * (SphSolid::updatePaticles, SphBase.cpp:95-128): T is the OpenGL column-major
+ * 4x4 transform of the body-frame samples, com the centre of mass, v and w the
+ * linear and angular velocity (particle velocity v + w x (x - com)). */
+int sphgpu_set_solid_pose(sphgpu_handle* h, int solid, const float T[16], const float com[3],
+                          const float v[3], const float w[3]);
+/* Total force and torque about the centre of mass that the fluid exerted on a
+ * dynamic solid in the last step (SphSolid::rigidForceJoin, SphBase.cpp:131-143,
+ * from the reaction forces of WcSph.cpp:438-460 / IISph_M.cpp:660-681), summed
+ * in double in a fixed order. */
+int sphgpu_get_solid_force_torque(sphgpu_handle* h, int solid, double f[3], double t[3]);
 /* Uploads, sorts the boundary, and computes the Akinci boundary volumes
  * (SphBase::prepareSolidPartWeight, SphBase.cpp:766-813).  Called by the first
  * step if not called before. */

@@ -89,6 +89,9 @@ def load():
         lib.orc_add_fluid.argtypes = [C.c_void_p, C.c_int, FP, FP, IP, C.c_float, C.c_float]
         lib.orc_add_solid.argtypes = [C.c_void_p, C.c_int, FP, C.c_int, C.c_float]
         lib.orc_init.argtypes = [C.c_void_p]
+        lib.orc_set_solid_pose.argtypes = [C.c_void_p, C.c_int, FP, FP, FP, FP]
+        lib.orc_solid_force_torque.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double)]
         lib.orc_step.argtypes = [C.c_void_p, C.POINTER(OrcStats)]
         lib.orc_num_fluid.argtypes = [C.c_void_p, C.c_int]
         lib.orc_get_fluid.argtypes = [C.c_void_p, C.c_int, C.c_int, DP]
@@ -176,6 +179,19 @@ class Oracle:
         return self.lib.orc_add_solid(self.h, len(x), x.ctypes.data_as(C.POINTER(C.c_float)),
                                       int(dynamic), visc)
 
+    def set_solid_pose(self, k, T16, com, v=(0.0, 0.0, 0.0), w=(0.0, 0.0, 0.0)):
+        f = lambda a: np.ascontiguousarray(a, np.float32).ctypes.data_as(C.POINTER(C.c_float))
+        t, c, vv, ww = (np.ascontiguousarray(a, np.float32) for a in (T16, com, v, w))
+        if self.lib.orc_set_solid_pose(self.h, k, f(t), f(c), f(vv), f(ww)) != 0:
+            raise ValueError("set_solid_pose: not a dynamic solid")
+
+    def solid_force_torque(self, k):
+        f = np.zeros(3)
+        t = np.zeros(3)
+        self.lib.orc_solid_force_torque(self.h, k, f.ctypes.data_as(C.POINTER(C.c_double)),
+                                        t.ctypes.data_as(C.POINTER(C.c_double)))
+        return f, t
+
     def init(self):
         if self.lib.orc_init(self.h) != 0:
             raise ValueError("a particle lies outside [space_min, space_max]")
@@ -245,5 +261,8 @@ def from_scene(sc, bits=64, **overrides) -> Oracle:
         o.add_fluid(xyz, rho0, visc, ids=ids)
     for s, xyz, visc in sc.solids:
         o.add_solid(xyz, visc)
+    for s, xyz, visc, pose in getattr(sc, "rigids", []):
+        k = o.add_solid(xyz, visc, dynamic=True)
+        o.set_solid_pose(k, *pose)
     o.init()
     return o

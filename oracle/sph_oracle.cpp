@@ -22,7 +22,10 @@
  *     (IISph_M.cpp:81, 269, 277, 283-284, 350; SphBase.cpp:387-388);
  *   - IISPH: the pressure force from boundary neighbours uses p_a/rho_a^2,
  *     the term the pressure solve predicts, not 2 p_a/rho_a^2
- *     (IISph_M.cpp:558-559 against 127, 271, 338);
+ *     (IISph_M.cpp:558-559 against 127, 271, 338), and so does the reaction
+ *     force on dynamic solids (2 p_b/rho_b^2 at IISph_M.cpp:596);
+ *   - IISPH phases are solved jointly (the reference iterates phase by phase
+ *     with stale cross-phase pressures, IISph_M.cpp:85);
  *   - a zero distance pair is counted once per neighbour search and
  *     contributes W(0) to densities and nothing to gradient terms (the
  *     reference counts it per force evaluation, WcSph.cpp:160, and divides
@@ -96,6 +99,8 @@ struct Solid {
     bool dynamic, rigid;
     T visc_alpha;
     std::vector<BoundPart<T>> parts;
+    std::vector<V3<T>> inner;   /* innerPositions: body-frame samples of a dynamic solid */
+    V3<T> com;                  /* centre of mass of the last pose */
 };
 
 struct PartIdx {             /* SphBase.h:596-617 */
@@ -338,6 +343,83 @@ public:
         if (!fp_[f].x.inside(spMin, spMax)) --f;
         fp_.resize(f + 1);
         return num - (f + 1);
+    }
+
+    /* ------------------------------ rigids ----------------------------- */
+    /* SphSolid::updatePaticles, SphBase.cpp:95-128: world position from the
+     * OpenGL column-major transform (with the projective divide), velocity
+     * v + w x (x - com) */
+    void updateParticles(int k, const float* Tm, const float* c, const float* lv, const float* av) {
+        Solid<T>& so = solids[k];
+        so.com = V3<T>(c[0], c[1], c[2]);
+        V3<T> vl(lv[0], lv[1], lv[2]), w(av[0], av[1], av[2]);
+        for (size_t i = 0; i < so.inner.size(); ++i) {
+            const V3<T>& p0 = so.inner[i];
+            V3<T>& bp = so.parts[i].x;
+            bp.x = T(Tm[0]) * p0.x + T(Tm[4]) * p0.y + T(Tm[8]) * p0.z + T(Tm[12]);
+            bp.y = T(Tm[1]) * p0.x + T(Tm[5]) * p0.y + T(Tm[9]) * p0.z + T(Tm[13]);
+            bp.z = T(Tm[2]) * p0.x + T(Tm[6]) * p0.y + T(Tm[10]) * p0.z + T(Tm[14]);
+            T alpha = T(Tm[3]) * p0.x + T(Tm[7]) * p0.y + T(Tm[11]) * p0.z + T(Tm[15]);
+            bp.x /= alpha; bp.y /= alpha; bp.z /= alpha;
+            V3<T> rr = bp - so.com;
+            so.parts[i].v = vl + V3<T>(w.y * rr.z - w.z * rr.y, w.z * rr.x - w.x * rr.z, w.x * rr.y - w.y * rr.x);
+        }
+    }
+
+    /* Reaction force on the particles of dynamic solids from their fluid
+     * neighbours: WcSph::wc_boundPartForce_f (WcSph.cpp:314-362, loop
+     * 438-460) and IISph_M::ii_boundPartForcePressure_f (IISph_M.cpp:586-607,
+     * loop 660-681).  The pressure factor is 2 for WCSPH (as its fluid-side
+     * boundary force, WcSph.cpp:285) and 1 for IISPH, matching the fluid-side
+     * IISPH boundary force this restatement uses (see the header); adhesion is
+     * off (m_TH.bt = 0). */
+    void rigidReaction(bool iisph) {
+        T v0 = r * r * r, Hs = H, cs = T(P.sound_speed);
+        const T pf = iisph ? T(1) : T(2);
+        for (int k = 0; k < int(solids.size()); ++k) {
+            Solid<T>& so = solids[k];
+            if (!so.dynamic) continue;
+            T ralpha = so.visc_alpha;
+            int n = int(so.parts.size());
+#pragma omp parallel for
+            for (int i = 0; i < n; ++i) {
+                BoundPart<T>& a = so.parts[i];
+                a.force = V3<T>();
+                const NeigbStr<T>& L = nbS[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    if (!nb.pidx.isFluid()) continue;
+                    T dis = nb.dis;
+                    if (dis == 0) continue;
+                    const FluidPart<T>& b = fp(nb.pidx);
+                    T frho0 = fluids[nb.pidx.toFluidI()].rho0, fm0 = frho0 * v0;
+                    V3<T> xab = a.x - b.x;
+                    T grad = -Wg(dis) / dis, force = 0;
+                    if (b.p > 0) force += grad * (-fm0 * frho0 * a.volume * (b.p / (b.rho * b.rho) * pf));
+                    T pro = (a.v - b.v).dot(xab);
+                    if (pro < 0) {
+                        T nu = 2 * ralpha * Hs * cs / (b.rho * 2);
+                        T pi = -nu * pro / (dis * dis + T(0.01) * Hs * Hs);
+                        force += grad * (-fm0 * frho0 * a.volume * pi);
+                    }
+                    a.force += xab * force;
+                }
+            }
+        }
+    }
+
+    /* SphSolid::rigidForceJoin, SphBase.cpp:131-143, summed in double */
+    void rigidForceJoin(int k, double* f, double* t) const {
+        const Solid<T>& so = solids[k];
+        double tf[3] = {0, 0, 0}, tt[3] = {0, 0, 0};
+        for (const auto& bp : so.parts) {
+            double fx = double(bp.force.x), fy = double(bp.force.y), fz = double(bp.force.z);
+            double rx = double(bp.x.x) - double(so.com.x), ry = double(bp.x.y) - double(so.com.y),
+                   rz = double(bp.x.z) - double(so.com.z);
+            tf[0] += fx; tf[1] += fy; tf[2] += fz;
+            tt[0] += ry * fz - rz * fy; tt[1] += rz * fx - rx * fz; tt[2] += rx * fy - ry * fx;
+        }
+        for (int q = 0; q < 3; ++q) { f[q] = tf[q]; t[q] = tt[q]; }
     }
 
     /* ------------------------------ WCSPH ------------------------------ */
@@ -1076,6 +1158,7 @@ public:
             updateSolidPartWeight();
             wc_computePressure();
             wc_computeForce();
+            rigidReaction(false);
             st->removed = updateFluidsWC();
             fillStats(st);
         } else if (P.solver == ORC_DFSPH) {                                /* DFSPH.cpp:9-75 */
@@ -1091,6 +1174,7 @@ public:
             updateSolidPartWeight();
             ii_computePressure(st);
             ii_computeForcePressure();
+            rigidReaction(true);
             st->removed = updateFluidsII();
             fillStats(st);
         }
@@ -1184,8 +1268,28 @@ int orc_add_solid(orc_sim* s, int n, const float* xyz, int dynamic, float visc) 
             so.parts[i].x = V3<T>(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
             so.parts[i].d = T(S.P.spacing_r);
         }
+        if (so.dynamic) {   /* samples in body coordinates; identity pose until one is set */
+            so.inner.resize(n);
+            for (int i = 0; i < n; ++i) so.inner[i] = so.parts[i].x;
+        }
         S.solids.push_back(std::move(so));
         return int(S.solids.size()) - 1;
+    });
+}
+
+int orc_set_solid_pose(orc_sim* s, int k, const float* T16, const float* com, const float* v, const float* w) {
+    return dispatch(s, [&](auto& S) {
+        if (k < 0 || k >= int(S.solids.size()) || !S.solids[k].dynamic) return -1;
+        S.updateParticles(k, T16, com, v, w);
+        return 0;
+    });
+}
+
+int orc_solid_force_torque(orc_sim* s, int k, double* f, double* t) {
+    return dispatch(s, [&](auto& S) {
+        if (k < 0 || k >= int(S.solids.size())) return -1;
+        S.rigidForceJoin(k, f, t);
+        return 0;
     });
 }
 

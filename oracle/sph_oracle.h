@@ -85,6 +85,12 @@ void     orc_destroy(orc_sim* s);
 int      orc_add_fluid(orc_sim* s, int n, const float* xyz, const float* vel,
                        const int* ids, float rest_density, float visc_alpha);
 int      orc_add_solid(orc_sim* s, int n, const float* xyz, int dynamic, float visc_alpha);
+/* dynamic solid k: OpenGL column-major transform of its body-frame samples,
+ * centre of mass, linear and angular velocity (SphSolid::updatePaticles) */
+int      orc_set_solid_pose(orc_sim* s, int k, const float* T16, const float* com, const float* v,
+                            const float* w);
+/* SphSolid::rigidForceJoin of the last step's reaction forces */
+int      orc_solid_force_torque(orc_sim* s, int k, double* f, double* t);
 int      orc_init(orc_sim* s);                       /* prepareSolidPartWeight */
 int      orc_step(orc_sim* s, orc_stats* st);        /* one sphStep + runOneStep bookkeeping */
 int      orc_num_fluid(const orc_sim* s, int k);

@@ -77,6 +77,30 @@ def test_scene_calls_validate_without_a_device():
     sim.close()
 
 
+def test_rigid_calls_validate_without_a_device():
+    from graphics_amd import scenes
+    sim = sg.Simulation(sg.default_params(solver=sg.WCSPH))
+    static = sim.add_solid(np.zeros((3, 3)))
+    body = sim.add_rigid(np.ones((5, 3)), visc_alpha=0.02)
+    assert (static, body) == (0, 1)
+    sim.set_solid_pose(body, *scenes.pose(0.3, (0.1, 0.2, 0.3)))
+    with pytest.raises(sg.SphError):
+        sim.set_solid_pose(static, *scenes.pose())              # not a dynamic solid
+    with pytest.raises(sg.SphError):
+        sim.set_solid_pose(7, *scenes.pose())                   # no such solid
+    bad = np.array(scenes.pose()[0])
+    bad[5] = np.nan
+    with pytest.raises(sg.SphError):
+        sim.set_solid_pose(body, bad, (0, 0, 0))
+    with pytest.raises(sg.SphError):
+        sim.solid_force_torque(body)                            # no step has run
+    sim.close()
+    sim = sg.Simulation(sg.default_params(solver=sg.DFSPH))
+    with pytest.raises(sg.SphError):                            # coupled for WCSPH and IISPH only
+        sim.add_rigid(np.ones((5, 3)))
+    sim.close()
+
+
 def test_box_sampler_counts():
     # makeCubeSampleParts: 1.0x0.6x0.6 m at 0.02 m, open top 6,381 / closed 7,802;
     # 4x2x2 m at 0.01 m closed 400,002

@@ -295,3 +295,130 @@ def test_wcsph_two_phase_pressure_forces_conserve_momentum():
     scale = np.abs(f0).sum() + np.abs(f1).sum()
     assert scale > 0
     assert total / scale < 1e-10
+
+
+# ------------------------------------------------------------------ rigids ----
+def _plate_and_blob(solver, visc, vel_seed=3, gravity=(0.0, 0.0, 0.0)):
+    """A slightly compressed fluid blob resting on a dynamic plate (the only
+    solid), gravity off: the plate is the only external body."""
+    s = 0.02
+    p = dict(solver=solver, spacing_r=s, smooth_radius_h=2 * s, dt=1e-4, sound_speed=20.0, nu=0.0,
+             gravity=gravity, space_min=(-1.0, -1.0, -1.0), space_max=(1.0, 1.0, 1.0),
+             min_iter_density=3, max_iter_density=3)
+    if solver == orc.IISPH:
+        p["dt"] = 1e-3
+    o = orc.Oracle(p, 64)
+    xyz = sg.lattice((8, 6, 8), (0.01, 0.02, 0.01), s * 0.96, 0.02, 7, 0)
+    rng = np.random.default_rng(vel_seed)
+    vel = rng.normal(0, 0.2, xyz.shape).astype(np.float32)
+    vel[:, 1] -= 0.3                                    # moving onto the plate
+    o.add_fluid(xyz, 1000.0, visc, vel=vel)
+    plate = sg.lattice((10, 1, 10), (0.0, 0.0, 0.0), s, 0.0, 0, 0) - np.float32([0.0, 0.0, 0.0])
+    return o, plate
+
+
+def _identity_pose(com=(0.1, 0.0, 0.1)):
+    return scenes.pose(0.0, (0.0, 0.0, 0.0), com=com)
+
+
+@pytest.mark.parametrize("solver,visc", [(orc.WCSPH, 0.05), (orc.IISPH, 0.0)])
+def test_rigid_reaction_balances_fluid_force(solver, visc):
+    """Newton's third law across the fluid-solid interface: the reaction on a
+    dynamic plate (WcSph.cpp:314-362; IISph_M.cpp:586-607) is minus the mass-
+    weighted fluid acceleration it causes (WCSPH: pressure x2 and artificial
+    viscosity, WcSph.cpp:285-302; IISPH: pressure, the fluid-side term this
+    restatement uses).  A wrong factor, mass or sign breaks the balance."""
+    o, plate = _plate_and_blob(solver, visc)
+    k = o.add_solid(plate, visc, dynamic=True)
+    o.set_solid_pose(k, *_identity_pose())
+    o.init()
+    o.step()
+    m = 1000.0 * float(np.float32(0.02)) ** 3        # the spacing is an FP32 parameter
+    ff = m * o.get(orc.F_ACC).sum(0)
+    f, _ = o.solid_force_torque(k)
+    scale = np.abs(m * o.get(orc.F_ACC)).sum()
+    assert np.abs(f).max() > 1e-3 * scale          # the plate carries load
+    assert np.abs(ff + f).max() / scale < 1e-10
+
+
+def test_rigid_torque_shifts_with_the_reference_point():
+    """t(c2) - t(c1) = (c1 - c2) x F for the same forces (angular velocity 0,
+    so the velocities and forces do not depend on the centre of mass)."""
+    out = []
+    for com in ((0.1, 0.0, 0.1), (0.3, -0.2, 0.05)):
+        o, plate = _plate_and_blob(orc.WCSPH, 0.05)
+        k = o.add_solid(plate, 0.05, dynamic=True)
+        o.set_solid_pose(k, *_identity_pose(com))
+        o.init()
+        o.step()
+        out.append(o.solid_force_torque(k))
+    (f1, t1), (f2, t2) = out
+    np.testing.assert_allclose(f1, f2, rtol=0, atol=1e-14 * np.abs(f1).max())
+    c1, c2 = (np.float32([0.1, 0.0, 0.1]).astype(np.float64),    # the pose is FP32
+              np.float32([0.3, -0.2, 0.05]).astype(np.float64))
+    np.testing.assert_allclose(t2 - t1, np.cross(c1 - c2, f1), rtol=1e-9, atol=1e-12 * np.abs(f1).max())
+
+
+def test_rigid_pose_is_column_major():
+    """A body posed by T (rotation about +y, then a translation; OpenGL
+    column-major, SphBase.cpp:95-128) acts like its samples placed in the
+    world by the same map: equal fluid accelerations.  A transposed matrix
+    (row-major reading) rotates the other way and fails."""
+    ang, tr = 0.7, (0.21, -0.005, 0.17)
+    T, com, v, w = scenes.pose(ang, tr)
+    c, s_ = np.cos(ang), np.sin(ang)
+    local = sg.box_walls((0.0, 0.0, 0.0), (0.1, 0.02, 0.06), 0.02, True)
+    R = np.array([[c, 0, s_], [0, 1, 0], [-s_, 0, c]])
+    world = (local.astype(np.float64) @ R.T + np.array(tr)).astype(np.float32)
+    acc = []
+    for posed in (True, False):
+        o, _ = _plate_and_blob(orc.WCSPH, 0.05)
+        if posed:
+            k = o.add_solid(local, 0.05, dynamic=True)
+            o.set_solid_pose(k, T, com, v, w)
+        else:
+            o.add_solid(world, 0.05)
+        o.init()
+        o.step()
+        acc.append(o.get(orc.F_ACC))
+    assert np.abs(acc[0] - acc[1]).max() / np.abs(acc[1]).max() < 1e-5
+    # the mirror rotation (what a transposed T gives) changes the answer
+    o, _ = _plate_and_blob(orc.WCSPH, 0.05)
+    k = o.add_solid(local, 0.05, dynamic=True)
+    o.set_solid_pose(k, *scenes.pose(-ang, tr))
+    o.init()
+    o.step()
+    assert np.abs(o.get(orc.F_ACC) - acc[1]).max() / np.abs(acc[1]).max() > 1e-3
+
+
+def test_rigid_surface_velocity_is_v_plus_w_cross_r():
+    """Samples move with v + w x (x - com) (SphBase.cpp:95-128).  Fluid
+    co-moving with the body (same rigid motion) has zero relative velocity
+    to every sample and to itself, so its accelerations equal the all-at-rest
+    ones; a wrong sign or lever arm makes the viscosity act and fails."""
+    s = 0.02
+    p = dict(solver=orc.WCSPH, spacing_r=s, smooth_radius_h=2 * s, dt=1e-4, sound_speed=20.0,
+             gravity=(0.0, 0.0, 0.0), space_min=(-1.0, -1.0, -1.0), space_max=(1.0, 1.0, 1.0))
+    xyz = sg.lattice((8, 6, 8), (0.01, 0.02, 0.01), s * 0.96, 0.02, 7, 0)
+    plate = sg.lattice((10, 1, 10), (0.0, 0.0, 0.0), s, 0.0, 0, 0)
+    com, lin, ang = np.array([0.08, 0.05, 0.11]), np.array([0.1, -0.2, 0.05]), np.array([0.5, -1.0, 2.0])
+    acc = []
+    for moving in (False, True):
+        o = orc.Oracle(p, 64)
+        vel = (lin + np.cross(ang, xyz.astype(np.float64) - com)) if moving else np.zeros_like(xyz)
+        o.add_fluid(xyz, 1000.0, 0.2, vel=vel.astype(np.float32))
+        k = o.add_solid(plate, 0.2, dynamic=True)
+        T = scenes.pose()[0]
+        o.set_solid_pose(k, T, com, lin if moving else (0, 0, 0), ang if moving else (0, 0, 0))
+        o.init()
+        o.step()
+        acc.append(o.get(orc.F_ACC))
+    assert np.abs(acc[0] - acc[1]).max() / np.abs(acc[0]).max() < 1e-4
+    # control: the body alone spinning (fluid at rest) does change the answer
+    o = orc.Oracle(p, 64)
+    o.add_fluid(xyz, 1000.0, 0.2)
+    k = o.add_solid(plate, 0.2, dynamic=True)
+    o.set_solid_pose(k, scenes.pose()[0], com, (0, 0, 0), ang)
+    o.init()
+    o.step()
+    assert np.abs(o.get(orc.F_ACC) - acc[0]).max() / np.abs(acc[0]).max() > 1e-3
