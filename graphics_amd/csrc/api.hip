@@ -57,6 +57,9 @@ struct sphgpu_handle {
     std::vector<float> solidVisc;
     std::vector<SolidPose> hsolids;    // per solid: pose, viscosity, dynamic flag, id range
     bool posesDirty = false;
+    sphgpu_update_solids_fn updateFn = nullptr;   // host rigid-body step inside sphgpu_step
+    void* updateUser = nullptr;
+    bool inCallback = false;
     SolidPose* dSolids = nullptr;
     double* dSolidFT = nullptr;        // [nsolids][6] force, torque of the last step
     long long nextId = 0;
@@ -523,10 +526,22 @@ int rigid_update(sphgpu_handle* h) {
 
 // reaction forces on dynamic solids and their totals (SphWithBullet::updateSolids,
 // the part before the rigid-body step); launched while x, v still hold x(t), v(t)
-void rigid_reaction(sphgpu_handle* h, int iisph) {
+// and the cell ranges match them.  mode: 0 WCSPH, 1 IISPH, 2 DFSPH
+void rigid_reaction(sphgpu_handle* h, int mode) {
     if (!h->D.anyDynamic) return;
-    launch_rigid_reaction(h->D, iisph, h->st);
+    launch_rigid_reaction(h->D, mode, h->st);
     launch_solid_reduce(h->D, h->dSolidFT, h->st);
+}
+
+// the point of SphWithBullet::updateSolids in the reference's step: the
+// host's rigid-body step, if one is registered
+int update_solids_point(sphgpu_handle* h) {
+    if (!h->updateFn) return 0;
+    h->inCallback = true;
+    int rc = h->updateFn(h, h->updateUser);
+    h->inCallback = false;
+    if (rc) return fail(h, SPHGPU_E_CALLBACK, "update-solids callback returned %d", rc);
+    return 0;
 }
 
 int step_wcsph(sphgpu_handle* h, sphgpu_stats* st) {
@@ -556,7 +571,10 @@ int step_wcsph(sphgpu_handle* h, sphgpu_stats* st) {
     } else {
         body();
     }
-    return finish_step(h, st);
+    // WcSph.cpp:85; a failing callback still lets the step finish
+    const int rcb = update_solids_point(h);
+    const int rc = finish_step(h, st);
+    return rcb ? rcb : rc;
 }
 
 int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
@@ -573,7 +591,12 @@ int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
         launch_df_aadv(D, h->st);
     }
     if ((rc = solver_loop(h, 0))) return rc;
+    rigid_reaction(h, 2);
     launch_df_integrate(D, h->st);
+    // DFSPH.cpp:54: updateSolids between the density solve and the next
+    // search; a failing callback still lets the step finish
+    const int rcb = update_solids_point(h);
+    if (D.anyDynamic && (rc = rigid_update(h))) return rc;
     run_ns(h);
     {
         ProfScope ps(h, "df_density_alpha", nf_live(h) * 20.0);
@@ -611,7 +634,8 @@ int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
         ++h->learned;
     }
     std::swap(D.v, D.vs);
-    return finish_step(h, st);
+    rc = finish_step(h, st);
+    return rcb ? rcb : rc;
 }
 
 int step_iisph(sphgpu_handle* h, sphgpu_stats* st) {
@@ -650,7 +674,9 @@ int step_iisph(sphgpu_handle* h, sphgpu_stats* st) {
     rigid_reaction(h, 1);
     std::swap(D.x, D.x2);
     std::swap(D.v, D.v2);
-    return finish_step(h, st);
+    const int rcb = update_solids_point(h);            // IISph_M.cpp:64
+    rc = finish_step(h, st);
+    return rcb ? rcb : rc;
 }
 
 // host: uniform grid of a scene (SphBase::SpaceGridStruct::initialize, SphBase.cpp:963-988)
@@ -855,8 +881,6 @@ int sphgpu_add_solid(sphgpu_handle* h, int scene, long long n, const float* xyz,
 int sphgpu_add_rigid(sphgpu_handle* h, int scene, long long n, const float* local_xyz, float visc_alpha,
                      int* solid) {
     if (!h) return SPHGPU_E_ARG;
-    if (h->P.solver == SPHGPU_DFSPH)
-        return fail(h, SPHGPU_E_ARG, "dynamic solids are coupled for WCSPH and IISPH only");
     int sid = -1;
     int rc = sphgpu_add_solid(h, scene, n, local_xyz, visc_alpha, &sid);
     if (rc) return rc;
@@ -878,6 +902,14 @@ int sphgpu_set_solid_pose(sphgpu_handle* h, int solid, const float T[16], const 
     std::memcpy(sp.v, v, sizeof(float) * 3);
     std::memcpy(sp.w, w, sizeof(float) * 3);
     h->posesDirty = true;
+    return SPHGPU_OK;
+}
+
+int sphgpu_set_update_solids(sphgpu_handle* h, sphgpu_update_solids_fn fn, void* user) {
+    if (!h) return SPHGPU_E_ARG;
+    if (h->inCallback) return fail(h, SPHGPU_E_STATE, "set_update_solids inside the callback");
+    h->updateFn = fn;
+    h->updateUser = user;
     return SPHGPU_OK;
 }
 
@@ -1151,20 +1183,26 @@ int sphgpu_init(sphgpu_handle* h) {
         launch_cell_build(D, D.bkey, int(nb), D.cellB, st);
         launch_boundary_volume(D, st);
         if (D.anyDynamic) {
-            // body-frame samples in sorted order, then the per-step update once
+            // body-frame samples in sorted order, then the per-step update once.
+            // tx is overwritten by a copy on the null stream, which does not
+            // wait for the non-blocking stream: the kernels above that read
+            // the world positions in tx must have finished first.
             std::vector<float4> hl(nb);
             for (long long i = 0; i < nb; ++i) hl[i] = make_float4(h->bx[3 * i], h->bx[3 * i + 1], h->bx[3 * i + 2], 0.0f);
+            HIPCHECK(h, hipStreamSynchronize(st));
             HIPCHECK(h, hipMemcpy(tx, hl.data(), sizeof(float4) * nb, hipMemcpyHostToDevice));
             launch_boundary_init_local(D, tx, st);
         }
     }
     {
-        float4 far = make_float4(DUMMY_FAR, DUMMY_FAR, DUMMY_FAR, 0.0f), zero = make_float4(0, 0, 0, 0);
-        HIPCHECK(h, hipMemcpyAsync(D.xb + nb, &far, sizeof(float4), hipMemcpyHostToDevice, st));
-        HIPCHECK(h, hipMemcpyAsync(D.vb + nb, &zero, sizeof(float4), hipMemcpyHostToDevice, st));
+        // list-padding dummy at index nb (synchronous copies from the stack)
+        const float4 far = make_float4(DUMMY_FAR, DUMMY_FAR, DUMMY_FAR, 0.0f), zero = make_float4(0, 0, 0, 0);
+        HIPCHECK(h, hipStreamSynchronize(st));
+        HIPCHECK(h, hipMemcpy(D.xb + nb, &far, sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHECK(h, hipMemcpy(D.vb + nb, &zero, sizeof(float4), hipMemcpyHostToDevice));
         if (D.anyDynamic) {
-            HIPCHECK(h, hipMemcpyAsync(D.xb2 + nb, &far, sizeof(float4), hipMemcpyHostToDevice, st));
-            HIPCHECK(h, hipMemcpyAsync(D.vb2 + nb, &zero, sizeof(float4), hipMemcpyHostToDevice, st));
+            HIPCHECK(h, hipMemcpy(D.xb2 + nb, &far, sizeof(float4), hipMemcpyHostToDevice));
+            HIPCHECK(h, hipMemcpy(D.vb2 + nb, &zero, sizeof(float4), hipMemcpyHostToDevice));
         }
     }
     if (D.anyDynamic && (rc = rigid_update(h))) return rc;
@@ -1175,6 +1213,7 @@ int sphgpu_init(sphgpu_handle* h) {
 
 int sphgpu_step(sphgpu_handle* h, sphgpu_stats* st) {
     if (!h) return SPHGPU_E_ARG;
+    if (h->inCallback) return fail(h, SPHGPU_E_STATE, "sphgpu_step inside the update-solids callback");
     int rc = sphgpu_init(h);
     if (rc) return rc;
     if (h->nLive == 0 && h->frame > 0) return fail(h, SPHGPU_E_STATE, "no fluid particles left");

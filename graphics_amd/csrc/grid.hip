@@ -400,21 +400,29 @@ __global__ __launch_bounds__(256) void k_boundary_volume_update(Dev D) {
 }
 
 // Reaction force on the particles of dynamic solids from their fluid
-// neighbours, summed from the boundary particle's side (a fixed order):
-// WcSph::wc_boundPartForce_f (WcSph.cpp:314-362) and
-// IISph_M::ii_boundPartForcePressure_f (IISph_M.cpp:586-607).  Pressure
-// factor 2 for WCSPH (its fluid-side boundary force, WcSph.cpp:285), 1 for
-// IISPH (the fluid-side boundary force of this implementation); fluid
-// pressure and density: WCSPH f1 and v.w, IISPH kr and rho.
-__global__ __launch_bounds__(256) void k_rigid_reaction(Dev D, int iisph) {
+// neighbours, summed from the boundary particle's side (a fixed order).
+// mode 0 WCSPH, WcSph::wc_boundPartForce_f (WcSph.cpp:314-362): pressure x2
+// (its fluid-side boundary force, WcSph.cpp:285) and artificial viscosity;
+// fluid pressure f1, density v.w.
+// mode 1 IISPH, IISph_M::ii_boundPartForcePressure_f (IISph_M.cpp:586-607):
+// pressure x1 (the fluid-side boundary force of this implementation) and
+// viscosity; pressure kr, density rho.
+// mode 2 DFSPH, the constant-density solve's boundary term accumulated over
+// its iterations (DFSPH.cpp:674-690, scaled by the fluid mass at 723-737):
+// F_b = sum_j m_j rho0_j V_b (sum_it kappa_j / rho_j) grad_j W_jb, with
+// sum_it kappa_j dt^2 = ksum_j; no viscosity.
+__global__ __launch_bounds__(256) void k_rigid_reaction(Dev D, int mode) {
     int a = blockIdx.x * 256 + threadIdx.x;
     if (a >= D.nb) return;
     const SolidPose& S = D.solids[D.bsolid[a]];
     if (!S.dynamic) return;
     const SceneGrid G = D.grids[D.bscene[a]];
     const float4 pa = D.xb[a], va = D.vb[a];
+    const bool iisph = mode == 1, dfsph = mode == 2;
     const float pf = iisph ? 1.0f : 2.0f;
     const float nuc = 2.0f * S.visc * D.Hs * D.cs;
+    const float dt = D.dt[D.bscene[a]];
+    const float rdt2 = 1.0f / (dt * dt);
     int cx, cy, cz;
     cell_of(G, pa, cx, cy, cz);
     float fx = 0.f, fy = 0.f, fz = 0.f;
@@ -429,13 +437,19 @@ __global__ __launch_bounds__(256) void k_rigid_reaction(Dev D, int iisph) {
                 const float4 pb = D.x[j];
                 const float l2 = dis_square_exact(pa, pb);
                 if (!(l2 < D.h2) || l2 == 0.0f) continue;
-                const float4 vb = D.v[j];
                 const int ph = (int)pb.w;
                 const float frho0 = D.rho0[ph], fm0 = D.mass[ph];
-                const float rhob = iisph ? D.rho[j] : vb.w;
-                const float pb_p = iisph ? D.kr[j] : D.f1[j];
                 const float dx = pa.x - pb.x, dy2 = pa.y - pb.y, dz2 = pa.z - pb.z;
                 const float g = kernel_gradfac(sqrtf(l2), D);
+                if (dfsph) {
+                    const float k = D.ksum[j] * rdt2 / D.rho[j];
+                    const float force = -g * (fm0 * frho0 * pa.w * k);
+                    fx += dx * force; fy += dy2 * force; fz += dz2 * force;
+                    continue;
+                }
+                const float4 vb = D.v[j];
+                const float rhob = iisph ? D.rho[j] : vb.w;
+                const float pb_p = iisph ? D.kr[j] : D.f1[j];
                 const float cvol = -fm0 * frho0 * pa.w;
                 float force = pb_p > 0.0f ? g * (cvol * (pb_p / (rhob * rhob) * pf)) : 0.0f;
                 const float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy2 + (va.z - vb.z) * dz2;
@@ -570,8 +584,8 @@ void launch_boundary_volume(const Dev& D, hipStream_t st) {
     hipLaunchKernelGGL(k_boundary_volume, dim3(blocks_for(D.nb)), dim3(256), 0, st, D);
 }
 
-void launch_rigid_reaction(const Dev& D, int iisph, hipStream_t st) {
-    if (D.nb > 0) hipLaunchKernelGGL(k_rigid_reaction, dim3(blocks_for(D.nb)), dim3(256), 0, st, D, iisph);
+void launch_rigid_reaction(const Dev& D, int mode, hipStream_t st) {
+    if (D.nb > 0) hipLaunchKernelGGL(k_rigid_reaction, dim3(blocks_for(D.nb)), dim3(256), 0, st, D, mode);
 }
 
 void launch_solid_reduce(const Dev& D, double* out, hipStream_t st) {

@@ -366,7 +366,7 @@ void launch_rigid_pose(const Dev& D, hipStream_t st);
 void launch_boundary_init_local(const Dev& D, const float4* local, hipStream_t st);
 void launch_boundary_regather(const Dev& D, const uint32_t* sk, const uint32_t* sv, hipStream_t st);
 void launch_boundary_volume_update(const Dev& D, hipStream_t st);
-void launch_rigid_reaction(const Dev& D, int iisph, hipStream_t st);
+void launch_rigid_reaction(const Dev& D, int mode, hipStream_t st);   // 0 WCSPH, 1 IISPH, 2 DFSPH
 void launch_solid_reduce(const Dev& D, double* out, hipStream_t st);
 void launch_neighbour_dump_ranked(const Dev& D, int cap, int* counts, int* kinds, long long* ids,
                                   float* dis, const int* rankOf, hipStream_t st);

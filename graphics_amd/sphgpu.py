@@ -20,7 +20,11 @@ F_POS, F_VEL, F_RHO, F_PRESSURE, F_ACC, F_ALPHA, F_KAPPA_SUM, F_PHASE, F_KAPPAV_
 FIELD_COMPONENTS = {F_POS: 3, F_VEL: 3, F_RHO: 1, F_PRESSURE: 1, F_ACC: 3, F_ALPHA: 1,
                     F_KAPPA_SUM: 1, F_PHASE: 1, F_KAPPAV_SUM: 1}
 
-ERRORS = {0: "OK", -1: "E_ARG", -2: "E_STATE", -3: "E_DEVICE", -4: "E_CAPACITY", -5: "E_DIVERGED"}
+ERRORS = {0: "OK", -1: "E_ARG", -2: "E_STATE", -3: "E_DEVICE", -4: "E_CAPACITY", -5: "E_DIVERGED",
+          -6: "E_CALLBACK"}
+
+# int (*)(sphgpu_handle*, void* user)
+UPDATE_SOLIDS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
 
 
 class Params(C.Structure):
@@ -101,6 +105,7 @@ SIGNATURES = {
     "sphgpu_add_rigid": (C.c_int, [C.c_void_p, C.c_int, C.c_longlong, _FP, C.c_float, _IP]),
     "sphgpu_set_solid_pose": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP, _FP, _FP]),
     "sphgpu_get_solid_force_torque": (C.c_int, [C.c_void_p, C.c_int, _DP, _DP]),
+    "sphgpu_set_update_solids": (C.c_int, [C.c_void_p, UPDATE_SOLIDS_FN, C.c_void_p]),
     "sphgpu_init": (C.c_int, [C.c_void_p]),
     "sphgpu_step": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "sphgpu_scene_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Stats)]),
@@ -288,7 +293,11 @@ class Simulation:
     # -- helpers
     def _check(self, rc: int, what: str):
         if rc != 0:
-            raise SphError(rc, f"{what}: {self.lib.sphgpu_last_error(self.h).decode()}")
+            err = SphError(rc, f"{what}: {self.lib.sphgpu_last_error(self.h).decode()}")
+            cause, self._cb_error = getattr(self, "_cb_error", None), None
+            if cause is not None:
+                raise err from cause
+            raise err
 
     def close(self):
         if self.h:
@@ -345,6 +354,27 @@ class Simulation:
         self._check(self.lib.sphgpu_get_solid_force_torque(self.h, solid, f.ctypes.data_as(_DP),
                                                            t.ctypes.data_as(_DP)), "solid_force_torque")
         return f, t
+
+    def set_update_solids(self, fn):
+        """fn(sim) runs inside step() where the reference calls
+        SphWithBullet::updateSolids; it may read solid_force_torque and call
+        set_solid_pose.  An exception in fn fails the step (E_CALLBACK) and
+        is re-raised from step().  fn=None removes it."""
+        if fn is None:
+            self._update_cb = None
+            self._check(self.lib.sphgpu_set_update_solids(self.h, UPDATE_SOLIDS_FN(), None), "set_update_solids")
+            return
+
+        def trampoline(_h, _user):
+            try:
+                fn(self)
+                return 0
+            except BaseException as e:        # no exception may cross the C frame
+                self._cb_error = e
+                return 1
+
+        self._update_cb = UPDATE_SOLIDS_FN(trampoline)   # kept alive with the handle
+        self._check(self.lib.sphgpu_set_update_solids(self.h, self._update_cb, None), "set_update_solids")
 
     def init(self):
         self._check(self.lib.sphgpu_init(self.h), "init")

@@ -38,7 +38,8 @@ enum sphgpu_status {
     SPHGPU_E_STATE = -2,    /* call not valid in the current state */
     SPHGPU_E_DEVICE = -3,   /* HIP runtime / device error, or no device */
     SPHGPU_E_CAPACITY = -4, /* a fixed capacity would be exceeded */
-    SPHGPU_E_DIVERGED = -5  /* non-finite particle state detected */
+    SPHGPU_E_DIVERGED = -5, /* non-finite particle state detected */
+    SPHGPU_E_CALLBACK = -6  /* the update-solids callback returned nonzero */
 };
 
 enum sphgpu_solver {
@@ -150,21 +151,37 @@ int sphgpu_add_solid(sphgpu_handle* h, int scene, long long n, const float* xyz,
                      float visc_alpha, int* solid);
 /* Appends a dynamic rigid body of n boundary particles given in body
  * coordinates (SphWithBullet::addSolid with a rigid body, SphBase.h:315-365;
- * innerPositions).  WCSPH and IISPH only.  Its world positions follow the
- * pose set with sphgpu_set_solid_pose (identity until then). */
+ * innerPositions).  Its world positions follow the pose set with
+ * sphgpu_set_solid_pose (identity until then).  Before sphgpu_init only. */
 int sphgpu_add_rigid(sphgpu_handle* h, int scene, long long n, const float* local_xyz,
                      float visc_alpha, int* solid);
-/* Pose of a dynamic solid for the next step, from the host's rigid-body step
+/* Pose of a dynamic solid from the host's rigid-body step
  * (SphSolid::updatePaticles, SphBase.cpp:95-128): T is the OpenGL column-major
  * 4x4 transform of the body-frame samples, com the centre of mass, v and w the
- * linear and angular velocity (particle velocity v + w x (x - com)). */
+ * linear and angular velocity (particle velocity v + w x (x - com)).  Takes
+ * effect at the next solids update point of sphgpu_step (see
+ * sphgpu_set_update_solids); set before sphgpu_init it places the body. */
 int sphgpu_set_solid_pose(sphgpu_handle* h, int solid, const float T[16], const float com[3],
                           const float v[3], const float w[3]);
 /* Total force and torque about the centre of mass that the fluid exerted on a
- * dynamic solid in the last step (SphSolid::rigidForceJoin, SphBase.cpp:131-143,
- * from the reaction forces of WcSph.cpp:438-460 / IISph_M.cpp:660-681), summed
- * in double in a fixed order. */
+ * dynamic solid (SphSolid::rigidForceJoin, SphBase.cpp:131-143), summed in
+ * double in a fixed order: WCSPH and IISPH the reaction of the last step
+ * (WcSph.cpp:438-460, IISph_M.cpp:660-681); DFSPH the pressure impulse of the
+ * last constant-density solve over dt (DFSPH.cpp:674-690, 723-737).  Zero for
+ * static solids. */
 int sphgpu_get_solid_force_torque(sphgpu_handle* h, int solid, double f[3], double t[3]);
+/* Host rigid-body step inside sphgpu_step, at the point where the reference
+ * calls SphWithBullet::updateSolids (SphWithBullet.cpp:180-224): after the
+ * fluid update for WCSPH and IISPH (WcSph.cpp:85, IISph_M.cpp:64), between
+ * the constant-density solve and the next neighbour search for DFSPH
+ * (DFSPH.cpp:54).  The callback may call sphgpu_get_solid_force_torque and
+ * sphgpu_set_solid_pose; poses it sets take effect before the step goes on.
+ * A nonzero return makes sphgpu_step return SPHGPU_E_CALLBACK once the step
+ * has finished (with the poses set so far).  fn = NULL removes
+ * it; without a callback, poses set between steps take effect at the next
+ * update point (for DFSPH: after the next constant-density solve). */
+typedef int (*sphgpu_update_solids_fn)(sphgpu_handle* h, void* user);
+int sphgpu_set_update_solids(sphgpu_handle* h, sphgpu_update_solids_fn fn, void* user);
 /* Uploads, sorts the boundary, and computes the Akinci boundary volumes
  * (SphBase::prepareSolidPartWeight, SphBase.cpp:766-813).  Called by the first
  * step if not called before. */

@@ -18,6 +18,7 @@ LIB_PATH = HERE / "liboracle.so"
 WCSPH, DFSPH, IISPH = 0, 1, 2
 F_POS, F_VEL, F_RHO, F_PRESSURE, F_ACC, F_ALPHA, F_KAPPA, F_ID, F_AADV, F_KAPPA_SUM, F_AII, F_RHO_ADV = range(12)
 COMPONENTS = {F_POS: 3, F_VEL: 3, F_ACC: 3, F_AADV: 3}
+_UPDATE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)      # int (*)(void* user)
 
 
 class OrcParams(C.Structure):
@@ -93,6 +94,7 @@ def load():
         lib.orc_solid_force_torque.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double),
                                                C.POINTER(C.c_double)]
         lib.orc_step.argtypes = [C.c_void_p, C.POINTER(OrcStats)]
+        lib.orc_step_cb.argtypes = [C.c_void_p, C.POINTER(OrcStats), _UPDATE_FN, C.c_void_p]
         lib.orc_num_fluid.argtypes = [C.c_void_p, C.c_int]
         lib.orc_get_fluid.argtypes = [C.c_void_p, C.c_int, C.c_int, DP]
         lib.orc_get_boundary_volume.argtypes = [C.c_void_p, C.c_int, DP]
@@ -199,9 +201,30 @@ class Oracle:
     def compute_density(self):
         self.lib.orc_compute_density(self.h)
 
-    def step(self) -> dict:
+    def step(self, update_solids=None) -> dict:
+        """One step; update_solids() (no arguments) runs where the reference
+        calls SphWithBullet::updateSolids and may read solid_force_torque and
+        call set_solid_pose."""
         st = OrcStats()
-        self.lib.orc_step(self.h, C.byref(st))
+        if update_solids is None:
+            self.lib.orc_step(self.h, C.byref(st))
+            return st.as_dict()
+        err = []
+
+        def tramp(_user):
+            try:
+                update_solids()
+                return 0
+            except BaseException as e:
+                err.append(e)
+                return 1
+
+        cb = _UPDATE_FN(tramp)
+        rc = self.lib.orc_step_cb(self.h, C.byref(st), cb, None)
+        if err:
+            raise err[0]
+        if rc != 0:
+            raise RuntimeError(f"orc_step_cb returned {rc}")
         return st.as_dict()
 
     def num_fluid(self, k=0) -> int:

@@ -101,6 +101,8 @@ struct Solid {
     std::vector<BoundPart<T>> parts;
     std::vector<V3<T>> inner;   /* innerPositions: body-frame samples of a dynamic solid */
     V3<T> com;                  /* centre of mass of the last pose */
+    bool posePending = false;   /* pose set by the host, applied at the next update point */
+    float pT[16] = {}, pCom[3] = {}, pV[3] = {}, pW[3] = {};
 };
 
 struct PartIdx {             /* SphBase.h:596-617 */
@@ -531,7 +533,6 @@ public:
     void df_first() {
         T v0 = r * r * r;
         int zd = 0;
-        for (auto& s : solids) for (auto& b : s.parts) b.force = V3<T>();
         for (int k = 0; k < int(fluids.size()); ++k) {
             auto& ps = fluids[k].parts; int n = int(ps.size());
             T rho0 = fluids[k].rho0, fm0 = rho0 * v0, wm0 = W0 * fm0;
@@ -657,6 +658,47 @@ public:
         }
     }
 
+    /* Dynamic rigid step 4 (DFSPH.cpp:674-690): the boundary term of this
+     * iteration's velocity correction, acc_p = gradW rho0 V_b kappa_a/rho_a,
+     * added to the dynamic boundary particle, times the fluid particle's mass
+     * (step 5, DFSPH.cpp:723-737, there m_Fluids[0]'s for every phase; here
+     * each phase's own, which keeps Newton's third law for two phases).
+     * Serial: the reference's += from a parallel loop races. */
+    void df_rigidReaction() {
+        bool any = false;
+        for (auto& so : solids) any |= so.dynamic;
+        if (!any) return;
+        T v0 = r * r * r;
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            const auto& ps = fluids[k].parts;
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0;
+            for (size_t i = 0; i < ps.size(); ++i) {
+                const FluidPart<T>& a = ps[i];
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    if (nb.dis == 0 || nb.pidx.isFluid()) continue;
+                    Solid<T>& so = solids[nb.pidx.toSolidI()];
+                    if (!so.dynamic) continue;
+                    BoundPart<T>& b = so.parts[nb.pidx.i];
+                    V3<T> g = (a.x - b.x) * (-Wg(nb.dis) / nb.dis);
+                    b.force += g * (rho0 * b.volume * (a.kappa / a.rho) * fm0);
+                }
+            }
+        }
+    }
+
+    /* the host's pose (set_solid_pose) reaches the samples here: at init, and
+     * at the point of SphWithBullet::updateSolids in the step */
+    void applyPendingPoses() {
+        for (int k = 0; k < int(solids.size()); ++k)
+            if (solids[k].posePending) {
+                Solid<T>& so = solids[k];
+                updateParticles(k, so.pT, so.pCom, so.pV, so.pW);
+                so.posePending = false;
+            }
+    }
+
     /* sum_j m (v*_a - v*_b).gradW + sum_b rho0 V_b (v*_a - v_b).gradW for particle i of
      * fluid k: the predicted density change rate of DFSPH.cpp:589-617 / 795-825. */
     T densityChange(int k, int i) const {
@@ -766,6 +808,11 @@ public:
          * (rho* is clamped at rho0, DFSPH.cpp:620-623), so an over-estimated
          * initial guess is never taken back and adds energy. */
         for (auto& f : fluids) for (auto& p : f.parts) p.kappa_sum = 0;
+        /* Dynamic rigid step 1 (DFSPH.cpp:90-104), here at the start of the
+         * density solve rather than in DF_constantDensitySolverFirst: the
+         * same forces reach updateSolids, and they stay readable after the
+         * step */
+        for (auto& so : solids) for (auto& b : so.parts) b.force = V3<T>();
         double err = 1; int it = 0;
         int maxIt = P.max_iter_density > 0 ? P.max_iter_density : std::numeric_limits<int>::max();
         while (err > double(P.density_tol) || it < P.min_iter_density) {
@@ -773,6 +820,7 @@ public:
             int n = 0;
             double s = df_densityAdv(dt, n);
             df_velocityCorrection(0, dt);
+            df_rigidReaction();
             err = n > 0 ? s / n - 1.0 : 0.0;                              /* DFSPH.cpp:696 */
             ++it;
         }
@@ -1149,33 +1197,46 @@ public:
         st->dt = float(dt_cur);
     }
 
-    void step(orc_stats* st) {
+    /* update point (SphWithBullet::updateSolids): the host callback, then
+     * the poses it (or the host before the step) set */
+    int updateSolids(int (*cb)(void*), void* user) {
+        int rc = cb ? cb(user) : 0;
+        applyPendingPoses();
+        return rc;
+    }
+
+    int step(orc_stats* st, int (*cb)(void*) = nullptr, void* user = nullptr) {
         std::memset(st, 0, sizeof(*st));
-        int zd0 = zeroDis, tm0 = tooMany;
+        int zd0 = zeroDis, tm0 = tooMany, rc = 0;
         cflDt();
         if (P.solver == ORC_WCSPH) {                                       /* WcSph.cpp:23-96 */
+            applyPendingPoses();      /* poses set after the previous update point */
             neighbourSearch();
             updateSolidPartWeight();
             wc_computePressure();
             wc_computeForce();
             rigidReaction(false);
             st->removed = updateFluidsWC();
+            rc = updateSolids(cb, user);                                   /* WcSph.cpp:85 */
             fillStats(st);
         } else if (P.solver == ORC_DFSPH) {                                /* DFSPH.cpp:9-75 */
             if (frame == 0) { neighbourSearch(); updateSolidPartWeight(); df_first(); }
             df_second(st);
+            rc = updateSolids(cb, user);                                   /* DFSPH.cpp:54 */
             neighbourSearch();
             updateSolidPartWeight();
             df_first();
             df_divergence(st);
             fillStats(st);
         } else {                                                           /* IISph_M.cpp:10-75 */
+            applyPendingPoses();
             neighbourSearch();
             updateSolidPartWeight();
             ii_computePressure(st);
             ii_computeForcePressure();
             rigidReaction(true);
             st->removed = updateFluidsII();
+            rc = updateSolids(cb, user);                                   /* IISph_M.cpp:64 */
             fillStats(st);
         }
         st->zero_dis = zeroDis - zd0;
@@ -1183,6 +1244,7 @@ public:
         st->dt = float(dt_cur);
         dt_prev = dt_cur;
         ++frame;
+        return rc;
     }
 };
 
@@ -1280,7 +1342,12 @@ int orc_add_solid(orc_sim* s, int n, const float* xyz, int dynamic, float visc) 
 int orc_set_solid_pose(orc_sim* s, int k, const float* T16, const float* com, const float* v, const float* w) {
     return dispatch(s, [&](auto& S) {
         if (k < 0 || k >= int(S.solids.size()) || !S.solids[k].dynamic) return -1;
-        S.updateParticles(k, T16, com, v, w);
+        auto& so = S.solids[k];
+        std::memcpy(so.pT, T16, sizeof(so.pT));
+        std::memcpy(so.pCom, com, sizeof(so.pCom));
+        std::memcpy(so.pV, v, sizeof(so.pV));
+        std::memcpy(so.pW, w, sizeof(so.pW));
+        so.posePending = true;
         return 0;
     });
 }
@@ -1298,6 +1365,7 @@ int orc_init(orc_sim* s) {
         // SphWithIO::addFluidCuboid / SphSolid::adjustSpace grow the domain to
         // contain every particle (SphWithIO.cpp:161-162, SphBase.cpp:149-161);
         // here the domain is given, so particles outside it are rejected.
+        S.applyPendingPoses();
         for (auto& f : S.fluids)
             for (auto& q : f.parts)
                 if (!q.x.inside(S.spMin, S.spMax)) return -1;
@@ -1310,7 +1378,11 @@ int orc_init(orc_sim* s) {
 }
 
 int orc_step(orc_sim* s, orc_stats* st) {
-    return dispatch(s, [&](auto& S) { S.step(st); return 0; });
+    return dispatch(s, [&](auto& S) { return S.step(st); });
+}
+
+int orc_step_cb(orc_sim* s, orc_stats* st, int (*cb)(void*), void* user) {
+    return dispatch(s, [&](auto& S) { return S.step(st, cb, user); });
 }
 
 int orc_num_fluid(const orc_sim* s, int k) {

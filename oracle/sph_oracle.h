@@ -93,6 +93,10 @@ int      orc_set_solid_pose(orc_sim* s, int k, const float* T16, const float* co
 int      orc_solid_force_torque(orc_sim* s, int k, double* f, double* t);
 int      orc_init(orc_sim* s);                       /* prepareSolidPartWeight */
 int      orc_step(orc_sim* s, orc_stats* st);        /* one sphStep + runOneStep bookkeeping */
+/* the same with a host callback at the point of SphWithBullet::updateSolids
+ * (WcSph.cpp:85, IISph_M.cpp:64, DFSPH.cpp:54); poses set with
+ * orc_set_solid_pose take effect at that point (or at orc_init) */
+int      orc_step_cb(orc_sim* s, orc_stats* st, int (*cb)(void*), void* user);
 int      orc_num_fluid(const orc_sim* s, int k);
 int      orc_get_fluid(const orc_sim* s, int k, int field, double* out);
 int      orc_get_boundary_volume(const orc_sim* s, int k, double* out);

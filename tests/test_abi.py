@@ -94,10 +94,10 @@ def test_rigid_calls_validate_without_a_device():
         sim.set_solid_pose(body, bad, (0, 0, 0))
     with pytest.raises(sg.SphError):
         sim.solid_force_torque(body)                            # no step has run
-    sim.close()
-    sim = sg.Simulation(sg.default_params(solver=sg.DFSPH))
-    with pytest.raises(sg.SphError):                            # coupled for WCSPH and IISPH only
-        sim.add_rigid(np.ones((5, 3)))
+    calls = []
+    sim.set_update_solids(lambda s: calls.append(s))            # registered, not run before a step
+    sim.set_update_solids(None)
+    assert calls == []
     sim.close()
 
 

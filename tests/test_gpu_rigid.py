@@ -24,7 +24,7 @@ from oracle import oracle as orc
 pytestmark = pytest.mark.gpu
 
 FIELD_TOL = 1e-5
-FIXED = dict(min_iter_density=3, max_iter_density=3)
+FIXED = dict(min_iter_density=3, max_iter_density=3, min_iter_div=2, max_iter_div=1)
 
 
 def _relerr(a, b):
@@ -42,25 +42,43 @@ def _moving_pose(t, centre=(0.31, 0.1, 0.14)):
     return scenes.pose(w[1] * t, tr, v=v, w=w)
 
 
-@pytest.mark.parametrize("solver", [sg.WCSPH, sg.IISPH])
+@pytest.mark.parametrize("solver", [sg.WCSPH, sg.IISPH, sg.DFSPH])
 def test_rigid_single_step_fields_and_load(gpu, solver):
     sc = scenes.dam_with_box(solver, **FIXED)
     sc.rigids[0] = sc.rigids[0][:3] + (_moving_pose(0.0),)
     sim = scenes.make_simulation(sc)
     o = orc.from_scene(sc, 64)
     o32 = orc.from_scene(sc, 32)
-    for _ in range(2):                       # the second step sees the fluid moved by the first
+    fields = [(sg.F_POS, orc.F_POS), (sg.F_VEL, orc.F_VEL), (sg.F_RHO, orc.F_RHO)]
+    if solver == sg.DFSPH:
+        # DFSPH: fields at the updateSolids point of the first step (after the
+        # constant-density solve, whose impulse the body receives), before
+        # the divergence solve: its clamp (DFSPH.cpp:829-832) is
+        # discontinuous where rho + dt Drho/Dt crosses rho0, so one
+        # particle on the edge can flip between FP32 roundings
+        fields += [(sg.F_ALPHA, orc.F_ALPHA)]
+        snap = {}
+        sim.set_update_solids(lambda s_: snap.setdefault("gpu", {f: s_.get(f) for f, _ in fields}))
         st = sim.step()
-        ost = o.step()
-        o32.step()
+        ost = o.step(update_solids=lambda: snap.setdefault("orc", {of: o.get_by_id(of)[1] for _, of in fields}))
+        o32.step(update_solids=lambda: snap.setdefault("o32", {of: o32.get_by_id(of)[1] for _, of in fields}))
+        got = lambda f: snap["gpu"][f]
+        want = lambda of: snap["orc"][of]
+        want32 = lambda of: snap["o32"][of]
+    else:
+        for _ in range(2):                   # the second step sees the fluid moved by the first
+            st = sim.step()
+            ost = o.step()
+            o32.step()
+        fields += [(sg.F_PRESSURE, orc.F_PRESSURE), (sg.F_ACC, orc.F_ACC)]
+        got = lambda f: sim.get(f)
+        want = lambda of: o.get_by_id(of)[1]
+        want32 = lambda of: o32.get_by_id(of)[1]
     assert np.array_equal(sim.ids(), o.get_by_id(orc.F_ID)[0])
-    fields = [(sg.F_POS, orc.F_POS), (sg.F_VEL, orc.F_VEL), (sg.F_RHO, orc.F_RHO),
-              (sg.F_PRESSURE, orc.F_PRESSURE), (sg.F_ACC, orc.F_ACC)]
     errs, bars = {}, {}
     for f, of in fields:
-        want = o.get_by_id(of)[1]
-        errs[f] = _relerr(sim.get(f), want)
-        bars[f] = max(FIELD_TOL, 3.0 * _relerr(o32.get_by_id(of)[1], want))
+        errs[f] = _relerr(got(f), want(of))
+        bars[f] = max(FIELD_TOL, 3.0 * _relerr(want32(of), want(of)))
     assert all(errs[f] < bars[f] for f in errs), (errs, bars)
     assert st["iter_density"] == ost["iter_density"]
     radius = float(np.linalg.norm(sc.rigids[0][1], axis=1).max())
@@ -78,26 +96,42 @@ def test_rigid_single_step_fields_and_load(gpu, solver):
     sim.close()
 
 
-@pytest.mark.parametrize("solver,steps", [(sg.WCSPH, 150), (sg.IISPH, 30)])
-def test_rigid_multi_step_prescribed_motion(gpu, solver, steps):
+@pytest.mark.parametrize("solver,steps,callback", [(sg.WCSPH, 150, False), (sg.IISPH, 30, False),
+                                                    (sg.IISPH, 30, True), (sg.DFSPH, 30, True)])
+def test_rigid_multi_step_prescribed_motion(gpu, solver, steps, callback):
     """The box is re-posed every step on both sides (moving samples, boundary
-    re-sort, volumes); statistics and the load history agree.  use_graph is
-    on: steps with dynamic solids run eagerly, and the result is the same."""
+    re-sort, volumes), from the host between steps or from the update-solids
+    callback inside the step (SphWithBullet::updateSolids' place; for DFSPH
+    between the density solve and the next search); statistics and the load
+    history agree.  use_graph is on: steps with dynamic solids run eagerly,
+    and the result is the same."""
     sc = scenes.dam_with_box(solver, use_graph=1)
     sc.rigids[0] = sc.rigids[0][:3] + (_moving_pose(0.0),)
     sim = scenes.make_simulation(sc)
     o = orc.from_scene(sc, 64)
+    dt = float(np.float32(sc.params["dt"]))
     g, r, fg, fr = [], [], [], []
-    t = 0.0
+    clock = {"gpu": 0, "orc": 0}
+
+    def gpu_update(s):
+        clock["gpu"] += 1
+        fg.append(s.solid_force_torque(1)[0])
+        s.set_solid_pose(1, *_moving_pose(clock["gpu"] * dt))
+
+    def orc_update():
+        clock["orc"] += 1
+        fr.append(o.solid_force_torque(1)[0])
+        o.set_solid_pose(1, *_moving_pose(clock["orc"] * dt))
+
+    if callback:
+        sim.set_update_solids(gpu_update)
     for _ in range(steps):
         g.append(sim.step())
-        r.append(o.step())
-        t += float(np.float32(g[-1]["dt"]))
-        fg.append(sim.solid_force_torque(1)[0])
-        fr.append(o.solid_force_torque(1)[0])
-        pose = _moving_pose(t)
-        sim.set_solid_pose(1, *pose)
-        o.set_solid_pose(1, *pose)
+        r.append(o.step(update_solids=orc_update if callback else None))
+        if not callback:
+            gpu_update(sim)
+            orc_update()
+    assert clock["gpu"] == clock["orc"] == steps
     for key in ("rho_mean_err", "rho_max_err"):
         a = np.array([s[key] for s in g[-10:]])
         b = np.array([s[key] for s in r[-10:]])
@@ -110,6 +144,40 @@ def test_rigid_multi_step_prescribed_motion(gpu, solver, steps):
     assert _relerr(fg[:3], fr[:3]) < 1e-3
     assert np.abs(fg.mean(0) - fr.mean(0)).max() < 0.05 * np.abs(fr).max()
     assert all(s["nan_count"] == 0 for s in g)
+    sim.close()
+
+
+@pytest.mark.parametrize("solver", [sg.WCSPH, sg.DFSPH])
+def test_rigid_boundary_volumes_after_init(gpu, solver):
+    """Akinci volumes of a posed box and the tank after init: the box's
+    kerSumFromSelf (SphWithBullet.cpp:134, from the placed samples) plus the
+    other solids (SphBase.cpp:817-841).  Run twice: the body-frame upload at
+    init must not overtake the sort of the placed samples."""
+    for _ in range(2):
+        sc = scenes.dam_with_box(solver)
+        sc.rigids[0] = sc.rigids[0][:3] + (scenes.pose(0.4, (0.31, 0.1, 0.14)),)
+        sim = scenes.make_simulation(sc)
+        got = sim.boundary_volume(sc.num_boundary)
+        o = orc.from_scene(sc, 64)
+        n0 = len(sc.solids[0][1])
+        want = np.concatenate([o.boundary_volume(0, n0), o.boundary_volume(1, sc.num_boundary - n0)])
+        assert _relerr(got, want) < FIELD_TOL
+        sim.close()
+
+
+def test_update_solids_callback_errors_fail_the_step(gpu):
+    sc = scenes.dam_with_box(sg.IISPH)
+    sim = scenes.make_simulation(sc)
+
+    def boom(s):
+        raise KeyError("host rigid step failed")
+
+    sim.set_update_solids(boom)
+    with pytest.raises(sg.SphError) as e:
+        sim.step()
+    assert e.value.code == -6 and isinstance(e.value.__cause__, KeyError)
+    sim.set_update_solids(None)
+    assert sim.step()["nan_count"] == 0
     sim.close()
 
 

@@ -422,3 +422,62 @@ def test_rigid_surface_velocity_is_v_plus_w_cross_r():
     o.init()
     o.step()
     assert np.abs(o.get(orc.F_ACC) - acc[0]).max() / np.abs(acc[0]).max() > 1e-3
+
+
+def test_dfsph_rigid_reaction_balances_density_solve_impulse():
+    """DFSPH: the force on a dynamic plate is the boundary part of the
+    constant-density solve's velocity corrections, summed over its
+    iterations (DFSPH.cpp:674-690, 723-737).  With gravity and viscosity off
+    the fluid's momentum change over that solve (read at the updateSolids
+    point, DFSPH.cpp:54, before the divergence solve) is -F dt."""
+    s = 0.02
+    p = dict(solver=orc.DFSPH, spacing_r=s, smooth_radius_h=2 * s, dt=1e-3, nu=0.0,
+             gravity=(0.0, 0.0, 0.0), space_min=(-1.0, -1.0, -1.0), space_max=(1.0, 1.0, 1.0),
+             min_iter_density=3, max_iter_density=3, density_tol=0.0)
+    o = orc.Oracle(p, 64)
+    xyz = sg.lattice((8, 6, 8), (0.01, 0.02, 0.01), s * 0.96, 0.02, 7, 0)
+    vel = np.zeros_like(xyz)
+    vel[:, 1] = -0.3                                   # onto the plate
+    o.add_fluid(xyz, 1000.0, 0.0, vel=vel)
+    plate = sg.lattice((10, 1, 10), (0.0, 0.0, 0.0), s, 0.0, 0, 0)
+    k = o.add_solid(plate, 0.0, dynamic=True)
+    o.set_solid_pose(k, *_identity_pose())
+    o.init()
+    o.step()
+    m = 1000.0 * float(np.float32(s)) ** 3
+    v0 = o.get_by_id(orc.F_VEL)[1]
+    seen = {}
+
+    def at_update_point():
+        seen["v"] = o.get_by_id(orc.F_VEL)[1]
+        seen["f"] = o.solid_force_torque(k)[0]
+
+    st = o.step(update_solids=at_update_point)
+    dp = m * (seen["v"] - v0).sum(0)
+    f = seen["f"]
+    assert np.abs(f).max() > 1e-2                      # the plate carries load
+    assert np.abs(dp + f * st["dt"]).max() / (np.abs(f).max() * st["dt"]) < 1e-10
+    # the divergence solve that follows adds nothing to the reported force
+    np.testing.assert_array_equal(o.solid_force_torque(k)[0], f)
+
+
+def test_pose_set_between_steps_applies_at_the_update_point():
+    """A pose set between DFSPH steps reaches the samples at the next
+    updateSolids point (after the density solve), not before it: the same
+    as setting it from inside that step's callback."""
+    runs = []
+    for how in ("between", "inside", "never"):
+        sc = scenes.dam_with_box(sg.DFSPH, min_iter_density=3, max_iter_density=3)
+        o = orc.from_scene(sc, 64)
+        o.step()
+        moved = scenes.pose(0.2, (0.30, 0.1, 0.14))
+        if how == "inside":
+            o.step(update_solids=lambda: o.set_solid_pose(1, *moved))
+        else:
+            if how == "between":
+                o.set_solid_pose(1, *moved)
+            o.step()
+        o.step()
+        runs.append(o.get_by_id(orc.F_POS)[1])
+    np.testing.assert_array_equal(runs[0], runs[1])
+    assert np.abs(runs[0] - runs[2]).max() > 1e-6          # the move did act
