@@ -150,6 +150,16 @@ def build_scene(name: str):
 def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, graph: int = 0,
             stats_every: int = 1):
     sim = scenes.make_simulation(sc, warm_start=warm_start, use_graph=graph)
+    if sc.rigids:
+        # the host rigid-body step (Bullet's place in the reference) inside
+        # every step, at the updateSolids point; part of the timed work
+        world = scenes.rigid_world(sc)
+
+        def update_solids(s):
+            world.step(s.step_dt(), s.solid_force_torque)
+            world.apply(s.set_solid_pose)
+
+        sim.set_update_solids(update_solids)
     t0 = time.perf_counter()
     sim.init()
     log(f"[bench] init {time.perf_counter() - t0:.2f}s  fluid={sc.num_fluid} boundary={sc.num_boundary}")

@@ -62,6 +62,8 @@ struct sphgpu_handle {
     bool inCallback = false;
     SolidPose* dSolids = nullptr;
     double* dSolidFT = nullptr;        // [nsolids][6] force, torque of the last step
+    std::vector<double> hSolidFT;      // host copy of dSolidFT, read once per step
+    bool ftValid = false;
     long long nextId = 0;
 
     bool initialized = false;
@@ -531,6 +533,7 @@ void rigid_reaction(sphgpu_handle* h, int mode) {
     if (!h->D.anyDynamic) return;
     launch_rigid_reaction(h->D, mode, h->st);
     launch_solid_reduce(h->D, h->dSolidFT, h->st);
+    h->ftValid = false;
 }
 
 // the point of SphWithBullet::updateSolids in the reference's step: the
@@ -917,12 +920,16 @@ int sphgpu_get_solid_force_torque(sphgpu_handle* h, int solid, double f[3], doub
     if (!h || !f || !t) return SPHGPU_E_ARG;
     if (solid < 0 || solid >= (int)h->hsolids.size()) return fail(h, SPHGPU_E_ARG, "solid %d out of range", solid);
     if (!h->initialized || !h->dSolidFT) return fail(h, SPHGPU_E_STATE, "no step has run");
-    double buf[6];
-    HIPCHECK(h, hipMemcpyAsync(buf, h->dSolidFT + 6 * solid, sizeof(buf), hipMemcpyDeviceToHost, h->st));
-    HIPCHECK(h, hipStreamSynchronize(h->st));
+    if (!h->ftValid) {   // all bodies in one copy: a host step reads every body
+        h->hSolidFT.resize(6 * (size_t)h->D.nsolids);
+        HIPCHECK(h, hipMemcpyAsync(h->hSolidFT.data(), h->dSolidFT, sizeof(double) * h->hSolidFT.size(),
+                                   hipMemcpyDeviceToHost, h->st));
+        HIPCHECK(h, hipStreamSynchronize(h->st));
+        h->ftValid = true;
+    }
     for (int q = 0; q < 3; ++q) {
-        f[q] = buf[q];
-        t[q] = buf[3 + q];
+        f[q] = h->hSolidFT[6 * solid + q];
+        t[q] = h->hSolidFT[6 * solid + 3 + q];
     }
     return SPHGPU_OK;
 }
@@ -1229,6 +1236,15 @@ int sphgpu_step(sphgpu_handle* h, sphgpu_stats* st) {
 int sphgpu_scene_stats(const sphgpu_handle* h, int scene, sphgpu_stats* st) {
     if (!h || !st || scene < 0 || scene >= (int)h->sceneStats.size()) return SPHGPU_E_ARG;
     *st = h->sceneStats[scene];
+    return SPHGPU_OK;
+}
+
+int sphgpu_get_step_dt(sphgpu_handle* h, int scene, float* dt) {
+    if (!h || !dt) return SPHGPU_E_ARG;
+    if (!h->initialized) return fail(h, SPHGPU_E_STATE, "not initialised");
+    if (scene < 0 || scene >= h->D.nscenes) return fail(h, SPHGPU_E_ARG, "scene %d out of range", scene);
+    HIPCHECK(h, hipMemcpyAsync(dt, h->D.dt + scene, sizeof(float), hipMemcpyDeviceToHost, h->st));
+    HIPCHECK(h, hipStreamSynchronize(h->st));
     return SPHGPU_OK;
 }
 

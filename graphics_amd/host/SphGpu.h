@@ -13,6 +13,12 @@
 //   getBoundaryParticles (SphBase.h:515-516)    getBoundaryParticles(i)
 //   SphWithIO::addFluidCuboid (SphWithIO.cpp:149-194)     addFluidCuboid()
 //   SphWithIO::makeCubeSampleParts (SphWithIO.cpp:489-522) addBoxWalls()
+//   SphWithBullet::addSolid, rigid (SphWithBullet.cpp:101-136) addRigid()
+//   SphWithBullet::setSolidTransform (SphWithBullet.h:73-80)  setSolidTransform()
+//   SphSolid::rigidForceJoin (SphBase.cpp:131-143)            rigidForceJoin()
+//   SphWithBullet::updateSolids (SphWithBullet.cpp:180-224)   virtual updateSolids(),
+//       called inside sphStep where the solvers call it (WcSph.cpp:85,
+//       IISph_M.cpp:64, DFSPH.cpp:54)
 //   m_TH (SphBase.h:546-583)                    m_TH (sphgpu_params)
 //
 // Errors: the reference asserts; these classes throw SphGpuError carrying
@@ -22,7 +28,9 @@
 
 #include "../../include/sphgpu.h"
 
+#include <array>
 #include <cstdint>
+#include <exception>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -81,6 +89,7 @@ public:
         ++frameNumber_;
         systemTime_ += lastStats_.dt;
         mirrorValid_ = false;
+        boundValid_ = false;
     }
 
     int getFrameNumber() const { return frameNumber_; }
@@ -145,8 +154,49 @@ public:
         return addSolid(xyz, viscosityAlpha);
     }
 
+    // A dynamic rigid body from samples in body coordinates (innerPositions);
+    // placed by setSolidTransform (identity until then).  Returns its index.
+    int addRigid(const std::vector<float>& localXyz, float viscosityAlpha) {
+        ensureHandle();
+        int s = -1;
+        check(sphgpu_add_rigid(h_, 0, (long long)(localXyz.size() / 3), localXyz.data(), viscosityAlpha, &s));
+        solidXyz_.push_back(localXyz);
+        solidPose_.resize(solidXyz_.size(), identity());
+        solidPose_[size_t(s)] = identity();
+        dynamic_.resize(solidXyz_.size(), false);
+        dynamic_[size_t(s)] = true;
+        return s;
+    }
+
+    // The body's transform (OpenGL column-major 4x4 of its body-frame
+    // samples), centre of mass, linear and angular velocity, as the
+    // rigid-body step left them.  Takes effect at the next update point.
+    void setSolidTransform(int s, const float T[16], const vec_t& com, const vec_t& v, const vec_t& w) {
+        ensureHandle();
+        const float c[3] = {com.x, com.y, com.z}, lv[3] = {v.x, v.y, v.z}, av[3] = {w.x, w.y, w.z};
+        check(sphgpu_set_solid_pose(h_, s, T, c, lv, av));
+        for (int q = 0; q < 16; ++q) solidPose_.at(size_t(s))[size_t(q)] = T[q];
+        boundValid_ = false;
+    }
+
+    // Total force and torque (about the centre of mass) the fluid exerted
+    // on dynamic solid s in the last step.
+    void rigidForceJoin(int s, double f[3], double t[3]) { check(sphgpu_get_solid_force_torque(h_, s, f, t)); }
+
 protected:
-    virtual void sphStep() { check(sphgpu_step(h_, &lastStats_)); }
+    virtual void sphStep() {
+        const int rc = sphgpu_step(h_, &lastStats_);
+        if (cbError_) {                  // an exception thrown by updateSolids
+            std::exception_ptr e = cbError_;
+            cbError_ = nullptr;
+            std::rethrow_exception(e);
+        }
+        check(rc);
+    }
+
+    // The host rigid-body step (Bullet in the reference): read rigidForceJoin,
+    // advance the bodies, setSolidTransform.  Runs inside sphStep.
+    virtual void updateSolids() {}
 
     void check(int rc) {
         if (rc != SPHGPU_OK) throw SphGpuError(rc, h_ ? sphgpu_last_error(h_) : "sphgpu call failed");
@@ -157,6 +207,23 @@ private:
         if (h_) return;
         int rc = sphgpu_create(&m_TH, &h_);
         if (rc != SPHGPU_OK) throw SphGpuError(rc, "sphgpu_create: invalid parameters");
+        check(sphgpu_set_update_solids(h_, &SphBase::updateSolidsThunk, this));
+    }
+    // no exception may cross the library's C frames
+    static int updateSolidsThunk(sphgpu_handle*, void* self) {
+        SphBase* me = static_cast<SphBase*>(self);
+        try {
+            me->updateSolids();
+            return 0;
+        } catch (...) {
+            me->cbError_ = std::current_exception();
+            return 1;
+        }
+    }
+    static std::array<float, 16> identity() {
+        std::array<float, 16> T{};
+        T[0] = T[5] = T[10] = T[15] = 1.0f;
+        return T;
     }
     void ensureScene() {
         if (sceneBuilt_) return;
@@ -196,12 +263,21 @@ private:
         std::vector<float> vol(total);
         check(sphgpu_get_boundary_volume(h_, vol.data()));
         boundMirror_.assign(solidXyz_.size(), {});
-        // static solids: positions are the caller's samples; volumes in add order
+        // static solids: the caller's samples; dynamic ones placed by their
+        // last transform (SphBase.cpp:95-128); volumes in add order
         size_t k = 0;
         for (size_t s = 0; s < solidXyz_.size(); ++s)
             for (size_t i = 0; i < solidXyz_[s].size() / 3; ++i, ++k) {
+                const float x = solidXyz_[s][3 * i], y = solidXyz_[s][3 * i + 1], z = solidXyz_[s][3 * i + 2];
                 BoundPart b;
-                b.position = {solidXyz_[s][3 * i], solidXyz_[s][3 * i + 1], solidXyz_[s][3 * i + 2]};
+                b.position = {x, y, z};
+                if (s < dynamic_.size() && dynamic_[s]) {
+                    const auto& T = solidPose_[s];
+                    const float a = T[3] * x + T[7] * y + T[11] * z + T[15];
+                    b.position = {(T[0] * x + T[4] * y + T[8] * z + T[12]) / a,
+                                  (T[1] * x + T[5] * y + T[9] * z + T[13]) / a,
+                                  (T[2] * x + T[6] * y + T[10] * z + T[14]) / a};
+                }
                 b.volume = vol[k];
                 boundMirror_[s].push_back(b);
             }
@@ -217,6 +293,9 @@ private:
     std::vector<int> phases_;
     std::vector<int> phaseOfId_;
     std::vector<std::vector<float>> solidXyz_;
+    std::vector<std::array<float, 16>> solidPose_;
+    std::vector<bool> dynamic_;
+    std::exception_ptr cbError_;
     std::vector<std::vector<FluidPart>> mirror_;
     std::vector<std::vector<BoundPart>> boundMirror_;
 };

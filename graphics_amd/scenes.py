@@ -26,6 +26,7 @@ class Scene:
     solids: list = field(default_factory=list)         # (scene, xyz[n,3], visc_alpha)
     domains: list = field(default_factory=list)        # (min[3], max[3]) per scene
     rigids: list = field(default_factory=list)         # (scene, body-frame xyz[n,3], visc_alpha, pose)
+    rigid_mass: list = field(default_factory=list)     # (mass, inertia[3]) per rigid, for a host step
 
     @property
     def num_fluid(self) -> int:
@@ -77,6 +78,86 @@ def dam_with_box(solver=sg.IISPH, n=(12, 10, 12), box=(0.5, 0.4, 0.4), spacing=0
     sc.name = f"dam_with_box_{solver}"
     local = sg.box_walls((0.0, 0.0, 0.0), half, spacing, True)
     sc.rigids.append((0, local, 0.01, pose(0.0, centre)))
+    return sc
+
+
+def rigid_world(sc: Scene, gravity=None):
+    """A host rigid-body world (graphics_amd.rigid_host) holding the scene's
+    rigids at their initial poses; solid indices follow the static solids."""
+    from .rigid_host import RigidWorld
+    w = RigidWorld(sc.params.get("gravity", (0.0, -9.81, 0.0)) if gravity is None else gravity)
+    for k, ((_, _, _, pz), (mass, inertia)) in enumerate(zip(sc.rigids, sc.rigid_mass)):
+        T = np.asarray(pz[0], np.float64)
+        R = np.stack([T[0:3], T[4:7], T[8:11]], axis=1)
+        w.add(len(sc.solids) + k, mass, inertia, pz[1], R=R, v=pz[2], w=pz[3])
+    return w
+
+
+def _layers(xyz, spacing, level, amp, length, x0):
+    """Heavy-layer mask: below level + amp * spacing * sin(2 pi (x - x0) / length)."""
+    iface = level + amp * spacing * np.sin(2.0 * np.pi * (xyz[:, 0] - x0) / length)
+    return xyz[:, 1] < iface
+
+
+def _add_box_body(sc: Scene, s: int, half, centre, spacing, visc, density):
+    from .rigid_host import box_inertia
+    local = sg.box_walls((0.0, 0.0, 0.0), half, spacing, True)
+    mass = density * 8.0 * half[0] * half[1] * half[2]
+    sc.rigids.append((s, local, visc, pose(0.0, centre)))
+    sc.rigid_mass.append((mass, box_inertia(mass, half)))
+    return len(local)
+
+
+def layers_with_boxes(solver=sg.IISPH, n=(20, 12, 10), box=(0.44, 0.5, 0.24), spacing=0.02, jitter=0.01,
+                      seed=1006, rho=(1000.0, 3000.0), amp=0.5, halves=((0.05, 0.04, 0.05),),
+                      body_density=2000.0, gap=2.0, **kw) -> Scene:
+    """Two immiscible layers (rho[1] below, rho[0] on top, interface
+    perturbed by amp * spacing) with box bodies of density body_density
+    placed `gap` spacings above the top fluid layer, in a row along x."""
+    sc = small_dam(solver, n=n, box=box, spacing=spacing, jitter=jitter, seed=seed, **kw)
+    sc.name = f"layers_with_boxes_{solver}"
+    sc.phases[:] = [(rho[0], 0.01), (rho[1], 0.01)]
+    xyz = sc.fluids[0][2]
+    heavy = _layers(xyz, spacing, spacing + 0.5 * n[1] * spacing, amp, n[0] * spacing, spacing)
+    sc.fluids[:] = [(0, 1, xyz[heavy]), (0, 0, xyz[~heavy])]
+    top = spacing * n[1] + gap * spacing
+    slot = box[0] / len(halves)
+    for k, h in enumerate(halves):
+        centre = (slot * (k + 0.5), top + h[1], 0.5 * box[2])
+        _add_box_body(sc, 0, h, centre, spacing, 0.01, body_density)
+    return sc
+
+
+def iisph_c5(seed=1005, scale=1.0) -> Scene:
+    """Multiphase IISPH with rigid coupling: two immiscible layers at 1000 and
+    3000 kg/m^3 (599 x 96 x 139 = 7,993,056 particles at 0.01 m, the heavy
+    layer below, interface perturbed by 0.5 spacing), and 20 box bodies of
+    2,000-20,000 surface samples each (log-uniform from the seed), density
+    2000 kg/m^3, starting just above the free surface in a 10 x 2 grid, in a
+    closed 6.0 x 2.0 x 1.4 m tank.  `scale` < 1 shrinks the fluid height and
+    box sizes for tests."""
+    s = 0.01
+    rng = np.random.default_rng(seed)
+    p = dict(solver=sg.IISPH, spacing_r=s, smooth_radius_h=2 * s, dt=1e-3, cfl=0.4, dt_min=1e-5,
+             gravity=(0.0, -9.81, 0.0), sound_speed=20.0, nu=1e-3, omega=0.5,
+             density_tol=1e-3, min_iter_density=2, max_iter_density=100)
+    size = (6.0, 2.0, 1.4)
+    ny = max(4, int(round(96 * scale)))
+    nx, nz = int(round(size[0] / s)) - 1, int(round(size[2] / s)) - 1
+    sc = Scene(f"iisph_layers_boxes_{nx * ny * nz}", p)
+    sc.phases[:] = [(1000.0, 0.01), (3000.0, 0.01)]
+    _tank(sc, 0, size, s, True, 0.01)
+    sc.domains.append(((-s, -s, -s), tuple(t + s for t in size)))
+    xyz = sg.lattice((nx, ny, nz), (s, s, s), s, 0.01, seed, 0)
+    heavy = _layers(xyz, s, s + 0.5 * ny * s, 0.5, nx * s, s)
+    sc.fluids[:] = [(0, 1, xyz[heavy]), (0, 0, xyz[~heavy])]
+    top = s * ny + 2.0 * s
+    for k in range(20):
+        target = math.exp(rng.uniform(math.log(2000.0), math.log(20000.0))) * scale ** 2
+        edge = min(0.56, s * math.sqrt(target / 6.0))
+        h = 0.5 * edge
+        centre = (0.6 * (k % 10) + 0.3, top + h, 0.7 * (k // 10) + 0.35)
+        _add_box_body(sc, 0, (h, h, h), centre, s, 0.01, 2000.0)
     return sc
 
 
@@ -200,4 +281,5 @@ WORKLOADS = {
     "dfsph_c2": dfsph_c2,
     "dfsph_c3": dfsph_c3,
     "iisph_c4": iisph_batched_c4,
+    "iisph_c5": iisph_c5,
 }

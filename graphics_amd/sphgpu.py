@@ -109,6 +109,7 @@ SIGNATURES = {
     "sphgpu_init": (C.c_int, [C.c_void_p]),
     "sphgpu_step": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "sphgpu_scene_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Stats)]),
+    "sphgpu_get_step_dt": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float)]),
     "sphgpu_synchronize": (C.c_int, [C.c_void_p]),
     "sphgpu_num_fluid": (C.c_longlong, [C.c_void_p]),
     "sphgpu_get_fluid": (C.c_int, [C.c_void_p, C.c_int, _FP]),
@@ -389,6 +390,12 @@ class Simulation:
         st = Stats()
         self._check(self.lib.sphgpu_step(self.h, C.byref(st)), "step")
         return st.as_dict()
+
+    def step_dt(self, scene=0) -> float:
+        """dt of the current step (inside the update-solids callback) or the last one."""
+        dt = C.c_float()
+        self._check(self.lib.sphgpu_get_step_dt(self.h, scene, C.byref(dt)), "step_dt")
+        return dt.value
 
     def scene_stats(self, scene) -> dict:
         st = Stats()

@@ -195,6 +195,10 @@ int sphgpu_init(sphgpu_handle* h);
 int sphgpu_step(sphgpu_handle* h, sphgpu_stats* st);
 /* Per-scene statistics of the last step (batched scenes). */
 int sphgpu_scene_stats(const sphgpu_handle* h, int scene, sphgpu_stats* st);
+/* Time step of scene `scene` in the current step (inside the update-solids
+ * callback) or the last one: m_TH.dt after the CFL adaptation, which the
+ * reference's updateSolids hands to Bullet (SphWithBullet.cpp:214). */
+int sphgpu_get_step_dt(sphgpu_handle* h, int scene, float* dt);
 /* Waits for all work of the handle's stream. */
 int sphgpu_synchronize(sphgpu_handle* h);
 

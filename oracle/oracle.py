@@ -95,6 +95,7 @@ def load():
                                                C.POINTER(C.c_double)]
         lib.orc_step.argtypes = [C.c_void_p, C.POINTER(OrcStats)]
         lib.orc_step_cb.argtypes = [C.c_void_p, C.POINTER(OrcStats), _UPDATE_FN, C.c_void_p]
+        lib.orc_step_dt.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         lib.orc_num_fluid.argtypes = [C.c_void_p, C.c_int]
         lib.orc_get_fluid.argtypes = [C.c_void_p, C.c_int, C.c_int, DP]
         lib.orc_get_boundary_volume.argtypes = [C.c_void_p, C.c_int, DP]
@@ -226,6 +227,11 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"orc_step_cb returned {rc}")
         return st.as_dict()
+
+    def step_dt(self) -> float:
+        dt = C.c_double()
+        self.lib.orc_step_dt(self.h, C.byref(dt))
+        return dt.value
 
     def num_fluid(self, k=0) -> int:
         return int(self.lib.orc_num_fluid(self.h, k))

@@ -1381,6 +1381,10 @@ int orc_step(orc_sim* s, orc_stats* st) {
     return dispatch(s, [&](auto& S) { return S.step(st); });
 }
 
+int orc_step_dt(const orc_sim* s, double* dt) {
+    return dispatchc(s, [&](const auto& S) { *dt = double(S.dt_cur); return 0; });
+}
+
 int orc_step_cb(orc_sim* s, orc_stats* st, int (*cb)(void*), void* user) {
     return dispatch(s, [&](auto& S) { return S.step(st, cb, user); });
 }

@@ -97,6 +97,7 @@ int      orc_step(orc_sim* s, orc_stats* st);        /* one sphStep + runOneStep
  * (WcSph.cpp:85, IISph_M.cpp:64, DFSPH.cpp:54); poses set with
  * orc_set_solid_pose take effect at that point (or at orc_init) */
 int      orc_step_cb(orc_sim* s, orc_stats* st, int (*cb)(void*), void* user);
+int      orc_step_dt(const orc_sim* s, double* dt);     /* dt of the current (or last) step */
 int      orc_num_fluid(const orc_sim* s, int k);
 int      orc_get_fluid(const orc_sim* s, int k, int field, double* out);
 int      orc_get_boundary_volume(const orc_sim* s, int k, double* out);

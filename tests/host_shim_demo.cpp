@@ -4,6 +4,8 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <string>
+#include <vector>
 
 class DamBreak : public sphgpu_host::DFSPH {
 public:
@@ -25,8 +27,78 @@ public:
     }
 };
 
+// A box dropped onto a resting pool, coupled both ways through a stand-in for the
+// reference's Bullet step (SphWithBullet::updateSolids): the fluid force
+// from rigidForceJoin plus gravity moves the box (explicit Euler, no
+// rotation), and setSolidTransform hands the new pose back.
+class FloatingBox : public sphgpu_host::IISph_M {
+public:
+    int calls = 0;
+    double lastFy = 0.0;
+    float boxY = 0.22f, boxVy = 0.0f;
+
+    FloatingBox() {
+        m_TH.spacing_r = 0.02f;
+        m_TH.smooth_radius_h = 0.04f;
+        float mn[3] = {-0.1f, -0.1f, -0.1f}, mx[3] = {0.5f, 0.4f, 0.4f};
+        for (int a = 0; a < 3; ++a) { m_TH.space_min[a] = mn[a]; m_TH.space_max[a] = mx[a]; }
+        m_TH.gravity[1] = -9.81f;
+        m_TH.density_tol = 1e-3f;
+    }
+    void setupScene() override {
+        int water = addFluid(1000.0f, 0.01f);
+        int n[3] = {19, 8, 14};              // fills the tank floor to 0.16 m
+        float o[3] = {0.02f, 0.02f, 0.02f};
+        addFluidCuboid(water, n, o, 0.01f, 1001);
+        float c[3] = {0.2f, 0.15f, 0.15f}, h[3] = {0.2f, 0.15f, 0.15f};
+        addBoxWalls(c, h, true, 0.01f);
+        const float zero[3] = {0.0f, 0.0f, 0.0f}, half[3] = {0.04f, 0.04f, 0.04f};
+        long long nb = sphgpu_scene_box(zero, half, m_TH.spacing_r, 1, nullptr);
+        std::vector<float> local(size_t(3 * nb));
+        sphgpu_scene_box(zero, half, m_TH.spacing_r, 1, local.data());
+        box_ = addRigid(local, 0.01f);
+        place();
+    }
+
+protected:
+    void updateSolids() override {
+        ++calls;
+        double f[3], t[3];
+        rigidForceJoin(box_, f, t);
+        lastFy = f[1];
+        const float mass = 0.3f, dt = m_TH.dt;
+        boxVy += (float(f[1]) / mass - 9.81f) * dt;
+        boxY += boxVy * dt;
+        place();
+    }
+
+private:
+    int box_ = -1;
+    void place() {
+        const float T[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0.2f, boxY, 0.15f, 1};
+        setSolidTransform(box_, T, {0.2f, boxY, 0.15f}, {0.0f, boxVy, 0.0f}, {0.0f, 0.0f, 0.0f});
+    }
+};
+
+int runFloatingBox(int steps) {
+    FloatingBox sim;
+    try {
+        for (int k = 0; k < steps; ++k) sim.runOneStep();
+    } catch (const sphgpu_host::SphGpuError& e) {
+        std::printf("error %d %s\n", e.code(), e.what());
+        return 3;
+    }
+    const auto& b = sim.getBoundaryParticles(1);
+    double ymin = 1e9;
+    for (const auto& p : b) ymin = p.position.y < ymin ? p.position.y : ymin;
+    std::printf("frame %d calls %d force_y %.6f box_y %.6f box_vy %.6f sample_ymin %.6f\n", sim.getFrameNumber(),
+                sim.calls, sim.lastFy, sim.boxY, sim.boxVy, ymin);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     int steps = argc > 1 ? std::atoi(argv[1]) : 3;
+    if (argc > 2 && std::string(argv[2]) == "rigid") return runFloatingBox(steps);
     DamBreak sim;
     try {
         for (int k = 0; k < steps; ++k) sim.runOneStep();

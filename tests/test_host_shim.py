@@ -44,3 +44,19 @@ def test_shim_steps_on_device(tmp_path, gpu):
     # a 6^3 block: a third of its particles sit on the free surface
     assert 800.0 < float(vals["mean_rho"]) < 1010.0
     assert int(vals["iters"].split("/")[0]) >= 2
+
+
+@pytest.mark.gpu
+def test_shim_rigid_host_step_on_device(tmp_path, gpu):
+    """updateSolids() runs once per step inside sphStep, reads the fluid force
+    (rigidForceJoin) and moves the box (setSolidTransform); the boundary
+    mirror follows the last transform."""
+    exe = _build(tmp_path)
+    r = subprocess.run([exe, "40", "rigid"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    tok = r.stdout.split()
+    vals = dict(zip(tok[0::2], tok[1::2]))
+    assert vals["frame"] == "40" and vals["calls"] == "40"
+    fy, y, ymin = float(vals["force_y"]), float(vals["box_y"]), float(vals["sample_ymin"])
+    assert fy > 0.0                              # the water pushes the submerged box up
+    assert abs(ymin - (y - 0.04)) < 1e-5         # samples sit where the last pose put them
