@@ -18,7 +18,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libsphgpu.so"
 
-SOURCES = ["api.hip", "grid.hip", "solvers.hip", "sort.hip", "mlp.hip", "scene.cpp"]
+SOURCES = ["api.hip", "grid.hip", "solvers.hip", "sort.hip", "mlp.hip", "scene.cpp", "io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [

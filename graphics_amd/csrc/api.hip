@@ -1,6 +1,7 @@
 // C-ABI of libsphgpu.so: handle lifecycle, scene upload, the per-step
 // drivers of the three solvers (the reference's sphStep overrides) and
 // readback.  All device work runs on the handle's single HIP stream.
+#include "io.h"
 #include "mlp.h"
 #include "sph_common.h"
 #include "sphgpu.h"
@@ -60,6 +61,21 @@ struct sphgpu_handle {
     sphgpu_update_solids_fn updateFn = nullptr;   // host rigid-body step inside sphgpu_step
     void* updateUser = nullptr;
     bool inCallback = false;
+    // frame-output capture: device copies on st, transfer to pinned memory on cs
+    struct Snap {
+        float4 *x = nullptr, *v = nullptr, *a = nullptr;
+        float *rho = nullptr, *p = nullptr;
+        int *id = nullptr, *tv = nullptr;
+        void* host = nullptr;
+        float4 *hx = nullptr, *hv = nullptr, *ha = nullptr;
+        float *hrho = nullptr, *hp = nullptr;
+        int *hid = nullptr, *htv = nullptr;
+        int npad = 0;
+        long long frame = 0;
+        bool pending = false, valid = false;
+        hipStream_t cs = nullptr;
+        hipEvent_t copied = nullptr, done = nullptr;
+    } snap;
     SolidPose* dSolids = nullptr;
     double* dSolidFT = nullptr;        // [nsolids][6] force, torque of the last step
     std::vector<double> hSolidFT;      // host copy of dSolidFT, read once per step
@@ -807,6 +823,13 @@ void sphgpu_destroy(sphgpu_handle* h) {
         for (void* m : h->allocs) hipFree(m);
         if (h->mlpLive) mlp_free(h->mlp);
         hipHostFree(h->hActive);
+        if (h->snap.cs) {
+            hipStreamSynchronize(h->snap.cs);
+            hipEventDestroy(h->snap.copied);
+            hipEventDestroy(h->snap.done);
+            hipStreamDestroy(h->snap.cs);
+            hipHostFree(h->snap.host);
+        }
         hipStreamDestroy(h->st);
     }
     delete h;
@@ -1325,6 +1348,158 @@ int sphgpu_get_fluid_ids(sphgpu_handle* h, long long* out) {
     int rc = live_slots(h, ls);
     if (rc) return rc;
     for (size_t k = 0; k < ls.size(); ++k) out[k] = ls[k].first;
+    return SPHGPU_OK;
+}
+
+// ---- frame output ---------------------------------------------------------
+int sphgpu_snapshot_wait(sphgpu_handle* h) {
+    if (!h) return SPHGPU_E_ARG;
+    if (!h->snap.pending) return SPHGPU_OK;
+    HIPCHECK(h, hipEventSynchronize(h->snap.done));
+    h->snap.pending = false;
+    h->snap.valid = true;
+    return SPHGPU_OK;
+}
+
+int sphgpu_snapshot_begin(sphgpu_handle* h) {
+    if (!h) return SPHGPU_E_ARG;
+    if (h->inCallback) return fail(h, SPHGPU_E_STATE, "snapshot inside the update-solids callback");
+    int rc = sphgpu_init(h);
+    if (rc) return rc;
+    if ((rc = sphgpu_snapshot_wait(h))) return rc;
+    Dev& D = h->D;
+    auto& S = h->snap;
+    const size_t cap = (size_t)D.ncap, tiles = (size_t)D.ncap / TILE;
+    if (!S.cs) {
+        if ((rc = dalloc(h, &S.x, cap)) || (rc = dalloc(h, &S.v, cap)) || (rc = dalloc(h, &S.a, cap)) ||
+            (rc = dalloc(h, &S.rho, cap)) || (rc = dalloc(h, &S.p, cap)) || (rc = dalloc(h, &S.id, cap)) ||
+            (rc = dalloc(h, &S.tv, tiles)))
+            return rc;
+        const size_t bytes = cap * (3 * sizeof(float4) + 2 * sizeof(float) + sizeof(int)) + tiles * sizeof(int);
+        HIPCHECK(h, hipHostMalloc(&S.host, bytes));
+        char* c = static_cast<char*>(S.host);
+        S.hx = reinterpret_cast<float4*>(c); c += cap * sizeof(float4);
+        S.hv = reinterpret_cast<float4*>(c); c += cap * sizeof(float4);
+        S.ha = reinterpret_cast<float4*>(c); c += cap * sizeof(float4);
+        S.hrho = reinterpret_cast<float*>(c); c += cap * sizeof(float);
+        S.hp = reinterpret_cast<float*>(c); c += cap * sizeof(float);
+        S.hid = reinterpret_cast<int*>(c); c += cap * sizeof(int);
+        S.htv = reinterpret_cast<int*>(c);
+        HIPCHECK(h, hipStreamCreateWithFlags(&S.cs, hipStreamNonBlocking));
+        HIPCHECK(h, hipEventCreateWithFlags(&S.copied, hipEventDisableTiming));
+        HIPCHECK(h, hipEventCreateWithFlags(&S.done, hipEventDisableTiming));
+    }
+    const bool ii = h->P.solver == SPHGPU_IISPH;
+    const size_t n = (size_t)D.npad, nt = (size_t)D.npad / TILE;
+    // device-side copies in stream order (microseconds), then the PCIe
+    // transfer on the second stream while later steps run
+    HIPCHECK(h, hipMemcpyAsync(S.x, D.x, sizeof(float4) * n, hipMemcpyDeviceToDevice, h->st));
+    HIPCHECK(h, hipMemcpyAsync(S.v, D.v, sizeof(float4) * n, hipMemcpyDeviceToDevice, h->st));
+    HIPCHECK(h, hipMemcpyAsync(S.a, ii ? D.f4b : D.f4a, sizeof(float4) * n, hipMemcpyDeviceToDevice, h->st));
+    HIPCHECK(h, hipMemcpyAsync(S.rho, D.rho, sizeof(float) * n, hipMemcpyDeviceToDevice, h->st));
+    HIPCHECK(h, hipMemcpyAsync(S.p, ii ? D.kr : D.f1, sizeof(float) * n, hipMemcpyDeviceToDevice, h->st));
+    HIPCHECK(h, hipMemcpyAsync(S.id, D.id, sizeof(int) * n, hipMemcpyDeviceToDevice, h->st));
+    HIPCHECK(h, hipMemcpyAsync(S.tv, D.tileValid, sizeof(int) * nt, hipMemcpyDeviceToDevice, h->st));
+    HIPCHECK(h, hipEventRecord(S.copied, h->st));
+    HIPCHECK(h, hipStreamWaitEvent(S.cs, S.copied, 0));
+    HIPCHECK(h, hipMemcpyAsync(S.hx, S.x, sizeof(float4) * n, hipMemcpyDeviceToHost, S.cs));
+    HIPCHECK(h, hipMemcpyAsync(S.hv, S.v, sizeof(float4) * n, hipMemcpyDeviceToHost, S.cs));
+    HIPCHECK(h, hipMemcpyAsync(S.ha, S.a, sizeof(float4) * n, hipMemcpyDeviceToHost, S.cs));
+    HIPCHECK(h, hipMemcpyAsync(S.hrho, S.rho, sizeof(float) * n, hipMemcpyDeviceToHost, S.cs));
+    HIPCHECK(h, hipMemcpyAsync(S.hp, S.p, sizeof(float) * n, hipMemcpyDeviceToHost, S.cs));
+    HIPCHECK(h, hipMemcpyAsync(S.hid, S.id, sizeof(int) * n, hipMemcpyDeviceToHost, S.cs));
+    HIPCHECK(h, hipMemcpyAsync(S.htv, S.tv, sizeof(int) * nt, hipMemcpyDeviceToHost, S.cs));
+    HIPCHECK(h, hipEventRecord(S.done, S.cs));
+    S.npad = D.npad;
+    S.frame = h->frame;
+    S.pending = true;
+    S.valid = false;
+    return SPHGPU_OK;
+}
+
+// live slots of one phase in the capture, ascending original id
+static int snapshot_phase(sphgpu_handle* h, int phase, std::vector<int>& slots) {
+    int rc;
+    if (!h->snap.valid && !h->snap.pending && (rc = sphgpu_snapshot_begin(h))) return rc;
+    if ((rc = sphgpu_snapshot_wait(h))) return rc;
+    const auto& S = h->snap;
+    std::vector<std::pair<int, int>> ls;
+    for (int i = 0; i < S.npad; ++i) {
+        const int t = i / TILE;
+        if ((i - t * TILE) < S.htv[t] && S.hid[i] >= 0 && int(S.hx[i].w) == phase) ls.push_back({S.hid[i], i});
+    }
+    std::sort(ls.begin(), ls.end());
+    slots.resize(ls.size());
+    for (size_t k = 0; k < ls.size(); ++k) slots[k] = ls[k].second;
+    return 0;
+}
+
+int sphgpu_write_pos(sphgpu_handle* h, const char* path, int phase, unsigned components, const char* log_name) {
+    if (!h || !path) return SPHGPU_E_ARG;
+    const unsigned known = SPHGPU_POS_POSITION | SPHGPU_POS_VELOCITY | SPHGPU_POS_DENSITY | SPHGPU_POS_PRESURE |
+                           SPHGPU_POS_ACCELERATION;
+    if (components == 0 || (components & ~known)) return fail(h, SPHGPU_E_ARG, "bad component mask %#x", components);
+    if (phase < 0 || phase >= (int)h->phases.size()) return fail(h, SPHGPU_E_ARG, "phase %d out of range", phase);
+    std::vector<int> slots;
+    if (int rc = snapshot_phase(h, phase, slots)) return rc;
+    const auto& S = h->snap;
+    const size_t n = slots.size();
+    std::vector<float> x(3 * n), v(3 * n), a(3 * n), rho(n), p(n);
+    for (size_t k = 0; k < n; ++k) {
+        const int i = slots[k];
+        x[3 * k] = S.hx[i].x; x[3 * k + 1] = S.hx[i].y; x[3 * k + 2] = S.hx[i].z;
+        v[3 * k] = S.hv[i].x; v[3 * k + 1] = S.hv[i].y; v[3 * k + 2] = S.hv[i].z;
+        a[3 * k] = S.ha[i].x; a[3 * k + 1] = S.ha[i].y; a[3 * k + 2] = S.ha[i].z;
+        rho[k] = S.hrho[i];
+        p[k] = S.hp[i];
+    }
+    if (sphio::write_pos(path, log_name, S.frame, "fluid", phase, (long long)n, components, x.data(), v.data(),
+                         rho.data(), p.data(), a.data()))
+        return fail(h, SPHGPU_E_ARG, "cannot write %s", path);
+    return SPHGPU_OK;
+}
+
+int sphgpu_write_json(sphgpu_handle* h, const char* path, int phase) {
+    if (!h || !path) return SPHGPU_E_ARG;
+    if (phase < 0 || phase >= (int)h->phases.size()) return fail(h, SPHGPU_E_ARG, "phase %d out of range", phase);
+    std::vector<int> slots;
+    if (int rc = snapshot_phase(h, phase, slots)) return rc;
+    const auto& S = h->snap;
+    const size_t n = slots.size();
+    std::vector<float> x(3 * n), v(3 * n);
+    for (size_t k = 0; k < n; ++k) {
+        const int i = slots[k];
+        x[3 * k] = S.hx[i].x; x[3 * k + 1] = S.hx[i].y; x[3 * k + 2] = S.hx[i].z;
+        v[3 * k] = S.hv[i].x; v[3 * k + 1] = S.hv[i].y; v[3 * k + 2] = S.hv[i].z;
+    }
+    // the particle's d: half the support radius (SphBase.cpp:883)
+    if (sphio::write_json(path, (long long)n, x.data(), v.data(), 0.5f * h->P.smooth_radius_h))
+        return fail(h, SPHGPU_E_ARG, "cannot write %s", path);
+    return SPHGPU_OK;
+}
+
+int sphgpu_write_solid_pos(sphgpu_handle* h, const char* path, int solid, const char* log_name) {
+    if (!h || !path) return SPHGPU_E_ARG;
+    if (solid < 0 || solid >= (int)h->hsolids.size()) return fail(h, SPHGPU_E_ARG, "solid %d out of range", solid);
+    int rc = sphgpu_init(h);
+    if (rc) return rc;
+    Dev& D = h->D;
+    std::vector<float4> xb(D.nb);
+    std::vector<int> bid(D.nb);
+    HIPCHECK(h, hipStreamSynchronize(h->st));
+    HIPCHECK(h, hipMemcpy(xb.data(), D.xb, sizeof(float4) * D.nb, hipMemcpyDeviceToHost));
+    HIPCHECK(h, hipMemcpy(bid.data(), D.bid, sizeof(int) * D.nb, hipMemcpyDeviceToHost));
+    const SolidPose& sp = h->hsolids[solid];
+    const int n = sp.endId - sp.firstId;
+    std::vector<float> x(3 * (size_t)n);
+    for (int r = 0; r < D.nb; ++r) {
+        const int k = bid[r] - sp.firstId;
+        if (k < 0 || k >= n) continue;
+        x[3 * k] = xb[r].x; x[3 * k + 1] = xb[r].y; x[3 * k + 2] = xb[r].z;
+    }
+    if (sphio::write_pos(path, log_name, h->frame, "solid", solid, n, SPHGPU_POS_POSITION, x.data(), nullptr,
+                         nullptr, nullptr, nullptr))
+        return fail(h, SPHGPU_E_ARG, "cannot write %s", path);
     return SPHGPU_OK;
 }
 

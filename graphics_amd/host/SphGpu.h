@@ -20,6 +20,9 @@
 //       called inside sphStep where the solvers call it (WcSph.cpp:85,
 //       IISph_M.cpp:64, DFSPH.cpp:54)
 //   m_TH (SphBase.h:546-583)                    m_TH (sphgpu_params)
+//   fUtility.h write_fluid_particles (254-303), write_fluid_particles_json
+//     (390-427), write_solid_particles (459-484), read_fluid_particles
+//     (486-510)                                 free functions of the same names
 //
 // Errors: the reference asserts; these classes throw SphGpuError carrying
 // the library's status code and message.  Like SphBase, an object is not
@@ -183,6 +186,27 @@ public:
     // on dynamic solid s in the last step.
     void rigidForceJoin(int s, double f[3], double t[3]) { check(sphgpu_get_solid_force_torque(h_, s, f, t)); }
 
+    // Frame output at video cadence: capture now (the transfer overlaps the
+    // next steps); the write_* functions below write the last capture.
+    void snapshotBegin() {
+        ensureScene();
+        check(sphgpu_snapshot_begin(h_));
+    }
+    void writePos(const char* file, int idx, unsigned components, const char* logName) {
+        ensureScene();
+        check(sphgpu_write_pos(h_, file, idx, components, logName));
+    }
+    void writeJson(const char* file, int idx) {
+        ensureScene();
+        check(sphgpu_write_json(h_, file, idx));
+    }
+    void writeSolidPos(const char* file, int idx, const char* logName) {
+        ensureScene();
+        check(sphgpu_write_solid_pos(h_, file, idx, logName));
+    }
+    // SphBase::getLogFileName: the name written on line 1 of .pos files
+    std::string logFileName = "sph.log";
+
 protected:
     virtual void sphStep() {
         const int rc = sphgpu_step(h_, &lastStats_);
@@ -299,6 +323,33 @@ private:
     std::vector<std::vector<FluidPart>> mirror_;
     std::vector<std::vector<BoundPart>> boundMirror_;
 };
+
+// fUtility.h's writers and reader, same names and argument order
+inline void write_fluid_particles(const char* filename, SphBase& sph, int idx = 0) {
+    sph.writePos(filename, idx, SPHGPU_POS_POSITION, sph.logFileName.c_str());
+}
+inline void write_fluid_particles_json(const char* filename, SphBase& sph, int idx = 0) {
+    sph.writeJson(filename, idx);
+}
+inline void write_solid_particles(const char* filename, SphBase& sph, int idx = 0) {
+    sph.writeSolidPos(filename, idx, sph.logFileName.c_str());
+}
+inline void read_fluid_particles(const char* filename, std::vector<float>& data, std::string* maskstr = nullptr) {
+    long long n = 0, frame = 0;
+    char mask[512];
+    int rc = sphgpu_read_pos(filename, nullptr, 0, &n, &frame, mask, int(sizeof(mask)));
+    if (rc != SPHGPU_OK) throw SphGpuError(rc, std::string("read_fluid_particles: ") + filename);
+    const std::string m(mask);
+    long long per = 0;
+    const char* names[] = {"position", "velocity", "color", "density", "presure", "acceleration", "dt"};
+    const long long sizes[] = {3, 3, 4, 1, 1, 3, 1};
+    for (int k = 0; k < 7; ++k)
+        if (m.find(names[k]) != std::string::npos) per += sizes[k];
+    data.resize(size_t(n * per));
+    rc = sphgpu_read_pos(filename, data.data(), (long long)data.size(), &n, &frame, nullptr, 0);
+    if (rc != SPHGPU_OK) throw SphGpuError(rc, std::string("read_fluid_particles: ") + filename);
+    if (maskstr) *maskstr = m;
+}
 
 // WcSph (WcSph.h:23-96): weakly compressible SPH, Tait EOS
 class WcSph : public SphBase {

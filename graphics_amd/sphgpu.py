@@ -23,6 +23,8 @@ FIELD_COMPONENTS = {F_POS: 3, F_VEL: 3, F_RHO: 1, F_PRESSURE: 1, F_ACC: 3, F_ALP
 ERRORS = {0: "OK", -1: "E_ARG", -2: "E_STATE", -3: "E_DEVICE", -4: "E_CAPACITY", -5: "E_DIVERGED",
           -6: "E_CALLBACK"}
 
+POS_POSITION, POS_VELOCITY, POS_DENSITY, POS_PRESURE, POS_ACCELERATION = 1, 2, 8, 16, 32
+
 # int (*)(sphgpu_handle*, void* user)
 UPDATE_SOLIDS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
 
@@ -110,6 +112,13 @@ SIGNATURES = {
     "sphgpu_step": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "sphgpu_scene_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Stats)]),
     "sphgpu_get_step_dt": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float)]),
+    "sphgpu_snapshot_begin": (C.c_int, [C.c_void_p]),
+    "sphgpu_snapshot_wait": (C.c_int, [C.c_void_p]),
+    "sphgpu_write_pos": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_uint, C.c_char_p]),
+    "sphgpu_write_solid_pos": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_char_p]),
+    "sphgpu_write_json": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "sphgpu_read_pos": (C.c_int, [C.c_char_p, _FP, C.c_longlong, C.POINTER(C.c_longlong),
+                                  C.POINTER(C.c_longlong), C.c_char_p, C.c_int]),
     "sphgpu_synchronize": (C.c_int, [C.c_void_p]),
     "sphgpu_num_fluid": (C.c_longlong, [C.c_void_p]),
     "sphgpu_get_fluid": (C.c_int, [C.c_void_p, C.c_int, _FP]),
@@ -190,6 +199,33 @@ def copy_probe(nbytes: int) -> float:
     if rc:
         raise SphError(rc, "copy probe failed")
     return ms.value
+
+
+def pos_record_floats(mask: str) -> int:
+    """Floats per record named by a .pos component line (fUtility.h:53-63)."""
+    size = 0
+    for name, nbytes in (("position", 12), ("velocity", 12), ("color", 16), ("density", 4),
+                         ("presure", 4), ("acceleration", 12), ("dt", 4)):
+        if name in mask:
+            size += nbytes
+    return size // 4
+
+
+def read_pos(path):
+    """read_fluid_particles (fUtility.h:486-510) through the library: returns
+    (frame, component line, records[n, floats per record])."""
+    lib = load()
+    n, fr = C.c_longlong(), C.c_longlong()
+    mask = C.create_string_buffer(512)
+    rc = lib.sphgpu_read_pos(str(path).encode(), None, 0, C.byref(n), C.byref(fr), mask, 512)
+    if rc != 0:
+        raise SphError(rc, f"read_pos {path}")
+    per = pos_record_floats(mask.value.decode())
+    data = np.zeros(n.value * per, np.float32)
+    rc = lib.sphgpu_read_pos(str(path).encode(), _fp(data), data.size, C.byref(n), C.byref(fr), mask, 512)
+    if rc != 0:
+        raise SphError(rc, f"read_pos {path}")
+    return fr.value, mask.value.decode(), data.reshape(n.value, per)
 
 
 def lattice(n, origin, spacing, jitter=0.0, seed=0, stream=0) -> np.ndarray:
@@ -390,6 +426,25 @@ class Simulation:
         st = Stats()
         self._check(self.lib.sphgpu_step(self.h, C.byref(st)), "step")
         return st.as_dict()
+
+    # -- frame output (the reference's .pos / .json formats)
+    def snapshot_begin(self):
+        self._check(self.lib.sphgpu_snapshot_begin(self.h), "snapshot_begin")
+
+    def snapshot_wait(self):
+        self._check(self.lib.sphgpu_snapshot_wait(self.h), "snapshot_wait")
+
+    def write_pos(self, path, phase=0, components=None, log_name=""):
+        comps = POS_POSITION if components is None else components
+        self._check(self.lib.sphgpu_write_pos(self.h, str(path).encode(), phase, comps, log_name.encode()),
+                    "write_pos")
+
+    def write_solid_pos(self, path, solid, log_name=""):
+        self._check(self.lib.sphgpu_write_solid_pos(self.h, str(path).encode(), solid, log_name.encode()),
+                    "write_solid_pos")
+
+    def write_json(self, path, phase=0):
+        self._check(self.lib.sphgpu_write_json(self.h, str(path).encode(), phase), "write_json")
 
     def step_dt(self, scene=0) -> float:
         """dt of the current step (inside the update-solids callback) or the last one."""

@@ -218,6 +218,47 @@ int sphgpu_get_boundary_volume(sphgpu_handle* h, float* out);
 int sphgpu_get_neighbours(sphgpu_handle* h, int cap, int* counts, int* kinds,
                           long long* ids, float* dis);
 
+/* ---- frame output (fUtility.h wire formats) ---------------------------- */
+/* Components of a .pos record, in the reference's order (fUtility.h:14-21;
+ * color and dt are not kept on the device). */
+enum {
+    SPHGPU_POS_POSITION = 1,
+    SPHGPU_POS_VELOCITY = 2,
+    SPHGPU_POS_DENSITY = 8,
+    SPHGPU_POS_PRESURE = 16,       /* reference spelling */
+    SPHGPU_POS_ACCELERATION = 32
+};
+/* Starts an asynchronous capture of the fluid state (positions, velocities,
+ * densities, pressures, accelerations, ids) at the current point of the
+ * stream: a device-side copy on the handle's stream, then the transfer to
+ * pinned host memory on a second stream, overlapping the steps issued after
+ * it.  The frame number is recorded with it.  One capture at a time: a new
+ * one first waits for the previous. */
+int sphgpu_snapshot_begin(sphgpu_handle* h);
+/* Waits for the capture started last. */
+int sphgpu_snapshot_wait(sphgpu_handle* h);
+/* write_fluid_particles (fUtility.h:254-303): four text lines (log-file
+ * name, component mask, frame number and fluid index, count), then per
+ * particle the selected components as raw float32 (vec_t = 3 floats), fluid
+ * phase `phase` in ascending original id, from the last capture (one is
+ * taken synchronously when none exists).  components: SPHGPU_POS_* bits. */
+int sphgpu_write_pos(sphgpu_handle* h, const char* path, int phase, unsigned components,
+                     const char* log_name);
+/* write_solid_particles (fUtility.h:459-484): positions of solid `solid`'s
+ * boundary particles in add order (dynamic solids at their current pose). */
+int sphgpu_write_solid_pos(sphgpu_handle* h, const char* path, int solid, const char* log_name);
+/* write_fluid_particles_json (fUtility.h:390-427): {"num", "position":
+ * [{x, y, z}], "spacing_r", "temp_beta": [{value}], "velocity": [{x, y,
+ * z}]} with y and z swapped as the reference writes them, members sorted as
+ * jsoncpp's styled writer emits them; spacing_r is the particle's d
+ * (smoothRadius_h / 2), temp_beta 0 (no heat transfer here). */
+int sphgpu_write_json(sphgpu_handle* h, const char* path, int phase);
+/* read_fluid_particles (fUtility.h:486-510), host only: reads a .pos file;
+ * *n = particle count, data (optional, cap floats) the raw records, mask
+ * (optional, mask_len bytes) the component line. */
+int sphgpu_read_pos(const char* path, float* data, long long cap, long long* n, long long* frame,
+                    char* mask, int mask_len);
+
 /* ---- measurement ------------------------------------------------------- */
 /* Selects one kernel (by name, NULL to stop) whose launches are bracketed by
  * HIP events on the handle's stream; sphgpu_kernel_timing returns the launch

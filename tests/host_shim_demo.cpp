@@ -109,6 +109,14 @@ int main(int argc, char** argv) {
     }
     const auto& f = sim.getFluidParticles(0);
     const auto& b = sim.getBoundaryParticles(0);
+    if (argc > 2) {   // frame output as Demo-Main writes it, read back
+        sphgpu_host::write_fluid_particles(argv[2], sim, 0);
+        std::vector<float> data;
+        std::string mask;
+        sphgpu_host::read_fluid_particles(argv[2], data, &mask);
+        std::printf("pos_floats %zu pos_mask_ok %d first_x_equal %d\n", data.size(),
+                    mask == "components writed: < position >", data.empty() ? 0 : int(data[0] == f[0].position.x));
+    }
     double rho = 0;
     for (const auto& p : f) rho += p.density;
     std::printf("frame %d time %.6f fluid %zu boundary %zu mean_rho %.4f iters %d/%d\n", sim.getFrameNumber(),

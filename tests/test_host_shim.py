@@ -35,11 +35,12 @@ def test_shim_compiles_and_reports_missing_device(tmp_path):
 @pytest.mark.gpu
 def test_shim_steps_on_device(tmp_path, gpu):
     exe = _build(tmp_path)
-    r = subprocess.run([exe, "3"], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "3", str(tmp_path / "frame.pos")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     tok = r.stdout.split()
     vals = dict(zip(tok[0::2], tok[1::2]))
     assert vals["frame"] == "3"
+    assert vals["pos_floats"] == str(3 * 216) and vals["pos_mask_ok"] == "1" and vals["first_x_equal"] == "1"
     assert vals["fluid"] == "216"
     # a 6^3 block: a third of its particles sit on the free surface
     assert 800.0 < float(vals["mean_rho"]) < 1010.0
