@@ -35,8 +35,10 @@ ALG_BYTES = {
     "df_pass_velocity": 44, "df_pass_density": 36, "df_pass_divergence": 36,
     "df_density_alpha": 20, "df_aadv": 40, "wc_pressure": 20, "wc_force": 56,
     "ii_sumdij": 32, "ii_pressure": 56, "ii_aadv_dii": 64, "ii_aii": 48, "ii_force": 56,
+    "pci_aext": 56, "pci_density": 44, "pci_pressure": 64, "pci_integrate": 84,
 }
-DOMINANT = {sg.DFSPH: "df_pass_velocity", sg.IISPH: "ii_pressure", sg.WCSPH: "wc_force"}
+DOMINANT = {sg.DFSPH: "df_pass_velocity", sg.IISPH: "ii_pressure", sg.WCSPH: "wc_force",
+            sg.PCISPH: "pci_pressure"}
 
 
 def log(*a):
@@ -332,7 +334,7 @@ def main() -> int:
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": sc.name, "solver": ["WCSPH", "DFSPH", "IISPH"][solver],
+        "config": {"workload": sc.name, "solver": ["WCSPH", "DFSPH", "IISPH", "PCISPH"][solver],
                    "fluid": nf, "boundary": sc.num_boundary, "scenes": sc.num_scenes,
                    "density_tol": sc.params.get("density_tol"),
                    "divergence_tol": sc.params.get("divergence_tol"),

@@ -294,6 +294,17 @@ void launch_iteration(sphgpu_handle* h, int which, int it) {
                 }
                 launch_scene_reduce(D, 1, P.divergence_tol, P.min_iter_div, P.max_iter_div + 1, h->st);
             }
+        } else if (P.solver == SPHGPU_PCISPH) {
+            {
+                ProfScope ps(h, "pci_density", nf_live(h) * 44.0);
+                launch_pci_density(D, h->st);
+            }
+            {
+                ProfScope ps(h, "pci_pressure", nf_live(h) * 64.0);
+                launch_pci_pressure(D, h->st);
+            }
+            launch_scene_reduce(D, 2, P.density_tol, P.min_iter_density,
+                                P.max_iter_density > 0 ? P.max_iter_density : 1 << 30, h->st);
         } else {  // IISPH
             {
                 ProfScope ps(h, "ii_sumdij", nf_live(h) * 32.0);
@@ -698,6 +709,41 @@ int step_iisph(sphgpu_handle* h, sphgpu_stats* st) {
     return rcb ? rcb : rc;
 }
 
+// PciSph::sphStep (PciSph.cpp:31-95): search, weights, pci_computeForce
+// (97-126), the boundary reaction, updateFluids, updateSolids
+int step_pcisph(sphgpu_handle* h, sphgpu_stats* st) {
+    Dev& D = h->D;
+    int rc;
+    const size_t profStart = h->profUsed;
+    const int batch = first_batch(h, 0);
+    auto body = [h, batch]() {
+        Dev& D = h->D;
+        run_ns(h);
+        launch_cfl(D, h->P.cfl, h->P.spacing_r, h->P.dt, h->P.dt_min, h->st);
+        {
+            ProfScope ps(h, "pci_aext", nf_live(h) * 56.0);
+            launch_pci_aext(D, h->st);
+        }
+        solver_loop_start(h, 0, batch);
+    };
+    if (h->D.anyDynamic && (rc = rigid_update(h))) return rc;
+    if (h->frame == 0) launch_pci_rest_density(D, h->st);
+    if (h->P.use_graph && !h->D.anyDynamic) {
+        if ((rc = run_segment(h, 2, batch, body))) return rc;
+    } else {
+        body();
+    }
+    if ((rc = solver_loop_finish(h, 0, batch, profStart))) return rc;
+    rigid_reaction(h, 0);       // pci_boundPartForce_f: the WCSPH form (PciSph.cpp:440-460)
+    {
+        ProfScope ps(h, "pci_integrate", nf_live(h) * 84.0);
+        launch_pci_integrate(D, h->st);
+    }
+    const int rcb = update_solids_point(h);
+    rc = finish_step(h, st);
+    return rcb ? rcb : rc;
+}
+
 // host: uniform grid of a scene (SphBase::SpaceGridStruct::initialize, SphBase.cpp:963-988)
 void build_grids(sphgpu_handle* h, std::vector<int>& rowRank) {
     const float H = h->P.smooth_radius_h;
@@ -792,7 +838,7 @@ int sphgpu_device_available(void) {
 int sphgpu_create(const sphgpu_params* p, sphgpu_handle** out) {
     if (!p || !out) return SPHGPU_E_ARG;
     *out = nullptr;
-    if (p->solver < SPHGPU_WCSPH || p->solver > SPHGPU_IISPH) return SPHGPU_E_ARG;
+    if (p->solver < SPHGPU_WCSPH || p->solver > SPHGPU_PCISPH) return SPHGPU_E_ARG;
     if (!(p->spacing_r > 0) || !(p->smooth_radius_h > 0) || !(p->dt > 0)) return SPHGPU_E_ARG;
     if (p->num_scenes < 1 || p->num_scenes > 4096) return SPHGPU_E_ARG;
     if (p->warm_start < 0 || p->warm_start > 2) return SPHGPU_E_ARG;
@@ -1041,6 +1087,28 @@ int sphgpu_init(sphgpu_handle* h) {
     D.Hs = H;
     D.visc_eps = 0.01f * H * H;
     D.v0 = s * s * s;
+    {
+        // PCISPH stiffness (PciSph::pci_updateDelta, PciSph.cpp:133-162): a
+        // prototype particle with a full neighbourhood on the lattice of
+        // spacing s; delta = -1 / (beta (-|sum gradW|^2 - sum |gradW|^2)),
+        // beta = 2 v0^2; the passes divide by dt^2 of their scene
+        const double kh = 0.5 * double(H), sig = 1.0 / M_PI / (kh * kh * kh), sd = s;
+        const int n = int(std::ceil(double(H) / sd));
+        double gs[3] = {0, 0, 0}, g2 = 0.0;
+        for (int z = -n; z <= n; ++z)
+            for (int y = -n; y <= n; ++y)
+                for (int x = -n; x <= n; ++x) {
+                    if (!x && !y && !z) continue;
+                    const double vx = x * sd, vy = y * sd, vz = z * sd, dis = std::sqrt(vx * vx + vy * vy + vz * vz);
+                    if (!(dis < double(H))) continue;
+                    const double q = dis / kh;
+                    const double dw = sig / kh * (q <= 1.0 ? -3.0 * q + 2.25 * q * q : -0.75 * (2 - q) * (2 - q));
+                    gs[0] += vx * dw / dis; gs[1] += vy * dw / dis; gs[2] += vz * dw / dis;
+                    g2 += dw * dw;
+                }
+        const double v0 = sd * sd * sd, beta = 2.0 * v0 * v0;
+        D.pci_delta = float(-1.0 / (beta * (-(gs[0] * gs[0] + gs[1] * gs[1] + gs[2] * gs[2]) - g2)));
+    }
     D.spacing = s;
     D.dt_max = P.dt;
     D.warm_scale = P.warm_scale;
@@ -1252,6 +1320,7 @@ int sphgpu_step(sphgpu_handle* h, sphgpu_stats* st) {
     switch (h->P.solver) {
         case SPHGPU_WCSPH: return step_wcsph(h, st);
         case SPHGPU_DFSPH: return step_dfsph(h, st);
+        case SPHGPU_PCISPH: return step_pcisph(h, st);
         default: return step_iisph(h, st);
     }
 }

@@ -80,23 +80,32 @@ __global__ __launch_bounds__(256) void k_cfl_reduce(Dev D, float cfl, float spac
 // ------------------------------------------------------- scene reduce ----
 // mode 0: DFSPH density  err = sum(rho*/rho0)/n - 1          (DFSPH.cpp:696)
 // mode 1: DFSPH divergence / IISPH error   err = sum/n      (DFSPH.cpp:896, IISph_M.cpp:353)
-// Loop continuation (while err > tol || it < minIt), stopping at `cap`.
+// mode 2: PCISPH  err = max error/rho0 (second partial)     (PciSph.cpp:344-350)
+// Loop continuation (while err > tol || it < minIt; PCISPH err >= tol,
+// PciSph.cpp:116), stopping at `cap`.
 __global__ __launch_bounds__(256) void k_scene_reduce(Dev D, int mode, float tol, int minIt, int cap) {
     __shared__ double lds[4];
     int sc = blockIdx.x;
     if (!D.active[sc]) return;   // uniform per block
     int t0 = D.scenePadStart[sc] / TILE, t1 = D.scenePadStart[sc + 1] / TILE;
+    __shared__ float ldsf[4];
     double s = 0.0;
-    for (int t = t0 + threadIdx.x; t < t1; t += 256) s += D.partial[2 * t];
+    float mx = 0.0f;
+    for (int t = t0 + threadIdx.x; t < t1; t += 256) {
+        s += D.partial[2 * t];
+        if (mode == 2) mx = fmaxf(mx, (float)D.partial[2 * t + 1]);
+    }
     s = block_sum_256(s, lds);
+    mx = block_max_256(mx, ldsf);
     if (threadIdx.x == 0) {
         int n = D.sceneCount[sc];
         double err = n > 0 ? s / n : 0.0;
         if (mode == 0) err = n > 0 ? err - 1.0 : 0.0;
+        if (mode == 2) err = (double)mx;
         int it = D.iters[sc] + 1;
         D.iters[sc] = it;
         D.sceneErr[sc] = err;
-        bool cont = (err > (double)tol) || (it < minIt);
+        bool cont = (mode == 2 ? err >= (double)tol : err > (double)tol) || (it < minIt);
         if (cont && it >= cap) {
             cont = false;
             D.counters[C_COUNT - 1] = 1;  // max-iteration flag
@@ -870,6 +879,187 @@ __global__ __launch_bounds__(256) void k_ii_force(Dev D) {
     }
 }
 
+// =========================================================== PCISPH ======
+// Solenthaler & Pajarola's predictive-corrective loop as PciSph restates it
+// (PciSph.cpp:97-126).  Gather records: xk = (x*, phase) for the density
+// pass, vs = (x, p/rho*^2) for the pressure pass; v.w carries the last
+// predicted density for the next step's viscosity.  The position prediction
+// (pci_predictPosition, PciSph.cpp:289-306) of each iteration is fused into
+// the pass before it (k_pci_aext for the first, k_pci_pressure after).
+
+// frame 0: no prediction exists yet; fluid is added at rest density
+// (SphWithIO.cpp:1175 sets FluidPart::density = 1000 before adding)
+__global__ __launch_bounds__(256) void k_pci_rest_density(Dev D) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    D.v[s.i].w = D.rho0[phase_of(D.x[s.i])];
+}
+
+// pci_forceExceptPressure (PciSph.cpp:235-281, viscosity helpers 172-232):
+// a = g + viscosity, p = 0; first prediction x* = x + (v + a dt) dt
+__global__ __launch_bounds__(256) void k_pci_aext(Dev D) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i;
+    float4 pa = D.x[i], va = D.v[i];
+    int ph = phase_of(pa);
+    float m = D.mass[ph], alpha = D.visc[ph], rho0 = D.rho0[ph];
+    float rhoa = va.w;
+    const float Hcs2 = 2.0f * D.Hs * D.cs;
+    const bool multi = D.nphases > 1;
+    float ax = D.gx, ay = D.gy, az = D.gz;
+    int c = D.cnt[i];
+    each_fluid(D, i, nf_of(c), [&](int j) {
+        float4 pb = D.x[j];
+        float4 vb = D.v[j];            // (v, rho)
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
+        float g = kernel_gradfac(fast_sqrt(r2), D);
+        int phb = multi ? phase_of(pb) : ph;
+        float mm = m, al = alpha;
+        if (phb != ph) {                                 /* PciSph.cpp:186-201 */
+            mm = 0.5f * (m + D.mass[phb]);
+            al = 0.5f * (alpha + D.visc[phb]);
+        }
+        float nu = al * Hcs2 / (rhoa + vb.w);
+        float f = g * (-mm * (-nu * pro * fast_rcp(r2 + D.visc_eps)));
+        f = pro < 0.0f ? f : 0.0f;
+        ax += dx * f; ay += dy * f; az += dz * f;
+    });
+    each_boundary(D, i, nb_of(c), [&](int j) {
+        float4 pb = D.xb[j];
+        float4 vb = D.vb[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
+        float g = kernel_gradfac(fast_sqrt(r2), D);
+        float nu = D.solid_visc * Hcs2 / (rhoa * 2.0f);
+        float f = g * (-rho0 * pb.w * (-nu * pro * fast_rcp(r2 + D.visc_eps)));
+        f = pro < 0.0f ? f : 0.0f;
+        ax += dx * f; ay += dy * f; az += dz * f;
+    });
+    const float dt = D.dt[s.scene];
+    D.f4a[i] = make_float4(ax, ay, az, 0.0f);
+    D.f1[i] = 0.0f;
+    D.xk[i] = make_float4(pa.x + (va.x + ax * dt) * dt, pa.y + (va.y + ay * dt) * dt,
+                          pa.z + (va.z + az * dt) * dt, pa.w);
+}
+
+// pci_predictDensity (PciSph.cpp:310-352): rho* at the predicted positions
+// over the neighbours of x(t), every fluid neighbour with a's mass (SP08
+// number density), Kahan; error clamped at 0 (NEGLECT_NEGATIVE_PRESSURE,
+// SphBase.h:32); p += delta / dt^2 * error.  Partials: sum and max of
+// error / rho0.
+__global__ __launch_bounds__(256) void k_pci_density(Dev D) {
+    __shared__ double lds[4];
+    __shared__ float ldsf[4];
+    Slot s = slot_of(D, true);
+    double contrib = 0.0;
+    float emax = 0.0f;
+    if (s.live) {
+        const int i = s.i;
+        const float4 xa = D.xk[i];
+        int ph = phase_of(xa);
+        float m = D.mass[ph], rho0 = D.rho0[ph];
+        Kahan rho;
+        rho.init(m * D.W0);
+        int c = D.cnt[i];
+        each_fluid(D, i, nf_of(c), [&](int j) {
+            float4 pb = D.xk[j];
+            float dx = xa.x - pb.x, dy = xa.y - pb.y, dz = xa.z - pb.z;
+            float w = kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
+            rho.add(j != i ? m * w : 0.0f);
+        });
+        each_boundary(D, i, nb_of(c), [&](int j) {
+            float4 pb = D.xb[j];
+            float dx = xa.x - pb.x, dy = xa.y - pb.y, dz = xa.z - pb.z;
+            rho.add(rho0 * pb.w * kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D));
+        });
+        const float density = rho.s;
+        const float err = fmaxf(density - rho0, 0.0f);
+        const float dt = D.dt[s.scene];
+        const float p = D.f1[i] + D.pci_delta / (dt * dt) * err;
+        const float4 pa = D.x[i];
+        D.f1[i] = p;
+        D.rho[i] = density;
+        D.v[i].w = density;
+        D.vs[i] = make_float4(pa.x, pa.y, pa.z, p / (density * density));
+        emax = err / rho0;
+        contrib = (double)emax;
+    }
+    write_partial(D, s.tile, 0, contrib, lds);
+    const float mx = block_max_256(emax, ldsf);
+    if (threadIdx.x == 0) D.partial[2 * s.tile + 1] = (double)mx;
+}
+
+// pci_forcePressure (PciSph.cpp:399-436 with 356-397): a_p from the (x,
+// p/rho*^2) records; across phases p/delta^2 with delta = rho/m; boundary
+// 2 p_a/rho_a^2 when p_a > 0.  Then the next prediction x* = x + (v + (a +
+// a_p) dt) dt.
+__global__ __launch_bounds__(256) void k_pci_pressure(Dev D) {
+    Slot s = slot_of(D, true);
+    if (!s.live) return;
+    const int i = s.i;
+    const float4 pa = D.vs[i];
+    const float4 xa = D.x[i];
+    int ph = phase_of(xa);
+    float m = D.mass[ph], rho0 = D.rho0[ph];
+    const float qa = pa.w;
+    const bool multi = D.nphases > 1;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    int c = D.cnt[i];
+    each_fluid(D, i, nf_of(c), [&](int j) {
+        float4 pb = D.vs[j];
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
+        float acce;
+        int phb = multi ? phase_of(D.x[j]) : ph;
+        if (phb == ph) {
+            acce = g * (-m * (qa + pb.w));
+        } else {                                         /* SP08, PciSph.cpp:369-381 */
+            float mb = D.mass[phb];
+            acce = g * (-1.0f / m * (qa * m * m + pb.w * mb * mb));
+        }
+        ax += dx * acce; ay += dy * acce; az += dz * acce;
+    });
+    if (qa > 0.0f) {
+        each_boundary(D, i, nb_of(c), [&](int j) {
+            float4 pb = D.xb[j];
+            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+            float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
+            float acce = g * (-rho0 * pb.w * (2.0f * qa));
+            ax += dx * acce; ay += dy * acce; az += dz * acce;
+        });
+    }
+    const float dt = D.dt[s.scene];
+    const float4 va = D.v[i], a = D.f4a[i];
+    D.f4b[i] = make_float4(ax, ay, az, 0.0f);
+    D.xk[i] = make_float4(xa.x + (va.x + (a.x + ax) * dt) * dt, xa.y + (va.y + (a.y + ay) * dt) * dt,
+                          xa.z + (va.z + (a.z + az) * dt) * dt, xa.w);
+}
+
+// pci_forceTotal (PciSph.cpp:463-474) and SphBase::updateFluids
+// (SphBase.cpp:377-402): a += a_p, v += a dt, x += v dt, removal; in place
+__global__ __launch_bounds__(256) void k_pci_integrate(Dev D) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i;
+    const float dt = D.dt[s.scene];
+    const float4 a0 = D.f4a[i], ap = D.f4b[i];
+    const float4 a = make_float4(a0.x + ap.x, a0.y + ap.y, a0.z + ap.z, 0.0f);
+    float4 v = D.v[i], x = D.x[i];
+    v.x += a.x * dt; v.y += a.y * dt; v.z += a.z * dt;
+    x.x += v.x * dt; x.y += v.y * dt; x.z += v.z * dt;
+    D.f4a[i] = a;
+    D.v[i] = v;
+    D.x[i] = x;
+    if (!inside_domain(D.grids[s.scene], x)) {
+        D.id[i] = -1;
+        atomicAdd(&D.counters[C_REMOVED], 1);
+    }
+}
+
 // plain streaming copy (calibration of the byte counters)
 __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
     for (size_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) b[k] = a[k];
@@ -941,5 +1131,10 @@ void launch_ii_pressure(const Dev& D, hipStream_t st) {
     SPH_LAUNCH(k_ii_publish);
 }
 void launch_ii_force_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_force); }
+void launch_pci_rest_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_pci_rest_density); }
+void launch_pci_aext(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_pci_aext); }
+void launch_pci_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_pci_density); }
+void launch_pci_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_pci_pressure); }
+void launch_pci_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_pci_integrate); }
 
 }  // namespace sphgpu

@@ -76,6 +76,7 @@ struct Dev {
     float gx, gy, gz;
     float cs, gamma, nu, omega;
     float rho0[MAX_PHASES], mass[MAX_PHASES], visc[MAX_PHASES], tait_B[MAX_PHASES];
+    float pci_delta;   // PCISPH stiffness before the division by dt^2
     float solid_visc;  // viscosity_alpha of the boundary (one value for all static solids)
 
     // ---- fluid state (sorted, padded slots)
@@ -392,6 +393,11 @@ void launch_ii_aii(const Dev& D, hipStream_t st);
 void launch_ii_sumdij(const Dev& D, hipStream_t st);
 void launch_ii_pressure(const Dev& D, hipStream_t st);
 void launch_ii_force_integrate(const Dev& D, hipStream_t st);
+void launch_pci_rest_density(const Dev& D, hipStream_t st);
+void launch_pci_aext(const Dev& D, hipStream_t st);
+void launch_pci_density(const Dev& D, hipStream_t st);
+void launch_pci_pressure(const Dev& D, hipStream_t st);
+void launch_pci_integrate(const Dev& D, hipStream_t st);
 void launch_scene_reduce(const Dev& D, int mode, float tol, int minIt, int cap, hipStream_t st);
 void launch_stats(const Dev& D, double* part, double* out, hipStream_t st);
 void launch_fill_i32(int* p, int v, int n, hipStream_t st);

@@ -170,8 +170,9 @@ def _tank(scene: Scene, s: int, size, spacing, have_top, visc, origin=(0.0, 0.0,
 
 
 def small_dam(solver=sg.DFSPH, n=(10, 10, 10), box=(0.4, 0.3, 0.3), spacing=0.02, jitter=0.01,
-              seed=1001, have_top=True, **kw) -> Scene:
-    """A small dam break for parity tests (seconds on the oracle)."""
+              seed=1001, have_top=True, offset=1.0, **kw) -> Scene:
+    """A small dam break for parity tests (seconds on the oracle); the fluid
+    lattice starts `offset` spacings from the walls."""
     p = dict(solver=solver, spacing_r=spacing, smooth_radius_h=2 * spacing, dt=1e-3,
              gravity=(0.0, -9.81, 0.0), sound_speed=20.0, gamma=7.0, nu=1e-3, cfl=0.0,
              density_tol=1e-4, divergence_tol=1.0, min_iter_density=2, max_iter_density=100,
@@ -180,13 +181,16 @@ def small_dam(solver=sg.DFSPH, n=(10, 10, 10), box=(0.4, 0.3, 0.3), spacing=0.02
         p["dt"] = 1e-4
     if solver == sg.IISPH:
         p["density_tol"] = 1e-3
+    if solver == sg.PCISPH:      # PciSph.h:35 and m_TH.densityErro_eta (SphBase.cpp:221)
+        p.update(density_tol=0.01, min_iter_density=3, max_iter_density=10)
     p.update(kw)
     sc = Scene(f"small_dam_{solver}", p)
     sc.phases.append((1000.0, 0.01))
     _tank(sc, 0, box, spacing, have_top, 0.01)
     top = box[1] if have_top else box[1] + 0.2
     sc.domains.append(((0.0, 0.0, 0.0), (box[0], top, box[2])))
-    xyz = sg.lattice(n, (spacing, spacing, spacing), spacing, jitter, seed, 0)
+    o = offset * spacing
+    xyz = sg.lattice(n, (o, o, o), spacing, jitter, seed, 0)
     sc.fluids.append((0, 0, xyz))
     return sc
 

@@ -14,7 +14,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libsphgpu.so"
 
-WCSPH, DFSPH, IISPH = 0, 1, 2
+WCSPH, DFSPH, IISPH, PCISPH = 0, 1, 2, 3
 WARM_NONE, WARM_PREVIOUS, WARM_LEARNED = 0, 1, 2
 F_POS, F_VEL, F_RHO, F_PRESSURE, F_ACC, F_ALPHA, F_KAPPA_SUM, F_PHASE, F_KAPPAV_SUM = range(9)
 FIELD_COMPONENTS = {F_POS: 3, F_VEL: 3, F_RHO: 1, F_PRESSURE: 1, F_ACC: 3, F_ALPHA: 1,

@@ -45,7 +45,9 @@ enum sphgpu_status {
 enum sphgpu_solver {
     SPHGPU_WCSPH = 0,       /* WcSph::sphStep            WcSph.cpp:23-96   */
     SPHGPU_DFSPH = 1,       /* DFSPH::sphStep            DFSPH.cpp:9-75    */
-    SPHGPU_IISPH = 2        /* IISph_M::sphStep (Ihmsen)  IISph_M.cpp:10-75 */
+    SPHGPU_IISPH = 2,       /* IISph_M::sphStep (Ihmsen)  IISph_M.cpp:10-75 */
+    SPHGPU_PCISPH = 3       /* PciSph::sphStep (SP09)     PciSph.cpp:31-95; eta = density_tol,
+                               min/max iterations = min/max_iter_density (reference 0.01, 3, 10) */
 };
 
 enum sphgpu_warm_start {

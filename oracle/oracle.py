@@ -15,7 +15,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
-WCSPH, DFSPH, IISPH = 0, 1, 2
+WCSPH, DFSPH, IISPH, PCISPH = 0, 1, 2, 3
 F_POS, F_VEL, F_RHO, F_PRESSURE, F_ACC, F_ALPHA, F_KAPPA, F_ID, F_AADV, F_KAPPA_SUM, F_AII, F_RHO_ADV = range(12)
 COMPONENTS = {F_POS: 3, F_VEL: 3, F_ACC: 3, F_AADV: 3}
 _UPDATE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)      # int (*)(void* user)
@@ -96,6 +96,7 @@ def load():
         lib.orc_step.argtypes = [C.c_void_p, C.POINTER(OrcStats)]
         lib.orc_step_cb.argtypes = [C.c_void_p, C.POINTER(OrcStats), _UPDATE_FN, C.c_void_p]
         lib.orc_step_dt.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        lib.orc_pci_delta.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         lib.orc_num_fluid.argtypes = [C.c_void_p, C.c_int]
         lib.orc_get_fluid.argtypes = [C.c_void_p, C.c_int, C.c_int, DP]
         lib.orc_get_boundary_volume.argtypes = [C.c_void_p, C.c_int, DP]
@@ -227,6 +228,11 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"orc_step_cb returned {rc}")
         return st.as_dict()
+
+    def pci_delta(self) -> float:
+        d = C.c_double()
+        self.lib.orc_pci_delta(self.h, C.byref(d))
+        return d.value
 
     def step_dt(self) -> float:
         dt = C.c_double()

@@ -74,7 +74,7 @@ struct V3 {
 
 template <class T>
 struct FluidPart {           /* the fields of FluidParticle (SphBase.h:66-165) the loops touch */
-    V3<T> x, v, a, a_adv, v_adv, dii, sum_dijpj, a_p;
+    V3<T> x, v, a, a_adv, v_adv, dii, sum_dijpj, a_p, xp;
     T rho = 0, p = 0, p_l = 0, aii = 0, rho_adv = 0;
     T alpha = 0, kappa = 0, kappaV = 0, divdev = 0;
     T kappa_sum = 0, kappaV_sum = 0;   /* accumulated kappa*dt^2 (warm start state) */
@@ -1180,6 +1180,181 @@ public:
         dt_cur = dt;
     }
 
+    /* ------------------------------ PCISPH ----------------------------- */
+    /* PciSph::pci_updateDelta (PciSph.cpp:133-162): the SP09 stiffness from
+     * a prototype particle with a full neighbourhood on the uniform lattice
+     * of spacing r, delta = -1 / (beta (-|sum gradW|^2 - sum |gradW|^2)),
+     * beta = 2 v0^2; the solve uses delta / dt^2.  gradW = x (dW/dr) / |x|. */
+    T pciDelta() const {
+        const T Hs = H;
+        const int n = int(std::ceil(double(Hs) / double(r)));
+        V3<T> gsum;
+        T g2 = 0;
+        for (int z = -n; z <= n; ++z)
+            for (int y = -n; y <= n; ++y)
+                for (int x = -n; x <= n; ++x) {
+                    if (x == 0 && y == 0 && z == 0) continue;
+                    V3<T> vf(T(x) * r, T(y) * r, T(z) * r);
+                    T dis = std::sqrt(vf.len_square());
+                    if (dis < Hs) {
+                        T grad = -Wg(dis);
+                        gsum += vf * (grad / dis);
+                        g2 += grad * grad;
+                    }
+                }
+        T v0 = r * r * r, beta = 2 * v0 * v0;
+        return -1 / (beta * (-gsum.dot(gsum) - g2));
+    }
+
+    /* PciSph::pci_forceExceptPressure (PciSph.cpp:235-281) with the three
+     * viscosity helpers (172-232): p = 0, a_p = 0, a = g + viscosity, the
+     * densities of the previous step's last prediction */
+    void pci_forceExceptPressure() {
+        T v0 = r * r * r, Hs = H, cs = T(P.sound_speed);
+        int zd = 0;
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0, alpha = fluids[k].visc_alpha;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n)) reduction(+ : zd)
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                a.p = 0; a.a_p = V3<T>(); a.a = grav;
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    T dis = nb.dis;
+                    if (dis == 0) continue;
+                    T grad = -Wg(dis) / dis;
+                    if (nb.pidx.isFluid()) {
+                        const FluidPart<T>& b = fp(nb.pidx);
+                        int kb = nb.pidx.toFluidI();
+                        V3<T> xab = a.x - b.x;
+                        T pro = (a.v - b.v).dot(xab);
+                        if (pro >= 0) continue;
+                        T fmb = v0 * fluids[kb].rho0;
+                        T al = kb == k ? alpha : (alpha + fluids[kb].visc_alpha) / 2;
+                        T nu = 2 * al * Hs * cs / (a.rho + b.rho);
+                        T pi = -nu * pro / (dis * dis + T(0.01) * Hs * Hs);
+                        T acce = grad * (kb == k ? -fm0 * pi : -(fm0 + fmb) / 2 * pi);
+                        a.a += xab * acce;
+                    } else {
+                        const BoundPart<T>& b = bp(nb.pidx);
+                        T ralpha = solids[nb.pidx.toSolidI()].visc_alpha;
+                        V3<T> xab = a.x - b.x;
+                        T pro = (a.v - b.v).dot(xab);
+                        if (pro >= 0) continue;
+                        T nu = 2 * ralpha * Hs * cs / (a.rho * 2);
+                        T pi = -nu * pro / (dis * dis + T(0.01) * Hs * Hs);
+                        a.a += xab * (grad * (-rho0 * b.volume * pi));
+                    }
+                }
+            }
+        }
+        zeroDis += zd;
+    }
+
+    /* one prediction-correction iteration (PciSph.cpp:111-115):
+     * pci_predictPosition (289-306), pci_predictDensity (310-352, negative
+     * errors clamped: NEGLECT_NEGATIVE_PRESSURE, SphBase.h:32),
+     * pci_forcePressure (399-436 with 356-397).  Returns the largest relative
+     * density error; *avg the mean. */
+    T pci_iteration(T deltaT, double* avg) {
+        T v0 = r * r * r, dt = dt_cur, Hs = H;
+        for (auto& f : fluids) {
+            int n = int(f.parts.size());
+#pragma omp parallel for
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& q = f.parts[i];
+                q.xp = q.x + (q.v + (q.a + q.a_p) * dt) * dt;
+            }
+        }
+        T emax = 0; double esum = 0; long long cnt = 0;
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T rho0 = fluids[k].rho0, fm0 = rho0 * v0, wm0 = W0 * fm0;
+            T kmax = 0;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n)) reduction(+ : esum, cnt) reduction(max : kmax)
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                T density = wm0;
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    if (nb.pidx.isFluid()) {
+                        T dis = std::sqrt((a.xp - fp(nb.pidx).xp).len_square());
+                        if (dis < Hs) density += fm0 * W(dis);     /* a's mass: SP08 number density */
+                    } else {
+                        const BoundPart<T>& b = bp(nb.pidx);
+                        T dis = std::sqrt((a.xp - b.x).len_square());
+                        if (dis < Hs) density += rho0 * b.volume * W(dis);
+                    }
+                }
+                a.rho = density;
+                T err = density - rho0;
+                if (err < 0) err = 0;
+                a.p += deltaT * err;
+                kmax = std::max(kmax, err / rho0);
+                esum += double(err / rho0); ++cnt;
+            }
+            emax = std::max(emax, kmax);
+        }
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+            T fm0 = fluids[k].rho0 * v0, rho0 = fluids[k].rho0;
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                a.a_p = V3<T>();
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    T dis = nb.dis;
+                    if (dis == 0) continue;
+                    T grad = -Wg(dis) / dis;
+                    if (nb.pidx.isFluid()) {
+                        const FluidPart<T>& b = fp(nb.pidx);
+                        int kb = nb.pidx.toFluidI();
+                        V3<T> xab = a.x - b.x;
+                        T acce;
+                        if (kb == k) {
+                            acce = grad * (-fm0 * (a.p / (a.rho * a.rho) + b.p / (b.rho * b.rho)));
+                        } else {                                   /* SP08, PciSph.cpp:369-381 */
+                            T fmb = v0 * fluids[kb].rho0, da = a.rho / fm0, db = b.rho / fmb;
+                            acce = grad * (-1 / fm0 * (a.p / (da * da) + b.p / (db * db)));
+                        }
+                        a.a_p += xab * acce;
+                    } else if (a.p > 0) {                         /* PciSph.cpp:384-397 */
+                        const BoundPart<T>& b = bp(nb.pidx);
+                        a.a_p += (a.x - b.x) * (grad * (-rho0 * b.volume * (2 * a.p / (a.rho * a.rho))));
+                    }
+                }
+            }
+        }
+        *avg = cnt ? esum / double(cnt) : 0.0;
+        return emax;
+    }
+
+    /* PciSph::pci_computeForce (PciSph.cpp:97-126) and pci_forceTotal
+     * (463-500): iterations while (max error >= eta || it < min) && it < max */
+    void pci_computeForce(orc_stats* st) {
+        const T deltaT = pciDelta() / (dt_cur * dt_cur);
+        pci_forceExceptPressure();
+        int it = 0; T emax = 0; double avg = 0;
+        const int maxIt = P.max_iter_density > 0 ? P.max_iter_density : std::numeric_limits<int>::max();
+        do {
+            emax = pci_iteration(deltaT, &avg);
+            ++it;
+        } while ((emax >= T(P.density_tol) || it < P.min_iter_density) && it < maxIt);
+        st->iter_density = it;
+        st->density_err = double(emax);
+        st->max_iter_hit = (emax >= T(P.density_tol) && it >= maxIt) ? 1 : 0;
+        for (auto& f : fluids) {
+            int n = int(f.parts.size());
+#pragma omp parallel for
+            for (int i = 0; i < n; ++i) f.parts[i].a += f.parts[i].a_p;
+        }
+    }
+
     void fillStats(orc_stats* st) {
         double se = 0, me = -1e300, ke = 0; long long n = 0; int nan = 0;
         T v0 = r * r * r;
@@ -1227,6 +1402,17 @@ public:
             updateSolidPartWeight();
             df_first();
             df_divergence(st);
+            fillStats(st);
+        } else if (P.solver == ORC_PCISPH) {                               /* PciSph.cpp:31-95 */
+            applyPendingPoses();
+            if (frame == 0)      /* the host adds fluid at rest density (SphWithIO.cpp:1175) */
+                for (auto& f : fluids) for (auto& q : f.parts) q.rho = f.rho0;
+            neighbourSearch();
+            updateSolidPartWeight();
+            pci_computeForce(st);
+            rigidReaction(false);           /* pci_boundPartForce_f = the WCSPH form, PciSph.cpp:440-460 */
+            st->removed = updateFluidsWC();
+            rc = updateSolids(cb, user);
             fillStats(st);
         } else {                                                           /* IISph_M.cpp:10-75 */
             applyPendingPoses();
@@ -1379,6 +1565,10 @@ int orc_init(orc_sim* s) {
 
 int orc_step(orc_sim* s, orc_stats* st) {
     return dispatch(s, [&](auto& S) { return S.step(st); });
+}
+
+int orc_pci_delta(const orc_sim* s, double* delta) {
+    return dispatchc(s, [&](const auto& S) { *delta = double(S.pciDelta()); return 0; });
 }
 
 int orc_step_dt(const orc_sim* s, double* dt) {

@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-enum { ORC_WCSPH = 0, ORC_DFSPH = 1, ORC_IISPH = 2 };
+enum { ORC_WCSPH = 0, ORC_DFSPH = 1, ORC_IISPH = 2, ORC_PCISPH = 3 };
 
 /* Mirrors ThresholdsBase m_TH (SphBase.h:546-583) plus the solver settings
  * that the reference hard-codes (DFSPH.cpp:225-230, 762-766; IISph_M.cpp:285,
@@ -98,6 +98,7 @@ int      orc_step(orc_sim* s, orc_stats* st);        /* one sphStep + runOneStep
  * orc_set_solid_pose take effect at that point (or at orc_init) */
 int      orc_step_cb(orc_sim* s, orc_stats* st, int (*cb)(void*), void* user);
 int      orc_step_dt(const orc_sim* s, double* dt);     /* dt of the current (or last) step */
+int      orc_pci_delta(const orc_sim* s, double* delta); /* PCISPH delta (before / dt^2) */
 int      orc_num_fluid(const orc_sim* s, int k);
 int      orc_get_fluid(const orc_sim* s, int k, int field, double* out);
 int      orc_get_boundary_volume(const orc_sim* s, int k, double* out);

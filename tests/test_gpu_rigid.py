@@ -42,7 +42,7 @@ def _moving_pose(t, centre=(0.31, 0.1, 0.14)):
     return scenes.pose(w[1] * t, tr, v=v, w=w)
 
 
-@pytest.mark.parametrize("solver", [sg.WCSPH, sg.IISPH, sg.DFSPH])
+@pytest.mark.parametrize("solver", [sg.WCSPH, sg.IISPH, sg.DFSPH, sg.PCISPH])
 def test_rigid_single_step_fields_and_load(gpu, solver):
     sc = scenes.dam_with_box(solver, **FIXED)
     sc.rigids[0] = sc.rigids[0][:3] + (_moving_pose(0.0),)

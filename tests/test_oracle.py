@@ -481,3 +481,99 @@ def test_pose_set_between_steps_applies_at_the_update_point():
         runs.append(o.get_by_id(orc.F_POS)[1])
     np.testing.assert_array_equal(runs[0], runs[1])
     assert np.abs(runs[0] - runs[2]).max() > 1e-6          # the move did act
+
+
+# ------------------------------------------------------------------ PCISPH ----
+def _pci_params(s=0.01, **kw):
+    p = dict(solver=orc.PCISPH, spacing_r=s, smooth_radius_h=2 * s, dt=1e-3, sound_speed=20.0,
+             gravity=(0.0, 0.0, 0.0), space_min=(-1.0, -1.0, -1.0), space_max=(1.0, 1.0, 1.0),
+             density_tol=0.01, min_iter_density=3, max_iter_density=10)
+    p.update(kw)
+    return p
+
+
+@pytest.mark.parametrize("s", [0.01, 0.02])
+def test_pcisph_delta_closed_form(s):
+    """SP09 eq. (8) on the symmetric prototype (PciSph.cpp:133-162): the
+    gradient sum vanishes, so delta = 1 / (2 v0^2 sum_j |gradW_j|^2) over the
+    lattice points inside the support, with |gradW| = -dW/dr from the
+    (separately pinned) kernel."""
+    o = orc.Oracle(_pci_params(s), 64)
+    r = float(np.float32(s))              # spacing_r and smoothRadius_h are FP32 parameters
+    H = float(np.float32(2 * s))
+    n = 2
+    g = np.arange(-n, n + 1) * r
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    d = np.sqrt(X ** 2 + Y ** 2 + Z ** 2).ravel()
+    d = d[(d > 0) & (d < H)]
+    _, G = orc.kernel_eval(d, H / 2)      # kernel h = smoothRadius_h / 2
+    want = 1.0 / (2.0 * (r ** 3) ** 2 * np.sum(G ** 2))
+    assert o.pci_delta() == pytest.approx(want, rel=1e-12)
+    o.close()
+
+
+def test_pcisph_rest_lattice_stays_at_rest():
+    """An interior lattice below rest density (999.97 < 1000) predicts no
+    compression: the clamped error (SphBase.h:32) gives p = 0, so nothing
+    moves, and the loop runs its minimum of 3 iterations."""
+    o = orc.Oracle(_pci_params(), 64)
+    xyz = sg.lattice((9, 9, 9), (0.0, 0.0, 0.0), 0.01, 0.0, 0, 0)
+    o.add_fluid(xyz, 1000.0, 0.01)
+    o.init()
+    st = o.step()
+    assert st["iter_density"] == 3 and st["density_err"] == 0.0
+    np.testing.assert_array_equal(o.get(orc.F_POS), xyz.astype(np.float64))
+    assert np.abs(o.get(orc.F_PRESSURE)).max() == 0.0
+
+
+@pytest.mark.parametrize("two_phase", [False, True])
+def test_pcisph_pressure_forces_conserve_momentum(two_phase):
+    """Pressure (same phase, PciSph.cpp:356-366; SP08 across phases, 369-381)
+    and same-phase viscosity are antisymmetric in each pair: with gravity off
+    and no walls the mass-weighted accelerations cancel.  (The cross-phase
+    viscosity, an acceleration with the mean mass, PciSph.cpp:186-201, is not
+    momentum-conserving, so the two-phase case runs without viscosity.)"""
+    s = 0.02
+    o = orc.Oracle(_pci_params(s, dt=2e-4), 64)
+    xyz = sg.lattice((8, 8, 8), (0.0, 0.0, 0.0), s * 0.95, 0.03, 7, 0)
+    rng = np.random.default_rng(5)
+    vel = rng.normal(0, 0.1, xyz.shape).astype(np.float32)
+    m = [1000.0 * float(np.float32(s)) ** 3]
+    if two_phase:
+        heavy = xyz[:, 1] < 3.5 * s
+        o.add_fluid(xyz[~heavy], 1000.0, 0.0, vel=vel[~heavy])
+        o.add_fluid(xyz[heavy], 3000.0, 0.0, vel=vel[heavy])
+        m.append(3000.0 * float(np.float32(s)) ** 3)
+    else:
+        o.add_fluid(xyz, 1000.0, 0.01, vel=vel)
+    o.init()
+    st = o.step()
+    assert st["iter_density"] >= 3
+    f = [m[k] * o.get(orc.F_ACC, k) for k in range(len(m))]
+    total = np.abs(sum(q.sum(0) for q in f)).max()
+    scale = sum(np.abs(q).sum() for q in f)
+    assert scale > 0 and total / scale < 1e-10
+
+
+def test_pcisph_iterations_reduce_the_density_error():
+    """Each correction lowers the predicted compression of a blob squeezed by
+    3 % (SP09 Algorithm 2): the largest error falls monotonically from 12 %
+    after one iteration to under 1 % of that after ten, independently of dt
+    (delta / dt^2 against displacements ~ dt^2); and the exit criterion (max
+    error < eta unless the cap is hit) holds on a dam break."""
+    for dt in (1e-3, 5e-4):
+        err = []
+        for it in (1, 2, 3, 4, 6, 10):
+            o = orc.Oracle(_pci_params(0.02, dt=dt, min_iter_density=it, max_iter_density=it,
+                                       density_tol=0.0), 64)
+            o.add_fluid(sg.lattice((8, 8, 8), (0.0, 0.0, 0.0), 0.02 * 0.97, 0.02, 3, 0), 1000.0, 0.01)
+            o.init()
+            err.append(o.step()["density_err"])
+        assert err[0] > 0.1 and all(b < a for a, b in zip(err, err[1:])) and err[-1] < 0.02 * err[0]
+    sc = scenes.small_dam(orc.PCISPH, n=(10, 10, 10), density_tol=0.01, min_iter_density=3,
+                          max_iter_density=10)
+    o = orc.from_scene(sc, 64)
+    for _ in range(10):
+        st = o.step()
+        assert st["max_iter_hit"] or st["density_err"] < 0.01
+        assert 3 <= st["iter_density"] <= 10
