@@ -239,6 +239,24 @@ def dfsph_c2(seed=1002, n=100) -> Scene:
     return sc
 
 
+def pcisph_dam(seed=1002, n=100) -> Scene:
+    """PCISPH dam break on the DFSPH dam's geometry (n^3 = 1,000,000 fluid at
+    0.01 m, closed 4x2x2 m box), dt 5e-4 with CFL 0.4, eta 1 %, 3-10
+    iterations (PciSph.h:35, SphBase.cpp:221); the fluid starts 1.5 spacings
+    from the walls (one spacing starts it 10 % compressed, which the
+    single-particle stiffness overshoots)."""
+    s = 0.01
+    p = dict(solver=sg.PCISPH, spacing_r=s, smooth_radius_h=2 * s, dt=5e-4, cfl=0.4, dt_min=1e-5,
+             gravity=(0.0, -9.81, 0.0), sound_speed=20.0, nu=1e-3,
+             density_tol=0.01, min_iter_density=3, max_iter_density=10)
+    sc = Scene(f"pcisph_dam_{n ** 3}", p)
+    sc.phases.append((1000.0, 0.01))
+    _tank(sc, 0, (4.0, 2.0, 2.0), s, True, 0.01)
+    sc.domains.append(((0.0, 0.0, 0.0), (4.0, 2.0, 2.0)))
+    sc.fluids.append((0, 0, sg.lattice((n, n, n), (1.5 * s, 1.5 * s, 1.5 * s), s, 0.01, seed, 0)))
+    return sc
+
+
 def dfsph_c3(seed=1003, n=126) -> Scene:
     """DFSPH, two facing n^3 blocks (4,000,752 at n=126) at 0.01 m in a closed 6x3x3 m box."""
     s = 0.01
@@ -286,4 +304,5 @@ WORKLOADS = {
     "dfsph_c3": dfsph_c3,
     "iisph_c4": iisph_batched_c4,
     "iisph_c5": iisph_c5,
+    "pcisph_dam": pcisph_dam,
 }
