@@ -119,6 +119,19 @@ SIGNATURES = {
     "sphgpu_write_json": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "sphgpu_read_pos": (C.c_int, [C.c_char_p, _FP, C.c_longlong, C.POINTER(C.c_longlong),
                                   C.POINTER(C.c_longlong), C.c_char_p, C.c_int]),
+    "sphgpu_surfacer_create": (C.c_int, [_FP, _FP, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_void_p)]),
+    "sphgpu_surfacer_destroy": (None, [C.c_void_p]),
+    "sphgpu_surfacer_last_error": (C.c_char_p, [C.c_void_p]),
+    "sphgpu_surfacer_grid": (C.c_int, [C.c_void_p, _DP, C.POINTER(C.c_int), _DP]),
+    "sphgpu_surfacer_aniso": (C.c_int, [C.c_void_p, C.c_longlong, _FP]),
+    "sphgpu_surfacer_isotro": (C.c_int, [C.c_void_p, C.c_longlong, _FP]),
+    "sphgpu_surfacer_particles": (C.c_int, [C.c_void_p, _FP, _FP, _FP, C.POINTER(C.c_int)]),
+    "sphgpu_surfacer_get_field": (C.c_int, [C.c_void_p, _FP]),
+    "sphgpu_surfacer_set_field": (C.c_int, [C.c_void_p, _FP]),
+    "sphgpu_surfacer_marching_cubes": (C.c_int, [C.c_void_p, C.c_float, _FP, C.c_longlong,
+                                                 C.POINTER(C.c_longlong)]),
+    "sphgpu_marching_cubes_table": (C.c_int, [C.POINTER(C.c_ushort), C.POINTER(C.c_ubyte),
+                                              C.POINTER(C.c_byte)]),
     "sphgpu_synchronize": (C.c_int, [C.c_void_p]),
     "sphgpu_num_fluid": (C.c_longlong, [C.c_void_p]),
     "sphgpu_get_fluid": (C.c_int, [C.c_void_p, C.c_int, _FP]),
@@ -226,6 +239,84 @@ def read_pos(path):
     if rc != 0:
         raise SphError(rc, f"read_pos {path}")
     return fr.value, mask.value.decode(), data.reshape(n.value, per)
+
+
+def marching_cubes_table():
+    """(edges[256] uint16, ntris[256] uint8, tris[256, 16] int8) of the surfacer."""
+    lib = load()
+    e = np.zeros(256, np.uint16)
+    n = np.zeros(256, np.uint8)
+    t = np.zeros((256, 16), np.int8)
+    lib.sphgpu_marching_cubes_table(e.ctypes.data_as(C.POINTER(C.c_ushort)), n.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                    t.ctypes.data_as(C.POINTER(C.c_byte)))
+    return e, n, t
+
+
+class Surfacer:
+    """The reference's offline surfacer (surface_field + marchingCubes) on the device."""
+
+    def __init__(self, mn, mx, radius_p, radius_h, cell_size):
+        self.lib = load()
+        h = C.c_void_p()
+        a, b = _f32(mn), _f32(mx)
+        rc = self.lib.sphgpu_surfacer_create(_fp(a), _fp(b), radius_p, radius_h, cell_size, C.byref(h))
+        if rc != 0:
+            raise SphError(rc, "surfacer_create")
+        self.h = h.value
+        gmin = np.zeros(3)
+        gnum = (C.c_int * 3)()
+        cell = C.c_double()
+        self.lib.sphgpu_surfacer_grid(self.h, gmin.ctypes.data_as(_DP), gnum, C.byref(cell))
+        self.grid_min, self.grid_num, self.cell = gmin, tuple(gnum), cell.value
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise SphError(rc, f"{what}: {self.lib.sphgpu_surfacer_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.lib.sphgpu_surfacer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def aniso(self, xyz):
+        x = _f32(xyz).reshape(-1, 3)
+        self._check(self.lib.sphgpu_surfacer_aniso(self.h, len(x), _fp(x)), "aniso")
+
+    def isotro(self, xyz):
+        x = _f32(xyz).reshape(-1, 3)
+        self._check(self.lib.sphgpu_surfacer_isotro(self.h, len(x), _fp(x)), "isotro")
+
+    def particles(self, n):
+        xb, G, det = np.zeros((n, 3), np.float32), np.zeros((n, 3, 3), np.float32), np.zeros(n, np.float32)
+        nb = np.zeros(n, np.int32)
+        self._check(self.lib.sphgpu_surfacer_particles(self.h, _fp(xb), _fp(G), _fp(det),
+                                                       nb.ctypes.data_as(C.POINTER(C.c_int))), "particles")
+        return xb, G, det, nb
+
+    def field(self) -> np.ndarray:
+        nx, ny, nz = self.grid_num
+        f = np.zeros(nx * ny * nz, np.float32)
+        self._check(self.lib.sphgpu_surfacer_get_field(self.h, _fp(f)), "get_field")
+        return f.reshape(nz, ny, nx)
+
+    def set_field(self, f):
+        a = _f32(f).ravel()
+        assert a.size == int(np.prod(self.grid_num))
+        self._check(self.lib.sphgpu_surfacer_set_field(self.h, _fp(a)), "set_field")
+
+    def marching_cubes(self, iso=0.5) -> np.ndarray:
+        n = C.c_longlong()
+        self._check(self.lib.sphgpu_surfacer_marching_cubes(self.h, iso, None, 0, C.byref(n)), "marching_cubes")
+        out = np.zeros((max(n.value, 1), 3, 3), np.float32)
+        self._check(self.lib.sphgpu_surfacer_marching_cubes(self.h, iso, _fp(out), n.value, C.byref(n)),
+                    "marching_cubes")
+        return out[:n.value]
 
 
 def lattice(n, origin, spacing, jitter=0.0, seed=0, stream=0) -> np.ndarray:

@@ -261,6 +261,47 @@ int sphgpu_write_json(sphgpu_handle* h, const char* path, int phase);
 int sphgpu_read_pos(const char* path, float* data, long long cap, long long* n, long long* frame,
                     char* mask, int mask_len);
 
+/* ---- surface reconstruction (Source/re.c, surface_field, marching_cubes) */
+/* The reference's offline surfacer: Yu & Turk anisotropic kernels
+ * (surface_field::surface_aniso, surface_field.cpp:83-176) or the isotropic
+ * field (surface_isotro, 45-75) on a node grid, then marching cubes
+ * (marchingCubes, marching_cubes.cpp:45-120).  Independent of a simulation
+ * handle (re.c reads .pos files). */
+typedef struct sphgpu_surfacer sphgpu_surfacer;
+/* surface_field::surface_setup (surface_field.cpp:8-43): field grid from
+ * min - 0.75 r_h with int((max - min + 1.5 r_h) / cell) + 1 nodes per axis;
+ * particle volume r_p^3; anisotropy radius 2 r_h; k_r 4, k_s 1.6 /
+ * (0.16 (2 r_h)^2), k_n 0.95, N_e 25, lambda 1.  re.c uses r_h = 2 r_p and
+ * cell = r_p / 4.  Without a device the parameters are still queryable and
+ * the compute calls return SPHGPU_E_DEVICE. */
+int sphgpu_surfacer_create(const float min[3], const float max[3], float radius_p, float radius_h,
+                           float cell_size, sphgpu_surfacer** out);
+void sphgpu_surfacer_destroy(sphgpu_surfacer* s);
+const char* sphgpu_surfacer_last_error(const sphgpu_surfacer* s);
+/* get_grid_min / get_grid_num / get_cell_size (surface_field.h:43-46) */
+int sphgpu_surfacer_grid(const sphgpu_surfacer* s, double grid_min[3], int grid_num[3], double* cell_size);
+/* Field from n particle positions (xyz, float32): surface_aniso / surface_isotro. */
+int sphgpu_surfacer_aniso(sphgpu_surfacer* s, long long n, const float* xyz);
+int sphgpu_surfacer_isotro(sphgpu_surfacer* s, long long n, const float* xyz);
+/* After sphgpu_surfacer_aniso, per particle in input order (each optional):
+ * ellipsoid centre x_bar (3), G row-major (9), det G, neighbours within 2 r_h. */
+int sphgpu_surfacer_particles(sphgpu_surfacer* s, float* xbar, float* G, float* det, int* neighbours);
+/* The node field (get_field), x fastest: grid_num[0] * grid_num[1] * grid_num[2]
+ * floats; set_field replaces it (marching cubes on a caller's field). */
+int sphgpu_surfacer_get_field(sphgpu_surfacer* s, float* out);
+int sphgpu_surfacer_set_field(sphgpu_surfacer* s, const float* in);
+/* marchingCubes: triangles of the iso-surface, 9 floats each (three
+ * vertices), in cube order (x fastest, then y, then z); *ntris = the count
+ * (tris NULL: count only; more than cap: SPHGPU_E_CAPACITY after writing
+ * cap).  Corners below iso are outside; normals point towards them. */
+int sphgpu_surfacer_marching_cubes(sphgpu_surfacer* s, float iso, float* tris, long long cap,
+                                   long long* ntris);
+/* The marching-cubes case table the surfacer uses (host only, no device):
+ * per case (bit k = corner k below iso; corners and edges numbered as in
+ * marching_cubes.cpp:1-30, 126-156) the crossed-edge mask (256), the
+ * triangle count (256) and up to 5 edge triples, -1 terminated (256 x 16). */
+int sphgpu_marching_cubes_table(unsigned short* edges, unsigned char* ntris, signed char* tris);
+
 /* ---- measurement ------------------------------------------------------- */
 /* Selects one kernel (by name, NULL to stop) whose launches are bracketed by
  * HIP events on the handle's stream; sphgpu_kernel_timing returns the launch

@@ -97,6 +97,7 @@ def load():
         lib.orc_step_cb.argtypes = [C.c_void_p, C.POINTER(OrcStats), _UPDATE_FN, C.c_void_p]
         lib.orc_step_dt.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         lib.orc_pci_delta.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        lib.orc_marching_cubes.restype = C.c_longlong
         lib.orc_num_fluid.argtypes = [C.c_void_p, C.c_int]
         lib.orc_get_fluid.argtypes = [C.c_void_p, C.c_int, C.c_int, DP]
         lib.orc_get_boundary_volume.argtypes = [C.c_void_p, C.c_int, DP]
@@ -272,6 +273,58 @@ class Oracle:
         over = self.lib.orc_neighbours(self.h, k, cap, counts.ctypes.data_as(IP), kinds.ctypes.data_as(IP),
                                        ids.ctypes.data_as(IP), dis.ctypes.data_as(FP))
         return counts, kinds, ids, dis, over
+
+
+# ---- surfacer (surface_field + marchingCubes, surface_oracle.cpp) ----------
+def surface_grid(mn, mx, rp, rh, cell):
+    lib = load()
+    f = lambda a: np.ascontiguousarray(a, np.float32).ctypes.data_as(C.POINTER(C.c_float))
+    gmin = np.zeros(3)
+    gnum = np.zeros(3, np.int32)
+    lib.orc_surface_grid(f(mn), f(mx), C.c_float(rp), C.c_float(rh), C.c_float(cell),
+                         gmin.ctypes.data_as(C.POINTER(C.c_double)), gnum.ctypes.data_as(C.POINTER(C.c_int)))
+    return gmin, tuple(int(v) for v in gnum)
+
+
+def surface_field(mn, mx, rp, rh, cell, xyz, aniso=True):
+    """(field[nz, ny, nx], xbar, G, det, neighbours) in FP64."""
+    lib = load()
+    a, b = np.ascontiguousarray(mn, np.float32), np.ascontiguousarray(mx, np.float32)
+    x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    n = len(x)
+    _, num = surface_grid(mn, mx, rp, rh, cell)
+    field = np.zeros(num[0] * num[1] * num[2])
+    xb, G, det, nb = np.zeros((n, 3)), np.zeros((n, 3, 3)), np.zeros(n), np.zeros(n, np.int32)
+    FP, DP = C.POINTER(C.c_float), C.POINTER(C.c_double)
+    lib.orc_surface_field(a.ctypes.data_as(FP), b.ctypes.data_as(FP), C.c_float(rp), C.c_float(rh),
+                          C.c_float(cell), int(aniso), C.c_longlong(n), x.ctypes.data_as(FP),
+                          field.ctypes.data_as(DP), xb.ctypes.data_as(DP), G.ctypes.data_as(DP),
+                          det.ctypes.data_as(DP), nb.ctypes.data_as(C.POINTER(C.c_int)))
+    return field.reshape(num[2], num[1], num[0]), xb, G, det, nb
+
+
+def marching_cubes(field, grid_min, cell, iso=0.5):
+    """Triangles [n, 3, 3] (FP64) over every cube of a float32 node field[nz, ny, nx]."""
+    lib = load()
+    f = np.ascontiguousarray(field, np.float32)
+    num = np.array([f.shape[2], f.shape[1], f.shape[0]], np.int32)
+    mn = np.ascontiguousarray(grid_min, np.float64)
+    args = (f.ctypes.data_as(C.POINTER(C.c_float)), num.ctypes.data_as(C.POINTER(C.c_int)),
+            mn.ctypes.data_as(C.POINTER(C.c_double)), C.c_double(cell), C.c_float(iso))
+    n = lib.orc_marching_cubes(*args, None, C.c_longlong(0))
+    out = np.zeros((max(n, 1), 3, 3))
+    lib.orc_marching_cubes(*args, out.ctypes.data_as(C.POINTER(C.c_double)), C.c_longlong(n))
+    return out[:n]
+
+
+def marching_cubes_table():
+    lib = load()
+    e = np.zeros(256, np.uint16)
+    n = np.zeros(256, np.uint8)
+    t = np.zeros((256, 16), np.int8)
+    lib.orc_marching_cubes_table(e.ctypes.data_as(C.POINTER(C.c_ushort)), n.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                 t.ctypes.data_as(C.POINTER(C.c_byte)))
+    return e, n, t
 
 
 def from_scene(sc, bits=64, **overrides) -> Oracle:
