@@ -224,12 +224,63 @@ def cpu_baseline(sc, warm_start: int, budget_s: float):
     return dict(steps=steps, elapsed=elapsed, threads=threads, model=model, per_step=per_step)
 
 
+def surface_bench(args) -> int:
+    """Surface reconstruction of one frame of the 1M-particle DFSPH dam (after
+    `--warmup` simulation steps) with the reference surfacer's parameters
+    (re.c: r_h = 2 r_p, node spacing r_p / 4, iso 0.5): a "step" is the
+    anisotropic field plus marching cubes of that frame; value = particles
+    surfaced per second."""
+    sc = scenes.dfsph_c2()
+    sim = scenes.make_simulation(sc, warm_start=sg.WARM_PREVIOUS)
+    for k in range(args.warmup):
+        sim.step(stats=(k + 1) % 50 == 0)
+    x = sim.get(sg.F_POS)
+    sim.close()
+    s = sc.params["spacing_r"]
+    mn, mx = x.min(0), x.max(0)
+    sf = sg.Surfacer(mn, mx, s, 2 * s, s / 4)
+    log(f"[bench] frame after {args.warmup} steps: {len(x)} particles, field grid {sf.grid_num}")
+    sf.aniso(x)                         # warm-up run (allocations, code load)
+    times, ntri = [], 0
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        sf.aniso(x)
+        tris = sf.marching_cubes(0.5)
+        times.append(time.perf_counter() - t0)
+        ntri = len(tris)
+    sf.close()
+    ms = float(np.mean(times)) * 1e3
+    value = len(x) / (ms * 1e-3)
+    log(f"[bench] surface: {ms:.1f} ms per frame, {ntri} triangles")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        # serial FP64 restatement (the reference's loop is serial too) on a
+        # sample: the particles of a sub-box holding about 3000 of them
+        lo = np.quantile(x, 0.45, axis=0)
+        sel = x[np.all((x >= lo) & (x <= lo + 0.15), axis=1)][:3000]
+        t0 = time.perf_counter()
+        orc.surface_field(sel.min(0), sel.max(0), s, 2 * s, s / 4, sel, True)
+        el = time.perf_counter() - t0
+        cpu = {"value": round(len(sel) / el, 2), "unit": "particles/s", "cores": 1, "kind": "port",
+               "sample": f"anisotropic field of {len(sel)} particles of the same frame, FP64 oracle, serial"}
+    out = {"metric": "particles surfaced per second (anisotropic field + marching cubes)", "value": round(value, 1),
+           "unit": "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"surface_dam_{len(x)}", "grid": list(map(int, sf.grid_num)),
+                      "triangles": ntri, "radius_p": s, "radius_h": 2 * s, "cell": s / 4, "iso": 0.5},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="dfsph_c2", choices=sorted(scenes.WORKLOADS))
+    ap.add_argument("--workload", default="dfsph_c2", choices=sorted(scenes.WORKLOADS) + ["surface_dam"])
     ap.add_argument("--warm-start", default="learned", choices=["none", "previous", "learned"])
     ap.add_argument("--no-ab", action="store_true", help="skip the plain-solver arm")
     ap.add_argument("--no-cpu", action="store_true")
@@ -249,6 +300,11 @@ def main() -> int:
     if args.summarize_trace:
         return summarize_trace(None, args.summarize_trace[0], build_scene(args.workload).name,
                                args.summarize_trace[1])
+    if args.workload == "surface_dam":
+        if args.gpus != 1 or not sg.device_available():
+            print("surface_dam needs exactly one gfx9 device", file=sys.stderr)
+            return 2
+        return surface_bench(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no "
               "multi-GPU path", file=sys.stderr)
