@@ -191,11 +191,17 @@ struct ProfScope {
         e.iter = h->curIter;
         e.skip = false;
         hipEventRecord(e.a, h->st);
+        // the kernel itself records both events at its dispatch boundaries
+        // (hipExtLaunchKernelGGL): kernel time, without the event packets
+        g_launch_timing = LaunchTiming{e.a, e.b};
         on = true;
     }
     ~ProfScope() {
         if (!on) return;
-        hipEventRecord(h->prof[h->profUsed].b, h->st);
+        if (g_launch_timing.start) {   // not a SPH_LAUNCH kernel: bracket with stream events
+            g_launch_timing = LaunchTiming{};
+            hipEventRecord(h->prof[h->profUsed].b, h->st);
+        }
         ++h->profUsed;
     }
 };

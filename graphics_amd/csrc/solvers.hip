@@ -1094,10 +1094,20 @@ void launch_fill_i32(int* p, int v, int n, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
 }
 
-#define SPH_LAUNCH(kern, ...)                                                              \
-    do {                                                                                   \
-        if (tiles_of(D) > 0)                                                               \
-            hipLaunchKernelGGL(kern, dim3(tiles_of(D)), dim3(256), 0, st, D, ##__VA_ARGS__); \
+thread_local LaunchTiming g_launch_timing;
+
+#define SPH_LAUNCH(kern, ...)                                                                          \
+    do {                                                                                               \
+        if (tiles_of(D) > 0) {                                                                         \
+            const LaunchTiming t_ = g_launch_timing;                                                   \
+            if (t_.start) {                                                                            \
+                g_launch_timing = LaunchTiming{};                                                      \
+                hipExtLaunchKernelGGL(kern, dim3(tiles_of(D)), dim3(256), 0, st, t_.start, t_.stop, 0, \
+                                      D, ##__VA_ARGS__);                                               \
+            } else {                                                                                   \
+                hipLaunchKernelGGL(kern, dim3(tiles_of(D)), dim3(256), 0, st, D, ##__VA_ARGS__);       \
+            }                                                                                          \
+        }                                                                                              \
     } while (0)
 
 void launch_wc_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_wc_pressure); }

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
 
 namespace sphgpu {
@@ -324,6 +325,14 @@ constexpr float DUMMY_FAR = 1.0e15f;   // coordinate of the dummy boundary parti
 
 // ---------------------------------------------------------------------------
 // host-side launchers implemented in the .hip files
+
+// Events of the launch being timed: set by the host's timing scope; the
+// first kernel launched through SPH_LAUNCH while they are set records them
+// at its own dispatch boundaries (kernel time only) and clears them.
+struct LaunchTiming {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchTiming g_launch_timing;
 
 // sort.hip
 struct SortScratch {
