@@ -74,7 +74,7 @@ def kernel_name(raw: str) -> str:
         k = int(m.group(1))
         start = m.end()
         return raw[start:start + k]
-    return raw.replace("sphgpu::(anonymous namespace)::", "").split("(")[0]
+    return raw.replace("sphgpu::", "").replace("(anonymous namespace)::", "").split("(")[0]
 
 
 def summarize_trace(stats_csv: str, trace_csv: str, workload: str, command: str) -> int:
@@ -295,11 +295,12 @@ def main() -> int:
     ap.add_argument("--summarize-trace", nargs=2, metavar=("TRACE_CSV", "COMMAND"),
                     help="write profiles/<workload>_kernel_stats.txt from a rocprofv3 --kernel-trace run")
     args = ap.parse_args()
+    wname = (lambda: "surface_dam_1000000") if args.workload == "surface_dam" else \
+        (lambda: build_scene(args.workload).name)
     if args.summarize_pmc:
-        return summarize_pmc(*args.summarize_pmc[:2], build_scene(args.workload).name, args.summarize_pmc[2])
+        return summarize_pmc(*args.summarize_pmc[:2], wname(), args.summarize_pmc[2])
     if args.summarize_trace:
-        return summarize_trace(None, args.summarize_trace[0], build_scene(args.workload).name,
-                               args.summarize_trace[1])
+        return summarize_trace(None, args.summarize_trace[0], wname(), args.summarize_trace[1])
     if args.workload == "surface_dam":
         if args.gpus != 1 or not sg.device_available():
             print("surface_dam needs exactly one gfx9 device", file=sys.stderr)
