@@ -126,6 +126,7 @@ struct sphgpu_handle {
     unsigned long long graphClock = 0;
     bool capturing = false;
     bool tuned = false;
+    bool listsBuilt = false;   // neighbour lists built at least once
     float tuneMs[4] = {0, 0, 0, 0};    // density pass: map 0, 1; list builder: 8, 12 groups
     uint32_t gatherSeed = 1;
     // profiling
@@ -218,6 +219,7 @@ void init_handle_mlp(std::vector<float>& p) {
 // ---- neighbour search pipeline ------------------------------------------
 int run_ns(sphgpu_handle* h) {
     Dev& D = h->D;
+    h->listsBuilt = true;
     hipStream_t st = h->st;
     int npad = D.npad;
     {
@@ -1158,6 +1160,8 @@ int sphgpu_init(sphgpu_handle* h) {
     ALLOC(D.xk, ncap); ALLOC(D.vs, ncap); ALLOC(D.f4a, ncap); ALLOC(D.f4b, ncap);
     ALLOC(D.rho, ncap); ALLOC(D.alpha, ncap); ALLOC(D.kr, ncap); ALLOC(D.kr0, ncap); ALLOC(D.f1, ncap); ALLOC(D.f2, ncap);
     ALLOC(D.nbF, (size_t)NBF_CAP * ncap); ALLOC(D.nbB, (size_t)NBB_CAP * ncap); ALLOC(D.cnt, ncap);
+    ALLOC(D.nbF16, (size_t)(NBF_CAP / 8) * ncap); ALLOC(D.nbB16, (size_t)(NBB_CAP / 8) * ncap);
+    ALLOC(D.nbTab, (size_t)NB16_TAB * (ncap / TILE));
     // boundary arrays carry one extra dummy particle at index nb (list padding)
     ALLOC(D.xb, nb + 1); ALLOC(D.vb, nb + 1); ALLOC(D.bid, nb + 1); ALLOC(D.bsolid, nb + 1);
     ALLOC(D.bscene, nb + 1); ALLOC(D.bkself, nb + 1); ALLOC(D.bkey, nb + 1);
@@ -1705,6 +1709,28 @@ int sphgpu_get_tuning(const sphgpu_handle* h, int* choice, float* ms) {
     choice[1] = h->D.nbStage;
     if (ms)
         for (int k = 0; k < 4; ++k) ms[k] = h->tuneMs[k];
+    return SPHGPU_OK;
+}
+
+int sphgpu_get_list_form(sphgpu_handle* h, int* packed_tiles, int* tiles) {
+    if (!h || !packed_tiles || !tiles) return SPHGPU_E_ARG;
+    if (!h->initialized || !h->listsBuilt) return fail(h, SPHGPU_E_STATE, "no neighbour lists built yet");
+    const int nt = h->D.npad / TILE;
+    std::vector<int> flag(nt), valid(nt);
+    HIPCHECK(h, hipStreamSynchronize(h->st));
+    if (nt > 0) {
+        HIPCHECK(h, hipMemcpy2D(flag.data(), sizeof(int), h->D.nbTab + NB16_FLAG, sizeof(int) * NB16_TAB, sizeof(int),
+                                nt, hipMemcpyDeviceToHost));
+        HIPCHECK(h, hipMemcpy(valid.data(), h->D.tileValid, sizeof(int) * nt, hipMemcpyDeviceToHost));
+    }
+    int p = 0, t = 0;
+    for (int k = 0; k < nt; ++k)
+        if (valid[k] > 0) {
+            ++t;
+            p += flag[k] != 0;
+        }
+    *packed_tiles = p;
+    *tiles = t;
     return SPHGPU_OK;
 }
 

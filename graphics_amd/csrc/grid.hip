@@ -164,18 +164,29 @@ __device__ __forceinline__ int2 run3(const int2* __restrict__ table, uint32_t k0
 }
 
 // Lists are built in LDS, one column per thread, and written out group by
-// group (four entries, 16 bytes per particle), so each store instruction of a
-// wave writes whole cache lines.  Entries beyond the staged 4 x NB_STAGE_GROUPS
-// are written to global memory directly; the depth (8 or 12 groups: LDS per
-// block against blocks per CU) is chosen by the autotuner.
+// group (16 bytes per particle), so each store instruction of a wave writes
+// whole cache lines.  A tile whose particles span at most NB16_ROWS own rows,
+// and whose row runs each fit NB16_OFF_BITS of offset, gets packed 16-bit
+// lists (staged whole: 8 groups of 8 entries); any other tile gets 32-bit
+// lists, staged 4 x NB_STAGE_GROUPS entries deep with the rest written to
+// global memory directly (8 or 12 groups: LDS per block against blocks per
+// CU, chosen by the autotuner).  Both forms hold the same entries in the
+// same order.
 template <int NB_STAGE_GROUPS>
 __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
+    static_assert(NB_STAGE_GROUPS * 8 >= NBF_CAP && NB_STAGE_GROUPS * 8 >= NBB_CAP, "packed lists are staged whole");
     __shared__ int4 stage[NB_STAGE_GROUPS][256];
+    __shared__ int sBase[2 * 32], sEnd[2 * 32];
+    __shared__ int sRow[256];
+    __shared__ int sWave[4];
+    __shared__ int sOverflow;
     const int tid = threadIdx.x;
-    const int i = tile_of_block(D.tileMap) * 256 + tid;
+    const int tile = tile_of_block(D.tileMap);
+    const int i = tile * 256 + tid;
     int scene = 0;
     const bool act = i < D.npad && slot_valid(D, i, &scene);
     int* st = reinterpret_cast<int*>(&stage[0][0]);
+    uint16_t* st16 = reinterpret_cast<uint16_t*>(&stage[0][0]);
     const int stride = D.ncap;
     int cx = 0, cy = 0, cz = 0;
     float4 pa = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -185,14 +196,66 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
         pa = D.x[i];
         cell_of(G, pa, cx, cy, cz);
     }
+    // ordinal of the particle's own row among the tile's rows: the valid
+    // slots are a prefix of the tile in key order, so one row's particles
+    // are contiguous
+    const int row = act ? cz * G.ny + cy : -1;
+    sRow[tid] = row;
+    if (tid < 64) {
+        sBase[tid] = INT_MAX;
+        sEnd[tid] = 0;
+    }
+    if (tid == 0) sOverflow = 0;
+    __syncthreads();
+    const bool first = act && (tid == 0 || sRow[tid - 1] != row);
+    const unsigned long long bal = __ballot(first);
+    const int lane = tid & 63, wave = tid >> 6;
+    if (lane == 0) sWave[wave] = __popcll(bal);
+    __syncthreads();
+    int ord = __popcll(bal & ((2ull << lane) - 1ull)) - 1;
+    for (int w = 0; w < wave; ++w) ord += sWave[w];
+    bool packed = sWave[0] + sWave[1] + sWave[2] + sWave[3] <= NB16_ROWS;
+    // per slot: the smallest run start and the largest run end of the tile
+    if (packed && act) {
+        for (int r = 0; r < 9; ++r) {
+            const int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
+            if (z < 0 || z >= G.nz || y < 0 || y >= G.ny) continue;
+            const uint32_t k0 = key_of(G, D.rowRank, cx, y, z);
+            const int slot = ord * 9 + r;
+            const int2 rf = run3(D.cellF, k0, cx, G.nx);
+            if (rf.y > rf.x) {
+                atomicMin(&sBase[slot], rf.x);
+                atomicMax(&sEnd[slot], rf.y);
+            }
+            const int2 rb = run3(D.cellB, k0, cx, G.nx);
+            if (rb.y > rb.x) {
+                atomicMin(&sBase[32 + slot], rb.x);
+                atomicMax(&sEnd[32 + slot], rb.y);
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < 64 && sBase[tid] != INT_MAX && sEnd[tid] - sBase[tid] > (1 << NB16_OFF_BITS)) sOverflow = 1;
+    __syncthreads();
+    packed = packed && sOverflow == 0;
+
     auto put = [&](int* list, int k, int j) {
         if (k < NB_STAGE_GROUPS * 4) st[((k >> 2) * 256 + tid) * 4 + (k & 3)] = j;
         else list[nb_slot(k, i, stride)] = j;
+    };
+    auto put16 = [&](int k, int slot, int j) {
+        st16[((k >> 3) * 256 + tid) * 8 + (k & 7)] = uint16_t((slot << NB16_OFF_BITS) + (j - sBase[slot]));
     };
     auto flush = [&](int* list, int count) {
         __syncthreads();
         const int ng = min((count + 3) >> 2, NB_STAGE_GROUPS);
         int4* out = reinterpret_cast<int4*>(list);
+        for (int g = 0; g < ng; ++g) out[(size_t)g * stride + i] = stage[g][tid];
+        __syncthreads();
+    };
+    auto flush16 = [&](int4* out, int count) {
+        __syncthreads();
+        const int ng = (count + 7) >> 3;
         for (int g = 0; g < ng; ++g) out[(size_t)g * stride + i] = stage[g][tid];
         __syncthreads();
     };
@@ -209,7 +272,8 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
                 float l2 = dis_square_exact(pa, D.x[j]);
                 if (l2 < D.h2) {
                     if (stored < NBF_CAP) {
-                        put(D.nbF, stored, j);
+                        if (packed) put16(stored, ord * 9 + r, j);
+                        else put(D.nbF, stored, j);
                         ++stored;
                     } else {
                         ++over;
@@ -221,8 +285,13 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
         }
     }
     const int cf = act ? stored : 0;
-    for (int k = cf; k < ((cf + 3) & ~3); ++k) put(D.nbF, k, i);   // pad: self
-    flush(D.nbF, cf);
+    // pad: the particle itself
+    for (int k = cf; k < ((cf + 3) & ~3); ++k) {
+        if (packed) put16(k, ord * 9 + 4, i);
+        else put(D.nbF, k, i);
+    }
+    if (packed) flush16(D.nbF16, cf);
+    else flush(D.nbF, cf);
     // boundary neighbours (SphBase.cpp:908-930)
     if (act) {
         for (int dz = -1; dz <= 1; ++dz) {
@@ -231,12 +300,17 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
             for (int dy = -1; dy <= 1; ++dy) {
                 int y = cy + dy;
                 if (y < 0 || y >= G.ny) continue;
+                const int slot = 32 + ord * 9 + (dz + 1) * 3 + (dy + 1);
                 int2 rb = run3(D.cellB, key_of(G, D.rowRank, cx, y, z), cx, G.nx);
                 for (int j = rb.x; j < rb.y; ++j) {
                     float l2 = dis_square_exact(pa, D.xb[j]);
                     if (l2 < D.h2) {
-                        if (nbn < NBB_CAP) put(D.nbB, nbn, j);
-                        else ++over;
+                        if (nbn < NBB_CAP) {
+                            if (packed) put16(nbn, slot, j);
+                            else put(D.nbB, nbn, j);
+                        } else {
+                            ++over;
+                        }
                         ++nbn;
                         zero += (l2 == 0.0f);
                     }
@@ -245,8 +319,21 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
         }
     }
     const int cb = act ? min(nbn, NBB_CAP) : 0;
-    for (int k = cb; k < ((cb + 3) & ~3); ++k) put(D.nbB, k, D.nb);   // pad: far dummy
-    flush(D.nbB, cb);
+    // pad: the far dummy particle
+    for (int k = cb; k < ((cb + 3) & ~3); ++k) {
+        if (packed) st16[((k >> 3) * 256 + tid) * 8 + (k & 7)] = uint16_t(NB16_DUMMY << NB16_OFF_BITS);
+        else put(D.nbB, k, D.nb);
+    }
+    if (packed) flush16(D.nbB16, cb);
+    else flush(D.nbB, cb);
+    if (tid < NB16_TAB) {
+        const int slot = tid & 31;
+        int v;
+        if (tid == NB16_FLAG) v = packed ? 1 : 0;
+        else if (tid == 32 + NB16_DUMMY) v = D.nb - (NB16_DUMMY << NB16_OFF_BITS);
+        else v = sBase[tid] != INT_MAX ? sBase[tid] - (slot << NB16_OFF_BITS) : 0;
+        D.nbTab[(size_t)tile * NB16_TAB + tid] = v;
+    }
     // stored fluid entries | fluid neighbours found << 8 | boundary << 16
     if (i < D.npad) D.cnt[i] = cf | (act ? min(nf, 255) : 0) << 8 | (cb << 16);
     if (!act) return;
@@ -506,9 +593,19 @@ __global__ __launch_bounds__(256) void k_neighbour_dump(Dev D, int cap, int* cou
     if (o < 0) return;
     const float4 pa = D.x[i];
     const int cnt = D.cnt[i], nf = cnt & 0xFF, nbn = cnt >> 16;
+    const int* tab = D.nbTab + (size_t)(i / TILE) * NB16_TAB;
+    const bool packed = tab[NB16_FLAG] != 0;
+    const uint16_t* f16 = reinterpret_cast<const uint16_t*>(D.nbF16);
+    const uint16_t* b16 = reinterpret_cast<const uint16_t*>(D.nbB16);
     int c = 0;
     for (int k = 0; k < nf; ++k) {
-        const int j = D.nbF[nb_slot(k, i, D.ncap)];
+        int j;
+        if (packed) {
+            const int e = f16[nb16_slot(k, i, D.ncap)];
+            j = tab[e >> NB16_OFF_BITS] + e;
+        } else {
+            j = D.nbF[nb_slot(k, i, D.ncap)];
+        }
         if (c < cap) {
             size_t q = (size_t)o * cap + c;
             kinds[q] = 0; ids[q] = D.id[j]; dis[q] = sqrtf(dis_square_exact(pa, D.x[j]));
@@ -516,7 +613,13 @@ __global__ __launch_bounds__(256) void k_neighbour_dump(Dev D, int cap, int* cou
         ++c;
     }
     for (int k = 0; k < nbn; ++k) {
-        const int j = D.nbB[nb_slot(k, i, D.ncap)];
+        int j;
+        if (packed) {
+            const int e = b16[nb16_slot(k, i, D.ncap)];
+            j = tab[32 + (e >> NB16_OFF_BITS)] + e;
+        } else {
+            j = D.nbB[nb_slot(k, i, D.ncap)];
+        }
         if (c < cap) {
             size_t q = (size_t)o * cap + c;
             kinds[q] = 1; ids[q] = D.bid[j]; dis[q] = sqrtf(dis_square_exact(pa, D.xb[j]));

@@ -15,6 +15,7 @@ namespace {
 struct Slot {
     int i, tile, scene;
     bool live;
+    bool packed;   // the tile's neighbour lists are in the packed 16-bit form
 };
 
 __device__ __forceinline__ Slot slot_of(const Dev& D, bool gated) {
@@ -24,7 +25,61 @@ __device__ __forceinline__ Slot slot_of(const Dev& D, bool gated) {
     s.scene = D.tileScene[s.tile];
     s.live = (int)threadIdx.x < D.tileValid[s.tile];
     if (gated) s.live = s.live && D.active[s.scene] != 0;
+    s.packed = false;
     return s;
+}
+
+// The tile's list table (fluid slots 0..31, boundary 32..63), staged in LDS
+// by the kernels that read neighbour lists.
+__shared__ int g_nbtab[NB16_TAB];
+
+// slot_of for kernels that read neighbour lists: every thread of the block
+// calls it before any returns (it stages the tile's table in LDS).
+__device__ __forceinline__ Slot slot_nb(const Dev& D, bool gated) {
+    Slot s = slot_of(D, gated);
+    if (threadIdx.x < NB16_TAB) g_nbtab[threadIdx.x] = D.nbTab[(size_t)s.tile * NB16_TAB + threadIdx.x];
+    __syncthreads();
+    s.packed = __builtin_amdgcn_readfirstlane(g_nbtab[NB16_FLAG]) != 0;
+    return s;
+}
+
+// Calls f(j) for the entries of a packed list, eight per 16-byte group (the
+// second four only where the list reaches them), with the next group loaded
+// before the current one is decoded.
+template <int TOFF, class F>
+__device__ __forceinline__ void each_in_list16(const int4* __restrict__ list, int ncap, int i, int n, F&& f) {
+    const int4* __restrict__ col = list + i;
+    const int ng = (n + 7) >> 3;
+    if (ng <= 0) return;
+    int4 cur = col[0];
+    for (int g = 0; g < ng; ++g) {
+        int4 nxt = cur;
+        if (g + 1 < ng) nxt = col[(size_t)(g + 1) * ncap];
+        auto dec = [&](unsigned e) { f(g_nbtab[TOFF + (e >> NB16_OFF_BITS)] + (int)e); };
+        dec((unsigned)cur.x & 0xffffu);
+        dec((unsigned)cur.x >> 16);
+        dec((unsigned)cur.y & 0xffffu);
+        dec((unsigned)cur.y >> 16);
+        if (8 * g + 4 < n) {
+            dec((unsigned)cur.z & 0xffffu);
+            dec((unsigned)cur.z >> 16);
+            dec((unsigned)cur.w & 0xffffu);
+            dec((unsigned)cur.w >> 16);
+        }
+        cur = nxt;
+    }
+}
+
+template <class F>
+__device__ __forceinline__ void each_fluid(const Dev& D, const Slot& s, int nf, F&& f) {
+    if (s.packed) each_in_list16<0>(D.nbF16, D.ncap, s.i, nf, f);
+    else each_in_list(D.nbF, D.ncap, s.i, nf, f);
+}
+
+template <class F>
+__device__ __forceinline__ void each_boundary(const Dev& D, const Slot& s, int nb, F&& f) {
+    if (s.packed) each_in_list16<32>(D.nbB16, D.ncap, s.i, nb, f);
+    else each_in_list(D.nbB, D.ncap, s.i, nb, f);
 }
 
 __device__ __forceinline__ int phase_of(float4 x) { return (int)x.w; }
@@ -176,16 +231,17 @@ __global__ __launch_bounds__(256) void k_stats_reduce(Dev D, const double* part,
 // density m W(0) + sum_f m W + sum_b rho0 V_b W, Kahan-compensated
 // (WcSph.cpp:121-138, DFSPH.cpp:132-169, IISph_M.cpp:101-112); coincident
 // pairs contribute W(0), the padding (self) entries nothing.
-__device__ __forceinline__ float density_sum(const Dev& D, int i, float4 pa, float m, float rho0, int c) {
+__device__ __forceinline__ float density_sum(const Dev& D, const Slot& s, float4 pa, float m, float rho0, int c) {
+    const int i = s.i;
     Kahan rho;
     rho.init(m * D.W0);
-    each_fluid(D, i, nf_of(c), [&](int j) {
+    each_fluid(D, s, nf_of(c), [&](int j) {
         float4 pb = D.x[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float w = kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
         rho.add(j != i ? m * w : 0.0f);
     });
-    each_boundary(D, i, nb_of(c), [&](int j) {
+    each_boundary(D, s, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         rho.add(rho0 * pb.w * kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D));
@@ -195,17 +251,17 @@ __device__ __forceinline__ float density_sum(const Dev& D, int i, float4 pa, flo
 
 // sum_f m gradW.(v*_a - v*_b) + sum_b rho0 V_b gradW.(v*_a - v_b): the
 // predicted density change rate (DFSPH.cpp:589-617, 795-825)
-__device__ __forceinline__ float density_change(const Dev& D, int i, float4 pa, float4 va, float m,
+__device__ __forceinline__ float density_change(const Dev& D, const Slot& s, float4 pa, float4 va, float m,
                                                 float rho0, int c) {
     float sf = 0.0f, sb = 0.0f;
-    each_fluid(D, i, nf_of(c), [&](int j) {
+    each_fluid(D, s, nf_of(c), [&](int j) {
         float4 pb = D.x[j];
         float4 vb = D.vs[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
         sf += g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
     });
-    each_boundary(D, i, nb_of(c), [&](int j) {
+    each_boundary(D, s, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
         float4 vb = D.vb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -219,13 +275,13 @@ __device__ __forceinline__ float density_change(const Dev& D, int i, float4 pa, 
 // WcSph::wc_computePressure (WcSph.cpp:99-153): rho (Kahan), Tait p >= 0.
 // Publishes the force pass's gather records: xk = (x, p/rho^2), v.w = rho.
 __global__ __launch_bounds__(256) void k_wc_pressure(Dev D) {
-    Slot s = slot_of(D, false);
+    Slot s = slot_nb(D, false);
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.x[i];
     int ph = phase_of(pa);
     float m = D.mass[ph], rho0 = D.rho0[ph];
-    float density = density_sum(D, i, pa, m, rho0, D.cnt[i]);
+    float density = density_sum(D, s, pa, m, rho0, D.cnt[i]);
     float p = D.tait_B[ph] * (powf(density / rho0, D.gamma) - 1.0f);
     p = p < 0.0f ? 0.0f : p;
     D.rho[i] = density;
@@ -238,7 +294,7 @@ __global__ __launch_bounds__(256) void k_wc_pressure(Dev D) {
 // Euler-Cromer update and domain removal of SphBase::updateFluids
 // (SphBase.cpp:377-402): writes x2/v2, the caller swaps buffers.
 __global__ __launch_bounds__(256) void k_wc_force(Dev D) {
-    Slot s = slot_of(D, false);
+    Slot s = slot_nb(D, false);
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.x[i], va = D.v[i];
@@ -250,7 +306,7 @@ __global__ __launch_bounds__(256) void k_wc_force(Dev D) {
     const bool multi = D.nphases > 1;
     float ax = D.gx, ay = D.gy, az = D.gz;
     int c = D.cnt[i];
-    each_fluid(D, i, nf_of(c), [&](int j) {
+    each_fluid(D, s, nf_of(c), [&](int j) {
         float4 pb = D.xk[j];           // (x, p/rho^2)
         float4 vb = D.v[j];            // (v, rho)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -273,7 +329,7 @@ __global__ __launch_bounds__(256) void k_wc_force(Dev D) {
         }
         ax += dx * acce; ay += dy * acce; az += dz * acce;
     });
-    each_boundary(D, i, nb_of(c), [&](int j) {
+    each_boundary(D, s, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
         float4 vb = D.vb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -301,7 +357,7 @@ __global__ __launch_bounds__(256) void k_wc_force(Dev D) {
 // ============================================================ DFSPH ======
 // DFSPH::DF_constantDensitySolverFirst (DFSPH.cpp:84-193): rho (Kahan) and alpha.
 __global__ __launch_bounds__(256) void k_df_density_alpha(Dev D) {
-    Slot s = slot_of(D, false);
+    Slot s = slot_nb(D, false);
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.x[i];
@@ -311,7 +367,7 @@ __global__ __launch_bounds__(256) void k_df_density_alpha(Dev D) {
     rho.init(m * D.W0);
     float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2 = 0.f;
     int c = D.cnt[i];
-    each_fluid(D, i, nf_of(c), [&](int j) {
+    each_fluid(D, s, nf_of(c), [&](int j) {
         float4 pb = D.x[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
@@ -321,7 +377,7 @@ __global__ __launch_bounds__(256) void k_df_density_alpha(Dev D) {
         float gs = gm * r;                     // |dW/dr| m (zero at r = 0)
         a2 += gs * gs;
     });
-    each_boundary(D, i, nb_of(c), [&](int j) {
+    each_boundary(D, s, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
@@ -343,7 +399,7 @@ __global__ __launch_bounds__(256) void k_df_density_alpha(Dev D) {
 // kappa: it only pushes (rho* >= rho0, DFSPH.cpp:620-623), so a warm guess
 // that overshoots would never be taken back.
 __global__ __launch_bounds__(256) void k_df_aadv(Dev D) {
-    Slot s = slot_of(D, false);
+    Slot s = slot_nb(D, false);
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.x[i], va = D.v[i];
@@ -355,7 +411,7 @@ __global__ __launch_bounds__(256) void k_df_aadv(Dev D) {
     const bool multi = D.nphases > 1;
     float ax = D.gx, ay = D.gy, az = D.gz;
     int c = D.cnt[i];
-    each_fluid(D, i, nf_of(c), [&](int j) {
+    each_fluid(D, s, nf_of(c), [&](int j) {
         float4 pb = D.x[j];
         float4 vb = D.v[j];            // (v, rho)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -374,7 +430,7 @@ __global__ __launch_bounds__(256) void k_df_aadv(Dev D) {
         f = pro < 0.0f ? f : 0.0f;
         ax += dx * f; ay += dy * f; az += dz * f;
     });
-    each_boundary(D, i, nb_of(c), [&](int j) {
+    each_boundary(D, s, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
         float4 vb = D.vb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -411,19 +467,19 @@ __global__ __launch_bounds__(256) void k_df_aadv(Dev D) {
 //   7 dt / dt_max
 // The kernel always stores Drho/Dt in f2 for the gate of k_df_warm_apply.
 __global__ __launch_bounds__(256) void k_df_warm_features(Dev D, float* __restrict__ feat) {
-    Slot s = slot_of(D, false);
+    Slot s = slot_nb(D, false);
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.x[i], va = D.vs[i];
     int ph = phase_of(pa);
     float rho0 = D.rho0[ph];
     int c = D.cnt[i];
-    float d = density_change(D, i, pa, va, D.mass[ph], rho0, c);
+    float d = density_change(D, s, pa, va, D.mass[ph], rho0, c);
     D.f2[i] = d;
     if (!feat) return;
     const float sp = D.spacing;
     float bmin2 = 4.0f * sp * sp;
-    each_boundary(D, i, nb_of(c), [&](int j) {
+    each_boundary(D, s, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         bmin2 = fminf(bmin2, dx * dx + dy * dy + dz * dz);
@@ -519,14 +575,14 @@ __global__ __launch_bounds__(256) void k_mlp_gather(Dev D, const float* __restri
 // kappa = (rho*-rho0)/dt^2 alpha -> xk.w = kappa/rho; partial sum of rho*/rho0.
 __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
     __shared__ double lds[4];
-    Slot s = slot_of(D, true);
+    Slot s = slot_nb(D, true);
     double contrib = 0.0;
     if (s.live) {
         const int i = s.i;
         float4 pa = D.x[i], va = D.vs[i];
         int ph = phase_of(pa);
         float m = D.mass[ph], rho0 = D.rho0[ph];
-        float dd = density_change(D, i, pa, va, m, rho0, D.cnt[i]);
+        float dd = density_change(D, s, pa, va, m, rho0, D.cnt[i]);
         float dt = D.dt[s.scene];
         float rhoa = D.rho[i];
         float rs = rhoa + dt * dd;
@@ -549,14 +605,14 @@ __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
 // oscillates).
 __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected) {
     __shared__ double lds[4];
-    Slot s = slot_of(D, true);
+    Slot s = slot_nb(D, true);
     double contrib = 0.0;
     if (s.live) {
         const int i = s.i;
         float4 pa = D.x[i], va = D.vs[i];
         int ph = phase_of(pa);
         float m = D.mass[ph], rho0 = D.rho0[ph];
-        float dd = density_change(D, i, pa, va, m, rho0, D.cnt[i]);
+        float dd = density_change(D, s, pa, va, m, rho0, D.cnt[i]);
         float dt = D.dt[s.scene];
         float rhoa = D.rho[i];
         if (!projected) {
@@ -583,7 +639,7 @@ __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected
 // pass B of both loops (DFSPH.cpp:647-693, 840-883):
 // v* -= dt (m sum (kr_a + kr_b) gradW + rho0 sum V_b kr_a gradW), kr = xk.w
 __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
-    Slot s = slot_of(D, true);
+    Slot s = slot_nb(D, true);
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.xk[i];
@@ -592,13 +648,13 @@ __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
     float ka = pa.w;
     float fx = 0.f, fy = 0.f, fz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
     int c = D.cnt[i];
-    each_fluid(D, i, nf_of(c), [&](int j) {
+    each_fluid(D, s, nf_of(c), [&](int j) {
         float4 pb = D.xk[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * (ka + pb.w);
         fx += dx * g; fy += dy * g; fz += dz * g;
     });
-    each_boundary(D, i, nb_of(c), [&](int j) {
+    each_boundary(D, s, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
@@ -634,19 +690,19 @@ __global__ __launch_bounds__(256) void k_df_integrate(Dev D) {
 // density (IISph_M.cpp:101-112), computed before a_adv so that a_adv reads
 // this step's densities; published as v.w for the a_adv gathers.
 __global__ __launch_bounds__(256) void k_ii_density(Dev D) {
-    Slot s = slot_of(D, false);
+    Slot s = slot_nb(D, false);
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.x[i];
     int ph = phase_of(pa);
-    float density = density_sum(D, i, pa, D.mass[ph], D.rho0[ph], D.cnt[i]);
+    float density = density_sum(D, s, pa, D.mass[ph], D.rho0[ph], D.cnt[i]);
     D.rho[i] = density;
     D.v[i].w = density;
 }
 
 // a_adv (IISph_M.cpp:364-476), d_ii (114-130) and v_adv (135)
 __global__ __launch_bounds__(256) void k_ii_aadv_dii(Dev D) {
-    Slot s = slot_of(D, false);
+    Slot s = slot_nb(D, false);
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.x[i], va = D.v[i];
@@ -658,7 +714,7 @@ __global__ __launch_bounds__(256) void k_ii_aadv_dii(Dev D) {
     float ax = D.gx, ay = D.gy, az = D.gz;
     float dfx = 0.f, dfy = 0.f, dfz = 0.f, dbx = 0.f, dby = 0.f, dbz = 0.f;
     int c = D.cnt[i];
-    each_fluid(D, i, nf_of(c), [&](int j) {
+    each_fluid(D, s, nf_of(c), [&](int j) {
         float4 pb = D.x[j];
         float4 vb = D.v[j];            // (v, rho)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -677,7 +733,7 @@ __global__ __launch_bounds__(256) void k_ii_aadv_dii(Dev D) {
         ax += dx * f; ay += dy * f; az += dz * f;
         dfx += dx * g; dfy += dy * g; dfz += dz * g;
     });
-    each_boundary(D, i, nb_of(c), [&](int j) {
+    each_boundary(D, s, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
         float4 vb = D.vb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -698,7 +754,7 @@ __global__ __launch_bounds__(256) void k_ii_aadv_dii(Dev D) {
 // a_ii, rho_adv, p_l = 0.5 p(t-dt) (IISph_M.cpp:243-278); publishes
 // xk = (x, p_l/rho^2) for the sum_j d_ij p_j gathers.
 __global__ __launch_bounds__(256) void k_ii_aii(Dev D) {
-    Slot s = slot_of(D, false);
+    Slot s = slot_nb(D, false);
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.x[i], va = D.vs[i], dii = D.f4a[i];
@@ -709,7 +765,7 @@ __global__ __launch_bounds__(256) void k_ii_aii(Dev D) {
     float cji = m * dt * dt / (rhoa * rhoa);
     float t1f = 0.f, t1b = 0.f, t2f = 0.f, t2b = 0.f;
     int c = D.cnt[i];
-    each_fluid(D, i, nf_of(c), [&](int j) {
+    each_fluid(D, s, nf_of(c), [&](int j) {
         float4 pb = D.x[j];
         float4 vb = D.vs[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -718,7 +774,7 @@ __global__ __launch_bounds__(256) void k_ii_aii(Dev D) {
         t1f += gx * (va.x - vb.x) + gy * (va.y - vb.y) + gz * (va.z - vb.z);
         t2f += (dii.x - cji * gx) * gx + (dii.y - cji * gy) * gy + (dii.z - cji * gz) * gz;
     });
-    each_boundary(D, i, nb_of(c), [&](int j) {
+    each_boundary(D, s, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
         float4 vb = D.vb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -740,14 +796,14 @@ __global__ __launch_bounds__(256) void k_ii_aii(Dev D) {
 // stores u = d_ii p_l + sum_j d_ij p_j, the per-particle part of the
 // Jacobi update that neighbours read.
 __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
-    Slot s = slot_of(D, true);
+    Slot s = slot_nb(D, true);
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.x[i];
     int ph = phase_of(pa);
     float m = D.mass[ph];
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    each_fluid(D, i, nf_of(D.cnt[i]), [&](int j) {
+    each_fluid(D, s, nf_of(D.cnt[i]), [&](int j) {
         float4 pb = D.xk[j];           // (x, p_l/rho^2)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
@@ -764,7 +820,7 @@ __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
 // relative density error of the particles that are not clamped.
 __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
     __shared__ double lds[4];
-    Slot s = slot_of(D, true);
+    Slot s = slot_nb(D, true);
     double contrib = 0.0;
     if (s.live) {
         const int i = s.i;
@@ -778,7 +834,7 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
         float sax = ua.x - dia.x * pla, say = ua.y - dia.y * pla, saz = ua.z - dia.z * pla;
         float ff = 0.f, fb = 0.f;
         int c = D.cnt[i];
-        each_fluid(D, i, nf_of(c), [&](int j) {
+        each_fluid(D, s, nf_of(c), [&](int j) {
             float4 pb = D.x[j];
             float4 ub = D.f4b[j];
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -790,7 +846,7 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
             float tz = saz - ub.z + cji * gz * pla;
             ff += tx * gx + ty * gy + tz * gz;
         });
-        each_boundary(D, i, nb_of(c), [&](int j) {
+        each_boundary(D, s, nb_of(c), [&](int j) {
             float4 pb = D.xb[j];
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
             float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
@@ -822,7 +878,7 @@ __global__ __launch_bounds__(256) void k_ii_publish(Dev D) {
 // IISph_M::ii_computeForcePressure (IISph_M.cpp:478-684) fused with the
 // update v = v_adv + dt a_p, x += dt v and domain removal.
 __global__ __launch_bounds__(256) void k_ii_force(Dev D) {
-    Slot s = slot_of(D, false);
+    Slot s = slot_nb(D, false);
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.x[i];
@@ -834,7 +890,7 @@ __global__ __launch_bounds__(256) void k_ii_force(Dev D) {
     const bool multi = D.nphases > 1;
     float ax = 0.f, ay = 0.f, az = 0.f;
     int c = D.cnt[i];
-    each_fluid(D, i, nf_of(c), [&](int j) {
+    each_fluid(D, s, nf_of(c), [&](int j) {
         float4 pb = D.xk[j];           // (x, p/rho^2)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
@@ -855,7 +911,7 @@ __global__ __launch_bounds__(256) void k_ii_force(Dev D) {
         ax += dx * acce; ay += dy * acce; az += dz * acce;
     });
     if (pra > 0.0f) {
-        each_boundary(D, i, nb_of(c), [&](int j) {
+        each_boundary(D, s, nb_of(c), [&](int j) {
             float4 pb = D.xb[j];
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
             float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
@@ -898,7 +954,7 @@ __global__ __launch_bounds__(256) void k_pci_rest_density(Dev D) {
 // pci_forceExceptPressure (PciSph.cpp:235-281, viscosity helpers 172-232):
 // a = g + viscosity, p = 0; first prediction x* = x + (v + a dt) dt
 __global__ __launch_bounds__(256) void k_pci_aext(Dev D) {
-    Slot s = slot_of(D, false);
+    Slot s = slot_nb(D, false);
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.x[i], va = D.v[i];
@@ -909,7 +965,7 @@ __global__ __launch_bounds__(256) void k_pci_aext(Dev D) {
     const bool multi = D.nphases > 1;
     float ax = D.gx, ay = D.gy, az = D.gz;
     int c = D.cnt[i];
-    each_fluid(D, i, nf_of(c), [&](int j) {
+    each_fluid(D, s, nf_of(c), [&](int j) {
         float4 pb = D.x[j];
         float4 vb = D.v[j];            // (v, rho)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -927,7 +983,7 @@ __global__ __launch_bounds__(256) void k_pci_aext(Dev D) {
         f = pro < 0.0f ? f : 0.0f;
         ax += dx * f; ay += dy * f; az += dz * f;
     });
-    each_boundary(D, i, nb_of(c), [&](int j) {
+    each_boundary(D, s, nb_of(c), [&](int j) {
         float4 pb = D.xb[j];
         float4 vb = D.vb[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -954,7 +1010,7 @@ __global__ __launch_bounds__(256) void k_pci_aext(Dev D) {
 __global__ __launch_bounds__(256) void k_pci_density(Dev D) {
     __shared__ double lds[4];
     __shared__ float ldsf[4];
-    Slot s = slot_of(D, true);
+    Slot s = slot_nb(D, true);
     double contrib = 0.0;
     float emax = 0.0f;
     if (s.live) {
@@ -965,13 +1021,13 @@ __global__ __launch_bounds__(256) void k_pci_density(Dev D) {
         Kahan rho;
         rho.init(m * D.W0);
         int c = D.cnt[i];
-        each_fluid(D, i, nf_of(c), [&](int j) {
+        each_fluid(D, s, nf_of(c), [&](int j) {
             float4 pb = D.xk[j];
             float dx = xa.x - pb.x, dy = xa.y - pb.y, dz = xa.z - pb.z;
             float w = kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
             rho.add(j != i ? m * w : 0.0f);
         });
-        each_boundary(D, i, nb_of(c), [&](int j) {
+        each_boundary(D, s, nb_of(c), [&](int j) {
             float4 pb = D.xb[j];
             float dx = xa.x - pb.x, dy = xa.y - pb.y, dz = xa.z - pb.z;
             rho.add(rho0 * pb.w * kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D));
@@ -998,7 +1054,7 @@ __global__ __launch_bounds__(256) void k_pci_density(Dev D) {
 // 2 p_a/rho_a^2 when p_a > 0.  Then the next prediction x* = x + (v + (a +
 // a_p) dt) dt.
 __global__ __launch_bounds__(256) void k_pci_pressure(Dev D) {
-    Slot s = slot_of(D, true);
+    Slot s = slot_nb(D, true);
     if (!s.live) return;
     const int i = s.i;
     const float4 pa = D.vs[i];
@@ -1009,7 +1065,7 @@ __global__ __launch_bounds__(256) void k_pci_pressure(Dev D) {
     const bool multi = D.nphases > 1;
     float ax = 0.f, ay = 0.f, az = 0.f;
     int c = D.cnt[i];
-    each_fluid(D, i, nf_of(c), [&](int j) {
+    each_fluid(D, s, nf_of(c), [&](int j) {
         float4 pb = D.vs[j];
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
@@ -1024,7 +1080,7 @@ __global__ __launch_bounds__(256) void k_pci_pressure(Dev D) {
         ax += dx * acce; ay += dy * acce; az += dz * acce;
     });
     if (qa > 0.0f) {
-        each_boundary(D, i, nb_of(c), [&](int j) {
+        each_boundary(D, s, nb_of(c), [&](int j) {
             float4 pb = D.xb[j];
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
             float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);

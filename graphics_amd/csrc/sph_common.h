@@ -108,7 +108,14 @@ struct Dev {
     float* f1;         // WCSPH pressure; IISPH p_l (next)
     float* f2;         // per-particle scratch (divergence deviation, errors)
 
-    // ---- neighbour lists (ELL, column k of particle i at k*ncap + i)
+    // ---- neighbour lists.  Packed form: 16-bit entries, eight per int4,
+    // group g of particle i at g*ncap + i, decoded through the tile's base
+    // table nbTab[64*tile ..] (fluid slots 0..31, boundary slots 32..63).
+    // Tiles whose runs do not fit the packed form keep 32-bit entries in
+    // nbF/nbB (groups of four, same layout); nbTab[64*tile + 31] says which.
+    int4* nbF16;
+    int4* nbB16;
+    int* nbTab;
     int* nbF;
     int* nbB;
     int* cnt;          // stored fluid entries | fluid neighbours found << 8 | boundary << 16
@@ -291,6 +298,23 @@ __device__ __forceinline__ size_t nb_slot(int k, int i, int ncap) {
     return ((size_t)(k >> 2) * ncap + i) * 4 + (k & 3);
 }
 
+// Packed lists.  The neighbours a particle can have lie in nine row runs
+// (cells cx-1..cx+1 of the rows y+dy, z+dz).  A tile spans at most
+// NB16_ROWS distinct own rows; slot = own-row ordinal * 9 + row, and an entry
+// is (slot << NB16_OFF_BITS) + (j - base[slot]) with base[slot] the smallest
+// run start over the tile's particles of that slot.  The table holds
+// base[slot] - (slot << NB16_OFF_BITS), so j = table[e >> NB16_OFF_BITS] + e.
+// Boundary padding uses slot NB16_DUMMY, whose base is the dummy particle.
+constexpr int NB16_OFF_BITS = 11;
+constexpr int NB16_ROWS = 3;
+constexpr int NB16_DUMMY = 31;
+constexpr int NB16_FLAG = 31;          // fluid table slot 31: 1 = tile packed
+constexpr int NB16_TAB = 64;           // ints of table per tile
+
+__device__ __forceinline__ size_t nb16_slot(int k, int i, int ncap) {
+    return ((size_t)(k >> 3) * ncap + i) * 8 + (k & 7);
+}
+
 // Calls f(j) for the neighbours of i, four at a time (lists are padded to a
 // multiple of four), with the next group's indices loaded before the
 // current group is processed.
@@ -309,16 +333,6 @@ __device__ __forceinline__ void each_in_list(const int* __restrict__ list, int n
         f(cur.w);
         cur = nxt;
     }
-}
-
-template <class F>
-__device__ __forceinline__ void each_fluid(const Dev& D, int i, int nf, F&& f) {
-    each_in_list(D.nbF, D.ncap, i, nf, f);
-}
-
-template <class F>
-__device__ __forceinline__ void each_boundary(const Dev& D, int i, int nb, F&& f) {
-    each_in_list(D.nbB, D.ncap, i, nb, f);
 }
 
 constexpr float DUMMY_FAR = 1.0e15f;   // coordinate of the dummy boundary particle

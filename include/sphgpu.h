@@ -318,6 +318,11 @@ int sphgpu_copy_probe(long long bytes, double* ms);
  * ms (optional, 4 floats) = best times of the candidates (mapping 0, 1;
  * depth 8, 12).  SPHGPU_E_STATE before the first step. */
 int sphgpu_get_tuning(const sphgpu_handle* h, int* choice, float* ms);
+/* Form of the neighbour lists of the last step: *tiles = tiles holding
+ * particles, *packed_tiles = those whose lists use 16-bit entries (the rest
+ * use 32-bit entries; both hold the same neighbours in the same order).
+ * Diagnostic; SPHGPU_E_STATE before lists were first built. */
+int sphgpu_get_list_form(sphgpu_handle* h, int* packed_tiles, int* tiles);
 int sphgpu_kernel_timing(sphgpu_handle* h, long long* launches, double* total_ms,
                          double* total_alg_bytes);
 
