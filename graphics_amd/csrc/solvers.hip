@@ -82,6 +82,108 @@ __device__ __forceinline__ void each_boundary(const Dev& D, const Slot& s, int n
     else each_in_list(D.nbB, D.ncap, s.i, nb, f);
 }
 
+// Split form for the passes that run every solver iteration: load(j) issues
+// the gather(s) of neighbour j and returns the record(s), use(j, r) consumes
+// them.  A batch of neighbours is gathered first, then the next list group is
+// requested, then the batch is used: a wave waits for its memory returns in
+// issue order, so a list load issued ahead of the gathers would hold up every
+// batch.  W = 8: one batch of eight per packed group (single-record passes);
+// W = 4: two batches of four (two-record passes: eight gathers in flight
+// either way).  Entries past the padded count still gather (from the padding
+// index) but are not used, so every pass sums exactly the listed entries.
+template <int W, int TOFF, class LD, class US>
+__device__ __forceinline__ void each_in_list16_lu(const int4* __restrict__ list, int ncap, int i, int n, int padj,
+                                                  LD&& load, US&& use) {
+    static_assert(W == 4 || W == 8, "batch of four or eight");
+    const int4* __restrict__ col = list + i;
+    const int ng = (n + 7) >> 3;
+    if (ng <= 0) return;
+    const int n4 = (n + 3) & ~3;
+    int4 cur = col[0];
+    for (int g = 0; g < ng; ++g) {
+        const unsigned w[4] = {(unsigned)cur.x, (unsigned)cur.y, (unsigned)cur.z, (unsigned)cur.w};
+        const bool second = 8 * g + 4 < n4;   // entries 8g+4..8g+7 are listed
+        int j[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const unsigned e = (w[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+            const int jj = g_nbtab[TOFF + (e >> NB16_OFF_BITS)] + (int)e;
+            j[k] = (k < 4 || second) ? jj : padj;
+        }
+        int4 nxt = cur;
+        if (W == 8) {
+            decltype(load(0)) r[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) r[k] = load(j[k]);
+            __builtin_amdgcn_sched_barrier(0);
+            if (g + 1 < ng) nxt = col[(size_t)(g + 1) * ncap];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) use(j[k], r[k]);
+            if (second) {
+#pragma unroll
+                for (int k = 4; k < 8; ++k) use(j[k], r[k]);
+            }
+        } else {
+            decltype(load(0)) r[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r[k] = load(j[k]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) use(j[k], r[k]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r[k] = load(j[4 + k]);
+            __builtin_amdgcn_sched_barrier(0);
+            if (g + 1 < ng) nxt = col[(size_t)(g + 1) * ncap];
+            __builtin_amdgcn_sched_barrier(0);
+            if (second) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) use(j[4 + k], r[k]);
+            }
+        }
+        cur = nxt;
+    }
+}
+
+// the same for 32-bit lists (groups of four, all listed or padding)
+template <class LD, class US>
+__device__ __forceinline__ void each_in_list_lu(const int* __restrict__ list, int ncap, int i, int n, LD&& load,
+                                                US&& use) {
+    const int4* __restrict__ col = reinterpret_cast<const int4*>(list) + i;
+    const int ng = (n + 3) >> 2;
+    if (ng <= 0) return;
+    int4 cur = col[0];
+    for (int g = 0; g < ng; ++g) {
+        const int j[4] = {cur.x, cur.y, cur.z, cur.w};
+        decltype(load(0)) r[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[k] = load(j[k]);
+        int4 nxt = cur;
+        __builtin_amdgcn_sched_barrier(0);
+        if (g + 1 < ng) nxt = col[(size_t)(g + 1) * ncap];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) use(j[k], r[k]);
+        cur = nxt;
+    }
+}
+
+template <int W, class LD, class US>
+__device__ __forceinline__ void each_fluid_lu(const Dev& D, const Slot& s, int nf, LD&& load, US&& use) {
+    if (s.packed) each_in_list16_lu<W, 0>(D.nbF16, D.ncap, s.i, nf, s.i, load, use);
+    else each_in_list_lu(D.nbF, D.ncap, s.i, nf, load, use);
+}
+
+template <int W, class LD, class US>
+__device__ __forceinline__ void each_boundary_lu(const Dev& D, const Slot& s, int nb, LD&& load, US&& use) {
+    if (s.packed) each_in_list16_lu<W, 32>(D.nbB16, D.ncap, s.i, nb, D.nb, load, use);
+    else each_in_list_lu(D.nbB, D.ncap, s.i, nb, load, use);
+}
+
+// two gather records of one neighbour
+struct Rec2 {
+    float4 a, b;
+};
+
 __device__ __forceinline__ int phase_of(float4 x) { return (int)x.w; }
 
 __device__ __forceinline__ void write_partial(const Dev& D, int tile, int q, double v, double* lds) {
@@ -235,14 +337,12 @@ __device__ __forceinline__ float density_sum(const Dev& D, const Slot& s, float4
     const int i = s.i;
     Kahan rho;
     rho.init(m * D.W0);
-    each_fluid(D, s, nf_of(c), [&](int j) {
-        float4 pb = D.x[j];
+    each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.x[j]; }, [&](int j, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float w = kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
         rho.add(j != i ? m * w : 0.0f);
     });
-    each_boundary(D, s, nb_of(c), [&](int j) {
-        float4 pb = D.xb[j];
+    each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         rho.add(rho0 * pb.w * kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D));
     });
@@ -254,16 +354,14 @@ __device__ __forceinline__ float density_sum(const Dev& D, const Slot& s, float4
 __device__ __forceinline__ float density_change(const Dev& D, const Slot& s, float4 pa, float4 va, float m,
                                                 float rho0, int c) {
     float sf = 0.0f, sb = 0.0f;
-    each_fluid(D, s, nf_of(c), [&](int j) {
-        float4 pb = D.x[j];
-        float4 vb = D.vs[j];
+    each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.vs[j]}; }, [&](int j, Rec2 r) {
+        const float4 pb = r.a, vb = r.b;
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
         sf += g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
     });
-    each_boundary(D, s, nb_of(c), [&](int j) {
-        float4 pb = D.xb[j];
-        float4 vb = D.vb[j];
+    each_boundary_lu<4>(D, s, nb_of(c), [&](int j) { return Rec2{D.xb[j], D.vb[j]}; }, [&](int j, Rec2 r) {
+        const float4 pb = r.a, vb = r.b;
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
         sb += pb.w * g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
@@ -648,14 +746,12 @@ __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
     float ka = pa.w;
     float fx = 0.f, fy = 0.f, fz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
     int c = D.cnt[i];
-    each_fluid(D, s, nf_of(c), [&](int j) {
-        float4 pb = D.xk[j];
+    each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, [&](int j, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * (ka + pb.w);
         fx += dx * g; fy += dy * g; fz += dz * g;
     });
-    each_boundary(D, s, nb_of(c), [&](int j) {
-        float4 pb = D.xb[j];
+    each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
         bx += dx * g; by += dy * g; bz += dz * g;
@@ -803,8 +899,7 @@ __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
     int ph = phase_of(pa);
     float m = D.mass[ph];
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    each_fluid(D, s, nf_of(D.cnt[i]), [&](int j) {
-        float4 pb = D.xk[j];           // (x, p_l/rho^2)
+    each_fluid_lu<8>(D, s, nf_of(D.cnt[i]), [&](int j) { return D.xk[j]; }, [&](int j, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
         sx += dx * g; sy += dy * g; sz += dz * g;
@@ -834,9 +929,8 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
         float sax = ua.x - dia.x * pla, say = ua.y - dia.y * pla, saz = ua.z - dia.z * pla;
         float ff = 0.f, fb = 0.f;
         int c = D.cnt[i];
-        each_fluid(D, s, nf_of(c), [&](int j) {
-            float4 pb = D.x[j];
-            float4 ub = D.f4b[j];
+        each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.f4b[j]}; }, [&](int j, Rec2 r) {
+            const float4 pb = r.a, ub = r.b;
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
             float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
             float gx = dx * g, gy = dy * g, gz = dz * g;
@@ -846,8 +940,7 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
             float tz = saz - ub.z + cji * gz * pla;
             ff += tx * gx + ty * gy + tz * gz;
         });
-        each_boundary(D, s, nb_of(c), [&](int j) {
-            float4 pb = D.xb[j];
+        each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
             float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
             fb += (sax * dx + say * dy + saz * dz) * g;
@@ -1021,14 +1114,12 @@ __global__ __launch_bounds__(256) void k_pci_density(Dev D) {
         Kahan rho;
         rho.init(m * D.W0);
         int c = D.cnt[i];
-        each_fluid(D, s, nf_of(c), [&](int j) {
-            float4 pb = D.xk[j];
+        each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, [&](int j, float4 pb) {
             float dx = xa.x - pb.x, dy = xa.y - pb.y, dz = xa.z - pb.z;
             float w = kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
             rho.add(j != i ? m * w : 0.0f);
         });
-        each_boundary(D, s, nb_of(c), [&](int j) {
-            float4 pb = D.xb[j];
+        each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
             float dx = xa.x - pb.x, dy = xa.y - pb.y, dz = xa.z - pb.z;
             rho.add(rho0 * pb.w * kernel_W(fast_sqrt(dx * dx + dy * dy + dz * dz), D));
         });
@@ -1065,8 +1156,7 @@ __global__ __launch_bounds__(256) void k_pci_pressure(Dev D) {
     const bool multi = D.nphases > 1;
     float ax = 0.f, ay = 0.f, az = 0.f;
     int c = D.cnt[i];
-    each_fluid(D, s, nf_of(c), [&](int j) {
-        float4 pb = D.vs[j];
+    each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.vs[j]; }, [&](int j, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
         float acce;
@@ -1080,8 +1170,7 @@ __global__ __launch_bounds__(256) void k_pci_pressure(Dev D) {
         ax += dx * acce; ay += dy * acce; az += dz * acce;
     });
     if (qa > 0.0f) {
-        each_boundary(D, s, nb_of(c), [&](int j) {
-            float4 pb = D.xb[j];
+        each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
             float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
             float acce = g * (-rho0 * pb.w * (2.0f * qa));
