@@ -43,48 +43,8 @@ __device__ __forceinline__ Slot slot_nb(const Dev& D, bool gated) {
     return s;
 }
 
-// Calls f(j) for the entries of a packed list, eight per 16-byte group (the
-// second four only where the list reaches them), with the next group loaded
-// before the current one is decoded.
-template <int TOFF, class F>
-__device__ __forceinline__ void each_in_list16(const int4* __restrict__ list, int ncap, int i, int n, F&& f) {
-    const int4* __restrict__ col = list + i;
-    const int ng = (n + 7) >> 3;
-    if (ng <= 0) return;
-    int4 cur = col[0];
-    for (int g = 0; g < ng; ++g) {
-        int4 nxt = cur;
-        if (g + 1 < ng) nxt = col[(size_t)(g + 1) * ncap];
-        auto dec = [&](unsigned e) { f(g_nbtab[TOFF + (e >> NB16_OFF_BITS)] + (int)e); };
-        dec((unsigned)cur.x & 0xffffu);
-        dec((unsigned)cur.x >> 16);
-        dec((unsigned)cur.y & 0xffffu);
-        dec((unsigned)cur.y >> 16);
-        if (8 * g + 4 < n) {
-            dec((unsigned)cur.z & 0xffffu);
-            dec((unsigned)cur.z >> 16);
-            dec((unsigned)cur.w & 0xffffu);
-            dec((unsigned)cur.w >> 16);
-        }
-        cur = nxt;
-    }
-}
-
-template <class F>
-__device__ __forceinline__ void each_fluid(const Dev& D, const Slot& s, int nf, F&& f) {
-    if (s.packed) each_in_list16<0>(D.nbF16, D.ncap, s.i, nf, f);
-    else each_in_list(D.nbF, D.ncap, s.i, nf, f);
-}
-
-template <class F>
-__device__ __forceinline__ void each_boundary(const Dev& D, const Slot& s, int nb, F&& f) {
-    if (s.packed) each_in_list16<32>(D.nbB16, D.ncap, s.i, nb, f);
-    else each_in_list(D.nbB, D.ncap, s.i, nb, f);
-}
-
-// Split form for the passes that run every solver iteration: load(j) issues
-// the gather(s) of neighbour j and returns the record(s), use(j, r) consumes
-// them.  A batch of neighbours is gathered first, then the next list group is
+// Neighbour loops in split form: load(j) issues the gather(s) of neighbour j
+// and returns the record(s), use(j, r) consumes them.  A batch of neighbours is gathered first, then the next list group is
 // requested, then the batch is used: a wave waits for its memory returns in
 // issue order, so a list load issued ahead of the gathers would hold up every
 // batch.  W = 8: one batch of eight per packed group (single-record passes);
@@ -354,14 +314,14 @@ __device__ __forceinline__ float density_sum(const Dev& D, const Slot& s, float4
 __device__ __forceinline__ float density_change(const Dev& D, const Slot& s, float4 pa, float4 va, float m,
                                                 float rho0, int c) {
     float sf = 0.0f, sb = 0.0f;
-    each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.vs[j]}; }, [&](int j, Rec2 r) {
-        const float4 pb = r.a, vb = r.b;
+    each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.vs[j]}; }, [&](int j, Rec2 rec) {
+        const float4 pb = rec.a, vb = rec.b;
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
         sf += g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
     });
-    each_boundary_lu<4>(D, s, nb_of(c), [&](int j) { return Rec2{D.xb[j], D.vb[j]}; }, [&](int j, Rec2 r) {
-        const float4 pb = r.a, vb = r.b;
+    each_boundary_lu<4>(D, s, nb_of(c), [&](int j) { return Rec2{D.xb[j], D.vb[j]}; }, [&](int j, Rec2 rec) {
+        const float4 pb = rec.a, vb = rec.b;
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
         sb += pb.w * g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
@@ -404,9 +364,8 @@ __global__ __launch_bounds__(256) void k_wc_force(Dev D) {
     const bool multi = D.nphases > 1;
     float ax = D.gx, ay = D.gy, az = D.gz;
     int c = D.cnt[i];
-    each_fluid(D, s, nf_of(c), [&](int j) {
-        float4 pb = D.xk[j];           // (x, p/rho^2)
-        float4 vb = D.v[j];            // (v, rho)
+    each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.xk[j], D.v[j]}; }, [&](int j, Rec2 rec) {
+        const float4 pb = rec.a, vb = rec.b;   // (x, p/rho^2), (v, rho)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
         float g = kernel_gradfac(fast_sqrt(r2), D);
@@ -427,9 +386,8 @@ __global__ __launch_bounds__(256) void k_wc_force(Dev D) {
         }
         ax += dx * acce; ay += dy * acce; az += dz * acce;
     });
-    each_boundary(D, s, nb_of(c), [&](int j) {
-        float4 pb = D.xb[j];
-        float4 vb = D.vb[j];
+    each_boundary_lu<4>(D, s, nb_of(c), [&](int j) { return Rec2{D.xb[j], D.vb[j]}; }, [&](int j, Rec2 rec) {
+        const float4 pb = rec.a, vb = rec.b;
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
         float g = kernel_gradfac(fast_sqrt(r2), D);
@@ -465,8 +423,7 @@ __global__ __launch_bounds__(256) void k_df_density_alpha(Dev D) {
     rho.init(m * D.W0);
     float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2 = 0.f;
     int c = D.cnt[i];
-    each_fluid(D, s, nf_of(c), [&](int j) {
-        float4 pb = D.x[j];
+    each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.x[j]; }, [&](int j, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
         rho.add(j != i ? m * kernel_W(r, D) : 0.0f);
@@ -475,8 +432,7 @@ __global__ __launch_bounds__(256) void k_df_density_alpha(Dev D) {
         float gs = gm * r;                     // |dW/dr| m (zero at r = 0)
         a2 += gs * gs;
     });
-    each_boundary(D, s, nb_of(c), [&](int j) {
-        float4 pb = D.xb[j];
+    each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
         float psi = rho0 * pb.w;
@@ -509,9 +465,8 @@ __global__ __launch_bounds__(256) void k_df_aadv(Dev D) {
     const bool multi = D.nphases > 1;
     float ax = D.gx, ay = D.gy, az = D.gz;
     int c = D.cnt[i];
-    each_fluid(D, s, nf_of(c), [&](int j) {
-        float4 pb = D.x[j];
-        float4 vb = D.v[j];            // (v, rho)
+    each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.v[j]}; }, [&](int j, Rec2 rec) {
+        const float4 pb = rec.a, vb = rec.b;   // vb = (v, rho)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
         float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
@@ -528,9 +483,8 @@ __global__ __launch_bounds__(256) void k_df_aadv(Dev D) {
         f = pro < 0.0f ? f : 0.0f;
         ax += dx * f; ay += dy * f; az += dz * f;
     });
-    each_boundary(D, s, nb_of(c), [&](int j) {
-        float4 pb = D.xb[j];
-        float4 vb = D.vb[j];
+    each_boundary_lu<4>(D, s, nb_of(c), [&](int j) { return Rec2{D.xb[j], D.vb[j]}; }, [&](int j, Rec2 rec) {
+        const float4 pb = rec.a, vb = rec.b;
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
         float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
@@ -577,8 +531,7 @@ __global__ __launch_bounds__(256) void k_df_warm_features(Dev D, float* __restri
     if (!feat) return;
     const float sp = D.spacing;
     float bmin2 = 4.0f * sp * sp;
-    each_boundary(D, s, nb_of(c), [&](int j) {
-        float4 pb = D.xb[j];
+    each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         bmin2 = fminf(bmin2, dx * dx + dy * dy + dz * dz);
     });
@@ -810,9 +763,8 @@ __global__ __launch_bounds__(256) void k_ii_aadv_dii(Dev D) {
     float ax = D.gx, ay = D.gy, az = D.gz;
     float dfx = 0.f, dfy = 0.f, dfz = 0.f, dbx = 0.f, dby = 0.f, dbz = 0.f;
     int c = D.cnt[i];
-    each_fluid(D, s, nf_of(c), [&](int j) {
-        float4 pb = D.x[j];
-        float4 vb = D.v[j];            // (v, rho)
+    each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.v[j]}; }, [&](int j, Rec2 rec) {
+        const float4 pb = rec.a, vb = rec.b;   // vb = (v, rho)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
         float g = kernel_gradfac(fast_sqrt(r2), D);
@@ -829,9 +781,8 @@ __global__ __launch_bounds__(256) void k_ii_aadv_dii(Dev D) {
         ax += dx * f; ay += dy * f; az += dz * f;
         dfx += dx * g; dfy += dy * g; dfz += dz * g;
     });
-    each_boundary(D, s, nb_of(c), [&](int j) {
-        float4 pb = D.xb[j];
-        float4 vb = D.vb[j];
+    each_boundary_lu<4>(D, s, nb_of(c), [&](int j) { return Rec2{D.xb[j], D.vb[j]}; }, [&](int j, Rec2 rec) {
+        const float4 pb = rec.a, vb = rec.b;
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
         float g = kernel_gradfac(fast_sqrt(r2), D);
@@ -861,18 +812,16 @@ __global__ __launch_bounds__(256) void k_ii_aii(Dev D) {
     float cji = m * dt * dt / (rhoa * rhoa);
     float t1f = 0.f, t1b = 0.f, t2f = 0.f, t2b = 0.f;
     int c = D.cnt[i];
-    each_fluid(D, s, nf_of(c), [&](int j) {
-        float4 pb = D.x[j];
-        float4 vb = D.vs[j];
+    each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.vs[j]}; }, [&](int j, Rec2 rec) {
+        const float4 pb = rec.a, vb = rec.b;
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
         float gx = dx * g, gy = dy * g, gz = dz * g;
         t1f += gx * (va.x - vb.x) + gy * (va.y - vb.y) + gz * (va.z - vb.z);
         t2f += (dii.x - cji * gx) * gx + (dii.y - cji * gy) * gy + (dii.z - cji * gz) * gz;
     });
-    each_boundary(D, s, nb_of(c), [&](int j) {
-        float4 pb = D.xb[j];
-        float4 vb = D.vb[j];
+    each_boundary_lu<4>(D, s, nb_of(c), [&](int j) { return Rec2{D.xb[j], D.vb[j]}; }, [&](int j, Rec2 rec) {
+        const float4 pb = rec.a, vb = rec.b;
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
         float gx = dx * g, gy = dy * g, gz = dz * g;
@@ -929,8 +878,8 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
         float sax = ua.x - dia.x * pla, say = ua.y - dia.y * pla, saz = ua.z - dia.z * pla;
         float ff = 0.f, fb = 0.f;
         int c = D.cnt[i];
-        each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.f4b[j]}; }, [&](int j, Rec2 r) {
-            const float4 pb = r.a, ub = r.b;
+        each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.f4b[j]}; }, [&](int j, Rec2 rec) {
+            const float4 pb = rec.a, ub = rec.b;
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
             float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
             float gx = dx * g, gy = dy * g, gz = dz * g;
@@ -983,8 +932,7 @@ __global__ __launch_bounds__(256) void k_ii_force(Dev D) {
     const bool multi = D.nphases > 1;
     float ax = 0.f, ay = 0.f, az = 0.f;
     int c = D.cnt[i];
-    each_fluid(D, s, nf_of(c), [&](int j) {
-        float4 pb = D.xk[j];           // (x, p/rho^2)
+    each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, [&](int j, float4 pb) {           // (x, p/rho^2)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
         float g = kernel_gradfac(fast_sqrt(r2), D);
@@ -1004,8 +952,7 @@ __global__ __launch_bounds__(256) void k_ii_force(Dev D) {
         ax += dx * acce; ay += dy * acce; az += dz * acce;
     });
     if (pra > 0.0f) {
-        each_boundary(D, s, nb_of(c), [&](int j) {
-            float4 pb = D.xb[j];
+        each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
             float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
             // p_a/rho_a^2: the boundary term the pressure solve predicts
@@ -1058,9 +1005,8 @@ __global__ __launch_bounds__(256) void k_pci_aext(Dev D) {
     const bool multi = D.nphases > 1;
     float ax = D.gx, ay = D.gy, az = D.gz;
     int c = D.cnt[i];
-    each_fluid(D, s, nf_of(c), [&](int j) {
-        float4 pb = D.x[j];
-        float4 vb = D.v[j];            // (v, rho)
+    each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.v[j]}; }, [&](int j, Rec2 rec) {
+        const float4 pb = rec.a, vb = rec.b;   // vb = (v, rho)
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
         float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
@@ -1076,9 +1022,8 @@ __global__ __launch_bounds__(256) void k_pci_aext(Dev D) {
         f = pro < 0.0f ? f : 0.0f;
         ax += dx * f; ay += dy * f; az += dz * f;
     });
-    each_boundary(D, s, nb_of(c), [&](int j) {
-        float4 pb = D.xb[j];
-        float4 vb = D.vb[j];
+    each_boundary_lu<4>(D, s, nb_of(c), [&](int j) { return Rec2{D.xb[j], D.vb[j]}; }, [&](int j, Rec2 rec) {
+        const float4 pb = rec.a, vb = rec.b;
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r2 = dx * dx + dy * dy + dz * dz;
         float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;

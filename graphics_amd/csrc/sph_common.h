@@ -289,9 +289,8 @@ __device__ __forceinline__ int scene_of_key(uint32_t key, const uint32_t* base, 
 // Neighbour lists are padded to a multiple of 4 entries: fluid lists with the
 // particle itself (distance 0: no gradient contribution; densities skip it by
 // index), boundary lists with the dummy boundary particle D.nb placed far
-// away (W = gradW = 0).  The loops below read four indices, then issue the
-// four gathers before using them.
-// ELL neighbour lists stored in groups of four entries: entries 4g..4g+3 of
+// away (W = gradW = 0).
+// 32-bit ELL neighbour lists stored in groups of four entries: entries 4g..4g+3 of
 // particle i are the int4 at group g, (g * ncap + i).  A wave reads one group
 // of its 64 particles with one 16-byte load per lane (1 KiB contiguous).
 __device__ __forceinline__ size_t nb_slot(int k, int i, int ncap) {
@@ -313,26 +312,6 @@ constexpr int NB16_TAB = 64;           // ints of table per tile
 
 __device__ __forceinline__ size_t nb16_slot(int k, int i, int ncap) {
     return ((size_t)(k >> 3) * ncap + i) * 8 + (k & 7);
-}
-
-// Calls f(j) for the neighbours of i, four at a time (lists are padded to a
-// multiple of four), with the next group's indices loaded before the
-// current group is processed.
-template <class F>
-__device__ __forceinline__ void each_in_list(const int* __restrict__ list, int ncap, int i, int n, F&& f) {
-    const int4* __restrict__ col = reinterpret_cast<const int4*>(list) + i;
-    const int ng = (n + 3) >> 2;
-    if (ng <= 0) return;
-    int4 cur = col[0];
-    for (int g = 0; g < ng; ++g) {
-        int4 nxt = cur;
-        if (g + 1 < ng) nxt = col[(size_t)(g + 1) * ncap];
-        f(cur.x);
-        f(cur.y);
-        f(cur.z);
-        f(cur.w);
-        cur = nxt;
-    }
 }
 
 constexpr float DUMMY_FAR = 1.0e15f;   // coordinate of the dummy boundary particle
