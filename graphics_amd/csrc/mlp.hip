@@ -35,18 +35,20 @@ __device__ __forceinline__ int kmap(int g, int j) { return j < 4 ? 4 * g + j : 1
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }   // round to nearest even
 
-// GELU, tanh form: 0.5 x (1 + tanh(c (x + 0.044715 x^3))), c = sqrt(2/pi)
-__device__ __forceinline__ float gelu(float x) {
+// GELU, tanh form: 0.5 x (1 + tanh(u)), u = c (x + 0.044715 x^3),
+// c = sqrt(2/pi), evaluated as x sigma(2u) = x / (1 + exp(-2u)) (the same
+// function: one exponential and one hardware reciprocal per activation)
+__device__ __forceinline__ float gelu_sigma(float x) {
     const float c = 0.7978845608028654f;
-    float u = c * (x + 0.044715f * x * x * x);
-    float t = 1.0f - 2.0f / (__expf(2.0f * u) + 1.0f);
-    return 0.5f * x * (1.0f + t);
+    const float u = x * (c + (c * 0.044715f) * (x * x));
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sigma(x); }
+// d/dx x sigma(2u) = sigma + 2 x sigma (1 - sigma) c (1 + 3 * 0.044715 x^2)
 __device__ __forceinline__ float gelu_grad(float x) {
     const float c = 0.7978845608028654f;
-    float u = c * (x + 0.044715f * x * x * x);
-    float t = 1.0f - 2.0f / (__expf(2.0f * u) + 1.0f);
-    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * x * x);
+    const float sg = gelu_sigma(x);
+    return sg + 2.0f * x * sg * (1.0f - sg) * c * (1.0f + 3.0f * 0.044715f * x * x);
 }
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
@@ -190,6 +192,62 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const bf16x8* __restrict__ 
         forward_tile(F, P, feat, tile * 16, n, o, store);
         int p = tile * 16 + (lane & 15);
         if ((lane >> 4) == 0 && p < n) y[p] = o.y;
+    }
+}
+
+// Inference only: the 22 weight fragments stay in registers and the biases
+// in LDS for all the tiles of 16 particles a wave processes (the training
+// forward above re-reads them per tile).
+__global__ __launch_bounds__(256, 3) void k_mlp_infer(const bf16x8* __restrict__ F, const float* __restrict__ P,
+                                                 const float* __restrict__ feat, float* __restrict__ y, int n) {
+    __shared__ float sb[3 * 64];
+    for (int k = threadIdx.x; k < 3 * 64; k += 256) {
+        const int off[3] = {OFF_B1, OFF_B2, OFF_B3};
+        sb[k] = P[off[k >> 6] + (k & 63)];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    bf16x8 w[MLP_FWD_FRAGS];
+#pragma unroll
+    for (int f = 0; f < MLP_FWD_FRAGS; ++f) w[f] = F[f * 64 + lane];
+    const float b4 = P[OFF_B4];
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    const int ntile = (n + 15) / 16;
+    for (int tile = wave; tile < ntile; tile += nw) {
+        const int p = tile * 16 + c;
+        bf16x8 x;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = to_bf16(0.0f);
+        if (g < 2 && p < n) {
+            const float4 f = reinterpret_cast<const float4*>(feat)[p * 2 + g];
+            x[0] = to_bf16(f.x); x[1] = to_bf16(f.y); x[2] = to_bf16(f.z); x[3] = to_bf16(f.w);
+        }
+        float h[4][4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma(w[t], x, acc);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) h[t][j] = gelu(acc[j] + sb[16 * t + 4 * g + j]);
+        }
+#pragma unroll
+        for (int L = 0; L < 2; ++L) {
+            const bf16x8 b0 = chain(h, 0), b1 = chain(h, 1);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                acc = mfma(w[4 + 8 * L + 2 * t], b0, acc);
+                acc = mfma(w[4 + 8 * L + 2 * t + 1], b1, acc);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) h[t][j] = gelu(acc[j] + sb[64 * (L + 1) + 16 * t + 4 * g + j]);
+            }
+        }
+        const bf16x8 b0 = chain(h, 0), b1 = chain(h, 1);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma(w[20], b0, acc);
+        acc = mfma(w[21], b1, acc);
+        if (g == 0 && p < n) y[p] = acc[0] + b4;
     }
 }
 
@@ -368,6 +426,13 @@ void mlp_pack(MlpDev& m, hipStream_t st) {
 void mlp_forward(const MlpDev& m, const float* feat, float* y, int n, float* store, hipStream_t st) {
     if (n <= 0) return;
     int waves = (n + 15) / 16;
+    if (!store) {
+        // about three waves per SIMD, each looping over tiles
+        const int blocks = std::min((waves + 3) / 4, 768);
+        hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, (const bf16x8*)m.fwd, (const float*)m.params,
+                           feat, y, n);
+        return;
+    }
     int blocks = std::min((waves + 3) / 4, 8192);
     hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(256), 0, st, (const bf16x8*)m.fwd,
                        (const float*)m.params, feat, y, n, store);
