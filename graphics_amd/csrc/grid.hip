@@ -267,19 +267,27 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
         const int z = cz + dz, y = cy + dy;
         if (act && z >= 0 && z < G.nz && y >= 0 && y < G.ny) {
             int2 rf = run3(D.cellF, key_of(G, D.rowRank, cx, y, z), cx, G.nx);
-            for (int j = rf.x; j < rf.y; ++j) {
-                if (j == i) continue;
-                float l2 = dis_square_exact(pa, D.x[j]);
-                if (l2 < D.h2) {
-                    if (stored < NBF_CAP) {
-                        if (packed) put16(stored, ord * 9 + r, j);
-                        else put(D.nbF, stored, j);
-                        ++stored;
-                    } else {
-                        ++over;
+            // candidates four at a time: four loads in flight, tested in order
+            for (int j0 = rf.x; j0 < rf.y; j0 += 4) {
+                float4 q[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) q[k] = D.x[min(j0 + k, rf.y - 1)];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int j = j0 + k;
+                    if (j >= rf.y || j == i) continue;
+                    float l2 = dis_square_exact(pa, q[k]);
+                    if (l2 < D.h2) {
+                        if (stored < NBF_CAP) {
+                            if (packed) put16(stored, ord * 9 + r, j);
+                            else put(D.nbF, stored, j);
+                            ++stored;
+                        } else {
+                            ++over;
+                        }
+                        ++nf;
+                        zero += (l2 == 0.0f);
                     }
-                    ++nf;
-                    zero += (l2 == 0.0f);
                 }
             }
         }
@@ -302,17 +310,25 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
                 if (y < 0 || y >= G.ny) continue;
                 const int slot = 32 + ord * 9 + (dz + 1) * 3 + (dy + 1);
                 int2 rb = run3(D.cellB, key_of(G, D.rowRank, cx, y, z), cx, G.nx);
-                for (int j = rb.x; j < rb.y; ++j) {
-                    float l2 = dis_square_exact(pa, D.xb[j]);
-                    if (l2 < D.h2) {
-                        if (nbn < NBB_CAP) {
-                            if (packed) put16(nbn, slot, j);
-                            else put(D.nbB, nbn, j);
-                        } else {
-                            ++over;
+                for (int j0 = rb.x; j0 < rb.y; j0 += 4) {
+                    float4 q[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) q[k] = D.xb[min(j0 + k, rb.y - 1)];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int j = j0 + k;
+                        if (j >= rb.y) continue;
+                        float l2 = dis_square_exact(pa, q[k]);
+                        if (l2 < D.h2) {
+                            if (nbn < NBB_CAP) {
+                                if (packed) put16(nbn, slot, j);
+                                else put(D.nbB, nbn, j);
+                            } else {
+                                ++over;
+                            }
+                            ++nbn;
+                            zero += (l2 == 0.0f);
                         }
-                        ++nbn;
-                        zero += (l2 == 0.0f);
                     }
                 }
             }
