@@ -187,6 +187,7 @@ def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, g
     timing = sim.kernel_timing() if profile else (0, 0.0, 0.0)
     n_live = sim.num_fluid()
     tuning = sim.tuning()
+    tuning["lists"] = sim.list_form()
     sim.close()
     return dict(elapsed=elapsed, stats=stats, warm_times=warm_times, warm_stats=warm_stats,
                 timing=timing, n_live=n_live, tuning=tuning)

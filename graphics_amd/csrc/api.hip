@@ -127,7 +127,8 @@ struct sphgpu_handle {
     bool capturing = false;
     bool tuned = false;
     bool listsBuilt = false;   // neighbour lists built at least once
-    float tuneMs[4] = {0, 0, 0, 0};    // density pass: map 0, 1; list builder: 8, 12 groups
+    float tuneMs[6] = {0, 0, 0, 0, 0, 0};   // density pass: map 0, 1; list builder: 8, 12 groups;
+                                            // density pass: plain, streaming list loads
     uint32_t gatherSeed = 1;
     // profiling
     std::string profName;
@@ -389,17 +390,20 @@ int solver_loop(sphgpu_handle* h, int which) {
 // lists of 16 or 32 lanes to common per-row counts, so that their gathers
 // share cache lines, was measured as a third choice: the density pass took
 // 0.124-0.127 ms against 0.086 unpadded on the 1M dam, so it was removed.)
-float time_launches(sphgpu_handle* h, const std::function<void()>& f) {
+// best of three: `prep` (untimed), then `reps` launches of `f`; per launch
+float time_launches(sphgpu_handle* h, const std::function<void()>& f, const std::function<void()>& prep = nullptr,
+                    int reps = 1) {
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return 0.0f;
     float best = 1e30f;
     for (int r = 0; r < 3; ++r) {
+        if (prep) prep();
         hipEventRecord(a, h->st);
-        f();
+        for (int q = 0; q < reps; ++q) f();
         hipEventRecord(b, h->st);
         float t = 0.0f;
         if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&t, a, b) == hipSuccess)
-            best = std::min(best, t);
+            best = std::min(best, t / reps);
     }
     hipEventDestroy(a);
     hipEventDestroy(b);
@@ -410,6 +414,7 @@ int autotune(sphgpu_handle* h) {
     Dev& D = h->D;
     D.tileMap = 1;
     D.nbStage = 8;
+    D.listNT = 0;
     run_ns(h);   // sorted layout and lists to time on (the first step sorts again: same order)
     auto density = [h]() {
         const Dev& D = h->D;
@@ -429,6 +434,13 @@ int autotune(sphgpu_handle* h) {
         });
     D.nbStage = h->tuneMs[2] <= h->tuneMs[3] ? 8 : 12;
     launch_neighbours(D, h->st);   // lists of the chosen variant
+    // list loads: a fresh list build, then four passes (as in a step, where
+    // whether the lists stay in the last-level cache between passes decides)
+    for (int k = 0; k < 2; ++k) {
+        D.listNT = k;
+        h->tuneMs[4 + k] = time_launches(h, density, [h]() { launch_neighbours(h->D, h->st); }, 4);
+    }
+    D.listNT = h->tuneMs[4] <= h->tuneMs[5] ? 0 : 1;
     h->tuned = true;
     HIPCHECK(h, hipStreamSynchronize(h->st));
     HIPCHECK(h, hipGetLastError());
@@ -1712,8 +1724,9 @@ int sphgpu_get_tuning(const sphgpu_handle* h, int* choice, float* ms) {
     return SPHGPU_OK;
 }
 
-int sphgpu_get_list_form(sphgpu_handle* h, int* packed_tiles, int* tiles) {
+int sphgpu_get_list_form(sphgpu_handle* h, int* packed_tiles, int* tiles, int* streamed) {
     if (!h || !packed_tiles || !tiles) return SPHGPU_E_ARG;
+    if (streamed) *streamed = h->D.listNT;
     if (!h->initialized || !h->listsBuilt) return fail(h, SPHGPU_E_STATE, "no neighbour lists built yet");
     const int nt = h->D.npad / TILE;
     std::vector<int> flag(nt), valid(nt);

@@ -43,6 +43,19 @@ __device__ __forceinline__ Slot slot_nb(const Dev& D, bool gated) {
     return s;
 }
 
+// A list group is read once per pass.  Where the lists and records together
+// fit the last-level cache the plain load lets the next pass find the list
+// there; on larger scenes a streaming load keeps the list from pushing the
+// gathered records out (Dev::listNT, chosen by the autotuner).
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int4 ld_group(const int4* p, bool nt) {
+    if (nt) {
+        const v4i_t v = __builtin_nontemporal_load(reinterpret_cast<const v4i_t*>(p));
+        return make_int4(v.x, v.y, v.z, v.w);
+    }
+    return *p;
+}
+
 // Neighbour loops in split form: load(j) issues the gather(s) of neighbour j
 // and returns the record(s), use(j, r) consumes them.  A batch of neighbours is gathered first, then the next list group is
 // requested, then the batch is used: a wave waits for its memory returns in
@@ -53,13 +66,13 @@ __device__ __forceinline__ Slot slot_nb(const Dev& D, bool gated) {
 // index) but are not used, so every pass sums exactly the listed entries.
 template <int W, int TOFF, class LD, class US>
 __device__ __forceinline__ void each_in_list16_lu(const int4* __restrict__ list, int ncap, int i, int n, int padj,
-                                                  LD&& load, US&& use) {
+                                                  bool nt, LD&& load, US&& use) {
     static_assert(W == 4 || W == 8, "batch of four or eight");
     const int4* __restrict__ col = list + i;
     const int ng = (n + 7) >> 3;
     if (ng <= 0) return;
     const int n4 = (n + 3) & ~3;
-    int4 cur = col[0];
+    int4 cur = ld_group(&col[0], nt);
     for (int g = 0; g < ng; ++g) {
         const unsigned w[4] = {(unsigned)cur.x, (unsigned)cur.y, (unsigned)cur.z, (unsigned)cur.w};
         const bool second = 8 * g + 4 < n4;   // entries 8g+4..8g+7 are listed
@@ -76,7 +89,7 @@ __device__ __forceinline__ void each_in_list16_lu(const int4* __restrict__ list,
 #pragma unroll
             for (int k = 0; k < 8; ++k) r[k] = load(j[k]);
             __builtin_amdgcn_sched_barrier(0);
-            if (g + 1 < ng) nxt = col[(size_t)(g + 1) * ncap];
+            if (g + 1 < ng) nxt = ld_group(&col[(size_t)(g + 1) * ncap], nt);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int k = 0; k < 4; ++k) use(j[k], r[k]);
@@ -93,7 +106,7 @@ __device__ __forceinline__ void each_in_list16_lu(const int4* __restrict__ list,
 #pragma unroll
             for (int k = 0; k < 4; ++k) r[k] = load(j[4 + k]);
             __builtin_amdgcn_sched_barrier(0);
-            if (g + 1 < ng) nxt = col[(size_t)(g + 1) * ncap];
+            if (g + 1 < ng) nxt = ld_group(&col[(size_t)(g + 1) * ncap], nt);
             __builtin_amdgcn_sched_barrier(0);
             if (second) {
 #pragma unroll
@@ -106,12 +119,12 @@ __device__ __forceinline__ void each_in_list16_lu(const int4* __restrict__ list,
 
 // the same for 32-bit lists (groups of four, all listed or padding)
 template <class LD, class US>
-__device__ __forceinline__ void each_in_list_lu(const int* __restrict__ list, int ncap, int i, int n, LD&& load,
-                                                US&& use) {
+__device__ __forceinline__ void each_in_list_lu(const int* __restrict__ list, int ncap, int i, int n, bool nt,
+                                                LD&& load, US&& use) {
     const int4* __restrict__ col = reinterpret_cast<const int4*>(list) + i;
     const int ng = (n + 3) >> 2;
     if (ng <= 0) return;
-    int4 cur = col[0];
+    int4 cur = ld_group(&col[0], nt);
     for (int g = 0; g < ng; ++g) {
         const int j[4] = {cur.x, cur.y, cur.z, cur.w};
         decltype(load(0)) r[4];
@@ -119,7 +132,7 @@ __device__ __forceinline__ void each_in_list_lu(const int* __restrict__ list, in
         for (int k = 0; k < 4; ++k) r[k] = load(j[k]);
         int4 nxt = cur;
         __builtin_amdgcn_sched_barrier(0);
-        if (g + 1 < ng) nxt = col[(size_t)(g + 1) * ncap];
+        if (g + 1 < ng) nxt = ld_group(&col[(size_t)(g + 1) * ncap], nt);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int k = 0; k < 4; ++k) use(j[k], r[k]);
@@ -129,14 +142,16 @@ __device__ __forceinline__ void each_in_list_lu(const int* __restrict__ list, in
 
 template <int W, class LD, class US>
 __device__ __forceinline__ void each_fluid_lu(const Dev& D, const Slot& s, int nf, LD&& load, US&& use) {
-    if (s.packed) each_in_list16_lu<W, 0>(D.nbF16, D.ncap, s.i, nf, s.i, load, use);
-    else each_in_list_lu(D.nbF, D.ncap, s.i, nf, load, use);
+    const bool nt = __builtin_amdgcn_readfirstlane(D.listNT) != 0;
+    if (s.packed) each_in_list16_lu<W, 0>(D.nbF16, D.ncap, s.i, nf, s.i, nt, load, use);
+    else each_in_list_lu(D.nbF, D.ncap, s.i, nf, nt, load, use);
 }
 
 template <int W, class LD, class US>
 __device__ __forceinline__ void each_boundary_lu(const Dev& D, const Slot& s, int nb, LD&& load, US&& use) {
-    if (s.packed) each_in_list16_lu<W, 32>(D.nbB16, D.ncap, s.i, nb, D.nb, load, use);
-    else each_in_list_lu(D.nbB, D.ncap, s.i, nb, load, use);
+    const bool nt = __builtin_amdgcn_readfirstlane(D.listNT) != 0;
+    if (s.packed) each_in_list16_lu<W, 32>(D.nbB16, D.ncap, s.i, nb, D.nb, nt, load, use);
+    else each_in_list_lu(D.nbB, D.ncap, s.i, nb, nt, load, use);
 }
 
 // two gather records of one neighbour

@@ -142,7 +142,7 @@ SIGNATURES = {
     "sphgpu_kernel_timing": (C.c_int, [C.c_void_p, _LP, _DP, _DP]),
     "sphgpu_copy_probe": (C.c_int, [C.c_longlong, _DP]),
     "sphgpu_get_tuning": (C.c_int, [C.c_void_p, _IP, _FP]),
-    "sphgpu_get_list_form": (C.c_int, [C.c_void_p, _IP, _IP]),
+    "sphgpu_get_list_form": (C.c_int, [C.c_void_p, _IP, _IP, _IP]),
     "sphgpu_mlp_create": (C.c_int, [C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
     "sphgpu_mlp_destroy": (None, [C.c_void_p]),
     "sphgpu_mlp_set_params": (C.c_int, [C.c_void_p, _FP]),
@@ -604,10 +604,11 @@ class Simulation:
                        "lists_8": ms[2], "lists_12": ms[3]}}
 
     def list_form(self) -> dict:
-        """Tiles holding particles and how many of them have packed (16-bit) lists."""
-        p, t = C.c_int(), C.c_int()
-        self._check(self.lib.sphgpu_get_list_form(self.h, C.byref(p), C.byref(t)), "get_list_form")
-        return {"packed_tiles": p.value, "tiles": t.value}
+        """Tiles holding particles, how many of them have packed (16-bit) lists,
+        and whether the passes stream the lists (autotuned)."""
+        p, t, st = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.lib.sphgpu_get_list_form(self.h, C.byref(p), C.byref(t), C.byref(st)), "get_list_form")
+        return {"packed_tiles": p.value, "tiles": t.value, "streamed": bool(st.value)}
 
     def profile_kernel(self, name):
         self._check(self.lib.sphgpu_profile_kernel(self.h, None if name is None else name.encode()),

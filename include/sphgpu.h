@@ -320,9 +320,12 @@ int sphgpu_copy_probe(long long bytes, double* ms);
 int sphgpu_get_tuning(const sphgpu_handle* h, int* choice, float* ms);
 /* Form of the neighbour lists of the last step: *tiles = tiles holding
  * particles, *packed_tiles = those whose lists use 16-bit entries (the rest
- * use 32-bit entries; both hold the same neighbours in the same order).
- * Diagnostic; SPHGPU_E_STATE before lists were first built. */
-int sphgpu_get_list_form(sphgpu_handle* h, int* packed_tiles, int* tiles);
+ * use 32-bit entries; both hold the same neighbours in the same order);
+ * *streamed (optional) = 1 if the passes read the lists with streaming
+ * loads (the autotuner's choice: faster where lists and records exceed the
+ * last-level cache).  Diagnostic; SPHGPU_E_STATE before lists were first
+ * built. */
+int sphgpu_get_list_form(sphgpu_handle* h, int* packed_tiles, int* tiles, int* streamed);
 int sphgpu_kernel_timing(sphgpu_handle* h, long long* launches, double* total_ms,
                          double* total_alg_bytes);
 
