@@ -257,9 +257,8 @@ int run_ns(sphgpu_handle* h) {
 int sync_active(sphgpu_handle* h, bool* any) {
     Dev& D = h->D;
     int ns = D.nscenes;
-    HIPCHECK(h, hipMemcpyAsync(h->hActive, D.active, sizeof(int) * ns, hipMemcpyDeviceToHost, h->st));
-    HIPCHECK(h, hipMemcpyAsync(h->hIters, D.iters, sizeof(int) * ns, hipMemcpyDeviceToHost, h->st));
-    HIPCHECK(h, hipMemcpyAsync(h->hErr, D.sceneErr, sizeof(double) * ns, hipMemcpyDeviceToHost, h->st));
+    // flags, iteration counts and errors: one block, same layout on both sides
+    HIPCHECK(h, hipMemcpyAsync(h->hActive, D.active, 24 * (size_t)ns, hipMemcpyDeviceToHost, h->st));
     HIPCHECK(h, hipStreamSynchronize(h->st));
     bool a = false;
     for (int s = 0; s < ns; ++s) a = a || h->hActive[s] != 0;
@@ -338,8 +337,7 @@ int first_batch(const sphgpu_handle* h, int which) {
 
 void solver_loop_start(sphgpu_handle* h, int which, int batch) {
     Dev& D = h->D;
-    launch_fill_i32(D.active, 1, D.nscenes, h->st);
-    launch_fill_i32(D.iters, 0, D.nscenes, h->st);
+    launch_loop_start(D, h->st);
     for (int q = 0; q < batch; ++q) launch_iteration(h, which, q);
 }
 
@@ -1201,7 +1199,17 @@ int sphgpu_init(sphgpu_handle* h) {
     ALLOC(D.sceneStart, ns); ALLOC(D.sceneCount, ns); ALLOC(D.scenePadStart, ns + 1);
     int numTiles = ncap / TILE;
     ALLOC(D.tileScene, numTiles); ALLOC(D.tileValid, numTiles);
-    ALLOC(D.dt, ns); ALLOC(D.active, ns); ALLOC(D.iters, ns); ALLOC(D.sceneErr, ns);
+    ALLOC(D.dt, ns);
+    {
+        // solver-loop state in one block, laid out as its pinned host copy
+        // (flags | iteration counts | errors, 8 bytes per scene each), so the
+        // host reads it with one copy
+        double* loopBlk = nullptr;
+        ALLOC(loopBlk, 3 * (size_t)ns);
+        D.active = reinterpret_cast<int*>(loopBlk);
+        D.iters = reinterpret_cast<int*>(loopBlk + ns);
+        D.sceneErr = loopBlk + 2 * ns;
+    }
     ALLOC(D.partial, 2 * (size_t)numTiles); ALLOC(D.counters, C_COUNT); ALLOC(D.vmaxPartial, numTiles);
     ALLOC(h->layoutOut, 2); ALLOC(h->statsPart, 4 * (size_t)numTiles); ALLOC(h->statsOut, 4 * (size_t)ns);
     int sortItems = std::max<int>(ncap, int(nb));
@@ -1229,6 +1237,7 @@ int sphgpu_init(sphgpu_handle* h) {
         void* hm;
         HIPCHECK(h, hipHostMalloc(&hm, words * 8));
         char* c = static_cast<char*>(hm);
+        // hActive, hIters, hErr: the layout of the device block D.active
         h->hActive = reinterpret_cast<int*>(c); c += 8 * (size_t)ns;
         h->hIters = reinterpret_cast<int*>(c); c += 8 * (size_t)ns;
         h->hErr = reinterpret_cast<double*>(c); c += 8 * (size_t)ns;

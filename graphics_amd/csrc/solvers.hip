@@ -223,9 +223,22 @@ __global__ __launch_bounds__(256) void k_scene_reduce(Dev D, int mode, float tol
     __shared__ float ldsf[4];
     double s = 0.0;
     float mx = 0.0f;
-    for (int t = t0 + threadIdx.x; t < t1; t += 256) {
-        s += D.partial[2 * t];
-        if (mode == 2) mx = fmaxf(mx, (float)D.partial[2 * t + 1]);
+    // each thread sums tiles t0 + tid, + 256, + 512, ... in that order; the
+    // loads of eight of them are issued together
+    for (int t = t0 + threadIdx.x; t < t1; t += 256 * 8) {
+        double v[8];
+        float w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int tt = t + 256 * k;
+            v[k] = tt < t1 ? D.partial[2 * tt] : 0.0;
+            w[k] = (mode == 2 && tt < t1) ? (float)D.partial[2 * tt + 1] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (t + 256 * k < t1) s += v[k];
+            mx = fmaxf(mx, w[k]);
+        }
     }
     s = block_sum_256(s, lds);
     mx = block_max_256(mx, ldsf);
@@ -1170,6 +1183,14 @@ __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ a
     for (size_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) b[k] = a[k];
 }
 
+// start of a solver loop: every scene active, iteration counts zero
+__global__ __launch_bounds__(64) void k_loop_start(Dev D) {
+    const int sc = blockIdx.x * 64 + threadIdx.x;
+    if (sc >= D.nscenes) return;
+    D.active[sc] = 1;
+    D.iters[sc] = 0;
+}
+
 __global__ void k_fill_i32(int* p, int v, int n) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1187,6 +1208,9 @@ void launch_copy_probe(const float4* a, float4* b, size_t n, hipStream_t st) {
 void launch_cfl(const Dev& D, float cfl, float spacing, float dtMax, float dtMin, hipStream_t st) {
     if (tiles_of(D) > 0) hipLaunchKernelGGL(k_cfl_partial, dim3(tiles_of(D)), dim3(256), 0, st, D);
     hipLaunchKernelGGL(k_cfl_reduce, dim3(D.nscenes), dim3(256), 0, st, D, cfl, spacing, dtMax, dtMin);
+}
+void launch_loop_start(const Dev& D, hipStream_t st) {
+    hipLaunchKernelGGL(k_loop_start, dim3((D.nscenes + 63) / 64), dim3(64), 0, st, D);
 }
 void launch_scene_reduce(const Dev& D, int mode, float tol, int minIt, int cap, hipStream_t st) {
     hipLaunchKernelGGL(k_scene_reduce, dim3(D.nscenes), dim3(256), 0, st, D, mode, tol, minIt, cap);

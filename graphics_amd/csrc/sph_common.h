@@ -402,6 +402,8 @@ void launch_pci_density(const Dev& D, hipStream_t st);
 void launch_pci_pressure(const Dev& D, hipStream_t st);
 void launch_pci_integrate(const Dev& D, hipStream_t st);
 void launch_scene_reduce(const Dev& D, int mode, float tol, int minIt, int cap, hipStream_t st);
+// start of a solver loop: every scene active, iteration counts zero
+void launch_loop_start(const Dev& D, hipStream_t st);
 void launch_stats(const Dev& D, double* part, double* out, hipStream_t st);
 void launch_fill_i32(int* p, int v, int n, hipStream_t st);
 void launch_copy_probe(const float4* a, float4* b, size_t n, hipStream_t st);
