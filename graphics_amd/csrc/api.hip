@@ -646,11 +646,6 @@ int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
     const int rcb = update_solids_point(h);
     if (D.anyDynamic && (rc = rigid_update(h))) return rc;
     run_ns(h);
-    {
-        ProfScope ps(h, "df_density_alpha", nf_live(h) * 20.0);
-        launch_df_density_alpha(D, h->st);
-    }
-    HIPCHECK(h, hipMemcpyAsync(D.vs, D.v, sizeof(float4) * D.npad, hipMemcpyDeviceToDevice, h->st));
     // divergence warm start: 0 none, 1 half the previous stiffness, 2 model
     const bool learnedMode = h->P.warm_start == SPHGPU_WARM_LEARNED;
     const bool training = learnedMode && !h->mlpPreset && h->learned < h->P.learn_steps;
@@ -659,7 +654,12 @@ int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
         if (warm || (learnedMode && training)) mode = 1;
         else if (learnedMode) mode = 2;
     }
-    if (mode != 0) launch_df_warm_features(D, learnedMode ? h->feat : nullptr, h->st);
+    {
+        ProfScope ps(h, "df_density_alpha", nf_live(h) * 20.0);
+        if (mode != 0) launch_df_density_alpha_warm(D, learnedMode ? h->feat : nullptr, h->st);
+        else launch_df_density_alpha(D, h->st);
+    }
+    HIPCHECK(h, hipMemcpyAsync(D.vs, D.v, sizeof(float4) * D.npad, hipMemcpyDeviceToDevice, h->st));
     if (mode == 2) {
         ProfScope ps(h, "mlp_forward", nf_live(h) * 36.0);
         mlp_forward(h->mlp, h->feat, h->ypred, D.npad, nullptr, h->st);

@@ -440,7 +440,8 @@ __global__ __launch_bounds__(256) void k_wc_force(Dev D) {
 
 // ============================================================ DFSPH ======
 // DFSPH::DF_constantDensitySolverFirst (DFSPH.cpp:84-193): rho (Kahan) and alpha.
-__global__ __launch_bounds__(256) void k_df_density_alpha(Dev D) {
+template <bool WARM>
+__global__ __launch_bounds__(256) void k_df_density_alpha(Dev D, float* __restrict__ feat) {
     Slot s = slot_nb(D, false);
     if (!s.live) return;
     const int i = s.i;
@@ -451,29 +452,65 @@ __global__ __launch_bounds__(256) void k_df_density_alpha(Dev D) {
     rho.init(m * D.W0);
     float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2 = 0.f;
     int c = D.cnt[i];
-    each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.x[j]; }, [&](int j, float4 pb) {
+    // WARM: also Drho/Dt with v* = v (density_change) and the nearest
+    // boundary distance, for the warm start of the divergence loop
+    const float4 va = WARM ? D.v[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float sp = D.spacing;
+    float sf = 0.0f, sb = 0.0f, bmin2 = 4.0f * sp * sp;
+    auto fluid = [&](int j, float4 pb, float4 vb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
         rho.add(j != i ? m * kernel_W(r, D) : 0.0f);
-        float gm = kernel_gradfac(r, D) * m;
+        float g = kernel_gradfac(r, D);
+        float gm = g * m;
         a1x += dx * gm; a1y += dy * gm; a1z += dz * gm;
         float gs = gm * r;                     // |dW/dr| m (zero at r = 0)
         a2 += gs * gs;
-    });
-    each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
+        if (WARM) sf += g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
+    };
+    auto boundary = [&](float4 pb, float4 vb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float r = fast_sqrt(dx * dx + dy * dy + dz * dz);
+        float r2 = dx * dx + dy * dy + dz * dz;
+        float r = fast_sqrt(r2);
         float psi = rho0 * pb.w;
         rho.add(psi * kernel_W(r, D));
-        float gm = kernel_gradfac(r, D) * psi;
+        float g = kernel_gradfac(r, D);
+        float gm = g * psi;
         a1x += dx * gm; a1y += dy * gm; a1z += dz * gm;
-    });
+        if (WARM) {
+            sb += pb.w * g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
+            bmin2 = fminf(bmin2, r2);
+        }
+    };
+    if (WARM) {
+        each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.v[j]}; },
+                         [&](int j, Rec2 rec) { fluid(j, rec.a, rec.b); });
+        each_boundary_lu<4>(D, s, nb_of(c), [&](int j) { return Rec2{D.xb[j], D.vb[j]}; },
+                            [&](int j, Rec2 rec) { boundary(rec.a, rec.b); });
+    } else {
+        each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.x[j]; },
+                         [&](int j, float4 pb) { fluid(j, pb, pb); });
+        each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; },
+                            [&](int j, float4 pb) { boundary(pb, pb); });
+    }
     float density = rho.s;
     float den = a1x * a1x + a1y * a1y + a1z * a1z + a2;
     if (den < 1.0e-6f) den = 1.0e-6f;
+    const float alpha = density / den;
     D.rho[i] = density;
-    D.alpha[i] = density / den;
+    D.alpha[i] = alpha;
     D.v[i].w = density;
+    if (!WARM) return;
+    const float d = m * sf + rho0 * sb;
+    D.f2[i] = d;
+    if (!feat) return;
+    const float dt = D.dt[s.scene];
+    float4* f = reinterpret_cast<float4*>(feat) + 2 * (size_t)i;
+    f[0] = make_float4((density / rho0 - 1.0f) * 100.0f, asinhf(d * dt / rho0 * 1000.0f),
+                       (float)(nf_real(c) + nb_of(c)) * (1.0f / 30.0f),
+                       log1pf(fmaxf(D.kvsum[i] * dt / (alpha * rho0) * 1000.0f, 0.0f)));
+    f[1] = make_float4(alpha / D.alpha_lat, sqrtf(va.x * va.x + va.y * va.y + va.z * va.z) * dt / sp,
+                       sqrtf(bmin2) / sp, dt / D.dt_max);
 }
 
 // DFSPH::DF_forceExceptPressure (DFSPH.cpp:1419-1475; helpers 1246-1359) and
@@ -545,32 +582,8 @@ __global__ __launch_bounds__(256) void k_df_aadv(Dev D) {
 //   5 |v| dt / spacing
 //   6 distance to the nearest boundary particle / spacing (2 if none)
 //   7 dt / dt_max
-// The kernel always stores Drho/Dt in f2 for the gate of k_df_warm_apply.
-__global__ __launch_bounds__(256) void k_df_warm_features(Dev D, float* __restrict__ feat) {
-    Slot s = slot_nb(D, false);
-    if (!s.live) return;
-    const int i = s.i;
-    float4 pa = D.x[i], va = D.vs[i];
-    int ph = phase_of(pa);
-    float rho0 = D.rho0[ph];
-    int c = D.cnt[i];
-    float d = density_change(D, s, pa, va, D.mass[ph], rho0, c);
-    D.f2[i] = d;
-    if (!feat) return;
-    const float sp = D.spacing;
-    float bmin2 = 4.0f * sp * sp;
-    each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
-        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        bmin2 = fminf(bmin2, dx * dx + dy * dy + dz * dz);
-    });
-    float dt = D.dt[s.scene], rho = D.rho[i], alpha = D.alpha[i];
-    float4* f = reinterpret_cast<float4*>(feat) + 2 * (size_t)i;
-    f[0] = make_float4((rho / rho0 - 1.0f) * 100.0f, asinhf(d * dt / rho0 * 1000.0f),
-                       (float)(nf_real(c) + nb_of(c)) * (1.0f / 30.0f),
-                       log1pf(fmaxf(D.kvsum[i] * dt / (alpha * rho0) * 1000.0f, 0.0f)));
-    f[1] = make_float4(alpha / D.alpha_lat, sqrtf(va.x * va.x + va.y * va.y + va.z * va.z) * dt / sp,
-                       sqrtf(bmin2) / sp, dt / D.dt_max);
-}
+// k_df_density_alpha<true> computes them with rho and alpha (one neighbour
+// pass) and always stores Drho/Dt in f2 for the gate of k_df_warm_apply.
 
 // mode 0: no warm start (accumulator restarts at zero); mode 1: warm_scale
 // times the previous step's accumulated stiffness K_prev (Bender & Koschier
@@ -1241,9 +1254,11 @@ thread_local LaunchTiming g_launch_timing;
 
 void launch_wc_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_wc_pressure); }
 void launch_wc_force_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_wc_force); }
-void launch_df_density_alpha(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_density_alpha); }
+void launch_df_density_alpha(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_density_alpha<false>, nullptr); }
+void launch_df_density_alpha_warm(const Dev& D, float* feat, hipStream_t st) {
+    SPH_LAUNCH(k_df_density_alpha<true>, feat);
+}
 void launch_df_aadv(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_aadv); }
-void launch_df_warm_features(const Dev& D, float* feat, hipStream_t st) { SPH_LAUNCH(k_df_warm_features, feat); }
 void launch_df_warm_apply(const Dev& D, const float* y, const float* feat, int mode, hipStream_t st) {
     SPH_LAUNCH(k_df_warm_apply, y, feat, mode);
 }

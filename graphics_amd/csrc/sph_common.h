@@ -382,7 +382,9 @@ void launch_wc_force_integrate(const Dev& D, hipStream_t st);
 void launch_df_density_alpha(const Dev& D, hipStream_t st);
 void launch_df_aadv(const Dev& D, hipStream_t st);
 void launch_df_pass_density(const Dev& D, hipStream_t st);
-void launch_df_warm_features(const Dev& D, float* feat, hipStream_t st);
+// density and alpha plus the warm start's inputs: Drho/Dt (v* = v) into f2
+// and, with feat, the learned model's eight features per particle
+void launch_df_density_alpha_warm(const Dev& D, float* feat, hipStream_t st);
 void launch_df_warm_apply(const Dev& D, const float* y, const float* feat, int mode, hipStream_t st);
 void launch_df_targets(const Dev& D, const float* feat, float* tgt, hipStream_t st);
 void launch_mlp_gather(const Dev& D, const float* feat, const float* tgt, float* bfeat, float* btgt, int n,
