@@ -127,7 +127,7 @@ struct sphgpu_handle {
     bool capturing = false;
     bool tuned = false;
     bool listsBuilt = false;   // neighbour lists built at least once
-    float tuneMs[6] = {0, 0, 0, 0, 0, 0};   // density pass: map 0, 1; list builder: 8, 12 groups;
+    float tuneMs[6] = {0, 0, 0, 0, 0, 0};   // density pass: map 0, 1; list builder: 6, 8 groups;
                                             // density pass: plain, streaming list loads
     uint32_t gatherSeed = 1;
     // profiling
@@ -382,7 +382,7 @@ int solver_loop(sphgpu_handle* h, int which) {
 // Before the first step, on the scene itself, two choices are timed (three
 // runs each, the fastest minimum wins): the tile-to-workgroup mapping
 // (contiguous runs of tiles per XCD, or round robin) on the solver's density
-// pass, and the LDS staging depth of the list builder (8 or 12 groups) on
+// pass, and the LDS staging depth of the list builder (6 or 8 groups) on
 // the list builder.  Every timed kernel recomputes its outputs from
 // positions alone, and every setting gives the same results.  (Padding the
 // lists of 16 or 32 lanes to common per-row counts, so that their gathers
@@ -425,12 +425,12 @@ int autotune(sphgpu_handle* h) {
         h->tuneMs[m] = time_launches(h, density);
     }
     D.tileMap = h->tuneMs[0] < h->tuneMs[1] ? 0 : 1;
-    const int depth[2] = {8, 12};
+    const int depth[2] = {6, 8};
     for (int k = 0; k < 2; ++k)
         h->tuneMs[2 + k] = time_launches(h, [h, &depth, k]() {
             launch_neighbours_variant(h->D, depth[k], h->st);
         });
-    D.nbStage = h->tuneMs[2] <= h->tuneMs[3] ? 8 : 12;
+    D.nbStage = h->tuneMs[2] <= h->tuneMs[3] ? 6 : 8;
     launch_neighbours(D, h->st);   // lists of the chosen variant
     // list loads: a fresh list build, then four passes (as in a step, where
     // whether the lists stay in the last-level cache between passes decides)

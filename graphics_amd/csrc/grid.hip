@@ -167,14 +167,13 @@ __device__ __forceinline__ int2 run3(const int2* __restrict__ table, uint32_t k0
 // group (16 bytes per particle), so each store instruction of a wave writes
 // whole cache lines.  A tile whose particles span at most NB16_ROWS own rows,
 // and whose row runs each fit NB16_OFF_BITS of offset, gets packed 16-bit
-// lists (staged whole: 8 groups of 8 entries); any other tile gets 32-bit
-// lists, staged 4 x NB_STAGE_GROUPS entries deep with the rest written to
-// global memory directly (8 or 12 groups: LDS per block against blocks per
+// lists (8 entries per group); any other tile gets 32-bit lists (4 per
+// group).  NB_STAGE_GROUPS groups are staged in LDS and the rest written to
+// global memory directly (6 or 8 groups: LDS per block against blocks per
 // CU, chosen by the autotuner).  Both forms hold the same entries in the
 // same order.
 template <int NB_STAGE_GROUPS>
 __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
-    static_assert(NB_STAGE_GROUPS * 8 >= NBF_CAP && NB_STAGE_GROUPS * 8 >= NBB_CAP, "packed lists are staged whole");
     __shared__ int4 stage[NB_STAGE_GROUPS][256];
     __shared__ int sBase[2 * 32], sEnd[2 * 32];
     __shared__ int sRow[256];
@@ -215,22 +214,34 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     int ord = __popcll(bal & ((2ull << lane) - 1ull)) - 1;
     for (int w = 0; w < wave; ++w) ord += sWave[w];
     bool packed = sWave[0] + sWave[1] + sWave[2] + sWave[3] <= NB16_ROWS;
+    // the nine fluid and boundary row runs (rows y+dy, z+dz, r = 3 (dz+1) +
+    // (dy+1)), looked up once and kept in registers; empty: (0, 0)
+    int2 runF[9], runB[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        runF[r] = make_int2(0, 0);
+        runB[r] = make_int2(0, 0);
+        const int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
+        if (act && z >= 0 && z < G.nz && y >= 0 && y < G.ny) {
+            const uint32_t k0 = key_of(G, D.rowRank, cx, y, z);
+            const int2 rf = run3(D.cellF, k0, cx, G.nx);
+            const int2 rb = run3(D.cellB, k0, cx, G.nx);
+            if (rf.y > rf.x) runF[r] = rf;
+            if (rb.y > rb.x) runB[r] = rb;
+        }
+    }
     // per slot: the smallest run start and the largest run end of the tile
     if (packed && act) {
+#pragma unroll
         for (int r = 0; r < 9; ++r) {
-            const int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
-            if (z < 0 || z >= G.nz || y < 0 || y >= G.ny) continue;
-            const uint32_t k0 = key_of(G, D.rowRank, cx, y, z);
             const int slot = ord * 9 + r;
-            const int2 rf = run3(D.cellF, k0, cx, G.nx);
-            if (rf.y > rf.x) {
-                atomicMin(&sBase[slot], rf.x);
-                atomicMax(&sEnd[slot], rf.y);
+            if (runF[r].y > runF[r].x) {
+                atomicMin(&sBase[slot], runF[r].x);
+                atomicMax(&sEnd[slot], runF[r].y);
             }
-            const int2 rb = run3(D.cellB, k0, cx, G.nx);
-            if (rb.y > rb.x) {
-                atomicMin(&sBase[32 + slot], rb.x);
-                atomicMax(&sEnd[32 + slot], rb.y);
+            if (runB[r].y > runB[r].x) {
+                atomicMin(&sBase[32 + slot], runB[r].x);
+                atomicMax(&sEnd[32 + slot], runB[r].y);
             }
         }
     }
@@ -243,8 +254,10 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
         if (k < NB_STAGE_GROUPS * 4) st[((k >> 2) * 256 + tid) * 4 + (k & 3)] = j;
         else list[nb_slot(k, i, stride)] = j;
     };
-    auto put16 = [&](int k, int slot, int j) {
-        st16[((k >> 3) * 256 + tid) * 8 + (k & 7)] = uint16_t((slot << NB16_OFF_BITS) + (j - sBase[slot]));
+    auto put16 = [&](int4* list, int k, int slot, int j) {
+        const uint16_t e = uint16_t((slot << NB16_OFF_BITS) + (j - sBase[slot]));
+        if (k < NB_STAGE_GROUPS * 8) st16[((k >> 3) * 256 + tid) * 8 + (k & 7)] = e;
+        else reinterpret_cast<uint16_t*>(list)[nb16_slot(k, i, stride)] = e;
     };
     auto flush = [&](int* list, int count) {
         __syncthreads();
@@ -255,18 +268,17 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     };
     auto flush16 = [&](int4* out, int count) {
         __syncthreads();
-        const int ng = (count + 7) >> 3;
+        const int ng = min((count + 7) >> 3, NB_STAGE_GROUPS);
         for (int g = 0; g < ng; ++g) out[(size_t)g * stride + i] = stage[g][tid];
         __syncthreads();
     };
     int nf = 0, nbn = 0, over = 0, zero = 0;
     int stored = 0;   // fluid entries written (at most NBF_CAP)
     // fluid neighbours (SphBase.cpp:867-905)
+#pragma unroll
     for (int r = 0; r < 9; ++r) {
-        const int dz = r / 3 - 1, dy = r % 3 - 1;
-        const int z = cz + dz, y = cy + dy;
-        if (act && z >= 0 && z < G.nz && y >= 0 && y < G.ny) {
-            int2 rf = run3(D.cellF, key_of(G, D.rowRank, cx, y, z), cx, G.nx);
+        {
+            const int2 rf = runF[r];
             // candidates four at a time: four loads in flight, tested in order
             for (int j0 = rf.x; j0 < rf.y; j0 += 4) {
                 float4 q[4];
@@ -279,7 +291,7 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
                     float l2 = dis_square_exact(pa, q[k]);
                     if (l2 < D.h2) {
                         if (stored < NBF_CAP) {
-                            if (packed) put16(stored, ord * 9 + r, j);
+                            if (packed) put16(D.nbF16, stored, ord * 9 + r, j);
                             else put(D.nbF, stored, j);
                             ++stored;
                         } else {
@@ -295,21 +307,18 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     const int cf = act ? stored : 0;
     // pad: the particle itself
     for (int k = cf; k < ((cf + 3) & ~3); ++k) {
-        if (packed) put16(k, ord * 9 + 4, i);
+        if (packed) put16(D.nbF16, k, ord * 9 + 4, i);
         else put(D.nbF, k, i);
     }
     if (packed) flush16(D.nbF16, cf);
     else flush(D.nbF, cf);
     // boundary neighbours (SphBase.cpp:908-930)
     if (act) {
-        for (int dz = -1; dz <= 1; ++dz) {
-            int z = cz + dz;
-            if (z < 0 || z >= G.nz) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                int y = cy + dy;
-                if (y < 0 || y >= G.ny) continue;
-                const int slot = 32 + ord * 9 + (dz + 1) * 3 + (dy + 1);
-                int2 rb = run3(D.cellB, key_of(G, D.rowRank, cx, y, z), cx, G.nx);
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            {
+                const int slot = 32 + ord * 9 + r;
+                const int2 rb = runB[r];
                 for (int j0 = rb.x; j0 < rb.y; j0 += 4) {
                     float4 q[4];
 #pragma unroll
@@ -321,7 +330,7 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
                         float l2 = dis_square_exact(pa, q[k]);
                         if (l2 < D.h2) {
                             if (nbn < NBB_CAP) {
-                                if (packed) put16(nbn, slot, j);
+                                if (packed) put16(D.nbB16, nbn, slot, j);
                                 else put(D.nbB, nbn, j);
                             } else {
                                 ++over;
@@ -337,7 +346,11 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     const int cb = act ? min(nbn, NBB_CAP) : 0;
     // pad: the far dummy particle
     for (int k = cb; k < ((cb + 3) & ~3); ++k) {
-        if (packed) st16[((k >> 3) * 256 + tid) * 8 + (k & 7)] = uint16_t(NB16_DUMMY << NB16_OFF_BITS);
+        if (packed) {
+            const uint16_t e = uint16_t(NB16_DUMMY << NB16_OFF_BITS);
+            if (k < NB_STAGE_GROUPS * 8) st16[((k >> 3) * 256 + tid) * 8 + (k & 7)] = e;
+            else reinterpret_cast<uint16_t*>(D.nbB16)[nb16_slot(k, i, stride)] = e;
+        }
         else put(D.nbB, k, D.nb);
     }
     if (packed) flush16(D.nbB16, cb);
@@ -682,8 +695,8 @@ void launch_cell_build(const Dev& D, const uint32_t* keys, int n, int2* table, h
 
 void launch_neighbours_variant(const Dev& D, int groups, hipStream_t st) {
     const dim3 g(blocks_for(D.npad)), b(256);
-    if (groups >= 12) hipLaunchKernelGGL(k_neighbours<12>, g, b, 0, st, D);
-    else hipLaunchKernelGGL(k_neighbours<8>, g, b, 0, st, D);
+    if (groups >= 8) hipLaunchKernelGGL(k_neighbours<8>, g, b, 0, st, D);
+    else hipLaunchKernelGGL(k_neighbours<6>, g, b, 0, st, D);
 }
 
 void launch_neighbours(const Dev& D, hipStream_t st) { launch_neighbours_variant(D, D.nbStage, st); }

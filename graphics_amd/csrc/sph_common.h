@@ -73,7 +73,7 @@ struct Dev {
     float dt_max;
     float warm_scale;  // PREVIOUS warm start: scale of the previous stiffness
     int tileMap;       // 1: contiguous tile runs per XCD, 0: round robin (autotuned)
-    int nbStage;       // neighbour-list groups staged in LDS: 8 or 12 (autotuned)
+    int nbStage;       // neighbour-list groups staged in LDS: 6 or 8 (autotuned)
     int listNT;        // 1: passes read list groups with streaming loads (autotuned)
     float gx, gy, gz;
     float cs, gamma, nu, omega;

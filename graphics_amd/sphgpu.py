@@ -601,7 +601,7 @@ class Simulation:
         self._check(self.lib.sphgpu_get_tuning(self.h, ch, ms), "get_tuning")
         return {"tile_map_xcd": bool(ch[0]), "list_stage_groups": ch[1],
                 "ms": {"density_round_robin": ms[0], "density_xcd": ms[1],
-                       "lists_8": ms[2], "lists_12": ms[3]}}
+                       "lists_6": ms[2], "lists_8": ms[3]}}
 
     def list_form(self) -> dict:
         """Tiles holding particles, how many of them have packed (16-bit) lists,

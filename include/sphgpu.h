@@ -314,9 +314,9 @@ int sphgpu_profile_kernel(sphgpu_handle* h, const char* name);
 int sphgpu_copy_probe(long long bytes, double* ms);
 /* Choices of the autotuner, which runs before the first step on the scene
  * itself: choice[0] = tile mapping (1: contiguous tile runs per XCD, 0: round
- * robin), choice[1] = LDS staging depth of the list builder (8 or 12 groups);
+ * robin), choice[1] = LDS staging depth of the list builder (6 or 8 groups);
  * ms (optional, 4 floats) = best times of the candidates (mapping 0, 1;
- * depth 8, 12).  SPHGPU_E_STATE before the first step. */
+ * depth 6, 8).  SPHGPU_E_STATE before the first step. */
 int sphgpu_get_tuning(const sphgpu_handle* h, int* choice, float* ms);
 /* Form of the neighbour lists of the last step: *tiles = tiles holding
  * particles, *packed_tiles = those whose lists use 16-bit entries (the rest
