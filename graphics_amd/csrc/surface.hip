@@ -227,12 +227,27 @@ __global__ __launch_bounds__(256) void k_sf_splat(SurfParams S, const SurfRec* _
     const int r = blockIdx.x;
     if (r >= n) return;
     const SurfRec o = rec[r];
+    // The nodes with |G d| <= rh lie in the ellipsoid's bounding box, whose
+    // half-width along axis a is rh sqrt(sum_b (G^-1)_ab^2) (G symmetric);
+    // it is searched instead of the cube of half-width ext (the largest
+    // stretch along every axis), one node of margin added for rounding.
+    const float c00 = o.g11 * o.g22 - o.g12 * o.g12, c01 = o.g02 * o.g12 - o.g01 * o.g22;
+    const float c02 = o.g01 * o.g12 - o.g02 * o.g11, c11 = o.g00 * o.g22 - o.g02 * o.g02;
+    const float c12 = o.g01 * o.g02 - o.g00 * o.g12, c22 = o.g00 * o.g11 - o.g01 * o.g01;
+    const float dg = o.g00 * c00 + o.g01 * c01 + o.g02 * c02;   // det G
+    const float half[3] = {sqrtf(c00 * c00 + c01 * c01 + c02 * c02), sqrtf(c01 * c01 + c11 * c11 + c12 * c12),
+                           sqrtf(c02 * c02 + c12 * c12 + c22 * c22)};
     int lo[3], hi[3];
     const float xb[3] = {o.x, o.y, o.z};
     for (int a = 0; a < 3; ++a) {
         const int p = int((xb[a] - S.fmin[a]) / S.cell);
-        lo[a] = max(p - o.ext, 0);
-        hi[a] = min(p + o.ext + 1, S.fnum[a] - 1);
+        int e = o.ext;
+        if (dg > 0.0f) {
+            const float w = S.rh * half[a] / dg / S.cell;    // half-width in nodes
+            if (w < float(o.ext)) e = min(o.ext, int(w) + 2);
+        }
+        lo[a] = max(p - e, 0);
+        hi[a] = min(p + e + 1, S.fnum[a] - 1);
     }
     const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
     if (nx <= 0 || ny <= 0 || nz <= 0) return;
