@@ -254,8 +254,10 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
         if (k < NB_STAGE_GROUPS * 4) st[((k >> 2) * 256 + tid) * 4 + (k & 3)] = j;
         else list[nb_slot(k, i, stride)] = j;
     };
-    auto put16 = [&](int4* list, int k, int slot, int j) {
-        const uint16_t e = uint16_t((slot << NB16_OFF_BITS) + (j - sBase[slot]));
+    // bias(slot) = (slot << NB16_OFF_BITS) - base[slot]: an entry is j + bias
+    auto bias16 = [&](int slot) { return packed ? (slot << NB16_OFF_BITS) - sBase[slot] : 0; };
+    auto put16 = [&](int4* list, int k, int bias, int j) {
+        const uint16_t e = uint16_t(j + bias);
         if (k < NB_STAGE_GROUPS * 8) st16[((k >> 3) * 256 + tid) * 8 + (k & 7)] = e;
         else reinterpret_cast<uint16_t*>(list)[nb16_slot(k, i, stride)] = e;
     };
@@ -279,6 +281,7 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     for (int r = 0; r < 9; ++r) {
         {
             const int2 rf = runF[r];
+            const int bias = rf.y > rf.x ? bias16(ord * 9 + r) : 0;
             // candidates four at a time: four loads in flight, tested in order
             for (int j0 = rf.x; j0 < rf.y; j0 += 4) {
                 float4 q[4];
@@ -291,7 +294,7 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
                     float l2 = dis_square_exact(pa, q[k]);
                     if (l2 < D.h2) {
                         if (stored < NBF_CAP) {
-                            if (packed) put16(D.nbF16, stored, ord * 9 + r, j);
+                            if (packed) put16(D.nbF16, stored, bias, j);
                             else put(D.nbF, stored, j);
                             ++stored;
                         } else {
@@ -307,7 +310,7 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     const int cf = act ? stored : 0;
     // pad: the particle itself
     for (int k = cf; k < ((cf + 3) & ~3); ++k) {
-        if (packed) put16(D.nbF16, k, ord * 9 + 4, i);
+        if (packed) put16(D.nbF16, k, bias16(ord * 9 + 4), i);
         else put(D.nbF, k, i);
     }
     if (packed) flush16(D.nbF16, cf);
@@ -317,8 +320,8 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
             {
-                const int slot = 32 + ord * 9 + r;
                 const int2 rb = runB[r];
+                const int bias = rb.y > rb.x ? bias16(32 + ord * 9 + r) : 0;
                 for (int j0 = rb.x; j0 < rb.y; j0 += 4) {
                     float4 q[4];
 #pragma unroll
@@ -330,7 +333,7 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
                         float l2 = dis_square_exact(pa, q[k]);
                         if (l2 < D.h2) {
                             if (nbn < NBB_CAP) {
-                                if (packed) put16(D.nbB16, nbn, slot, j);
+                                if (packed) put16(D.nbB16, nbn, bias, j);
                                 else put(D.nbB, nbn, j);
                             } else {
                                 ++over;
