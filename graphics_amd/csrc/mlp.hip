@@ -39,9 +39,11 @@ __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }   // ro
 // c = sqrt(2/pi), evaluated as x sigma(2u) = x / (1 + exp(-2u)) (the same
 // function: one exponential and one hardware reciprocal per activation)
 __device__ __forceinline__ float gelu_sigma(float x) {
-    const float c = 0.7978845608028654f;
-    const float u = x * (c + (c * 0.044715f) * (x * x));
-    return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+    // exp(-2u) = exp2(t), t = x (k1 + k2 x^2), k1 = -2 c log2(e), k2 = 0.044715 k1
+    const float k1 = -2.0f * 0.7978845608028654f * 1.4426950408889634f;
+    const float k2 = 0.044715f * k1;
+    const float t = x * (k1 + k2 * (x * x));
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
 }
 __device__ __forceinline__ float gelu(float x) { return x * gelu_sigma(x); }
 // d/dx x sigma(2u) = sigma + 2 x sigma (1 - sigma) c (1 + 3 * 0.044715 x^2)
