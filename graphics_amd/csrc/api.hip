@@ -126,6 +126,7 @@ struct sphgpu_handle {
     unsigned long long graphClock = 0;
     bool capturing = false;
     bool tuned = false;
+    int tuneFix[3] = {-1, -1, -1};   // fixed tile map, staging depth, streaming lists (-1: timed)
     bool listsBuilt = false;   // neighbour lists built at least once
     float tuneMs[6] = {0, 0, 0, 0, 0, 0};   // density pass: map 0, 1; list builder: 6, 8 groups;
                                             // density pass: plain, streaming list loads
@@ -420,25 +421,39 @@ int autotune(sphgpu_handle* h) {
         else if (h->P.solver == SPHGPU_IISPH) launch_ii_density(D, h->st);
         else launch_wc_pressure(D, h->st);
     };
-    for (int m = 0; m < 2; ++m) {
-        D.tileMap = m;
-        h->tuneMs[m] = time_launches(h, density);
+    // each choice is timed unless sphgpu_set_tuning fixed it
+    if (h->tuneFix[0] >= 0) {
+        D.tileMap = h->tuneFix[0];
+    } else {
+        for (int m = 0; m < 2; ++m) {
+            D.tileMap = m;
+            h->tuneMs[m] = time_launches(h, density);
+        }
+        D.tileMap = h->tuneMs[0] < h->tuneMs[1] ? 0 : 1;
     }
-    D.tileMap = h->tuneMs[0] < h->tuneMs[1] ? 0 : 1;
-    const int depth[2] = {6, 8};
-    for (int k = 0; k < 2; ++k)
-        h->tuneMs[2 + k] = time_launches(h, [h, &depth, k]() {
-            launch_neighbours_variant(h->D, depth[k], h->st);
-        });
-    D.nbStage = h->tuneMs[2] <= h->tuneMs[3] ? 6 : 8;
+    if (h->tuneFix[1] >= 0) {
+        D.nbStage = h->tuneFix[1];
+    } else {
+        const int depth[2] = {6, 8};
+        for (int k = 0; k < 2; ++k)
+            h->tuneMs[2 + k] = time_launches(h, [h, &depth, k]() {
+                launch_neighbours_variant(h->D, depth[k], h->st);
+            });
+        D.nbStage = h->tuneMs[2] <= h->tuneMs[3] ? 6 : 8;
+    }
     launch_neighbours(D, h->st);   // lists of the chosen variant
-    // list loads: a fresh list build, then four passes (as in a step, where
-    // whether the lists stay in the last-level cache between passes decides)
-    for (int k = 0; k < 2; ++k) {
-        D.listNT = k;
-        h->tuneMs[4 + k] = time_launches(h, density, [h]() { launch_neighbours(h->D, h->st); }, 4);
+    if (h->tuneFix[2] >= 0) {
+        D.listNT = h->tuneFix[2];
+    } else {
+        // list loads: a fresh list build, then four passes (as in a step,
+        // where whether the lists stay in the last-level cache between
+        // passes decides)
+        for (int k = 0; k < 2; ++k) {
+            D.listNT = k;
+            h->tuneMs[4 + k] = time_launches(h, density, [h]() { launch_neighbours(h->D, h->st); }, 4);
+        }
+        D.listNT = h->tuneMs[4] <= h->tuneMs[5] ? 0 : 1;
     }
-    D.listNT = h->tuneMs[4] <= h->tuneMs[5] ? 0 : 1;
     h->tuned = true;
     HIPCHECK(h, hipStreamSynchronize(h->st));
     HIPCHECK(h, hipGetLastError());
@@ -1730,6 +1745,18 @@ int sphgpu_get_tuning(const sphgpu_handle* h, int* choice, float* ms) {
     choice[1] = h->D.nbStage;
     if (ms)
         for (int k = 0; k < 4; ++k) ms[k] = h->tuneMs[k];
+    return SPHGPU_OK;
+}
+
+int sphgpu_set_tuning(sphgpu_handle* h, int tile_map, int stage_groups, int stream_lists) {
+    if (!h) return SPHGPU_E_ARG;
+    if (h->tuned) return fail(h, SPHGPU_E_STATE, "tuning is chosen before the first step");
+    if (tile_map < -1 || tile_map > 1 || !(stage_groups == -1 || stage_groups == 6 || stage_groups == 8) ||
+        stream_lists < -1 || stream_lists > 1)
+        return fail(h, SPHGPU_E_ARG, "tile_map in {-1,0,1}, stage_groups in {-1,6,8}, stream_lists in {-1,0,1}");
+    h->tuneFix[0] = tile_map;
+    h->tuneFix[1] = stage_groups;
+    h->tuneFix[2] = stream_lists;
     return SPHGPU_OK;
 }
 

@@ -143,6 +143,7 @@ SIGNATURES = {
     "sphgpu_copy_probe": (C.c_int, [C.c_longlong, _DP]),
     "sphgpu_get_tuning": (C.c_int, [C.c_void_p, _IP, _FP]),
     "sphgpu_get_list_form": (C.c_int, [C.c_void_p, _IP, _IP, _IP]),
+    "sphgpu_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "sphgpu_mlp_create": (C.c_int, [C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
     "sphgpu_mlp_destroy": (None, [C.c_void_p]),
     "sphgpu_mlp_set_params": (C.c_int, [C.c_void_p, _FP]),
@@ -602,6 +603,12 @@ class Simulation:
         return {"tile_map_xcd": bool(ch[0]), "list_stage_groups": ch[1],
                 "ms": {"density_round_robin": ms[0], "density_xcd": ms[1],
                        "lists_6": ms[2], "lists_8": ms[3]}}
+
+    def set_tuning(self, tile_map=None, stage_groups=None, stream_lists=None):
+        """Fixes autotuner choices (None: timed at the first step)."""
+        f = lambda v: -1 if v is None else int(v)
+        self._check(self.lib.sphgpu_set_tuning(self.h, f(tile_map), f(stage_groups), f(stream_lists)),
+                    "set_tuning")
 
     def list_form(self) -> dict:
         """Tiles holding particles, how many of them have packed (16-bit) lists,

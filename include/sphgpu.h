@@ -318,6 +318,12 @@ int sphgpu_copy_probe(long long bytes, double* ms);
  * ms (optional, 4 floats) = best times of the candidates (mapping 0, 1;
  * depth 6, 8).  SPHGPU_E_STATE before the first step. */
 int sphgpu_get_tuning(const sphgpu_handle* h, int* choice, float* ms);
+/* Fixes autotuner choices instead of timing them (no counterpart in the
+ * reference; for reproducible set-ups and for tests): tile_map 0/1,
+ * stage_groups 6/8, stream_lists 0/1, -1 leaves a choice to the timing.
+ * Every choice gives bit-identical results.  Before the first step only
+ * (SPHGPU_E_STATE after). */
+int sphgpu_set_tuning(sphgpu_handle* h, int tile_map, int stage_groups, int stream_lists);
 /* Form of the neighbour lists of the last step: *tiles = tiles holding
  * particles, *packed_tiles = those whose lists use 16-bit entries (the rest
  * use 32-bit entries; both hold the same neighbours in the same order);

@@ -64,3 +64,23 @@ def test_steps_without_statistics_match(gpu, use_graph):
     assert st["num_fluid"] == a.num_fluid()
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("shape", ["cube", "long"])
+def test_tuning_choices_are_bit_identical(gpu, shape):
+    """Every autotuner choice (tile mapping, list staging depth, plain or
+    streaming list loads) gives bit-identical steps, on a scene with 32-bit
+    lists (cube) and one with packed lists (long tank)."""
+    kw = dict(n=(12, 12, 12)) if shape == "cube" else dict(n=(96, 6, 8), box=(2.0, 0.3, 0.3))
+    runs = []
+    for fix in [(0, 6, 0), (1, 8, 1), (1, 6, 1), (0, 8, 0)]:
+        sc = scenes.small_dam(sg.DFSPH, **kw)
+        sim = scenes.make_simulation(sc, warm_start=sg.WARM_PREVIOUS)
+        sim.set_tuning(*fix)
+        st = [sim.step() for _ in range(6)]
+        t = sim.tuning()
+        assert (int(t["tile_map_xcd"]), t["list_stage_groups"], int(sim.list_form()["streamed"])) == fix
+        runs.append((sim.get(sg.F_POS).tobytes(), sim.get(sg.F_VEL).tobytes(),
+                     [s["iter_density"] for s in st], [s["iter_divergence"] for s in st]))
+        sim.close()
+    assert all(r == runs[0] for r in runs[1:])
