@@ -366,8 +366,8 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
         else v = sBase[tid] != INT_MAX ? sBase[tid] - (slot << NB16_OFF_BITS) : 0;
         D.nbTab[(size_t)tile * NB16_TAB + tid] = v;
     }
-    // stored fluid entries | fluid neighbours found << 8 | boundary << 16
-    if (i < D.npad) D.cnt[i] = cf | (act ? min(nf, 255) : 0) << 8 | (cb << 16);
+    // stored fluid entries | fluid neighbours found << 8 | boundary << 16 | phase << 24
+    if (i < D.npad) D.cnt[i] = cf | (act ? min(nf, 255) : 0) << 8 | (cb << 16) | ((act ? int(pa.w) : 0) << 24);
     if (!act) return;
     int tm = nf + nbn - REF_NEIGHBOUR_CAP;
     if (tm > 0) atomicAdd(&D.counters[C_TOO_MANY], tm);
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(256) void k_neighbour_dump(Dev D, int cap, int* cou
     int o = rankOf[i];
     if (o < 0) return;
     const float4 pa = D.x[i];
-    const int cnt = D.cnt[i], nf = cnt & 0xFF, nbn = cnt >> 16;
+    const int cnt = D.cnt[i], nf = cnt & 0xFF, nbn = (cnt >> 16) & 0xFF;
     const int* tab = D.nbTab + (size_t)(i / TILE) * NB16_TAB;
     const bool packed = tab[NB16_FLAG] != 0;
     const uint16_t* f16 = reinterpret_cast<const uint16_t*>(D.nbF16);

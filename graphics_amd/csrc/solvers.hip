@@ -167,10 +167,12 @@ __device__ __forceinline__ void write_partial(const Dev& D, int tile, int q, dou
 }
 
 // packed counts: stored fluid entries | fluid neighbours found (the stored
-// ones plus any beyond the list capacity, up to 255) << 8 | boundary << 16
+// ones plus any beyond the list capacity, up to 255) << 8 | boundary << 16 |
+// the particle's phase << 24
 __device__ __forceinline__ int nf_of(int c) { return c & 0xff; }
 __device__ __forceinline__ int nf_real(int c) { return (c >> 8) & 0xff; }
-__device__ __forceinline__ int nb_of(int c) { return c >> 16; }
+__device__ __forceinline__ int nb_of(int c) { return (c >> 16) & 0xff; }
+__device__ __forceinline__ int phase_of_cnt(int c) { return c >> 24; }
 
 __device__ __forceinline__ bool inside_domain(const SceneGrid& G, float4 x) {
     return G.smin[0] <= x.x && G.smin[1] <= x.y && G.smin[2] <= x.z && x.x <= G.smax[0] &&
@@ -735,11 +737,11 @@ __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.xk[i];
-    int ph = phase_of(D.x[i]);
+    int c = D.cnt[i];
+    int ph = phase_of_cnt(c);   // (the phase from the count word: no read of x)
     float m = D.mass[ph], rho0 = D.rho0[ph];
     float ka = pa.w;
     float fx = 0.f, fy = 0.f, fz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
-    int c = D.cnt[i];
     each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, [&](int j, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * (ka + pb.w);
