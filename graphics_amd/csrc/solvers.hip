@@ -887,11 +887,13 @@ __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
     Slot s = slot_nb(D, true);
     if (!s.live) return;
     const int i = s.i;
-    float4 pa = D.x[i];
-    int ph = phase_of(pa);
-    float m = D.mass[ph];
+    // own position from the xk records the pass gathers (xyz = x during
+    // the loop), the phase from the count word: no read of x
+    const float4 pa = D.xk[i];
+    const int c = D.cnt[i];
+    float m = D.mass[phase_of_cnt(c)];
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    each_fluid_lu<8>(D, s, nf_of(D.cnt[i]), [&](int j) { return D.xk[j]; }, [&](int j, float4 pb) {
+    each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, [&](int j, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
         sx += dx * g; sy += dy * g; sz += dz * g;
