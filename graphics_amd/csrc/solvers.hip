@@ -562,7 +562,7 @@ __global__ __launch_bounds__(256) void k_df_aadv(Dev D) {
     });
     float dt = D.dt[s.scene];
     D.f4a[i] = make_float4(ax, ay, az, 0.0f);
-    D.vs[i] = make_float4(va.x + ax * dt, va.y + ay * dt, va.z + az * dt, 0.0f);
+    D.vs[i] = make_float4(va.x + ax * dt, va.y + ay * dt, va.z + az * dt, rhoa);   // w: rho, as in v
     D.ksum[i] = 0.0f;
 }
 
@@ -678,7 +678,7 @@ __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
         float m = D.mass[ph], rho0 = D.rho0[ph];
         float dd = density_change(D, s, pa, va, m, rho0, D.cnt[i]);
         float dt = D.dt[s.scene];
-        float rhoa = D.rho[i];
+        float rhoa = va.w;   // v* carries rho in w (as v does), one array fewer to stream
         float rs = rhoa + dt * dd;
         if (rs < rho0) rs = rho0;
         float dev = (rs - rho0) * D.alpha[i];     // kappa * dt^2
@@ -708,7 +708,7 @@ __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected
         float m = D.mass[ph], rho0 = D.rho0[ph];
         float dd = density_change(D, s, pa, va, m, rho0, D.cnt[i]);
         float dt = D.dt[s.scene];
-        float rhoa = D.rho[i];
+        float rhoa = va.w;   // v* carries rho in w (as v does), one array fewer to stream
         if (!projected) {
             if (dd < 0.0f || (rhoa + dd * dt) < rho0) dd = 0.0f;
             float kv = dd * D.alpha[i] / dt;
