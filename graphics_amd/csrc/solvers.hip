@@ -947,19 +947,13 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
         } else {
             contrib = ((double)radv + (double)aii * (double)pla + (double)fin - (double)rho0) / (double)rho0;
         }
+        // the new p_l, published in place: this pass gathers x and
+        // sum_j d_ij p_j of its neighbours, never their p_l
         D.f1[i] = t;
+        D.kr[i] = t;
+        D.xk[i].w = t / (rhoa * rhoa);
     }
     write_partial(D, s.tile, 0, contrib, lds);
-}
-
-// publish the new p_l of the Jacobi sweep (own value and the gather record)
-__global__ __launch_bounds__(256) void k_ii_publish(Dev D) {
-    Slot s = slot_of(D, true);
-    if (!s.live) return;
-    const int i = s.i;
-    float t = D.f1[i], r = D.rho[i];
-    D.kr[i] = t;
-    D.xk[i].w = t / (r * r);
 }
 
 // IISph_M::ii_computeForcePressure (IISph_M.cpp:478-684) fused with the
@@ -1284,10 +1278,7 @@ void launch_ii_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_density);
 void launch_ii_aadv_dii(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_aadv_dii); }
 void launch_ii_aii(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_aii); }
 void launch_ii_sumdij(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_sumdij); }
-void launch_ii_pressure(const Dev& D, hipStream_t st) {
-    SPH_LAUNCH(k_ii_pressure);
-    SPH_LAUNCH(k_ii_publish);
-}
+void launch_ii_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_pressure); }
 void launch_ii_force_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_force); }
 void launch_pci_rest_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_pci_rest_density); }
 void launch_pci_aext(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_pci_aext); }
