@@ -6,15 +6,19 @@ A step is one SphBase::runOneStep of the solver over the whole particle set
 divergence solver loops, integration).  Inputs are resident in HBM before the
 timed region.  Prints ONE JSON line on stdout; progress goes to stderr.
 
-Default workload: DFSPH dam break, 1,000,000 fluid particles at 0.01 m plus
-400,002 wall particles, CFL dt (0.4, dt_max 1e-3 s), density tolerance 0.01 %
-average, divergence tolerance 1 kg m^-3 s^-1 average.
+Default workload: DFSPH, two facing blocks of 126^3 fluid particles
+(4,000,752) at 0.01 m in a closed 6x3x3 m box, CFL dt (0.4, dt_max 1e-3 s),
+density tolerance 0.01 % average, divergence tolerance 1 kg m^-3 s^-1
+average, with the learned warm start trained online during the untimed
+steps; the plain solver and the previous-stiffness warm start run as A/B
+arms over the same steps.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -37,7 +41,7 @@ ALG_BYTES = {
     "ii_sumdij": 32, "ii_pressure": 56, "ii_aadv_dii": 64, "ii_aii": 48, "ii_force": 56,
     "pci_aext": 56, "pci_density": 44, "pci_pressure": 64, "pci_integrate": 84,
 }
-DOMINANT = {sg.DFSPH: "df_pass_velocity", sg.IISPH: "ii_pressure", sg.WCSPH: "wc_force",
+DOMINANT = {sg.DFSPH: "df_pass_divergence", sg.IISPH: "ii_pressure", sg.WCSPH: "wc_force",
             sg.PCISPH: "pci_pressure"}
 
 
@@ -150,7 +154,10 @@ def build_scene(name: str):
 
 
 def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, graph: int = 0,
-            stats_every: int = 1):
+            stats_every: int = 1, capture_at: int | None = None, label: str = ""):
+    """One arm: `warmup` untimed steps, then `steps` timed steps.  Returns
+    per-step statistics of both phases; with capture_at the fluid state after
+    that many warm-up steps (for the CPU baseline)."""
     sim = scenes.make_simulation(sc, warm_start=warm_start, use_graph=graph)
     if sc.rigids:
         # the host rigid-body step (Bullet's place in the reference) inside
@@ -164,15 +171,20 @@ def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, g
         sim.set_update_solids(update_solids)
     t0 = time.perf_counter()
     sim.init()
-    log(f"[bench] init {time.perf_counter() - t0:.2f}s  fluid={sc.num_fluid} boundary={sc.num_boundary}")
-    warm_times, warm_stats = [], []
+    log(f"[bench] {label} init {time.perf_counter() - t0:.2f}s  fluid={sc.num_fluid} boundary={sc.num_boundary}")
+    warm_times, warm_stats, captured = [], [], None
+    t_warm = time.perf_counter()
     for k in range(warmup):
         t = time.perf_counter()
         warm_stats.append(sim.step())
         warm_times.append(time.perf_counter() - t)
-        if k % 10 == 0:
+        if capture_at is not None and k + 1 == capture_at:
+            captured = dict(x=sim.get(sg.F_POS), v=sim.get(sg.F_VEL), ids=sim.ids(),
+                            phase=sim.get(sg.F_PHASE).astype(np.int64), step=k + 1)
+        if k % 25 == 0 or k == warmup - 1:
             s = warm_stats[-1]
-            log(f"[bench] warmup {k}: {warm_times[-1] * 1e3:.2f} ms it={s['iter_density']}/{s['iter_divergence']}")
+            log(f"[bench] {label} warmup {k}: {warm_times[-1] * 1e3:.2f} ms it={s['iter_density']}/{s['iter_divergence']}")
+    warm_elapsed = time.perf_counter() - t_warm
     if profile:
         sim.profile_kernel(profile)
     sim.synchronize()
@@ -190,27 +202,72 @@ def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, g
     tuning["lists"] = sim.list_form()
     sim.close()
     return dict(elapsed=elapsed, stats=stats, warm_times=warm_times, warm_stats=warm_stats,
-                timing=timing, n_live=n_live, tuning=tuning)
+                warm_elapsed=warm_elapsed, timing=timing, n_live=n_live, tuning=tuning, captured=captured)
 
 
-def cpu_baseline(sc, warm_start: int, budget_s: float):
-    from oracle import oracle as orc
+def arm_summary(run: dict, nf: int, steps: int, tol: float | None) -> dict:
+    """Throughput and solver statistics of one arm over its timed steps."""
+    st = run["stats"]
+    mean = lambda k: float(np.mean([s[k] for s in st]))
+    mx = lambda k: float(np.max([s[k] for s in st]))
+    out = {"particle_steps_per_s": round(nf * steps / run["elapsed"], 2),
+           "ms_per_step": round(run["elapsed"] / steps * 1e3, 4),
+           "iter_density": mean("iter_density"), "iter_divergence": mean("iter_divergence"),
+           "max_iter_hit_rate": mean("max_iter_hit"),
+           "density_err_mean": mean("density_err"), "density_err_max": mx("density_err"),
+           "divergence_err_mean": mean("divergence_err"), "divergence_err_max": mx("divergence_err"),
+           "rho_mean_err": mean("rho_mean_err"),
+           "too_many_neigb": int(sum(s["too_many_neigb"] for s in st)),
+           "list_overflow": int(sum(s["list_overflow"] for s in st)),
+           "zero_dis": int(sum(s["zero_dis"] for s in st)), "removed": int(sum(s["removed"] for s in st)),
+           "warmup_s": round(run["warm_elapsed"], 3)}
+    if tol is not None:
+        out["density_err_within_tol"] = bool(out["density_err_max"] <= tol)
+    return out
+
+
+def cpu_threads() -> int:
+    """Host threads the CPU baseline may use: the cores this process may run
+    on (omp_get_num_procs() of the reference, SphBase.cpp:242-243), within
+    OMP_NUM_THREADS when the environment sets a per-GPU share."""
     try:
         ncpu = len(os.sched_getaffinity(0))
     except AttributeError:
         ncpu = os.cpu_count() or 1
-    threads = max(1, min(16, ncpu))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(ncpu, int(omp))) if omp.isdigit() and int(omp) > 0 else ncpu
+
+
+def cpu_baseline(sc, state, budget_s: float):
+    """The FP32 oracle (the reference's loop structure and algorithm, OpenMP)
+    built for this host's CPU, timed on the same workload: from the GPU's
+    state after `state['step']` steps where one is given, else from the
+    initial state; as many steps as fit in budget_s (at least one)."""
+    import tempfile
+    from oracle import oracle as orc
+    isa = "native"
+    try:
+        orc.build_native(tempfile.mkdtemp(prefix="oracle_native_"))
+    except (OSError, subprocess.CalledProcessError, RuntimeError) as e:
+        isa = f"x86-64-v3 (native build failed: {e})"
+    threads = cpu_threads()
     orc.set_threads(threads)
-    o = orc.from_scene(sc, 32, warm_start=warm_start)
-    steps, t0 = 0, time.perf_counter()
-    per_step = []
+    # the reference algorithm: no warm start
+    if state is not None:
+        o = orc.from_state(sc, state["x"], state["v"], state["ids"], state["phase"], 32, warm_start=0)
+        first = state["step"]
+    else:
+        o = orc.from_scene(sc, 32, warm_start=0)
+        first = 0
+    per_step, iters = [], []
+    t0 = time.perf_counter()
     while True:
         t = time.perf_counter()
-        o.step()
+        s = o.step()
         per_step.append(time.perf_counter() - t)
-        steps += 1
-        log(f"[bench] cpu step {steps - 1}: {per_step[-1]:.2f}s")
-        if time.perf_counter() - t0 >= budget_s or steps >= 50:
+        iters.append((s["iter_density"], s["iter_divergence"]))
+        log(f"[bench] cpu step {first + len(per_step) - 1}: {per_step[-1]:.2f}s it={iters[-1]}")
+        if time.perf_counter() - t0 >= budget_s or len(per_step) >= 50:
             break
     elapsed = time.perf_counter() - t0
     model = "unknown"
@@ -222,7 +279,8 @@ def cpu_baseline(sc, warm_start: int, budget_s: float):
     except OSError:
         pass
     o.close()
-    return dict(steps=steps, elapsed=elapsed, threads=threads, model=model, per_step=per_step)
+    return dict(steps=len(per_step), elapsed=elapsed, threads=threads, model=model, per_step=per_step,
+                first=first, isa=isa, iters=iters)
 
 
 def surface_bench(args) -> int:
@@ -279,18 +337,22 @@ def surface_bench(args) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="dfsph_c2", choices=sorted(scenes.WORKLOADS) + ["surface_dam"])
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps before the timed ones (default: the workload's learn_steps, "
+                         "so the timed steps run the trained model, at least 20)")
+    ap.add_argument("--workload", default="dfsph_c3", choices=sorted(scenes.WORKLOADS) + ["surface_dam"])
     ap.add_argument("--warm-start", default="learned", choices=["none", "previous", "learned"])
     ap.add_argument("--no-ab", action="store_true", help="skip the plain-solver arm")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=25.0)
+    ap.add_argument("--cpu-from-step", type=int, default=10,
+                    help="the CPU baseline starts from the GPU plain arm's state after this many steps")
     ap.add_argument("--profile-kernel", default=None)
     ap.add_argument("--stats-every", type=int, default=1,
                     help="read statistics back every N timed steps (the others do not wait for the device)")
     ap.add_argument("--graph", action="store_true",
-                    help="replay captured HIP graphs of the WCSPH / IISPH steps (no per-kernel timing)")
+                    help="replay captured HIP graphs of whole steps (no per-kernel timing)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COMMAND"),
                     help="write profiles/<workload>_traffic.txt from two rocprofv3 --pmc runs")
     ap.add_argument("--summarize-trace", nargs=2, metavar=("TRACE_CSV", "COMMAND"),
@@ -302,15 +364,17 @@ def main() -> int:
         return summarize_pmc(*args.summarize_pmc[:2], wname(), args.summarize_pmc[2])
     if args.summarize_trace:
         return summarize_trace(None, args.summarize_trace[0], wname(), args.summarize_trace[1])
-    if args.workload == "surface_dam":
-        if args.gpus != 1 or not sg.device_available():
-            print("surface_dam needs exactly one gfx9 device", file=sys.stderr)
-            return 2
-        return surface_bench(args)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no "
-              "multi-GPU path", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path runs on one "
+              "GPU's share; sharding across GPUs is a design, not built", file=sys.stderr)
         return 2
+    if args.workload == "surface_dam":
+        if not sg.device_available():
+            print("surface_dam needs one gfx9 device", file=sys.stderr)
+            return 3
+        if args.warmup is None:
+            args.warmup = 300
+        return surface_bench(args)
     if not sg.device_available():
         print("no gfx9 HIP device visible", file=sys.stderr)
         return 3
@@ -324,19 +388,29 @@ def main() -> int:
     sc = build_scene(args.workload)
     solver = sc.params["solver"]
     warm = {"none": sg.WARM_NONE, "previous": sg.WARM_PREVIOUS, "learned": sg.WARM_LEARNED}[args.warm_start]
+    if warm == sg.WARM_LEARNED and solver not in (sg.DFSPH, sg.IISPH):
+        warm = sg.WARM_PREVIOUS
+    learn_steps = int(sc.params.get("learn_steps", 40))
+    if args.warmup is None:
+        args.warmup = max(20, learn_steps if warm == sg.WARM_LEARNED else 20)
+    if warm == sg.WARM_LEARNED and args.warmup < learn_steps:
+        log(f"[bench] note: warmup {args.warmup} < learn_steps {learn_steps}: timed steps include training")
     prof = args.profile_kernel or DOMINANT[solver]
+    tol = sc.params.get("density_tol")
+    labels = {sg.WARM_NONE: "plain", sg.WARM_PREVIOUS: "previous", sg.WARM_LEARNED: "learned"}
+    # the CPU baseline starts from the plain arm's state (the reference algorithm)
+    want_cpu = not args.no_cpu and sc.num_scenes == 1
+    cap_main = args.cpu_from_step if (want_cpu and warm == sg.WARM_NONE and not sc.rigids
+                                      and args.cpu_from_step <= args.warmup) else None
     run = run_gpu(sc, warm, args.steps, args.warmup, None if args.graph else prof, int(args.graph),
-                  max(1, args.stats_every))
+                  max(1, args.stats_every), capture_at=cap_main, label=labels[warm])
     nf = sc.num_fluid
-    stats = run["stats"]
     value = nf * args.steps / run["elapsed"]
     ms = run["elapsed"] / args.steps * 1e3
-    it_d = float(np.mean([s["iter_density"] for s in stats]))
-    it_v = float(np.mean([s["iter_divergence"] for s in stats]))
-    dens_err = float(max(s["density_err"] for s in stats))
-    rho_mean = float(np.mean([s["rho_mean_err"] for s in stats]))
-    log(f"[bench] timed {args.steps} steps: {ms:.3f} ms/step, {value:.4g} particle-steps/s, "
-        f"iters {it_d:.2f}/{it_v:.2f}, max density err {dens_err:.3g}")
+    main_sum = arm_summary(run, nf, args.steps, tol)
+    log(f"[bench] {labels[warm]}: {ms:.3f} ms/step, {value:.4g} particle-steps/s, "
+        f"iters {main_sum['iter_density']:.2f}/{main_sum['iter_divergence']:.2f}, "
+        f"max density err {main_sum['density_err_max']:.3g}")
 
     launches, tot_ms, tot_bytes = run["timing"]
     roofline = None
@@ -353,39 +427,47 @@ def main() -> int:
                     "traffic_achieved_GBs": round(tr["traffic_bytes"] / avg_s / 1e9, 1) if tr else None,
                     "copy_probe_GBs": round(copy_gbs, 1)}
 
-    iters = {"warm_start": args.warm_start, "density": it_d, "divergence": it_v}
+    arms = {labels[warm]: main_sum}
+    captured = run["captured"]
     if not args.no_ab and warm != sg.WARM_NONE:
-        # same workload, same steps: the plain solver (the reference), and for
-        # the learned guess also the previous-stiffness guess it competes with
-        ke = np.array([s["kinetic_energy"] for s in run["warm_stats"] + stats])
-        arms = [("plain", sg.WARM_NONE)] + ([("previous", sg.WARM_PREVIOUS)] if warm == sg.WARM_LEARNED else [])
-        for label, w in arms:
-            arm = run_gpu(sc, w, args.steps, args.warmup, None)
-            ps = arm["stats"]
-            ke_a = np.array([s["kinetic_energy"] for s in arm["warm_stats"] + ps])
-            iters[label] = {"density": float(np.mean([s["iter_density"] for s in ps])),
-                            "divergence": float(np.mean([s["iter_divergence"] for s in ps])),
-                            "particle_steps_per_s": nf * args.steps / arm["elapsed"],
-                            "max_kinetic_energy_dev": float(np.abs(ke - ke_a).max() / max(ke_a.max(), 1e-30))}
-            log(f"[bench] {label} arm: iters {iters[label]['density']:.2f}/{iters[label]['divergence']:.2f}")
-        tot_w = it_d + it_v
-        tot_p = iters["plain"]["density"] + iters["plain"]["divergence"]
-        iters["reduction_vs_plain"] = round(1.0 - tot_w / tot_p, 4) if tot_p else None
-        pv = iters["plain"]["divergence"]
-        iters["divergence_reduction_vs_plain"] = round(1.0 - it_v / pv, 4) if pv else None
+        # same workload, same steps: the plain solver (the reference) and, for
+        # the learned guess, the previous-stiffness guess it competes with
+        ke = np.array([s["kinetic_energy"] for s in run["warm_stats"] + run["stats"]])
+        others = [sg.WARM_NONE] + ([sg.WARM_PREVIOUS] if warm == sg.WARM_LEARNED else [])
+        for w in others:
+            cap = args.cpu_from_step if (w == sg.WARM_NONE and want_cpu and not sc.rigids
+                                         and args.cpu_from_step <= args.warmup) else None
+            arm = run_gpu(sc, w, args.steps, args.warmup, None, capture_at=cap, label=labels[w])
+            if cap is not None:
+                captured = arm["captured"]
+            a = arm_summary(arm, nf, args.steps, tol)
+            ke_a = np.array([s["kinetic_energy"] for s in arm["warm_stats"] + arm["stats"]])
+            a["max_kinetic_energy_dev"] = float(np.abs(ke - ke_a).max() / max(ke_a.max(), 1e-30))
+            arms[labels[w]] = a
+            log(f"[bench] {labels[w]} arm: {a['ms_per_step']:.3f} ms/step, iters "
+                f"{a['iter_density']:.2f}/{a['iter_divergence']:.2f}, cap hit {a['max_iter_hit_rate']:.2f}")
+    iters = {"warm_start": labels[warm], "density": main_sum["iter_density"],
+             "divergence": main_sum["iter_divergence"]}
+    if "plain" in arms and warm != sg.WARM_NONE:
+        tot = lambda a: a["iter_density"] + a["iter_divergence"]
+        iters["reduction_vs_plain"] = round(1.0 - tot(main_sum) / tot(arms["plain"]), 4)
+        pv = arms["plain"]["iter_divergence"]
+        iters["divergence_reduction_vs_plain"] = round(1.0 - main_sum["iter_divergence"] / pv, 4) if pv else None
+        if "previous" in arms and warm == sg.WARM_LEARNED:
+            iters["reduction_vs_previous"] = round(1.0 - tot(main_sum) / tot(arms["previous"]), 4)
 
     cpu = None
-    if not args.no_cpu:
-        # the oracle has no learned model: its divergence solve warm-starts from the previous step
-        c = cpu_baseline(sc, min(warm, sg.WARM_PREVIOUS), args.cpu_seconds)
+    if want_cpu:
+        c = cpu_baseline(sc, captured, args.cpu_seconds)
         cpu_rate = nf * c["steps"] / c["elapsed"]
-        gpu_same = nf * c["steps"] / sum(run["warm_times"][:c["steps"]]) if len(run["warm_times"]) >= c["steps"] else None
+        start = (f"the GPU plain arm's state after step {c['first']}" if c["first"] else "the initial state")
         cpu = {"value": round(cpu_rate, 2), "unit": "particle-steps/s", "cores": c["threads"],
                "kind": "port",
-               "sample": (f"steps 0-{c['steps'] - 1} of the same workload from the same initial state, "
-                          f"FP32 oracle (reference loop structure, OpenMP) on {c['model']}"),
-               "gpu_same_sample": round(gpu_same, 2) if gpu_same else None,
-               "speedup_same_sample": round(gpu_same / cpu_rate, 1) if gpu_same else None}
+               "sample": (f"steps {c['first']}-{c['first'] + c['steps'] - 1} of the same workload from {start}, "
+                          f"FP32 oracle (the reference's loop structure and algorithm, no warm start; OpenMP, "
+                          f"-march={c['isa']}) on {c['model']}"),
+               "iters_per_step": [list(t) for t in c["iters"]],
+               "speedup": round(value / cpu_rate, 1)}
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "particle-steps/s", "n_gpus": 1,
@@ -394,12 +476,15 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": sc.name, "solver": ["WCSPH", "DFSPH", "IISPH", "PCISPH"][solver],
                    "fluid": nf, "boundary": sc.num_boundary, "scenes": sc.num_scenes,
-                   "density_tol": sc.params.get("density_tol"),
-                   "divergence_tol": sc.params.get("divergence_tol"),
-                   "max_density_err": dens_err, "mean_rho_err": rho_mean,
+                   "density_tol": tol, "divergence_tol": sc.params.get("divergence_tol"),
+                   "max_iter_density": sc.params.get("max_iter_density"),
+                   "max_iter_div": sc.params.get("max_iter_div"),
+                   "learn_steps": learn_steps if warm == sg.WARM_LEARNED else None,
+                   "density_err_within_tol": main_sum.get("density_err_within_tol"),
                    "live_after": run["n_live"], "hip_graph": bool(args.graph),
                    "stats_every": max(1, args.stats_every)},
         "iters_per_step": iters,
+        "arms": arms,
         "autotune": run["tuning"],
         "roofline": roofline,
         "cpu_baseline": cpu,

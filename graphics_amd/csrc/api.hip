@@ -291,7 +291,7 @@ void launch_iteration(sphgpu_handle* h, int which, int it) {
                     launch_df_pass_velocity(D, h->st);
                 }
                 launch_scene_reduce(D, 0, P.density_tol, P.min_iter_density,
-                                    P.max_iter_density > 0 ? P.max_iter_density : 1 << 30, h->st);
+                                    P.max_iter_density, h->st);
             } else {
                 {
                     ProfScope ps(h, "df_pass_divergence", nf_live(h) * 36.0);
@@ -313,7 +313,7 @@ void launch_iteration(sphgpu_handle* h, int which, int it) {
                 launch_pci_pressure(D, h->st);
             }
             launch_scene_reduce(D, 2, P.density_tol, P.min_iter_density,
-                                P.max_iter_density > 0 ? P.max_iter_density : 1 << 30, h->st);
+                                P.max_iter_density, h->st);
         } else {  // IISPH
             {
                 ProfScope ps(h, "ii_sumdij", nf_live(h) * 32.0);
@@ -324,7 +324,7 @@ void launch_iteration(sphgpu_handle* h, int which, int it) {
                 launch_ii_pressure(D, h->st);
             }
             launch_scene_reduce(D, 1, P.density_tol, P.min_iter_density,
-                                P.max_iter_density > 0 ? P.max_iter_density : 1 << 30, h->st);
+                                P.max_iter_density, h->st);
         }
     }
 }
@@ -882,6 +882,10 @@ int sphgpu_create(const sphgpu_params* p, sphgpu_handle** out) {
         return SPHGPU_E_ARG;
     if (!(p->warm_scale >= 0.0f) || !(p->warm_scale <= 4.0f)) return SPHGPU_E_ARG;
     if (p->use_graph < 0 || p->use_graph > 1) return SPHGPU_E_ARG;
+    // every solver loop ends at a finite cap (and reports reaching it)
+    if (p->max_iter_density < 1 || p->max_iter_density > SPHGPU_MAX_ITER || p->max_iter_div < 0 ||
+        p->max_iter_div > SPHGPU_MAX_ITER || p->min_iter_density < 0 || p->min_iter_div < 0)
+        return SPHGPU_E_ARG;
     sphgpu_handle* h = new sphgpu_handle();
     h->P = *p;
     h->scenes.resize(p->num_scenes);

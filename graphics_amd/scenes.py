@@ -225,12 +225,16 @@ def wcsph_c1(seed=1001) -> Scene:
 
 
 def dfsph_c2(seed=1002, n=100) -> Scene:
-    """DFSPH dam break: n^3 fluid (1,000,000 at n=100) at 0.01 m in a closed 4x2x2 m box."""
+    """DFSPH dam break: n^3 fluid (1,000,000 at n=100) at 0.01 m in a closed
+    4x2x2 m box.  Density tolerance 0.01 % average, divergence tolerance 0.1 %
+    of rho0 per second; both loops may run 100 iterations (the reference breaks
+    the divergence loop after 30, DFSPH.cpp:772, where the plain solve has
+    mostly not reached this tolerance yet)."""
     s = 0.01
     p = dict(solver=sg.DFSPH, spacing_r=s, smooth_radius_h=2 * s, dt=1e-3, cfl=0.4, dt_min=1e-5,
              gravity=(0.0, -9.81, 0.0), sound_speed=20.0, nu=1e-3,
              density_tol=1e-4, divergence_tol=1.0, min_iter_density=2, max_iter_density=100,
-             min_iter_div=1, max_iter_div=30)
+             min_iter_div=1, max_iter_div=100)
     sc = Scene(f"dfsph_dam_{n ** 3}", p)
     sc.phases.append((1000.0, 0.01))
     _tank(sc, 0, (4.0, 2.0, 2.0), s, True, 0.01)
@@ -258,10 +262,13 @@ def pcisph_dam(seed=1002, n=100) -> Scene:
 
 
 def dfsph_c3(seed=1003, n=126) -> Scene:
-    """DFSPH, two facing n^3 blocks (4,000,752 at n=126) at 0.01 m in a closed 6x3x3 m box."""
+    """DFSPH, two facing n^3 blocks (4,000,752 at n=126) at 0.01 m in a closed
+    6x3x3 m box, tolerances as dfsph_c2; the learned warm start trains on the
+    first 200 steps."""
     s = 0.01
     sc = dfsph_c2(seed, 1)
     sc.name = f"dfsph_two_blocks_{2 * n ** 3}"
+    sc.params["learn_steps"] = 200
     sc.solids.clear()
     sc.fluids.clear()
     sc.domains.clear()

@@ -31,6 +31,7 @@ extern "C" {
 #endif
 
 #define SPHGPU_ABI_VERSION 1
+#define SPHGPU_MAX_ITER 10000   /* largest accepted solver-loop cap */
 
 enum sphgpu_status {
     SPHGPU_OK = 0,
@@ -74,9 +75,9 @@ typedef struct sphgpu_params {
     float density_tol;         /* DFSPH avg (rho*-rho0)/rho0 (DFSPH.cpp:225); IISPH eta/rho0 (IISph_M.cpp:285) */
     float divergence_tol;      /* DFSPH avg Drho/Dt in kg m^-3 s^-1 (DFSPH.cpp:762) */
     int   min_iter_density;    /* 2 (DFSPH.cpp:229, IISph_M.cpp:286) */
-    int   max_iter_density;    /* cap (the reference DFSPH loop has none) */
+    int   max_iter_density;    /* cap, 1..SPHGPU_MAX_ITER (the reference DFSPH loop has none) */
     int   min_iter_div;        /* 1 (DFSPH.cpp:765) */
-    int   max_iter_div;        /* 30: the loop breaks when it > max (DFSPH.cpp:772) */
+    int   max_iter_div;        /* 30: the loop breaks when it > max (DFSPH.cpp:772); 0..SPHGPU_MAX_ITER */
     float omega;               /* IISPH relaxation, 0.5 (IISph_M.cpp:341) */
     int   warm_start;          /* sphgpu_warm_start */
     int   num_scenes;          /* >= 1 independent scenes advanced together */

@@ -16,8 +16,9 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
 WCSPH, DFSPH, IISPH, PCISPH = 0, 1, 2, 3
-F_POS, F_VEL, F_RHO, F_PRESSURE, F_ACC, F_ALPHA, F_KAPPA, F_ID, F_AADV, F_KAPPA_SUM, F_AII, F_RHO_ADV = range(12)
-COMPONENTS = {F_POS: 3, F_VEL: 3, F_ACC: 3, F_AADV: 3}
+(F_POS, F_VEL, F_RHO, F_PRESSURE, F_ACC, F_ALPHA, F_KAPPA, F_ID, F_AADV, F_KAPPA_SUM, F_AII, F_RHO_ADV,
+ F_KAPPAV, F_DIVDEV, F_VADV) = range(15)
+COMPONENTS = {F_POS: 3, F_VEL: 3, F_ACC: 3, F_AADV: 3, F_VADV: 3}
 _UPDATE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)      # int (*)(void* user)
 
 
@@ -70,6 +71,7 @@ class OrcStats(C.Structure):
 
 
 _lib = None
+_lib_path = None       # another build of the oracle to load instead (build_native)
 
 
 def build() -> Path:
@@ -77,12 +79,28 @@ def build() -> Path:
     return LIB_PATH
 
 
+def build_native(out_dir) -> Path:
+    """Compiles the oracle for the CPU it runs on (-march=native) into
+    out_dir and makes load() use that build.  For the timed CPU baseline: the
+    in-tree library is built for a portable ISA level in another container."""
+    global _lib_path
+    if _lib is not None:
+        raise RuntimeError("build_native must run before the oracle is first loaded")
+    out = Path(out_dir) / "liboracle_native.so"
+    subprocess.run(["g++", "-O3", "-march=native", "-fopenmp", "-ffp-contract=off", "-fPIC", "-std=c++17",
+                    "-shared", "-o", str(out), str(HERE / "sph_oracle.cpp"), str(HERE / "surface_oracle.cpp")],
+                   check=True)
+    _lib_path = out
+    return out
+
+
 def load():
     global _lib
     if _lib is None:
-        if not LIB_PATH.exists():
+        path = _lib_path or LIB_PATH
+        if _lib_path is None and not LIB_PATH.exists():
             build()
-        lib = C.CDLL(str(LIB_PATH))
+        lib = C.CDLL(str(path))
         FP, IP, DP = C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_double)
         lib.orc_create.restype = C.c_void_p
         lib.orc_create.argtypes = [C.c_int, C.POINTER(OrcParams)]
@@ -101,6 +119,7 @@ def load():
         lib.orc_num_fluid.argtypes = [C.c_void_p, C.c_int]
         lib.orc_get_fluid.argtypes = [C.c_void_p, C.c_int, C.c_int, DP]
         lib.orc_get_boundary_volume.argtypes = [C.c_void_p, C.c_int, DP]
+        lib.orc_set_fluid_state.argtypes = [C.c_void_p, C.c_int, DP, DP]
         lib.orc_neighbours.argtypes = [C.c_void_p, C.c_int, C.c_int, IP, IP, IP, FP]
         lib.orc_compute_density.argtypes = [C.c_void_p]
         lib.orc_kernel_eval.argtypes = [C.c_int, C.c_double, C.c_int, DP, DP, DP]
@@ -250,6 +269,17 @@ class Oracle:
         self.lib.orc_get_fluid(self.h, k, field, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out if comp > 1 else out[:, 0]
 
+    def set_state(self, xyz, vel=None, k=0):
+        """Positions (and velocities) of fluid k in its current order, at the
+        oracle's precision."""
+        x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        assert len(x) == self.num_fluid(k)
+        v = None if vel is None else np.ascontiguousarray(vel, np.float64).reshape(-1, 3)
+        DP = C.POINTER(C.c_double)
+        rc = self.lib.orc_set_fluid_state(self.h, k, x.ctypes.data_as(DP), None if v is None else v.ctypes.data_as(DP))
+        if rc < 0:
+            raise ValueError("set_state failed")
+
     def get_by_id(self, field, k=None):
         """Field values in ascending original id; k=None: all fluids (phases)."""
         ks = range(self.nfluid_phases) if k is None else [k]
@@ -352,5 +382,27 @@ def from_scene(sc, bits=64, **overrides) -> Oracle:
     for s, xyz, visc, pose in getattr(sc, "rigids", []):
         k = o.add_solid(xyz, visc, dynamic=True)
         o.set_solid_pose(k, *pose)
+    o.init()
+    return o
+
+
+def from_state(sc, xyz, vel, ids, phase, bits=64, **overrides) -> Oracle:
+    """Oracle for a single-scene Scene whose fluid is replaced by a given
+    state (positions, velocities, original ids and phase per particle, e.g.
+    read back from the GPU after some steps); walls and solids as in the
+    scene.  Rigid bodies are not supported here."""
+    assert sc.num_scenes == 1 and not getattr(sc, "rigids", []), "single scene without rigid bodies"
+    p = dict(sc.params)
+    p.update(overrides)
+    p["space_min"], p["space_max"] = sc.domains[0]
+    o = Oracle(p, bits)
+    phase = np.asarray(phase).astype(np.int64)
+    for ph in sorted(set(phase.tolist())):
+        sel = phase == ph
+        rho0, visc = sc.phases[ph]
+        o.add_fluid(np.asarray(xyz)[sel], rho0, visc, vel=np.asarray(vel)[sel],
+                    ids=np.asarray(ids)[sel].astype(np.int32))
+    for s, w, visc in sc.solids:
+        o.add_solid(w, visc)
     o.init()
     return o

@@ -26,6 +26,13 @@
  *     force on dynamic solids (2 p_b/rho_b^2 at IISph_M.cpp:596);
  *   - IISPH phases are solved jointly (the reference iterates phase by phase
  *     with stale cross-phase pressures, IISph_M.cpp:85);
+ *   - IISPH convergence error: the mean over all particles of
+ *     (rho_adv + a_ii p + sum_j ...)/rho0 - 1 where p is not clamped, and 0
+ *     where it is.  The reference sums the predicted density of the
+ *     unclamped particles only, divides by all of them and subtracts rho0
+ *     (IISph_M.cpp:345-354), so every clamped particle counts as an error of
+ *     -rho0 and a free surface ends the loop at once; without clamped
+ *     particles the two are the same number;
  *   - a zero distance pair is counted once per neighbour search and
  *     contributes W(0) to densities and nothing to gradient terms (the
  *     reference counts it per force evaluation, WcSph.cpp:160, and divides
@@ -1608,10 +1615,26 @@ int orc_get_fluid(const orc_sim* s, int k, int field, double* out) {
                 case ORC_F_KAPPA_SUM: out[i] = q.kappa_sum; break;
                 case ORC_F_AII: out[i] = q.aii; break;
                 case ORC_F_RHO_ADV: out[i] = q.rho_adv; break;
+                case ORC_F_KAPPAV: out[i] = q.kappaV; break;
+                case ORC_F_DIVDEV: out[i] = q.divdev; break;
+                case ORC_F_VADV: out[3 * i] = q.v_adv.x; out[3 * i + 1] = q.v_adv.y; out[3 * i + 2] = q.v_adv.z; break;
                 default: return -2;
             }
         }
         return int(n);
+    });
+}
+
+int orc_set_fluid_state(orc_sim* s, int k, const double* xyz, const double* vel) {
+    return dispatch(s, [&](auto& S) {
+        using T = std::decay_t<decltype(S.r)>;
+        if (k < 0 || k >= int(S.fluids.size()) || !xyz) return -1;
+        auto& ps = S.fluids[k].parts;
+        for (size_t i = 0; i < ps.size(); ++i) {
+            ps[i].x = V3<T>(T(xyz[3 * i]), T(xyz[3 * i + 1]), T(xyz[3 * i + 2]));
+            if (vel) ps[i].v = V3<T>(T(vel[3 * i]), T(vel[3 * i + 1]), T(vel[3 * i + 2]));
+        }
+        return int(ps.size());
     });
 }
 

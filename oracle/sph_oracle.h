@@ -75,7 +75,10 @@ enum {
     ORC_F_AADV = 8,     /* 3  non-pressure acceleration */
     ORC_F_KAPPA_SUM = 9,/* 1  accumulated kappa*dt^2 of the density solve */
     ORC_F_AII = 10,     /* 1  IISPH a_ii */
-    ORC_F_RHO_ADV = 11  /* 1  IISPH rho_adv */
+    ORC_F_RHO_ADV = 11, /* 1  IISPH rho_adv; DFSPH rho* of the last density iteration */
+    ORC_F_KAPPAV = 12,  /* 1  DFSPH kappaV of the last divergence iteration */
+    ORC_F_DIVDEV = 13,  /* 1  DFSPH clamped Drho/Dt of the last divergence iteration */
+    ORC_F_VADV = 14     /* 3  DFSPH v* / IISPH v_adv */
 };
 
 typedef struct orc_sim orc_sim;
@@ -100,6 +103,10 @@ int      orc_step_cb(orc_sim* s, orc_stats* st, int (*cb)(void*), void* user);
 int      orc_step_dt(const orc_sim* s, double* dt);     /* dt of the current (or last) step */
 int      orc_pci_delta(const orc_sim* s, double* delta); /* PCISPH delta (before / dt^2) */
 int      orc_num_fluid(const orc_sim* s, int k);
+/* Overwrites the positions (and, with vel != NULL, the velocities) of fluid
+ * k's particles in their current order, at the oracle's precision (no FP32
+ * rounding in the 64-bit oracle): the hook of the finite-difference pins. */
+int      orc_set_fluid_state(orc_sim* s, int k, const double* xyz, const double* vel);
 int      orc_get_fluid(const orc_sim* s, int k, int field, double* out);
 int      orc_get_boundary_volume(const orc_sim* s, int k, double* out);
 /* Runs gridSetup + neighbourSearch on the current state and dumps the fluid

@@ -54,6 +54,10 @@ def test_default_params_mirror_reference_defaults():
 @pytest.mark.parametrize("bad", [dict(solver=7), dict(spacing_r=0.0), dict(dt=-1.0),
                                  dict(num_scenes=0), dict(warm_start=9), dict(use_graph=2),
                                  dict(warm_scale=-0.5),
+                                 # solver loops must end at a finite cap
+                                 dict(max_iter_density=0), dict(max_iter_density=-3),
+                                 dict(max_iter_div=-1), dict(max_iter_density=10 ** 6),
+                                 dict(min_iter_density=-1),
                                  dict(solver=sg.IISPH, warm_start=sg.WARM_LEARNED),
                                  dict(solver=sg.DFSPH, warm_start=sg.WARM_LEARNED, learn_batch=4)])
 def test_create_rejects_bad_parameters(bad):

@@ -1,0 +1,260 @@
+"""Pins the oracle's solver iterations and viscosity against quantities fixed
+independently of the formulas they restate.
+
+The reference ships no tests or golden vectors.  The pins here use the
+density sum, which is pinned on its own (kernel quadrature, W(0), lattice
+KATs in test_oracle.py), and differentiate it numerically.  With
+J_ik = d rho_i / d x_k (central differences in FP64, boundary particles held
+fixed):
+
+- a velocity field v changes density at the rate (J v)_i, the predictor of
+  DFSPH.cpp:589-617 / 795-825 and the rho_adv of IISph_M.cpp:260-276;
+- a symmetric pressure-type correction with per-particle weights w_l
+  (kappa/rho for DFSPH, p/rho^2 for IISPH) moves particle k by
+  -dt sum_l w_l J_lk (DFSPH.cpp:647-693, 840-883; IISph_M.cpp:479-584);
+- DFSPH's alpha_i = rho_i / sum_k |J_ik|^2 makes particle i's own stiffness
+  cancel its own predicted compression exactly (Bender & Koschier 2015,
+  eq. 8-9; DFSPH.cpp:159-190, 636-643, 835);
+- IISPH solves rho_adv + A p = rho0 with A_il = -dt^2 (J J^T)_il / rho_l^2
+  (Ihmsen et al. 2014; IISph_M.cpp:114-130, 253-341).
+
+A factor, sign or index error in any of these passes fails one of the checks
+below; a 0.5x stiffness in either DFSPH loop fails the self-cancellation
+check.  Viscosity (DFSPH.cpp:1246-1359, IISph_M.cpp:364-428, WcSph.cpp:180-186)
+is pinned by Galilean invariance, zero response to rigid rotation, the
+v.x < 0 gate and its continuum limit nu (lap v + 2 grad div v).
+"""
+import numpy as np
+import pytest
+
+from graphics_amd import sphgpu as sg
+from oracle import oracle as orc
+
+S = 0.02            # particle spacing; kernel h = S, support 2 S
+RHO0 = 1000.0
+G32 = float(np.float32(-9.81))
+
+
+def _params(solver, **kw):
+    p = dict(solver=solver, spacing_r=S, smooth_radius_h=2 * S, dt=1e-3, gravity=(0.0, 0.0, 0.0), nu=0.0,
+             cfl=0.0, space_min=(-1.0, -1.0, -1.0), space_max=(1.0, 1.0, 1.0), sound_speed=20.0)
+    p.update(kw)
+    return p
+
+
+def _block(n, spacing=S, seed=3):
+    return sg.lattice((n, n, n), (0.0, 0.0, 0.0), spacing, 0.01, seed, 0).astype(np.float64)
+
+
+def _floor(n):
+    """A plane of boundary particles one spacing below the block."""
+    m = n + 4
+    return sg.lattice((m, 1, m), (-2 * S, -S, -2 * S), S, 0.0, 0, 0)
+
+
+def _oracle(p, x, v, wall=None):
+    o = orc.Oracle(p, 64)
+    o.add_fluid(x.astype(np.float32), RHO0, 0.0)
+    if wall is not None:
+        o.add_solid(wall, 0.0)
+    o.init()
+    o.set_state(x, v)
+    return o
+
+
+def _density_jacobian(o, x, eps=1e-7):
+    """J[i, k, :] = d rho_i / d x_k by central differences of the density sum."""
+    n = len(x)
+    J = np.zeros((n, n, 3))
+    for k in range(n):
+        for a in range(3):
+            xp = x.copy()
+            xp[k, a] += eps
+            o.set_state(xp)
+            o.compute_density()
+            rp = o.get(orc.F_RHO)
+            xm = x.copy()
+            xm[k, a] -= eps
+            o.set_state(xm)
+            o.compute_density()
+            rm = o.get(orc.F_RHO)
+            J[:, k, a] = (rp - rm) / (2 * eps)
+    o.set_state(x)
+    o.compute_density()
+    return J
+
+
+# ----------------------------------------------------------- DFSPH loops ----
+def test_dfsph_density_iteration_matches_density_jacobian():
+    """One iteration of the constant-density loop (DFSPH.cpp:572-693) on a
+    compressed block above a wall, gravity and viscosity off (v* = v)."""
+    n = 5
+    x0 = _block(n)
+    rng = np.random.default_rng(1)
+    v0 = -30.0 * (x0 - x0.mean(0)) + rng.normal(0.0, 0.05, x0.shape)
+    p = _params(orc.DFSPH, min_iter_density=1, max_iter_density=1, density_tol=0.0,
+                min_iter_div=0, divergence_tol=1e30)
+    a = _oracle(p, x0, v0, _floor(n))
+    st = a.step()
+    dt = float(st["dt"])
+    kappa, rho_star, v1 = a.get(orc.F_KAPPA), a.get(orc.F_RHO_ADV), a.get(orc.F_VEL)
+    b = _oracle(p, x0, v0, _floor(n))
+    b.compute_density()
+    rho, alpha = b.get(orc.F_RHO), b.get(orc.F_ALPHA)
+    J = _density_jacobian(b, x0)
+    G = (J * J).sum(axis=(1, 2))
+    # predictor: rho* = max(rho + dt J v*, rho0)
+    pred = np.maximum(rho + dt * np.einsum("ika,ka->i", J, v0), RHO0)
+    assert np.abs(rho_star - pred).max() < 1e-7 * RHO0
+    active = kappa > 0
+    assert 10 <= active.sum() <= len(x0) - 10
+    assert np.array_equal(active, pred > RHO0)
+    # alpha is rho / sum_k |J_ik|^2
+    np.testing.assert_allclose(alpha, rho / G, rtol=1e-8)
+    # own stiffness cancels own predicted compression
+    own = dt * dt * G[active] * kappa[active] / rho[active]
+    np.testing.assert_allclose(own, rho_star[active] - RHO0, rtol=1e-6)
+    # velocity correction: dv_k = -dt sum_l kappa_l / rho_l J_lk
+    dv = -dt * np.einsum("l,lka->ka", kappa / rho, J)
+    assert np.abs((v1 - v0) - dv).max() < 1e-6 * np.abs(v1 - v0).max()
+
+
+def test_dfsph_divergence_iteration_matches_density_jacobian():
+    """One iteration of the divergence-free loop (DFSPH.cpp:780-883) after the
+    particles moved by dt v (no density iterations)."""
+    n = 5
+    x0 = _block(n)
+    rng = np.random.default_rng(2)
+    v0 = -10.0 * (x0 - x0.mean(0)) + rng.normal(0.0, 0.05, x0.shape)
+    p = _params(orc.DFSPH, min_iter_density=0, density_tol=10.0, min_iter_div=1, max_iter_div=0,
+                divergence_tol=0.0)
+    a = _oracle(p, x0, v0, _floor(n))
+    st = a.step()
+    dt = float(st["dt"])
+    x1, rho1 = a.get(orc.F_POS), a.get(orc.F_RHO)
+    kv, dd, v1 = a.get(orc.F_KAPPAV), a.get(orc.F_DIVDEV), a.get(orc.F_VEL)
+    assert np.abs(x1 - (x0 + dt * v0)).max() < 1e-15
+    b = _oracle(p, x1, v0, _floor(n))
+    J = _density_jacobian(b, x1)
+    np.testing.assert_allclose(b.get(orc.F_RHO), rho1, rtol=1e-14)
+    G = (J * J).sum(axis=(1, 2))
+    D = np.einsum("ika,ka->i", J, v0)
+    # clamp: no stiffness where the fluid expands or stays below rest density (829-832)
+    expect = np.where((D > 0) & (rho1 + dt * D >= RHO0), D, 0.0)
+    assert np.abs(dd - expect).max() < 1e-8 * np.abs(D).max()
+    active = dd > 0
+    assert 10 <= active.sum() <= len(x0) - 10
+    own = dt * G[active] * kv[active] / rho1[active]
+    np.testing.assert_allclose(own, dd[active], rtol=1e-6)
+    dv = -dt * np.einsum("l,lka->ka", kv / rho1, J)
+    assert np.abs((v1 - v0) - dv).max() < 1e-6 * np.abs(v1 - v0).max()
+
+
+def test_dfsph_density_error_decreases_every_iteration():
+    """The constant-density loop's error falls with every iteration on a
+    compressed block (the same step rerun with caps 1..8)."""
+    n = 6
+    x0 = _block(n)
+    v0 = -20.0 * (x0 - x0.mean(0))
+    errs = []
+    for k in range(1, 9):
+        p = _params(orc.DFSPH, min_iter_density=k, max_iter_density=k, density_tol=0.0,
+                    min_iter_div=0, divergence_tol=1e30)
+        errs.append(_oracle(p, x0, v0, _floor(n)).step()["density_err"])
+    assert errs[0] > 1e-3
+    assert all(b < a for a, b in zip(errs, errs[1:]))
+    assert errs[-1] < 0.3 * errs[0]
+
+
+# ------------------------------------------------------------ IISPH solve ----
+def test_iisph_solves_the_density_jacobian_system():
+    """IISPH's converged pressure satisfies rho_adv + A p = rho0 where p > 0
+    and rho_adv + A p <= rho0 where p = 0, with A built from J; a_ii is A's
+    diagonal and rho_adv = rho + dt J v_adv."""
+    x0 = sg.lattice((4, 4, 4), (0.0, 0.0, 0.0), 0.95 * S, 0.01, 5, 0).astype(np.float64)
+    rng = np.random.default_rng(3)
+    v0 = rng.normal(0.0, 0.05, x0.shape)
+    p = _params(orc.IISPH, density_tol=1e-15, min_iter_density=3000, max_iter_density=3000, omega=0.5)
+    a = _oracle(p, x0, v0, _floor(4))
+    st = a.step()
+    dt = float(st["dt"])
+    pr, aii, radv, rho = (a.get(f) for f in (orc.F_PRESSURE, orc.F_AII, orc.F_RHO_ADV, orc.F_RHO))
+    b = _oracle(p, x0, v0, _floor(4))
+    J = _density_jacobian(b, x0)
+    np.testing.assert_allclose(b.get(orc.F_RHO), rho, rtol=1e-14)
+    np.testing.assert_allclose(radv, rho + dt * np.einsum("ika,ka->i", J, v0), rtol=1e-9)
+    A = -dt * dt * np.einsum("ika,lka->il", J, J) / (rho[None, :] ** 2)
+    np.testing.assert_allclose(aii, np.diag(A), rtol=1e-6)
+    r = radv + A @ pr - RHO0
+    on = pr > 0
+    assert 5 <= on.sum() <= len(pr) - 5
+    assert np.abs(r[on]).max() < 1e-7 * RHO0
+    assert r[~on].max() <= 1e-7 * RHO0
+
+
+# -------------------------------------------------------------- viscosity ----
+def _a_visc(solver, x, v, **kw):
+    """Non-pressure acceleration minus gravity of a free block (no walls)."""
+    p = _params(solver, gravity=(0.0, -9.81, 0.0), nu=1e-3, min_iter_density=1, max_iter_density=1,
+                density_tol=10.0, min_iter_div=0, divergence_tol=1e30, dt=1e-4)
+    p.update(kw)
+    o = orc.Oracle(p, 64)
+    o.add_fluid(x.astype(np.float32), RHO0, 0.05)
+    o.init()
+    o.set_state(x, v)
+    o.step()
+    f = orc.F_ACC if solver == orc.WCSPH else orc.F_AADV
+    return o.get(f) - np.array([0.0, G32, 0.0])
+
+
+_LAT = sg.lattice((7, 7, 7), (0.0, 0.0, 0.0), S, 0.0, 0, 0).astype(np.float64)
+_CTR = _LAT.mean(0)
+_MID = 3 + 7 * 3 + 49 * 3          # a particle whose neighbours all have full neighbourhoods
+
+
+@pytest.mark.parametrize("solver", [orc.DFSPH, orc.IISPH, orc.WCSPH])
+def test_viscosity_is_galilean_invariant(solver):
+    rng = np.random.default_rng(4)
+    v = rng.normal(0.0, 0.1, _LAT.shape)
+    a0 = _a_visc(solver, _LAT, v)
+    a1 = _a_visc(solver, _LAT, v + np.array([0.3, -0.2, 0.5]))
+    assert np.abs(a0).max() > 0
+    assert np.abs(a1 - a0).max() < 1e-12 * np.abs(a0).max()
+
+
+@pytest.mark.parametrize("solver", [orc.DFSPH, orc.IISPH, orc.WCSPH])
+def test_viscosity_vanishes_under_rigid_rotation(solver):
+    rng = np.random.default_rng(5)
+    scale = np.abs(_a_visc(solver, _LAT, rng.normal(0.0, 0.5, _LAT.shape))).max()
+    w = np.array([0.4, -1.1, 0.7])
+    v = np.cross(w, _LAT - _CTR) + np.array([0.1, 0.2, 0.3])
+    assert np.abs(_a_visc(solver, _LAT, v)).max() < 1e-10 * scale
+
+
+def test_viscosity_continuum_limit():
+    """The ungated form (IISph_M.cpp:377) approximates nu (lap v + 2 grad div
+    v) (the factor 10 = 2(d+2)); on the 26-neighbour lattice the discrete sum
+    lies within 30 % of it, which a factor-2 error in nu, the factor 10 or
+    the m/rho_b weight would leave."""
+    nu, c = 1e-3, 50.0
+    shear = np.stack([c * (_LAT[:, 1] - _CTR[1]) ** 2, 0 * _LAT[:, 0], 0 * _LAT[:, 0]], 1)   # lap = 2c, div = 0
+    stretch = np.stack([c * (_LAT[:, 0] - _CTR[0]) ** 2, 0 * _LAT[:, 0], 0 * _LAT[:, 0]], 1)  # lap + 2 grad div = 6c
+    r1 = _a_visc(orc.IISPH, _LAT, shear)[_MID, 0] / (nu * 2 * c)
+    r2 = _a_visc(orc.IISPH, _LAT, stretch)[_MID, 0] / (nu * 6 * c)
+    assert 0.55 < r1 < 0.85
+    assert 1.0 < r2 < 1.4
+
+
+def test_dfsph_viscosity_acts_on_approaching_pairs_only():
+    """DFSPH gates every pair on v_ab.x_ab < 0 (DFSPH.cpp:1260, 1349): an
+    expanding block feels none of it, an approaching one all of it, equal to
+    the ungated IISPH form."""
+    expand = 5.0 * (_LAT - _CTR)
+    assert np.abs(_a_visc(orc.IISPH, _LAT, expand)).max() > 0.1
+    assert np.abs(_a_visc(orc.DFSPH, _LAT, expand)).max() < 1e-12
+    shear = np.stack([50.0 * (_LAT[:, 1] - _CTR[1]) ** 2, 0 * _LAT[:, 0], 0 * _LAT[:, 0]], 1)
+    approach = -5.0 * (_LAT - _CTR) + 0.01 * shear
+    a_df = _a_visc(orc.DFSPH, _LAT, approach)
+    a_ii = _a_visc(orc.IISPH, _LAT, approach)
+    assert np.abs(a_df).max() > 0
+    assert np.abs(a_df - a_ii).max() < 1e-12 * np.abs(a_ii).max()
