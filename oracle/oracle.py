@@ -15,7 +15,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
-WCSPH, DFSPH, IISPH, PCISPH = 0, 1, 2, 3
+WCSPH, DFSPH, IISPH, PCISPH, IISPH_DF = 0, 1, 2, 3, 4
 (F_POS, F_VEL, F_RHO, F_PRESSURE, F_ACC, F_ALPHA, F_KAPPA, F_ID, F_AADV, F_KAPPA_SUM, F_AII, F_RHO_ADV,
  F_KAPPAV, F_DIVDEV, F_VADV) = range(15)
 COMPONENTS = {F_POS: 3, F_VEL: 3, F_ACC: 3, F_AADV: 3, F_VADV: 3}
@@ -45,7 +45,9 @@ class OrcParams(C.Structure):
         ("omega", C.c_float),
         ("warm_start", C.c_int),
         ("warm_scale", C.c_float),
-        ("reserved", C.c_int * 5),
+        ("boundary_density", C.c_float),
+        ("adjust_density", C.c_int),
+        ("reserved", C.c_int * 3),
     ]
 
 
@@ -107,6 +109,7 @@ def load():
         lib.orc_destroy.argtypes = [C.c_void_p]
         lib.orc_add_fluid.argtypes = [C.c_void_p, C.c_int, FP, FP, IP, C.c_float, C.c_float]
         lib.orc_add_solid.argtypes = [C.c_void_p, C.c_int, FP, C.c_int, C.c_float]
+        lib.orc_set_fluid_beta.argtypes = [C.c_void_p, C.c_int, C.c_int, FP]
         lib.orc_init.argtypes = [C.c_void_p]
         lib.orc_set_solid_pose.argtypes = [C.c_void_p, C.c_int, FP, FP, FP, FP]
         lib.orc_solid_force_torque.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double),
@@ -119,6 +122,7 @@ def load():
         lib.orc_num_fluid.argtypes = [C.c_void_p, C.c_int]
         lib.orc_get_fluid.argtypes = [C.c_void_p, C.c_int, C.c_int, DP]
         lib.orc_get_boundary_volume.argtypes = [C.c_void_p, C.c_int, DP]
+        lib.orc_get_boundary_field.argtypes = [C.c_void_p, C.c_int, C.c_int, DP]
         lib.orc_set_fluid_state.argtypes = [C.c_void_p, C.c_int, DP, DP]
         lib.orc_neighbours.argtypes = [C.c_void_p, C.c_int, C.c_int, IP, IP, IP, FP]
         lib.orc_compute_density.argtypes = [C.c_void_p]
@@ -148,12 +152,16 @@ def get_threads() -> int:
     return int(load().orc_get_threads())
 
 
+B_RHO, B_ALPHA, B_KAPPA, B_RHO_ADV, B_KAPPAV = range(5)
+
+
 def params_from(d: dict) -> OrcParams:
     p = OrcParams()
     defaults = dict(solver=DFSPH, spacing_r=0.1, smooth_radius_h=0.2, dt=1e-3, sound_speed=100.0,
                     gamma=7.0, nu=1e-3, cfl=0.0, dt_min=1e-6, density_tol=1e-4, divergence_tol=2.5,
                     min_iter_density=2, max_iter_density=100, min_iter_div=1, max_iter_div=30,
-                    omega=0.5, warm_start=0, warm_scale=0.7, gravity=(0.0, -9.8, 0.0),
+                    omega=0.5, warm_start=0, warm_scale=0.7, boundary_density=1000.0, adjust_density=0,
+                    gravity=(0.0, -9.8, 0.0),
                     space_min=(0.0, 0.0, 0.0), space_max=(1.0, 1.0, 1.0))
     defaults.update({k: v for k, v in d.items() if k in defaults})
     for k, v in defaults.items():
@@ -197,6 +205,12 @@ class Oracle:
                                    None if i is None else i.ctypes.data_as(IP), rho0, visc)
         self.nfluid_phases = max(self.nfluid_phases, k + 1)
         return k
+
+    def set_beta(self, k, beta):
+        """Per-particle rest-density factor of fluid k, in its add order."""
+        b = np.ascontiguousarray(beta, np.float32)
+        if self.lib.orc_set_fluid_beta(self.h, k, len(b), b.ctypes.data_as(C.POINTER(C.c_float))) != 0:
+            raise ValueError("set_beta: one value per particle of the fluid")
 
     def add_solid(self, xyz, visc=0.0, dynamic=False) -> int:
         x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
@@ -293,6 +307,15 @@ class Oracle:
         self.lib.orc_get_boundary_volume(self.h, k, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
 
+    def boundary_field(self, field, k=0, n=None) -> np.ndarray:
+        """IISph DF_* solvers: B_RHO, B_ALPHA, B_KAPPA, B_RHO_ADV or B_KAPPAV of
+        the boundary particles of solid k."""
+        out = np.zeros(n if n is not None else 1 << 20)
+        m = self.lib.orc_get_boundary_field(self.h, k, field, out.ctypes.data_as(C.POINTER(C.c_double)))
+        if m < 0:
+            raise ValueError("boundary_field")
+        return out[:m]
+
     def neighbours(self, k=0, cap=48):
         n = self.num_fluid(k)
         counts = np.zeros(n, np.int32)
@@ -372,11 +395,14 @@ def from_scene(sc, bits=64, **overrides) -> Oracle:
         ids = np.arange(first, first + len(xyz), dtype=np.int32)
         per_phase.setdefault(ph, []).append((xyz, ids))
         first += len(xyz)
+    beta = getattr(sc, "beta", None)
     for ph in sorted(per_phase):
         rho0, visc = sc.phases[ph]
         xyz = np.concatenate([a for a, _ in per_phase[ph]])
         ids = np.concatenate([b for _, b in per_phase[ph]])
-        o.add_fluid(xyz, rho0, visc, ids=ids)
+        k = o.add_fluid(xyz, rho0, visc, ids=ids)
+        if beta is not None:
+            o.set_beta(k, np.asarray(beta, np.float32)[ids])
     for s, xyz, visc in sc.solids:
         o.add_solid(xyz, visc)
     for s, xyz, visc, pose in getattr(sc, "rigids", []):

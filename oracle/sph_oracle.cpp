@@ -86,6 +86,7 @@ struct FluidPart {           /* the fields of FluidParticle (SphBase.h:66-165) t
     T alpha = 0, kappa = 0, kappaV = 0, divdev = 0;
     T kappa_sum = 0, kappaV_sum = 0;   /* accumulated kappa*dt^2 (warm start state) */
     T d = 0;
+    T beta = 1;                        /* IISph per-particle rest-density factor (SphBase.h:98) */
     int id = 0;
 };
 
@@ -93,6 +94,7 @@ template <class T>
 struct BoundPart {           /* BoundaryParticle (SphBase.h:168-199) */
     V3<T> x, v, force;
     T volume = 0, kerSumFromSelf = 0, d = 0;
+    T rho = 0, alpha = 0, rho_adv = 0, kappa = 0, kappaV = 0;   /* IISph DF_* solvers */
 };
 
 template <class T>
@@ -1362,6 +1364,281 @@ public:
         }
     }
 
+    /* ---- IISph class, header-consistent DFSPH (IISph.cpp:100-288, 816-1217) ----
+     * Per-particle mass m = rho0 beta d^3 and rest density rho0 beta
+     * (IISph.cpp:110-115); boundary particles carry mass boundary_density V_b
+     * and rest density boundary_density (IISph.cpp:130; BoundPart::restDensity
+     * is the scene's value, Demo-mySPH.h:64, and m_TH.boundaryDensity its
+     * default, SphBase.h:576).  adjust_density (m_TH.adjustDensity,
+     * SphBase.h:575) makes the fluid-fluid density and rho_adv terms use the
+     * particle's own mass (a number density).  Boundary particles get their
+     * own kappa and kappaV each iteration; the reference computes them for
+     * m_Solids[k] with k the fluid index (IISph.cpp:847, 1069), here for every
+     * solid.  Every particle has d = spacing_r, so each pair's kernel h =
+     * (d_a + d_b)/2 is the one-argument kernel's. */
+    T iidfMass(int k, const FluidPart<T>& q) const { return fluids[k].rho0 * q.beta * r * r * r; }
+    T iidfRest(int k, const FluidPart<T>& q) const { return fluids[k].rho0 * q.beta; }
+    T bDensity() const { return T(P.boundary_density); }
+    T bMass(const BoundPart<T>& b) const { return bDensity() * b.volume; }
+
+    /* IISph::DF_prepareAttribute, IISph.cpp:100-288: densities and alpha of
+     * fluid and boundary particles */
+    void iidf_prepare() {
+        const bool adjust = P.adjust_density != 0;
+        int zd = 0;
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n)) reduction(+ : zd)
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                const T ma = iidfMass(k, a), resta = iidfRest(k, a);
+                T density = W0 * ma, a2 = 0;
+                V3<T> a1;
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    const T w = nb.dis == 0 ? W0 : W(nb.dis);
+                    if (nb.pidx.isFluid()) {
+                        const FluidPart<T>& b = fp(nb.pidx);
+                        const T mb = iidfMass(nb.pidx.toFluidI(), b);
+                        density += (adjust ? ma : mb) * w;                     /* IISph.cpp:163-164 */
+                        if (nb.dis == 0) continue;
+                        T gs = -Wg(nb.dis);
+                        a1 += (a.x - b.x) * (gs / nb.dis) * mb;
+                        a2 += (gs * mb) * (gs * mb);
+                    } else {
+                        const Solid<T>& so = solids[nb.pidx.toSolidI()];
+                        const BoundPart<T>& b = so.parts[nb.pidx.i];
+                        density += resta * b.volume * w;                        /* IISph.cpp:187 */
+                        if (nb.dis == 0) continue;
+                        T gs = -Wg(nb.dis);
+                        a1 += (a.x - b.x) * (gs / nb.dis) * (bDensity() * b.volume);   /* IISph.cpp:192 */
+                        if (so.dynamic) a2 += (gs * bMass(b)) * (gs * bMass(b));       /* IISph.cpp:193-195 */
+                    }
+                }
+                a.rho = density;
+                T den = a1.len_square() + a2;
+                if (den < T(1.0e-6)) den = T(1.0e-6);
+                a.alpha = density / den;
+            }
+        }
+        zeroDis += zd;
+        for (int k = 0; k < int(solids.size()); ++k) {                    /* IISph.cpp:234-287 */
+            auto& ps = solids[k].parts; int n = int(ps.size());
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+            for (int i = 0; i < n; ++i) {
+                BoundPart<T>& a = ps[i];
+                const T ma = bMass(a);
+                T density = W0 * ma, a2 = 0;
+                const NeigbStr<T>& L = nbS[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    const T w = nb.dis == 0 ? W0 : W(nb.dis);
+                    if (nb.pidx.isFluid()) {
+                        const FluidPart<T>& b = fp(nb.pidx);
+                        const T mb = iidfMass(nb.pidx.toFluidI(), b);
+                        density += (P.adjust_density ? ma : mb) * w;
+                        if (nb.dis == 0) continue;
+                        T gs = -Wg(nb.dis);
+                        a2 += (gs * mb) * (gs * mb);
+                    } else {
+                        density += bDensity() * bp(nb.pidx).volume * w;
+                    }
+                }
+                a.rho = density;
+                T den = a2 < T(1.0e-6) ? T(1.0e-6) : a2;
+                a.alpha = density / den;
+            }
+        }
+    }
+
+    /* sum over a boundary particle's neighbours of gradW.(v_b - u_j) m_j,
+     * with u = v_adv for fluid neighbours (the boundary passes of both loops,
+     * IISph.cpp:859-899 with fluidMass, 1079-1110) */
+    T iidfBoundaryRate(int k, int i, bool adjust) const {
+        const BoundPart<T>& a = solids[k].parts[i];
+        T d = 0;
+        const NeigbStr<T>& L = nbS[k][i];
+        for (int j = 0; j < L.num; ++j) {
+            const auto& nb = L.neigs[j];
+            if (nb.dis == 0) continue;
+            T gs = -Wg(nb.dis);
+            if (nb.pidx.isFluid()) {
+                const FluidPart<T>& b = fp(nb.pidx);
+                V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                d += g.dot(a.v - b.v_adv) * (adjust ? bMass(a) : iidfMass(nb.pidx.toFluidI(), b));
+            } else {
+                const BoundPart<T>& b = bp(nb.pidx);
+                V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                d += g.dot(a.v - b.v) * (bDensity() * b.volume);
+            }
+        }
+        return d;
+    }
+
+    /* the velocity pass of both loops (IISph.cpp:956-1000, 1158-1200):
+     * v_adv -= dt sum gradW m_b (k_a/rho_a + k_b/rho_b), boundary neighbours
+     * with their own kappa and mass boundary_density V_b */
+    void iidf_velocity(int which, T dt) {
+        for (int k = 0; k < int(fluids.size()); ++k) {
+            auto& ps = fluids[k].parts; int n = int(ps.size());
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = ps[i];
+                const T ka = (which == 0 ? a.kappa : a.kappaV) / a.rho;
+                const NeigbStr<T>& L = nbF[k][i];
+                for (int j = 0; j < L.num; ++j) {
+                    const auto& nb = L.neigs[j];
+                    if (nb.dis == 0) continue;
+                    T gs = -Wg(nb.dis);
+                    if (nb.pidx.isFluid()) {
+                        const FluidPart<T>& b = fp(nb.pidx);
+                        const T kb = (which == 0 ? b.kappa : b.kappaV) / b.rho;
+                        V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                        a.v_adv = a.v_adv - g * (iidfMass(nb.pidx.toFluidI(), b) * (ka + kb) * dt);
+                    } else {
+                        const BoundPart<T>& b = bp(nb.pidx);
+                        const T kb = (which == 0 ? b.kappa : b.kappaV) / b.rho;
+                        V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                        a.v_adv = a.v_adv - g * (bDensity() * b.volume * (ka + kb) * dt);
+                    }
+                }
+            }
+        }
+    }
+
+    /* IISph::DF_divergenceFreeSolver, IISph.cpp:1039-1217 */
+    void iidf_divergence(orc_stats* st) {
+        const T dt = dt_cur;
+        for (auto& f : fluids) for (auto& p : f.parts) p.v_adv = p.v;
+        double avg = 10; int it = 0;
+        while (avg > double(P.divergence_tol) || it < P.min_iter_div) {
+            if (it > P.max_iter_div) { st->max_iter_hit = 1; break; }     /* IISph.cpp:1062 */
+            ++it;
+            for (int k = 0; k < int(solids.size()); ++k) {                /* every solid (IISph.cpp:1069) */
+                auto& ps = solids[k].parts; int n = int(ps.size());
+#pragma omp parallel for
+                for (int i = 0; i < n; ++i) {
+                    BoundPart<T>& a = ps[i];
+                    T dd = iidfBoundaryRate(k, i, false);
+                    if (dd < 0 || (a.rho + dd * dt) < bDensity()) dd = 0;
+                    a.kappaV = dd * a.alpha / dt;
+                }
+            }
+            double sum = 0; int ntot = 0;
+            for (int k = 0; k < int(fluids.size()); ++k) {
+                auto& ps = fluids[k].parts; int n = int(ps.size());
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+                for (int i = 0; i < n; ++i) {
+                    FluidPart<T>& a = ps[i];
+                    const T resta = iidfRest(k, a);
+                    T dd = 0;
+                    const NeigbStr<T>& L = nbF[k][i];
+                    for (int j = 0; j < L.num; ++j) {
+                        const auto& nb = L.neigs[j];
+                        if (nb.dis == 0) continue;
+                        T gs = -Wg(nb.dis);
+                        if (nb.pidx.isFluid()) {
+                            const FluidPart<T>& b = fp(nb.pidx);
+                            V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                            dd += g.dot(a.v_adv - b.v_adv) * iidfMass(nb.pidx.toFluidI(), b);
+                        } else {
+                            const BoundPart<T>& b = bp(nb.pidx);
+                            V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                            dd += g.dot(a.v_adv - b.v) * (resta * b.volume);    /* IISph.cpp:1146 */
+                        }
+                    }
+                    if (dd < 0 || (a.rho + dd * dt) < resta) dd = 0;          /* IISph.cpp:1150-1152 */
+                    a.divdev = dd;
+                    a.kappaV = dd * a.alpha / dt;
+                }
+                for (int i = 0; i < n; ++i) sum += double(ps[i].divdev);     /* IISph.cpp:1204-1205 */
+                ntot += n;
+            }
+            iidf_velocity(1, dt);
+            avg = ntot > 0 ? sum / ntot * double(dt) : 0.0;                  /* IISph.cpp:1207 */
+        }
+        st->iter_divergence = it;
+        st->divergence_err = avg;
+        for (auto& f : fluids) for (auto& p : f.parts) p.v = p.v_adv;
+    }
+
+    /* IISph::DF_constantDensitySolver, IISph.cpp:816-1037 */
+    void iidf_density(orc_stats* st) {
+        const T dt = dt_cur;
+        const bool adjust = P.adjust_density != 0;
+        for (auto& f : fluids) {
+            int n = int(f.parts.size());
+#pragma omp parallel for
+            for (int i = 0; i < n; ++i) f.parts[i].v_adv = f.parts[i].v + f.parts[i].a_adv * dt;
+        }
+        double err = 1; int it = 0;
+        while (err > double(P.density_tol) || it < P.min_iter_density) {
+            if (it > P.max_iter_density) { st->max_iter_hit = 1; break; }   /* IISph.cpp:840 */
+            ++it;
+            for (int k = 0; k < int(solids.size()); ++k) {                /* every solid (IISph.cpp:847) */
+                auto& ps = solids[k].parts; int n = int(ps.size());
+#pragma omp parallel for
+                for (int i = 0; i < n; ++i) {
+                    BoundPart<T>& a = ps[i];
+                    T radv = a.rho + iidfBoundaryRate(k, i, adjust) * dt;
+                    if (radv < bDensity()) radv = bDensity();
+                    a.rho_adv = radv;
+                    a.kappa = (radv - bDensity()) / (dt * dt) * a.alpha;
+                }
+            }
+            double sum = 0; int ntot = 0;
+            for (int k = 0; k < int(fluids.size()); ++k) {
+                auto& ps = fluids[k].parts; int n = int(ps.size());
+#pragma omp parallel for schedule(dynamic, dyn_chunk(n))
+                for (int i = 0; i < n; ++i) {
+                    FluidPart<T>& a = ps[i];
+                    const T ma = iidfMass(k, a), resta = iidfRest(k, a);
+                    T radv = a.rho;
+                    const NeigbStr<T>& L = nbF[k][i];
+                    for (int j = 0; j < L.num; ++j) {
+                        const auto& nb = L.neigs[j];
+                        if (nb.dis == 0) continue;
+                        T gs = -Wg(nb.dis);
+                        if (nb.pidx.isFluid()) {
+                            const FluidPart<T>& b = fp(nb.pidx);
+                            V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                            radv += g.dot(a.v_adv - b.v_adv) * (adjust ? ma : iidfMass(nb.pidx.toFluidI(), b)) * dt;
+                        } else {
+                            const BoundPart<T>& b = bp(nb.pidx);
+                            V3<T> g = (a.x - b.x) * (gs / nb.dis);
+                            radv += g.dot(a.v_adv - b.v) * bDensity() * b.volume * dt;   /* IISph.cpp:943 */
+                        }
+                    }
+                    if (radv < resta) radv = resta;
+                    a.rho_adv = radv;
+                    a.kappa = (radv - resta) / (dt * dt) * a.alpha;              /* IISph.cpp:962-965 */
+                }
+                for (int i = 0; i < n; ++i)                                   /* IISph.cpp:956-957 */
+                    sum += (double(ps[i].rho_adv) - double(iidfRest(k, ps[i]))) / double(iidfRest(k, ps[i]));
+                ntot += n;
+            }
+            iidf_velocity(0, dt);
+            err = ntot > 0 ? sum / ntot : 0.0;                                  /* IISph.cpp:1004 */
+        }
+        st->iter_density = it;
+        st->density_err = err;
+        /* acce_presure = (v_adv - v)/dt (IISph.cpp:1008-1011), and updateFluids
+         * under the IISph define: v += acce_presure dt, x += v dt
+         * (SphBase.cpp:387-391) */
+        for (auto& f : fluids) {
+            int n = int(f.parts.size());
+#pragma omp parallel for
+            for (int i = 0; i < n; ++i) {
+                FluidPart<T>& a = f.parts[i];
+                a.a_p = (a.v_adv - a.v) * (T(1) / dt);
+                a.v = a.v_adv;
+                a.x += a.v * dt;
+            }
+            st->removed += removeOutside(f.parts);
+        }
+    }
+
     void fillStats(orc_stats* st) {
         double se = 0, me = -1e300, ke = 0; long long n = 0; int nan = 0;
         T v0 = r * r * r;
@@ -1409,6 +1686,22 @@ public:
             updateSolidPartWeight();
             df_first();
             df_divergence(st);
+            fillStats(st);
+        } else if (P.solver == ORC_IISPH_DF) {
+            /* IISph::sphStep with the DF_* solvers in place of the VF_* ones
+             * (the commented-out calls at IISph.cpp:59-60): search, V_b,
+             * DF_prepareAttribute, the non-pressure acceleration (DFSPH's
+             * DF_forceExceptPressure: the IISph class computes it in its VF_*
+             * advection step, IISph.cpp:1511-1567), the divergence-free solve,
+             * the constant-density solve, updateFluids, updateSolids */
+            applyPendingPoses();
+            neighbourSearch();
+            updateSolidPartWeight();
+            iidf_prepare();
+            df_forceExceptPressure();
+            iidf_divergence(st);
+            iidf_density(st);
+            rc = updateSolids(cb, user);
             fillStats(st);
         } else if (P.solver == ORC_PCISPH) {                               /* PciSph.cpp:31-95 */
             applyPendingPoses();
@@ -1511,6 +1804,15 @@ int orc_add_fluid(orc_sim* s, int n, const float* xyz, const float* vel, const i
         }
         S.fluids.push_back(std::move(f));
         return int(S.fluids.size()) - 1;
+    });
+}
+
+int orc_set_fluid_beta(orc_sim* s, int k, int n, const float* beta) {
+    return dispatch(s, [&](auto& S) {
+        using T = std::decay_t<decltype(S.r)>;
+        if (k < 0 || k >= int(S.fluids.size()) || !beta || n != int(S.fluids[k].parts.size())) return -1;
+        for (int i = 0; i < n; ++i) S.fluids[k].parts[i].beta = T(beta[i]);
+        return 0;
     });
 }
 
@@ -1674,8 +1976,21 @@ int orc_compute_density(orc_sim* s) {
     return dispatch(s, [&](auto& S) {
         S.neighbourSearch();
         S.updateSolidPartWeight();
-        S.df_first();
+        if (S.P.solver == ORC_IISPH_DF) S.iidf_prepare();
+        else S.df_first();
         return 0;
+    });
+}
+
+int orc_get_boundary_field(const orc_sim* s, int k, int field, double* out) {
+    return dispatchc(s, [&](const auto& S) {
+        if (k < 0 || k >= int(S.solids.size()) || field < 0 || field > 4) return -1;
+        const auto& ps = S.solids[k].parts;
+        for (size_t i = 0; i < ps.size(); ++i) {
+            const auto& q = ps[i];
+            out[i] = field == 0 ? q.rho : field == 1 ? q.alpha : field == 2 ? q.kappa : field == 3 ? q.rho_adv : q.kappaV;
+        }
+        return int(ps.size());
     });
 }
 

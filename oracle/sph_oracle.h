@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-enum { ORC_WCSPH = 0, ORC_DFSPH = 1, ORC_IISPH = 2, ORC_PCISPH = 3 };
+enum { ORC_WCSPH = 0, ORC_DFSPH = 1, ORC_IISPH = 2, ORC_PCISPH = 3, ORC_IISPH_DF = 4 };
 
 /* Mirrors ThresholdsBase m_TH (SphBase.h:546-583) plus the solver settings
  * that the reference hard-codes (DFSPH.cpp:225-230, 762-766; IISph_M.cpp:285,
@@ -43,7 +43,9 @@ typedef struct orc_params {
     float omega;               /* IISPH relaxed Jacobi (IISph_M.cpp:341) */
     int   warm_start;          /* 0 none; 1 divergence loop warm-started from the previous step */
     float warm_scale;          /* scale of the previous step's stiffness in the guess */
-    int   reserved[5];
+    float boundary_density;    /* m_TH.boundaryDensity (SphBase.h:576), IISph DF_* solvers */
+    int   adjust_density;      /* m_TH.adjustDensity (SphBase.h:575), IISph DF_* solvers */
+    int   reserved[3];
 } orc_params;
 
 typedef struct orc_stats {
@@ -87,6 +89,9 @@ orc_sim* orc_create(int precision_bits, const orc_params* p);
 void     orc_destroy(orc_sim* s);
 int      orc_add_fluid(orc_sim* s, int n, const float* xyz, const float* vel,
                        const int* ids, float rest_density, float visc_alpha);
+/* per-particle rest-density factor beta of fluid k (IISph DF_* solvers,
+ * m = rho0 beta d^3, IISph.cpp:110-115); default 1 */
+int      orc_set_fluid_beta(orc_sim* s, int k, int n, const float* beta);
 int      orc_add_solid(orc_sim* s, int n, const float* xyz, int dynamic, float visc_alpha);
 /* dynamic solid k: OpenGL column-major transform of its body-frame samples,
  * centre of mass, linear and angular velocity (SphSolid::updatePaticles) */
@@ -109,6 +114,10 @@ int      orc_num_fluid(const orc_sim* s, int k);
 int      orc_set_fluid_state(orc_sim* s, int k, const double* xyz, const double* vel);
 int      orc_get_fluid(const orc_sim* s, int k, int field, double* out);
 int      orc_get_boundary_volume(const orc_sim* s, int k, double* out);
+/* IISph DF_* solvers: boundary particle fields of solid k, field 0 density,
+ * 1 alpha, 2 kappa, 3 predicted density of the last density iteration,
+ * 4 kappaV; returns the count */
+int      orc_get_boundary_field(const orc_sim* s, int k, int field, double* out);
 /* Runs gridSetup + neighbourSearch on the current state and dumps the fluid
  * neighbour lists: per fluid particle (current order) up to `cap` entries of
  * (kind 0 fluid / 1 boundary, id, dis).  Returns total overflow count. */
