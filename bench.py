@@ -154,10 +154,12 @@ def build_scene(name: str):
 
 
 def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, graph: int = 0,
-            stats_every: int = 1, capture_at: int | None = None, label: str = ""):
-    """One arm: `warmup` untimed steps, then `steps` timed steps.  Returns
-    per-step statistics of both phases; with capture_at the fluid state after
-    that many warm-up steps (for the CPU baseline)."""
+            stats_every: int = 1, capture_at: int | None = None, label: str = "", train: int = 0):
+    """One arm: `train` untimed steps in which a learned warm start trains its
+    network (every arm runs them, so that all arms time the same steps of the
+    simulation), `warmup` untimed steps, then `steps` timed steps.  Returns
+    per-step statistics of the untimed and the timed steps; with capture_at
+    the fluid state after that many untimed steps (for the CPU baseline)."""
     sim = scenes.make_simulation(sc, warm_start=warm_start, use_graph=graph)
     if sc.rigids:
         # the host rigid-body step (Bullet's place in the reference) inside
@@ -174,16 +176,20 @@ def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, g
     log(f"[bench] {label} init {time.perf_counter() - t0:.2f}s  fluid={sc.num_fluid} boundary={sc.num_boundary}")
     warm_times, warm_stats, captured = [], [], None
     t_warm = time.perf_counter()
-    for k in range(warmup):
+    train_elapsed = 0.0
+    for k in range(train + warmup):
         t = time.perf_counter()
         warm_stats.append(sim.step())
         warm_times.append(time.perf_counter() - t)
+        if k + 1 == train:
+            train_elapsed = time.perf_counter() - t_warm
         if capture_at is not None and k + 1 == capture_at:
             captured = dict(x=sim.get(sg.F_POS), v=sim.get(sg.F_VEL), ids=sim.ids(),
                             phase=sim.get(sg.F_PHASE).astype(np.int64), step=k + 1)
-        if k % 25 == 0 or k == warmup - 1:
+        if k % 25 == 0 or k == train + warmup - 1:
             s = warm_stats[-1]
-            log(f"[bench] {label} warmup {k}: {warm_times[-1] * 1e3:.2f} ms it={s['iter_density']}/{s['iter_divergence']}")
+            what = "train" if k < train else "warmup"
+            log(f"[bench] {label} {what} {k}: {warm_times[-1] * 1e3:.2f} ms it={s['iter_density']}/{s['iter_divergence']}")
     warm_elapsed = time.perf_counter() - t_warm
     if profile:
         sim.profile_kernel(profile)
@@ -202,7 +208,8 @@ def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, g
     tuning["lists"] = sim.list_form()
     sim.close()
     return dict(elapsed=elapsed, stats=stats, warm_times=warm_times, warm_stats=warm_stats,
-                warm_elapsed=warm_elapsed, timing=timing, n_live=n_live, tuning=tuning, captured=captured)
+                warm_elapsed=warm_elapsed, train_elapsed=train_elapsed, timing=timing, n_live=n_live,
+                tuning=tuning, captured=captured)
 
 
 def arm_summary(run: dict, nf: int, steps: int, tol: float | None) -> dict:
@@ -220,7 +227,7 @@ def arm_summary(run: dict, nf: int, steps: int, tol: float | None) -> dict:
            "too_many_neigb": int(sum(s["too_many_neigb"] for s in st)),
            "list_overflow": int(sum(s["list_overflow"] for s in st)),
            "zero_dis": int(sum(s["zero_dis"] for s in st)), "removed": int(sum(s["removed"] for s in st)),
-           "warmup_s": round(run["warm_elapsed"], 3)}
+           "untimed_s": round(run["warm_elapsed"], 3), "train_s": round(run["train_elapsed"], 3)}
     if tol is not None:
         out["density_err_within_tol"] = bool(out["density_err_max"] <= tol)
     return out
@@ -338,9 +345,9 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=None,
-                    help="untimed steps before the timed ones (default: the workload's learn_steps, "
-                         "so the timed steps run the trained model, at least 20)")
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed steps before the timed ones (after the training steps of a learned "
+                         "warm start)")
     ap.add_argument("--workload", default="dfsph_c3", choices=sorted(scenes.WORKLOADS) + ["surface_dam"])
     ap.add_argument("--warm-start", default="learned", choices=["none", "previous", "learned"])
     ap.add_argument("--no-ab", action="store_true", help="skip the plain-solver arm")
@@ -391,19 +398,18 @@ def main() -> int:
     if warm == sg.WARM_LEARNED and solver not in (sg.DFSPH, sg.IISPH):
         warm = sg.WARM_PREVIOUS
     learn_steps = int(sc.params.get("learn_steps", 40))
-    if args.warmup is None:
-        args.warmup = max(20, learn_steps if warm == sg.WARM_LEARNED else 20)
-    if warm == sg.WARM_LEARNED and args.warmup < learn_steps:
-        log(f"[bench] note: warmup {args.warmup} < learn_steps {learn_steps}: timed steps include training")
+    # the learned arm trains its network online during its first learn_steps
+    # steps; these run untimed before the warm-up, in every arm
+    train = learn_steps if warm == sg.WARM_LEARNED else 0
     prof = args.profile_kernel or DOMINANT[solver]
     tol = sc.params.get("density_tol")
     labels = {sg.WARM_NONE: "plain", sg.WARM_PREVIOUS: "previous", sg.WARM_LEARNED: "learned"}
     # the CPU baseline starts from the plain arm's state (the reference algorithm)
     want_cpu = not args.no_cpu and sc.num_scenes == 1
     cap_main = args.cpu_from_step if (want_cpu and warm == sg.WARM_NONE and not sc.rigids
-                                      and args.cpu_from_step <= args.warmup) else None
+                                      and args.cpu_from_step <= train + args.warmup) else None
     run = run_gpu(sc, warm, args.steps, args.warmup, None if args.graph else prof, int(args.graph),
-                  max(1, args.stats_every), capture_at=cap_main, label=labels[warm])
+                  max(1, args.stats_every), capture_at=cap_main, label=labels[warm], train=train)
     nf = sc.num_fluid
     value = nf * args.steps / run["elapsed"]
     ms = run["elapsed"] / args.steps * 1e3
@@ -436,8 +442,8 @@ def main() -> int:
         others = [sg.WARM_NONE] + ([sg.WARM_PREVIOUS] if warm == sg.WARM_LEARNED else [])
         for w in others:
             cap = args.cpu_from_step if (w == sg.WARM_NONE and want_cpu and not sc.rigids
-                                         and args.cpu_from_step <= args.warmup) else None
-            arm = run_gpu(sc, w, args.steps, args.warmup, None, capture_at=cap, label=labels[w])
+                                         and args.cpu_from_step <= train + args.warmup) else None
+            arm = run_gpu(sc, w, args.steps, args.warmup, None, capture_at=cap, label=labels[w], train=train)
             if cap is not None:
                 captured = arm["captured"]
             a = arm_summary(arm, nf, args.steps, tol)
@@ -480,6 +486,8 @@ def main() -> int:
                    "max_iter_density": sc.params.get("max_iter_density"),
                    "max_iter_div": sc.params.get("max_iter_div"),
                    "learn_steps": learn_steps if warm == sg.WARM_LEARNED else None,
+                   "untimed_training_steps": train,
+                   "training_s": round(run["train_elapsed"], 3),
                    "density_err_within_tol": main_sum.get("density_err_within_tol"),
                    "live_after": run["n_live"], "hip_graph": bool(args.graph),
                    "stats_every": max(1, args.stats_every)},
