@@ -18,8 +18,8 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libsphgpu.so"
 
-SOURCES = ["api.hip", "grid.hip", "solvers.hip", "sort.hip", "mlp.hip", "scene.cpp", "io.cpp", "surface.hip",
-           "mc_table.cpp"]
+SOURCES = ["api.hip", "grid.hip", "solvers.hip", "iisph_df.hip", "sort.hip", "mlp.hip", "scene.cpp", "io.cpp",
+           "surface.hip", "mc_table.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [

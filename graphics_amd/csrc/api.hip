@@ -52,6 +52,7 @@ struct sphgpu_handle {
     std::vector<Phase> phases;
     // host staging (add order)
     std::vector<float> fx, fv;
+    std::vector<float> fbeta;          // per-particle beta (IISph DF_*), empty = all 1
     std::vector<int> fphase, fscene;
     std::vector<float> bx;
     std::vector<int> bsolid, bscene;
@@ -128,6 +129,8 @@ struct sphgpu_handle {
     bool tuned = false;
     int tuneFix[3] = {-1, -1, -1};   // fixed tile map, staging depth, streaming lists (-1: timed)
     bool listsBuilt = false;   // neighbour lists built at least once
+    bool needSearch = false;   // particles appended since the last neighbour search (DFSPH searches first)
+    long long appended = 0;    // particles appended after init
     float tuneMs[6] = {0, 0, 0, 0, 0, 0};   // density pass: map 0, 1; list builder: 6, 8 groups;
                                             // density pass: plain, streaming list loads
     uint32_t gatherSeed = 1;
@@ -222,6 +225,7 @@ void init_handle_mlp(std::vector<float>& p) {
 int run_ns(sphgpu_handle* h) {
     Dev& D = h->D;
     h->listsBuilt = true;
+    h->needSearch = false;
     hipStream_t st = h->st;
     int npad = D.npad;
     {
@@ -303,6 +307,21 @@ void launch_iteration(sphgpu_handle* h, int which, int it) {
                 }
                 launch_scene_reduce(D, 1, P.divergence_tol, P.min_iter_div, P.max_iter_div + 1, h->st);
             }
+        } else if (P.solver == SPHGPU_IISPH_DF) {
+            // IISph.cpp:840-1004 (which 0) and 1062-1207 (which 1): the loops
+            // break when the iteration count exceeds the maximum
+            {
+                ProfScope ps(h, which == 0 ? "idf_kappa_density" : "idf_kappa_divergence", nf_live(h) * 36.0);
+                launch_idf_kappa(D, which == 0 ? 0 : 1, h->st);
+            }
+            {
+                ProfScope ps(h, "idf_velocity", nf_live(h) * 44.0);
+                launch_idf_velocity(D, h->st);
+            }
+            if (which == 0)
+                launch_scene_reduce(D, 1, P.density_tol, P.min_iter_density, P.max_iter_density + 1, h->st);
+            else
+                launch_scene_reduce(D, 1, P.divergence_tol, P.min_iter_div, P.max_iter_div + 1, h->st);
         } else if (P.solver == SPHGPU_PCISPH) {
             {
                 ProfScope ps(h, "pci_density", nf_live(h) * 44.0);
@@ -332,7 +351,8 @@ void launch_iteration(sphgpu_handle* h, int which, int it) {
 // first batch of a solver loop: resets the per-scene flags and launches
 // `batch` iterations without reading anything back (capturable)
 int first_batch(const sphgpu_handle* h, int which) {
-    const int minIt = which == 0 || h->P.solver != SPHGPU_DFSPH ? h->P.min_iter_density : h->P.min_iter_div;
+    const bool divLoop = which == 1 && (h->P.solver == SPHGPU_DFSPH || h->P.solver == SPHGPU_IISPH_DF);
+    const int minIt = divLoop ? h->P.min_iter_div : h->P.min_iter_density;
     return std::max({1, minIt, h->lastIters[which] - 1});
 }
 
@@ -344,16 +364,18 @@ void solver_loop_start(sphgpu_handle* h, int which, int batch) {
 
 // rest of a solver loop whose first `launched` iterations are queued:
 // reads the flags, adds iterations two at a time until every scene is done
-int solver_loop_finish(sphgpu_handle* h, int which, int launched, size_t profStart) {
+int solver_loop_finish(sphgpu_handle* h, int which, int launched, size_t profStart, int* extra = nullptr) {
     Dev& D = h->D;
     int ns = D.nscenes;
     int maxIt = 0;
+    if (extra) *extra = 0;
     for (;;) {
         bool any = false;
         int rc = sync_active(h, &any);
         if (rc) return rc;
         if (!any) break;
         for (int q = 0; q < 2; ++q) launch_iteration(h, which, launched++);
+        if (extra) *extra += 2;
     }
     for (int s = 0; s < ns; ++s) maxIt = std::max(maxIt, h->hIters[s]);
     h->lastIters[which] = maxIt;
@@ -417,7 +439,7 @@ int autotune(sphgpu_handle* h) {
     run_ns(h);   // sorted layout and lists to time on (the first step sorts again: same order)
     auto density = [h]() {
         const Dev& D = h->D;
-        if (h->P.solver == SPHGPU_DFSPH) launch_df_density_alpha(D, h->st);
+        if (h->P.solver == SPHGPU_DFSPH || h->P.solver == SPHGPU_IISPH_DF) launch_df_density_alpha(D, h->st);
         else if (h->P.solver == SPHGPU_IISPH) launch_ii_density(D, h->st);
         else launch_wc_pressure(D, h->st);
     };
@@ -643,7 +665,9 @@ int step_wcsph(sphgpu_handle* h, sphgpu_stats* st) {
 int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
     Dev& D = h->D;
     int rc;
-    if (h->frame == 0) {
+    if (h->frame == 0 || h->needSearch) {
+        // the density solve reads the lists of the previous search: after an
+        // append they must cover the new particles
         run_ns(h);
         launch_df_density_alpha(D, h->st);
     }
@@ -705,10 +729,15 @@ int step_iisph(sphgpu_handle* h, sphgpu_stats* st) {
     Dev& D = h->D;
     int rc;
     const size_t profStart = h->profUsed;
-    const int batch = first_batch(h, 0);
-    // neighbour search, the per-step passes and the first `batch` solver
-    // iterations: one captured segment in graph mode
-    auto body = [h, batch]() {
+    const bool graph = h->P.use_graph && !h->D.anyDynamic;
+    // graph mode: the whole step is one captured segment -- search, the
+    // per-step passes, enough solver iterations for the previous step's
+    // count plus one (iterations after a scene converged return at once),
+    // and the pressure force + integration of every scene whose loop ended.
+    // The host reads the flags once, after it; a scene that needed more
+    // iterations continues outside the capture and is integrated then.
+    const int batch = graph ? std::max({2, h->P.min_iter_density, (h->lastIters[0] + 2) & ~1}) : first_batch(h, 0);
+    auto body = [h, batch, graph]() {
         Dev& D = h->D;
         run_ns(h);
         launch_cfl(D, h->P.cfl, h->P.spacing_r, h->P.dt, h->P.dt_min, h->st);
@@ -722,22 +751,55 @@ int step_iisph(sphgpu_handle* h, sphgpu_stats* st) {
             launch_ii_aii(D, h->st);
         }
         solver_loop_start(h, 0, batch);
+        if (graph) launch_ii_force_integrate(h->D, 1, h->st);
     };
     if (h->D.anyDynamic && (rc = rigid_update(h))) return rc;
-    if (h->P.use_graph && !h->D.anyDynamic) {
+    if (graph) {
         if ((rc = run_segment(h, 1, batch, body))) return rc;
     } else {
         body();
     }
-    if ((rc = solver_loop_finish(h, 0, batch, profStart))) return rc;
-    {
+    int extra = 0;
+    if ((rc = solver_loop_finish(h, 0, batch, profStart, &extra))) return rc;
+    if (!graph || extra > 0) {
         ProfScope ps(h, "ii_force", nf_live(h) * 56.0);
-        launch_ii_force_integrate(D, h->st);
+        launch_ii_force_integrate(D, graph ? 2 : 0, h->st);
     }
     rigid_reaction(h, 1);
     std::swap(D.x, D.x2);
     std::swap(D.v, D.v2);
     const int rcb = update_solids_point(h);            // IISph_M.cpp:64
+    rc = finish_step(h, st);
+    return rcb ? rcb : rc;
+}
+
+// IISph::sphStep with the DF_* solvers in place of the VF_* ones (the
+// commented-out calls at IISph.cpp:59-60): search, V_b, DF_prepareAttribute
+// (IISph.cpp:100-288), the non-pressure acceleration (DFSPH's
+// DF_forceExceptPressure, DFSPH.cpp:1419-1475: the IISph class computes it in
+// its VF_* advection step), DF_divergenceFreeSolver (1039-1217),
+// DF_constantDensitySolver (816-1037), updateFluids, updateSolids.  These
+// solvers compute no reaction force on solids.
+int step_iisph_df(sphgpu_handle* h, sphgpu_stats* st) {
+    Dev& D = h->D;
+    int rc;
+    if (D.anyDynamic && (rc = rigid_update(h))) return rc;
+    run_ns(h);
+    launch_cfl(D, h->P.cfl, h->P.spacing_r, h->P.dt, h->P.dt_min, h->st);
+    {
+        ProfScope ps(h, "idf_prepare", nf_live(h) * 24.0);
+        launch_idf_prepare(D, h->st);
+    }
+    {
+        ProfScope ps(h, "df_aadv", nf_live(h) * 40.0);
+        launch_df_aadv(D, h->st);
+    }
+    launch_idf_vadv(D, 0, h->st);
+    if ((rc = solver_loop(h, 1))) return rc;
+    launch_idf_vadv(D, 1, h->st);
+    if ((rc = solver_loop(h, 0))) return rc;
+    launch_df_integrate(D, h->st);
+    const int rcb = update_solids_point(h);
     rc = finish_step(h, st);
     return rcb ? rcb : rc;
 }
@@ -858,6 +920,9 @@ void sphgpu_default_params(sphgpu_params* p) {
     p->learn_rate = 1e-3f;
     p->learn_over_weight = 1.0f;
     p->warm_scale = 0.7f;
+    p->boundary_density = 1000.0f;        // SphBase.h:576
+    p->adjust_density = 0;                // SphBase.h:575
+    p->fluid_capacity = 0;
 }
 
 int sphgpu_device_available(void) {
@@ -871,7 +936,13 @@ int sphgpu_device_available(void) {
 int sphgpu_create(const sphgpu_params* p, sphgpu_handle** out) {
     if (!p || !out) return SPHGPU_E_ARG;
     *out = nullptr;
-    if (p->solver < SPHGPU_WCSPH || p->solver > SPHGPU_PCISPH) return SPHGPU_E_ARG;
+    static_assert(SPHGPU_IISPH_DF == SOLVER_IISPH_DF, "solver id of the IISph DF_* kernels");
+    if (p->solver < SPHGPU_WCSPH || p->solver > SPHGPU_IISPH_DF) return SPHGPU_E_ARG;
+    // the IISph DF_* solvers have no warm start and are not graph-captured
+    if (p->solver == SPHGPU_IISPH_DF && (p->warm_start != SPHGPU_WARM_NONE || p->use_graph)) return SPHGPU_E_ARG;
+    if (!(p->boundary_density > 0.0f) || !std::isfinite(p->boundary_density) || p->adjust_density < 0 ||
+        p->adjust_density > 1 || p->fluid_capacity < 0)
+        return SPHGPU_E_ARG;
     if (!(p->spacing_r > 0) || !(p->smooth_radius_h > 0) || !(p->dt > 0)) return SPHGPU_E_ARG;
     if (p->num_scenes < 1 || p->num_scenes > 4096) return SPHGPU_E_ARG;
     if (p->warm_start < 0 || p->warm_start > 2) return SPHGPU_E_ARG;
@@ -893,6 +964,7 @@ int sphgpu_create(const sphgpu_params* p, sphgpu_handle** out) {
     std::memcpy(h->scenes[0].mx, p->space_max, sizeof(float) * 3);
     h->scenes[0].set = true;
     h->sceneStats.resize(p->num_scenes);
+    h->D.winOn = 1;
     *out = h;
     return SPHGPU_OK;
 }
@@ -949,7 +1021,7 @@ int sphgpu_add_phase(sphgpu_handle* h, float rest_density, float visc_alpha, int
 int sphgpu_add_fluid(sphgpu_handle* h, int scene, int phase, long long n, const float* xyz,
                      const float* vxyz, long long* first_id) {
     if (!h) return SPHGPU_E_ARG;
-    if (h->initialized) return fail(h, SPHGPU_E_STATE, "fluid after init (append is not supported yet)");
+    if (h->initialized) return sphgpu_append_fluid(h, scene, phase, n, xyz, vxyz, first_id);
     if (scene < 0 || scene >= h->P.num_scenes) return fail(h, SPHGPU_E_ARG, "scene %d out of range", scene);
     if (phase < 0 || phase >= (int)h->phases.size()) return fail(h, SPHGPU_E_ARG, "phase %d not defined", phase);
     if (n < 0 || (n > 0 && !xyz)) return fail(h, SPHGPU_E_ARG, "bad particle array");
@@ -962,6 +1034,84 @@ int sphgpu_add_fluid(sphgpu_handle* h, int scene, int phase, long long n, const 
     h->fphase.insert(h->fphase.end(), (size_t)n, phase);
     h->fscene.insert(h->fscene.end(), (size_t)n, scene);
     h->nextId += n;
+    return SPHGPU_OK;
+}
+
+int sphgpu_append_fluid(sphgpu_handle* h, int scene, int phase, long long n, const float* xyz,
+                        const float* vxyz, long long* first_id) {
+    if (!h) return SPHGPU_E_ARG;
+    if (!h->initialized) return sphgpu_add_fluid(h, scene, phase, n, xyz, vxyz, first_id);
+    if (h->inCallback) return fail(h, SPHGPU_E_STATE, "append inside the update-solids callback");
+    if (scene < 0 || scene >= h->P.num_scenes) return fail(h, SPHGPU_E_ARG, "scene %d out of range", scene);
+    if (phase < 0 || phase >= (int)h->phases.size()) return fail(h, SPHGPU_E_ARG, "phase %d not defined", phase);
+    if (n < 0 || (n > 0 && !xyz)) return fail(h, SPHGPU_E_ARG, "bad particle array");
+    const SceneDesc& d = h->scenes[scene];
+    for (long long i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a)
+            if (!(xyz[3 * i + a] >= d.mn[a] && xyz[3 * i + a] <= d.mx[a]))
+                return fail(h, SPHGPU_E_ARG, "appended particle %lld outside its scene domain", i);
+    if (first_id) *first_id = h->nextId;
+    if (n == 0) return SPHGPU_OK;
+    Dev& D = h->D;
+    // the new particles take fresh tiles after the current slots; the next
+    // neighbour search sorts them into the scenes' layout
+    const int slots = int((n + TILE - 1) / TILE * TILE);
+    const long long cap = std::max<long long>(h->P.fluid_capacity, (long long)h->fscene.size() - h->appended);
+    if (h->nLive + n > cap)
+        return fail(h, SPHGPU_E_CAPACITY, "appending %lld particles to %lld exceeds fluid_capacity %lld", n, h->nLive,
+                    cap);
+    if ((long long)D.npad + slots > D.ncap)
+        return fail(h, SPHGPU_E_CAPACITY, "appending %lld particles exceeds the fluid capacity (%d slots in use of %d;"
+                    " set sphgpu_params.fluid_capacity)", n, D.npad, D.ncap);
+    if (h->nextId + n > (1LL << 30)) return fail(h, SPHGPU_E_CAPACITY, "too many fluid particles");
+    HIPCHECK(h, hipStreamSynchronize(h->st));   // a step without statistics may still run
+    std::vector<float4> hx(slots, make_float4(0, 0, 0, 0)), hv(slots, make_float4(0, 0, 0, 0));
+    std::vector<int> hid(slots, -1);
+    std::vector<float> hk(slots, h->P.solver == SPHGPU_IISPH_DF ? 1.0f : 0.0f), hz(slots, 0.0f),
+        hr(slots, h->phases[phase].rho0);
+    std::vector<uint32_t> hkey(slots, KEY_INVALID);
+    for (long long i = 0; i < n; ++i) {
+        hx[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], float(phase));
+        if (vxyz) hv[i] = make_float4(vxyz[3 * i], vxyz[3 * i + 1], vxyz[3 * i + 2], 0.0f);
+        hid[i] = int(h->nextId + i);
+    }
+    const size_t o = (size_t)D.npad;
+    HIPCHECK(h, hipMemcpy(D.x + o, hx.data(), sizeof(float4) * slots, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(D.v + o, hv.data(), sizeof(float4) * slots, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(D.id + o, hid.data(), sizeof(int) * slots, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(D.ksum + o, hk.data(), sizeof(float) * slots, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(D.kvsum + o, hz.data(), sizeof(float) * slots, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(D.rho + o, hr.data(), sizeof(float) * slots, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(D.key + o, hkey.data(), sizeof(uint32_t) * slots, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(D.key2 + o, hkey.data(), sizeof(uint32_t) * slots, hipMemcpyHostToDevice));
+    // the key pass picks each slot's grid by its tile's scene
+    std::vector<int> ts(slots / TILE, scene), tv(slots / TILE, 0);
+    for (int t = 0; t < slots / TILE; ++t) tv[t] = int(std::min<long long>(TILE, n - (long long)t * TILE));
+    HIPCHECK(h, hipMemcpy(D.tileScene + o / TILE, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(D.tileValid + o / TILE, tv.data(), sizeof(int) * tv.size(), hipMemcpyHostToDevice));
+    D.npad += slots;
+    h->nLive += n;
+    h->needSearch = true;
+    h->fx.insert(h->fx.end(), xyz, xyz + 3 * n);
+    if (vxyz) h->fv.insert(h->fv.end(), vxyz, vxyz + 3 * n);
+    else h->fv.resize(h->fx.size(), 0.0f);
+    h->fphase.insert(h->fphase.end(), (size_t)n, phase);
+    h->fscene.insert(h->fscene.end(), (size_t)n, scene);
+    if (!h->fbeta.empty()) h->fbeta.resize(h->fphase.size(), 1.0f);
+    h->nextId += n;
+    h->appended += n;
+    return SPHGPU_OK;
+}
+
+int sphgpu_set_fluid_beta(sphgpu_handle* h, long long first, long long n, const float* beta) {
+    if (!h || !beta || first < 0 || n < 0) return SPHGPU_E_ARG;
+    if (h->initialized) return fail(h, SPHGPU_E_STATE, "beta is set before sphgpu_init");
+    const long long nf = (long long)h->fscene.size();
+    if (first + n > nf) return fail(h, SPHGPU_E_ARG, "particles [%lld, %lld) beyond the %lld added", first, first + n, nf);
+    for (long long i = 0; i < n; ++i)
+        if (!(beta[i] > 0.0f) || !std::isfinite(beta[i])) return fail(h, SPHGPU_E_ARG, "beta must be positive");
+    if (h->fbeta.size() < (size_t)nf) h->fbeta.resize(nf, 1.0f);
+    std::copy(beta, beta + n, h->fbeta.begin() + first);
     return SPHGPU_OK;
 }
 
@@ -1100,7 +1250,13 @@ int sphgpu_init(sphgpu_handle* h) {
     for (int s : h->fscene) ++count[s];
     for (int s = 0; s < ns; ++s) padStart[s + 1] = padStart[s] + (count[s] + TILE - 1) / TILE * TILE;
     int npad = padStart[ns];
-    int ncap = std::max(npad, TILE);
+    // room for particles appended later (fluid_capacity): every scene's
+    // layout rounds up to whole tiles, so one spare tile per scene
+    const long long capSlots = P.fluid_capacity > 0
+                                   ? ((long long)P.fluid_capacity + TILE - 1) / TILE * TILE + (long long)ns * TILE
+                                   : 0;
+    if (capSlots > (1LL << 30)) return fail(h, SPHGPU_E_CAPACITY, "fluid_capacity too large");
+    int ncap = std::max({npad, TILE, int(capSlots)});
     long long nb = (long long)h->bscene.size();
 
     Dev& D = h->D;
@@ -1146,6 +1302,8 @@ int sphgpu_init(sphgpu_handle* h) {
         const double v0 = sd * sd * sd, beta = 2.0 * v0 * v0;
         D.pci_delta = float(-1.0 / (beta * (-(gs[0] * gs[0] + gs[1] * gs[1] + gs[2] * gs[2]) - g2)));
     }
+    D.boundaryDensity = P.boundary_density;
+    D.adjustDensity = P.adjust_density;
     D.spacing = s;
     D.dt_max = P.dt;
     D.warm_scale = P.warm_scale;
@@ -1191,6 +1349,7 @@ int sphgpu_init(sphgpu_handle* h) {
     ALLOC(D.nbF, (size_t)NBF_CAP * ncap); ALLOC(D.nbB, (size_t)NBB_CAP * ncap); ALLOC(D.cnt, ncap);
     ALLOC(D.nbF16, (size_t)(NBF_CAP / 8) * ncap); ALLOC(D.nbB16, (size_t)(NBB_CAP / 8) * ncap);
     ALLOC(D.nbTab, (size_t)NB16_TAB * (ncap / TILE));
+    ALLOC(D.nbLen, (size_t)32 * (ncap / TILE));
     // boundary arrays carry one extra dummy particle at index nb (list padding)
     ALLOC(D.xb, nb + 1); ALLOC(D.vb, nb + 1); ALLOC(D.bid, nb + 1); ALLOC(D.bsolid, nb + 1);
     ALLOC(D.bscene, nb + 1); ALLOC(D.bkself, nb + 1); ALLOC(D.bkey, nb + 1);
@@ -1225,6 +1384,7 @@ int sphgpu_init(sphgpu_handle* h) {
         // host reads it with one copy
         double* loopBlk = nullptr;
         ALLOC(loopBlk, 3 * (size_t)ns);
+        ALLOC(D.integ, ns);
         D.active = reinterpret_cast<int*>(loopBlk);
         D.iters = reinterpret_cast<int*>(loopBlk + ns);
         D.sceneErr = loopBlk + 2 * ns;
@@ -1237,6 +1397,12 @@ int sphgpu_init(sphgpu_handle* h) {
     HIPCHECK(h, hipMalloc(&sortMem, sort_scratch_bytes(sortItems)));
     h->allocs.push_back(sortMem);
     sort_scratch_bind(h->sort, sortMem, sortItems);
+    if (P.solver == SPHGPU_IISPH_DF) {
+        ALLOC(D.pm, ncap);
+        ALLOC(D.brho, nb + 1); ALLOC(D.balpha, nb + 1); ALLOC(D.bk, nb + 1);
+        HIPCHECK(h, hipMemset(D.bk, 0, sizeof(float) * (nb + 1)));   // the list-padding dummy stays 0
+        HIPCHECK(h, hipMemset(D.pm, 0, sizeof(float) * ncap));
+    }
     if (P.solver == SPHGPU_DFSPH && P.warm_start == SPHGPU_WARM_LEARNED) {
         ALLOC(h->feat, 8 * (size_t)ncap); ALLOC(h->tgt, ncap); ALLOC(h->ypred, ncap);
         ALLOC(h->bfeat, 8 * (size_t)P.learn_batch); ALLOC(h->btgt, P.learn_batch);
@@ -1308,6 +1474,13 @@ int sphgpu_init(sphgpu_handle* h) {
         HIPCHECK(h, hipMemcpy(D.sceneStart, sStart.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
         std::vector<float> dt0(ns, P.dt);
         HIPCHECK(h, hipMemcpy(D.dt, dt0.data(), sizeof(float) * ns, hipMemcpyHostToDevice));
+        if (P.solver == SPHGPU_IISPH_DF) {
+            // per-particle beta (default 1) rides in ksum
+            std::vector<float> hb(ncap, 1.0f);
+            for (int q = 0; q < ncap; ++q)
+                if (hid[q] >= 0 && !h->fbeta.empty()) hb[q] = h->fbeta[hid[q]];
+            HIPCHECK(h, hipMemcpy(D.ksum, hb.data(), sizeof(float) * ncap, hipMemcpyHostToDevice));
+        }
         h->nLive = (long long)h->fscene.size();
     }
     // boundary: sort by cell once, Akinci volumes (prepareSolidPartWeight)
@@ -1371,6 +1544,7 @@ int sphgpu_step(sphgpu_handle* h, sphgpu_stats* st) {
         case SPHGPU_WCSPH: return step_wcsph(h, st);
         case SPHGPU_DFSPH: return step_dfsph(h, st);
         case SPHGPU_PCISPH: return step_pcisph(h, st);
+        case SPHGPU_IISPH_DF: return step_iisph_df(h, st);
         default: return step_iisph(h, st);
     }
 }
@@ -1761,6 +1935,12 @@ int sphgpu_set_tuning(sphgpu_handle* h, int tile_map, int stage_groups, int stre
     h->tuneFix[0] = tile_map;
     h->tuneFix[1] = stage_groups;
     h->tuneFix[2] = stream_lists;
+    return SPHGPU_OK;
+}
+
+int sphgpu_set_lds_windows(sphgpu_handle* h, int on) {
+    if (!h || on < 0 || on > 1) return SPHGPU_E_ARG;
+    h->D.winOn = on;
     return SPHGPU_OK;
 }
 

@@ -16,20 +16,6 @@ __device__ __forceinline__ bool slot_valid(const Dev& D, int i, int* scene) {
     return (i - tile * TILE) < D.tileValid[tile];
 }
 
-// cell coordinates of a point inside scene grid G, clamped into the grid
-__device__ __forceinline__ void cell_of(const SceneGrid& G, float4 p, int& cx, int& cy, int& cz) {
-    cx = cell_coord(p.x, G.minx, G.divisor);
-    cy = cell_coord(p.y, G.miny, G.divisor);
-    cz = cell_coord(p.z, G.minz, G.divisor);
-    cx = min(max(cx, 0), G.nx - 1);
-    cy = min(max(cy, 0), G.ny - 1);
-    cz = min(max(cz, 0), G.nz - 1);
-}
-
-__device__ __forceinline__ uint32_t key_of(const SceneGrid& G, const int* rowRank, int cx, int cy, int cz) {
-    int row = rowRank[G.rowBase + cz * G.ny + cy];
-    return G.keyBase + (uint32_t)row * (uint32_t)G.nx + (uint32_t)cx;
-}
 
 __global__ __launch_bounds__(256) void k_fluid_keys(Dev D, uint32_t* keys, uint32_t* vals) {
     int i = blockIdx.x * 256 + threadIdx.x;
@@ -148,20 +134,6 @@ __global__ __launch_bounds__(256) void k_cell_build(const uint32_t* __restrict__
     if (i == n - 1 || keys[i + 1] != k) table[k].y = i + 1;
 }
 
-// union of the [start,end) ranges of three consecutive keys (empty = start >= end)
-__device__ __forceinline__ int2 run3(const int2* __restrict__ table, uint32_t k0, int cx, int nx) {
-    int s = 0x7fffffff, e = 0;
-    for (int dx = -1; dx <= 1; ++dx) {
-        int c = cx + dx;
-        if (c < 0 || c >= nx) continue;
-        int2 r = table[k0 + dx];
-        if (r.y > r.x) {
-            s = min(s, r.x);
-            e = max(e, r.y);
-        }
-    }
-    return make_int2(s, e);
-}
 
 // Lists are built in LDS, one column per thread, and written out group by
 // group (16 bytes per particle), so each store instruction of a wave writes
@@ -365,6 +337,9 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
         else if (tid == 32 + NB16_DUMMY) v = D.nb - (NB16_DUMMY << NB16_OFF_BITS);
         else v = sBase[tid] != INT_MAX ? sBase[tid] - (slot << NB16_OFF_BITS) : 0;
         D.nbTab[(size_t)tile * NB16_TAB + tid] = v;
+        if (tid < 32)
+            D.nbLen[(size_t)tile * 32 + tid] =
+                (packed && tid != NB16_FLAG && sBase[tid] != INT_MAX) ? sEnd[tid] - sBase[tid] : 0;
     }
     // stored fluid entries | fluid neighbours found << 8 | boundary << 16 | phase << 24
     if (i < D.npad) D.cnt[i] = cf | (act ? min(nf, 255) : 0) << 8 | (cb << 16) | ((act ? int(pa.w) : 0) << 24);

@@ -11,7 +11,7 @@ constexpr int MLP_FWD_FRAGS = 22;    // BF16 A fragments of the forward pass
 constexpr int MLP_BWD_FRAGS = 20;    // BF16 A fragments of the backward pass
 constexpr int MLP_STORE = 196;       // per-sample row: z1 z2 z3 (3 x 64), y at 192
 constexpr int MLP_WGRAD_TILES = 53;  // 40 weight tiles + 13 bias tiles
-constexpr int MLP_WGRAD_SLICE = 512; // samples per weight-gradient block
+constexpr int MLP_WGRAD_SLICE = 128; // samples per weight-gradient block
 
 struct MlpDev {
     float* params = nullptr;    // FP32 master parameters (W1 b1 W2 b2 W3 b3 W4 b4)

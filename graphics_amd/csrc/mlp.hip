@@ -304,13 +304,6 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const bf16x8* __restrict__
     }
 }
 
-// value of the layer input h_L[unit] for sample p (h_0 = features)
-__device__ __forceinline__ float layer_input(int L, const float* __restrict__ feat, const float* __restrict__ store,
-                                             int p, int unit) {
-    if (L == 0) return unit < 8 ? feat[(size_t)p * 8 + unit] : 0.0f;
-    return gelu(store[(size_t)p * MLP_STORE + (L - 1) * 64 + unit]);
-}
-
 // Tile t of the weight/bias gradient: 16x16 blocks of dW (out x in) per
 // layer, then bias blocks.  L1 4x1, L2 4x4, L3 4x4, L4 1x4 (W), then bias
 // tiles 4+4+4+1.
@@ -335,39 +328,80 @@ __device__ __forceinline__ GradTile grad_tile(int t) {
 }
 
 // Weight and bias gradients, K = samples split into slices of
-// MLP_WGRAD_SLICE: block (tile, slice) is one wave that accumulates its 16x16
-// tile over the slice's samples in chunks of 32 on the matrix cores and
-// stores the partial tile; k_mlp_wgrad_reduce sums the slices in a fixed
-// order (deterministic).
-__global__ __launch_bounds__(64) void k_mlp_wgrad(const float* __restrict__ feat, const float* __restrict__ store,
-                                                const float* __restrict__ dz, float* __restrict__ part, int n) {
-    const int tile = blockIdx.x % MLP_WGRAD_TILES, slice = blockIdx.x / MLP_WGRAD_TILES;
-    const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
-    const GradTile T = grad_tile(tile);
-    const int nin = (T.L == 0) ? 8 : 64, nout = (T.L == 3) ? 1 : 64;
-    // dz of layer L output lives at store row offset: L<3 -> L*64, L==3 -> 192
-    const int zoff = (T.L < 3) ? T.L * 64 : 192;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    const int orow = 16 * T.to + c;          // A row (output unit)
-    const int icol = 16 * T.ti + c;          // B column (input unit)
+// MLP_WGRAD_SLICE: one block per slice.  For each chunk of 32 samples the
+// block stages in LDS, once and with row-contiguous loads, the chunk's
+// pre-activation gradients dz (the A operand: output unit x sample) and its
+// layer inputs h = features, GELU(z1), GELU(z2), GELU(z3) (the B operand:
+// sample x input unit), both as BF16 and transposed to unit-major so that a
+// fragment's eight samples are two 8-byte LDS reads; the four waves then
+// accumulate the 53 tiles (each wave every fourth) on the matrix cores.
+// k_mlp_wgrad_reduce sums the slices in a fixed order (deterministic).
+constexpr int WG_CHUNK = 32;
+constexpr int WG_ROW = WG_CHUNK + 4;     // LDS row pitch in BF16 (8-byte aligned, staggered banks)
+constexpr int WG_DZ = 193;               // dz rows: z1 z2 z3 (192) and y
+constexpr int WG_H = 200;                // input rows: features (8), h1, h2, h3
+constexpr int WG_TILES_PER_WAVE = (MLP_WGRAD_TILES + 3) / 4;
+
+__device__ __forceinline__ bf16x8 lds_frag(const __bf16* row, int g) {
+    // samples kmap(g, 0..3) = 4g..4g+3 and kmap(g, 4..7) = 16+4g..16+4g+3
+    const uint2 lo = *reinterpret_cast<const uint2*>(row + 4 * g);
+    const uint2 hi = *reinterpret_cast<const uint2*>(row + 16 + 4 * g);
+    bf16x8 v;
+    __builtin_memcpy(&v, &lo, 8);
+    __builtin_memcpy(reinterpret_cast<char*>(&v) + 8, &hi, 8);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_wgrad(const float* __restrict__ feat, const float* __restrict__ store,
+                                                 const float* __restrict__ dz, float* __restrict__ part, int n) {
+    __shared__ __attribute__((aligned(16))) __bf16 sA[WG_DZ * WG_ROW];
+    __shared__ __attribute__((aligned(16))) __bf16 sB[(WG_H + 1) * WG_ROW];   // last row: bias input 1 / 0
+    const int slice = blockIdx.x, tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63, g = lane >> 4, c = lane & 15;
+    f32x4 acc[WG_TILES_PER_WAVE];
+#pragma unroll
+    for (int k = 0; k < WG_TILES_PER_WAVE; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int p0 = slice * MLP_WGRAD_SLICE, p1 = min(n, p0 + MLP_WGRAD_SLICE);
-    for (int q = p0; q < p1; q += 32) {
-        bf16x8 a, b;
-        for (int j = 0; j < 8; ++j) {
-            int p = q + kmap(g, j);
-            bool ok = p < p1;
-            float dv = (ok && orow < nout) ? dz[(size_t)p * MLP_STORE + zoff + orow] : 0.0f;
-            float hv;
-            if (T.bias) hv = ok ? 1.0f : 0.0f;
-            else hv = (ok && icol < nin) ? layer_input(T.L, feat, store, p, icol) : 0.0f;
-            a[j] = to_bf16(dv);
-            b[j] = to_bf16(hv);
+    for (int q = p0; q < p1; q += WG_CHUNK) {
+        __syncthreads();   // the previous chunk's fragments are read
+        // the chunk's store and dz rows are contiguous: 32 x 196 floats each
+        for (int e = tid; e < WG_CHUNK * MLP_STORE; e += 256) {
+            const int smp = e / MLP_STORE, col = e - smp * MLP_STORE;
+            const int p = q + smp;
+            const bool ok = p < p1;
+            if (col < 192) sB[(8 + col) * WG_ROW + smp] = to_bf16(ok ? gelu(store[(size_t)p * MLP_STORE + col]) : 0.0f);
+            if (col < WG_DZ) sA[col * WG_ROW + smp] = to_bf16(ok ? dz[(size_t)p * MLP_STORE + col] : 0.0f);
         }
-        acc = mfma(a, b, acc);
+        for (int e = tid; e < WG_CHUNK * 8; e += 256) {
+            const int smp = e >> 3, u = e & 7, p = q + smp;
+            sB[u * WG_ROW + smp] = to_bf16(p < p1 ? feat[(size_t)p * 8 + u] : 0.0f);
+        }
+        if (tid < WG_CHUNK) sB[WG_H * WG_ROW + tid] = to_bf16(q + tid < p1 ? 1.0f : 0.0f);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < WG_TILES_PER_WAVE; ++k) {
+            const int t = wave + 4 * k;
+            if (t >= MLP_WGRAD_TILES) break;
+            const GradTile T = grad_tile(t);
+            const int nin = (T.L == 0) ? 8 : 64, nout = (T.L == 3) ? 1 : 64;
+            const int zoff = (T.L < 3) ? T.L * 64 : 192;            // dz rows of layer L's output
+            const int hoff = (T.L == 0) ? 0 : 8 + (T.L - 1) * 64;   // rows of layer L's input
+            const int orow = 16 * T.to + c, icol = 16 * T.ti + c;
+            const bf16x8 zero = bf16x8{};
+            const bf16x8 a = orow < nout ? lds_frag(sA + (zoff + orow) * WG_ROW, g) : zero;
+            const bf16x8 b = T.bias ? lds_frag(sB + WG_H * WG_ROW, g)
+                                    : (icol < nin ? lds_frag(sB + (hoff + icol) * WG_ROW, g) : zero);
+            acc[k] = mfma(a, b, acc[k]);
+        }
     }
-    // D[4g+j][c]
-    float* out = part + ((size_t)slice * MLP_WGRAD_TILES + tile) * 256;
-    for (int j = 0; j < 4; ++j) out[(4 * g + j) * 16 + c] = acc[j];
+    // D[4g+j][c] of each tile
+#pragma unroll
+    for (int k = 0; k < WG_TILES_PER_WAVE; ++k) {
+        const int t = wave + 4 * k;
+        if (t >= MLP_WGRAD_TILES) break;
+        float* out = part + ((size_t)slice * MLP_WGRAD_TILES + t) * 256;
+        for (int j = 0; j < 4; ++j) out[(4 * g + j) * 16 + c] = acc[k][j];
+    }
 }
 
 // thread per (tile, element): sum of the slices, written to its parameter
@@ -450,7 +484,7 @@ void mlp_grad(MlpDev& m, const float* feat, const float* target, int n, hipStrea
     hipLaunchKernelGGL(k_mlp_backward, dim3(blocks), dim3(256), 0, st, (const bf16x8*)m.bwd,
                        (const float*)m.store, (const float*)m.dy, m.dz, n);
     const int slices = (n + MLP_WGRAD_SLICE - 1) / MLP_WGRAD_SLICE;
-    hipLaunchKernelGGL(k_mlp_wgrad, dim3(MLP_WGRAD_TILES * slices), dim3(64), 0, st, feat, (const float*)m.store,
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3(slices), dim3(256), 0, st, feat, (const float*)m.store,
                        (const float*)m.dz, m.gradPart, n);
     hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3(MLP_WGRAD_TILES), dim3(256), 0, st, (const float*)m.gradPart, m.grad,
                        slices, 1.0f);

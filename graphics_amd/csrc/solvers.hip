@@ -159,6 +159,112 @@ struct Rec2 {
     float4 a, b;
 };
 
+// ---- LDS windows --------------------------------------------------------
+// The fluid neighbours of a packed tile all lie in its slot runs (one
+// contiguous run of the sorted particles per (own row, stencil row) pair), so
+// the concatenation of those runs -- the tile's window, about ten times the
+// tile's own 256 particles -- holds every record the tile's neighbour loops
+// read.  A pass stages the window in LDS with coalesced loads (each record
+// is needed by about 28 lanes of the tile) and the loops then read it from
+// LDS instead of gathering per lane through the L1.  A list entry e
+// (slot << NB16_OFF_BITS | offset) decodes to the window index
+// g_wintab[e >> NB16_OFF_BITS] + e.  Tiles with 32-bit lists, or whose window
+// exceeds WIN_MAX records, keep the per-lane gathers (the same sums in the
+// same order, so both forms give identical results).
+constexpr int WIN_MAX = 3072;
+__shared__ int g_wintab[32];   // window offset of each fluid slot - (slot << NB16_OFF_BITS)
+__shared__ int g_winoff[33];   // window offset of each fluid slot; [32] = window size
+
+// Window layout of the block's tile (every thread calls it, after slot_nb):
+// returns the window size, or 0 when the tile's loops gather from memory.
+__device__ __forceinline__ int win_layout(const Dev& D, const Slot& s) {
+    if (!s.packed || !D.winOn) return 0;   // uniform per block
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const int len = lane < 32 ? D.nbLen[(size_t)s.tile * 32 + lane] : 0;
+        int incl = len;
+        for (int off = 1; off < 32; off <<= 1) {
+            const int t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
+        }
+        if (lane < 32) {
+            g_winoff[lane] = incl - len;
+            g_wintab[lane] = (incl - len) - (lane << NB16_OFF_BITS);
+        }
+        if (lane == 31) g_winoff[32] = incl;
+    }
+    __syncthreads();
+    const int n = g_winoff[32];
+    return n <= WIN_MAX ? n : 0;
+}
+
+// sorted particle index of window record k (0 <= k < window size): the slot
+// holding k is the largest s with g_winoff[s] <= k (empty slots share their
+// offset with the next one)
+__device__ __forceinline__ int win_particle(int k) {
+    int s = 0;
+#pragma unroll
+    for (int b = 16; b > 0; b >>= 1)
+        if (g_winoff[s + b] <= k) s += b;
+    return g_nbtab[s] + (s << NB16_OFF_BITS) + (k - g_winoff[s]);
+}
+
+// Stages the window: load(j) reads what a record needs from particle j,
+// store(k, r) writes it to window record k; four records per thread in flight.
+template <class LD, class ST>
+__device__ __forceinline__ void win_stage(int n, LD&& load, ST&& store) {
+    for (int k0 = threadIdx.x; k0 < n; k0 += 4 * 256) {
+        int kk[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) kk[u] = k0 + 256 * u;
+        decltype(load(0)) r[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) r[u] = load(win_particle(min(kk[u], n - 1)));
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (kk[u] < n) store(kk[u], r[u]);
+    }
+    __syncthreads();
+}
+
+// Neighbour loop over a packed fluid list whose records are in the LDS
+// window: load(k) reads window record k, use(k, r) consumes it.  The next
+// list group is requested before the current one is used (list loads and
+// LDS reads wait on separate counters); entries past the padded count are
+// read from record 0 and not used.
+template <class LD, class US>
+__device__ __forceinline__ void each_in_win16(const int4* __restrict__ list, int ncap, int i, int n, bool nt,
+                                              LD&& load, US&& use) {
+    const int4* __restrict__ col = list + i;
+    const int ng = (n + 7) >> 3;
+    if (ng <= 0) return;
+    const int n4 = (n + 3) & ~3;
+    int4 cur = ld_group(&col[0], nt);
+    for (int g = 0; g < ng; ++g) {
+        int4 nxt = cur;
+        if (g + 1 < ng) nxt = ld_group(&col[(size_t)(g + 1) * ncap], nt);
+        const unsigned w[4] = {(unsigned)cur.x, (unsigned)cur.y, (unsigned)cur.z, (unsigned)cur.w};
+        const bool second = 8 * g + 4 < n4;
+        int k[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const unsigned e = (w[q >> 1] >> (16 * (q & 1))) & 0xffffu;
+            const int kq = g_wintab[e >> NB16_OFF_BITS] + (int)e;
+            k[q] = (q < 4 || second) ? kq : 0;
+        }
+        decltype(load(0)) r[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) r[q] = load(k[q]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) use(k[q], r[q]);
+        if (second) {
+#pragma unroll
+            for (int q = 4; q < 8; ++q) use(k[q], r[q]);
+        }
+        cur = nxt;
+    }
+}
+
 __device__ __forceinline__ int phase_of(float4 x) { return (int)x.w; }
 
 __device__ __forceinline__ void write_partial(const Dev& D, int tile, int q, double v, double* lds) {
@@ -339,23 +445,32 @@ __device__ __forceinline__ float density_sum(const Dev& D, const Slot& s, float4
     return rho.s;
 }
 
+// one pair's term g(r) (v*_a - v*_b).x_ab of the density change rate
+__device__ __forceinline__ float dchange_term(const Dev& D, float4 pa, float4 va, float bx, float by, float bz,
+                                              float ux, float uy, float uz) {
+    const float dx = pa.x - bx, dy = pa.y - by, dz = pa.z - bz;
+    const float g = kernel_gradfac(fast_sqrt(dot3f(dx, dy, dz, dx, dy, dz)), D);
+    return g * dot3f(va.x - ux, va.y - uy, va.z - uz, dx, dy, dz);
+}
+
+// sum_b V_b gradW.(v*_a - v_b) over the boundary neighbours
+__device__ __forceinline__ float dchange_boundary(const Dev& D, const Slot& s, float4 pa, float4 va, int c) {
+    float sb = 0.0f;
+    each_boundary_lu<4>(D, s, nb_of(c), [&](int j) { return Rec2{D.xb[j], D.vb[j]}; }, [&](int j, Rec2 rec) {
+        sb += rec.a.w * dchange_term(D, pa, va, rec.a.x, rec.a.y, rec.a.z, rec.b.x, rec.b.y, rec.b.z);
+    });
+    return sb;
+}
+
 // sum_f m gradW.(v*_a - v*_b) + sum_b rho0 V_b gradW.(v*_a - v_b): the
 // predicted density change rate (DFSPH.cpp:589-617, 795-825)
 __device__ __forceinline__ float density_change(const Dev& D, const Slot& s, float4 pa, float4 va, float m,
                                                 float rho0, int c) {
     float sf = 0.0f, sb = 0.0f;
     each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.vs[j]}; }, [&](int j, Rec2 rec) {
-        const float4 pb = rec.a, vb = rec.b;
-        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
-        sf += g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
+        sf += dchange_term(D, pa, va, rec.a.x, rec.a.y, rec.a.z, rec.b.x, rec.b.y, rec.b.z);
     });
-    each_boundary_lu<4>(D, s, nb_of(c), [&](int j) { return Rec2{D.xb[j], D.vb[j]}; }, [&](int j, Rec2 rec) {
-        const float4 pb = rec.a, vb = rec.b;
-        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
-        sb += pb.w * g * ((va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz);
-    });
+    sb = dchange_boundary(D, s, pa, va, c);
     return m * sf + rho0 * sb;
 }
 
@@ -563,7 +678,7 @@ __global__ __launch_bounds__(256) void k_df_aadv(Dev D) {
     float dt = D.dt[s.scene];
     D.f4a[i] = make_float4(ax, ay, az, 0.0f);
     D.vs[i] = make_float4(va.x + ax * dt, va.y + ay * dt, va.z + az * dt, rhoa);   // w: rho, as in v
-    D.ksum[i] = 0.0f;
+    if (D.solver != SOLVER_IISPH_DF) D.ksum[i] = 0.0f;   // (there ksum carries beta)
 }
 
 // Warm start of the divergence loop (not in the reference).  The stiffness
@@ -665,18 +780,65 @@ __global__ __launch_bounds__(256) void k_mlp_gather(Dev D, const float* __restri
     btgt[k] = tgt[slot];
 }
 
+// Window records of the two-record passes: position and v* of each
+// neighbour, 24 bytes in two LDS arrays (x, y) and (z, v*x, v*y, v*z), each
+// read with one aligned LDS load.
+struct WinXV {
+    float2* xy;
+    float4* zv;
+};
+struct RecW {
+    float2 a;
+    float4 b;
+};
+
+__device__ __forceinline__ void stage_xv(const Dev& D, int n, WinXV w) {
+    win_stage(n, [&](int j) { return Rec2{D.x[j], D.vs[j]}; }, [&](int k, Rec2 r) {
+        w.xy[k] = make_float2(r.a.x, r.a.y);
+        w.zv[k] = make_float4(r.a.z, r.b.x, r.b.y, r.b.z);
+    });
+}
+
+// density_change with the fluid records read from the LDS window
+__device__ __forceinline__ float density_change_win(const Dev& D, const Slot& s, float4 pa, float4 va, float m,
+                                                    float rho0, int c, WinXV w) {
+    float sf = 0.0f;
+    const bool nt = __builtin_amdgcn_readfirstlane(D.listNT) != 0;
+    each_in_win16(D.nbF16, D.ncap, s.i, nf_of(c), nt, [&](int k) { return RecW{w.xy[k], w.zv[k]}; },
+                  [&](int, RecW rec) {
+        sf += dchange_term(D, pa, va, rec.a.x, rec.a.y, rec.b.x, rec.b.y, rec.b.z, rec.b.w);
+    });
+    const float sb = dchange_boundary(D, s, pa, va, c);
+    return m * sf + rho0 * sb;
+}
+
+// LDS of the two-record window (WIN_MAX records of 8 + 16 bytes)
+#define WIN_XV_DECL                              \
+    __shared__ float2 s_wxy[WIN_MAX];            \
+    __shared__ float4 s_wzv[WIN_MAX];            \
+    const WinXV win{s_wxy, s_wzv};
+
 // pass A of the density loop (DFSPH.cpp:572-643): rho*, clamp rho* >= rho0,
 // kappa = (rho*-rho0)/dt^2 alpha -> xk.w = kappa/rho; partial sum of rho*/rho0.
 __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
     __shared__ double lds[4];
+    WIN_XV_DECL
     Slot s = slot_nb(D, true);
+    if (!D.active[s.scene]) {   // uniform: a converged scene's tiles contribute nothing
+        write_partial(D, s.tile, 0, 0.0, lds);
+        return;
+    }
+    const int nw = win_layout(D, s);
+    if (nw) stage_xv(D, nw, win);
     double contrib = 0.0;
     if (s.live) {
         const int i = s.i;
         float4 pa = D.x[i], va = D.vs[i];
         int ph = phase_of(pa);
         float m = D.mass[ph], rho0 = D.rho0[ph];
-        float dd = density_change(D, s, pa, va, m, rho0, D.cnt[i]);
+        const int c = D.cnt[i];
+        float dd = nw ? density_change_win(D, s, pa, va, m, rho0, c, win)
+                      : density_change(D, s, pa, va, m, rho0, c);
         float dt = D.dt[s.scene];
         float rhoa = va.w;   // v* carries rho in w (as v does), one array fewer to stream
         float rs = rhoa + dt * dd;
@@ -699,14 +861,23 @@ __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
 // oscillates).
 __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected) {
     __shared__ double lds[4];
+    WIN_XV_DECL
     Slot s = slot_nb(D, true);
+    if (!D.active[s.scene]) {   // uniform: a converged scene's tiles contribute nothing
+        write_partial(D, s.tile, 0, 0.0, lds);
+        return;
+    }
+    const int nw = win_layout(D, s);
+    if (nw) stage_xv(D, nw, win);
     double contrib = 0.0;
     if (s.live) {
         const int i = s.i;
         float4 pa = D.x[i], va = D.vs[i];
         int ph = phase_of(pa);
         float m = D.mass[ph], rho0 = D.rho0[ph];
-        float dd = density_change(D, s, pa, va, m, rho0, D.cnt[i]);
+        const int c = D.cnt[i];
+        float dd = nw ? density_change_win(D, s, pa, va, m, rho0, c, win)
+                      : density_change(D, s, pa, va, m, rho0, c);
         float dt = D.dt[s.scene];
         float rhoa = va.w;   // v* carries rho in w (as v does), one array fewer to stream
         if (!projected) {
@@ -733,7 +904,11 @@ __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected
 // pass B of both loops (DFSPH.cpp:647-693, 840-883):
 // v* -= dt (m sum (kr_a + kr_b) gradW + rho0 sum V_b kr_a gradW), kr = xk.w
 __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
+    __shared__ float4 s_wxk[WIN_MAX];   // window records: xk of each neighbour
     Slot s = slot_nb(D, true);
+    if (!D.active[s.scene]) return;     // uniform per block
+    const int nw = win_layout(D, s);
+    if (nw) win_stage(nw, [&](int j) { return D.xk[j]; }, [&](int k, float4 r) { s_wxk[k] = r; });
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.xk[i];
@@ -742,11 +917,17 @@ __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
     float m = D.mass[ph], rho0 = D.rho0[ph];
     float ka = pa.w;
     float fx = 0.f, fy = 0.f, fz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
-    each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, [&](int j, float4 pb) {
+    auto fluid = [&](int, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * (ka + pb.w);
+        float g = kernel_gradfac(fast_sqrt(dot3f(dx, dy, dz, dx, dy, dz)), D) * (ka + pb.w);
         fx += dx * g; fy += dy * g; fz += dz * g;
-    });
+    };
+    if (nw) {
+        const bool nt = __builtin_amdgcn_readfirstlane(D.listNT) != 0;
+        each_in_win16(D.nbF16, D.ncap, i, nf_of(c), nt, [&](int k) { return s_wxk[k]; }, fluid);
+    } else {
+        each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, fluid);
+    }
     each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
@@ -884,7 +1065,11 @@ __global__ __launch_bounds__(256) void k_ii_aii(Dev D) {
 // stores u = d_ii p_l + sum_j d_ij p_j, the per-particle part of the
 // Jacobi update that neighbours read.
 __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
+    __shared__ float4 s_wxk[WIN_MAX];   // window records: xk of each neighbour
     Slot s = slot_nb(D, true);
+    if (!D.active[s.scene]) return;     // uniform per block
+    const int nw = win_layout(D, s);
+    if (nw) win_stage(nw, [&](int j) { return D.xk[j]; }, [&](int k, float4 r) { s_wxk[k] = r; });
     if (!s.live) return;
     const int i = s.i;
     // own position from the xk records the pass gathers (xyz = x during
@@ -893,11 +1078,17 @@ __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
     const int c = D.cnt[i];
     float m = D.mass[phase_of_cnt(c)];
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, [&](int j, float4 pb) {
+    auto fluid = [&](int, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
+        float g = kernel_gradfac(fast_sqrt(dot3f(dx, dy, dz, dx, dy, dz)), D) * pb.w;
         sx += dx * g; sy += dy * g; sz += dz * g;
-    });
+    };
+    if (nw) {
+        const bool nt = __builtin_amdgcn_readfirstlane(D.listNT) != 0;
+        each_in_win16(D.nbF16, D.ncap, i, nf_of(c), nt, [&](int k) { return s_wxk[k]; }, fluid);
+    } else {
+        each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, fluid);
+    }
     float dt = D.dt[s.scene];
     float cf = -m * dt * dt;
     float4 dii = D.f4a[i];
@@ -909,7 +1100,18 @@ __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
 // relative density error of the particles that are not clamped.
 __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
     __shared__ double lds[4];
+    WIN_XV_DECL
     Slot s = slot_nb(D, true);
+    if (!D.active[s.scene]) {   // uniform: a converged scene's tiles contribute nothing
+        write_partial(D, s.tile, 0, 0.0, lds);
+        return;
+    }
+    // window records: x and u = d_ii p_l + sum_j d_ij p_j of each neighbour
+    const int nw = win_layout(D, s);
+    if (nw) win_stage(nw, [&](int j) { return Rec2{D.x[j], D.f4b[j]}; }, [&](int k, Rec2 r) {
+        win.xy[k] = make_float2(r.a.x, r.a.y);
+        win.zv[k] = make_float4(r.a.z, r.b.x, r.b.y, r.b.z);
+    });
     double contrib = 0.0;
     if (s.live) {
         const int i = s.i;
@@ -923,17 +1125,25 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
         float sax = ua.x - dia.x * pla, say = ua.y - dia.y * pla, saz = ua.z - dia.z * pla;
         float ff = 0.f, fb = 0.f;
         int c = D.cnt[i];
-        each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.f4b[j]}; }, [&](int j, Rec2 rec) {
-            const float4 pb = rec.a, ub = rec.b;
-            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-            float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
+        auto fluid = [&](float bx, float by, float bz, float ux, float uy, float uz) {
+            float dx = pa.x - bx, dy = pa.y - by, dz = pa.z - bz;
+            float g = kernel_gradfac(fast_sqrt(dot3f(dx, dy, dz, dx, dy, dz)), D);
             float gx = dx * g, gy = dy * g, gz = dz * g;
             // (sum d_ij p_j|a - d_jj p_j) - (sum d_jk p_k|b - d_ji p_a)
-            float tx = sax - ub.x + cji * gx * pla;
-            float ty = say - ub.y + cji * gy * pla;
-            float tz = saz - ub.z + cji * gz * pla;
-            ff += tx * gx + ty * gy + tz * gz;
-        });
+            float tx = sax - ux + cji * gx * pla;
+            float ty = say - uy + cji * gy * pla;
+            float tz = saz - uz + cji * gz * pla;
+            ff += dot3f(tx, ty, tz, gx, gy, gz);
+        };
+        if (nw) {
+            const bool nt = __builtin_amdgcn_readfirstlane(D.listNT) != 0;
+            each_in_win16(D.nbF16, D.ncap, i, nf_of(c), nt, [&](int k) { return RecW{win.xy[k], win.zv[k]}; },
+                          [&](int, RecW r) { fluid(r.a.x, r.a.y, r.b.x, r.b.y, r.b.z, r.b.w); });
+        } else {
+            each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.f4b[j]}; }, [&](int j, Rec2 rec) {
+                fluid(rec.a.x, rec.a.y, rec.a.z, rec.b.x, rec.b.y, rec.b.z);
+            });
+        }
         each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
             float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
@@ -958,8 +1168,17 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
 
 // IISph_M::ii_computeForcePressure (IISph_M.cpp:478-684) fused with the
 // update v = v_adv + dt a_p, x += dt v and domain removal.
-__global__ __launch_bounds__(256) void k_ii_force(Dev D) {
+// gate 0: every scene; gate 1 (end of a captured step): the scenes whose
+// solver loop has ended, marked as integrated; gate 2 (after the loop was
+// continued outside the capture): the scenes not yet integrated.
+__global__ __launch_bounds__(256) void k_ii_force(Dev D, int gate) {
     Slot s = slot_nb(D, false);
+    if (gate == 1) {                          // uniform per block
+        if (D.active[s.scene]) return;
+        if (threadIdx.x == 0) D.integ[s.scene] = 1;
+    } else if (gate == 2 && D.integ[s.scene]) {
+        return;
+    }
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.x[i];
@@ -1200,6 +1419,7 @@ __global__ __launch_bounds__(64) void k_loop_start(Dev D) {
     if (sc >= D.nscenes) return;
     D.active[sc] = 1;
     D.iters[sc] = 0;
+    D.integ[sc] = 0;
 }
 
 __global__ void k_fill_i32(int* p, int v, int n) {
@@ -1279,7 +1499,7 @@ void launch_ii_aadv_dii(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_aadv_dii
 void launch_ii_aii(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_aii); }
 void launch_ii_sumdij(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_sumdij); }
 void launch_ii_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_pressure); }
-void launch_ii_force_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_force); }
+void launch_ii_force_integrate(const Dev& D, int gate, hipStream_t st) { SPH_LAUNCH(k_ii_force, gate); }
 void launch_pci_rest_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_pci_rest_density); }
 void launch_pci_aext(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_pci_aext); }
 void launch_pci_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_pci_density); }

@@ -13,6 +13,7 @@ constexpr int NBB_CAP = 48;          // boundary neighbours kept per particle
 constexpr int REF_NEIGHBOUR_CAP = 48;  // NeigbStr::maxNeigbNum (SphBase.h:630)
 constexpr int MAX_PHASES = 8;
 constexpr uint32_t KEY_INVALID = 0xFFFFFFFFu;
+constexpr int SOLVER_IISPH_DF = 4;   // SPHGPU_IISPH_DF: the IISph class's DF_* solvers
 
 // Uniform grid of one scene (SphBase::SpaceGridStruct, SphBase.h:682-705).
 // Cells are numbered row-major along x inside "rows" (cy, cz); rows are
@@ -75,10 +76,13 @@ struct Dev {
     int tileMap;       // 1: contiguous tile runs per XCD, 0: round robin (autotuned)
     int nbStage;       // neighbour-list groups staged in LDS: 6 or 8 (autotuned)
     int listNT;        // 1: passes read list groups with streaming loads (autotuned)
+    int winOn;         // 1: solver-loop passes stage packed tiles' neighbour windows in LDS
     float gx, gy, gz;
     float cs, gamma, nu, omega;
     float rho0[MAX_PHASES], mass[MAX_PHASES], visc[MAX_PHASES], tait_B[MAX_PHASES];
     float pci_delta;   // PCISPH stiffness before the division by dt^2
+    float boundaryDensity;   // m_TH.boundaryDensity (IISph DF_* solvers)
+    int adjustDensity;       // m_TH.adjustDensity (IISph DF_* solvers)
     float solid_visc;  // viscosity_alpha of the boundary (one value for all static solids)
 
     // ---- fluid state (sorted, padded slots)
@@ -108,6 +112,10 @@ struct Dev {
     float* kr0;        // warm-start kappa/rho; IISPH rho_adv
     float* f1;         // WCSPH pressure; IISPH p_l (next)
     float* f2;         // per-particle scratch (divergence deviation, errors)
+    float* pm;         // IISph DF_*: mass of each slot, rho0 beta v0
+    float* brho;       // IISph DF_*: boundary particles' density, alpha, kappa/rho
+    float* balpha;
+    float* bk;
 
     // ---- neighbour lists.  Packed form: 16-bit entries, eight per int4,
     // group g of particle i at g*ncap + i, decoded through the tile's base
@@ -117,6 +125,8 @@ struct Dev {
     int4* nbF16;
     int4* nbB16;
     int* nbTab;
+    int* nbLen;        // per tile, 32 ints: length of each fluid slot's run (packed tiles;
+                       // the window a pass stages in LDS is their concatenation)
     int* nbF;
     int* nbB;
     int* cnt;          // stored fluid entries | fluid neighbours found << 8 | boundary << 16
@@ -160,6 +170,7 @@ struct Dev {
     float* dt;           // per scene
     int* active;         // per scene: 1 while its solver loop runs
     int* iters;          // per scene: iterations of the current loop
+    int* integ;          // per scene: 1 once a captured IISPH step integrated it
     double* sceneErr;    // per scene
     double* partial;     // per tile partial sums (2 per tile)
     int* counters;       // Counter
@@ -176,6 +187,36 @@ __device__ __forceinline__ int cell_coord(float p, float mn, float div) {
 #pragma clang fp contract(off)
     float t = (p - mn) * div;
     return int(t);
+}
+
+// cell coordinates of a point inside scene grid G, clamped into the grid
+__device__ __forceinline__ void cell_of(const SceneGrid& G, float4 p, int& cx, int& cy, int& cz) {
+    cx = cell_coord(p.x, G.minx, G.divisor);
+    cy = cell_coord(p.y, G.miny, G.divisor);
+    cz = cell_coord(p.z, G.minz, G.divisor);
+    cx = min(max(cx, 0), G.nx - 1);
+    cy = min(max(cy, 0), G.ny - 1);
+    cz = min(max(cz, 0), G.nz - 1);
+}
+
+__device__ __forceinline__ uint32_t key_of(const SceneGrid& G, const int* rowRank, int cx, int cy, int cz) {
+    int row = rowRank[G.rowBase + cz * G.ny + cy];
+    return G.keyBase + (uint32_t)row * (uint32_t)G.nx + (uint32_t)cx;
+}
+
+// union of the [start,end) ranges of three consecutive keys (empty = start >= end)
+__device__ __forceinline__ int2 run3(const int2* __restrict__ table, uint32_t k0, int cx, int nx) {
+    int s = 0x7fffffff, e = 0;
+    for (int dx = -1; dx <= 1; ++dx) {
+        int c = cx + dx;
+        if (c < 0 || c >= nx) continue;
+        int2 r = table[k0 + dx];
+        if (r.y > r.x) {
+            s = min(s, r.x);
+            e = max(e, r.y);
+        }
+    }
+    return make_int2(s, e);
 }
 
 // squared distance exactly as Vec23.h:170-171 ((dx*dx + dy*dy) + dz*dz, no FMA)
@@ -240,6 +281,15 @@ __device__ __forceinline__ float kernel_gradfac(float r, const Dev& D) {
     float g2 = -0.75f * D.sigma_h * qm2 * qm2 * fast_rcp(r);
     float g = q <= 1.0f ? g1 : g2;
     return q <= 2.0f ? g : 0.0f;
+}
+
+// a.x b.x + a.y b.y + a.z b.z in one fixed rounding order,
+// fma(az, bz, fma(ay, by, ax bx)).  Left to the compiler, the contraction of
+// a three-term sum may fuse a different product first in different loop
+// bodies; pair terms evaluated by two code paths (LDS window or gathers)
+// must round identically.
+__device__ __forceinline__ float dot3f(float ax, float ay, float az, float bx, float by, float bz) {
+    return __builtin_fmaf(az, bz, __builtin_fmaf(ay, by, __fmul_rn(ax, bx)));
 }
 
 // Kahan-compensated FP32 accumulator.
@@ -397,7 +447,7 @@ void launch_ii_aadv_dii(const Dev& D, hipStream_t st);
 void launch_ii_aii(const Dev& D, hipStream_t st);
 void launch_ii_sumdij(const Dev& D, hipStream_t st);
 void launch_ii_pressure(const Dev& D, hipStream_t st);
-void launch_ii_force_integrate(const Dev& D, hipStream_t st);
+void launch_ii_force_integrate(const Dev& D, int gate, hipStream_t st);
 void launch_pci_rest_density(const Dev& D, hipStream_t st);
 void launch_pci_aext(const Dev& D, hipStream_t st);
 void launch_pci_density(const Dev& D, hipStream_t st);
@@ -409,5 +459,11 @@ void launch_loop_start(const Dev& D, hipStream_t st);
 void launch_stats(const Dev& D, double* part, double* out, hipStream_t st);
 void launch_fill_i32(int* p, int v, int n, hipStream_t st);
 void launch_copy_probe(const float4* a, float4* b, size_t n, hipStream_t st);
+
+// iisph_df.hip: the IISph class's DF_* solvers
+void launch_idf_prepare(const Dev& D, hipStream_t st);
+void launch_idf_vadv(const Dev& D, int mode, hipStream_t st);
+void launch_idf_kappa(const Dev& D, int mode, hipStream_t st);
+void launch_idf_velocity(const Dev& D, hipStream_t st);
 
 }  // namespace sphgpu

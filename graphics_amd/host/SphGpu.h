@@ -266,7 +266,8 @@ private:
         check(sphgpu_get_fluid(h_, SPHGPU_F_POS, pos.data()));
         check(sphgpu_get_fluid(h_, SPHGPU_F_VEL, vel.data()));
         check(sphgpu_get_fluid(h_, SPHGPU_F_RHO, rho.data()));
-        if (m_TH.solver != SPHGPU_DFSPH) check(sphgpu_get_fluid(h_, SPHGPU_F_PRESSURE, p.data()));
+        if (m_TH.solver != SPHGPU_DFSPH && m_TH.solver != SPHGPU_IISPH_DF)
+            check(sphgpu_get_fluid(h_, SPHGPU_F_PRESSURE, p.data()));
         mirror_.assign(phases_.size(), {});
         for (size_t i = 0; i < n; ++i) {
             FluidPart f;
@@ -367,6 +368,28 @@ public:
 class IISph_M : public SphBase {
 public:
     IISph_M() : SphBase(SPHGPU_IISPH) {}
+};
+
+// IISph (IISph.h:10-100) with its DF_* solvers (IISph.cpp:100-288, 816-1217):
+// per-particle mass rho0 beta d^3 (setFluidBeta), m_TH.adjustDensity and
+// m_TH.boundaryDensity (adjust_density, boundary_density), and the
+// reference's tolerances as defaults
+class IISph : public SphBase {
+public:
+    IISph() : SphBase(SPHGPU_IISPH_DF) {
+        m_TH.density_tol = 1e-4f;          // IISph.cpp:826
+        m_TH.min_iter_density = 3;         // IISph.cpp:829
+        m_TH.max_iter_density = 100;       // IISph.cpp:828
+        m_TH.divergence_tol = 1e-3f;       // IISph.cpp:1047 (on mean(Drho/Dt) dt)
+        m_TH.min_iter_div = 1;             // IISph.cpp:1050
+        m_TH.max_iter_div = 30;            // IISph.cpp:1049
+    }
+    // FluidParticle::beta of the particles [first, first + n) in add order,
+    // after they were added and before the first step
+    void setFluidBeta(long long first, const std::vector<float>& beta) {
+        if (!handle()) throw SphGpuError(SPHGPU_E_STATE, "setFluidBeta: no fluid added yet");
+        check(sphgpu_set_fluid_beta(handle(), first, (long long)beta.size(), beta.data()));
+    }
 };
 
 }  // namespace sphgpu_host

@@ -27,6 +27,8 @@ class Scene:
     domains: list = field(default_factory=list)        # (min[3], max[3]) per scene
     rigids: list = field(default_factory=list)         # (scene, body-frame xyz[n,3], visc_alpha, pose)
     rigid_mass: list = field(default_factory=list)     # (mass, inertia[3]) per rigid, for a host step
+    beta: object = None                                # per-particle rest-density factor by add order (IISph DF_*)
+    velocity: object = None                            # initial velocity (N, 3) by add order (default 0)
 
     @property
     def num_fluid(self) -> int:
@@ -51,13 +53,18 @@ def make_simulation(sc: Scene, **overrides) -> sg.Simulation:
         sim.set_scene_domain(s, mn, mx)
     for rho0, visc in sc.phases:
         sim.add_phase(rho0, visc)
+    first = 0
     for s, ph, xyz in sc.fluids:
-        sim.add_fluid(xyz, phase=ph, scene=s)
+        vel = None if sc.velocity is None else np.asarray(sc.velocity)[first:first + len(xyz)]
+        sim.add_fluid(xyz, vel, phase=ph, scene=s)
+        first += len(xyz)
     for s, xyz, visc in sc.solids:
         sim.add_solid(xyz, visc_alpha=visc, scene=s)
     for s, xyz, visc, pose in sc.rigids:
         k = sim.add_rigid(xyz, visc_alpha=visc, scene=s)
         sim.set_solid_pose(k, *pose)
+    if sc.beta is not None:
+        sim.set_fluid_beta(0, sc.beta)
     return sim
 
 

@@ -14,7 +14,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libsphgpu.so"
 
-WCSPH, DFSPH, IISPH, PCISPH = 0, 1, 2, 3
+WCSPH, DFSPH, IISPH, PCISPH, IISPH_DF = 0, 1, 2, 3, 4
 WARM_NONE, WARM_PREVIOUS, WARM_LEARNED = 0, 1, 2
 F_POS, F_VEL, F_RHO, F_PRESSURE, F_ACC, F_ALPHA, F_KAPPA_SUM, F_PHASE, F_KAPPAV_SUM = range(9)
 FIELD_COMPONENTS = {F_POS: 3, F_VEL: 3, F_RHO: 1, F_PRESSURE: 1, F_ACC: 3, F_ALPHA: 1,
@@ -59,7 +59,9 @@ class Params(C.Structure):
         ("learn_over_weight", C.c_float),
         ("warm_scale", C.c_float),
         ("use_graph", C.c_int),
-        ("reserved", C.c_int * 1),
+        ("boundary_density", C.c_float),
+        ("adjust_density", C.c_int),
+        ("fluid_capacity", C.c_int),
     ]
 
 
@@ -104,6 +106,8 @@ SIGNATURES = {
     "sphgpu_add_phase": (C.c_int, [C.c_void_p, C.c_float, C.c_float, _IP]),
     "sphgpu_add_fluid": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_longlong, _FP, _FP, _LP]),
     "sphgpu_add_solid": (C.c_int, [C.c_void_p, C.c_int, C.c_longlong, _FP, C.c_float, _IP]),
+    "sphgpu_set_fluid_beta": (C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong, _FP]),
+    "sphgpu_append_fluid": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_longlong, _FP, _FP, _LP]),
     "sphgpu_add_rigid": (C.c_int, [C.c_void_p, C.c_int, C.c_longlong, _FP, C.c_float, _IP]),
     "sphgpu_set_solid_pose": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP, _FP, _FP]),
     "sphgpu_get_solid_force_torque": (C.c_int, [C.c_void_p, C.c_int, _DP, _DP]),
@@ -144,6 +148,7 @@ SIGNATURES = {
     "sphgpu_get_tuning": (C.c_int, [C.c_void_p, _IP, _FP]),
     "sphgpu_get_list_form": (C.c_int, [C.c_void_p, _IP, _IP, _IP]),
     "sphgpu_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "sphgpu_set_lds_windows": (C.c_int, [C.c_void_p, C.c_int]),
     "sphgpu_mlp_create": (C.c_int, [C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
     "sphgpu_mlp_destroy": (None, [C.c_void_p]),
     "sphgpu_mlp_set_params": (C.c_int, [C.c_void_p, _FP]),
@@ -459,6 +464,21 @@ class Simulation:
                     "add_fluid")
         return first.value
 
+    def append_fluid(self, xyz, vel=None, phase=0, scene=0) -> int:
+        """Fluid particles added between steps (an emitter); returns the first new id."""
+        x = _f32(xyz).reshape(-1, 3)
+        v = None if vel is None else _f32(vel).reshape(-1, 3)
+        first = C.c_longlong()
+        self._check(self.lib.sphgpu_append_fluid(self.h, scene, phase, len(x), _fp(x),
+                                                 None if v is None else _fp(v), C.byref(first)), "append_fluid")
+        return first.value
+
+    def set_fluid_beta(self, first, beta):
+        """Rest-density factor beta of the fluid particles [first, first + len(beta))
+        in add order (IISph DF_* solvers); before init."""
+        b = _f32(beta).ravel()
+        self._check(self.lib.sphgpu_set_fluid_beta(self.h, int(first), len(b), _fp(b)), "set_fluid_beta")
+
     def add_solid(self, xyz, visc_alpha=0.0, scene=0) -> int:
         x = _f32(xyz).reshape(-1, 3)
         sid = C.c_int()
@@ -609,6 +629,11 @@ class Simulation:
         f = lambda v: -1 if v is None else int(v)
         self._check(self.lib.sphgpu_set_tuning(self.h, f(tile_map), f(stage_groups), f(stream_lists)),
                     "set_tuning")
+
+    def set_lds_windows(self, on: bool):
+        """Solver-loop passes read packed tiles' neighbours from LDS windows
+        (default) or gather them per lane; the results are bit-identical."""
+        self._check(self.lib.sphgpu_set_lds_windows(self.h, int(bool(on))), "set_lds_windows")
 
     def list_form(self) -> dict:
         """Tiles holding particles, how many of them have packed (16-bit) lists,

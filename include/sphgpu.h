@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SPHGPU_ABI_VERSION 1
+#define SPHGPU_ABI_VERSION 2
 #define SPHGPU_MAX_ITER 10000   /* largest accepted solver-loop cap */
 
 enum sphgpu_status {
@@ -47,8 +47,14 @@ enum sphgpu_solver {
     SPHGPU_WCSPH = 0,       /* WcSph::sphStep            WcSph.cpp:23-96   */
     SPHGPU_DFSPH = 1,       /* DFSPH::sphStep            DFSPH.cpp:9-75    */
     SPHGPU_IISPH = 2,       /* IISph_M::sphStep (Ihmsen)  IISph_M.cpp:10-75 */
-    SPHGPU_PCISPH = 3       /* PciSph::sphStep (SP09)     PciSph.cpp:31-95; eta = density_tol,
+    SPHGPU_PCISPH = 3,      /* PciSph::sphStep (SP09)     PciSph.cpp:31-95; eta = density_tol,
                                min/max iterations = min/max_iter_density (reference 0.01, 3, 10) */
+    SPHGPU_IISPH_DF = 4     /* IISph class, DF_* solvers  IISph.cpp:100-288, 816-1217: per-particle mass
+                               rho0 beta d^3 (sphgpu_set_fluid_beta), adjust_density, boundary_density,
+                               boundary particles' own kappa; divergence-free solve, then the constant-
+                               density solve (reference: density_tol 1e-4 relative, 3..100 iterations;
+                               divergence_tol 1e-3 on mean(Drho/Dt) dt, 1..30); loops break when the
+                               iteration count exceeds max_iter_*.  No warm start, no graph capture. */
 };
 
 enum sphgpu_warm_start {
@@ -89,7 +95,11 @@ typedef struct sphgpu_params {
     float warm_scale;          /* the guess is warm_scale x the previous stiffness (PREVIOUS) or x the
                                   bounded prediction (LEARNED); default 0.7 (Bender & Koschier use 0.5) */
     int   use_graph;           /* 1: WCSPH/IISPH steps are captured once into HIP graphs and replayed */
-    int   reserved[1];
+    float boundary_density;    /* m_TH.boundaryDensity (SphBase.h:576, 1000): IISph DF_* boundary terms and
+                                  the boundary particles' rest density */
+    int   adjust_density;      /* m_TH.adjustDensity (SphBase.h:575, 0): IISph DF_* number density */
+    int   fluid_capacity;      /* fluid particles the handle can hold, counting later appends (emitters);
+                                  0 = the particles added before init */
 } sphgpu_params;
 
 typedef struct sphgpu_stats {
@@ -150,6 +160,20 @@ int sphgpu_add_fluid(sphgpu_handle* h, int scene, int phase, long long n,
 /* Appends a static solid of n boundary particles to scene `scene`
  * (SphWithBullet::addSolid -> SphSolid, SphBase.h:315-365, rigid type).
  * visc_alpha is SphSolid::viscosity_alpha. */
+/* Appends fluid particles after sphgpu_init, between steps: the emitters a
+ * stepEvent runs (SphWithIO::addFluidParts, bangzi.h:193-196 inside
+ * generateWater1, 129-208).  Same arguments and ids as sphgpu_add_fluid
+ * (which forwards here once the handle is initialized); the next step's
+ * neighbour search takes the new particles in (a DFSPH step then searches
+ * before its density solve, since that solve reads the previous search's
+ * lists).  SPHGPU_E_CAPACITY beyond sphgpu_params.fluid_capacity. */
+int sphgpu_append_fluid(sphgpu_handle* h, int scene, int phase, long long n, const float* xyz,
+                        const float* vxyz, long long* first_id);
+/* Per-particle rest-density factor beta (FluidParticle::beta, SphBase.h:98)
+ * of the fluid particles [first, first + n) in add order, for the IISph DF_*
+ * solvers (m = rho0 beta d^3, IISph.cpp:110-115); default 1.  Before
+ * sphgpu_init. */
+int sphgpu_set_fluid_beta(sphgpu_handle* h, long long first, long long n, const float* beta);
 int sphgpu_add_solid(sphgpu_handle* h, int scene, long long n, const float* xyz,
                      float visc_alpha, int* solid);
 /* Appends a dynamic rigid body of n boundary particles given in body
@@ -325,6 +349,12 @@ int sphgpu_get_tuning(const sphgpu_handle* h, int* choice, float* ms);
  * Every choice gives bit-identical results.  Before the first step only
  * (SPHGPU_E_STATE after). */
 int sphgpu_set_tuning(sphgpu_handle* h, int tile_map, int stage_groups, int stream_lists);
+/* LDS windows of the solver-loop passes (no counterpart in the reference):
+ * with on = 1 (the default) the DFSPH and IISPH iteration passes stage each
+ * packed tile's fluid neighbour records in LDS and read neighbours from
+ * there; on = 0 makes every lane gather them from memory.  Both give
+ * bit-identical results (a switch for measurements and tests). */
+int sphgpu_set_lds_windows(sphgpu_handle* h, int on);
 /* Form of the neighbour lists of the last step: *tiles = tiles holding
  * particles, *packed_tiles = those whose lists use 16-bit entries (the rest
  * use 32-bit entries; both hold the same neighbours in the same order);

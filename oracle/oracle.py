@@ -110,6 +110,7 @@ def load():
         lib.orc_add_fluid.argtypes = [C.c_void_p, C.c_int, FP, FP, IP, C.c_float, C.c_float]
         lib.orc_add_solid.argtypes = [C.c_void_p, C.c_int, FP, C.c_int, C.c_float]
         lib.orc_set_fluid_beta.argtypes = [C.c_void_p, C.c_int, C.c_int, FP]
+        lib.orc_append_fluid.argtypes = [C.c_void_p, C.c_int, C.c_int, FP, FP, IP]
         lib.orc_init.argtypes = [C.c_void_p]
         lib.orc_set_solid_pose.argtypes = [C.c_void_p, C.c_int, FP, FP, FP, FP]
         lib.orc_solid_force_torque.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double),
@@ -205,6 +206,16 @@ class Oracle:
                                    None if i is None else i.ctypes.data_as(IP), rho0, visc)
         self.nfluid_phases = max(self.nfluid_phases, k + 1)
         return k
+
+    def append_fluid(self, k, xyz, ids, vel=None):
+        """Particles added to fluid k between steps (an emitter)."""
+        x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        v = None if vel is None else np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+        i = np.ascontiguousarray(ids, np.int32)
+        FP, IP = C.POINTER(C.c_float), C.POINTER(C.c_int)
+        if self.lib.orc_append_fluid(self.h, k, len(x), x.ctypes.data_as(FP), None if v is None else v.ctypes.data_as(FP),
+                                     i.ctypes.data_as(IP)) != 0:
+            raise ValueError("append_fluid: bad fluid index or a particle outside the domain")
 
     def set_beta(self, k, beta):
         """Per-particle rest-density factor of fluid k, in its add order."""
@@ -400,7 +411,8 @@ def from_scene(sc, bits=64, **overrides) -> Oracle:
         rho0, visc = sc.phases[ph]
         xyz = np.concatenate([a for a, _ in per_phase[ph]])
         ids = np.concatenate([b for _, b in per_phase[ph]])
-        k = o.add_fluid(xyz, rho0, visc, ids=ids)
+        vel = None if getattr(sc, "velocity", None) is None else np.asarray(sc.velocity, np.float32)[ids]
+        k = o.add_fluid(xyz, rho0, visc, vel=vel, ids=ids)
         if beta is not None:
             o.set_beta(k, np.asarray(beta, np.float32)[ids])
     for s, xyz, visc in sc.solids:

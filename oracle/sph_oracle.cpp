@@ -149,6 +149,7 @@ public:
     std::vector<Solid<T>> solids;
     std::vector<std::vector<NeigbStr<T>>> nbF, nbS;
     int frame = 0;
+    bool needSearch = false;   /* fluid appended since the last neighbour search */
     T dt_cur, dt_prev = 0;
     int zeroDis = 0, tooMany = 0;
 
@@ -256,6 +257,7 @@ public:
     /* SphBase::neighbourSearch, SphBase.cpp:849-960 */
     void neighbourSearch(bool fluidPart = true, bool solidPart = true, bool RR_same = false) {
         gridSetup();
+        if (fluidPart) needSearch = false;
         int n_off = int(G.offset.size());
         int overflow = 0, zero = 0;
         if (fluidPart)
@@ -1476,7 +1478,7 @@ public:
         return d;
     }
 
-    /* the velocity pass of both loops (IISph.cpp:956-1000, 1158-1200):
+    /* the velocity pass of both loops (IISph.cpp:966-1001, 1166-1206):
      * v_adv -= dt sum gradW m_b (k_a/rho_a + k_b/rho_b), boundary neighbours
      * with their own kappa and mass boundary_density V_b */
     void iidf_velocity(int which, T dt) {
@@ -1545,10 +1547,10 @@ public:
                         } else {
                             const BoundPart<T>& b = bp(nb.pidx);
                             V3<T> g = (a.x - b.x) * (gs / nb.dis);
-                            dd += g.dot(a.v_adv - b.v) * (resta * b.volume);    /* IISph.cpp:1146 */
+                            dd += g.dot(a.v_adv - b.v) * (resta * b.volume);    /* IISph.cpp:1156 */
                         }
                     }
-                    if (dd < 0 || (a.rho + dd * dt) < resta) dd = 0;          /* IISph.cpp:1150-1152 */
+                    if (dd < 0 || (a.rho + dd * dt) < resta) dd = 0;          /* IISph.cpp:1160-1162 */
                     a.divdev = dd;
                     a.kappaV = dd * a.alpha / dt;
                 }
@@ -1607,7 +1609,7 @@ public:
                         } else {
                             const BoundPart<T>& b = bp(nb.pidx);
                             V3<T> g = (a.x - b.x) * (gs / nb.dis);
-                            radv += g.dot(a.v_adv - b.v) * bDensity() * b.volume * dt;   /* IISph.cpp:943 */
+                            radv += g.dot(a.v_adv - b.v) * bDensity() * b.volume * dt;   /* IISph.cpp:948 */
                         }
                     }
                     if (radv < resta) radv = resta;
@@ -1679,7 +1681,10 @@ public:
             rc = updateSolids(cb, user);                                   /* WcSph.cpp:85 */
             fillStats(st);
         } else if (P.solver == ORC_DFSPH) {                                /* DFSPH.cpp:9-75 */
-            if (frame == 0) { neighbourSearch(); updateSolidPartWeight(); df_first(); }
+            /* after an append the density solve needs lists and densities
+             * that cover the new particles (in the reference the step after
+             * a stepEvent emitter reads lists sized before it) */
+            if (frame == 0 || needSearch) { neighbourSearch(); updateSolidPartWeight(); df_first(); }
             df_second(st);
             rc = updateSolids(cb, user);                                   /* DFSPH.cpp:54 */
             neighbourSearch();
@@ -1804,6 +1809,26 @@ int orc_add_fluid(orc_sim* s, int n, const float* xyz, const float* vel, const i
         }
         S.fluids.push_back(std::move(f));
         return int(S.fluids.size()) - 1;
+    });
+}
+
+int orc_append_fluid(orc_sim* s, int k, int n, const float* xyz, const float* vel, const int* ids) {
+    return dispatch(s, [&](auto& S) {
+        using T = std::decay_t<decltype(S.r)>;
+        if (k < 0 || k >= int(S.fluids.size()) || n < 0 || !xyz || !ids) return -1;
+        auto& ps = S.fluids[k].parts;
+        for (int i = 0; i < n; ++i) {
+            FluidPart<T> q;
+            q.x = V3<T>(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+            if (!q.x.inside(S.spMin, S.spMax)) return -1;
+            if (vel) q.v = V3<T>(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+            q.d = T(S.P.spacing_r);
+            q.id = ids[i];
+            q.rho = S.fluids[k].rho0;
+            ps.push_back(q);
+        }
+        S.needSearch = true;
+        return 0;
     });
 }
 

@@ -89,6 +89,9 @@ orc_sim* orc_create(int precision_bits, const orc_params* p);
 void     orc_destroy(orc_sim* s);
 int      orc_add_fluid(orc_sim* s, int n, const float* xyz, const float* vel,
                        const int* ids, float rest_density, float visc_alpha);
+/* appends n particles (given ids) to fluid k between steps (an emitter in
+ * stepEvent, SphWithIO::addFluidParts) */
+int      orc_append_fluid(orc_sim* s, int k, int n, const float* xyz, const float* vel, const int* ids);
 /* per-particle rest-density factor beta of fluid k (IISph DF_* solvers,
  * m = rho0 beta d^3, IISph.cpp:110-115); default 1 */
 int      orc_set_fluid_beta(orc_sim* s, int k, int n, const float* beta);

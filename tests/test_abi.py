@@ -29,13 +29,13 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layouts_match_header_sizes():
-    # 4 (solver) + 2*4 + 6*4 + 4 + 3*4 + 6*4 + 2*4 + 4*4 + 4 + 4 + 4 + 8*4
-    assert C.sizeof(sg.Params) == 140
+    # 4 (solver) + 2*4 + 6*4 + 4 + 3*4 + 6*4 + 2*4 + 4*4 + 4 + 4 + 4 + 8*4 + 3*4
+    assert C.sizeof(sg.Params) == 148
     assert C.sizeof(sg.Stats) % 8 == 0
 
 
 def test_abi_version():
-    assert sg.load().sphgpu_abi_version() == 1
+    assert sg.load().sphgpu_abi_version() == 2
 
 
 def test_default_params_mirror_reference_defaults():
@@ -58,6 +58,9 @@ def test_default_params_mirror_reference_defaults():
                                  dict(max_iter_density=0), dict(max_iter_density=-3),
                                  dict(max_iter_div=-1), dict(max_iter_density=10 ** 6),
                                  dict(min_iter_density=-1),
+                                 # the IISph DF_* solvers: no warm start, no graph capture
+                                 dict(solver=4, warm_start=1), dict(solver=4, use_graph=1), dict(solver=5),
+                                 dict(boundary_density=0.0), dict(adjust_density=2), dict(fluid_capacity=-1),
                                  dict(solver=sg.IISPH, warm_start=sg.WARM_LEARNED),
                                  dict(solver=sg.DFSPH, warm_start=sg.WARM_LEARNED, learn_batch=4)])
 def test_create_rejects_bad_parameters(bad):

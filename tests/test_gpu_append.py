@@ -1,0 +1,102 @@
+"""Fluid appended between steps (the emitters a stepEvent runs,
+SphWithIO::addFluidParts, bangzi.h:193-196) against the oracle appending the
+same particles: fields after the next step, statistics of an emitter run,
+and the capacity limit."""
+import numpy as np
+import pytest
+
+from graphics_amd import scenes, sphgpu as sg
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-5
+
+
+def _relerr(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def _jet(k, n=3, spacing=0.02):
+    """An n^3 block of particles above the fluid, falling at 0.5 m/s (the k-th shot)."""
+    x = sg.lattice((n, n, n), (0.1 + 0.01 * k, 0.26, 0.1), spacing, 0.01, 77, k).astype(np.float32)
+    v = np.tile(np.array([0.2, -0.5, 0.0], np.float32), (len(x), 1))
+    return x, v
+
+
+@pytest.mark.parametrize("solver", [sg.WCSPH, sg.DFSPH, sg.IISPH])
+def test_append_then_step_matches_oracle(gpu, solver):
+    print(f"[append] solver {solver}", flush=True)
+    kw = dict(min_iter_density=3, max_iter_density=3, min_iter_div=2, max_iter_div=1)
+    sc = scenes.small_dam(solver, n=(10, 10, 10), box=(0.4, 0.35, 0.3), fluid_capacity=2000, **kw)
+    sim = scenes.make_simulation(sc)
+    o = orc.from_scene(sc, 64)
+    o32 = orc.from_scene(sc, 32)
+    sim.step()
+    o.step()
+    o32.step()
+    x, v = _jet(0)
+    first = sim.append_fluid(x, v)
+    assert first == sc.num_fluid
+    ids = np.arange(first, first + len(x), dtype=np.int32)
+    o.append_fluid(0, x, ids, v)
+    o32.append_fluid(0, x, ids, v)
+    st = sim.step()
+    o.step()
+    o32.step()
+    assert st["num_fluid"] == sc.num_fluid + len(x)
+    assert np.array_equal(sim.ids(), o.get_by_id(orc.F_ID)[0])
+    for f, of in ((sg.F_POS, orc.F_POS), (sg.F_VEL, orc.F_VEL), (sg.F_RHO, orc.F_RHO)):
+        err = _relerr(sim.get(f), o.get_by_id(of)[1])
+        # two steps: the bar is 1e-5 or 3x the oracle's own FP32 error, whichever is larger
+        e32 = _relerr(o32.get_by_id(of)[1], o.get_by_id(of)[1])
+        assert err < max(FIELD_TOL, 3 * e32), (f, err, e32)
+    sim.close()
+
+
+def test_emitter_run_statistics(gpu):
+    """A WCSPH dam fed by an emitter every 25 steps for 200 steps: live
+    counts equal, density and kinetic-energy statistics match the oracle."""
+    sc = scenes.small_dam(sg.WCSPH, n=(10, 10, 10), box=(0.4, 0.35, 0.3), fluid_capacity=1500)
+    sim = scenes.make_simulation(sc)
+    o = orc.from_scene(sc, 32)
+    g, r = [], []
+    nid = sc.num_fluid
+    for k in range(200):
+        if k % 25 == 0:
+            x, v = _jet(k // 25)
+            assert sim.append_fluid(x, v) == nid
+            o.append_fluid(0, x, np.arange(nid, nid + len(x), dtype=np.int32), v)
+            nid += len(x)
+        g.append(sim.step())
+        r.append(o.step())
+    # particles that a splash carries out of the domain are removed in both;
+    # which ones is chaotic, so the live counts agree to a few particles
+    assert nid == sc.num_fluid + 8 * 27
+    assert sc.num_fluid + 8 * 27 - 40 <= g[-1]["num_fluid"] <= sc.num_fluid + 8 * 27
+    assert abs(g[-1]["num_fluid"] - o.num_fluid()) <= 10
+    for key in ("rho_mean_err", "rho_max_err"):
+        a = np.array([s[key] for s in g[-20:]])
+        b = np.array([s[key] for s in r[-20:]])
+        assert np.abs(a - b).max() < 3e-3, key
+    ke_g = np.array([s["kinetic_energy"] for s in g])
+    ke_r = np.array([s["kinetic_energy"] for s in r])
+    assert np.abs(ke_g - ke_r).max() / ke_r.max() < 0.03
+    assert all(s["nan_count"] == 0 and s["list_overflow"] == 0 for s in g)
+    sim.close()
+
+
+def test_append_beyond_capacity_is_refused(gpu):
+    sc = scenes.small_dam(sg.WCSPH, n=(6, 6, 6), box=(0.3, 0.3, 0.3), fluid_capacity=300)
+    sim = scenes.make_simulation(sc)
+    sim.step()
+    x, v = _jet(0, n=4)         # 64 particles: 216 + 64 fit in 300, the next 64 do not
+    sim.append_fluid(x, v)
+    with pytest.raises(sg.SphError) as e:
+        sim.append_fluid(_jet(1, n=4)[0])
+    assert e.value.code == -4
+    st = sim.step()
+    assert st["num_fluid"] == 216 + 64 and st["nan_count"] == 0
+    sim.close()

@@ -84,3 +84,20 @@ def test_tuning_choices_are_bit_identical(gpu, shape):
                      [s["iter_density"] for s in st], [s["iter_divergence"] for s in st]))
         sim.close()
     assert all(r == runs[0] for r in runs[1:])
+
+
+@pytest.mark.parametrize("solver", [sg.DFSPH, sg.IISPH])
+def test_lds_windows_are_bit_identical(gpu, solver):
+    """Solver-loop passes reading neighbours from LDS windows give the same
+    bits as per-lane gathers, on a tank whose tiles have packed lists."""
+    runs = []
+    for on in (1, 0):
+        sc = scenes.small_dam(solver, n=(96, 6, 8), box=(2.0, 0.3, 0.3))
+        sim = scenes.make_simulation(sc, warm_start=sg.WARM_PREVIOUS if solver == sg.DFSPH else 0)
+        sim.set_lds_windows(on)
+        st = [sim.step() for _ in range(6)]
+        assert sim.list_form()["packed_tiles"] > 0
+        runs.append((sim.get(sg.F_POS).tobytes(), sim.get(sg.F_VEL).tobytes(),
+                     [s["iter_density"] for s in st], [s["iter_divergence"] for s in st]))
+        sim.close()
+    assert runs[0] == runs[1]

@@ -11,11 +11,11 @@ differences of it (boundary particles held fixed), for fluid rows and for
 boundary rows (the boundary particles' own densities).  Then:
 
 - alpha_i = rho_i / sum_k |J_ik|^2 (each particle's stiffness cancels its own
-  predicted compression; IISph.cpp:190-200, 962-965, 1153);
+  predicted compression; IISph.cpp:197-200, 963, 1164);
 - the predicted density and the divergence are (J v)_i, clamped as the
-  reference clamps them (IISph.cpp:947-949, 1150-1152);
+  reference clamps them (IISph.cpp:952-954, 1160-1162);
 - the velocity pass moves particle k by -dt sum_l (kappa_l/rho_l)(m_l/m_k) J_lk
-  over fluid and boundary rows (IISph.cpp:966-1000, 1158-1200), which also
+  over fluid and boundary rows (IISph.cpp:966-1001, 1166-1206), which also
   conserves momentum where no wall is near.
 
 A factor error in kappa, a neighbour's mass taken for the particle's own (or
