@@ -662,6 +662,12 @@ int step_wcsph(sphgpu_handle* h, sphgpu_stats* st) {
     return rcb ? rcb : rc;
 }
 
+// DFSPH::sphStep (DFSPH.cpp:9-75).  In graph mode (no dynamic solid, no
+// update-solids callback, no online training in this step) the step is two
+// captured segments, one per solver loop, each ending with enough
+// iterations for the previous step's count (iterations after a scene
+// converged return at once); the host reads the loop flags after each, and
+// a loop that needs more iterations continues outside the capture.
 int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
     Dev& D = h->D;
     int rc;
@@ -671,44 +677,75 @@ int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
         run_ns(h);
         launch_df_density_alpha(D, h->st);
     }
-    launch_cfl(D, h->P.cfl, h->P.spacing_r, h->P.dt, h->P.dt_min, h->st);
-    int warm = h->P.warm_start == SPHGPU_WARM_PREVIOUS ? 1 : 0;
-    {
-        ProfScope ps(h, "df_aadv", nf_live(h) * 40.0);
-        launch_df_aadv(D, h->st);
-    }
-    if ((rc = solver_loop(h, 0))) return rc;
-    rigid_reaction(h, 2);
-    launch_df_integrate(D, h->st);
-    // DFSPH.cpp:54: updateSolids between the density solve and the next
-    // search; a failing callback still lets the step finish
-    const int rcb = update_solids_point(h);
-    if (D.anyDynamic && (rc = rigid_update(h))) return rc;
-    run_ns(h);
-    // divergence warm start: 0 none, 1 half the previous stiffness, 2 model
+    // divergence warm start: 0 none, 1 a scaled previous stiffness, 2 model
     const bool learnedMode = h->P.warm_start == SPHGPU_WARM_LEARNED;
     const bool training = learnedMode && !h->mlpPreset && h->learned < h->P.learn_steps;
     int mode = 0;
     if (h->frame > 0) {
-        if (warm || (learnedMode && training)) mode = 1;
+        if (h->P.warm_start == SPHGPU_WARM_PREVIOUS || (learnedMode && training)) mode = 1;
         else if (learnedMode) mode = 2;
     }
-    {
-        ProfScope ps(h, "df_density_alpha", nf_live(h) * 20.0);
-        if (mode != 0) launch_df_density_alpha_warm(D, learnedMode ? h->feat : nullptr, h->st);
-        else launch_df_density_alpha(D, h->st);
+    const bool graph = h->P.use_graph && !D.anyDynamic && !h->updateFn && !training;
+    auto batch = [h, graph](int which) {
+        return graph ? std::max({2, which ? h->P.min_iter_div : h->P.min_iter_density, (h->lastIters[which] + 2) & ~1})
+                     : first_batch(h, which);
+    };
+    // density solve (DFSPH.cpp:195-738)
+    size_t profStart = h->profUsed;
+    const int b0 = batch(0);
+    auto segA = [h, b0]() {
+        Dev& D = h->D;
+        launch_cfl(D, h->P.cfl, h->P.spacing_r, h->P.dt, h->P.dt_min, h->st);
+        {
+            ProfScope ps(h, "df_aadv", nf_live(h) * 40.0);
+            launch_df_aadv(D, h->st);
+        }
+        solver_loop_start(h, 0, b0);
+    };
+    if (graph) {
+        if ((rc = run_segment(h, 3, b0, segA))) return rc;
+    } else {
+        segA();
     }
-    HIPCHECK(h, hipMemcpyAsync(D.vs, D.v, sizeof(float4) * D.npad, hipMemcpyDeviceToDevice, h->st));
-    if (mode == 2) {
-        ProfScope ps(h, "mlp_forward", nf_live(h) * 36.0);
-        mlp_forward(h->mlp, h->feat, h->ypred, D.npad, nullptr, h->st);
+    if ((rc = solver_loop_finish(h, 0, b0, profStart))) return rc;
+    rigid_reaction(h, 2);
+    // DFSPH.cpp:54: updateSolids between the density solve and the next
+    // search; a failing callback still lets the step finish
+    int rcb = 0;
+    profStart = h->profUsed;
+    const int b1 = batch(1);
+    auto segB = [h, b1, mode, learnedMode, &rcb, &rc]() {
+        Dev& D = h->D;
+        launch_df_integrate(D, h->st);
+        rcb = update_solids_point(h);
+        if (D.anyDynamic && (rc = rigid_update(h))) return;
+        run_ns(h);
+        {
+            ProfScope ps(h, "df_density_alpha", nf_live(h) * 20.0);
+            if (mode != 0) launch_df_density_alpha_warm(D, learnedMode ? h->feat : nullptr, h->st);
+            else launch_df_density_alpha(D, h->st);
+        }
+        launch_copy_f4(D.vs, D.v, D.npad, h->st);
+        if (mode == 2) {
+            ProfScope ps(h, "mlp_forward", nf_live(h) * 36.0);
+            mlp_forward(h->mlp, h->feat, h->ypred, D.npad, nullptr, h->st);
+        }
+        launch_df_warm_apply(D, h->ypred, h->feat, mode, h->st);
+        if (mode != 0) {
+            launch_fill_i32(D.active, 1, D.nscenes, h->st);
+            launch_df_pass_velocity(D, h->st);
+        }
+        solver_loop_start(h, 1, b1);
+    };
+    rc = 0;
+    if (graph) {
+        int rg = run_segment(h, 4 + mode, b1, segB);
+        if (rg) return rg;
+    } else {
+        segB();
     }
-    launch_df_warm_apply(D, h->ypred, h->feat, mode, h->st);
-    if (mode != 0) {
-        launch_fill_i32(D.active, 1, D.nscenes, h->st);
-        launch_df_pass_velocity(D, h->st);
-    }
-    if ((rc = solver_loop(h, 1))) return rc;
+    if (rc) return rc;
+    if ((rc = solver_loop_finish(h, 1, b1, profStart))) return rc;
     if (training && h->frame > 0) {
         // online training on this step's converged stiffness
         launch_df_targets(D, h->feat, h->tgt, h->st);
@@ -729,7 +766,13 @@ int step_iisph(sphgpu_handle* h, sphgpu_stats* st) {
     Dev& D = h->D;
     int rc;
     const size_t profStart = h->profUsed;
-    const bool graph = h->P.use_graph && !h->D.anyDynamic;
+    // learned start of the pressure solve: the reference's 0.5 p(t-dt) while
+    // the network trains (the first learn_steps steps), the network's
+    // prediction after
+    const bool learnedMode = h->P.warm_start == SPHGPU_WARM_LEARNED;
+    const bool training = learnedMode && !h->mlpPreset && h->learned < h->P.learn_steps;
+    const bool predict = learnedMode && !training && h->frame > 0;
+    const bool graph = h->P.use_graph && !h->D.anyDynamic && !training;
     // graph mode: the whole step is one captured segment -- search, the
     // per-step passes, enough solver iterations for the previous step's
     // count plus one (iterations after a scene converged return at once),
@@ -737,7 +780,7 @@ int step_iisph(sphgpu_handle* h, sphgpu_stats* st) {
     // The host reads the flags once, after it; a scene that needed more
     // iterations continues outside the capture and is integrated then.
     const int batch = graph ? std::max({2, h->P.min_iter_density, (h->lastIters[0] + 2) & ~1}) : first_batch(h, 0);
-    auto body = [h, batch, graph]() {
+    auto body = [h, batch, graph, learnedMode, predict]() {
         Dev& D = h->D;
         run_ns(h);
         launch_cfl(D, h->P.cfl, h->P.spacing_r, h->P.dt, h->P.dt_min, h->st);
@@ -748,7 +791,12 @@ int step_iisph(sphgpu_handle* h, sphgpu_stats* st) {
         }
         {
             ProfScope ps(h, "ii_aii", nf_live(h) * 48.0);
-            launch_ii_aii(D, h->st);
+            launch_ii_aii(D, learnedMode ? h->feat : nullptr, h->st);
+        }
+        if (predict) {
+            ProfScope ps(h, "mlp_forward", nf_live(h) * 36.0);
+            mlp_forward(h->mlp, h->feat, h->ypred, D.npad, nullptr, h->st);
+            launch_ii_warm_apply(D, h->ypred, h->feat, h->st);
         }
         solver_loop_start(h, 0, batch);
         if (graph) launch_ii_force_integrate(h->D, 1, h->st);
@@ -761,6 +809,17 @@ int step_iisph(sphgpu_handle* h, sphgpu_stats* st) {
     }
     int extra = 0;
     if ((rc = solver_loop_finish(h, 0, batch, profStart, &extra))) return rc;
+    if (training && h->frame > 0) {
+        // online training on this step's converged pressure
+        launch_ii_targets(D, h->feat, h->tgt, h->st);
+        const int b = h->P.learn_batch;
+        for (int it = 0; it < h->P.learn_iters; ++it) {
+            launch_mlp_gather(D, h->feat, h->tgt, h->bfeat, h->btgt, b, h->gatherSeed++, h->st);
+            mlp_grad(h->mlp, h->bfeat, h->btgt, b, h->st);
+            mlp_adam(h->mlp, h->P.learn_rate, h->st);
+        }
+        ++h->learned;
+    }
     if (!graph || extra > 0) {
         ProfScope ps(h, "ii_force", nf_live(h) * 56.0);
         launch_ii_force_integrate(D, graph ? 2 : 0, h->st);
@@ -920,6 +979,7 @@ void sphgpu_default_params(sphgpu_params* p) {
     p->learn_rate = 1e-3f;
     p->learn_over_weight = 1.0f;
     p->warm_scale = 0.7f;
+    p->learn_scale = 0.7f;
     p->boundary_density = 1000.0f;        // SphBase.h:576
     p->adjust_density = 0;                // SphBase.h:575
     p->fluid_capacity = 0;
@@ -947,11 +1007,12 @@ int sphgpu_create(const sphgpu_params* p, sphgpu_handle** out) {
     if (p->num_scenes < 1 || p->num_scenes > 4096) return SPHGPU_E_ARG;
     if (p->warm_start < 0 || p->warm_start > 2) return SPHGPU_E_ARG;
     if (p->warm_start == SPHGPU_WARM_LEARNED &&
-        (p->solver != SPHGPU_DFSPH || p->learn_steps < 0 || p->learn_iters < 0 || p->learn_batch < 16 ||
+        ((p->solver != SPHGPU_DFSPH && p->solver != SPHGPU_IISPH) || p->learn_steps < 0 || p->learn_iters < 0 || p->learn_batch < 16 ||
          p->learn_batch > (1 << 22) || !(p->learn_rate >= 0) || !(p->learn_over_weight > 0) ||
          !std::isfinite(p->learn_over_weight)))
         return SPHGPU_E_ARG;
     if (!(p->warm_scale >= 0.0f) || !(p->warm_scale <= 4.0f)) return SPHGPU_E_ARG;
+    if (!(p->learn_scale >= 0.0f) || !(p->learn_scale <= 4.0f)) return SPHGPU_E_ARG;
     if (p->use_graph < 0 || p->use_graph > 1) return SPHGPU_E_ARG;
     // every solver loop ends at a finite cap (and reports reaching it)
     if (p->max_iter_density < 1 || p->max_iter_density > SPHGPU_MAX_ITER || p->max_iter_div < 0 ||
@@ -1307,6 +1368,7 @@ int sphgpu_init(sphgpu_handle* h) {
     D.spacing = s;
     D.dt_max = P.dt;
     D.warm_scale = P.warm_scale;
+    D.learn_scale = P.learn_scale;
     {
         // alpha of a particle inside a lattice of the sampling spacing
         // (DFSPH.cpp:541-566 on a full neighbourhood): rho / sum (m |grad W|)^2
@@ -1324,6 +1386,9 @@ int sphgpu_init(sphgpu_handle* h) {
                     g2 += (m * dw) * (m * dw);
                 }
         D.alpha_lat = float(rho / std::max(g2, 1e-6));
+        // IISPH a_ii of the same particle at dt = 1: -(m/rho0)^2 sum |grad W|^2
+        // (its d_ii vanishes on the symmetric lattice)
+        D.aii_lat = float(-g2 / (double(h->phases[0].rho0) * double(h->phases[0].rho0)));
     }
     D.gx = P.gravity[0]; D.gy = P.gravity[1]; D.gz = P.gravity[2];
     D.cs = P.sound_speed;
@@ -1403,7 +1468,7 @@ int sphgpu_init(sphgpu_handle* h) {
         HIPCHECK(h, hipMemset(D.bk, 0, sizeof(float) * (nb + 1)));   // the list-padding dummy stays 0
         HIPCHECK(h, hipMemset(D.pm, 0, sizeof(float) * ncap));
     }
-    if (P.solver == SPHGPU_DFSPH && P.warm_start == SPHGPU_WARM_LEARNED) {
+    if ((P.solver == SPHGPU_DFSPH || P.solver == SPHGPU_IISPH) && P.warm_start == SPHGPU_WARM_LEARNED) {
         ALLOC(h->feat, 8 * (size_t)ncap); ALLOC(h->tgt, ncap); ALLOC(h->ypred, ncap);
         ALLOC(h->bfeat, 8 * (size_t)P.learn_batch); ALLOC(h->btgt, P.learn_batch);
         HIPCHECK(h, hipMemset(h->feat, 0, sizeof(float) * 8 * ncap));

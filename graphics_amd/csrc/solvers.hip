@@ -704,7 +704,7 @@ __global__ __launch_bounds__(256) void k_df_aadv(Dev D) {
 
 // mode 0: no warm start (accumulator restarts at zero); mode 1: warm_scale
 // times the previous step's accumulated stiffness K_prev (Bender & Koschier
-// use one half); mode 2: warm_scale times the model's prediction
+// use one half); mode 2: learn_scale times the model's prediction
 // (s = exp(u_prev + y) - 1), bounded by max(K_prev, K_first), K_first = Drho/Dt alpha
 // being the first increment the solve would make: the model may lower the
 // guess anywhere but raise it only as far as one of those two.  Modes 1
@@ -733,7 +733,7 @@ __global__ __launch_bounds__(256) void k_df_warm_apply(Dev D, const float* __res
             const float alpha = D.alpha[i];
             float u = feat[8 * (size_t)i + 3] + y[i];
             float kp = expm1f(fminf(fmaxf(u, 0.0f), 12.0f)) * 1.0e-3f * alpha * D.rho0[ph] / dt;
-            kv = D.warm_scale * fminf(kp, fmaxf(kprev, d * alpha));
+            kv = D.learn_scale * fminf(kp, fmaxf(kprev, d * alpha));
         }
     }
     D.kvsum[i] = kv;
@@ -1023,8 +1023,19 @@ __global__ __launch_bounds__(256) void k_ii_aadv_dii(Dev D) {
 }
 
 // a_ii, rho_adv, p_l = 0.5 p(t-dt) (IISph_M.cpp:243-278); publishes
-// xk = (x, p_l/rho^2) for the sum_j d_ij p_j gathers.
-__global__ __launch_bounds__(256) void k_ii_aii(Dev D) {
+// xk = (x, p_l/rho^2) for the sum_j d_ij p_j gathers.  With feat (the
+// learned warm start) also the model's eight features per particle:
+//   0 (rho/rho0 - 1) * 100              density deviation
+//   1 (rho_adv/rho0 - 1) * 100          predicted density deviation
+//   2 (fluid + boundary neighbours)/30
+//   3 log(1 + s_prev), s = -a_ii p / rho0 * 1000   previous pressure, as the
+//     relative density change it removes (in 1e-3)
+//   4 a_ii / (dt^2 a_lat)               a_ii against a lattice particle's
+//   5 |v_adv| dt / spacing
+//   6 distance to the nearest boundary particle / spacing (2 if none)
+//   7 dt / dt_max
+template <bool WARM>
+__global__ __launch_bounds__(256) void k_ii_aii(Dev D, float* __restrict__ feat) {
     Slot s = slot_nb(D, false);
     if (!s.live) return;
     const int i = s.i;
@@ -1035,6 +1046,8 @@ __global__ __launch_bounds__(256) void k_ii_aii(Dev D) {
     float dt = D.dt[s.scene];
     float cji = m * dt * dt / (rhoa * rhoa);
     float t1f = 0.f, t1b = 0.f, t2f = 0.f, t2b = 0.f;
+    const float sp = D.spacing;
+    float bmin2 = 4.0f * sp * sp;
     int c = D.cnt[i];
     each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.vs[j]}; }, [&](int j, Rec2 rec) {
         const float4 pb = rec.a, vb = rec.b;
@@ -1047,18 +1060,63 @@ __global__ __launch_bounds__(256) void k_ii_aii(Dev D) {
     each_boundary_lu<4>(D, s, nb_of(c), [&](int j) { return Rec2{D.xb[j], D.vb[j]}; }, [&](int j, Rec2 rec) {
         const float4 pb = rec.a, vb = rec.b;
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        float g = kernel_gradfac(fast_sqrt(r2), D) * pb.w;
         float gx = dx * g, gy = dy * g, gz = dz * g;
         t1b += gx * (va.x - vb.x) + gy * (va.y - vb.y) + gz * (va.z - vb.z);   // v_adv (IISph_M.cpp:269 uses v)
         t2b += dii.x * gx + dii.y * gy + dii.z * gz;
+        if (WARM) bmin2 = fminf(bmin2, r2);
     });
     float aii = m * t2f + rho0 * t2b;
     if (aii == 0.0f) aii = 1.0f;
     D.alpha[i] = aii;
-    D.kr0[i] = rhoa + dt * (m * t1f + rho0 * t1b);
-    float pl = 0.5f * D.ksum[i];
+    const float radv = rhoa + dt * (m * t1f + rho0 * t1b);
+    D.kr0[i] = radv;
+    const float pprev = D.ksum[i];
+    float pl = 0.5f * pprev;
     D.kr[i] = pl;
     D.xk[i] = make_float4(pa.x, pa.y, pa.z, pl / (rhoa * rhoa));
+    if (!WARM) return;
+    float4* f = reinterpret_cast<float4*>(feat) + 2 * (size_t)i;
+    f[0] = make_float4((rhoa / rho0 - 1.0f) * 100.0f, (radv / rho0 - 1.0f) * 100.0f,
+                       (float)(nf_real(c) + nb_of(c)) * (1.0f / 30.0f),
+                       log1pf(fmaxf(-aii * pprev / rho0 * 1000.0f, 0.0f)));
+    f[1] = make_float4(aii / (dt * dt * D.aii_lat), sqrtf(va.x * va.x + va.y * va.y + va.z * va.z) * dt / sp,
+                       sqrtf(bmin2) / sp, dt / D.dt_max);
+}
+
+// Learned start of the IISPH pressure solve (not in the reference, whose
+// start is p_l = 0.5 p(t-dt), IISph_M.cpp:277): p_l = learn_scale times the
+// model's converged pressure, from its prediction y of the change
+// u - u_prev of u = log(1 + s), s = -a_ii p / rho0 * 1000; the guess may
+// fall anywhere below but rise only as far as the larger of the previous
+// pressure and the particle's own Jacobi estimate (rho0 - rho_adv)/a_ii.
+__global__ __launch_bounds__(256) void k_ii_warm_apply(Dev D, const float* __restrict__ y,
+                                                     const float* __restrict__ feat) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i;
+    const float4 pa = D.x[i];
+    const int ph = phase_of(pa);
+    const float rho0 = D.rho0[ph], aii = D.alpha[i], rhoa = D.rho[i];
+    const float u = feat[8 * (size_t)i + 3] + y[i];
+    float pl = 0.0f;
+    if (aii < 0.0f) {
+        const float pm = expm1f(fminf(fmaxf(u, 0.0f), 12.0f)) * 1.0e-3f * rho0 / (-aii);
+        const float pj = fmaxf((rho0 - D.kr0[i]) / aii, 0.0f);
+        pl = D.learn_scale * fminf(pm, fmaxf(D.ksum[i], pj));
+    }
+    D.kr[i] = pl;
+    D.xk[i].w = pl / (rhoa * rhoa);
+}
+
+// converged pressure in the model's units, as the change from feature 3
+__global__ __launch_bounds__(256) void k_ii_targets(Dev D, const float* __restrict__ feat, float* __restrict__ tgt) {
+    Slot s = slot_of(D, false);
+    if (!s.live) return;
+    const int i = s.i;
+    const float rho0 = D.rho0[phase_of(D.x[i])];
+    tgt[i] = log1pf(fmaxf(-D.alpha[i] * D.kr[i] / rho0 * 1000.0f, 0.0f)) - feat[8 * (size_t)i + 3];
 }
 
 // sum_j d_ij p_j = dt^2 sum_f -m p_j/rho_j^2 gradW (IISph_M.cpp:289-309);
@@ -1436,6 +1494,8 @@ void launch_copy_probe(const float4* a, float4* b, size_t n, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, st, a, b, n);
 }
 
+void launch_copy_f4(float4* dst, const float4* src, size_t n, hipStream_t st) { launch_copy_probe(src, dst, n, st); }
+
 void launch_cfl(const Dev& D, float cfl, float spacing, float dtMax, float dtMin, hipStream_t st) {
     if (tiles_of(D) > 0) hipLaunchKernelGGL(k_cfl_partial, dim3(tiles_of(D)), dim3(256), 0, st, D);
     hipLaunchKernelGGL(k_cfl_reduce, dim3(D.nscenes), dim3(256), 0, st, D, cfl, spacing, dtMax, dtMin);
@@ -1496,7 +1556,14 @@ void launch_df_pass_velocity(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_pas
 void launch_df_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_integrate); }
 void launch_ii_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_density); }
 void launch_ii_aadv_dii(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_aadv_dii); }
-void launch_ii_aii(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_aii); }
+void launch_ii_aii(const Dev& D, float* feat, hipStream_t st) {
+    if (feat) SPH_LAUNCH(k_ii_aii<true>, feat);
+    else SPH_LAUNCH(k_ii_aii<false>, feat);
+}
+void launch_ii_warm_apply(const Dev& D, const float* y, const float* feat, hipStream_t st) {
+    SPH_LAUNCH(k_ii_warm_apply, y, feat);
+}
+void launch_ii_targets(const Dev& D, const float* feat, float* tgt, hipStream_t st) { SPH_LAUNCH(k_ii_targets, feat, tgt); }
 void launch_ii_sumdij(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_sumdij); }
 void launch_ii_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_pressure); }
 void launch_ii_force_integrate(const Dev& D, int gate, hipStream_t st) { SPH_LAUNCH(k_ii_force, gate); }

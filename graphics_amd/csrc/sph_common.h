@@ -71,8 +71,10 @@ struct Dev {
     float v0;          // spacing^3 (DFSPH.cpp:88)
     float spacing;     // m_TH.spacing_r
     float alpha_lat;   // DFSPH alpha of an interior lattice particle (feature scale)
+    float aii_lat;     // IISPH a_ii / dt^2 of an interior lattice particle (feature scale)
     float dt_max;
     float warm_scale;  // PREVIOUS warm start: scale of the previous stiffness
+    float learn_scale; // LEARNED warm start: scale of the bounded prediction
     int tileMap;       // 1: contiguous tile runs per XCD, 0: round robin (autotuned)
     int nbStage;       // neighbour-list groups staged in LDS: 6 or 8 (autotuned)
     int listNT;        // 1: passes read list groups with streaming loads (autotuned)
@@ -444,7 +446,10 @@ void launch_df_pass_velocity(const Dev& D, hipStream_t st);
 void launch_df_integrate(const Dev& D, hipStream_t st);
 void launch_ii_density(const Dev& D, hipStream_t st);
 void launch_ii_aadv_dii(const Dev& D, hipStream_t st);
-void launch_ii_aii(const Dev& D, hipStream_t st);
+// with feat: also the learned warm start's eight features per particle
+void launch_ii_aii(const Dev& D, float* feat, hipStream_t st);
+void launch_ii_warm_apply(const Dev& D, const float* y, const float* feat, hipStream_t st);
+void launch_ii_targets(const Dev& D, const float* feat, float* tgt, hipStream_t st);
 void launch_ii_sumdij(const Dev& D, hipStream_t st);
 void launch_ii_pressure(const Dev& D, hipStream_t st);
 void launch_ii_force_integrate(const Dev& D, int gate, hipStream_t st);
@@ -459,6 +464,8 @@ void launch_loop_start(const Dev& D, hipStream_t st);
 void launch_stats(const Dev& D, double* part, double* out, hipStream_t st);
 void launch_fill_i32(int* p, int v, int n, hipStream_t st);
 void launch_copy_probe(const float4* a, float4* b, size_t n, hipStream_t st);
+// dst[i] = src[i] with a kernel (graph-capturable, no copy engine)
+void launch_copy_f4(float4* dst, const float4* src, size_t n, hipStream_t st);
 
 // iisph_df.hip: the IISph class's DF_* solvers
 void launch_idf_prepare(const Dev& D, hipStream_t st);

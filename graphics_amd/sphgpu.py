@@ -62,6 +62,7 @@ class Params(C.Structure):
         ("boundary_density", C.c_float),
         ("adjust_density", C.c_int),
         ("fluid_capacity", C.c_int),
+        ("learn_scale", C.c_float),
     ]
 
 

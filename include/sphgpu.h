@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SPHGPU_ABI_VERSION 2
+#define SPHGPU_ABI_VERSION 3
 #define SPHGPU_MAX_ITER 10000   /* largest accepted solver-loop cap */
 
 enum sphgpu_status {
@@ -92,14 +92,15 @@ typedef struct sphgpu_params {
     int   learn_batch;         /* samples per Adam iteration */
     float learn_rate;          /* Adam learning rate */
     float learn_over_weight;   /* loss weight of over-predictions (>= 1 biases the guess low) */
-    float warm_scale;          /* the guess is warm_scale x the previous stiffness (PREVIOUS) or x the
-                                  bounded prediction (LEARNED); default 0.7 (Bender & Koschier use 0.5) */
+    float warm_scale;          /* PREVIOUS (and LEARNED while it trains): the guess is warm_scale x the
+                                  previous stiffness; default 0.7 (Bender & Koschier use 0.5) */
     int   use_graph;           /* 1: WCSPH/IISPH steps are captured once into HIP graphs and replayed */
     float boundary_density;    /* m_TH.boundaryDensity (SphBase.h:576, 1000): IISph DF_* boundary terms and
                                   the boundary particles' rest density */
     int   adjust_density;      /* m_TH.adjustDensity (SphBase.h:575, 0): IISph DF_* number density */
     int   fluid_capacity;      /* fluid particles the handle can hold, counting later appends (emitters);
                                   0 = the particles added before init */
+    float learn_scale;         /* LEARNED once trained: the guess is learn_scale x the bounded prediction */
 } sphgpu_params;
 
 typedef struct sphgpu_stats {

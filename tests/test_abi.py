@@ -30,12 +30,12 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header_sizes():
     # 4 (solver) + 2*4 + 6*4 + 4 + 3*4 + 6*4 + 2*4 + 4*4 + 4 + 4 + 4 + 8*4 + 3*4
-    assert C.sizeof(sg.Params) == 148
+    assert C.sizeof(sg.Params) == 152
     assert C.sizeof(sg.Stats) % 8 == 0
 
 
 def test_abi_version():
-    assert sg.load().sphgpu_abi_version() == 2
+    assert sg.load().sphgpu_abi_version() == 3
 
 
 def test_default_params_mirror_reference_defaults():
@@ -53,7 +53,7 @@ def test_default_params_mirror_reference_defaults():
 
 @pytest.mark.parametrize("bad", [dict(solver=7), dict(spacing_r=0.0), dict(dt=-1.0),
                                  dict(num_scenes=0), dict(warm_start=9), dict(use_graph=2),
-                                 dict(warm_scale=-0.5),
+                                 dict(warm_scale=-0.5), dict(learn_scale=5.0),
                                  # solver loops must end at a finite cap
                                  dict(max_iter_density=0), dict(max_iter_density=-3),
                                  dict(max_iter_div=-1), dict(max_iter_density=10 ** 6),
@@ -61,7 +61,7 @@ def test_default_params_mirror_reference_defaults():
                                  # the IISph DF_* solvers: no warm start, no graph capture
                                  dict(solver=4, warm_start=1), dict(solver=4, use_graph=1), dict(solver=5),
                                  dict(boundary_density=0.0), dict(adjust_density=2), dict(fluid_capacity=-1),
-                                 dict(solver=sg.IISPH, warm_start=sg.WARM_LEARNED),
+                                 dict(solver=sg.WCSPH, warm_start=sg.WARM_LEARNED),
                                  dict(solver=sg.DFSPH, warm_start=sg.WARM_LEARNED, learn_batch=4)])
 def test_create_rejects_bad_parameters(bad):
     with pytest.raises(sg.SphError) as e:

@@ -9,8 +9,10 @@ from graphics_amd import scenes, sphgpu as sg
 pytestmark = pytest.mark.gpu
 
 
-def _trajectory(sc, use_graph, steps, every=3):
-    sim = scenes.make_simulation(sc, use_graph=use_graph)
+def _trajectory(sc, use_graph, steps, every=3, net=None, **kw):
+    sim = scenes.make_simulation(sc, use_graph=use_graph, **kw)
+    if net is not None:
+        sim.set_mlp_params(net)
     out = []
     for k in range(steps):
         st = sim.step()
@@ -21,16 +23,37 @@ def _trajectory(sc, use_graph, steps, every=3):
     return out
 
 
-@pytest.mark.parametrize("case", ["wcsph", "iisph_batched"])
+def _net():
+    """A network with a non-zero output layer (random, scaled down), so that
+    the learned guess differs from the previous stiffness."""
+    m = sg.Mlp(64, seed=3)
+    p = m.get_params()
+    m.close()
+    p[-65:] *= 0.05
+    return p
+
+
+@pytest.mark.parametrize("case", ["wcsph", "iisph_batched", "dfsph_plain", "dfsph_previous", "dfsph_learned"])
 def test_graph_replay_matches_eager(gpu, case):
+    """Whole captured steps (WCSPH, IISPH: one graph; DFSPH: one per solver
+    loop, with a plain, previous-stiffness or learned start of the
+    divergence loop) against eager launches."""
+    kw, net = {}, None
     if case == "wcsph":
         sc = scenes.small_dam(sg.WCSPH, n=(10, 12, 10), box=(0.5, 0.4, 0.35))
         steps = 40
-    else:
+    elif case == "iisph_batched":
         sc = scenes.iisph_batched_c4(scenes=3, nmin=300, nmax=2000, seed=11)
         steps = 24
-    eager = _trajectory(sc, 0, steps)
-    graph = _trajectory(sc, 1, steps)
+    else:
+        sc = scenes.small_dam(sg.DFSPH, n=(10, 12, 10), box=(0.5, 0.4, 0.35))
+        steps = 24
+        kw = dict(warm_start={"dfsph_plain": sg.WARM_NONE, "dfsph_previous": sg.WARM_PREVIOUS,
+                              "dfsph_learned": sg.WARM_LEARNED}[case])
+        if case == "dfsph_learned":
+            net = _net()
+    eager = _trajectory(sc, 0, steps, net=net, **kw)
+    graph = _trajectory(sc, 1, steps, net=net, **kw)
     assert len(eager) == len(graph)
     for a, b in zip(eager, graph):
         assert a == b, f"step {a[0]} differs"

@@ -1,7 +1,8 @@
 """Learned warm start of the DFSPH divergence solve, on the device: the
 network trained online by the library's own kernels cuts solver iterations
-against the plain solver at the same tolerances, and leaves the flow where
-the plain solver leaves it."""
+against the plain solver and the previous step's stiffness at the same
+tolerances, and leaves the flow as close to the tightly converged one as the
+plain solver does."""
 import numpy as np
 import pytest
 
@@ -24,23 +25,51 @@ def _iters(st):
     return np.array([[s["iter_density"], s["iter_divergence"]] for s in st], float)
 
 
+def _tight(sc, steps, **kw):
+    """The plain solver at a tenth of both tolerances: the flow that the
+    solves approach as they converge."""
+    p = sc.params
+    st, _ = _run(sc, sg.WARM_NONE, steps, density_tol=p["density_tol"] / 10,
+                 divergence_tol=p["divergence_tol"] / 10, max_iter_density=300, max_iter_div=300, **kw)
+    return st
+
+
+def _flow_gap(a, b):
+    """Largest kinetic-energy difference of two runs over the simulated time
+    both cover.  The runs may step with different CFL time steps, so the
+    energies are compared at equal times, not at equal step numbers."""
+    ta, tb = np.cumsum([s["dt"] for s in a]), np.cumsum([s["dt"] for s in b])
+    ka = np.array([s["kinetic_energy"] for s in a])
+    kb = np.array([s["kinetic_energy"] for s in b])
+    t = np.linspace(max(ta[0], tb[0]), min(ta[-1], tb[-1]), 400)
+    return float(np.abs(np.interp(t, ta, ka) - np.interp(t, tb, kb)).max())
+
+
 def test_learned_warm_start_cuts_iterations_1m_dam(gpu):
     sc = scenes.WORKLOADS["dfsph_c2"]()
     steps = 140
     plain, _ = _run(sc, sg.WARM_NONE, steps)
+    previous, _ = _run(sc, sg.WARM_PREVIOUS, steps)
     learned, params = _run(sc, sg.WARM_LEARNED, steps)
-    ip, il = _iters(plain)[LEARN_STEPS + 5:], _iters(learned)[LEARN_STEPS + 5:]
+    tight = _tight(sc, steps)
+    ip, iq, il = (_iters(r)[LEARN_STEPS + 5:] for r in (plain, previous, learned))
     reduction = 1.0 - il.sum(1).mean() / ip.sum(1).mean()
-    print(f"iterations plain {ip.mean(0)}, learned {il.mean(0)}, reduction {reduction:.3f}")
-    assert reduction >= 0.25
-    # both meet the same tolerances ...
+    print(f"iterations plain {ip.mean(0)}, previous {iq.mean(0)}, learned {il.mean(0)}, "
+          f"reduction {reduction:.3f}")
+    assert reduction >= 0.30
+    # the trained network beats the previous step's stiffness as a start
+    assert il.sum(1).mean() < 0.97 * iq.sum(1).mean()
+    # every solve meets its tolerances (or reports its cap) ...
     for s in learned:
         assert s["density_err"] <= sc.params["density_tol"] + 1e-9 or s["max_iter_hit"]
         assert s["nan_count"] == 0
-    # ... and the flow stays where the plain solver takes it
-    kp = np.array([s["kinetic_energy"] for s in plain])
-    kl = np.array([s["kinetic_energy"] for s in learned])
-    assert np.abs(kl - kp).max() / kp.max() < 0.06
+    # ... and the learned start leaves the flow no further from the tightly
+    # converged one than the plain solver's own tolerance does
+    gap_plain, gap_learned = _flow_gap(plain, tight), _flow_gap(learned, tight)
+    kmax = max(s["kinetic_energy"] for s in tight)
+    print(f"kinetic-energy gap to the tight solve: plain {gap_plain / kmax:.4f}, "
+          f"learned {gap_learned / kmax:.4f} of the peak")
+    assert gap_learned <= 1.5 * gap_plain + 0.02 * kmax
     assert np.all(np.isfinite(params)) and np.abs(params).max() > 0
 
 
@@ -59,3 +88,28 @@ def test_set_network_is_used_from_the_first_step(gpu):
         assert s["nan_count"] == 0
     np.testing.assert_array_equal(b.mlp_params(), p)   # no training on a set network
     b.close()
+
+
+def test_learned_iisph_pressure_start(gpu):
+    """The learned start of the IISPH pressure solve: trained online during
+    the first steps, then used; every step still meets the tolerance (or
+    reports its cap), the flow stays as close to the tightly converged one as
+    with the reference's 0.5 p(t-dt) start, and the iterations do not grow."""
+    sc = scenes.small_dam(sg.IISPH, n=(16, 16, 16), box=(0.8, 0.5, 0.4))
+    steps = 80
+    ref, _ = _run(sc, sg.WARM_NONE, steps)
+    learned, params = _run(sc, sg.WARM_LEARNED, steps, learn_batch=2048)
+    for s in learned:
+        assert s["nan_count"] == 0
+        assert s["max_iter_hit"] or s["density_err"] <= sc.params["density_tol"] + 1e-9
+    tight = _tight(sc, steps)
+    gap_ref, gap_learned = _flow_gap(ref, tight), _flow_gap(learned, tight)
+    kmax = max(s["kinetic_energy"] for s in tight)
+    print(f"kinetic-energy gap to the tight solve: reference start {gap_ref / kmax:.4f}, "
+          f"learned {gap_learned / kmax:.4f} of the peak")
+    assert gap_learned <= 1.5 * gap_ref + 0.02 * kmax
+    ir = np.array([s["iter_density"] for s in ref[LEARN_STEPS + 5:]]).mean()
+    il = np.array([s["iter_density"] for s in learned[LEARN_STEPS + 5:]]).mean()
+    print(f"IISPH iterations: reference start {ir:.2f}, learned start {il:.2f}")
+    assert il <= ir + 0.5
+    assert np.all(np.isfinite(params)) and np.abs(params).max() > 0
