@@ -210,39 +210,52 @@ __device__ __forceinline__ int win_particle(int k) {
 }
 
 // Stages the window: load(j) reads what a record needs from particle j,
-// store(k, r) writes it to window record k; four records per thread in flight.
+// store(k, r) writes it to window record k.  A window holds at most
+// WIN_MAX = 12 x 256 records, so each thread requests all of its records
+// before it stores the first.
 template <class LD, class ST>
 __device__ __forceinline__ void win_stage(int n, LD&& load, ST&& store) {
-    for (int k0 = threadIdx.x; k0 < n; k0 += 4 * 256) {
-        int kk[4];
+    constexpr int DEPTH = WIN_MAX / 256;
+    decltype(load(0)) r[DEPTH];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) kk[u] = k0 + 256 * u;
-        decltype(load(0)) r[4];
+    for (int u = 0; u < DEPTH; ++u) {
+        const int k = threadIdx.x + 256 * u;
+        if (k < n) r[u] = load(win_particle(k));
+    }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) r[u] = load(win_particle(min(kk[u], n - 1)));
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (kk[u] < n) store(kk[u], r[u]);
+    for (int u = 0; u < DEPTH; ++u) {
+        const int k = threadIdx.x + 256 * u;
+        if (k < n) store(k, r[u]);
     }
     __syncthreads();
 }
 
-// Neighbour loop over a packed fluid list whose records are in the LDS
-// window: load(k) reads window record k, use(k, r) consumes it.  The next
-// list group is requested before the current one is used (list loads and
-// LDS reads wait on separate counters); entries past the padded count are
-// read from record 0 and not used.
-template <class LD, class US>
-__device__ __forceinline__ void each_in_win16(const int4* __restrict__ list, int ncap, int i, int n, bool nt,
-                                              LD&& load, US&& use) {
-    const int4* __restrict__ col = list + i;
+// The packed fluid list of one particle in registers: every group is
+// requested at once, before the pass stages its window, so the list loads
+// overlap the staging instead of waiting one group at a time.
+constexpr int NB16_GROUPS = NBF_CAP / 8;
+struct List16 {
+    int4 g[NB16_GROUPS];
+};
+__device__ __forceinline__ void list16_load(const int4* __restrict__ list, int ncap, int i, int n, bool nt,
+                                            List16& L) {
     const int ng = (n + 7) >> 3;
-    if (ng <= 0) return;
+#pragma unroll
+    for (int g = 0; g < NB16_GROUPS; ++g)
+        L.g[g] = g < ng ? ld_group(&list[(size_t)g * ncap + i], nt) : make_int4(0, 0, 0, 0);
+}
+
+// Neighbour loop over a packed fluid list (in registers) whose records are
+// in the LDS window: load(k) reads window record k, use(k, r) consumes it.
+// Entries past the padded count are read from record 0 and not used.
+template <class LD, class US>
+__device__ __forceinline__ void each_in_win16(const List16& L, int n, LD&& load, US&& use) {
+    const int ng = (n + 7) >> 3;
     const int n4 = (n + 3) & ~3;
-    int4 cur = ld_group(&col[0], nt);
-    for (int g = 0; g < ng; ++g) {
-        int4 nxt = cur;
-        if (g + 1 < ng) nxt = ld_group(&col[(size_t)(g + 1) * ncap], nt);
+#pragma unroll
+    for (int g = 0; g < NB16_GROUPS; ++g) {
+        if (g >= ng) break;
+        const int4 cur = L.g[g];
         const unsigned w[4] = {(unsigned)cur.x, (unsigned)cur.y, (unsigned)cur.z, (unsigned)cur.w};
         const bool second = 8 * g + 4 < n4;
         int k[8];
@@ -261,7 +274,6 @@ __device__ __forceinline__ void each_in_win16(const int4* __restrict__ list, int
 #pragma unroll
             for (int q = 4; q < 8; ++q) use(k[q], r[q]);
         }
-        cur = nxt;
     }
 }
 
@@ -279,6 +291,15 @@ __device__ __forceinline__ int nf_of(int c) { return c & 0xff; }
 __device__ __forceinline__ int nf_real(int c) { return (c >> 8) & 0xff; }
 __device__ __forceinline__ int nb_of(int c) { return (c >> 16) & 0xff; }
 __device__ __forceinline__ int phase_of_cnt(int c) { return c >> 24; }
+
+// The list of a live particle of a tile that may stage a window (packed,
+// windows on); the window itself is decided by win_layout.
+__device__ __forceinline__ void list16_prefetch(const Dev& D, const Slot& s, int c, List16& L) {
+    if (s.live && s.packed && D.winOn) {
+        const bool nt = __builtin_amdgcn_readfirstlane(D.listNT) != 0;
+        list16_load(D.nbF16, D.ncap, s.i, nf_of(c), nt, L);
+    }
+}
 
 __device__ __forceinline__ bool inside_domain(const SceneGrid& G, float4 x) {
     return G.smin[0] <= x.x && G.smin[1] <= x.y && G.smin[2] <= x.z && x.x <= G.smax[0] &&
@@ -801,11 +822,9 @@ __device__ __forceinline__ void stage_xv(const Dev& D, int n, WinXV w) {
 
 // density_change with the fluid records read from the LDS window
 __device__ __forceinline__ float density_change_win(const Dev& D, const Slot& s, float4 pa, float4 va, float m,
-                                                    float rho0, int c, WinXV w) {
+                                                    float rho0, int c, WinXV w, const List16& L) {
     float sf = 0.0f;
-    const bool nt = __builtin_amdgcn_readfirstlane(D.listNT) != 0;
-    each_in_win16(D.nbF16, D.ncap, s.i, nf_of(c), nt, [&](int k) { return RecW{w.xy[k], w.zv[k]}; },
-                  [&](int, RecW rec) {
+    each_in_win16(L, nf_of(c), [&](int k) { return RecW{w.xy[k], w.zv[k]}; }, [&](int, RecW rec) {
         sf += dchange_term(D, pa, va, rec.a.x, rec.a.y, rec.b.x, rec.b.y, rec.b.z, rec.b.w);
     });
     const float sb = dchange_boundary(D, s, pa, va, c);
@@ -828,6 +847,10 @@ __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
         write_partial(D, s.tile, 0, 0.0, lds);
         return;
     }
+    // own records and list first: their loads overlap the window staging
+    const int c = s.live ? D.cnt[s.i] : 0;
+    List16 L;
+    list16_prefetch(D, s, c, L);
     const int nw = win_layout(D, s);
     if (nw) stage_xv(D, nw, win);
     double contrib = 0.0;
@@ -836,8 +859,7 @@ __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
         float4 pa = D.x[i], va = D.vs[i];
         int ph = phase_of(pa);
         float m = D.mass[ph], rho0 = D.rho0[ph];
-        const int c = D.cnt[i];
-        float dd = nw ? density_change_win(D, s, pa, va, m, rho0, c, win)
+        float dd = nw ? density_change_win(D, s, pa, va, m, rho0, c, win, L)
                       : density_change(D, s, pa, va, m, rho0, c);
         float dt = D.dt[s.scene];
         float rhoa = va.w;   // v* carries rho in w (as v does), one array fewer to stream
@@ -867,6 +889,10 @@ __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected
         write_partial(D, s.tile, 0, 0.0, lds);
         return;
     }
+    // own records and list first: their loads overlap the window staging
+    const int c = s.live ? D.cnt[s.i] : 0;
+    List16 L;
+    list16_prefetch(D, s, c, L);
     const int nw = win_layout(D, s);
     if (nw) stage_xv(D, nw, win);
     double contrib = 0.0;
@@ -875,8 +901,7 @@ __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected
         float4 pa = D.x[i], va = D.vs[i];
         int ph = phase_of(pa);
         float m = D.mass[ph], rho0 = D.rho0[ph];
-        const int c = D.cnt[i];
-        float dd = nw ? density_change_win(D, s, pa, va, m, rho0, c, win)
+        float dd = nw ? density_change_win(D, s, pa, va, m, rho0, c, win, L)
                       : density_change(D, s, pa, va, m, rho0, c);
         float dt = D.dt[s.scene];
         float rhoa = va.w;   // v* carries rho in w (as v does), one array fewer to stream
@@ -907,12 +932,14 @@ __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
     __shared__ float4 s_wxk[WIN_MAX];   // window records: xk of each neighbour
     Slot s = slot_nb(D, true);
     if (!D.active[s.scene]) return;     // uniform per block
+    const int c = s.live ? D.cnt[s.i] : 0;
+    List16 L;
+    list16_prefetch(D, s, c, L);
     const int nw = win_layout(D, s);
     if (nw) win_stage(nw, [&](int j) { return D.xk[j]; }, [&](int k, float4 r) { s_wxk[k] = r; });
     if (!s.live) return;
     const int i = s.i;
     float4 pa = D.xk[i];
-    int c = D.cnt[i];
     int ph = phase_of_cnt(c);   // (the phase from the count word: no read of x)
     float m = D.mass[ph], rho0 = D.rho0[ph];
     float ka = pa.w;
@@ -923,8 +950,7 @@ __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
         fx += dx * g; fy += dy * g; fz += dz * g;
     };
     if (nw) {
-        const bool nt = __builtin_amdgcn_readfirstlane(D.listNT) != 0;
-        each_in_win16(D.nbF16, D.ncap, i, nf_of(c), nt, [&](int k) { return s_wxk[k]; }, fluid);
+        each_in_win16(L, nf_of(c), [&](int k) { return s_wxk[k]; }, fluid);
     } else {
         each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, fluid);
     }
@@ -1126,6 +1152,9 @@ __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
     __shared__ float4 s_wxk[WIN_MAX];   // window records: xk of each neighbour
     Slot s = slot_nb(D, true);
     if (!D.active[s.scene]) return;     // uniform per block
+    const int c = s.live ? D.cnt[s.i] : 0;
+    List16 L;
+    list16_prefetch(D, s, c, L);
     const int nw = win_layout(D, s);
     if (nw) win_stage(nw, [&](int j) { return D.xk[j]; }, [&](int k, float4 r) { s_wxk[k] = r; });
     if (!s.live) return;
@@ -1133,7 +1162,6 @@ __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
     // own position from the xk records the pass gathers (xyz = x during
     // the loop), the phase from the count word: no read of x
     const float4 pa = D.xk[i];
-    const int c = D.cnt[i];
     float m = D.mass[phase_of_cnt(c)];
     float sx = 0.f, sy = 0.f, sz = 0.f;
     auto fluid = [&](int, float4 pb) {
@@ -1142,8 +1170,7 @@ __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
         sx += dx * g; sy += dy * g; sz += dz * g;
     };
     if (nw) {
-        const bool nt = __builtin_amdgcn_readfirstlane(D.listNT) != 0;
-        each_in_win16(D.nbF16, D.ncap, i, nf_of(c), nt, [&](int k) { return s_wxk[k]; }, fluid);
+        each_in_win16(L, nf_of(c), [&](int k) { return s_wxk[k]; }, fluid);
     } else {
         each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, fluid);
     }
@@ -1164,6 +1191,9 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
         write_partial(D, s.tile, 0, 0.0, lds);
         return;
     }
+    const int c = s.live ? D.cnt[s.i] : 0;
+    List16 L;
+    list16_prefetch(D, s, c, L);
     // window records: x and u = d_ii p_l + sum_j d_ij p_j of each neighbour
     const int nw = win_layout(D, s);
     if (nw) win_stage(nw, [&](int j) { return Rec2{D.x[j], D.f4b[j]}; }, [&](int k, Rec2 r) {
@@ -1182,7 +1212,6 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
         // own sum_j d_ij p_j
         float sax = ua.x - dia.x * pla, say = ua.y - dia.y * pla, saz = ua.z - dia.z * pla;
         float ff = 0.f, fb = 0.f;
-        int c = D.cnt[i];
         auto fluid = [&](float bx, float by, float bz, float ux, float uy, float uz) {
             float dx = pa.x - bx, dy = pa.y - by, dz = pa.z - bz;
             float g = kernel_gradfac(fast_sqrt(dot3f(dx, dy, dz, dx, dy, dz)), D);
@@ -1194,8 +1223,7 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
             ff += dot3f(tx, ty, tz, gx, gy, gz);
         };
         if (nw) {
-            const bool nt = __builtin_amdgcn_readfirstlane(D.listNT) != 0;
-            each_in_win16(D.nbF16, D.ncap, i, nf_of(c), nt, [&](int k) { return RecW{win.xy[k], win.zv[k]}; },
+            each_in_win16(L, nf_of(c), [&](int k) { return RecW{win.xy[k], win.zv[k]}; },
                           [&](int, RecW r) { fluid(r.a.x, r.a.y, r.b.x, r.b.y, r.b.z, r.b.w); });
         } else {
             each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.f4b[j]}; }, [&](int j, Rec2 rec) {
