@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace sphgpu {
@@ -552,6 +553,54 @@ double mlp_loss_host(const MlpDev& m, int n, hipStream_t st) {
     return n > 0 ? s / n : 0.0;
 }
 
+// ---- gradient on caller-provided device buffers, scratch in a static array
+// Every buffer mlp_grad needs beyond its inputs and outputs (weight
+// fragments, activations, their gradients, loss and gradient partials) lives
+// in one file-scope device array, so the entry point allocates nothing.
+// Each GPU holds its own copy until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress returns on that device.
+constexpr int MLP_DEV_MAX = SPHGPU_MLP_DEVICE_MAX;
+constexpr size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t SCR_FWD = 0;
+constexpr size_t SCR_BWD = SCR_FWD + al256(16 * 64 * MLP_FWD_FRAGS);
+constexpr size_t SCR_STORE = SCR_BWD + al256(16 * 64 * MLP_BWD_FRAGS);
+constexpr size_t SCR_DZ = SCR_STORE + al256(sizeof(float) * MLP_STORE * (size_t)MLP_DEV_MAX);
+constexpr size_t SCR_DY = SCR_DZ + al256(sizeof(float) * MLP_STORE * (size_t)MLP_DEV_MAX);
+constexpr size_t SCR_Y = SCR_DY + al256(sizeof(float) * MLP_DEV_MAX);
+constexpr size_t SCR_LOSS = SCR_Y + al256(sizeof(float) * MLP_DEV_MAX);
+constexpr size_t SCR_GPART = SCR_LOSS + al256(sizeof(double) * (MLP_DEV_MAX / 256 + 2));
+constexpr size_t SCR_BYTES =
+    SCR_GPART + al256(sizeof(float) * 256 * MLP_WGRAD_TILES * ((MLP_DEV_MAX + MLP_WGRAD_SLICE - 1) / MLP_WGRAD_SLICE));
+
+static __device__ __align__(256) unsigned char scratch[SCR_BYTES];
+
+namespace {
+// mean loss: the block partials summed in order by one thread (the order
+// mlp_loss_host sums them in), divided by n
+__global__ void k_mlp_loss_final(const double* __restrict__ part, int parts, int n, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int k = 0; k < parts; ++k) s += part[k];
+    *out = s / n;
+}
+}  // namespace
+
+// device address of `scratch` on the current device, looked up once per device
+static int scratch_on_device(unsigned char** out) {
+    static std::mutex mu;
+    static unsigned char* ptr[64] = {};
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return SPHGPU_E_DEVICE;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SPHGPU_E_DEVICE;
+        ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = ptr[dev];
+    return SPHGPU_OK;
+}
+
 }  // namespace sphgpu
 
 // ------------------------------------------------------------- C-ABI ----
@@ -648,5 +697,39 @@ extern "C" int sphgpu_mlp_train(sphgpu_mlp* h, long long n, const float* feat, c
     sphgpu::mlp_grad(h->m, h->feat, h->target, k, h->st);
     double l = sphgpu::mlp_loss_host(h->m, k, h->st);
     if (loss) *loss = l;
+    return hipGetLastError() == hipSuccess ? SPHGPU_OK : SPHGPU_E_DEVICE;
+}
+
+extern "C" int sphgpu_mlp_gradient_device(const float* params, long long n, const float* feat, const float* target,
+                                          float over_weight, float* grad, double* loss, void* stream) {
+    using namespace sphgpu;
+    if (!params || !feat || !target || !grad || !loss || n <= 0 || !(over_weight > 0.0f) ||
+        !std::isfinite(over_weight))
+        return SPHGPU_E_ARG;
+    if (n > MLP_DEV_MAX) return SPHGPU_E_CAPACITY;
+    unsigned char* base = nullptr;
+    if (int rc = scratch_on_device(&base)) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    MlpDev m;
+    m.params = const_cast<float*>(params);   // read only: no Adam step here
+    m.grad = grad;
+    m.fwd = base + SCR_FWD;
+    m.bwd = base + SCR_BWD;
+    m.store = reinterpret_cast<float*>(base + SCR_STORE);
+    m.dz = reinterpret_cast<float*>(base + SCR_DZ);
+    m.dy = reinterpret_cast<float*>(base + SCR_DY);
+    m.scratchY = reinterpret_cast<float*>(base + SCR_Y);
+    m.lossPart = reinterpret_cast<double*>(base + SCR_LOSS);
+    m.gradPart = reinterpret_cast<float*>(base + SCR_GPART);
+    m.overWeight = over_weight;
+    m.cap = MLP_DEV_MAX;
+    const int k = int(n);
+    // every scratch word the kernels read is written earlier in this call:
+    // fragments by the pack, activations and gradients for samples < n, loss
+    // partials per block, gradient partials per slice
+    mlp_pack(m, st);
+    mlp_grad(m, feat, target, k, st);
+    hipLaunchKernelGGL(k_mlp_loss_final, dim3(1), dim3(64), 0, st, (const double*)m.lossPart, (k + 255) / 256, k,
+                       loss);
     return hipGetLastError() == hipSuccess ? SPHGPU_OK : SPHGPU_E_DEVICE;
 }

@@ -157,6 +157,8 @@ SIGNATURES = {
     "sphgpu_mlp_forward": (C.c_int, [C.c_void_p, C.c_longlong, _FP, _FP]),
     "sphgpu_mlp_gradient": (C.c_int, [C.c_void_p, C.c_longlong, _FP, _FP, _FP, _DP]),
     "sphgpu_mlp_set_loss": (C.c_int, [C.c_void_p, C.c_float]),
+    "sphgpu_mlp_gradient_device": (C.c_int, [C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_float,
+                                             C.c_void_p, C.c_void_p, C.c_void_p]),
     "sphgpu_mlp_train": (C.c_int, [C.c_void_p, C.c_longlong, _FP, _FP, C.c_int, C.c_float, _DP]),
     "sphgpu_get_mlp_params": (C.c_int, [C.c_void_p, _FP]),
     "sphgpu_set_mlp_params": (C.c_int, [C.c_void_p, _FP]),

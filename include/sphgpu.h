@@ -395,6 +395,23 @@ int  sphgpu_mlp_set_loss(sphgpu_mlp* m, float over_weight);
 /* `iters` Adam steps on the full set (n <= capacity); *loss = final mean squared error */
 int  sphgpu_mlp_train(sphgpu_mlp* m, long long n, const float* feat, const float* target,
                       int iters, float lr, double* loss);
+/* Gradient of the loss of sphgpu_mlp_gradient on caller-provided DEVICE
+ * buffers, allocating nothing: params (SPHGPU_MLP_PARAMS FP32 master
+ * weights), feat (n x 8), target (n) in; grad (SPHGPU_MLP_PARAMS) and loss
+ * (one double, the weighted mean squared error) out; over_weight as in
+ * sphgpu_mlp_set_loss (1 = mean squared error).  Work is queued on `stream`
+ * (a hipStream_t; NULL = the default stream) and the call returns without
+ * waiting.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight training
+ * loops).  Its scratch (weight fragments, activations, partial sums; 8.2
+ * MB) is one static array per GPU, held until the process ends, so
+ * n <= SPHGPU_MLP_DEVICE_MAX (else SPHGPU_E_CAPACITY), and calls that
+ * overlap on one GPU (on different streams, or from different threads)
+ * race on it; calls on one stream run one after another.  Same kernels and
+ * the same results as sphgpu_mlp_gradient. */
+#define SPHGPU_MLP_DEVICE_MAX 4096
+int  sphgpu_mlp_gradient_device(const float* params, long long n, const float* feat, const float* target,
+                                float over_weight, float* grad, double* loss, void* stream);
 /* network of a simulation handle (SPHGPU_MLP_PARAMS floats) */
 int  sphgpu_get_mlp_params(sphgpu_handle* h, float* params);
 int  sphgpu_set_mlp_params(sphgpu_handle* h, const float* params);
