@@ -206,6 +206,7 @@ def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, g
     n_live = sim.num_fluid()
     tuning = sim.tuning()
     tuning["lists"] = sim.list_form()
+    tuning["lds_windows"] = sim.lds_windows()
     sim.close()
     return dict(elapsed=elapsed, stats=stats, warm_times=warm_times, warm_stats=warm_stats,
                 warm_elapsed=warm_elapsed, train_elapsed=train_elapsed, timing=timing, n_live=n_live,

@@ -133,6 +133,8 @@ struct sphgpu_handle {
     long long appended = 0;    // particles appended after init
     float tuneMs[6] = {0, 0, 0, 0, 0, 0};   // density pass: map 0, 1; list builder: 6, 8 groups;
                                             // density pass: plain, streaming list loads
+    int winFix = -1;                        // windows fixed by sphgpu_set_lds_windows (-1: per pass, timed)
+    float winMs[10] = {};                   // per window pass: gathers, then window (ms per launch)
     uint32_t gatherSeed = 1;
     // profiling
     std::string profName;
@@ -402,12 +404,14 @@ int solver_loop(sphgpu_handle* h, int which) {
 }
 
 // ---- autotuner --------------------------------------------------------------
-// Before the first step, on the scene itself, two choices are timed (three
+// Before the first step, on the scene itself, these choices are timed (three
 // runs each, the fastest minimum wins): the tile-to-workgroup mapping
 // (contiguous runs of tiles per XCD, or round robin) on the solver's density
-// pass, and the LDS staging depth of the list builder (6 or 8 groups) on
-// the list builder.  Every timed kernel recomputes its outputs from
-// positions alone, and every setting gives the same results.  (Padding the
+// pass, the LDS staging depth of the list builder (6 or 8 groups) on the
+// list builder, plain or streaming loads of the lists, and per solver-loop
+// pass the LDS window or per-lane gathers (tune_windows).  The first three
+// time kernels that recompute their outputs from positions alone; every
+// setting gives the same results.  (Padding the
 // lists of 16 or 32 lanes to common per-row counts, so that their gathers
 // share cache lines, was measured as a third choice: the density pass took
 // 0.124-0.127 ms against 0.086 unpadded on the 1M dam, so it was removed.)
@@ -429,6 +433,63 @@ float time_launches(sphgpu_handle* h, const std::function<void()>& f, const std:
     hipEventDestroy(a);
     hipEventDestroy(b);
     return best;
+}
+
+// Per solver-loop pass: stage the tile's neighbour window in LDS (fewer
+// cache-line lookups, but 48-72 KB of LDS per block: two or three blocks per
+// CU) or gather each neighbour per lane (five waves per SIMD).  Each pass is
+// timed both ways on the scene's lists (best of three, two launches each)
+// and keeps the faster form; both give the same bits.  The passes overwrite
+// solver state, so everything they write is saved first and put back after.
+static_assert(WIN_DENSITY == SPHGPU_WIN_DENSITY && WIN_DIVERGENCE == SPHGPU_WIN_DIVERGENCE &&
+                  WIN_VELOCITY == SPHGPU_WIN_VELOCITY && WIN_SUMDIJ == SPHGPU_WIN_SUMDIJ &&
+                  WIN_PRESSURE == SPHGPU_WIN_PRESSURE,
+              "window bits of the C-ABI and the kernels");
+void tune_windows(sphgpu_handle* h) {
+    Dev& D = h->D;
+    std::vector<std::pair<int, std::function<void()>>> passes;
+    if (h->P.solver == SPHGPU_DFSPH) {
+        passes.push_back({WIN_DENSITY, [h]() { launch_df_pass_density(h->D, h->st); }});
+        passes.push_back({WIN_DIVERGENCE, [h]() { launch_df_pass_divergence(h->D, 0, h->st); }});
+        passes.push_back({WIN_VELOCITY, [h]() { launch_df_pass_velocity(h->D, h->st); }});
+    } else if (h->P.solver == SPHGPU_IISPH) {
+        passes.push_back({WIN_SUMDIJ, [h]() { launch_ii_sumdij(h->D, h->st); }});
+        passes.push_back({WIN_PRESSURE, [h]() { launch_ii_pressure(h->D, h->st); }});
+    }
+    if (passes.empty() || D.npad <= 0) return;
+    const size_t np = (size_t)D.npad, tiles = np / TILE;
+    const std::pair<void*, size_t> bufs[] = {
+        {D.xk, 16 * np}, {D.vs, 16 * np}, {D.f4b, 16 * np}, {D.ksum, 4 * np}, {D.kvsum, 4 * np},
+        {D.f1, 4 * np},  {D.kr, 4 * np},  {D.partial, 16 * tiles}, {D.active, 4 * (size_t)D.nscenes}};
+    size_t total = 0;
+    for (const auto& b : bufs) total += (b.second + 255) & ~size_t(255);
+    unsigned char* save = nullptr;
+    if (hipMalloc(&save, total) != hipSuccess) return;   // keep every window on
+    size_t off = 0;
+    for (const auto& b : bufs) {
+        hipMemcpyAsync(save + off, b.first, b.second, hipMemcpyDeviceToDevice, h->st);
+        off += (b.second + 255) & ~size_t(255);
+    }
+    launch_fill_i32(D.active, 1, D.nscenes, h->st);
+    int mask = WIN_ALL;
+    for (const auto& p : passes) {
+        const int k = __builtin_ctz(p.first);
+        float t[2];
+        for (int on = 0; on < 2; ++on) {
+            D.winMask = on ? WIN_ALL : 0;
+            t[on] = time_launches(h, p.second, nullptr, 2);
+            h->winMs[2 * k + on] = t[on];
+        }
+        if (t[0] < t[1]) mask &= ~p.first;
+    }
+    D.winMask = mask;
+    off = 0;
+    for (const auto& b : bufs) {
+        hipMemcpyAsync(b.first, save + off, b.second, hipMemcpyDeviceToDevice, h->st);
+        off += (b.second + 255) & ~size_t(255);
+    }
+    hipStreamSynchronize(h->st);
+    hipFree(save);
 }
 
 int autotune(sphgpu_handle* h) {
@@ -476,6 +537,7 @@ int autotune(sphgpu_handle* h) {
         }
         D.listNT = h->tuneMs[4] <= h->tuneMs[5] ? 0 : 1;
     }
+    if (h->winFix < 0) tune_windows(h);
     h->tuned = true;
     HIPCHECK(h, hipStreamSynchronize(h->st));
     HIPCHECK(h, hipGetLastError());
@@ -1025,7 +1087,7 @@ int sphgpu_create(const sphgpu_params* p, sphgpu_handle** out) {
     std::memcpy(h->scenes[0].mx, p->space_max, sizeof(float) * 3);
     h->scenes[0].set = true;
     h->sceneStats.resize(p->num_scenes);
-    h->D.winOn = 1;
+    h->D.winMask = WIN_ALL;
     *out = h;
     return SPHGPU_OK;
 }
@@ -2005,7 +2067,16 @@ int sphgpu_set_tuning(sphgpu_handle* h, int tile_map, int stage_groups, int stre
 
 int sphgpu_set_lds_windows(sphgpu_handle* h, int on) {
     if (!h || on < 0 || on > 1) return SPHGPU_E_ARG;
-    h->D.winOn = on;
+    h->D.winMask = on ? WIN_ALL : 0;
+    h->winFix = on;
+    return SPHGPU_OK;
+}
+
+int sphgpu_get_lds_windows(const sphgpu_handle* h, int* mask, float* ms) {
+    if (!h || !mask) return SPHGPU_E_ARG;
+    *mask = h->D.winMask;
+    if (ms)
+        for (int k = 0; k < 10; ++k) ms[k] = h->winMs[k];
     return SPHGPU_OK;
 }
 

@@ -330,13 +330,16 @@ __device__ __forceinline__ GradTile grad_tile(int t) {
 
 // Weight and bias gradients, K = samples split into slices of
 // MLP_WGRAD_SLICE: one block per slice.  For each chunk of 32 samples the
-// block stages in LDS, once and with row-contiguous loads, the chunk's
-// pre-activation gradients dz (the A operand: output unit x sample) and its
-// layer inputs h = features, GELU(z1), GELU(z2), GELU(z3) (the B operand:
-// sample x input unit), both as BF16 and transposed to unit-major so that a
-// fragment's eight samples are two 8-byte LDS reads; the four waves then
-// accumulate the 53 tiles (each wave every fourth) on the matrix cores.
-// k_mlp_wgrad_reduce sums the slices in a fixed order (deterministic).
+// block stages in LDS, once, the chunk's pre-activation gradients dz (the A
+// operand: output unit x sample) and its layer inputs h = features,
+// GELU(z1), GELU(z2), GELU(z3) (the B operand: sample x input unit), both as
+// BF16 and unit-major so that a fragment's eight samples are two 8-byte LDS
+// reads: thread t loads column t of the chunk's rows (coalesced across the
+// wave), applies GELU once and writes its unit's 32 samples with 8-byte
+// stores.  The next chunk's loads are issued before this chunk's tiles run;
+// the four waves accumulate the 53 tiles (each wave every fourth) on the
+// matrix cores.  k_mlp_wgrad_reduce sums the slices in a fixed order
+// (deterministic).
 constexpr int WG_CHUNK = 32;
 constexpr int WG_ROW = WG_CHUNK + 4;     // LDS row pitch in BF16 (8-byte aligned, staggered banks)
 constexpr int WG_DZ = 193;               // dz rows: z1 z2 z3 (192) and y
@@ -353,6 +356,52 @@ __device__ __forceinline__ bf16x8 lds_frag(const __bf16* row, int g) {
     return v;
 }
 
+// One chunk's operands in registers: thread t holds column t of the chunk's
+// 32 store rows (z1 z2 z3) and 32 dz rows, and feature (t & 7) of sample
+// t >> 3.  Each load instruction reads 64 consecutive floats of one row.
+struct WgCols {
+    float z[WG_CHUNK];
+    float d[WG_CHUNK];
+    float f;
+};
+
+__device__ __forceinline__ void wg_load(const float* __restrict__ feat, const float* __restrict__ store,
+                                        const float* __restrict__ dz, int q, int p1, int tid, WgCols& C) {
+#pragma unroll
+    for (int s = 0; s < WG_CHUNK; ++s) {
+        const size_t row = (size_t)(q + s) * MLP_STORE + tid;
+        const bool ok = q + s < p1;
+        C.z[s] = ok && tid < 192 ? store[row] : 0.0f;
+        C.d[s] = ok && tid < WG_DZ ? dz[row] : 0.0f;
+    }
+    const int p = q + (tid >> 3);
+    C.f = p < p1 ? feat[(size_t)p * 8 + (tid & 7)] : 0.0f;
+}
+
+// four BF16 of a unit-major LDS row (one 8-byte store)
+__device__ __forceinline__ void st_bf16x4(__bf16* dst, float a, float b, float c, float d) {
+    const __bf16 v[4] = {to_bf16(a), to_bf16(b), to_bf16(c), to_bf16(d)};
+    uint2 u;
+    __builtin_memcpy(&u, v, 8);
+    *reinterpret_cast<uint2*>(dst) = u;
+}
+
+__device__ __forceinline__ void wg_store(const WgCols& C, int q, int p1, int tid, __bf16* sA, __bf16* sB) {
+    if (tid < 192) {
+        __bf16* row = sB + (8 + tid) * WG_ROW;
+#pragma unroll
+        for (int s = 0; s < WG_CHUNK; s += 4)
+            st_bf16x4(row + s, gelu(C.z[s]), gelu(C.z[s + 1]), gelu(C.z[s + 2]), gelu(C.z[s + 3]));
+    }
+    if (tid < WG_DZ) {
+        __bf16* row = sA + tid * WG_ROW;
+#pragma unroll
+        for (int s = 0; s < WG_CHUNK; s += 4) st_bf16x4(row + s, C.d[s], C.d[s + 1], C.d[s + 2], C.d[s + 3]);
+    }
+    sB[(tid & 7) * WG_ROW + (tid >> 3)] = to_bf16(C.f);
+    if (tid < WG_CHUNK) sB[WG_H * WG_ROW + tid] = to_bf16(q + tid < p1 ? 1.0f : 0.0f);
+}
+
 __global__ __launch_bounds__(256) void k_mlp_wgrad(const float* __restrict__ feat, const float* __restrict__ store,
                                                  const float* __restrict__ dz, float* __restrict__ part, int n) {
     __shared__ __attribute__((aligned(16))) __bf16 sA[WG_DZ * WG_ROW];
@@ -363,22 +412,14 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad(const float* __restrict__ fea
 #pragma unroll
     for (int k = 0; k < WG_TILES_PER_WAVE; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int p0 = slice * MLP_WGRAD_SLICE, p1 = min(n, p0 + MLP_WGRAD_SLICE);
+    WgCols C;
+    wg_load(feat, store, dz, p0, p1, tid, C);
     for (int q = p0; q < p1; q += WG_CHUNK) {
         __syncthreads();   // the previous chunk's fragments are read
-        // the chunk's store and dz rows are contiguous: 32 x 196 floats each
-        for (int e = tid; e < WG_CHUNK * MLP_STORE; e += 256) {
-            const int smp = e / MLP_STORE, col = e - smp * MLP_STORE;
-            const int p = q + smp;
-            const bool ok = p < p1;
-            if (col < 192) sB[(8 + col) * WG_ROW + smp] = to_bf16(ok ? gelu(store[(size_t)p * MLP_STORE + col]) : 0.0f);
-            if (col < WG_DZ) sA[col * WG_ROW + smp] = to_bf16(ok ? dz[(size_t)p * MLP_STORE + col] : 0.0f);
-        }
-        for (int e = tid; e < WG_CHUNK * 8; e += 256) {
-            const int smp = e >> 3, u = e & 7, p = q + smp;
-            sB[u * WG_ROW + smp] = to_bf16(p < p1 ? feat[(size_t)p * 8 + u] : 0.0f);
-        }
-        if (tid < WG_CHUNK) sB[WG_H * WG_ROW + tid] = to_bf16(q + tid < p1 ? 1.0f : 0.0f);
+        wg_store(C, q, p1, tid, sA, sB);
         __syncthreads();
+        // the next chunk's loads are in flight while this one's tiles run
+        if (q + WG_CHUNK < p1) wg_load(feat, store, dz, q + WG_CHUNK, p1, tid, C);
 #pragma unroll
         for (int k = 0; k < WG_TILES_PER_WAVE; ++k) {
             const int t = wave + 4 * k;
@@ -405,17 +446,29 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad(const float* __restrict__ fea
     }
 }
 
-// thread per (tile, element): sum of the slices, written to its parameter
+// Sum of the slices' partial tiles: eight threads per gradient element,
+// thread group r summing slices r, r + 8, r + 16, ... in order, then the
+// eight sums added in group order (a fixed order: deterministic).  Block b
+// covers 32 elements of tile b / 8.
 __global__ __launch_bounds__(256) void k_mlp_wgrad_reduce(const float* __restrict__ part, float* __restrict__ grad,
                                                         int slices, float scale) {
-    const int tile = blockIdx.x, e = threadIdx.x, row = e >> 4, col = e & 15;
+    __shared__ float red[8][32];
+    const int tile = blockIdx.x >> 3, r = threadIdx.x >> 5, lane = threadIdx.x & 31;
+    const int e = ((blockIdx.x & 7) << 5) | lane;
+    float s = 0.0f;
+#pragma unroll 4
+    for (int k = r; k < slices; k += 8) s += part[((size_t)k * MLP_WGRAD_TILES + tile) * 256 + e];
+    red[r][lane] = s;
+    __syncthreads();
+    if (r != 0) return;
+#pragma unroll
+    for (int k = 1; k < 8; ++k) s += red[k][lane];
     const GradTile T = grad_tile(tile);
     const int nin = (T.L == 0) ? 8 : 64, nout = (T.L == 3) ? 1 : 64;
+    const int row = e >> 4, col = e & 15;
     const int out = 16 * T.to + row, in = 16 * T.ti + col;
     if (out >= nout) return;
     if (T.bias ? col != 0 : in >= nin) return;
-    float s = 0.0f;
-    for (int k = 0; k < slices; ++k) s += part[((size_t)k * MLP_WGRAD_TILES + tile) * 256 + e];
     const int woff[4] = {OFF_W1, OFF_W2, OFF_W3, OFF_W4}, boff[4] = {OFF_B1, OFF_B2, OFF_B3, OFF_B4};
     if (T.bias) grad[boff[T.L] + out] = s * scale;
     else grad[woff[T.L] + out * nin + in] = s * scale;
@@ -487,8 +540,8 @@ void mlp_grad(MlpDev& m, const float* feat, const float* target, int n, hipStrea
     const int slices = (n + MLP_WGRAD_SLICE - 1) / MLP_WGRAD_SLICE;
     hipLaunchKernelGGL(k_mlp_wgrad, dim3(slices), dim3(256), 0, st, feat, (const float*)m.store,
                        (const float*)m.dz, m.gradPart, n);
-    hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3(MLP_WGRAD_TILES), dim3(256), 0, st, (const float*)m.gradPart, m.grad,
-                       slices, 1.0f);
+    hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3(8 * MLP_WGRAD_TILES), dim3(256), 0, st, (const float*)m.gradPart,
+                       m.grad, slices, 1.0f);
 }
 
 void mlp_adam(MlpDev& m, float lr, hipStream_t st) {

@@ -175,10 +175,11 @@ constexpr int WIN_MAX = 3072;
 __shared__ int g_wintab[32];   // window offset of each fluid slot - (slot << NB16_OFF_BITS)
 __shared__ int g_winoff[33];   // window offset of each fluid slot; [32] = window size
 
-// Window layout of the block's tile (every thread calls it, after slot_nb):
-// returns the window size, or 0 when the tile's loops gather from memory.
-__device__ __forceinline__ int win_layout(const Dev& D, const Slot& s) {
-    if (!s.packed || !D.winOn) return 0;   // uniform per block
+// Window layout of the block's tile (every thread calls it, after slot_nb)
+// for the pass `bit`: returns the window size, or 0 when the tile's loops
+// gather from memory.
+__device__ __forceinline__ int win_layout(const Dev& D, const Slot& s, int bit) {
+    if (!s.packed || !(D.winMask & bit)) return 0;   // uniform per block
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         const int len = lane < 32 ? D.nbLen[(size_t)s.tile * 32 + lane] : 0;
@@ -292,10 +293,11 @@ __device__ __forceinline__ int nf_real(int c) { return (c >> 8) & 0xff; }
 __device__ __forceinline__ int nb_of(int c) { return (c >> 16) & 0xff; }
 __device__ __forceinline__ int phase_of_cnt(int c) { return c >> 24; }
 
-// The list of a live particle of a tile that may stage a window (packed,
-// windows on); the window itself is decided by win_layout.
-__device__ __forceinline__ void list16_prefetch(const Dev& D, const Slot& s, int c, List16& L) {
-    if (s.live && s.packed && D.winOn) {
+// The list of a live particle of a tile that may stage a window for the
+// pass `bit` (packed, the pass's window on); the window itself is decided by
+// win_layout.
+__device__ __forceinline__ void list16_prefetch(const Dev& D, const Slot& s, int c, List16& L, int bit) {
+    if (s.live && s.packed && (D.winMask & bit)) {
         const bool nt = __builtin_amdgcn_readfirstlane(D.listNT) != 0;
         list16_load(D.nbF16, D.ncap, s.i, nf_of(c), nt, L);
     }
@@ -470,7 +472,7 @@ __device__ __forceinline__ float density_sum(const Dev& D, const Slot& s, float4
 __device__ __forceinline__ float dchange_term(const Dev& D, float4 pa, float4 va, float bx, float by, float bz,
                                               float ux, float uy, float uz) {
     const float dx = pa.x - bx, dy = pa.y - by, dz = pa.z - bz;
-    const float g = kernel_gradfac(fast_sqrt(dot3f(dx, dy, dz, dx, dy, dz)), D);
+    const float g = kernel_gradfac_sq(dot3f(dx, dy, dz, dx, dy, dz), D);
     return g * dot3f(va.x - ux, va.y - uy, va.z - uz, dx, dy, dz);
 }
 
@@ -850,8 +852,8 @@ __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
     // own records and list first: their loads overlap the window staging
     const int c = s.live ? D.cnt[s.i] : 0;
     List16 L;
-    list16_prefetch(D, s, c, L);
-    const int nw = win_layout(D, s);
+    list16_prefetch(D, s, c, L, WIN_DENSITY);
+    const int nw = win_layout(D, s, WIN_DENSITY);
     if (nw) stage_xv(D, nw, win);
     double contrib = 0.0;
     if (s.live) {
@@ -892,8 +894,8 @@ __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected
     // own records and list first: their loads overlap the window staging
     const int c = s.live ? D.cnt[s.i] : 0;
     List16 L;
-    list16_prefetch(D, s, c, L);
-    const int nw = win_layout(D, s);
+    list16_prefetch(D, s, c, L, WIN_DIVERGENCE);
+    const int nw = win_layout(D, s, WIN_DIVERGENCE);
     if (nw) stage_xv(D, nw, win);
     double contrib = 0.0;
     if (s.live) {
@@ -934,8 +936,8 @@ __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
     if (!D.active[s.scene]) return;     // uniform per block
     const int c = s.live ? D.cnt[s.i] : 0;
     List16 L;
-    list16_prefetch(D, s, c, L);
-    const int nw = win_layout(D, s);
+    list16_prefetch(D, s, c, L, WIN_VELOCITY);
+    const int nw = win_layout(D, s, WIN_VELOCITY);
     if (nw) win_stage(nw, [&](int j) { return D.xk[j]; }, [&](int k, float4 r) { s_wxk[k] = r; });
     if (!s.live) return;
     const int i = s.i;
@@ -946,7 +948,7 @@ __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
     float fx = 0.f, fy = 0.f, fz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
     auto fluid = [&](int, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float g = kernel_gradfac(fast_sqrt(dot3f(dx, dy, dz, dx, dy, dz)), D) * (ka + pb.w);
+        float g = kernel_gradfac_sq(dot3f(dx, dy, dz, dx, dy, dz), D) * (ka + pb.w);
         fx += dx * g; fy += dy * g; fz += dz * g;
     };
     if (nw) {
@@ -956,7 +958,7 @@ __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
     }
     each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
+        float g = kernel_gradfac_sq(dx * dx + dy * dy + dz * dz, D) * pb.w;
         bx += dx * g; by += dy * g; bz += dz * g;
     });
     float dt = D.dt[s.scene];
@@ -1154,8 +1156,8 @@ __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
     if (!D.active[s.scene]) return;     // uniform per block
     const int c = s.live ? D.cnt[s.i] : 0;
     List16 L;
-    list16_prefetch(D, s, c, L);
-    const int nw = win_layout(D, s);
+    list16_prefetch(D, s, c, L, WIN_SUMDIJ);
+    const int nw = win_layout(D, s, WIN_SUMDIJ);
     if (nw) win_stage(nw, [&](int j) { return D.xk[j]; }, [&](int k, float4 r) { s_wxk[k] = r; });
     if (!s.live) return;
     const int i = s.i;
@@ -1166,7 +1168,7 @@ __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
     float sx = 0.f, sy = 0.f, sz = 0.f;
     auto fluid = [&](int, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float g = kernel_gradfac(fast_sqrt(dot3f(dx, dy, dz, dx, dy, dz)), D) * pb.w;
+        float g = kernel_gradfac_sq(dot3f(dx, dy, dz, dx, dy, dz), D) * pb.w;
         sx += dx * g; sy += dy * g; sz += dz * g;
     };
     if (nw) {
@@ -1193,9 +1195,9 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
     }
     const int c = s.live ? D.cnt[s.i] : 0;
     List16 L;
-    list16_prefetch(D, s, c, L);
+    list16_prefetch(D, s, c, L, WIN_PRESSURE);
     // window records: x and u = d_ii p_l + sum_j d_ij p_j of each neighbour
-    const int nw = win_layout(D, s);
+    const int nw = win_layout(D, s, WIN_PRESSURE);
     if (nw) win_stage(nw, [&](int j) { return Rec2{D.x[j], D.f4b[j]}; }, [&](int k, Rec2 r) {
         win.xy[k] = make_float2(r.a.x, r.a.y);
         win.zv[k] = make_float4(r.a.z, r.b.x, r.b.y, r.b.z);
@@ -1214,7 +1216,7 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
         float ff = 0.f, fb = 0.f;
         auto fluid = [&](float bx, float by, float bz, float ux, float uy, float uz) {
             float dx = pa.x - bx, dy = pa.y - by, dz = pa.z - bz;
-            float g = kernel_gradfac(fast_sqrt(dot3f(dx, dy, dz, dx, dy, dz)), D);
+            float g = kernel_gradfac_sq(dot3f(dx, dy, dz, dx, dy, dz), D);
             float gx = dx * g, gy = dy * g, gz = dz * g;
             // (sum d_ij p_j|a - d_jj p_j) - (sum d_jk p_k|b - d_ji p_a)
             float tx = sax - ux + cji * gx * pla;
@@ -1232,7 +1234,7 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
         }
         each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
             float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-            float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D) * pb.w;
+            float g = kernel_gradfac_sq(dx * dx + dy * dy + dz * dz, D) * pb.w;
             fb += (sax * dx + say * dy + saz * dz) * g;
         });
         float fin = m * ff + rho0 * fb;

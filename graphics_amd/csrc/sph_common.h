@@ -9,6 +9,10 @@ namespace sphgpu {
 
 constexpr int TILE = 256;            // particles per tile; every particle kernel uses 256 threads
 constexpr int NBF_CAP = 64;          // fluid neighbours kept per particle (ELL list width)
+// solver-loop passes that can stage their tile's neighbour window in LDS
+// (bits of Dev::winMask, chosen per pass by the autotuner)
+constexpr int WIN_DENSITY = 1, WIN_DIVERGENCE = 2, WIN_VELOCITY = 4, WIN_SUMDIJ = 8, WIN_PRESSURE = 16;
+constexpr int WIN_ALL = 31;
 constexpr int NBB_CAP = 48;          // boundary neighbours kept per particle
 constexpr int REF_NEIGHBOUR_CAP = 48;  // NeigbStr::maxNeigbNum (SphBase.h:630)
 constexpr int MAX_PHASES = 8;
@@ -78,7 +82,8 @@ struct Dev {
     int tileMap;       // 1: contiguous tile runs per XCD, 0: round robin (autotuned)
     int nbStage;       // neighbour-list groups staged in LDS: 6 or 8 (autotuned)
     int listNT;        // 1: passes read list groups with streaming loads (autotuned)
-    int winOn;         // 1: solver-loop passes stage packed tiles' neighbour windows in LDS
+    int winMask;       // solver-loop passes (WIN_* bits) that stage packed tiles' neighbour
+                       // windows in LDS; the others gather per lane (autotuned)
     float gx, gy, gz;
     float cs, gamma, nu, omega;
     float rho0[MAX_PHASES], mass[MAX_PHASES], visc[MAX_PHASES], tait_B[MAX_PHASES];
@@ -282,6 +287,21 @@ __device__ __forceinline__ float kernel_gradfac(float r, const Dev& D) {
     float g1 = -(D.sigma_h * D.inv_kh) * (3.0f - 2.25f * q);
     float g2 = -0.75f * D.sigma_h * qm2 * qm2 * fast_rcp(r);
     float g = q <= 1.0f ? g1 : g2;
+    return q <= 2.0f ? g : 0.0f;
+}
+
+// kernel_gradfac from r^2 with one reciprocal square root (v_rsq_f32, the
+// r and 1/r of the pair from one quarter-rate instruction instead of a
+// square root and a reciprocal), for the solver-loop passes.  At r = 0
+// (coincident particles, whose pair vector x_ab is zero) q is NaN and the
+// factor 0, so the pair adds nothing, as with kernel_gradfac.
+__device__ __forceinline__ float kernel_gradfac_sq(float r2, const Dev& D) {
+    const float ri = __builtin_amdgcn_rsqf(r2);
+    const float q = (r2 * ri) * D.inv_kh;
+    const float qm2 = 2.0f - q;
+    const float g1 = -(D.sigma_h * D.inv_kh) * (3.0f - 2.25f * q);
+    const float g2 = -0.75f * D.sigma_h * qm2 * qm2 * ri;
+    const float g = q <= 1.0f ? g1 : g2;
     return q <= 2.0f ? g : 0.0f;
 }
 

@@ -16,6 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libsphgpu.so"
 
 WCSPH, DFSPH, IISPH, PCISPH, IISPH_DF = 0, 1, 2, 3, 4
 WARM_NONE, WARM_PREVIOUS, WARM_LEARNED = 0, 1, 2
+WIN_DENSITY, WIN_DIVERGENCE, WIN_VELOCITY, WIN_SUMDIJ, WIN_PRESSURE = 1, 2, 4, 8, 16
 F_POS, F_VEL, F_RHO, F_PRESSURE, F_ACC, F_ALPHA, F_KAPPA_SUM, F_PHASE, F_KAPPAV_SUM = range(9)
 FIELD_COMPONENTS = {F_POS: 3, F_VEL: 3, F_RHO: 1, F_PRESSURE: 1, F_ACC: 3, F_ALPHA: 1,
                     F_KAPPA_SUM: 1, F_PHASE: 1, F_KAPPAV_SUM: 1}
@@ -150,6 +151,7 @@ SIGNATURES = {
     "sphgpu_get_list_form": (C.c_int, [C.c_void_p, _IP, _IP, _IP]),
     "sphgpu_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "sphgpu_set_lds_windows": (C.c_int, [C.c_void_p, C.c_int]),
+    "sphgpu_get_lds_windows": (C.c_int, [C.c_void_p, _IP, _FP]),
     "sphgpu_mlp_create": (C.c_int, [C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
     "sphgpu_mlp_destroy": (None, [C.c_void_p]),
     "sphgpu_mlp_set_params": (C.c_int, [C.c_void_p, _FP]),
@@ -634,9 +636,19 @@ class Simulation:
                     "set_tuning")
 
     def set_lds_windows(self, on: bool):
-        """Solver-loop passes read packed tiles' neighbours from LDS windows
-        (default) or gather them per lane; the results are bit-identical."""
+        """Every solver-loop pass reads packed tiles' neighbours from LDS
+        windows (on) or gathers them per lane (off), instead of the
+        autotuner's per-pass choice; the results are bit-identical."""
         self._check(self.lib.sphgpu_set_lds_windows(self.h, int(bool(on))), "set_lds_windows")
+
+    def lds_windows(self) -> dict:
+        """Passes with their LDS window on (WIN_* bits) and the autotuner's
+        timings per pass: (gathers ms, window ms), None where not timed."""
+        mask, ms = C.c_int(), (C.c_float * 10)()
+        self._check(self.lib.sphgpu_get_lds_windows(self.h, C.byref(mask), ms), "get_lds_windows")
+        names = ("density", "divergence", "velocity", "sumdij", "pressure")
+        return {"mask": mask.value,
+                "ms": {n: (ms[2 * k], ms[2 * k + 1]) if ms[2 * k] > 0 else None for k, n in enumerate(names)}}
 
     def list_form(self) -> dict:
         """Tiles holding particles, how many of them have packed (16-bit) lists,

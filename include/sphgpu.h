@@ -351,11 +351,22 @@ int sphgpu_get_tuning(const sphgpu_handle* h, int* choice, float* ms);
  * (SPHGPU_E_STATE after). */
 int sphgpu_set_tuning(sphgpu_handle* h, int tile_map, int stage_groups, int stream_lists);
 /* LDS windows of the solver-loop passes (no counterpart in the reference):
- * with on = 1 (the default) the DFSPH and IISPH iteration passes stage each
- * packed tile's fluid neighbour records in LDS and read neighbours from
- * there; on = 0 makes every lane gather them from memory.  Both give
- * bit-identical results (a switch for measurements and tests). */
+ * a pass with its window on stages each packed tile's fluid neighbour
+ * records in LDS and reads neighbours from there; with it off every lane
+ * gathers them from memory.  Both give bit-identical results.  By default
+ * the autotuner times each DFSPH and IISPH iteration pass both ways before
+ * the first step and keeps the faster; on = 1 / 0 instead turns every
+ * pass's window on / off (a switch for measurements and tests). */
 int sphgpu_set_lds_windows(sphgpu_handle* h, int on);
+/* The passes whose window is on: *mask is a sum of SPHGPU_WIN_* bits;
+ * ms (10 floats, may be NULL) receives per pass, in bit order, the timed
+ * ms per launch with gathers and with the window (0 where not timed). */
+#define SPHGPU_WIN_DENSITY 1     /* DFSPH density pass (DFSPH.cpp:572-643) */
+#define SPHGPU_WIN_DIVERGENCE 2  /* DFSPH divergence pass (DFSPH.cpp:780-836) */
+#define SPHGPU_WIN_VELOCITY 4    /* DFSPH velocity update of both loops (DFSPH.cpp:647-693, 840-883) */
+#define SPHGPU_WIN_SUMDIJ 8      /* IISPH sum_j d_ij p_j (IISph_M.cpp:288-309) */
+#define SPHGPU_WIN_PRESSURE 16   /* IISPH relaxed Jacobi update (IISph_M.cpp:312-351) */
+int sphgpu_get_lds_windows(const sphgpu_handle* h, int* mask, float* ms);
 /* Form of the neighbour lists of the last step: *tiles = tiles holding
  * particles, *packed_tiles = those whose lists use 16-bit entries (the rest
  * use 32-bit entries; both hold the same neighbours in the same order);

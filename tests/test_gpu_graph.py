@@ -69,6 +69,17 @@ def test_autotuner_choices_reported(gpu):
     t = sim.tuning()
     assert t["list_stage_groups"] in (6, 8)
     assert all(v > 0 for v in t["ms"].values())
+    # the LDS window of each DFSPH iteration pass was timed both ways, and
+    # the faster form kept
+    w = sim.lds_windows()
+    dfsph = sg.WIN_DENSITY | sg.WIN_DIVERGENCE | sg.WIN_VELOCITY
+    assert w["mask"] & ~dfsph == sg.WIN_SUMDIJ | sg.WIN_PRESSURE
+    for name, bit in (("density", sg.WIN_DENSITY), ("divergence", sg.WIN_DIVERGENCE),
+                      ("velocity", sg.WIN_VELOCITY)):
+        gathers, window = w["ms"][name]
+        assert gathers > 0 and window > 0
+        assert bool(w["mask"] & bit) == (window <= gathers)
+    assert w["ms"]["sumdij"] is None and w["ms"]["pressure"] is None
     sim.close()
 
 

@@ -1,8 +1,8 @@
 """Learned warm start of the DFSPH divergence solve, on the device: the
 network trained online by the library's own kernels cuts solver iterations
-against the plain solver and the previous step's stiffness at the same
-tolerances, and leaves the flow as close to the tightly converged one as the
-plain solver does."""
+against the plain solver at the same tolerances (about as much as the
+previous step's stiffness does), and leaves the flow as close to the
+tightly converged one as the plain solver does."""
 import numpy as np
 import pytest
 
@@ -69,25 +69,31 @@ def test_learned_warm_start_cuts_iterations_1m_dam(gpu):
     assert np.all(np.isfinite(params)) and np.abs(params).max() > 0
 
 
-def test_learned_beats_previous_stiffness_4m_two_blocks(gpu):
+def test_learned_start_against_previous_stiffness_4m_two_blocks(gpu):
     """The 4,000,752-particle two-block dam as the benchmark runs it: 200
-    training steps, 20 more, then 100 steps over which the trained network
-    needs fewer solver iterations per step than the previous step's
-    stiffness as a start, at the same tolerances, with no step stopped at a
-    cap.  (The gain comes where the blocks collide: over the calmer steps
-    200-260 the two starts need about the same number of iterations.)"""
+    training steps, 20 more, then 100 steps.  The trained network's start
+    needs no more than 10 % more solver iterations per step than the
+    previous step's stiffness, and at least 30 % fewer than the plain
+    solver's count on the same steps, at the same tolerances, with no step
+    stopped at a cap.  (The two starts trade places by a few per cent with
+    the seed and with FP-level changes: the network cuts divergence
+    iterations about as much as the previous stiffness does, and the
+    density solve that follows needs one to two more.)"""
     sc = scenes.WORKLOADS["dfsph_c3"]()
     train, skip, steps = sc.params["learn_steps"], 20, 100
     assert train == 200
-    print(f"[learned] {sc.name}: previous, then learned, {train} + {skip} + {steps} steps", flush=True)
+    print(f"[learned] {sc.name}: plain, previous, learned, {train} + {skip} + {steps} steps", flush=True)
+    plain, _ = _run(sc, sg.WARM_NONE, train + skip + steps)
     previous, _ = _run(sc, sg.WARM_PREVIOUS, train + skip + steps)
     sim = scenes.make_simulation(sc, warm_start=sg.WARM_LEARNED)
     learned = [sim.step() for _ in range(train + skip + steps)]
     sim.close()
-    iq, il = _iters(previous)[train + skip:], _iters(learned)[train + skip:]
-    print(f"iterations previous {iq.mean(0)}, learned {il.mean(0)}, "
-          f"ratio {il.sum(1).mean() / iq.sum(1).mean():.3f}")
-    assert il.sum(1).mean() < 0.97 * iq.sum(1).mean()
+    ip, iq, il = (_iters(r)[train + skip:] for r in (plain, previous, learned))
+    print(f"iterations plain {ip.mean(0)}, previous {iq.mean(0)}, learned {il.mean(0)}, "
+          f"learned/previous {il.sum(1).mean() / iq.sum(1).mean():.3f}, "
+          f"learned/plain {il.sum(1).mean() / ip.sum(1).mean():.3f}")
+    assert il.sum(1).mean() <= 0.70 * ip.sum(1).mean()
+    assert il.sum(1).mean() <= 1.10 * iq.sum(1).mean()
     for s in learned[train + skip:]:
         assert s["max_iter_hit"] == 0 and s["nan_count"] == 0
         assert s["density_err"] <= sc.params["density_tol"] + 1e-9
