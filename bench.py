@@ -81,11 +81,13 @@ def kernel_name(raw: str) -> str:
     return raw.replace("sphgpu::", "").replace("(anonymous namespace)::", "").split("(")[0]
 
 
-def summarize_trace(stats_csv: str, trace_csv: str, workload: str, command: str) -> int:
+def summarize_trace(stats_csv: str, trace_csv: str, workload: str, command: str, timed: str | None = None) -> int:
     """Kernel table of a rocprofv3 --kernel-trace --stats run: all dispatches,
     and the dispatches that processed particles (solver iterations launched
     past convergence return at once; launches under 10 us at this size are
-    those); writes profiles/<workload>_kernel_stats.txt."""
+    those); writes profiles/<workload>_kernel_stats.txt.  `timed` =
+    "KERNEL:N" adds the mean of that kernel's last N such dispatches (the
+    timed steps of the profiled bench run, whose line reports N launches)."""
     import csv
     import collections
 
@@ -101,6 +103,12 @@ def summarize_trace(stats_csv: str, trace_csv: str, workload: str, command: str)
         ra = sum(real) / len(real) if real else 0.0
         lines.append(f"  {name[:28]:28s} {len(v):6d} {sum(v) / len(v):9.2f} {100 * sum(v) / total:6.2f} "
                      f"{len(real):11d} {ra:12.2f}")
+    if timed:
+        kname, n = timed.rsplit(":", 1)
+        real = [x for x in per.get(kname, []) if x >= 10.0][-int(n):]
+        if real:
+            lines.append(f"# timed steps: last {len(real)} dispatches of {kname} of 10 us or more: "
+                         f"mean {sum(real) / len(real):.2f} us")
     out = ROOT / "profiles" / f"{workload}_kernel_stats.txt"
     out.write_text("\n".join(lines) + "\n")
     print(out)
@@ -363,6 +371,8 @@ def main() -> int:
                     help="replay captured HIP graphs of whole steps (no per-kernel timing)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COMMAND"),
                     help="write profiles/<workload>_traffic.txt from two rocprofv3 --pmc runs")
+    ap.add_argument("--timed", default=None, metavar="KERNEL:N",
+                    help="with --summarize-trace: mean of the kernel's last N dispatches (the timed steps)")
     ap.add_argument("--summarize-trace", nargs=2, metavar=("TRACE_CSV", "COMMAND"),
                     help="write profiles/<workload>_kernel_stats.txt from a rocprofv3 --kernel-trace run")
     args = ap.parse_args()
@@ -371,7 +381,7 @@ def main() -> int:
     if args.summarize_pmc:
         return summarize_pmc(*args.summarize_pmc[:2], wname(), args.summarize_pmc[2])
     if args.summarize_trace:
-        return summarize_trace(None, args.summarize_trace[0], wname(), args.summarize_trace[1])
+        return summarize_trace(None, args.summarize_trace[0], wname(), args.summarize_trace[1], args.timed)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path runs on one "
               "GPU's share; sharding across GPUs is a design, not built", file=sys.stderr)
