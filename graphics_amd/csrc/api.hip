@@ -445,7 +445,7 @@ static_assert(WIN_DENSITY == SPHGPU_WIN_DENSITY && WIN_DIVERGENCE == SPHGPU_WIN_
                   WIN_VELOCITY == SPHGPU_WIN_VELOCITY && WIN_SUMDIJ == SPHGPU_WIN_SUMDIJ &&
                   WIN_PRESSURE == SPHGPU_WIN_PRESSURE,
               "window bits of the C-ABI and the kernels");
-void tune_windows(sphgpu_handle* h) {
+int tune_windows(sphgpu_handle* h) {
     Dev& D = h->D;
     std::vector<std::pair<int, std::function<void()>>> passes;
     if (h->P.solver == SPHGPU_DFSPH) {
@@ -456,7 +456,7 @@ void tune_windows(sphgpu_handle* h) {
         passes.push_back({WIN_SUMDIJ, [h]() { launch_ii_sumdij(h->D, h->st); }});
         passes.push_back({WIN_PRESSURE, [h]() { launch_ii_pressure(h->D, h->st); }});
     }
-    if (passes.empty() || D.npad <= 0) return;
+    if (passes.empty() || D.npad <= 0) return 0;
     const size_t np = (size_t)D.npad, tiles = np / TILE;
     const std::pair<void*, size_t> bufs[] = {
         {D.xk, 16 * np}, {D.vs, 16 * np}, {D.f4b, 16 * np}, {D.ksum, 4 * np}, {D.kvsum, 4 * np},
@@ -464,11 +464,20 @@ void tune_windows(sphgpu_handle* h) {
     size_t total = 0;
     for (const auto& b : bufs) total += (b.second + 255) & ~size_t(255);
     unsigned char* save = nullptr;
-    if (hipMalloc(&save, total) != hipSuccess) return;   // keep every window on
+    if (hipMalloc(&save, total) != hipSuccess) {   // no room to save the state: keep every window on
+        (void)hipGetLastError();
+        return 0;
+    }
     size_t off = 0;
+    hipError_t e = hipSuccess;
     for (const auto& b : bufs) {
-        hipMemcpyAsync(save + off, b.first, b.second, hipMemcpyDeviceToDevice, h->st);
+        if (e == hipSuccess) e = hipMemcpyAsync(save + off, b.first, b.second, hipMemcpyDeviceToDevice, h->st);
         off += (b.second + 255) & ~size_t(255);
+    }
+    if (e != hipSuccess) {
+        hipFree(save);
+        return fail(h, SPHGPU_E_DEVICE, "saving the solver state for the window tuning failed: %s",
+                    hipGetErrorString(e));
     }
     launch_fill_i32(D.active, 1, D.nscenes, h->st);
     int mask = WIN_ALL;
@@ -485,11 +494,15 @@ void tune_windows(sphgpu_handle* h) {
     D.winMask = mask;
     off = 0;
     for (const auto& b : bufs) {
-        hipMemcpyAsync(b.first, save + off, b.second, hipMemcpyDeviceToDevice, h->st);
+        if (e == hipSuccess) e = hipMemcpyAsync(b.first, save + off, b.second, hipMemcpyDeviceToDevice, h->st);
         off += (b.second + 255) & ~size_t(255);
     }
-    hipStreamSynchronize(h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
     hipFree(save);
+    if (e != hipSuccess)
+        return fail(h, SPHGPU_E_DEVICE, "restoring the solver state after the window tuning failed: %s",
+                    hipGetErrorString(e));
+    return 0;
 }
 
 int autotune(sphgpu_handle* h) {
@@ -537,7 +550,9 @@ int autotune(sphgpu_handle* h) {
         }
         D.listNT = h->tuneMs[4] <= h->tuneMs[5] ? 0 : 1;
     }
-    if (h->winFix < 0) tune_windows(h);
+    if (h->winFix < 0) {
+        if (int rc = tune_windows(h)) return rc;
+    }
     h->tuned = true;
     HIPCHECK(h, hipStreamSynchronize(h->st));
     HIPCHECK(h, hipGetLastError());

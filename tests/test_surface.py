@@ -146,3 +146,21 @@ def test_gpu_field_and_mesh_match_oracle(gpu, aniso):
     np.testing.assert_allclose(tg, to, rtol=0, atol=1e-6)
     assert not _edge_closure(tg.astype(np.float64))
     sf.close()
+
+
+@pytest.mark.gpu
+def test_gpu_field_run_to_run_within_atomic_rounding(gpu):
+    """The splat adds to the field nodes with FP32 atomics, so the order of
+    the additions (and the last bits of a node) can change from run to run;
+    two runs on the same particles agree to within the rounding of those
+    sums (relative 1e-6 of the field's maximum)."""
+    s = 0.01
+    mn, mx = (0.0, 0.0, 0.0), (0.4, 0.4, 0.4)
+    x = sg.lattice((14, 10, 12), (0.1, 0.1, 0.1), s, 0.3, 22, 0)
+    fields = []
+    for _ in range(2):
+        sf = sg.Surfacer(mn, mx, s, 2 * s, s / 4)
+        sf.aniso(x)
+        fields.append(sf.field().astype(np.float64))
+        sf.close()
+    assert np.abs(fields[0] - fields[1]).max() <= 1e-6 * np.abs(fields[0]).max()
