@@ -1057,6 +1057,7 @@ void sphgpu_default_params(sphgpu_params* p) {
     p->learn_over_weight = 1.0f;
     p->warm_scale = 0.7f;
     p->learn_scale = 0.7f;
+    p->multiphase_consistent = 0;
     p->boundary_density = 1000.0f;        // SphBase.h:576
     p->adjust_density = 0;                // SphBase.h:575
     p->fluid_capacity = 0;
@@ -1090,6 +1091,7 @@ int sphgpu_create(const sphgpu_params* p, sphgpu_handle** out) {
         return SPHGPU_E_ARG;
     if (!(p->warm_scale >= 0.0f) || !(p->warm_scale <= 4.0f)) return SPHGPU_E_ARG;
     if (!(p->learn_scale >= 0.0f) || !(p->learn_scale <= 4.0f)) return SPHGPU_E_ARG;
+    if (p->multiphase_consistent < 0 || p->multiphase_consistent > 1) return SPHGPU_E_ARG;
     if (p->use_graph < 0 || p->use_graph > 1) return SPHGPU_E_ARG;
     // every solver loop ends at a finite cap (and reports reaching it)
     if (p->max_iter_density < 1 || p->max_iter_density > SPHGPU_MAX_ITER || p->max_iter_div < 0 ||
@@ -1446,6 +1448,7 @@ int sphgpu_init(sphgpu_handle* h) {
     D.dt_max = P.dt;
     D.warm_scale = P.warm_scale;
     D.learn_scale = P.learn_scale;
+    D.mpCons = P.multiphase_consistent;
     {
         // alpha of a particle inside a lattice of the sampling spacing
         // (DFSPH.cpp:541-566 on a full neighbourhood): rho / sum (m |grad W|)^2

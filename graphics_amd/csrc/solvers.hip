@@ -1073,17 +1073,26 @@ __global__ __launch_bounds__(256) void k_ii_aii(Dev D, float* __restrict__ feat)
     float rhoa = D.rho[i];
     float dt = D.dt[s.scene];
     float cji = m * dt * dt / (rhoa * rhoa);
-    float t1f = 0.f, t1b = 0.f, t2f = 0.f, t2b = 0.f;
+    float t1f = 0.f, t1b = 0.f, t2f = 0.f, t2b = 0.f, cself = 0.f;
     const float sp = D.spacing;
     float bmin2 = 4.0f * sp * sp;
     int c = D.cnt[i];
+    const bool cons = D.mpCons && D.nphases > 1;
     each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.vs[j]}; }, [&](int j, Rec2 rec) {
         const float4 pb = rec.a, vb = rec.b;
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float g = kernel_gradfac(fast_sqrt(dx * dx + dy * dy + dz * dz), D);
         float gx = dx * g, gy = dy * g, gz = dz * g;
         t1f += gx * (va.x - vb.x) + gy * (va.y - vb.y) + gz * (va.z - vb.z);
-        t2f += (dii.x - cji * gx) * gx + (dii.y - cji * gy) * gy + (dii.z - cji * gz) * gz;
+        if (!cons) {
+            t2f += (dii.x - cji * gx) * gx + (dii.y - cji * gy) * gy + (dii.z - cji * gz) * gz;
+        } else {
+            // d_ji of the number-density force: dt^2 m_a^2 / (m_b rho_a^2) gradW
+            const float cb = cji * (m / D.mass[phase_of(pb)]);
+            const float g2 = gx * gx + gy * gy + gz * gz;
+            t2f += dii.x * gx + dii.y * gy + dii.z * gz - cb * g2;
+            cself += cb * g2;
+        }
     });
     each_boundary_lu<4>(D, s, nb_of(c), [&](int j) { return Rec2{D.xb[j], D.vb[j]}; }, [&](int j, Rec2 rec) {
         const float4 pb = rec.a, vb = rec.b;
@@ -1103,7 +1112,8 @@ __global__ __launch_bounds__(256) void k_ii_aii(Dev D, float* __restrict__ feat)
     const float pprev = D.ksum[i];
     float pl = 0.5f * pprev;
     D.kr[i] = pl;
-    D.xk[i] = make_float4(pa.x, pa.y, pa.z, pl / (rhoa * rhoa));
+    D.xk[i] = make_float4(pa.x, pa.y, pa.z, cons ? pl * (m * m / (rhoa * rhoa)) : pl / (rhoa * rhoa));
+    if (cons) D.f2[i] = m * cself;   // sum_b m_a d_ba.gradW_ab: the p_a term of the Jacobi update
     if (!WARM) return;
     float4* f = reinterpret_cast<float4*>(feat) + 2 * (size_t)i;
     f[0] = make_float4((rhoa / rho0 - 1.0f) * 100.0f, (radv / rho0 - 1.0f) * 100.0f,
@@ -1135,7 +1145,8 @@ __global__ __launch_bounds__(256) void k_ii_warm_apply(Dev D, const float* __res
         pl = D.learn_scale * fminf(pm, fmaxf(D.ksum[i], pj));
     }
     D.kr[i] = pl;
-    D.xk[i].w = pl / (rhoa * rhoa);
+    const float m = D.mass[ph];
+    D.xk[i].w = (D.mpCons && D.nphases > 1) ? pl * (m * m / (rhoa * rhoa)) : pl / (rhoa * rhoa);
 }
 
 // converged pressure in the model's units, as the change from feature 3
@@ -1177,7 +1188,9 @@ __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
         each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, fluid);
     }
     float dt = D.dt[s.scene];
-    float cf = -m * dt * dt;
+    // the records carry p/rho^2, or m^2 p/rho^2 with the number-density
+    // force between phases (whose d_ij is dt^2 m_j^2 / (m_i rho_j^2) gradW)
+    float cf = (D.mpCons && D.nphases > 1) ? -dt * dt / m : -m * dt * dt;
     float4 dii = D.f4a[i];
     float pl = D.kr[i];
     D.f4b[i] = make_float4(dii.x * pl + cf * sx, dii.y * pl + cf * sy, dii.z * pl + cf * sz, 0.0f);
@@ -1214,14 +1227,19 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
         // own sum_j d_ij p_j
         float sax = ua.x - dia.x * pla, say = ua.y - dia.y * pla, saz = ua.z - dia.z * pla;
         float ff = 0.f, fb = 0.f;
+        // with the number-density force between phases d_ji depends on the
+        // neighbour's mass; its part sum_b m_a d_ba.gradW_ab p_a is then
+        // taken whole from k_ii_aii (f2) instead of per pair
+        const bool cons = D.mpCons && D.nphases > 1;
+        const float cj = cons ? 0.0f : cji;
         auto fluid = [&](float bx, float by, float bz, float ux, float uy, float uz) {
             float dx = pa.x - bx, dy = pa.y - by, dz = pa.z - bz;
             float g = kernel_gradfac_sq(dot3f(dx, dy, dz, dx, dy, dz), D);
             float gx = dx * g, gy = dy * g, gz = dz * g;
             // (sum d_ij p_j|a - d_jj p_j) - (sum d_jk p_k|b - d_ji p_a)
-            float tx = sax - ux + cji * gx * pla;
-            float ty = say - uy + cji * gy * pla;
-            float tz = saz - uz + cji * gz * pla;
+            float tx = sax - ux + cj * gx * pla;
+            float ty = say - uy + cj * gy * pla;
+            float tz = saz - uz + cj * gz * pla;
             ff += dot3f(tx, ty, tz, gx, gy, gz);
         };
         if (nw) {
@@ -1238,6 +1256,7 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
             fb += (sax * dx + say * dy + saz * dz) * g;
         });
         float fin = m * ff + rho0 * fb;
+        if (cons) fin += D.f2[i] * pla;
         float aii = D.alpha[i], radv = D.kr0[i];
         float t = (1.0f - D.omega) * pla + (D.omega / aii) * (rho0 - radv - fin);
         if (t < 0.0f) {
@@ -1249,7 +1268,7 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
         // sum_j d_ij p_j of its neighbours, never their p_l
         D.f1[i] = t;
         D.kr[i] = t;
-        D.xk[i].w = t / (rhoa * rhoa);
+        D.xk[i].w = cons ? t * (m * m / (rhoa * rhoa)) : t / (rhoa * rhoa);
     }
     write_partial(D, s.tile, 0, contrib, lds);
 }
@@ -1276,6 +1295,7 @@ __global__ __launch_bounds__(256) void k_ii_force(Dev D, int gate) {
     float pa_r2 = pra / (rhoa * rhoa);
     const float Hcs2 = 2.0f * D.Hs * D.cs;
     const bool multi = D.nphases > 1;
+    const bool cons = D.mpCons && multi;   // neighbour records carry m^2 p/rho^2
     float ax = 0.f, ay = 0.f, az = 0.f;
     int c = D.cnt[i];
     each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, [&](int j, float4 pb) {           // (x, p/rho^2)
@@ -1285,11 +1305,11 @@ __global__ __launch_bounds__(256) void k_ii_force(Dev D, int gate) {
         int phb = multi ? phase_of(D.x[j]) : ph;
         float acce;
         if (phb == ph) {
-            acce = g * (-m * (pa_r2 + pb.w));
+            acce = g * (-m * (pa_r2 + (cons ? pb.w / (m * m) : pb.w)));
         } else {
             float4 va = D.v[i], vb = D.v[j];
             float mb = D.mass[phb], al = 0.5f * (alpha + D.visc[phb]);
-            acce = g * (-1.0f / m * (pa_r2 * m * m + pb.w * mb * mb));
+            acce = g * (-1.0f / m * (pa_r2 * m * m + (cons ? pb.w : pb.w * mb * mb)));
             float pro = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
             float nuv = al * Hcs2 / (rhoa + vb.w);
             float pi = pro < 0.0f ? -nuv * pro * fast_rcp(r2 + D.visc_eps) : 0.0f;

@@ -79,6 +79,8 @@ struct Dev {
     float dt_max;
     float warm_scale;  // PREVIOUS warm start: scale of the previous stiffness
     float learn_scale; // LEARNED warm start: scale of the bounded prediction
+    int mpCons;        // IISPH with several phases: pressure solve consistent with the
+                       // number-density pressure force between phases (xk.w = m^2 p/rho^2)
     int tileMap;       // 1: contiguous tile runs per XCD, 0: round robin (autotuned)
     int nbStage;       // neighbour-list groups staged in LDS: 6 or 8 (autotuned)
     int listNT;        // 1: passes read list groups with streaming loads (autotuned)

@@ -64,6 +64,7 @@ class Params(C.Structure):
         ("adjust_density", C.c_int),
         ("fluid_capacity", C.c_int),
         ("learn_scale", C.c_float),
+        ("multiphase_consistent", C.c_int),
     ]
 
 

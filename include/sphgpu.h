@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SPHGPU_ABI_VERSION 3
+#define SPHGPU_ABI_VERSION 4
 #define SPHGPU_MAX_ITER 10000   /* largest accepted solver-loop cap */
 
 enum sphgpu_status {
@@ -101,6 +101,12 @@ typedef struct sphgpu_params {
     int   fluid_capacity;      /* fluid particles the handle can hold, counting later appends (emitters);
                                   0 = the particles added before init */
     float learn_scale;         /* LEARNED once trained: the guess is learn_scale x the bounded prediction */
+    int   multiphase_consistent; /* IISPH with several phases.  0 (default): IISph_M's pressure solve, which
+                                  counts every fluid neighbour with the particle's own phase mass
+                                  (IISph_M.cpp:262, 302, 329) while its pressure force between phases takes
+                                  the number-density form (IISph_M.cpp:526-546).  1: the solve's d_ij and
+                                  d_ji follow that force (d_ij = dt^2 m_j^2/(m_i rho_j^2) gradW), so the
+                                  solved pressure is the one the force applies; same phase: no change */
 } sphgpu_params;
 
 typedef struct sphgpu_stats {

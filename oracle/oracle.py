@@ -47,7 +47,8 @@ class OrcParams(C.Structure):
         ("warm_scale", C.c_float),
         ("boundary_density", C.c_float),
         ("adjust_density", C.c_int),
-        ("reserved", C.c_int * 3),
+        ("multiphase_consistent", C.c_int),
+        ("reserved", C.c_int * 2),
     ]
 
 
@@ -162,6 +163,7 @@ def params_from(d: dict) -> OrcParams:
                     gamma=7.0, nu=1e-3, cfl=0.0, dt_min=1e-6, density_tol=1e-4, divergence_tol=2.5,
                     min_iter_density=2, max_iter_density=100, min_iter_div=1, max_iter_div=30,
                     omega=0.5, warm_start=0, warm_scale=0.7, boundary_density=1000.0, adjust_density=0,
+                    multiphase_consistent=0,
                     gravity=(0.0, -9.8, 0.0),
                     space_min=(0.0, 0.0, 0.0), space_max=(1.0, 1.0, 1.0))
     defaults.update({k: v for k, v in d.items() if k in defaults})

@@ -26,6 +26,12 @@
  *     force on dynamic solids (2 p_b/rho_b^2 at IISph_M.cpp:596);
  *   - IISPH phases are solved jointly (the reference iterates phase by phase
  *     with stale cross-phase pressures, IISph_M.cpp:85);
+ *   - IISPH with several phases, only with multiphase_consistent = 1 (off
+ *     by default): the solve's d_ij p_j and d_ji terms between phases take
+ *     the masses of the number-density pressure force the reference
+ *     applies (IISph_M.cpp:526-546: m_j^2 / m_i and m_i^2 / m_j) instead of
+ *     the particle's own mass (IISph_M.cpp:262, 302, 329), so the solved
+ *     pressure is the one the force applies; within a phase nothing changes;
  *   - IISPH convergence error: the mean over all particles of
  *     (rho_adv + a_ii p + sum_j ...)/rho0 - 1 where p is not clamped, and 0
  *     where it is.  The reference sums the predicted density of the
@@ -1025,7 +1031,10 @@ public:
                         const FluidPart<T>& b = fp(nb.pidx);
                         V3<T> g = (a.x - b.x) * (-Wg(nb.dis) / nb.dis);
                         t1 += g.dot(a.v_adv - b.v_adv) * fm0 * dt;
-                        V3<T> dji = g * (fm0 * dt * dt / (a.rho * a.rho));
+                        /* d_ji (IISph_M.cpp:262: m_i); consistent: m_i^2 / m_j */
+                        T fmb = v0 * fluids[nb.pidx.toFluidI()].rho0;
+                        T cm = P.multiphase_consistent ? fm0 * fm0 / fmb : fm0;
+                        V3<T> dji = g * (cm * dt * dt / (a.rho * a.rho));
                         t2 += (a.dii - dji).dot(g) * fm0;
                     } else {
                         const BoundPart<T>& b = bp(nb.pidx);
@@ -1058,10 +1067,12 @@ public:
                         const auto& nb = L.neigs[j];
                         if (nb.dis == 0 || !nb.pidx.isFluid()) continue;
                         const FluidPart<T>& b = fp(nb.pidx);
-                        T fmb = v0 * fluids[nb.pidx.toFluidI()].rho0;
-                        (void)fmb;
+                        /* d_ij p_j: IISph_M.cpp:302 takes the particle's own mass m_i; with
+                         * multiphase_consistent the number-density force's m_j^2 / m_i */
+                        T fmb = v0 * fluids[nb.pidx.toFluidI()].rho0, fma = v0 * fluids[k].rho0;
+                        T cm = P.multiphase_consistent ? fmb * fmb / fma : fma;
                         V3<T> g = (a.x - b.x) * (-Wg(nb.dis) / nb.dis);
-                        t1 += g * (-(v0 * fluids[k].rho0 * b.p_l) / (b.rho * b.rho));
+                        t1 += g * (-(cm * b.p_l) / (b.rho * b.rho));
                     }
                     a.sum_dijpj = t1 * (dt * dt);
                 }
@@ -1086,7 +1097,9 @@ public:
                         if (nb.pidx.isFluid()) {
                             const FluidPart<T>& b = fp(nb.pidx);
                             V3<T> g = (a.x - b.x) * (-Wg(nb.dis) / nb.dis);
-                            V3<T> dji = g * (dt * dt * fm0 / (a.rho * a.rho));
+                            T fmb = v0 * fluids[nb.pidx.toFluidI()].rho0;
+                            T cm = P.multiphase_consistent ? fm0 * fm0 / fmb : fm0;   /* IISph_M.cpp:329 */
+                            V3<T> dji = g * (dt * dt * cm / (a.rho * a.rho));
                             V3<T> t1 = (a.sum_dijpj - b.dii * b.p_l) - (b.sum_dijpj - dji * a.p_l);
                             fin += fm0 * t1.dot(g);
                         } else {

@@ -45,7 +45,9 @@ typedef struct orc_params {
     float warm_scale;          /* scale of the previous step's stiffness in the guess */
     float boundary_density;    /* m_TH.boundaryDensity (SphBase.h:576), IISph DF_* solvers */
     int   adjust_density;      /* m_TH.adjustDensity (SphBase.h:575), IISph DF_* solvers */
-    int   reserved[3];
+    int   multiphase_consistent; /* IISPH, several phases: 1 = the solve's d_ij, d_ji follow the
+                                  number-density force between phases (m_j^2/m_i instead of m_i) */
+    int   reserved[2];
 } orc_params;
 
 typedef struct orc_stats {

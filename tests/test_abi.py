@@ -30,12 +30,12 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header_sizes():
     # 4 (solver) + 2*4 + 6*4 + 4 + 3*4 + 6*4 + 2*4 + 4*4 + 4 + 4 + 4 + 8*4 + 3*4
-    assert C.sizeof(sg.Params) == 152
+    assert C.sizeof(sg.Params) == 156
     assert C.sizeof(sg.Stats) % 8 == 0
 
 
 def test_abi_version():
-    assert sg.load().sphgpu_abi_version() == 3
+    assert sg.load().sphgpu_abi_version() == 4
 
 
 def test_default_params_mirror_reference_defaults():
@@ -53,7 +53,7 @@ def test_default_params_mirror_reference_defaults():
 
 @pytest.mark.parametrize("bad", [dict(solver=7), dict(spacing_r=0.0), dict(dt=-1.0),
                                  dict(num_scenes=0), dict(warm_start=9), dict(use_graph=2),
-                                 dict(warm_scale=-0.5), dict(learn_scale=5.0),
+                                 dict(warm_scale=-0.5), dict(learn_scale=5.0), dict(multiphase_consistent=2),
                                  # solver loops must end at a finite cap
                                  dict(max_iter_density=0), dict(max_iter_density=-3),
                                  dict(max_iter_div=-1), dict(max_iter_density=10 ** 6),

@@ -192,6 +192,85 @@ def test_iisph_solves_the_density_jacobian_system():
     assert r[~on].max() <= 1e-7 * RHO0
 
 
+def _two_phase_oracle(p, xs, vs, rhos):
+    o = orc.Oracle(p, 64)
+    ks = [o.add_fluid(x.astype(np.float32), r, 0.0) for x, r in zip(xs, rhos)]
+    o.init()
+    for k, x, v in zip(ks, xs, vs):
+        o.set_state(x, v, k)
+    return o, ks
+
+
+def _density_jacobian_phases(o, xs, ks, eps=1e-7):
+    """J over the particles of every fluid, in the order of xs."""
+    x = np.concatenate(xs)
+    sizes = [len(t) for t in xs]
+
+    def put(xx):
+        off = 0
+        for k, m in zip(ks, sizes):
+            o.set_state(xx[off:off + m], None, k)
+            off += m
+
+    def rho():
+        o.compute_density()
+        return np.concatenate([o.get(orc.F_RHO, k) for k in ks])
+
+    n = len(x)
+    J = np.zeros((n, n, 3))
+    for k in range(n):
+        for a in range(3):
+            xp = x.copy()
+            xp[k, a] += eps
+            put(xp)
+            rp = rho()
+            xm = x.copy()
+            xm[k, a] -= eps
+            put(xm)
+            J[:, k, a] = (rp - rho()) / (2 * eps)
+    put(x)
+    o.compute_density()
+    return J
+
+
+@pytest.mark.parametrize("consistent", [1, 0])
+def test_iisph_two_phase_solve_against_the_pressure_force(consistent):
+    """Two phases, 3000 kg/m^3 below 1000 kg/m^3.  Densities count every
+    neighbour with the particle's own mass (rho_i = m_i n_i, IISph_M.cpp:101-112),
+    so J_ik = m_i dn_i/dx_k; the number-density pressure force between phases
+    (IISph_M.cpp:526-546, and within a phase, 478-488) is
+    a_k = -(1/m_k) sum_l p_l (m_l/rho_l^2) J_lk, so the density it produces is
+    rho_adv + A p with A_il = -dt^2 (sum_k J_ik J_lk / m_k) m_l / rho_l^2.  With
+    multiphase_consistent = 1 the converged pressure satisfies
+    rho_adv + A p = rho0 where p > 0; the reference's own-mass solve
+    (IISph_M.cpp:262, 302, 329) leaves residuals of percents at the interface."""
+    x = sg.lattice((4, 4, 4), (0.0, 0.0, 0.0), 0.95 * S, 0.01, 7, 0).astype(np.float64)
+    heavy = x[:, 1] < x[:, 1].mean()
+    xs, rhos = [x[heavy], x[~heavy]], [3.0 * RHO0, RHO0]
+    rng = np.random.default_rng(4)
+    vs = [rng.normal(0.0, 0.05, t.shape) for t in xs]
+    p = _params(orc.IISPH, density_tol=1e-15, min_iter_density=3000, max_iter_density=3000, omega=0.5,
+                multiphase_consistent=consistent)
+    a, ks = _two_phase_oracle(p, xs, vs, rhos)
+    st = a.step()
+    dt = float(st["dt"])
+    pr, radv, rho = (np.concatenate([a.get(f, k) for k in ks]) for f in (orc.F_PRESSURE, orc.F_RHO_ADV, orc.F_RHO))
+    m = np.concatenate([np.full(len(t), r * S ** 3) for t, r in zip(xs, rhos)])
+    rho0 = np.concatenate([np.full(len(t), r) for t, r in zip(xs, rhos)])
+    b, kb = _two_phase_oracle(p, xs, vs, rhos)
+    J = _density_jacobian_phases(b, xs, kb)
+    np.testing.assert_allclose(radv, rho + dt * np.einsum("ika,ka->i", J, np.concatenate(vs)), rtol=1e-9)
+    A = -dt * dt * np.einsum("ika,lka,k->il", J, J, 1.0 / m) * (m / rho ** 2)[None, :]
+    r = (radv + A @ pr - rho0) / rho0
+    on = pr > 0
+    assert 5 <= on.sum() <= len(pr) - 5
+    if consistent:
+        assert np.abs(r[on]).max() < 1e-7
+        assert r[~on].max() <= 1e-7
+    else:
+        assert np.abs(r[on]).max() > 1e-2
+
+
 # -------------------------------------------------------------- viscosity ----
 def _a_visc(solver, x, v, **kw):
     """Non-pressure acceleration minus gravity of a free block (no walls)."""
