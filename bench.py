@@ -450,7 +450,9 @@ def main() -> int:
         # same workload, same steps: the plain solver (the reference) and, for
         # the learned guess, the previous-stiffness guess it competes with
         ke = np.array([s["kinetic_energy"] for s in run["warm_stats"] + run["stats"]])
-        others = [sg.WARM_NONE] + ([sg.WARM_PREVIOUS] if warm == sg.WARM_LEARNED else [])
+        # (IISPH has no previous-stiffness guess of its own: its plain start
+        # is the reference's 0.5 p(t-dt))
+        others = [sg.WARM_NONE] + ([sg.WARM_PREVIOUS] if warm == sg.WARM_LEARNED and solver == sg.DFSPH else [])
         for w in others:
             cap = args.cpu_from_step if (w == sg.WARM_NONE and want_cpu and not sc.rigids
                                          and args.cpu_from_step <= train + args.warmup) else None

@@ -100,7 +100,8 @@ typedef struct sphgpu_params {
     int   adjust_density;      /* m_TH.adjustDensity (SphBase.h:575, 0): IISph DF_* number density */
     int   fluid_capacity;      /* fluid particles the handle can hold, counting later appends (emitters);
                                   0 = the particles added before init */
-    float learn_scale;         /* LEARNED once trained: the guess is learn_scale x the bounded prediction */
+    float learn_scale;         /* LEARNED once trained, DFSPH divergence solve: the guess is learn_scale x
+                                  the bounded prediction (the IISPH pressure start takes it whole) */
     int   multiphase_consistent; /* IISPH with several phases.  0 (default): IISph_M's pressure solve, which
                                   counts every fluid neighbour with the particle's own phase mass
                                   (IISph_M.cpp:262, 302, 329) while its pressure force between phases takes

@@ -140,3 +140,32 @@ def test_learned_iisph_pressure_start(gpu):
     print(f"IISPH iterations: reference start {ir:.2f}, learned start {il:.2f}")
     assert il <= ir + 0.5
     assert np.all(np.isfinite(params)) and np.abs(params).max() > 0
+
+
+def test_learned_iisph_start_8m_two_phase(gpu):
+    """The 7,993,056-particle two-phase tank with 20 bodies: after 40
+    training steps the learned start of the pressure solve needs fewer
+    Jacobi iterations per step (steps 60-160) than the reference's
+    0.5 p(t-dt), at the same tolerance, with no step at its cap and no
+    particle lost."""
+    sc = scenes.iisph_c5()
+    out = {}
+    for warm in (sg.WARM_NONE, sg.WARM_LEARNED):
+        print(f"[learned] {sc.name} warm start {warm}, 160 steps", flush=True)
+        sim = scenes.make_simulation(sc, warm_start=warm, learn_steps=LEARN_STEPS)
+        world = scenes.rigid_world(sc)
+
+        def upd(s):
+            world.step(s.step_dt(), s.solid_force_torque)
+            world.apply(s.set_solid_pose)
+
+        sim.set_update_solids(upd)
+        st = [sim.step() for _ in range(160)]
+        sim.close()
+        assert all(s["removed"] == 0 and s["nan_count"] == 0 for s in st)
+        assert all(s["max_iter_hit"] == 0 for s in st[60:])
+        assert all(s["density_err"] <= sc.params["density_tol"] + 1e-9 for s in st[60:])
+        out[warm] = np.mean([s["iter_density"] for s in st[60:]])
+    print(f"IISPH iterations, 8M two phases: reference start {out[sg.WARM_NONE]:.2f}, "
+          f"learned {out[sg.WARM_LEARNED]:.2f}")
+    assert out[sg.WARM_LEARNED] < 0.95 * out[sg.WARM_NONE]
