@@ -834,16 +834,17 @@ __device__ __forceinline__ float density_change_win(const Dev& D, const Slot& s,
 }
 
 // LDS of the two-record window (WIN_MAX records of 8 + 16 bytes)
-#define WIN_XV_DECL                              \
-    __shared__ float2 s_wxy[WIN_MAX];            \
-    __shared__ float4 s_wzv[WIN_MAX];            \
+#define WIN_XV_DECL(N)                           \
+    __shared__ float2 s_wxy[N];                  \
+    __shared__ float4 s_wzv[N];                  \
     const WinXV win{s_wxy, s_wzv};
 
 // pass A of the density loop (DFSPH.cpp:572-643): rho*, clamp rho* >= rho0,
 // kappa = (rho*-rho0)/dt^2 alpha -> xk.w = kappa/rho; partial sum of rho*/rho0.
+template <bool WIN>
 __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
     __shared__ double lds[4];
-    WIN_XV_DECL
+    WIN_XV_DECL(WIN ? WIN_MAX : 1)
     Slot s = slot_nb(D, true);
     if (!D.active[s.scene]) {   // uniform: a converged scene's tiles contribute nothing
         write_partial(D, s.tile, 0, 0.0, lds);
@@ -852,8 +853,8 @@ __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
     // own records and list first: their loads overlap the window staging
     const int c = s.live ? D.cnt[s.i] : 0;
     List16 L;
-    list16_prefetch(D, s, c, L, WIN_DENSITY);
-    const int nw = win_layout(D, s, WIN_DENSITY);
+    if (WIN) list16_prefetch(D, s, c, L, WIN_DENSITY);
+    const int nw = WIN ? win_layout(D, s, WIN_DENSITY) : 0;
     if (nw) stage_xv(D, nw, win);
     double contrib = 0.0;
     if (s.live) {
@@ -883,9 +884,10 @@ __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
 // the mean applied |increment| / alpha.  Later iterations are the
 // reference's (projecting every iteration is unstable: signed Jacobi
 // oscillates).
+template <bool WIN>
 __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected) {
     __shared__ double lds[4];
-    WIN_XV_DECL
+    WIN_XV_DECL(WIN ? WIN_MAX : 1)
     Slot s = slot_nb(D, true);
     if (!D.active[s.scene]) {   // uniform: a converged scene's tiles contribute nothing
         write_partial(D, s.tile, 0, 0.0, lds);
@@ -894,8 +896,8 @@ __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected
     // own records and list first: their loads overlap the window staging
     const int c = s.live ? D.cnt[s.i] : 0;
     List16 L;
-    list16_prefetch(D, s, c, L, WIN_DIVERGENCE);
-    const int nw = win_layout(D, s, WIN_DIVERGENCE);
+    if (WIN) list16_prefetch(D, s, c, L, WIN_DIVERGENCE);
+    const int nw = WIN ? win_layout(D, s, WIN_DIVERGENCE) : 0;
     if (nw) stage_xv(D, nw, win);
     double contrib = 0.0;
     if (s.live) {
@@ -930,14 +932,15 @@ __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected
 
 // pass B of both loops (DFSPH.cpp:647-693, 840-883):
 // v* -= dt (m sum (kr_a + kr_b) gradW + rho0 sum V_b kr_a gradW), kr = xk.w
+template <bool WIN>
 __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
-    __shared__ float4 s_wxk[WIN_MAX];   // window records: xk of each neighbour
+    __shared__ float4 s_wxk[WIN ? WIN_MAX : 1];   // window records: xk of each neighbour
     Slot s = slot_nb(D, true);
     if (!D.active[s.scene]) return;     // uniform per block
     const int c = s.live ? D.cnt[s.i] : 0;
     List16 L;
-    list16_prefetch(D, s, c, L, WIN_VELOCITY);
-    const int nw = win_layout(D, s, WIN_VELOCITY);
+    if (WIN) list16_prefetch(D, s, c, L, WIN_VELOCITY);
+    const int nw = WIN ? win_layout(D, s, WIN_VELOCITY) : 0;
     if (nw) win_stage(nw, [&](int j) { return D.xk[j]; }, [&](int k, float4 r) { s_wxk[k] = r; });
     if (!s.live) return;
     const int i = s.i;
@@ -1164,14 +1167,15 @@ __global__ __launch_bounds__(256) void k_ii_targets(Dev D, const float* __restri
 // sum_j d_ij p_j = dt^2 sum_f -m p_j/rho_j^2 gradW (IISph_M.cpp:289-309);
 // stores u = d_ii p_l + sum_j d_ij p_j, the per-particle part of the
 // Jacobi update that neighbours read.
+template <bool WIN>
 __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
-    __shared__ float4 s_wxk[WIN_MAX];   // window records: xk of each neighbour
+    __shared__ float4 s_wxk[WIN ? WIN_MAX : 1];   // window records: xk of each neighbour
     Slot s = slot_nb(D, true);
     if (!D.active[s.scene]) return;     // uniform per block
     const int c = s.live ? D.cnt[s.i] : 0;
     List16 L;
-    list16_prefetch(D, s, c, L, WIN_SUMDIJ);
-    const int nw = win_layout(D, s, WIN_SUMDIJ);
+    if (WIN) list16_prefetch(D, s, c, L, WIN_SUMDIJ);
+    const int nw = WIN ? win_layout(D, s, WIN_SUMDIJ) : 0;
     if (nw) win_stage(nw, [&](int j) { return D.xk[j]; }, [&](int k, float4 r) { s_wxk[k] = r; });
     if (!s.live) return;
     const int i = s.i;
@@ -1201,9 +1205,10 @@ __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
 
 // relaxed Jacobi update (IISph_M.cpp:312-351) into f1; partial sum of the
 // relative density error of the particles that are not clamped.
+template <bool WIN>
 __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
     __shared__ double lds[4];
-    WIN_XV_DECL
+    WIN_XV_DECL(WIN ? WIN_MAX : 1)
     Slot s = slot_nb(D, true);
     if (!D.active[s.scene]) {   // uniform: a converged scene's tiles contribute nothing
         write_partial(D, s.tile, 0, 0.0, lds);
@@ -1211,9 +1216,9 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
     }
     const int c = s.live ? D.cnt[s.i] : 0;
     List16 L;
-    list16_prefetch(D, s, c, L, WIN_PRESSURE);
+    if (WIN) list16_prefetch(D, s, c, L, WIN_PRESSURE);
     // window records: x and u = d_ii p_l + sum_j d_ij p_j of each neighbour
-    const int nw = win_layout(D, s, WIN_PRESSURE);
+    const int nw = WIN ? win_layout(D, s, WIN_PRESSURE) : 0;
     if (nw) win_stage(nw, [&](int j) { return Rec2{D.x[j], D.f4b[j]}; }, [&](int k, Rec2 r) {
         win.xy[k] = make_float2(r.a.x, r.a.y);
         win.zv[k] = make_float4(r.a.z, r.b.x, r.b.y, r.b.z);
@@ -1611,11 +1616,21 @@ void launch_mlp_gather(const Dev& D, const float* feat, const float* tgt, float*
     if (n > 0)
         hipLaunchKernelGGL(k_mlp_gather, dim3((n + 255) / 256), dim3(256), 0, st, D, feat, tgt, bfeat, btgt, n, seed);
 }
-void launch_df_pass_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_pass_density); }
-void launch_df_pass_divergence(const Dev& D, int projected, hipStream_t st) {
-    SPH_LAUNCH(k_df_pass_divergence, projected);
+// Each window pass has two builds: with the LDS window (48-72 KB of LDS per
+// block) and without (per-lane gathers only: no window LDS and fewer
+// registers, so more waves per SIMD); Dev::winMask picks per pass.
+void launch_df_pass_density(const Dev& D, hipStream_t st) {
+    if (D.winMask & WIN_DENSITY) SPH_LAUNCH(k_df_pass_density<true>);
+    else SPH_LAUNCH(k_df_pass_density<false>);
 }
-void launch_df_pass_velocity(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_pass_velocity); }
+void launch_df_pass_divergence(const Dev& D, int projected, hipStream_t st) {
+    if (D.winMask & WIN_DIVERGENCE) SPH_LAUNCH(k_df_pass_divergence<true>, projected);
+    else SPH_LAUNCH(k_df_pass_divergence<false>, projected);
+}
+void launch_df_pass_velocity(const Dev& D, hipStream_t st) {
+    if (D.winMask & WIN_VELOCITY) SPH_LAUNCH(k_df_pass_velocity<true>);
+    else SPH_LAUNCH(k_df_pass_velocity<false>);
+}
 void launch_df_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_integrate); }
 void launch_ii_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_density); }
 void launch_ii_aadv_dii(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_aadv_dii); }
@@ -1627,8 +1642,14 @@ void launch_ii_warm_apply(const Dev& D, const float* y, const float* feat, hipSt
     SPH_LAUNCH(k_ii_warm_apply, y, feat);
 }
 void launch_ii_targets(const Dev& D, const float* feat, float* tgt, hipStream_t st) { SPH_LAUNCH(k_ii_targets, feat, tgt); }
-void launch_ii_sumdij(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_sumdij); }
-void launch_ii_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_pressure); }
+void launch_ii_sumdij(const Dev& D, hipStream_t st) {
+    if (D.winMask & WIN_SUMDIJ) SPH_LAUNCH(k_ii_sumdij<true>);
+    else SPH_LAUNCH(k_ii_sumdij<false>);
+}
+void launch_ii_pressure(const Dev& D, hipStream_t st) {
+    if (D.winMask & WIN_PRESSURE) SPH_LAUNCH(k_ii_pressure<true>);
+    else SPH_LAUNCH(k_ii_pressure<false>);
+}
 void launch_ii_force_integrate(const Dev& D, int gate, hipStream_t st) { SPH_LAUNCH(k_ii_force, gate); }
 void launch_pci_rest_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_pci_rest_density); }
 void launch_pci_aext(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_pci_aext); }
