@@ -480,16 +480,29 @@ int tune_windows(sphgpu_handle* h) {
                     hipGetErrorString(e));
     }
     launch_fill_i32(D.active, 1, D.nscenes, h->st);
+    // the gather build at its register-limited occupancy and capped at 6, 5,
+    // 4 and 3 blocks per CU (unused dynamic LDS of 160 KB / k less a margin
+    // for the kernel's own few hundred bytes)
+    constexpr int kLdsPerCu = 160 * 1024;
+    const int caps[5] = {0, kLdsPerCu / 6 - 1024, kLdsPerCu / 5 - 1024, kLdsPerCu / 4 - 1024,
+                         kLdsPerCu / 3 - 1024};
     int mask = WIN_ALL;
     for (const auto& p : passes) {
         const int k = __builtin_ctz(p.first);
-        float t[2];
-        for (int on = 0; on < 2; ++on) {
-            D.winMask = on ? WIN_ALL : 0;
-            t[on] = time_launches(h, p.second, nullptr, 2);
-            h->winMs[2 * k + on] = t[on];
+        D.winMask = WIN_ALL;
+        const float tw = time_launches(h, p.second, nullptr, 2);
+        float tg = 1e30f;
+        int best = 0;
+        D.winMask = 0;
+        for (int c : caps) {
+            D.gatherLds[k] = c;
+            const float t = time_launches(h, p.second, nullptr, 2);
+            if (t < tg) tg = t, best = c;
         }
-        if (t[0] < t[1]) mask &= ~p.first;
+        D.gatherLds[k] = best;
+        h->winMs[2 * k] = tg;
+        h->winMs[2 * k + 1] = tw;
+        if (tg < tw) mask &= ~p.first;
     }
     D.winMask = mask;
     off = 0;
@@ -1684,6 +1697,12 @@ int sphgpu_step(sphgpu_handle* h, sphgpu_stats* st) {
     if (rc) return rc;
     if (h->nLive == 0 && h->frame > 0) return fail(h, SPHGPU_E_STATE, "no fluid particles left");
     if (!h->tuned && (rc = autotune(h))) return rc;
+    // the window and occupancy choices are timed again on the flow itself,
+    // after ten steps and every 200 after that (the initial lattice is not
+    // the state the passes spend their time on); every choice gives the same
+    // bits, so a run does not depend on the timings
+    if (h->winFix < 0 && h->frame > 0 && (h->frame == 10 || h->frame % 200 == 0) && (rc = tune_windows(h)))
+        return rc;
     for (auto& s : h->sceneStats) s = sphgpu_stats{};
     switch (h->P.solver) {
         case SPHGPU_WCSPH: return step_wcsph(h, st);
@@ -2090,11 +2109,16 @@ int sphgpu_set_lds_windows(sphgpu_handle* h, int on) {
     return SPHGPU_OK;
 }
 
-int sphgpu_get_lds_windows(const sphgpu_handle* h, int* mask, float* ms) {
+int sphgpu_get_lds_windows(const sphgpu_handle* h, int* mask, float* ms, int* blocks_per_cu) {
     if (!h || !mask) return SPHGPU_E_ARG;
     *mask = h->D.winMask;
     if (ms)
         for (int k = 0; k < 10; ++k) ms[k] = h->winMs[k];
+    if (blocks_per_cu)
+        for (int k = 0; k < 5; ++k) {
+            const int l = h->D.gatherLds[k];
+            blocks_per_cu[k] = (h->D.winMask >> k & 1) || l == 0 ? 0 : (160 * 1024) / (l + 1024);
+        }
     return SPHGPU_OK;
 }
 

@@ -1584,19 +1584,23 @@ void launch_fill_i32(int* p, int v, int n, hipStream_t st) {
 
 thread_local LaunchTiming g_launch_timing;
 
-#define SPH_LAUNCH(kern, ...)                                                                          \
+// One block of 256 threads per tile; `lds` bytes of dynamic LDS that the
+// kernel does not use cap how many blocks share a CU (the gather builds of
+// the solver-loop passes run fastest below their register-limited occupancy)
+#define SPH_LAUNCH_L(kern, lds, ...)                                                                    \
     do {                                                                                               \
         if (tiles_of(D) > 0) {                                                                         \
             const LaunchTiming t_ = g_launch_timing;                                                   \
             if (t_.start) {                                                                            \
                 g_launch_timing = LaunchTiming{};                                                      \
-                hipExtLaunchKernelGGL(kern, dim3(tiles_of(D)), dim3(256), 0, st, t_.start, t_.stop, 0, \
-                                      D, ##__VA_ARGS__);                                               \
+                hipExtLaunchKernelGGL(kern, dim3(tiles_of(D)), dim3(256), (lds), st, t_.start, t_.stop, \
+                                      0, D, ##__VA_ARGS__);                                            \
             } else {                                                                                   \
-                hipLaunchKernelGGL(kern, dim3(tiles_of(D)), dim3(256), 0, st, D, ##__VA_ARGS__);       \
+                hipLaunchKernelGGL(kern, dim3(tiles_of(D)), dim3(256), (lds), st, D, ##__VA_ARGS__);   \
             }                                                                                          \
         }                                                                                              \
     } while (0)
+#define SPH_LAUNCH(kern, ...) SPH_LAUNCH_L(kern, 0, ##__VA_ARGS__)
 
 void launch_wc_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_wc_pressure); }
 void launch_wc_force_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_wc_force); }
@@ -1618,18 +1622,19 @@ void launch_mlp_gather(const Dev& D, const float* feat, const float* tgt, float*
 }
 // Each window pass has two builds: with the LDS window (48-72 KB of LDS per
 // block) and without (per-lane gathers only: no window LDS and fewer
-// registers, so more waves per SIMD); Dev::winMask picks per pass.
+// registers, so more waves per SIMD); Dev::winMask picks per pass, and
+// Dev::gatherLds caps the gather build's blocks per CU.
 void launch_df_pass_density(const Dev& D, hipStream_t st) {
     if (D.winMask & WIN_DENSITY) SPH_LAUNCH(k_df_pass_density<true>);
-    else SPH_LAUNCH(k_df_pass_density<false>);
+    else SPH_LAUNCH_L(k_df_pass_density<false>, D.gatherLds[0]);
 }
 void launch_df_pass_divergence(const Dev& D, int projected, hipStream_t st) {
     if (D.winMask & WIN_DIVERGENCE) SPH_LAUNCH(k_df_pass_divergence<true>, projected);
-    else SPH_LAUNCH(k_df_pass_divergence<false>, projected);
+    else SPH_LAUNCH_L(k_df_pass_divergence<false>, D.gatherLds[1], projected);
 }
 void launch_df_pass_velocity(const Dev& D, hipStream_t st) {
     if (D.winMask & WIN_VELOCITY) SPH_LAUNCH(k_df_pass_velocity<true>);
-    else SPH_LAUNCH(k_df_pass_velocity<false>);
+    else SPH_LAUNCH_L(k_df_pass_velocity<false>, D.gatherLds[2]);
 }
 void launch_df_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_integrate); }
 void launch_ii_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_ii_density); }
@@ -1644,11 +1649,11 @@ void launch_ii_warm_apply(const Dev& D, const float* y, const float* feat, hipSt
 void launch_ii_targets(const Dev& D, const float* feat, float* tgt, hipStream_t st) { SPH_LAUNCH(k_ii_targets, feat, tgt); }
 void launch_ii_sumdij(const Dev& D, hipStream_t st) {
     if (D.winMask & WIN_SUMDIJ) SPH_LAUNCH(k_ii_sumdij<true>);
-    else SPH_LAUNCH(k_ii_sumdij<false>);
+    else SPH_LAUNCH_L(k_ii_sumdij<false>, D.gatherLds[3]);
 }
 void launch_ii_pressure(const Dev& D, hipStream_t st) {
     if (D.winMask & WIN_PRESSURE) SPH_LAUNCH(k_ii_pressure<true>);
-    else SPH_LAUNCH(k_ii_pressure<false>);
+    else SPH_LAUNCH_L(k_ii_pressure<false>, D.gatherLds[4]);
 }
 void launch_ii_force_integrate(const Dev& D, int gate, hipStream_t st) { SPH_LAUNCH(k_ii_force, gate); }
 void launch_pci_rest_density(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_pci_rest_density); }

@@ -86,6 +86,8 @@ struct Dev {
     int listNT;        // 1: passes read list groups with streaming loads (autotuned)
     int winMask;       // solver-loop passes (WIN_* bits) that stage packed tiles' neighbour
                        // windows in LDS; the others gather per lane (autotuned)
+    int gatherLds[5];  // per window pass (bit order): unused dynamic LDS bytes of its gather
+                       // build, capping its blocks per CU (0: register-limited; autotuned)
     float gx, gy, gz;
     float cs, gamma, nu, omega;
     float rho0[MAX_PHASES], mass[MAX_PHASES], visc[MAX_PHASES], tait_B[MAX_PHASES];

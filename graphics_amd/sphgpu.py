@@ -152,7 +152,7 @@ SIGNATURES = {
     "sphgpu_get_list_form": (C.c_int, [C.c_void_p, _IP, _IP, _IP]),
     "sphgpu_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "sphgpu_set_lds_windows": (C.c_int, [C.c_void_p, C.c_int]),
-    "sphgpu_get_lds_windows": (C.c_int, [C.c_void_p, _IP, _FP]),
+    "sphgpu_get_lds_windows": (C.c_int, [C.c_void_p, _IP, _FP, _IP]),
     "sphgpu_mlp_create": (C.c_int, [C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
     "sphgpu_mlp_destroy": (None, [C.c_void_p]),
     "sphgpu_mlp_set_params": (C.c_int, [C.c_void_p, _FP]),
@@ -645,11 +645,12 @@ class Simulation:
     def lds_windows(self) -> dict:
         """Passes with their LDS window on (WIN_* bits) and the autotuner's
         timings per pass: (gathers ms, window ms), None where not timed."""
-        mask, ms = C.c_int(), (C.c_float * 10)()
-        self._check(self.lib.sphgpu_get_lds_windows(self.h, C.byref(mask), ms), "get_lds_windows")
+        mask, ms, bl = C.c_int(), (C.c_float * 10)(), (C.c_int * 5)()
+        self._check(self.lib.sphgpu_get_lds_windows(self.h, C.byref(mask), ms, bl), "get_lds_windows")
         names = ("density", "divergence", "velocity", "sumdij", "pressure")
         return {"mask": mask.value,
-                "ms": {n: (ms[2 * k], ms[2 * k + 1]) if ms[2 * k] > 0 else None for k, n in enumerate(names)}}
+                "ms": {n: (ms[2 * k], ms[2 * k + 1]) if ms[2 * k] > 0 else None for k, n in enumerate(names)},
+                "blocks_per_cu": {n: bl[k] for k, n in enumerate(names)}}
 
     def list_form(self) -> dict:
         """Tiles holding particles, how many of them have packed (16-bit) lists,

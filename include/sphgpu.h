@@ -367,13 +367,18 @@ int sphgpu_set_tuning(sphgpu_handle* h, int tile_map, int stage_groups, int stre
 int sphgpu_set_lds_windows(sphgpu_handle* h, int on);
 /* The passes whose window is on: *mask is a sum of SPHGPU_WIN_* bits;
  * ms (10 floats, may be NULL) receives per pass, in bit order, the timed
- * ms per launch with gathers and with the window (0 where not timed). */
+ * ms per launch with gathers (the best occupancy tried) and with the
+ * window (0 where not timed); blocks_per_cu (5 ints, may be NULL) the
+ * blocks per CU each gathering pass is capped at (0: not capped, or the
+ * pass uses its window).  The gather build of a pass is timed at its
+ * register-limited occupancy and capped at 6, 5, 4 and 3 blocks per CU
+ * (by dynamic LDS it does not use); both builds give the same bits. */
 #define SPHGPU_WIN_DENSITY 1     /* DFSPH density pass (DFSPH.cpp:572-643) */
 #define SPHGPU_WIN_DIVERGENCE 2  /* DFSPH divergence pass (DFSPH.cpp:780-836) */
 #define SPHGPU_WIN_VELOCITY 4    /* DFSPH velocity update of both loops (DFSPH.cpp:647-693, 840-883) */
 #define SPHGPU_WIN_SUMDIJ 8      /* IISPH sum_j d_ij p_j (IISph_M.cpp:288-309) */
 #define SPHGPU_WIN_PRESSURE 16   /* IISPH relaxed Jacobi update (IISph_M.cpp:312-351) */
-int sphgpu_get_lds_windows(const sphgpu_handle* h, int* mask, float* ms);
+int sphgpu_get_lds_windows(const sphgpu_handle* h, int* mask, float* ms, int* blocks_per_cu);
 /* Form of the neighbour lists of the last step: *tiles = tiles holding
  * particles, *packed_tiles = those whose lists use 16-bit entries (the rest
  * use 32-bit entries; both hold the same neighbours in the same order);
