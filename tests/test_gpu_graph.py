@@ -79,6 +79,9 @@ def test_autotuner_choices_reported(gpu):
         gathers, window = w["ms"][name]
         assert gathers > 0 and window > 0
         assert bool(w["mask"] & bit) == (window <= gathers)
+        # a gathering pass runs at its register-limited occupancy or capped
+        # at 6, 5, 4 or 3 blocks per CU; a windowed pass reports no cap
+        assert w["blocks_per_cu"][name] in ((0,) if w["mask"] & bit else (0, 3, 4, 5, 6))
     assert w["ms"]["sumdij"] is None and w["ms"]["pressure"] is None
     sim.close()
 
