@@ -1426,6 +1426,13 @@ int sphgpu_init(sphgpu_handle* h) {
     D.sigma_h = 1.0f / float(M_PI) / (D.kh * D.kh * D.kh) / D.kh;
     D.W0 = D.sigma;
     {
+        const double sh = double(D.sigma_h), ik = double(D.inv_kh), c = std::sqrt(0.75 * sh);
+        D.gA = float(-3.0 * sh * ik);
+        D.gB = float(2.25 * sh * ik * ik);
+        D.gU = float(-c * ik);
+        D.gV = float(2.0 * c);
+    }
+    {
         volatile float hh = s + s;
         volatile float h2 = hh * hh;
         D.h2 = h2;
@@ -1506,8 +1513,9 @@ int sphgpu_init(sphgpu_handle* h) {
     ALLOC(D.rho, ncap); ALLOC(D.alpha, ncap); ALLOC(D.kr, ncap); ALLOC(D.kr0, ncap); ALLOC(D.f1, ncap); ALLOC(D.f2, ncap);
     ALLOC(D.nbF, (size_t)NBF_CAP * ncap); ALLOC(D.nbB, (size_t)NBB_CAP * ncap); ALLOC(D.cnt, ncap);
     ALLOC(D.nbF16, (size_t)(NBF_CAP / 8) * ncap); ALLOC(D.nbB16, (size_t)(NBB_CAP / 8) * ncap);
+    ALLOC(D.nbW16, (size_t)(NBF_CAP / 8) * ncap);
     ALLOC(D.nbTab, (size_t)NB16_TAB * (ncap / TILE));
-    ALLOC(D.nbLen, (size_t)32 * (ncap / TILE));
+    ALLOC(D.nbWin, (size_t)32 * (ncap / TILE));
     // boundary arrays carry one extra dummy particle at index nb (list padding)
     ALLOC(D.xb, nb + 1); ALLOC(D.vb, nb + 1); ALLOC(D.bid, nb + 1); ALLOC(D.bsolid, nb + 1);
     ALLOC(D.bscene, nb + 1); ALLOC(D.bkself, nb + 1); ALLOC(D.bkey, nb + 1);

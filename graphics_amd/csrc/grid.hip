@@ -151,6 +151,7 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     __shared__ int sRow[256];
     __shared__ int sWave[4];
     __shared__ int sOverflow;
+    __shared__ int sWin[33];   // window offset of each fluid slot - (slot << NB16_OFF_BITS); [32] = size
     const int tid = threadIdx.x;
     const int tile = tile_of_block(D.tileMap);
     const int i = tile * 256 + tid;
@@ -221,6 +222,22 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     if (tid < 64 && sBase[tid] != INT_MAX && sEnd[tid] - sBase[tid] > (1 << NB16_OFF_BITS)) sOverflow = 1;
     __syncthreads();
     packed = packed && sOverflow == 0;
+    // the tile's window: its fluid slot runs concatenated in slot order (the
+    // order in which the solver passes stage it); entry e of slot s is
+    // record e + sWin[s] of it
+    if (tid < 64) {
+        const int len = (packed && tid < 32 && tid != NB16_FLAG && sBase[tid] != INT_MAX) ? sEnd[tid] - sBase[tid] : 0;
+        int incl = len;
+        for (int off = 1; off < 32; off <<= 1) {
+            const int t = __shfl_up(incl, off, 64);
+            if (tid >= off) incl += t;
+        }
+        if (tid < 32) sWin[tid] = (incl - len) - (tid << NB16_OFF_BITS);
+        if (tid == 31) sWin[32] = incl;
+    }
+    __syncthreads();
+    const bool win = packed && sWin[32] <= WIN_MAX;
+    auto to_win = [&](unsigned e) { return (e + (unsigned)sWin[e >> NB16_OFF_BITS]) & 0xffffu; };
 
     auto put = [&](int* list, int k, int j) {
         if (k < NB_STAGE_GROUPS * 4) st[((k >> 2) * 256 + tid) * 4 + (k & 3)] = j;
@@ -230,8 +247,12 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     auto bias16 = [&](int slot) { return packed ? (slot << NB16_OFF_BITS) - sBase[slot] : 0; };
     auto put16 = [&](int4* list, int k, int bias, int j) {
         const uint16_t e = uint16_t(j + bias);
-        if (k < NB_STAGE_GROUPS * 8) st16[((k >> 3) * 256 + tid) * 8 + (k & 7)] = e;
-        else reinterpret_cast<uint16_t*>(list)[nb16_slot(k, i, stride)] = e;
+        if (k < NB_STAGE_GROUPS * 8) {
+            st16[((k >> 3) * 256 + tid) * 8 + (k & 7)] = e;
+        } else {
+            reinterpret_cast<uint16_t*>(list)[nb16_slot(k, i, stride)] = e;
+            if (win && list == D.nbF16) reinterpret_cast<uint16_t*>(D.nbW16)[nb16_slot(k, i, stride)] = uint16_t(to_win(e));
+        }
     };
     auto flush = [&](int* list, int count) {
         __syncthreads();
@@ -244,6 +265,18 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
         __syncthreads();
         const int ng = min((count + 7) >> 3, NB_STAGE_GROUPS);
         for (int g = 0; g < ng; ++g) out[(size_t)g * stride + i] = stage[g][tid];
+        if (win && out == D.nbF16) {
+            // the same groups as window indices (entries past the padded
+            // count are converted too, and never read)
+            for (int g = 0; g < ng; ++g) {
+                const int4 v = stage[g][tid];
+                const unsigned w[4] = {(unsigned)v.x, (unsigned)v.y, (unsigned)v.z, (unsigned)v.w};
+                int r[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) r[q] = (int)(to_win(w[q] & 0xffffu) | to_win(w[q] >> 16) << 16);
+                D.nbW16[(size_t)g * stride + i] = make_int4(r[0], r[1], r[2], r[3]);
+            }
+        }
         __syncthreads();
     };
     int nf = 0, nbn = 0, over = 0, zero = 0;
@@ -337,9 +370,7 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
         else if (tid == 32 + NB16_DUMMY) v = D.nb - (NB16_DUMMY << NB16_OFF_BITS);
         else v = sBase[tid] != INT_MAX ? sBase[tid] - (slot << NB16_OFF_BITS) : 0;
         D.nbTab[(size_t)tile * NB16_TAB + tid] = v;
-        if (tid < 32)
-            D.nbLen[(size_t)tile * 32 + tid] =
-                (packed && tid != NB16_FLAG && sBase[tid] != INT_MAX) ? sEnd[tid] - sBase[tid] : 0;
+        if (tid < 32) D.nbWin[(size_t)tile * 32 + tid] = packed ? sWin[tid] + (tid << NB16_OFF_BITS) : 0;
     }
     // stored fluid entries | fluid neighbours found << 8 | boundary << 16 | phase << 24
     if (i < D.npad) D.cnt[i] = cf | (act ? min(nf, 255) : 0) << 8 | (cb << 16) | ((act ? int(pa.w) : 0) << 24);

@@ -29,15 +29,20 @@ __device__ __forceinline__ Slot slot_of(const Dev& D, bool gated) {
     return s;
 }
 
-// The tile's list table (fluid slots 0..31, boundary 32..63), staged in LDS
-// by the kernels that read neighbour lists.
+// The tile's list table (fluid slots 0..31, boundary 32..63) and the window
+// offset of each fluid slot ([31]: the window size), staged in LDS by the
+// kernels that read neighbour lists.
 __shared__ int g_nbtab[NB16_TAB];
+__shared__ int g_winoff[32];
 
 // slot_of for kernels that read neighbour lists: every thread of the block
-// calls it before any returns (it stages the tile's table in LDS).
+// calls it before any returns (it stages the tile's tables in LDS, both with
+// one round trip).
 __device__ __forceinline__ Slot slot_nb(const Dev& D, bool gated) {
     Slot s = slot_of(D, gated);
     if (threadIdx.x < NB16_TAB) g_nbtab[threadIdx.x] = D.nbTab[(size_t)s.tile * NB16_TAB + threadIdx.x];
+    else if (threadIdx.x < NB16_TAB + 32)
+        g_winoff[threadIdx.x - NB16_TAB] = D.nbWin[(size_t)s.tile * 32 + threadIdx.x - NB16_TAB];
     __syncthreads();
     s.packed = __builtin_amdgcn_readfirstlane(g_nbtab[NB16_FLAG]) != 0;
     return s;
@@ -166,36 +171,16 @@ struct Rec2 {
 // tile's own 256 particles -- holds every record the tile's neighbour loops
 // read.  A pass stages the window in LDS with coalesced loads (each record
 // is needed by about 28 lanes of the tile) and the loops then read it from
-// LDS instead of gathering per lane through the L1.  A list entry e
-// (slot << NB16_OFF_BITS | offset) decodes to the window index
-// g_wintab[e >> NB16_OFF_BITS] + e.  Tiles with 32-bit lists, or whose window
-// exceeds WIN_MAX records, keep the per-lane gathers (the same sums in the
-// same order, so both forms give identical results).
-constexpr int WIN_MAX = 3072;
-__shared__ int g_wintab[32];   // window offset of each fluid slot - (slot << NB16_OFF_BITS)
-__shared__ int g_winoff[33];   // window offset of each fluid slot; [32] = window size
-
-// Window layout of the block's tile (every thread calls it, after slot_nb)
-// for the pass `bit`: returns the window size, or 0 when the tile's loops
-// gather from memory.
+// LDS instead of gathering per lane through the L1, taking the window index
+// of each neighbour from the window-index copy of the fluid lists (nbW16).
+// Tiles with 32-bit lists, or whose window exceeds WIN_MAX records, keep the
+// per-lane gathers (the same sums in the same order, so both forms give
+// identical results).
+// Window size of the block's tile (after slot_nb) for the pass `bit`, or 0
+// when the tile's loops gather from memory.
 __device__ __forceinline__ int win_layout(const Dev& D, const Slot& s, int bit) {
     if (!s.packed || !(D.winMask & bit)) return 0;   // uniform per block
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        const int len = lane < 32 ? D.nbLen[(size_t)s.tile * 32 + lane] : 0;
-        int incl = len;
-        for (int off = 1; off < 32; off <<= 1) {
-            const int t = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += t;
-        }
-        if (lane < 32) {
-            g_winoff[lane] = incl - len;
-            g_wintab[lane] = (incl - len) - (lane << NB16_OFF_BITS);
-        }
-        if (lane == 31) g_winoff[32] = incl;
-    }
-    __syncthreads();
-    const int n = g_winoff[32];
+    const int n = __builtin_amdgcn_readfirstlane(g_winoff[31]);
     return n <= WIN_MAX ? n : 0;
 }
 
@@ -246,7 +231,7 @@ __device__ __forceinline__ void list16_load(const int4* __restrict__ list, int n
         L.g[g] = g < ng ? ld_group(&list[(size_t)g * ncap + i], nt) : make_int4(0, 0, 0, 0);
 }
 
-// Neighbour loop over a packed fluid list (in registers) whose records are
+// Neighbour loop over a window-index list (in registers) whose records are
 // in the LDS window: load(k) reads window record k, use(k, r) consumes it.
 // Entries past the padded count are read from record 0 and not used.
 template <class LD, class US>
@@ -262,8 +247,7 @@ __device__ __forceinline__ void each_in_win16(const List16& L, int n, LD&& load,
         int k[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const unsigned e = (w[q >> 1] >> (16 * (q & 1))) & 0xffffu;
-            const int kq = g_wintab[e >> NB16_OFF_BITS] + (int)e;
+            const int kq = (int)((w[q >> 1] >> (16 * (q & 1))) & 0xffffu);
             k[q] = (q < 4 || second) ? kq : 0;
         }
         decltype(load(0)) r[8];
@@ -293,13 +277,14 @@ __device__ __forceinline__ int nf_real(int c) { return (c >> 8) & 0xff; }
 __device__ __forceinline__ int nb_of(int c) { return (c >> 16) & 0xff; }
 __device__ __forceinline__ int phase_of_cnt(int c) { return c >> 24; }
 
-// The list of a live particle of a tile that may stage a window for the
-// pass `bit` (packed, the pass's window on); the window itself is decided by
-// win_layout.
+// The window-index list of a live particle of a tile that may stage a
+// window for the pass `bit` (packed, the pass's window on); the window itself
+// is decided by win_layout (the list builder writes this list for exactly the
+// tiles whose window it accepts).
 __device__ __forceinline__ void list16_prefetch(const Dev& D, const Slot& s, int c, List16& L, int bit) {
     if (s.live && s.packed && (D.winMask & bit)) {
         const bool nt = __builtin_amdgcn_readfirstlane(D.listNT) != 0;
-        list16_load(D.nbF16, D.ncap, s.i, nf_of(c), nt, L);
+        list16_load(D.nbW16, D.ncap, s.i, nf_of(c), nt, L);
     }
 }
 

@@ -69,6 +69,7 @@ struct Dev {
 
     // ---- kernel constants (SphBase.h:808-829, h = smoothRadius_h / 2)
     float kh, inv_kh, sigma, sigma_h, W0;
+    float gA, gB, gU, gV;   // kernel_gradfac_sq's spline pieces in r
     float h2;          // neighbour cutoff fl((d+d)^2), SphBase.cpp:883-884
     float Hs;          // smoothRadius_h (artificial viscosity length, WcSph.cpp:182)
     float visc_eps;    // 0.01*Hs*Hs (WcSph.cpp:184)
@@ -135,9 +136,15 @@ struct Dev {
     // nbF/nbB (groups of four, same layout); nbTab[64*tile + 31] says which.
     int4* nbF16;
     int4* nbB16;
+    // The fluid lists again, as window indices: a packed tile whose window
+    // (its fluid slot runs concatenated in slot order, WIN_MAX records at
+    // most) fits gets each entry as the neighbour's index in that window, so
+    // a pass that stages the window in LDS reads its records with no table.
+    int4* nbW16;
     int* nbTab;
-    int* nbLen;        // per tile, 32 ints: length of each fluid slot's run (packed tiles;
-                       // the window a pass stages in LDS is their concatenation)
+    int* nbWin;        // per tile, 32 ints: offset of each fluid slot's run in the tile's
+                       // window (packed tiles; [31], the flag slot, whose run is
+                       // empty, holds the window size; 0 for other tiles)
     int* nbF;
     int* nbB;
     int* cnt;          // stored fluid entries | fluid neighbours found << 8 | boundary << 16
@@ -294,19 +301,22 @@ __device__ __forceinline__ float kernel_gradfac(float r, const Dev& D) {
     return q <= 2.0f ? g : 0.0f;
 }
 
-// kernel_gradfac from r^2 with one reciprocal square root (v_rsq_f32, the
-// r and 1/r of the pair from one quarter-rate instruction instead of a
-// square root and a reciprocal), for the solver-loop passes.  At r = 0
-// (coincident particles, whose pair vector x_ab is zero) q is NaN and the
-// factor 0, so the pair adds nothing, as with kernel_gradfac.
+// kernel_gradfac from r^2 for the solver-loop passes: one reciprocal square
+// root gives r and 1/r of the pair, and both spline pieces are written in r
+// with the constants folded (Dev::gA..gV): for r <= h,
+// -(sigma_h/h)(3 - 2.25 r/h) = gA + gB r; for h < r, -0.75 sigma_h (2 - r/h)^2 / r
+// = -u^2 / r with u = sqrt(0.75 sigma_h)(2 - r/h) = gU r + gV, clamped at 0
+// past the support (ten VALU instructions a pair).  r^2 is floored at 1e-30
+// so that coincident particles, whose pair vector x_ab is zero, get the
+// finite r <= h value and add nothing.
+constexpr float R2_FLOOR = 1.0e-30f;
 __device__ __forceinline__ float kernel_gradfac_sq(float r2, const Dev& D) {
-    const float ri = __builtin_amdgcn_rsqf(r2);
-    const float q = (r2 * ri) * D.inv_kh;
-    const float qm2 = 2.0f - q;
-    const float g1 = -(D.sigma_h * D.inv_kh) * (3.0f - 2.25f * q);
-    const float g2 = -0.75f * D.sigma_h * qm2 * qm2 * ri;
-    const float g = q <= 1.0f ? g1 : g2;
-    return q <= 2.0f ? g : 0.0f;
+    const float ri = __builtin_amdgcn_rsqf(fmaxf(r2, R2_FLOOR));
+    const float r = r2 * ri;
+    const float g1 = __builtin_fmaf(r, D.gB, D.gA);
+    const float u = fmaxf(__builtin_fmaf(r, D.gU, D.gV), 0.0f);
+    const float g2 = -(u * u) * ri;
+    return r <= D.kh ? g1 : g2;
 }
 
 // a.x b.x + a.y b.y + a.z b.z in one fixed rounding order,
@@ -392,6 +402,10 @@ __device__ __forceinline__ size_t nb16_slot(int k, int i, int ncap) {
 }
 
 constexpr float DUMMY_FAR = 1.0e15f;   // coordinate of the dummy boundary particle
+
+// Records of the LDS window of a packed tile at most (12 x 256): a larger
+// window is not staged and the tile's loops gather from memory.
+constexpr int WIN_MAX = 3072;
 
 // ---------------------------------------------------------------------------
 // host-side launchers implemented in the .hip files
