@@ -41,6 +41,9 @@ ALG_BYTES = {
     "ii_sumdij": 32, "ii_pressure": 56, "ii_aadv_dii": 64, "ii_aii": 48, "ii_force": 56,
     "pci_aext": 56, "pci_density": 44, "pci_pressure": 64, "pci_integrate": 84,
 }
+# the solver-loop passes with an LDS-window build and a gathers-only build
+WIN_BIT = {"df_pass_density": sg.WIN_DENSITY, "df_pass_divergence": sg.WIN_DIVERGENCE,
+           "df_pass_velocity": sg.WIN_VELOCITY, "ii_sumdij": sg.WIN_SUMDIJ, "ii_pressure": sg.WIN_PRESSURE}
 DOMINANT = {sg.DFSPH: "df_pass_divergence", sg.IISPH: "ii_pressure", sg.WCSPH: "wc_force",
             sg.PCISPH: "pci_pressure"}
 
@@ -56,16 +59,21 @@ def traffic_file(workload: str) -> Path:
     return ROOT / "profiles" / f"{workload}_traffic.txt"
 
 
-def read_traffic(workload: str, kernel: str):
+def read_traffic(workload: str, kernel: str, window: bool | None = None):
     """HBM bytes per launch of `kernel` measured by the FETCH_SIZE/WRITE_SIZE
-    passes recorded in profiles/<workload>_traffic.txt (None if absent)."""
+    passes recorded in profiles/<workload>_traffic.txt (None if absent); for
+    a solver-loop pass, of its LDS-window build (window) or its gathers-only
+    build."""
     path = traffic_file(workload)
     if not path.exists():
         return None
+    names = {"k_" + kernel}
+    if window is not None:
+        names.add(f"k_{kernel}<{'true' if window else 'false'}>")
     for line in path.read_text().splitlines():
         if line.startswith("{"):
             rec = json.loads(line)
-            if rec.get("kernel") == "k_" + kernel:
+            if rec.get("kernel") in names:
                 return rec
     return None
 
@@ -78,7 +86,8 @@ def kernel_name(raw: str) -> str:
         k = int(m.group(1))
         start = m.end()
         return raw[start:start + k]
-    return raw.replace("sphgpu::", "").replace("(anonymous namespace)::", "").split("(")[0]
+    name = raw.replace("sphgpu::", "").replace("(anonymous namespace)::", "").split("(")[0]
+    return name[5:] if name.startswith("void ") else name
 
 
 def summarize_trace(stats_csv: str, trace_csv: str, workload: str, command: str, timed: str | None = None) -> int:
@@ -105,10 +114,14 @@ def summarize_trace(stats_csv: str, trace_csv: str, workload: str, command: str,
                      f"{len(real):11d} {ra:12.2f}")
     if timed:
         kname, n = timed.rsplit(":", 1)
-        real = [x for x in per.get(kname, []) if x >= 10.0][-int(n):]
-        if real:
-            lines.append(f"# timed steps: last {len(real)} dispatches of {kname} of 10 us or more: "
-                         f"mean {sum(real) / len(real):.2f} us")
+        allv = per.get(kname, [])
+        if allv:
+            # iterations launched past convergence return early; at this
+            # size they take well under a quarter of a working launch
+            thr = max(10.0, 0.25 * sorted(allv)[len(allv) // 2])
+            real = [x for x in allv if x >= thr][-int(n):]
+            lines.append(f"# timed steps: last {len(real)} dispatches of {kname} of {thr:.0f} us or more "
+                         f"(a quarter of its median): mean {sum(real) / len(real):.2f} us")
     out = ROOT / "profiles" / f"{workload}_kernel_stats.txt"
     out.write_text("\n".join(lines) + "\n")
     print(out)
@@ -117,18 +130,20 @@ def summarize_trace(stats_csv: str, trace_csv: str, workload: str, command: str,
 
 def summarize_pmc(fetch_csv: str, write_csv: str, workload: str, command: str) -> int:
     """Per-kernel median FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of two
-    rocprofv3 --pmc runs, scaled by what the counters read for the copy
-    probe (k_copy_probe, COPY_PROBE_BYTES read and written); writes
+    rocprofv3 --pmc runs (or of several pairs: comma-separated CSV lists, e.g.
+    runs with and without LDS windows), scaled by what the counters read for
+    the copy probe (k_copy_probe, COPY_PROBE_BYTES read and written); writes
     profiles/<workload>_traffic.txt."""
     import csv
     import collections
 
-    def load(path):
+    def load(paths):
         d = collections.defaultdict(list)
-        for r in csv.DictReader(open(path)):
-            name = kernel_name(r["Kernel_Name"])
-            dur = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
-            d[name].append((float(r["Counter_Value"]), dur, int(r["Grid_Size"])))
+        for path in paths.split(","):
+            for r in csv.DictReader(open(path)):
+                name = kernel_name(r["Kernel_Name"])
+                dur = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+                d[name].append((float(r["Counter_Value"]), dur, int(r["Grid_Size"])))
         return d
 
     f, w = load(fetch_csv), load(write_csv)
@@ -162,13 +177,16 @@ def build_scene(name: str):
 
 
 def run_gpu(sc, warm_start: int, steps: int, warmup: int, profile: str | None, graph: int = 0,
-            stats_every: int = 1, capture_at: int | None = None, label: str = "", train: int = 0):
+            stats_every: int = 1, capture_at: int | None = None, label: str = "", train: int = 0,
+            lds_windows: str = "auto"):
     """One arm: `train` untimed steps in which a learned warm start trains its
     network (every arm runs them, so that all arms time the same steps of the
     simulation), `warmup` untimed steps, then `steps` timed steps.  Returns
     per-step statistics of the untimed and the timed steps; with capture_at
     the fluid state after that many untimed steps (for the CPU baseline)."""
     sim = scenes.make_simulation(sc, warm_start=warm_start, use_graph=graph)
+    if lds_windows != "auto":
+        sim.set_lds_windows(lds_windows == "on")
     if sc.rigids:
         # the host rigid-body step (Bullet's place in the reference) inside
         # every step, at the updateSolids point; part of the timed work
@@ -365,6 +383,8 @@ def main() -> int:
     ap.add_argument("--cpu-from-step", type=int, default=10,
                     help="the CPU baseline starts from the GPU plain arm's state after this many steps")
     ap.add_argument("--profile-kernel", default=None)
+    ap.add_argument("--lds-windows", default="auto", choices=["auto", "on", "off"],
+                    help="LDS windows in every solver-loop pass (on), in none (off), or per pass as the autotuner times them")
     ap.add_argument("--stats-every", type=int, default=1,
                     help="read statistics back every N timed steps (the others do not wait for the device)")
     ap.add_argument("--graph", action="store_true",
@@ -420,7 +440,8 @@ def main() -> int:
     cap_main = args.cpu_from_step if (want_cpu and warm == sg.WARM_NONE and not sc.rigids
                                       and args.cpu_from_step <= train + args.warmup) else None
     run = run_gpu(sc, warm, args.steps, args.warmup, None if args.graph else prof, int(args.graph),
-                  max(1, args.stats_every), capture_at=cap_main, label=labels[warm], train=train)
+                  max(1, args.stats_every), capture_at=cap_main, label=labels[warm], train=train,
+                  lds_windows=args.lds_windows)
     nf = sc.num_fluid
     value = nf * args.steps / run["elapsed"]
     ms = run["elapsed"] / args.steps * 1e3
@@ -435,11 +456,14 @@ def main() -> int:
         avg_s = tot_ms / launches * 1e-3
         alg = ALG_BYTES[prof] * nf
         achieved = alg / avg_s / 1e9
-        tr = read_traffic(sc.name, prof)
+        bit = WIN_BIT.get(prof)
+        lw = (run["tuning"] or {}).get("lds_windows")
+        window = bool(lw["mask"] & bit) if (bit and lw) else None
+        tr = read_traffic(sc.name, prof, window)
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                     "traffic": tr["traffic_bytes"] if tr else None,
-                    "kernel": prof, "avg_us": round(avg_s * 1e6, 3), "launches": launches,
+                    "kernel": prof, "lds_window": window, "avg_us": round(avg_s * 1e6, 3), "launches": launches,
                     "alg_bytes_per_launch": alg,
                     "traffic_achieved_GBs": round(tr["traffic_bytes"] / avg_s / 1e9, 1) if tr else None,
                     "copy_probe_GBs": round(copy_gbs, 1)}
@@ -456,7 +480,8 @@ def main() -> int:
         for w in others:
             cap = args.cpu_from_step if (w == sg.WARM_NONE and want_cpu and not sc.rigids
                                          and args.cpu_from_step <= train + args.warmup) else None
-            arm = run_gpu(sc, w, args.steps, args.warmup, None, capture_at=cap, label=labels[w], train=train)
+            arm = run_gpu(sc, w, args.steps, args.warmup, None, capture_at=cap, label=labels[w], train=train,
+                          lds_windows=args.lds_windows)
             if cap is not None:
                 captured = arm["captured"]
             a = arm_summary(arm, nf, args.steps, tol)
@@ -510,6 +535,8 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if args.lds_windows != "auto":
+        out["config"]["lds_windows"] = args.lds_windows
     print(json.dumps(out), flush=True)
     return 0
 

@@ -417,11 +417,11 @@ int solver_loop(sphgpu_handle* h, int which) {
 // 0.124-0.127 ms against 0.086 unpadded on the 1M dam, so it was removed.)
 // best of three: `prep` (untimed), then `reps` launches of `f`; per launch
 float time_launches(sphgpu_handle* h, const std::function<void()>& f, const std::function<void()>& prep = nullptr,
-                    int reps = 1) {
+                    int reps = 1, int tries = 3) {
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return 0.0f;
     float best = 1e30f;
-    for (int r = 0; r < 3; ++r) {
+    for (int r = 0; r < tries; ++r) {
         if (prep) prep();
         hipEventRecord(a, h->st);
         for (int q = 0; q < reps; ++q) f();
@@ -438,8 +438,10 @@ float time_launches(sphgpu_handle* h, const std::function<void()>& f, const std:
 // Per solver-loop pass: stage the tile's neighbour window in LDS (fewer
 // cache-line lookups, but 48-72 KB of LDS per block: two or three blocks per
 // CU) or gather each neighbour per lane (five waves per SIMD).  Each pass is
-// timed both ways on the scene's lists (best of three, two launches each)
-// and keeps the faster form; both give the same bits.  The passes overwrite
+// timed both ways on the scene's lists (best of two single launches: six
+// forms, twelve launches a pass, about 4 ms per pass at 4M particles, so the
+// re-tuning every 200 steps costs well under 0.1 ms a step) and keeps the
+// faster form; both give the same bits.  The passes overwrite
 // solver state, so everything they write is saved first and put back after.
 static_assert(WIN_DENSITY == SPHGPU_WIN_DENSITY && WIN_DIVERGENCE == SPHGPU_WIN_DIVERGENCE &&
                   WIN_VELOCITY == SPHGPU_WIN_VELOCITY && WIN_SUMDIJ == SPHGPU_WIN_SUMDIJ &&
@@ -490,13 +492,13 @@ int tune_windows(sphgpu_handle* h) {
     for (const auto& p : passes) {
         const int k = __builtin_ctz(p.first);
         D.winMask = WIN_ALL;
-        const float tw = time_launches(h, p.second, nullptr, 2);
+        const float tw = time_launches(h, p.second, nullptr, 1, 2);
         float tg = 1e30f;
         int best = 0;
         D.winMask = 0;
         for (int c : caps) {
             D.gatherLds[k] = c;
-            const float t = time_launches(h, p.second, nullptr, 2);
+            const float t = time_launches(h, p.second, nullptr, 1, 2);
             if (t < tg) tg = t, best = c;
         }
         D.gatherLds[k] = best;

@@ -37,9 +37,12 @@ __shared__ int g_winoff[32];
 
 // slot_of for kernels that read neighbour lists: every thread of the block
 // calls it before any returns (it stages the tile's tables in LDS, both with
-// one round trip).
+// one round trip).  A gated block whose scene has converged loads nothing
+// (its particles are not live): solver iterations launched past convergence
+// return at once.
 __device__ __forceinline__ Slot slot_nb(const Dev& D, bool gated) {
     Slot s = slot_of(D, gated);
+    if (gated && !D.active[s.scene]) return s;   // uniform per block
     if (threadIdx.x < NB16_TAB) g_nbtab[threadIdx.x] = D.nbTab[(size_t)s.tile * NB16_TAB + threadIdx.x];
     else if (threadIdx.x < NB16_TAB + 32)
         g_winoff[threadIdx.x - NB16_TAB] = D.nbWin[(size_t)s.tile * 32 + threadIdx.x - NB16_TAB];
@@ -832,7 +835,7 @@ __global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
     WIN_XV_DECL(WIN ? WIN_MAX : 1)
     Slot s = slot_nb(D, true);
     if (!D.active[s.scene]) {   // uniform: a converged scene's tiles contribute nothing
-        write_partial(D, s.tile, 0, 0.0, lds);
+        if (threadIdx.x == 0) D.partial[2 * s.tile] = 0.0;
         return;
     }
     // own records and list first: their loads overlap the window staging
@@ -875,7 +878,7 @@ __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected
     WIN_XV_DECL(WIN ? WIN_MAX : 1)
     Slot s = slot_nb(D, true);
     if (!D.active[s.scene]) {   // uniform: a converged scene's tiles contribute nothing
-        write_partial(D, s.tile, 0, 0.0, lds);
+        if (threadIdx.x == 0) D.partial[2 * s.tile] = 0.0;
         return;
     }
     // own records and list first: their loads overlap the window staging
@@ -1196,7 +1199,7 @@ __global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
     WIN_XV_DECL(WIN ? WIN_MAX : 1)
     Slot s = slot_nb(D, true);
     if (!D.active[s.scene]) {   // uniform: a converged scene's tiles contribute nothing
-        write_partial(D, s.tile, 0, 0.0, lds);
+        if (threadIdx.x == 0) D.partial[2 * s.tile] = 0.0;
         return;
     }
     const int c = s.live ? D.cnt[s.i] : 0;
