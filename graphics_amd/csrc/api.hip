@@ -261,14 +261,17 @@ int run_ns(sphgpu_handle* h) {
     return 0;
 }
 
-int sync_active(sphgpu_handle* h, bool* any) {
+// `odd`: the flags to read are those of an odd iteration (the second buffer
+// of the DFSPH loops)
+int sync_active(sphgpu_handle* h, bool* any, bool odd = false) {
     Dev& D = h->D;
     int ns = D.nscenes;
     // flags, iteration counts and errors: one block, same layout on both sides
     HIPCHECK(h, hipMemcpyAsync(h->hActive, D.active, 24 * (size_t)ns, hipMemcpyDeviceToHost, h->st));
     HIPCHECK(h, hipStreamSynchronize(h->st));
+    const int* flags = odd ? h->hActive + ns : h->hActive;
     bool a = false;
-    for (int s = 0; s < ns; ++s) a = a || h->hActive[s] != 0;
+    for (int s = 0; s < ns; ++s) a = a || flags[s] != 0;
     *any = a;
     return 0;
 }
@@ -287,28 +290,35 @@ void launch_iteration(sphgpu_handle* h, int which, int it) {
     h->curIter = it;
     {
         if (P.solver == SPHGPU_DFSPH) {
-            if (which == 0) {
-                {
-                    ProfScope ps(h, "df_pass_density", nf_live(h) * 36.0);
-                    launch_df_pass_density(D, h->st);
-                }
-                {
-                    ProfScope ps(h, "df_pass_velocity", nf_live(h) * 44.0);
-                    launch_df_pass_velocity(D, h->st);
-                }
-                launch_scene_reduce(D, 0, P.density_tol, P.min_iter_density,
-                                    P.max_iter_density, h->st);
-            } else {
-                {
-                    ProfScope ps(h, "df_pass_divergence", nf_live(h) * 36.0);
-                    launch_df_pass_divergence(D, P.warm_start != SPHGPU_WARM_NONE && it == 0, h->st);
-                }
-                {
-                    ProfScope ps(h, "df_pass_velocity", nf_live(h) * 44.0);
-                    launch_df_pass_velocity(D, h->st);
-                }
-                launch_scene_reduce(D, 1, P.divergence_tol, P.min_iter_div, P.max_iter_div + 1, h->st);
+            // iteration `it` gates on flag buffer it & 1; the velocity pass
+            // tests convergence on the partials pass A left and writes the
+            // other buffer for the next iteration (a separate reduction
+            // kernel when a scene has no tile to host its test)
+            Dev Dq = D;
+            Dq.active = (it & 1) ? D.active1 : D.active;
+            const bool fused = D.npad / TILE >= D.nscenes;
+            Dq.activeNext = (it & 1) ? D.active : D.active1;
+            if (fused) {
+                Dq.redMode = which == 0 ? 0 : 1;
+                Dq.redTol = which == 0 ? P.density_tol : P.divergence_tol;
+                Dq.redMinIt = which == 0 ? P.min_iter_density : P.min_iter_div;
+                Dq.redCap = which == 0 ? P.max_iter_density : P.max_iter_div + 1;
             }
+            if (which == 0) {
+                ProfScope ps(h, "df_pass_density", nf_live(h) * 36.0);
+                launch_df_pass_density(Dq, h->st);
+            } else {
+                ProfScope ps(h, "df_pass_divergence", nf_live(h) * 36.0);
+                launch_df_pass_divergence(Dq, P.warm_start != SPHGPU_WARM_NONE && it == 0, h->st);
+            }
+            {
+                ProfScope ps(h, "df_pass_velocity", nf_live(h) * 44.0);
+                launch_df_pass_velocity(Dq, h->st);
+            }
+            if (!fused)
+                launch_scene_reduce(Dq, which == 0 ? 0 : 1, which == 0 ? P.density_tol : P.divergence_tol,
+                                    which == 0 ? P.min_iter_density : P.min_iter_div,
+                                    which == 0 ? P.max_iter_density : P.max_iter_div + 1, h->st);
         } else if (P.solver == SPHGPU_IISPH_DF) {
             // IISph.cpp:840-1004 (which 0) and 1062-1207 (which 1): the loops
             // break when the iteration count exceeds the maximum
@@ -373,7 +383,8 @@ int solver_loop_finish(sphgpu_handle* h, int which, int launched, size_t profSta
     if (extra) *extra = 0;
     for (;;) {
         bool any = false;
-        int rc = sync_active(h, &any);
+        // the DFSPH loops keep the flag of iteration k in buffer k & 1
+        int rc = sync_active(h, &any, h->P.solver == SPHGPU_DFSPH && (launched & 1));
         if (rc) return rc;
         if (!any) break;
         for (int q = 0; q < 2; ++q) launch_iteration(h, which, launched++);
@@ -1548,14 +1559,19 @@ int sphgpu_init(sphgpu_handle* h) {
     ALLOC(D.dt, ns);
     {
         // solver-loop state in one block, laid out as its pinned host copy
-        // (flags | iteration counts | errors, 8 bytes per scene each), so the
-        // host reads it with one copy
+        // (flags of even and odd iterations | iteration counts | errors, 8
+        // bytes per scene each), so the host reads it with one copy
         double* loopBlk = nullptr;
         ALLOC(loopBlk, 3 * (size_t)ns);
         ALLOC(D.integ, ns);
         D.active = reinterpret_cast<int*>(loopBlk);
+        D.active1 = D.active + ns;
         D.iters = reinterpret_cast<int*>(loopBlk + ns);
         D.sceneErr = loopBlk + 2 * ns;
+        D.activeNext = nullptr;
+        D.redMode = -1;
+        D.redMinIt = D.redCap = 0;
+        D.redTol = 0.0f;
     }
     ALLOC(D.partial, 2 * (size_t)numTiles); ALLOC(D.counters, C_COUNT); ALLOC(D.vmaxPartial, numTiles);
     ALLOC(h->layoutOut, 2); ALLOC(h->statsPart, 4 * (size_t)numTiles); ALLOC(h->statsOut, 4 * (size_t)ns);

@@ -334,10 +334,16 @@ __global__ __launch_bounds__(256) void k_cfl_reduce(Dev D, float cfl, float spac
 // mode 2: PCISPH  err = max error/rho0 (second partial)     (PciSph.cpp:344-350)
 // Loop continuation (while err > tol || it < minIt; PCISPH err >= tol,
 // PciSph.cpp:116), stopping at `cap`.
-__global__ __launch_bounds__(256) void k_scene_reduce(Dev D, int mode, float tol, int minIt, int cap) {
+// Writes the scene's continuation flag to out (the active array itself, or
+// the next iteration's buffer of the DFSPH loops; a scene already done stays
+// done).  Every thread of the block calls it.
+__device__ __forceinline__ void scene_reduce_block(const Dev& D, int sc, int mode, float tol, int minIt, int cap,
+                                                   int* out) {
     __shared__ double lds[4];
-    int sc = blockIdx.x;
-    if (!D.active[sc]) return;   // uniform per block
+    if (!D.active[sc]) {   // uniform per block
+        if (threadIdx.x == 0 && out != D.active) out[sc] = 0;
+        return;
+    }
     int t0 = D.scenePadStart[sc] / TILE, t1 = D.scenePadStart[sc + 1] / TILE;
     __shared__ float ldsf[4];
     double s = 0.0;
@@ -374,8 +380,12 @@ __global__ __launch_bounds__(256) void k_scene_reduce(Dev D, int mode, float tol
             cont = false;
             D.counters[C_COUNT - 1] = 1;  // max-iteration flag
         }
-        D.active[sc] = cont ? 1 : 0;
+        out[sc] = cont ? 1 : 0;
     }
+}
+
+__global__ __launch_bounds__(256) void k_scene_reduce(Dev D, int mode, float tol, int minIt, int cap) {
+    scene_reduce_block(D, blockIdx.x, mode, tol, minIt, cap, D.activeNext ? D.activeNext : D.active);
 }
 
 // ------------------------------------------------------------- stats -----
@@ -923,6 +933,9 @@ __global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected
 template <bool WIN>
 __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
     __shared__ float4 s_wxk[WIN ? WIN_MAX : 1];   // window records: xk of each neighbour
+    // the loop's convergence test on pass A's partial sums (block b: scene b)
+    if (D.redMode >= 0 && (int)blockIdx.x < D.nscenes)
+        scene_reduce_block(D, blockIdx.x, D.redMode, D.redTol, D.redMinIt, D.redCap, D.activeNext);
     Slot s = slot_nb(D, true);
     if (!D.active[s.scene]) return;     // uniform per block
     const int c = s.live ? D.cnt[s.i] : 0;

@@ -187,10 +187,19 @@ struct Dev {
     // ---- per-scene solver scalars
     float* dt;           // per scene
     int* active;         // per scene: 1 while its solver loop runs
+    int* active1;        // the DFSPH loops' second flag buffer (odd iterations)
     int* iters;          // per scene: iterations of the current loop
     int* integ;          // per scene: 1 once a captured IISPH step integrated it
     double* sceneErr;    // per scene
     double* partial;     // per tile partial sums (2 per tile)
+    // The DFSPH loops run their convergence test inside the velocity pass
+    // (pass B): its first nscenes blocks reduce the partial sums pass A left
+    // and write the flag of the next iteration to activeNext, while every
+    // block gates on `active`, the flag pass A ran under (the launcher swaps
+    // the two buffers by iteration parity).  redMode < 0: no test.
+    int* activeNext;
+    int redMode, redMinIt, redCap;
+    float redTol;
     int* counters;       // Counter
     float* vmaxPartial;  // per tile
 };
