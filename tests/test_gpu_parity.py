@@ -214,6 +214,45 @@ def test_batched_scenes_equal_single_scenes(gpu):
     sim.close()
 
 
+@pytest.mark.parametrize("warm", [sg.WARM_NONE, sg.WARM_PREVIOUS])
+def test_batched_dfsph_scenes_equal_single_scenes(gpu, warm):
+    """Batched DFSPH scenes: each scene's solver loops stop at its own
+    convergence (the velocity pass tests every scene), so a scene advanced
+    with others is bit-identical to the scene alone, iteration counts
+    included, and the scenes do not all take the same counts."""
+    base = scenes.iisph_batched_c4(scenes=3, nmin=300, nmax=2000, seed=11)
+    p = {k: v for k, v in base.params.items() if k != "omega"}
+    p.update(solver=sg.DFSPH, density_tol=1e-4, divergence_tol=1.0, min_iter_density=2, max_iter_density=100,
+             min_iter_div=1, max_iter_div=100)
+    sc = scenes.Scene("dfsph_batched_3", p, list(base.phases), list(base.fluids), list(base.solids),
+                      list(base.domains))
+    print(f"[parity] batched DFSPH, {sc.num_scenes} scenes, warm start {warm}", flush=True)
+    steps = 6
+    sim = scenes.make_simulation(sc, warm_start=warm)
+    batched = []
+    for _ in range(steps):
+        sim.step()
+        batched.append([(sim.scene_stats(k)["iter_density"], sim.scene_stats(k)["iter_divergence"])
+                        for k in range(sc.num_scenes)])
+    ids = sim.ids()
+    pos, vel = sim.get(sg.F_POS), sim.get(sg.F_VEL)
+    first = 0
+    for k, (s, ph, xyz) in enumerate(sc.fluids):
+        single = scenes.Scene("one", p, list(sc.phases), [(0, 0, xyz)], [(0, sc.solids[k][1], sc.solids[k][2])],
+                              [sc.domains[k]])
+        one = scenes.make_simulation(single, warm_start=warm)
+        alone = [(s1["iter_density"], s1["iter_divergence"]) for s1 in (one.step() for _ in range(steps))]
+        assert alone == [b[k] for b in batched], k
+        sel = (ids >= first) & (ids < first + len(xyz))
+        assert np.array_equal(ids[sel] - first, one.ids())
+        assert pos[sel].tobytes() == one.get(sg.F_POS).tobytes()
+        assert vel[sel].tobytes() == one.get(sg.F_VEL).tobytes()
+        first += len(xyz)
+        one.close()
+    sim.close()
+    assert len({tuple(b) for b in zip(*batched)}) > 1   # the scenes converge at different iterations
+
+
 # ------------------------------------------------------------- faults ------
 def test_fault_counters(gpu):
     s = 0.02
