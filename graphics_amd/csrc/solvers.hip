@@ -179,6 +179,7 @@ struct Rec2 {
 // Tiles with 32-bit lists, or whose window exceeds WIN_MAX records, keep the
 // per-lane gathers (the same sums in the same order, so both forms give
 // identical results).
+
 // Window size of the block's tile (after slot_nb) for the pass `bit`, or 0
 // when the tile's loops gather from memory.
 __device__ __forceinline__ int win_layout(const Dev& D, const Slot& s, int bit) {
