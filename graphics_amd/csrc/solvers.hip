@@ -838,95 +838,203 @@ __device__ __forceinline__ float density_change_win(const Dev& D, const Slot& s,
     __shared__ float4 s_wzv[N];                  \
     const WinXV win{s_wxy, s_wzv};
 
-// pass A of the density loop (DFSPH.cpp:572-643): rho*, clamp rho* >= rho0,
-// kappa = (rho*-rho0)/dt^2 alpha -> xk.w = kappa/rho; partial sum of rho*/rho0.
-template <bool WIN>
-__global__ __launch_bounds__(256) void k_df_pass_density(Dev D) {
-    __shared__ double lds[4];
-    WIN_XV_DECL(WIN ? WIN_MAX : 1)
-    Slot s = slot_nb(D, true);
-    if (!D.active[s.scene]) {   // uniform: a converged scene's tiles contribute nothing
-        if (threadIdx.x == 0) D.partial[2 * s.tile] = 0.0;
-        return;
-    }
-    // own records and list first: their loads overlap the window staging
-    const int c = s.live ? D.cnt[s.i] : 0;
-    List16 L;
-    if (WIN) list16_prefetch(D, s, c, L, WIN_DENSITY);
-    const int nw = WIN ? win_layout(D, s, WIN_DENSITY) : 0;
-    if (nw) stage_xv(D, nw, win);
-    double contrib = 0.0;
-    if (s.live) {
-        const int i = s.i;
-        float4 pa = D.x[i], va = D.vs[i];
-        int ph = phase_of(pa);
-        float m = D.mass[ph], rho0 = D.rho0[ph];
-        float dd = nw ? density_change_win(D, s, pa, va, m, rho0, c, win, L)
-                      : density_change(D, s, pa, va, m, rho0, c);
-        float dt = D.dt[s.scene];
-        float rhoa = va.w;   // v* carries rho in w (as v does), one array fewer to stream
-        float rs = rhoa + dt * dd;
-        if (rs < rho0) rs = rho0;
-        float dev = (rs - rho0) * D.alpha[i];     // kappa * dt^2
-        D.xk[i] = make_float4(pa.x, pa.y, pa.z, dev / (dt * dt) / rhoa);
-        D.ksum[i] += dev;
-        contrib = (double)rs / (double)rho0;
-    }
-    write_partial(D, s.tile, 0, contrib, lds);
+// ---- pass A of the DFSPH loops -------------------------------------------
+// Density loop (MODE 0, DFSPH.cpp:572-643): rho* = rho + dt Drho/Dt, clamp
+// rho* >= rho0, kappa = (rho*-rho0)/dt^2 alpha -> xk.w = kappa/rho; partial
+// sum of rho*/rho0.  Divergence loop (MODE 1, DFSPH.cpp:780-836): Drho/Dt,
+// clamp, kappaV = Drho/Dt alpha / dt -> xk.w = kappaV/rho; partial sum of
+// Drho/Dt.  projected (first divergence iteration of a warm-started solve):
+// the accumulated stiffness kvsum is clamped at zero instead of the
+// increment, so a guess that was too large is taken back where the fluid
+// now expands; the error is the mean applied |increment| / alpha.  Later
+// iterations are the reference's (projecting every iteration is unstable:
+// signed Jacobi oscillates).
+
+// what a particle of pass A reads of itself
+struct OwnAB {
+    float4 pa, va;
+    float m, rho0, dt, alpha, acc;
+};
+template <int MODE>
+__device__ __forceinline__ OwnAB own_ab(const Dev& D, const Slot& s) {
+    OwnAB o{};
+    if (!s.live) return o;
+    o.pa = D.x[s.i];
+    o.va = D.vs[s.i];   // v* carries rho in w (as v does), one array fewer to stream
+    o.dt = D.dt[s.scene];
+    o.alpha = D.alpha[s.i];
+    o.acc = MODE == 0 ? D.ksum[s.i] : D.kvsum[s.i];
+    const int ph = phase_of(o.pa);
+    o.m = D.mass[ph];
+    o.rho0 = D.rho0[ph];
+    return o;
 }
 
-// pass 1 of the divergence loop (DFSPH.cpp:780-836): Drho/Dt, clamp,
-// kappaV = Drho/Dt alpha / dt -> xk.w = kappaV/rho; partial sum of Drho/Dt.
-// projected (first iteration of a warm-started solve): the accumulated
-// stiffness kvsum is clamped at zero instead of the increment, so a guess
-// that was too large is taken back where the fluid now expands; the error is
-// the mean applied |increment| / alpha.  Later iterations are the
-// reference's (projecting every iteration is unstable: signed Jacobi
-// oscillates).
-template <bool WIN>
-__global__ __launch_bounds__(256) void k_df_pass_divergence(Dev D, int projected) {
+// one particle of pass A; nw > 0: its fluid neighbours are in the LDS window
+template <int MODE>
+__device__ __forceinline__ double pass_ab_particle(const Dev& D, const Slot& s, const OwnAB& o, int c, int nw,
+                                                   WinXV win, const List16& L, int projected) {
+    if (!s.live) return 0.0;
+    const int i = s.i;
+    const float4 pa = o.pa, va = o.va;
+    const float m = o.m, rho0 = o.rho0, dt = o.dt, rhoa = va.w;
+    float dd = nw ? density_change_win(D, s, pa, va, m, rho0, c, win, L) : density_change(D, s, pa, va, m, rho0, c);
+    if (MODE == 0) {
+        float rs = rhoa + dt * dd;
+        if (rs < rho0) rs = rho0;
+        const float dev = (rs - rho0) * o.alpha;   // kappa * dt^2
+        D.xk[i] = make_float4(pa.x, pa.y, pa.z, dev / (dt * dt) / rhoa);
+        D.ksum[i] = o.acc + dev;
+        return (double)rs / (double)rho0;
+    }
+    if (!projected) {
+        if (dd < 0.0f || (rhoa + dd * dt) < rho0) dd = 0.0f;
+        const float kv = dd * o.alpha / dt;
+        D.xk[i] = make_float4(pa.x, pa.y, pa.z, kv / rhoa);
+        D.kvsum[i] = o.acc + kv * dt;
+        return (double)dd;
+    }
+    // below rest density the solve applies no stiffness: the whole guess is
+    // taken back
+    const float alpha = o.alpha, K = o.acc;
+    float inc = dd * alpha;
+    if ((rhoa + dd * dt) < rho0 || inc < -K) inc = -K;
+    const float kv = inc / dt;
+    D.xk[i] = make_float4(pa.x, pa.y, pa.z, kv / rhoa);
+    D.kvsum[i] = K + inc;
+    return (double)(fabsf(inc) / alpha);
+}
+
+// Gathers-only build: one tile per block.
+template <int MODE>
+__device__ __forceinline__ void pass_ab_gather(const Dev& D, int projected) {
     __shared__ double lds[4];
-    WIN_XV_DECL(WIN ? WIN_MAX : 1)
     Slot s = slot_nb(D, true);
     if (!D.active[s.scene]) {   // uniform: a converged scene's tiles contribute nothing
         if (threadIdx.x == 0) D.partial[2 * s.tile] = 0.0;
         return;
     }
-    // own records and list first: their loads overlap the window staging
     const int c = s.live ? D.cnt[s.i] : 0;
-    List16 L;
-    if (WIN) list16_prefetch(D, s, c, L, WIN_DIVERGENCE);
-    const int nw = WIN ? win_layout(D, s, WIN_DIVERGENCE) : 0;
-    if (nw) stage_xv(D, nw, win);
-    double contrib = 0.0;
-    if (s.live) {
-        const int i = s.i;
-        float4 pa = D.x[i], va = D.vs[i];
-        int ph = phase_of(pa);
-        float m = D.mass[ph], rho0 = D.rho0[ph];
-        float dd = nw ? density_change_win(D, s, pa, va, m, rho0, c, win, L)
-                      : density_change(D, s, pa, va, m, rho0, c);
-        float dt = D.dt[s.scene];
-        float rhoa = va.w;   // v* carries rho in w (as v does), one array fewer to stream
-        if (!projected) {
-            if (dd < 0.0f || (rhoa + dd * dt) < rho0) dd = 0.0f;
-            float kv = dd * D.alpha[i] / dt;
-            D.xk[i] = make_float4(pa.x, pa.y, pa.z, kv / rhoa);
-            D.kvsum[i] += kv * dt;
-            contrib = (double)dd;
-        } else {
-            // below rest density the solve applies no stiffness: the whole
-            // guess is taken back
-            float alpha = D.alpha[i], K = D.kvsum[i];
-            float inc = dd * alpha;
-            if ((rhoa + dd * dt) < rho0 || inc < -K) inc = -K;
-            float kv = inc / dt;
-            D.xk[i] = make_float4(pa.x, pa.y, pa.z, kv / rhoa);
-            D.kvsum[i] = K + inc;
-            contrib = (double)(fabsf(inc) / alpha);
+    const OwnAB o = own_ab<MODE>(D, s);
+    const List16 L{};
+    write_partial(D, s.tile, 0, pass_ab_particle<MODE>(D, s, o, c, 0, WinXV{nullptr, nullptr}, L, projected), lds);
+}
+
+// LDS-window build: two consecutive tiles per block.  The second tile's
+// window records are requested into registers before the first tile's
+// neighbour loops run, so their latency is spent on the first tile's work
+// instead of in a second staging phase (the particle's own values are loaded
+// before them: a wait for a later load waits for every earlier one).
+struct Rec3x2 {
+    float3 a, b;
+};
+__device__ __forceinline__ Slot slot_tile(const Dev& D, int tile) {
+    Slot s;
+    s.tile = tile;
+    s.i = tile * TILE + threadIdx.x;
+    s.scene = D.tileScene[tile];
+    s.live = (int)threadIdx.x < D.tileValid[tile] && D.active[s.scene] != 0;
+    s.packed = false;
+    return s;
+}
+// win_particle with the tables in `tab` (list table, then window offsets)
+__device__ __forceinline__ int win_particle_tab(const int* tab, int k) {
+    const int* off = tab + NB16_TAB;
+    int s = 0;
+#pragma unroll
+    for (int b = 16; b > 0; b >>= 1)
+        if (off[s + b] <= k) s += b;
+    return tab[s] + (s << NB16_OFF_BITS) + (k - off[s]);
+}
+
+template <int MODE>
+__device__ __forceinline__ void pass_ab_window_pair(const Dev& D, int projected) {
+    __shared__ double lds[4];
+    WIN_XV_DECL(WIN_MAX)
+    __shared__ int tabB[NB16_TAB + 32];   // the second tile's list table and window offsets
+    const int bit = MODE == 0 ? WIN_DENSITY : WIN_DIVERGENCE;
+    const int tA = 2 * tile_of_block(D.tileMap), tB = tA + 1;
+    const bool hasB = tB < D.npad / TILE;
+    Slot sA = slot_tile(D, tA);
+    Slot sB = slot_tile(D, hasB ? tB : tA);
+    const bool actA = D.active[sA.scene] != 0;
+    const bool actB = hasB && D.active[sB.scene] != 0;
+    if (!hasB) sB.live = false;
+    const int t = threadIdx.x;
+    // both tiles' tables with one round trip
+    if (actA) {
+        if (t < NB16_TAB) g_nbtab[t] = D.nbTab[(size_t)tA * NB16_TAB + t];
+        else if (t < NB16_TAB + 32) g_winoff[t - NB16_TAB] = D.nbWin[(size_t)tA * 32 + t - NB16_TAB];
+    }
+    if (actB) {
+        if (t >= 128 && t < 128 + NB16_TAB) tabB[t - 128] = D.nbTab[(size_t)tB * NB16_TAB + t - 128];
+        else if (t >= 128 + NB16_TAB && t < 128 + NB16_TAB + 32)
+            tabB[t - 128] = D.nbWin[(size_t)tB * 32 + t - 128 - NB16_TAB];
+    }
+    __syncthreads();
+    sA.packed = actA && __builtin_amdgcn_readfirstlane(g_nbtab[NB16_FLAG]) != 0;
+    sB.packed = actB && __builtin_amdgcn_readfirstlane(tabB[NB16_FLAG]) != 0;
+    // the first tile: list and own values, then its window
+    const int cA = sA.live ? D.cnt[sA.i] : 0;
+    List16 LA;
+    list16_prefetch(D, sA, cA, LA, bit);
+    const OwnAB oA = own_ab<MODE>(D, sA);
+    const int nwA = win_layout(D, sA, bit);
+    if (nwA) stage_xv(D, nwA, win);
+    // the second tile's window records, in flight during the first tile's loops
+    int nwB = 0;
+    if (sB.packed && (D.winMask & bit)) {
+        nwB = __builtin_amdgcn_readfirstlane(tabB[NB16_TAB + 31]);
+        if (nwB > WIN_MAX) nwB = 0;
+    }
+    constexpr int DEPTH = WIN_MAX / 256;
+    Rec3x2 rB[DEPTH];
+#pragma unroll
+    for (int u = 0; u < DEPTH; ++u) {
+        const int k = t + 256 * u;
+        if (k < nwB) {
+            const int j = win_particle_tab(tabB, k);
+            rB[u].a = *reinterpret_cast<const float3*>(&D.x[j]);
+            rB[u].b = *reinterpret_cast<const float3*>(&D.vs[j]);
         }
     }
-    write_partial(D, s.tile, 0, contrib, lds);
+    const double sumA = pass_ab_particle<MODE>(D, sA, oA, cA, nwA, win, LA, projected);
+    if (actA) write_partial(D, tA, 0, sumA, lds);
+    else if (t == 0) D.partial[2 * tA] = 0.0;
+    if (!hasB) return;   // uniform
+    __syncthreads();     // every lane is done with the first tile's window and tables
+    if (actB) {
+        if (t < NB16_TAB) g_nbtab[t] = tabB[t];
+        else if (t < NB16_TAB + 32) g_winoff[t - NB16_TAB] = tabB[t];
+    }
+#pragma unroll
+    for (int u = 0; u < DEPTH; ++u) {
+        const int k = t + 256 * u;
+        if (k < nwB) {
+            win.xy[k] = make_float2(rB[u].a.x, rB[u].a.y);
+            win.zv[k] = make_float4(rB[u].a.z, rB[u].b.x, rB[u].b.y, rB[u].b.z);
+        }
+    }
+    __syncthreads();
+    const int cB = sB.live ? D.cnt[sB.i] : 0;
+    List16 LB;
+    list16_prefetch(D, sB, cB, LB, bit);
+    const OwnAB oB = own_ab<MODE>(D, sB);
+    const double sumB = pass_ab_particle<MODE>(D, sB, oB, cB, nwB, win, LB, projected);
+    if (actB) write_partial(D, tB, 0, sumB, lds);
+    else if (t == 0) D.partial[2 * tB] = 0.0;
+}
+
+template <bool WIN>
+__global__ __launch_bounds__(256, WIN ? 2 : 1) void k_df_pass_density(Dev D) {
+    if constexpr (WIN) pass_ab_window_pair<0>(D, 0);
+    else pass_ab_gather<0>(D, 0);
+}
+
+template <bool WIN>
+__global__ __launch_bounds__(256, WIN ? 2 : 1) void k_df_pass_divergence(Dev D, int projected) {
+    if constexpr (WIN) pass_ab_window_pair<1>(D, projected);
+    else pass_ab_gather<1>(D, projected);
 }
 
 // pass B of both loops (DFSPH.cpp:647-693, 840-883):
@@ -1589,19 +1697,20 @@ thread_local LaunchTiming g_launch_timing;
 // One block of 256 threads per tile; `lds` bytes of dynamic LDS that the
 // kernel does not use cap how many blocks share a CU (the gather builds of
 // the solver-loop passes run fastest below their register-limited occupancy)
-#define SPH_LAUNCH_L(kern, lds, ...)                                                                    \
+#define SPH_LAUNCH_GL(kern, blocks, lds, ...)                                                          \
     do {                                                                                               \
-        if (tiles_of(D) > 0) {                                                                         \
+        if ((blocks) > 0) {                                                                            \
             const LaunchTiming t_ = g_launch_timing;                                                   \
             if (t_.start) {                                                                            \
                 g_launch_timing = LaunchTiming{};                                                      \
-                hipExtLaunchKernelGGL(kern, dim3(tiles_of(D)), dim3(256), (lds), st, t_.start, t_.stop, \
-                                      0, D, ##__VA_ARGS__);                                            \
+                hipExtLaunchKernelGGL(kern, dim3(blocks), dim3(256), (lds), st, t_.start, t_.stop, 0, D, \
+                                      ##__VA_ARGS__);                                                  \
             } else {                                                                                   \
-                hipLaunchKernelGGL(kern, dim3(tiles_of(D)), dim3(256), (lds), st, D, ##__VA_ARGS__);   \
+                hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), (lds), st, D, ##__VA_ARGS__);        \
             }                                                                                          \
         }                                                                                              \
     } while (0)
+#define SPH_LAUNCH_L(kern, lds, ...) SPH_LAUNCH_GL(kern, tiles_of(D), lds, ##__VA_ARGS__)
 #define SPH_LAUNCH(kern, ...) SPH_LAUNCH_L(kern, 0, ##__VA_ARGS__)
 
 void launch_wc_pressure(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_wc_pressure); }
@@ -1623,15 +1732,16 @@ void launch_mlp_gather(const Dev& D, const float* feat, const float* tgt, float*
         hipLaunchKernelGGL(k_mlp_gather, dim3((n + 255) / 256), dim3(256), 0, st, D, feat, tgt, bfeat, btgt, n, seed);
 }
 // Each window pass has two builds: with the LDS window (48-72 KB of LDS per
-// block) and without (per-lane gathers only: no window LDS and fewer
-// registers, so more waves per SIMD); Dev::winMask picks per pass, and
-// Dev::gatherLds caps the gather build's blocks per CU.
+// block; the DFSPH pass A takes two tiles per block) and without (per-lane
+// gathers only: no window LDS and fewer registers, so more waves per SIMD);
+// Dev::winMask picks per pass, and Dev::gatherLds caps the gather build's
+// blocks per CU.
 void launch_df_pass_density(const Dev& D, hipStream_t st) {
-    if (D.winMask & WIN_DENSITY) SPH_LAUNCH(k_df_pass_density<true>);
+    if (D.winMask & WIN_DENSITY) SPH_LAUNCH_GL(k_df_pass_density<true>, (tiles_of(D) + 1) / 2, 0);
     else SPH_LAUNCH_L(k_df_pass_density<false>, D.gatherLds[0]);
 }
 void launch_df_pass_divergence(const Dev& D, int projected, hipStream_t st) {
-    if (D.winMask & WIN_DIVERGENCE) SPH_LAUNCH(k_df_pass_divergence<true>, projected);
+    if (D.winMask & WIN_DIVERGENCE) SPH_LAUNCH_GL(k_df_pass_divergence<true>, (tiles_of(D) + 1) / 2, 0, projected);
     else SPH_LAUNCH_L(k_df_pass_divergence<false>, D.gatherLds[1], projected);
 }
 void launch_df_pass_velocity(const Dev& D, hipStream_t st) {
