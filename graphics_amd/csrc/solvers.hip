@@ -292,6 +292,114 @@ __device__ __forceinline__ void list16_prefetch(const Dev& D, const Slot& s, int
     }
 }
 
+// ---- two tiles per block ------------------------------------------------
+// The LDS-window builds take two consecutive tiles per block: the second
+// tile's window records are requested into registers before the first
+// tile's neighbour loops run, so their latency is spent on the first tile's
+// work instead of in a second staging phase.  Each particle's own values are
+// loaded before them (a wait for a later load waits for every earlier one).
+struct Rec3x2 {
+    float3 a, b;
+};
+__device__ __forceinline__ Slot slot_tile(const Dev& D, int tile) {
+    Slot s;
+    s.tile = tile;
+    s.i = tile * TILE + threadIdx.x;
+    s.scene = D.tileScene[tile];
+    s.live = (int)threadIdx.x < D.tileValid[tile] && D.active[s.scene] != 0;
+    s.packed = false;
+    return s;
+}
+// win_particle with the tables in `tab` (list table, then window offsets)
+__device__ __forceinline__ int win_particle_tab(const int* tab, int k) {
+    const int* off = tab + NB16_TAB;
+    int s = 0;
+#pragma unroll
+    for (int b = 16; b > 0; b >>= 1)
+        if (off[s + b] <= k) s += b;
+    return tab[s] + (s << NB16_OFF_BITS) + (k - off[s]);
+}
+
+// The LDS-window build of a solver-loop pass: two consecutive tiles per
+// block.  own(s, c) loads what a particle reads of itself, load(j) a window
+// record of particle j, store(k, r) writes window record k, body(s, c, nw,
+// L, own) runs the tile's particles (nw > 0: their fluid neighbours are in
+// the window) and returns the particle's part of the tile's partial sum,
+// which PARTIAL blocks write.  The second tile's records are requested
+// before the first tile's body runs and stored after it.
+template <int BIT, bool PARTIAL, class OwnF, class LoadF, class StoreF, class BodyF>
+__device__ __forceinline__ void window_pair(const Dev& D, OwnF&& own, LoadF&& load, StoreF&& store, BodyF&& body) {
+    __shared__ double lds[4];
+    __shared__ int tabB[NB16_TAB + 32];   // the second tile's list table and window offsets
+    const int tA = 2 * tile_of_block(D.tileMap), tB = tA + 1;
+    const bool hasB = tB < D.npad / TILE;
+    Slot sA = slot_tile(D, tA);
+    Slot sB = slot_tile(D, hasB ? tB : tA);
+    const bool actA = D.active[sA.scene] != 0;
+    const bool actB = hasB && D.active[sB.scene] != 0;
+    if (!hasB) sB.live = false;
+    const int t = threadIdx.x;
+    // both tiles' tables with one round trip
+    if (actA) {
+        if (t < NB16_TAB) g_nbtab[t] = D.nbTab[(size_t)tA * NB16_TAB + t];
+        else if (t < NB16_TAB + 32) g_winoff[t - NB16_TAB] = D.nbWin[(size_t)tA * 32 + t - NB16_TAB];
+    }
+    if (actB) {
+        if (t >= 128 && t < 128 + NB16_TAB) tabB[t - 128] = D.nbTab[(size_t)tB * NB16_TAB + t - 128];
+        else if (t >= 128 + NB16_TAB && t < 128 + NB16_TAB + 32)
+            tabB[t - 128] = D.nbWin[(size_t)tB * 32 + t - 128 - NB16_TAB];
+    }
+    __syncthreads();
+    sA.packed = actA && __builtin_amdgcn_readfirstlane(g_nbtab[NB16_FLAG]) != 0;
+    sB.packed = actB && __builtin_amdgcn_readfirstlane(tabB[NB16_FLAG]) != 0;
+    // the first tile: list and own values, then its window
+    const int cA = sA.live ? D.cnt[sA.i] : 0;
+    List16 LA;
+    list16_prefetch(D, sA, cA, LA, BIT);
+    const auto oA = own(sA, cA);
+    const int nwA = win_layout(D, sA, BIT);
+    if (nwA) win_stage(nwA, load, store);
+    // the second tile's window records, in flight during the first tile's body
+    int nwB = 0;
+    if (sB.packed && (D.winMask & BIT)) {
+        nwB = __builtin_amdgcn_readfirstlane(tabB[NB16_TAB + 31]);
+        if (nwB > WIN_MAX) nwB = 0;
+    }
+    constexpr int DEPTH = WIN_MAX / 256;
+    decltype(load(0)) rB[DEPTH];
+#pragma unroll
+    for (int u = 0; u < DEPTH; ++u) {
+        const int k = t + 256 * u;
+        if (k < nwB) rB[u] = load(win_particle_tab(tabB, k));
+    }
+    const double sumA = body(sA, cA, nwA, LA, oA);
+    if (PARTIAL) {
+        if (actA) write_partial(D, tA, 0, sumA, lds);
+        else if (t == 0) D.partial[2 * tA] = 0.0;
+    }
+    if (!hasB) return;   // uniform
+    __syncthreads();     // every lane is done with the first tile's window and tables
+    if (actB) {
+        if (t < NB16_TAB) g_nbtab[t] = tabB[t];
+        else if (t < NB16_TAB + 32) g_winoff[t - NB16_TAB] = tabB[t];
+    }
+#pragma unroll
+    for (int u = 0; u < DEPTH; ++u) {
+        const int k = t + 256 * u;
+        if (k < nwB) store(k, rB[u]);
+    }
+    __syncthreads();
+    const int cB = sB.live ? D.cnt[sB.i] : 0;
+    List16 LB;
+    list16_prefetch(D, sB, cB, LB, BIT);
+    const auto oB = own(sB, cB);
+    const double sumB = body(sB, cB, nwB, LB, oB);
+    if (PARTIAL) {
+        if (actB) write_partial(D, tB, 0, sumB, lds);
+        else if (t == 0) D.partial[2 * tB] = 0.0;
+    }
+}
+
 __device__ __forceinline__ bool inside_domain(const SceneGrid& G, float4 x) {
     return G.smin[0] <= x.x && G.smin[1] <= x.y && G.smin[2] <= x.z && x.x <= G.smax[0] &&
            x.y <= G.smax[1] && x.z <= G.smax[2];
@@ -814,6 +922,16 @@ struct RecW {
     float4 b;
 };
 
+// a two-record window record: x, y, z of one array and the first three
+// components of another (v* or u), 24 bytes
+__device__ __forceinline__ Rec3x2 load_xyz2(const float4* __restrict__ a, const float4* __restrict__ b, int j) {
+    return Rec3x2{*reinterpret_cast<const float3*>(&a[j]), *reinterpret_cast<const float3*>(&b[j])};
+}
+__device__ __forceinline__ void store_xyz2(WinXV w, int k, const Rec3x2& r) {
+    w.xy[k] = make_float2(r.a.x, r.a.y);
+    w.zv[k] = make_float4(r.a.z, r.b.x, r.b.y, r.b.z);
+}
+
 __device__ __forceinline__ void stage_xv(const Dev& D, int n, WinXV w) {
     win_stage(n, [&](int j) { return Rec2{D.x[j], D.vs[j]}; }, [&](int k, Rec2 r) {
         w.xy[k] = make_float2(r.a.x, r.a.y);
@@ -920,109 +1038,16 @@ __device__ __forceinline__ void pass_ab_gather(const Dev& D, int projected) {
     write_partial(D, s.tile, 0, pass_ab_particle<MODE>(D, s, o, c, 0, WinXV{nullptr, nullptr}, L, projected), lds);
 }
 
-// LDS-window build: two consecutive tiles per block.  The second tile's
-// window records are requested into registers before the first tile's
-// neighbour loops run, so their latency is spent on the first tile's work
-// instead of in a second staging phase (the particle's own values are loaded
-// before them: a wait for a later load waits for every earlier one).
-struct Rec3x2 {
-    float3 a, b;
-};
-__device__ __forceinline__ Slot slot_tile(const Dev& D, int tile) {
-    Slot s;
-    s.tile = tile;
-    s.i = tile * TILE + threadIdx.x;
-    s.scene = D.tileScene[tile];
-    s.live = (int)threadIdx.x < D.tileValid[tile] && D.active[s.scene] != 0;
-    s.packed = false;
-    return s;
-}
-// win_particle with the tables in `tab` (list table, then window offsets)
-__device__ __forceinline__ int win_particle_tab(const int* tab, int k) {
-    const int* off = tab + NB16_TAB;
-    int s = 0;
-#pragma unroll
-    for (int b = 16; b > 0; b >>= 1)
-        if (off[s + b] <= k) s += b;
-    return tab[s] + (s << NB16_OFF_BITS) + (k - off[s]);
-}
-
+// LDS-window build: two tiles per block (window_pair)
 template <int MODE>
 __device__ __forceinline__ void pass_ab_window_pair(const Dev& D, int projected) {
-    __shared__ double lds[4];
     WIN_XV_DECL(WIN_MAX)
-    __shared__ int tabB[NB16_TAB + 32];   // the second tile's list table and window offsets
-    const int bit = MODE == 0 ? WIN_DENSITY : WIN_DIVERGENCE;
-    const int tA = 2 * tile_of_block(D.tileMap), tB = tA + 1;
-    const bool hasB = tB < D.npad / TILE;
-    Slot sA = slot_tile(D, tA);
-    Slot sB = slot_tile(D, hasB ? tB : tA);
-    const bool actA = D.active[sA.scene] != 0;
-    const bool actB = hasB && D.active[sB.scene] != 0;
-    if (!hasB) sB.live = false;
-    const int t = threadIdx.x;
-    // both tiles' tables with one round trip
-    if (actA) {
-        if (t < NB16_TAB) g_nbtab[t] = D.nbTab[(size_t)tA * NB16_TAB + t];
-        else if (t < NB16_TAB + 32) g_winoff[t - NB16_TAB] = D.nbWin[(size_t)tA * 32 + t - NB16_TAB];
-    }
-    if (actB) {
-        if (t >= 128 && t < 128 + NB16_TAB) tabB[t - 128] = D.nbTab[(size_t)tB * NB16_TAB + t - 128];
-        else if (t >= 128 + NB16_TAB && t < 128 + NB16_TAB + 32)
-            tabB[t - 128] = D.nbWin[(size_t)tB * 32 + t - 128 - NB16_TAB];
-    }
-    __syncthreads();
-    sA.packed = actA && __builtin_amdgcn_readfirstlane(g_nbtab[NB16_FLAG]) != 0;
-    sB.packed = actB && __builtin_amdgcn_readfirstlane(tabB[NB16_FLAG]) != 0;
-    // the first tile: list and own values, then its window
-    const int cA = sA.live ? D.cnt[sA.i] : 0;
-    List16 LA;
-    list16_prefetch(D, sA, cA, LA, bit);
-    const OwnAB oA = own_ab<MODE>(D, sA);
-    const int nwA = win_layout(D, sA, bit);
-    if (nwA) stage_xv(D, nwA, win);
-    // the second tile's window records, in flight during the first tile's loops
-    int nwB = 0;
-    if (sB.packed && (D.winMask & bit)) {
-        nwB = __builtin_amdgcn_readfirstlane(tabB[NB16_TAB + 31]);
-        if (nwB > WIN_MAX) nwB = 0;
-    }
-    constexpr int DEPTH = WIN_MAX / 256;
-    Rec3x2 rB[DEPTH];
-#pragma unroll
-    for (int u = 0; u < DEPTH; ++u) {
-        const int k = t + 256 * u;
-        if (k < nwB) {
-            const int j = win_particle_tab(tabB, k);
-            rB[u].a = *reinterpret_cast<const float3*>(&D.x[j]);
-            rB[u].b = *reinterpret_cast<const float3*>(&D.vs[j]);
-        }
-    }
-    const double sumA = pass_ab_particle<MODE>(D, sA, oA, cA, nwA, win, LA, projected);
-    if (actA) write_partial(D, tA, 0, sumA, lds);
-    else if (t == 0) D.partial[2 * tA] = 0.0;
-    if (!hasB) return;   // uniform
-    __syncthreads();     // every lane is done with the first tile's window and tables
-    if (actB) {
-        if (t < NB16_TAB) g_nbtab[t] = tabB[t];
-        else if (t < NB16_TAB + 32) g_winoff[t - NB16_TAB] = tabB[t];
-    }
-#pragma unroll
-    for (int u = 0; u < DEPTH; ++u) {
-        const int k = t + 256 * u;
-        if (k < nwB) {
-            win.xy[k] = make_float2(rB[u].a.x, rB[u].a.y);
-            win.zv[k] = make_float4(rB[u].a.z, rB[u].b.x, rB[u].b.y, rB[u].b.z);
-        }
-    }
-    __syncthreads();
-    const int cB = sB.live ? D.cnt[sB.i] : 0;
-    List16 LB;
-    list16_prefetch(D, sB, cB, LB, bit);
-    const OwnAB oB = own_ab<MODE>(D, sB);
-    const double sumB = pass_ab_particle<MODE>(D, sB, oB, cB, nwB, win, LB, projected);
-    if (actB) write_partial(D, tB, 0, sumB, lds);
-    else if (t == 0) D.partial[2 * tB] = 0.0;
+    window_pair<MODE == 0 ? WIN_DENSITY : WIN_DIVERGENCE, true>(
+        D, [&](const Slot& s, int) { return own_ab<MODE>(D, s); }, [&](int j) { return load_xyz2(D.x, D.vs, j); },
+        [&](int k, const Rec3x2& r) { store_xyz2(win, k, r); },
+        [&](const Slot& s, int c, int nw, const List16& L, const OwnAB& o) {
+            return pass_ab_particle<MODE>(D, s, o, c, nw, win, L, projected);
+        });
 }
 
 template <bool WIN>
@@ -1039,25 +1064,27 @@ __global__ __launch_bounds__(256, WIN ? 2 : 1) void k_df_pass_divergence(Dev D, 
 
 // pass B of both loops (DFSPH.cpp:647-693, 840-883):
 // v* -= dt (m sum (kr_a + kr_b) gradW + rho0 sum V_b kr_a gradW), kr = xk.w
-template <bool WIN>
-__global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
-    __shared__ float4 s_wxk[WIN ? WIN_MAX : 1];   // window records: xk of each neighbour
-    // the loop's convergence test on pass A's partial sums (block b: scene b)
-    if (D.redMode >= 0 && (int)blockIdx.x < D.nscenes)
-        scene_reduce_block(D, blockIdx.x, D.redMode, D.redTol, D.redMinIt, D.redCap, D.activeNext);
-    Slot s = slot_nb(D, true);
-    if (!D.active[s.scene]) return;     // uniform per block
-    const int c = s.live ? D.cnt[s.i] : 0;
-    List16 L;
-    if (WIN) list16_prefetch(D, s, c, L, WIN_VELOCITY);
-    const int nw = WIN ? win_layout(D, s, WIN_VELOCITY) : 0;
-    if (nw) win_stage(nw, [&](int j) { return D.xk[j]; }, [&](int k, float4 r) { s_wxk[k] = r; });
+struct OwnV {
+    float4 pa, v;
+    float m, rho0, dt;
+};
+__device__ __forceinline__ OwnV own_v(const Dev& D, const Slot& s, int c) {
+    OwnV o{};
+    if (!s.live) return o;
+    o.pa = D.xk[s.i];
+    o.v = D.vs[s.i];
+    o.dt = D.dt[s.scene];
+    const int ph = phase_of_cnt(c);   // (the phase from the count word: no read of x)
+    o.m = D.mass[ph];
+    o.rho0 = D.rho0[ph];
+    return o;
+}
+// one particle of pass B; nw > 0: its fluid neighbours are in the window wxk
+__device__ __forceinline__ void velocity_particle(const Dev& D, const Slot& s, const OwnV& o, int c, int nw,
+                                                  const float4* wxk, const List16& L) {
     if (!s.live) return;
-    const int i = s.i;
-    float4 pa = D.xk[i];
-    int ph = phase_of_cnt(c);   // (the phase from the count word: no read of x)
-    float m = D.mass[ph], rho0 = D.rho0[ph];
-    float ka = pa.w;
+    const float4 pa = o.pa;
+    const float ka = pa.w;
     float fx = 0.f, fy = 0.f, fz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
     auto fluid = [&](int, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -1065,7 +1092,7 @@ __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
         fx += dx * g; fy += dy * g; fz += dz * g;
     };
     if (nw) {
-        each_in_win16(L, nf_of(c), [&](int k) { return s_wxk[k]; }, fluid);
+        each_in_win16(L, nf_of(c), [&](int k) { return wxk[k]; }, fluid);
     } else {
         each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, fluid);
     }
@@ -1074,13 +1101,38 @@ __global__ __launch_bounds__(256) void k_df_pass_velocity(Dev D) {
         float g = kernel_gradfac_sq(dx * dx + dy * dy + dz * dz, D) * pb.w;
         bx += dx * g; by += dy * g; bz += dz * g;
     });
-    float dt = D.dt[s.scene];
-    float cf = dt * m, cb = dt * rho0 * ka;
-    float4 v = D.vs[i];
+    const float cf = o.dt * o.m, cb = o.dt * o.rho0 * ka;
+    float4 v = o.v;
     v.x -= cf * fx + cb * bx;
     v.y -= cf * fy + cb * by;
     v.z -= cf * fz + cb * bz;
-    D.vs[i] = v;
+    D.vs[s.i] = v;
+}
+
+template <bool WIN>
+__global__ __launch_bounds__(256, WIN ? 2 : 1) void k_df_pass_velocity(Dev D) {
+    // the loop's convergence test on pass A's partial sums (scene b on
+    // blocks b, b + gridDim, ...)
+    if (D.redMode >= 0)
+        for (int sc = blockIdx.x; sc < D.nscenes; sc += gridDim.x)
+            scene_reduce_block(D, sc, D.redMode, D.redTol, D.redMinIt, D.redCap, D.activeNext);
+    if constexpr (WIN) {
+        __shared__ float4 s_wxk[WIN_MAX];   // window records: xk of each neighbour
+        window_pair<WIN_VELOCITY, false>(
+            D, [&](const Slot& s, int c) { return own_v(D, s, c); }, [&](int j) { return D.xk[j]; },
+            [&](int k, const float4& r) { s_wxk[k] = r; },
+            [&](const Slot& s, int c, int nw, const List16& L, const OwnV& o) {
+                velocity_particle(D, s, o, c, nw, s_wxk, L);
+                return 0.0;
+            });
+    } else {
+        Slot s = slot_nb(D, true);
+        if (!D.active[s.scene]) return;   // uniform per block
+        const int c = s.live ? D.cnt[s.i] : 0;
+        const OwnV o = own_v(D, s, c);
+        const List16 L{};
+        velocity_particle(D, s, o, c, 0, nullptr, L);
+    }
 }
 
 // v = v*, x += dt v, removal outside [spaceMin, spaceMax] (DFSPH.cpp:705-718)
@@ -1277,22 +1329,26 @@ __global__ __launch_bounds__(256) void k_ii_targets(Dev D, const float* __restri
 // sum_j d_ij p_j = dt^2 sum_f -m p_j/rho_j^2 gradW (IISph_M.cpp:289-309);
 // stores u = d_ii p_l + sum_j d_ij p_j, the per-particle part of the
 // Jacobi update that neighbours read.
-template <bool WIN>
-__global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
-    __shared__ float4 s_wxk[WIN ? WIN_MAX : 1];   // window records: xk of each neighbour
-    Slot s = slot_nb(D, true);
-    if (!D.active[s.scene]) return;     // uniform per block
-    const int c = s.live ? D.cnt[s.i] : 0;
-    List16 L;
-    if (WIN) list16_prefetch(D, s, c, L, WIN_SUMDIJ);
-    const int nw = WIN ? win_layout(D, s, WIN_SUMDIJ) : 0;
-    if (nw) win_stage(nw, [&](int j) { return D.xk[j]; }, [&](int k, float4 r) { s_wxk[k] = r; });
+struct OwnS {
+    float4 pa, dii;
+    float m, dt, pl;
+};
+__device__ __forceinline__ OwnS own_s(const Dev& D, const Slot& s, int c) {
+    OwnS o{};
+    if (!s.live) return o;
+    // own position from the xk records the pass gathers (xyz = x during the
+    // loop), the phase from the count word: no read of x
+    o.pa = D.xk[s.i];
+    o.dii = D.f4a[s.i];
+    o.pl = D.kr[s.i];
+    o.dt = D.dt[s.scene];
+    o.m = D.mass[phase_of_cnt(c)];
+    return o;
+}
+__device__ __forceinline__ void sumdij_particle(const Dev& D, const Slot& s, const OwnS& o, int c, int nw,
+                                                const float4* wxk, const List16& L) {
     if (!s.live) return;
-    const int i = s.i;
-    // own position from the xk records the pass gathers (xyz = x during
-    // the loop), the phase from the count word: no read of x
-    const float4 pa = D.xk[i];
-    float m = D.mass[phase_of_cnt(c)];
+    const float4 pa = o.pa;
     float sx = 0.f, sy = 0.f, sz = 0.f;
     auto fluid = [&](int, float4 pb) {
         float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -1300,95 +1356,144 @@ __global__ __launch_bounds__(256) void k_ii_sumdij(Dev D) {
         sx += dx * g; sy += dy * g; sz += dz * g;
     };
     if (nw) {
-        each_in_win16(L, nf_of(c), [&](int k) { return s_wxk[k]; }, fluid);
+        each_in_win16(L, nf_of(c), [&](int k) { return wxk[k]; }, fluid);
     } else {
         each_fluid_lu<8>(D, s, nf_of(c), [&](int j) { return D.xk[j]; }, fluid);
     }
-    float dt = D.dt[s.scene];
+    const float dt = o.dt, m = o.m;
     // the records carry p/rho^2, or m^2 p/rho^2 with the number-density
     // force between phases (whose d_ij is dt^2 m_j^2 / (m_i rho_j^2) gradW)
-    float cf = (D.mpCons && D.nphases > 1) ? -dt * dt / m : -m * dt * dt;
-    float4 dii = D.f4a[i];
-    float pl = D.kr[i];
-    D.f4b[i] = make_float4(dii.x * pl + cf * sx, dii.y * pl + cf * sy, dii.z * pl + cf * sz, 0.0f);
+    const float cf = (D.mpCons && D.nphases > 1) ? -dt * dt / m : -m * dt * dt;
+    const float4 dii = o.dii;
+    const float pl = o.pl;
+    D.f4b[s.i] = make_float4(dii.x * pl + cf * sx, dii.y * pl + cf * sy, dii.z * pl + cf * sz, 0.0f);
+}
+
+template <bool WIN>
+__global__ __launch_bounds__(256, WIN ? 2 : 1) void k_ii_sumdij(Dev D) {
+    if constexpr (WIN) {
+        __shared__ float4 s_wxk[WIN_MAX];   // window records: xk of each neighbour
+        window_pair<WIN_SUMDIJ, false>(
+            D, [&](const Slot& s, int c) { return own_s(D, s, c); }, [&](int j) { return D.xk[j]; },
+            [&](int k, const float4& r) { s_wxk[k] = r; },
+            [&](const Slot& s, int c, int nw, const List16& L, const OwnS& o) {
+                sumdij_particle(D, s, o, c, nw, s_wxk, L);
+                return 0.0;
+            });
+    } else {
+        Slot s = slot_nb(D, true);
+        if (!D.active[s.scene]) return;   // uniform per block
+        const int c = s.live ? D.cnt[s.i] : 0;
+        const OwnS o = own_s(D, s, c);
+        const List16 L{};
+        sumdij_particle(D, s, o, c, 0, nullptr, L);
+    }
+}
+
+struct OwnP {
+    float4 pa, ua, dia;
+    float m, rho0, rhoa, pla, dt, aii, radv, f2;
+};
+__device__ __forceinline__ OwnP own_p(const Dev& D, const Slot& s) {
+    OwnP o{};
+    if (!s.live) return o;
+    const int i = s.i;
+    o.pa = D.x[i];
+    o.ua = D.f4b[i];
+    o.dia = D.f4a[i];
+    o.rhoa = D.rho[i];
+    o.pla = D.kr[i];
+    o.dt = D.dt[s.scene];
+    o.aii = D.alpha[i];
+    o.radv = D.kr0[i];
+    o.f2 = (D.mpCons && D.nphases > 1) ? D.f2[i] : 0.0f;
+    const int ph = phase_of(o.pa);
+    o.m = D.mass[ph];
+    o.rho0 = D.rho0[ph];
+    return o;
+}
+// one particle's Jacobi update; nw > 0: its fluid neighbours (x and u =
+// d_ii p_l + sum_j d_ij p_j) are in the window
+__device__ __forceinline__ double pressure_particle(const Dev& D, const Slot& s, const OwnP& o, int c, int nw,
+                                                    WinXV win, const List16& L) {
+    if (!s.live) return 0.0;
+    const int i = s.i;
+    const float4 pa = o.pa, ua = o.ua, dia = o.dia;
+    const float m = o.m, rho0 = o.rho0, rhoa = o.rhoa, pla = o.pla, dt = o.dt;
+    float cji = dt * dt * m / (rhoa * rhoa);
+    // own sum_j d_ij p_j
+    float sax = ua.x - dia.x * pla, say = ua.y - dia.y * pla, saz = ua.z - dia.z * pla;
+    float ff = 0.f, fb = 0.f;
+    // with the number-density force between phases d_ji depends on the
+    // neighbour's mass; its part sum_b m_a d_ba.gradW_ab p_a is then taken
+    // whole from k_ii_aii (f2) instead of per pair
+    const bool cons = D.mpCons && D.nphases > 1;
+    const float cj = cons ? 0.0f : cji;
+    auto fluid = [&](float bx, float by, float bz, float ux, float uy, float uz) {
+        float dx = pa.x - bx, dy = pa.y - by, dz = pa.z - bz;
+        float g = kernel_gradfac_sq(dot3f(dx, dy, dz, dx, dy, dz), D);
+        float gx = dx * g, gy = dy * g, gz = dz * g;
+        // (sum d_ij p_j|a - d_jj p_j) - (sum d_jk p_k|b - d_ji p_a)
+        float tx = sax - ux + cj * gx * pla;
+        float ty = say - uy + cj * gy * pla;
+        float tz = saz - uz + cj * gz * pla;
+        ff += dot3f(tx, ty, tz, gx, gy, gz);
+    };
+    if (nw) {
+        each_in_win16(L, nf_of(c), [&](int k) { return RecW{win.xy[k], win.zv[k]}; },
+                      [&](int, RecW r) { fluid(r.a.x, r.a.y, r.b.x, r.b.y, r.b.z, r.b.w); });
+    } else {
+        each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.f4b[j]}; }, [&](int j, Rec2 rec) {
+            fluid(rec.a.x, rec.a.y, rec.a.z, rec.b.x, rec.b.y, rec.b.z);
+        });
+    }
+    each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
+        float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float g = kernel_gradfac_sq(dx * dx + dy * dy + dz * dz, D) * pb.w;
+        fb += (sax * dx + say * dy + saz * dz) * g;
+    });
+    float fin = m * ff + rho0 * fb;
+    if (cons) fin += o.f2 * pla;
+    const float aii = o.aii, radv = o.radv;
+    float t = (1.0f - D.omega) * pla + (D.omega / aii) * (rho0 - radv - fin);
+    double contrib = 0.0;
+    if (t < 0.0f) {
+        t = 0.0f;
+    } else {
+        contrib = ((double)radv + (double)aii * (double)pla + (double)fin - (double)rho0) / (double)rho0;
+    }
+    // the new p_l, published in place: this pass gathers x and
+    // sum_j d_ij p_j of its neighbours, never their p_l
+    D.f1[i] = t;
+    D.kr[i] = t;
+    D.xk[i].w = cons ? t * (m * m / (rhoa * rhoa)) : t / (rhoa * rhoa);
+    return contrib;
 }
 
 // relaxed Jacobi update (IISph_M.cpp:312-351) into f1; partial sum of the
 // relative density error of the particles that are not clamped.
 template <bool WIN>
-__global__ __launch_bounds__(256) void k_ii_pressure(Dev D) {
-    __shared__ double lds[4];
-    WIN_XV_DECL(WIN ? WIN_MAX : 1)
-    Slot s = slot_nb(D, true);
-    if (!D.active[s.scene]) {   // uniform: a converged scene's tiles contribute nothing
-        if (threadIdx.x == 0) D.partial[2 * s.tile] = 0.0;
-        return;
-    }
-    const int c = s.live ? D.cnt[s.i] : 0;
-    List16 L;
-    if (WIN) list16_prefetch(D, s, c, L, WIN_PRESSURE);
-    // window records: x and u = d_ii p_l + sum_j d_ij p_j of each neighbour
-    const int nw = WIN ? win_layout(D, s, WIN_PRESSURE) : 0;
-    if (nw) win_stage(nw, [&](int j) { return Rec2{D.x[j], D.f4b[j]}; }, [&](int k, Rec2 r) {
-        win.xy[k] = make_float2(r.a.x, r.a.y);
-        win.zv[k] = make_float4(r.a.z, r.b.x, r.b.y, r.b.z);
-    });
-    double contrib = 0.0;
-    if (s.live) {
-        const int i = s.i;
-        float4 pa = D.x[i], ua = D.f4b[i], dia = D.f4a[i];
-        int ph = phase_of(pa);
-        float m = D.mass[ph], rho0 = D.rho0[ph];
-        float rhoa = D.rho[i], pla = D.kr[i];
-        float dt = D.dt[s.scene];
-        float cji = dt * dt * m / (rhoa * rhoa);
-        // own sum_j d_ij p_j
-        float sax = ua.x - dia.x * pla, say = ua.y - dia.y * pla, saz = ua.z - dia.z * pla;
-        float ff = 0.f, fb = 0.f;
-        // with the number-density force between phases d_ji depends on the
-        // neighbour's mass; its part sum_b m_a d_ba.gradW_ab p_a is then
-        // taken whole from k_ii_aii (f2) instead of per pair
-        const bool cons = D.mpCons && D.nphases > 1;
-        const float cj = cons ? 0.0f : cji;
-        auto fluid = [&](float bx, float by, float bz, float ux, float uy, float uz) {
-            float dx = pa.x - bx, dy = pa.y - by, dz = pa.z - bz;
-            float g = kernel_gradfac_sq(dot3f(dx, dy, dz, dx, dy, dz), D);
-            float gx = dx * g, gy = dy * g, gz = dz * g;
-            // (sum d_ij p_j|a - d_jj p_j) - (sum d_jk p_k|b - d_ji p_a)
-            float tx = sax - ux + cj * gx * pla;
-            float ty = say - uy + cj * gy * pla;
-            float tz = saz - uz + cj * gz * pla;
-            ff += dot3f(tx, ty, tz, gx, gy, gz);
-        };
-        if (nw) {
-            each_in_win16(L, nf_of(c), [&](int k) { return RecW{win.xy[k], win.zv[k]}; },
-                          [&](int, RecW r) { fluid(r.a.x, r.a.y, r.b.x, r.b.y, r.b.z, r.b.w); });
-        } else {
-            each_fluid_lu<4>(D, s, nf_of(c), [&](int j) { return Rec2{D.x[j], D.f4b[j]}; }, [&](int j, Rec2 rec) {
-                fluid(rec.a.x, rec.a.y, rec.a.z, rec.b.x, rec.b.y, rec.b.z);
+__global__ __launch_bounds__(256, WIN ? 2 : 1) void k_ii_pressure(Dev D) {
+    if constexpr (WIN) {
+        WIN_XV_DECL(WIN_MAX)
+        window_pair<WIN_PRESSURE, true>(
+            D, [&](const Slot& s, int) { return own_p(D, s); }, [&](int j) { return load_xyz2(D.x, D.f4b, j); },
+            [&](int k, const Rec3x2& r) { store_xyz2(win, k, r); },
+            [&](const Slot& s, int c, int nw, const List16& L, const OwnP& o) {
+                return pressure_particle(D, s, o, c, nw, win, L);
             });
+    } else {
+        __shared__ double lds[4];
+        Slot s = slot_nb(D, true);
+        if (!D.active[s.scene]) {   // uniform: a converged scene's tiles contribute nothing
+            if (threadIdx.x == 0) D.partial[2 * s.tile] = 0.0;
+            return;
         }
-        each_boundary_lu<8>(D, s, nb_of(c), [&](int j) { return D.xb[j]; }, [&](int j, float4 pb) {
-            float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-            float g = kernel_gradfac_sq(dx * dx + dy * dy + dz * dz, D) * pb.w;
-            fb += (sax * dx + say * dy + saz * dz) * g;
-        });
-        float fin = m * ff + rho0 * fb;
-        if (cons) fin += D.f2[i] * pla;
-        float aii = D.alpha[i], radv = D.kr0[i];
-        float t = (1.0f - D.omega) * pla + (D.omega / aii) * (rho0 - radv - fin);
-        if (t < 0.0f) {
-            t = 0.0f;
-        } else {
-            contrib = ((double)radv + (double)aii * (double)pla + (double)fin - (double)rho0) / (double)rho0;
-        }
-        // the new p_l, published in place: this pass gathers x and
-        // sum_j d_ij p_j of its neighbours, never their p_l
-        D.f1[i] = t;
-        D.kr[i] = t;
-        D.xk[i].w = cons ? t * (m * m / (rhoa * rhoa)) : t / (rhoa * rhoa);
+        const int c = s.live ? D.cnt[s.i] : 0;
+        const OwnP o = own_p(D, s);
+        const List16 L{};
+        write_partial(D, s.tile, 0, pressure_particle(D, s, o, c, 0, WinXV{nullptr, nullptr}, L), lds);
     }
-    write_partial(D, s.tile, 0, contrib, lds);
 }
 
 // IISph_M::ii_computeForcePressure (IISph_M.cpp:478-684) fused with the
@@ -1732,7 +1837,7 @@ void launch_mlp_gather(const Dev& D, const float* feat, const float* tgt, float*
         hipLaunchKernelGGL(k_mlp_gather, dim3((n + 255) / 256), dim3(256), 0, st, D, feat, tgt, bfeat, btgt, n, seed);
 }
 // Each window pass has two builds: with the LDS window (48-72 KB of LDS per
-// block; the DFSPH pass A takes two tiles per block) and without (per-lane
+// block, two tiles per block) and without (per-lane
 // gathers only: no window LDS and fewer registers, so more waves per SIMD);
 // Dev::winMask picks per pass, and Dev::gatherLds caps the gather build's
 // blocks per CU.
@@ -1745,7 +1850,7 @@ void launch_df_pass_divergence(const Dev& D, int projected, hipStream_t st) {
     else SPH_LAUNCH_L(k_df_pass_divergence<false>, D.gatherLds[1], projected);
 }
 void launch_df_pass_velocity(const Dev& D, hipStream_t st) {
-    if (D.winMask & WIN_VELOCITY) SPH_LAUNCH(k_df_pass_velocity<true>);
+    if (D.winMask & WIN_VELOCITY) SPH_LAUNCH_GL(k_df_pass_velocity<true>, (tiles_of(D) + 1) / 2, 0);
     else SPH_LAUNCH_L(k_df_pass_velocity<false>, D.gatherLds[2]);
 }
 void launch_df_integrate(const Dev& D, hipStream_t st) { SPH_LAUNCH(k_df_integrate); }
@@ -1760,11 +1865,11 @@ void launch_ii_warm_apply(const Dev& D, const float* y, const float* feat, hipSt
 }
 void launch_ii_targets(const Dev& D, const float* feat, float* tgt, hipStream_t st) { SPH_LAUNCH(k_ii_targets, feat, tgt); }
 void launch_ii_sumdij(const Dev& D, hipStream_t st) {
-    if (D.winMask & WIN_SUMDIJ) SPH_LAUNCH(k_ii_sumdij<true>);
+    if (D.winMask & WIN_SUMDIJ) SPH_LAUNCH_GL(k_ii_sumdij<true>, (tiles_of(D) + 1) / 2, 0);
     else SPH_LAUNCH_L(k_ii_sumdij<false>, D.gatherLds[3]);
 }
 void launch_ii_pressure(const Dev& D, hipStream_t st) {
-    if (D.winMask & WIN_PRESSURE) SPH_LAUNCH(k_ii_pressure<true>);
+    if (D.winMask & WIN_PRESSURE) SPH_LAUNCH_GL(k_ii_pressure<true>, (tiles_of(D) + 1) / 2, 0);
     else SPH_LAUNCH_L(k_ii_pressure<false>, D.gatherLds[4]);
 }
 void launch_ii_force_integrate(const Dev& D, int gate, hipStream_t st) { SPH_LAUNCH(k_ii_force, gate); }
