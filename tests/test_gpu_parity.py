@@ -191,10 +191,15 @@ def test_run_is_deterministic(gpu):
     assert out[0] == out[1]
 
 
-def test_batched_scenes_equal_single_scenes(gpu):
-    """Scenes advanced together give bit-identical results to each scene alone."""
+@pytest.mark.parametrize("windows", [None, True])
+def test_batched_scenes_equal_single_scenes(gpu, windows):
+    """Scenes advanced together give bit-identical results to each scene
+    alone (with the LDS windows forced on as well: blocks of two tiles then
+    straddle scenes)."""
     sc = scenes.iisph_batched_c4(scenes=3, nmin=300, nmax=2000, seed=11)
     sim = scenes.make_simulation(sc)
+    if windows is not None:
+        sim.set_lds_windows(windows)
     for _ in range(5):
         sim.step()
     ids = sim.ids()
@@ -204,6 +209,8 @@ def test_batched_scenes_equal_single_scenes(gpu):
         single = scenes.Scene("one", sc.params, list(sc.phases), [(0, 0, xyz)],
                               [(0, sc.solids[k][1], sc.solids[k][2])], [sc.domains[k]])
         one = scenes.make_simulation(single)
+        if windows is not None:
+            one.set_lds_windows(windows)
         for _ in range(5):
             one.step()
         sel = (ids >= first) & (ids < first + len(xyz))
@@ -214,21 +221,26 @@ def test_batched_scenes_equal_single_scenes(gpu):
     sim.close()
 
 
-@pytest.mark.parametrize("warm", [sg.WARM_NONE, sg.WARM_PREVIOUS])
-def test_batched_dfsph_scenes_equal_single_scenes(gpu, warm):
+@pytest.mark.parametrize("warm,windows", [(sg.WARM_NONE, None), (sg.WARM_PREVIOUS, None), (sg.WARM_NONE, True),
+                                          (sg.WARM_PREVIOUS, False)])
+def test_batched_dfsph_scenes_equal_single_scenes(gpu, warm, windows):
     """Batched DFSPH scenes: each scene's solver loops stop at its own
     convergence (the velocity pass tests every scene), so a scene advanced
     with others is bit-identical to the scene alone, iteration counts
-    included, and the scenes do not all take the same counts."""
+    included, and the scenes do not all take the same counts.  With the LDS
+    windows forced on, blocks of two tiles straddle scenes that converge at
+    different iterations."""
     base = scenes.iisph_batched_c4(scenes=3, nmin=300, nmax=2000, seed=11)
     p = {k: v for k, v in base.params.items() if k != "omega"}
     p.update(solver=sg.DFSPH, density_tol=1e-4, divergence_tol=1.0, min_iter_density=2, max_iter_density=100,
              min_iter_div=1, max_iter_div=100)
     sc = scenes.Scene("dfsph_batched_3", p, list(base.phases), list(base.fluids), list(base.solids),
                       list(base.domains))
-    print(f"[parity] batched DFSPH, {sc.num_scenes} scenes, warm start {warm}", flush=True)
+    print(f"[parity] batched DFSPH, {sc.num_scenes} scenes, warm start {warm}, windows {windows}", flush=True)
     steps = 6
     sim = scenes.make_simulation(sc, warm_start=warm)
+    if windows is not None:
+        sim.set_lds_windows(windows)
     batched = []
     for _ in range(steps):
         sim.step()
@@ -241,6 +253,8 @@ def test_batched_dfsph_scenes_equal_single_scenes(gpu, warm):
         single = scenes.Scene("one", p, list(sc.phases), [(0, 0, xyz)], [(0, sc.solids[k][1], sc.solids[k][2])],
                               [sc.domains[k]])
         one = scenes.make_simulation(single, warm_start=warm)
+        if windows is not None:
+            one.set_lds_windows(windows)
         alone = [(s1["iter_density"], s1["iter_divergence"]) for s1 in (one.step() for _ in range(steps))]
         assert alone == [b[k] for b in batched], k
         sel = (ids >= first) & (ids < first + len(xyz))
