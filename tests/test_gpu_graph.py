@@ -123,14 +123,16 @@ def test_tuning_choices_are_bit_identical(gpu, shape):
     assert all(r == runs[0] for r in runs[1:])
 
 
-@pytest.mark.parametrize("solver", [sg.DFSPH, sg.IISPH])
-def test_lds_windows_are_bit_identical(gpu, solver):
+@pytest.mark.parametrize("solver,warm", [(sg.DFSPH, sg.WARM_PREVIOUS), (sg.DFSPH, sg.WARM_LEARNED), (sg.IISPH, 0)])
+def test_lds_windows_are_bit_identical(gpu, solver, warm):
     """Solver-loop passes reading neighbours from LDS windows give the same
-    bits as per-lane gathers, on a tank whose tiles have packed lists."""
+    bits as per-lane gathers, on a tank whose tiles have packed lists, with
+    the previous-stiffness and the learned DFSPH start (whose divergence
+    solve starts from the projected iteration) and for IISPH."""
     runs = []
     for on in (1, 0):
         sc = scenes.small_dam(solver, n=(96, 6, 8), box=(2.0, 0.3, 0.3))
-        sim = scenes.make_simulation(sc, warm_start=sg.WARM_PREVIOUS if solver == sg.DFSPH else 0)
+        sim = scenes.make_simulation(sc, warm_start=warm)
         sim.set_lds_windows(on)
         st = [sim.step() for _ in range(6)]
         assert sim.list_form()["packed_tiles"] > 0
