@@ -172,8 +172,9 @@ struct Rec2 {
 // contiguous run of the sorted particles per (own row, stencil row) pair), so
 // the concatenation of those runs -- the tile's window, about ten times the
 // tile's own 256 particles -- holds every record the tile's neighbour loops
-// read.  A pass stages the window in LDS with coalesced loads (each record
-// is needed by about 28 lanes of the tile) and the loops then read it from
+// read.  A pass stages the window in LDS with coalesced loads (a particle
+// has about 28 fluid neighbours, so a record is read by about three lanes
+// of the tile) and the loops then read it from
 // LDS instead of gathering per lane through the L1, taking the window index
 // of each neighbour from the window-index copy of the fluid lists (nbW16).
 // Tiles with 32-bit lists, or whose window exceeds WIN_MAX records, keep the
