@@ -1520,8 +1520,10 @@ int sphgpu_init(sphgpu_handle* h) {
 
     int rc = 0;
 #define ALLOC(ptr, n) if ((rc = dalloc(h, &(ptr), (n)))) return rc
-    ALLOC(D.x, ncap); ALLOC(D.v, ncap); ALLOC(D.id, ncap); ALLOC(D.ksum, ncap); ALLOC(D.kvsum, ncap); ALLOC(D.key, ncap);
-    ALLOC(D.x2, ncap); ALLOC(D.v2, ncap); ALLOC(D.id2, ncap); ALLOC(D.ksum2, ncap); ALLOC(D.kvsum2, ncap); ALLOC(D.key2, ncap);
+    // the position arrays carry X_TAIL records more: the list builder's
+    // candidate loads run up to that far past a cell run
+    ALLOC(D.x, ncap + X_TAIL); ALLOC(D.v, ncap); ALLOC(D.id, ncap); ALLOC(D.ksum, ncap); ALLOC(D.kvsum, ncap); ALLOC(D.key, ncap);
+    ALLOC(D.x2, ncap + X_TAIL); ALLOC(D.v2, ncap); ALLOC(D.id2, ncap); ALLOC(D.ksum2, ncap); ALLOC(D.kvsum2, ncap); ALLOC(D.key2, ncap);
     ALLOC(D.xk, ncap); ALLOC(D.vs, ncap); ALLOC(D.f4a, ncap); ALLOC(D.f4b, ncap);
     ALLOC(D.rho, ncap); ALLOC(D.alpha, ncap); ALLOC(D.kr, ncap); ALLOC(D.kr0, ncap); ALLOC(D.f1, ncap); ALLOC(D.f2, ncap);
     ALLOC(D.nbF, (size_t)NBF_CAP * ncap); ALLOC(D.nbB, (size_t)NBB_CAP * ncap); ALLOC(D.cnt, ncap);
@@ -1530,7 +1532,7 @@ int sphgpu_init(sphgpu_handle* h) {
     ALLOC(D.nbTab, (size_t)NB16_TAB * (ncap / TILE));
     ALLOC(D.nbWin, (size_t)32 * (ncap / TILE));
     // boundary arrays carry one extra dummy particle at index nb (list padding)
-    ALLOC(D.xb, nb + 1); ALLOC(D.vb, nb + 1); ALLOC(D.bid, nb + 1); ALLOC(D.bsolid, nb + 1);
+    ALLOC(D.xb, nb + 1 + X_TAIL); ALLOC(D.vb, nb + 1); ALLOC(D.bid, nb + 1); ALLOC(D.bsolid, nb + 1);
     ALLOC(D.bscene, nb + 1); ALLOC(D.bkself, nb + 1); ALLOC(D.bkey, nb + 1);
     // solids: pose table (all solids), dynamic-solid state and re-sort targets
     D.nsolids = int(h->hsolids.size());
@@ -1545,7 +1547,7 @@ int sphgpu_init(sphgpu_handle* h) {
     }
     if (D.anyDynamic) {
         ALLOC(D.bloc, nb + 1); ALLOC(D.bforce, nb + 1); ALLOC(D.brank, nb + 1);
-        ALLOC(D.xb2, nb + 1); ALLOC(D.vb2, nb + 1); ALLOC(D.bloc2, nb + 1); ALLOC(D.bid2, nb + 1);
+        ALLOC(D.xb2, nb + 1 + X_TAIL); ALLOC(D.vb2, nb + 1); ALLOC(D.bloc2, nb + 1); ALLOC(D.bid2, nb + 1);
         ALLOC(D.bsolid2, nb + 1); ALLOC(D.bscene2, nb + 1); ALLOC(D.bkself2, nb + 1); ALLOC(D.bkey2, nb + 1);
         HIPCHECK(h, hipMemset(D.bforce, 0, sizeof(float4) * (nb + 1)));
     }

@@ -6,6 +6,8 @@
 // that the neighbour sets are bit-identical.
 #include "sph_common.h"
 
+#include <type_traits>
+
 namespace sphgpu {
 
 namespace {
@@ -239,8 +241,11 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     const bool win = packed && sWin[32] <= WIN_MAX;
     auto to_win = [&](unsigned e) { return (e + (unsigned)sWin[e >> NB16_OFF_BITS]) & 0xffffu; };
 
+    // staged entry k of this thread: element k * 256 + tid of the stage
+    // (entries of one thread 256 elements apart, so one store instruction of
+    // a wave writes consecutive elements and its address is one operation)
     auto put = [&](int* list, int k, int j) {
-        if (k < NB_STAGE_GROUPS * 4) st[((k >> 2) * 256 + tid) * 4 + (k & 3)] = j;
+        if (k < NB_STAGE_GROUPS * 4) st[(k << 8) + tid] = j;
         else list[nb_slot(k, i, stride)] = j;
     };
     // bias(slot) = (slot << NB16_OFF_BITS) - base[slot]: an entry is j + bias
@@ -248,7 +253,7 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     auto put16 = [&](int4* list, int k, int bias, int j) {
         const uint16_t e = uint16_t(j + bias);
         if (k < NB_STAGE_GROUPS * 8) {
-            st16[((k >> 3) * 256 + tid) * 8 + (k & 7)] = e;
+            st16[(k << 8) + tid] = e;
         } else {
             reinterpret_cast<uint16_t*>(list)[nb16_slot(k, i, stride)] = e;
             if (win && list == D.nbF16) reinterpret_cast<uint16_t*>(D.nbW16)[nb16_slot(k, i, stride)] = uint16_t(to_win(e));
@@ -258,19 +263,27 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
         __syncthreads();
         const int ng = min((count + 3) >> 2, NB_STAGE_GROUPS);
         int4* out = reinterpret_cast<int4*>(list);
-        for (int g = 0; g < ng; ++g) out[(size_t)g * stride + i] = stage[g][tid];
+        for (int g = 0; g < ng; ++g) {
+            const int b = (4 * g) << 8;
+            out[(size_t)g * stride + i] = make_int4(st[b + tid], st[b + 256 + tid], st[b + 512 + tid], st[b + 768 + tid]);
+        }
         __syncthreads();
     };
     auto flush16 = [&](int4* out, int count) {
         __syncthreads();
         const int ng = min((count + 7) >> 3, NB_STAGE_GROUPS);
-        for (int g = 0; g < ng; ++g) out[(size_t)g * stride + i] = stage[g][tid];
-        if (win && out == D.nbF16) {
-            // the same groups as window indices (entries past the padded
-            // count are converted too, and never read)
-            for (int g = 0; g < ng; ++g) {
-                const int4 v = stage[g][tid];
-                const unsigned w[4] = {(unsigned)v.x, (unsigned)v.y, (unsigned)v.z, (unsigned)v.w};
+        const bool towin = win && out == D.nbF16;
+        for (int g = 0; g < ng; ++g) {
+            unsigned w[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int b = (8 * g + 2 * q) << 8;
+                w[q] = (unsigned)st16[b + tid] | (unsigned)st16[b + 256 + tid] << 16;
+            }
+            out[(size_t)g * stride + i] = make_int4((int)w[0], (int)w[1], (int)w[2], (int)w[3]);
+            if (towin) {
+                // the same group as window indices (entries past the padded
+                // count are converted too, and never read)
                 int r[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) r[q] = (int)(to_win(w[q] & 0xffffu) | to_win(w[q] >> 16) << 16);
@@ -279,39 +292,59 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
         }
         __syncthreads();
     };
-    int nf = 0, nbn = 0, over = 0, zero = 0;
-    int stored = 0;   // fluid entries written (at most NBF_CAP)
-    // fluid neighbours (SphBase.cpp:867-905)
+    int nbn = 0, over = 0, zero = 0;
+    // Candidate loop of one kind (fluid or boundary), candidates four at a
+    // time (four loads in flight, tested in order).  A candidate's entry is
+    // stored in LDS at the next free entry before the test decides (a miss
+    // is overwritten by the next hit, or lies past the padded count that
+    // the passes read), so the common path has no branch; entries past the
+    // LDS stage go to global memory and entries past the list width are
+    // counted (both rare, in a branch of their own).  n: entries found.
+    auto candidates = [&](auto packed_c, const int2* runs, const float4* __restrict__ xs, int selfj, int cap,
+                          int slot0, int4* list16, int* list32, int& n) {
+        constexpr bool PK = decltype(packed_c)::value;
+        constexpr int STAGE = PK ? NB_STAGE_GROUPS * 8 : NB_STAGE_GROUPS * 4;
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        {
-            const int2 rf = runF[r];
-            const int bias = rf.y > rf.x ? bias16(ord * 9 + r) : 0;
-            // candidates four at a time: four loads in flight, tested in order
-            for (int j0 = rf.x; j0 < rf.y; j0 += 4) {
+        for (int r = 0; r < 9; ++r) {
+            const int2 rr = runs[r];
+            const int bias = PK && rr.y > rr.x ? bias16(slot0 + ord * 9 + r) : 0;
+            for (int j0 = rr.x; j0 < rr.y; j0 += 4) {
+                // (up to X_TAIL records past the run are read, and not used:
+                // the position arrays are that much longer)
+                const float4* __restrict__ xg = xs + j0;
                 float4 q[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) q[k] = D.x[min(j0 + k, rf.y - 1)];
+                for (int k = 0; k < 4; ++k) q[k] = xg[k];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const int j = j0 + k;
-                    if (j >= rf.y || j == i) continue;
-                    float l2 = dis_square_exact(pa, q[k]);
-                    if (l2 < D.h2) {
-                        if (stored < NBF_CAP) {
-                            if (packed) put16(D.nbF16, stored, bias, j);
-                            else put(D.nbF, stored, j);
-                            ++stored;
+                    const float l2 = dis_square_exact(pa, q[k]);
+                    const bool hit = j < rr.y && j != selfj && l2 < D.h2;
+                    if (n < STAGE) {
+                        if (PK) st16[(n << 8) + tid] = uint16_t(j + bias);
+                        else st[(n << 8) + tid] = j;
+                    }
+                    if (hit && n >= STAGE) {
+                        if (n < cap) {
+                            if (PK) put16(list16, n, bias, j);
+                            else put(list32, n, j);
                         } else {
                             ++over;
                         }
-                        ++nf;
-                        zero += (l2 == 0.0f);
                     }
+                    zero += (hit && l2 == 0.0f);
+                    n += hit;
                 }
             }
         }
-    }
+    };
+    using PackedT = std::integral_constant<bool, true>;
+    using PlainT = std::integral_constant<bool, false>;
+    // fluid neighbours (SphBase.cpp:867-905); the particle itself is skipped
+    int nf = 0;
+    if (packed) candidates(PackedT{}, runF, D.x, i, NBF_CAP, 0, D.nbF16, nullptr, nf);
+    else candidates(PlainT{}, runF, D.x, i, NBF_CAP, 0, nullptr, D.nbF, nf);
+    const int stored = min(nf, NBF_CAP);
     const int cf = act ? stored : 0;
     // pad: the particle itself
     for (int k = cf; k < ((cf + 3) & ~3); ++k) {
@@ -320,43 +353,17 @@ __global__ __launch_bounds__(256) void k_neighbours(Dev D) {
     }
     if (packed) flush16(D.nbF16, cf);
     else flush(D.nbF, cf);
-    // boundary neighbours (SphBase.cpp:908-930)
+    // boundary neighbours (SphBase.cpp:908-930); no index is skipped
     if (act) {
-#pragma unroll
-        for (int r = 0; r < 9; ++r) {
-            {
-                const int2 rb = runB[r];
-                const int bias = rb.y > rb.x ? bias16(32 + ord * 9 + r) : 0;
-                for (int j0 = rb.x; j0 < rb.y; j0 += 4) {
-                    float4 q[4];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) q[k] = D.xb[min(j0 + k, rb.y - 1)];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        const int j = j0 + k;
-                        if (j >= rb.y) continue;
-                        float l2 = dis_square_exact(pa, q[k]);
-                        if (l2 < D.h2) {
-                            if (nbn < NBB_CAP) {
-                                if (packed) put16(D.nbB16, nbn, bias, j);
-                                else put(D.nbB, nbn, j);
-                            } else {
-                                ++over;
-                            }
-                            ++nbn;
-                            zero += (l2 == 0.0f);
-                        }
-                    }
-                }
-            }
-        }
+        if (packed) candidates(PackedT{}, runB, D.xb, -1, NBB_CAP, 32, D.nbB16, nullptr, nbn);
+        else candidates(PlainT{}, runB, D.xb, -1, NBB_CAP, 32, nullptr, D.nbB, nbn);
     }
     const int cb = act ? min(nbn, NBB_CAP) : 0;
     // pad: the far dummy particle
     for (int k = cb; k < ((cb + 3) & ~3); ++k) {
         if (packed) {
             const uint16_t e = uint16_t(NB16_DUMMY << NB16_OFF_BITS);
-            if (k < NB_STAGE_GROUPS * 8) st16[((k >> 3) * 256 + tid) * 8 + (k & 7)] = e;
+            if (k < NB_STAGE_GROUPS * 8) st16[(k << 8) + tid] = e;
             else reinterpret_cast<uint16_t*>(D.nbB16)[nb16_slot(k, i, stride)] = e;
         }
         else put(D.nbB, k, D.nb);

@@ -415,6 +415,10 @@ constexpr float DUMMY_FAR = 1.0e15f;   // coordinate of the dummy boundary parti
 // Records of the LDS window of a packed tile at most (12 x 256): a larger
 // window is not staged and the tile's loops gather from memory.
 constexpr int WIN_MAX = 3072;
+// records past the end of a cell run that the list builder may read (its
+// candidate loads come four at a time); the position arrays are allocated
+// that much longer
+constexpr int X_TAIL = 3;
 
 // ---------------------------------------------------------------------------
 // host-side launchers implemented in the .hip files
