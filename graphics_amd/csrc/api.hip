@@ -831,7 +831,8 @@ int step_dfsph(sphgpu_handle* h, sphgpu_stats* st) {
         launch_copy_f4(D.vs, D.v, D.npad, h->st);
         if (mode == 2) {
             ProfScope ps(h, "mlp_forward", nf_live(h) * 36.0);
-            mlp_forward(h->mlp, h->feat, h->ypred, D.npad, nullptr, h->st);
+            // (the warm start reads the prediction only where Drho/Dt > 0)
+            mlp_forward(h->mlp, h->feat, h->ypred, D.npad, nullptr, h->st, true);
         }
         launch_df_warm_apply(D, h->ypred, h->feat, mode, h->st);
         if (mode != 0) {

@@ -33,7 +33,10 @@ struct MlpDev {
 
 void mlp_pack(MlpDev& m, hipStream_t st);
 // y[i] = MLP(feat[8 i .. 8 i + 7]); store (cap x MLP_STORE) receives activations when non-null
-void mlp_forward(const MlpDev& m, const float* feat, float* y, int n, float* store, hipStream_t st);
+// gate1 (inference only): tiles of 16 particles whose feature 1 is negative
+// for all 16 are skipped, their outputs left as they were
+void mlp_forward(const MlpDev& m, const float* feat, float* y, int n, float* store, hipStream_t st,
+                 bool gate1 = false);
 // gradient of mean((MLP(feat) - target)^2) into m.grad (n <= m.cap); loss parts in m.lossPart
 void mlp_grad(MlpDev& m, const float* feat, const float* target, int n, hipStream_t st);
 void mlp_adam(MlpDev& m, float lr, hipStream_t st);

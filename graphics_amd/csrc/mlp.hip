@@ -200,9 +200,13 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const bf16x8* __restrict__ 
 
 // Inference only: the 22 weight fragments stay in registers and the biases
 // in LDS for all the tiles of 16 particles a wave processes (the training
-// forward above re-reads them per tile).
+// forward above re-reads them per tile).  gate1: a tile whose 16 particles
+// all have feature 1 below zero is skipped and its outputs are not written
+// (the DFSPH warm start reads the output only where Drho/Dt > 0, and feature
+// 1 is asinh of a positive multiple of Drho/Dt, negative only where it is).
 __global__ __launch_bounds__(256, 3) void k_mlp_infer(const bf16x8* __restrict__ F, const float* __restrict__ P,
-                                                 const float* __restrict__ feat, float* __restrict__ y, int n) {
+                                                 const float* __restrict__ feat, float* __restrict__ y, int n,
+                                                 int gate1) {
     __shared__ float sb[3 * 64];
     for (int k = threadIdx.x; k < 3 * 64; k += 256) {
         const int off[3] = {OFF_B1, OFF_B2, OFF_B3};
@@ -222,10 +226,13 @@ __global__ __launch_bounds__(256, 3) void k_mlp_infer(const bf16x8* __restrict__
         bf16x8 x;
 #pragma unroll
         for (int j = 0; j < 8; ++j) x[j] = to_bf16(0.0f);
+        bool need = false;
         if (g < 2 && p < n) {
             const float4 f = reinterpret_cast<const float4*>(feat)[p * 2 + g];
             x[0] = to_bf16(f.x); x[1] = to_bf16(f.y); x[2] = to_bf16(f.z); x[3] = to_bf16(f.w);
+            need = g == 0 && !(f.y < 0.0f);
         }
+        if (gate1 && __ballot(need) == 0) continue;   // uniform per wave
         float h[4][4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
@@ -513,14 +520,14 @@ void mlp_pack(MlpDev& m, hipStream_t st) {
                        (bf16x8*)m.fwd, (bf16x8*)m.bwd);
 }
 
-void mlp_forward(const MlpDev& m, const float* feat, float* y, int n, float* store, hipStream_t st) {
+void mlp_forward(const MlpDev& m, const float* feat, float* y, int n, float* store, hipStream_t st, bool gate1) {
     if (n <= 0) return;
     int waves = (n + 15) / 16;
     if (!store) {
         // about three waves per SIMD, each looping over tiles
         const int blocks = std::min((waves + 3) / 4, 768);
         hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, (const bf16x8*)m.fwd, (const float*)m.params,
-                           feat, y, n);
+                           feat, y, n, gate1 ? 1 : 0);
         return;
     }
     int blocks = std::min((waves + 3) / 4, 8192);
